@@ -1,0 +1,377 @@
+"""Python mirror of breseq's pileup-scoring interface on the MI355X engine.
+
+Function names and argument meanings follow the reference entry points:
+  error_count(...)            error_count.h:41-52
+  identify_mutations(...)     identify_mutations.h:46-60
+  test_RA_evidence(...)       identify_mutations.h:63-67
+  calculate_trims(...)        calculate_trims.h:34 (SequenceTrims, calculate_trims.cpp:67-138)
+  fit_coverage_distribution   coverage_distribution.cpp:345-399
+Errors raise BsqError with the C ABI's code instead of ending the process.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import BsqError  # noqa: F401  (re-exported)
+
+CONSENSUS = "consensus"
+POLYMORPHISM = "polymorphism"
+
+
+def _status():
+    return L.Status()
+
+
+def _np(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).view(dtype)
+
+
+class Sample:
+    """A reference plus sorted alignments (bsq_sample)."""
+
+    def __init__(self, handle, read_file_groups):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        self.read_file_groups = read_file_groups
+        self._keep = []
+
+    @classmethod
+    def simulate(cls, seed=1, ref_length=48502, gc=0.5, read_length=100, coverage=100.0, paired=False,
+                 insert_mean=350.0, insert_sd=50.0, qual_mean=32.0, qual_sd=6.0, qual_min=2, qual_max=41,
+                 n_snps=20, n_indels=2, indel_max=5, variant_freq_min=1.0, variant_freq_max=1.0,
+                 redundant_fraction=0.01, mapq=42):
+        cfg = L.SimConfig(seed, ref_length, gc, read_length, coverage, 1 if paired else 0, insert_mean, insert_sd,
+                          qual_mean, qual_sd, qual_min, qual_max, n_snps, n_indels, indel_max, variant_freq_min,
+                          variant_freq_max, redundant_fraction, mapq)
+        h = C.c_void_p()
+        st = _status()
+        L.check(L.lib().bsq_simulate(C.byref(cfg), C.byref(h), C.byref(st)), st)
+        groups = [["reads_1", "reads_2"]] if paired else [["reads_1"]]
+        return cls(h, groups)
+
+    @classmethod
+    def read(cls, bam_path, fasta_path, read_file_groups=None):
+        h = C.c_void_p()
+        st = _status()
+        L.check(L.lib().bsq_sample_read(bam_path.encode(), fasta_path.encode(), C.byref(h), C.byref(st)), st)
+        s = cls(h, read_file_groups)
+        if s.read_file_groups is None:
+            n = L.lib().bsq_sample_n_read_groups(s._h)
+            s.read_file_groups = [["rg%d" % i] for i in range(max(n, 1))]
+        return s
+
+    def write_bam(self, path):
+        st = _status()
+        L.check(L.lib().bsq_sample_write_bam(self._h, path.encode(), C.byref(st)), st)
+
+    def write_fasta(self, path):
+        st = _status()
+        L.check(L.lib().bsq_sample_write_fasta(self._h, path.encode(), C.byref(st)), st)
+
+    @property
+    def ref(self):
+        return L.lib().bsq_sample_reference(self._h)
+
+    @property
+    def aln(self):
+        return L.lib().bsq_sample_alignments(self._h)
+
+    def close(self):
+        if self._h:
+            L.lib().bsq_sample_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # numpy views of the alignment arrays (valid while the sample lives)
+    def arrays(self):
+        a = self.aln.contents
+        n = a.n_reads
+        nb = int(_np(a.seq_off, n + 1, np.int64)[-1]) if n else 0
+        nc = int(_np(a.cigar_off, n + 1, np.int64)[-1]) if n else 0
+        return dict(
+            tid=_np(a.tid, n, np.int32), pos=_np(a.pos, n, np.int32), flag=_np(a.flag, n, np.uint16),
+            mapq=_np(a.mapq, n, np.uint8), cigar_off=_np(a.cigar_off, n + 1, np.int64),
+            cigar=_np(a.cigar, nc, np.uint32), seq_off=_np(a.seq_off, n + 1, np.int64),
+            seq=_np(a.seq, nb, np.uint8), qual=_np(a.qual, nb, np.uint8), x1=_np(a.x1, n, np.int32),
+            xl=_np(a.xl, n, np.int32), xr=_np(a.xr, n, np.int32), rg=_np(a.rg, n, np.int32))
+
+    def reference_sequences(self):
+        r = self.ref.contents
+        out = []
+        for i in range(r.n_targets):
+            out.append((r.names[i].decode(), C.string_at(r.seqs[i], r.lengths[i]).decode()))
+        return out
+
+    @property
+    def n_targets(self):
+        return self.ref.contents.n_targets
+
+    def target_lengths(self):
+        r = self.ref.contents
+        return [r.lengths[i] for i in range(r.n_targets)]
+
+    def max_quality(self):
+        q = self.arrays()["qual"]
+        return int(q.max()) if q.size else 0
+
+    def aligned_bases(self):
+        a = self.arrays()
+        ops = a["cigar"] & 0xF
+        lens = a["cigar"] >> 4
+        refc = np.isin(ops, [0, 2, 3, 7, 8])
+        mapped = (a["flag"] & 4) == 0
+        per = np.add.reduceat(np.where(refc, lens, 0).astype(np.int64), a["cigar_off"][:-1]) if len(ops) else np.zeros(0)
+        n = len(a["pos"])
+        has = a["cigar_off"][1:] > a["cigar_off"][:-1]
+        per = np.where(has, per[: n], 0)
+        return int(per[mapped].sum())
+
+    def read_files(self):
+        """bsq_read_files for this sample's read-group -> read-file partition."""
+        groups = self.read_file_groups or [["reads_1"]]
+        base, count, names = [], [], []
+        for g in groups:
+            base.append(len(names))
+            count.append(len(g))
+            names.extend(g)
+        b = (C.c_uint32 * len(base))(*base)
+        c = (C.c_uint32 * len(count))(*count)
+        nm = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        self._keep.extend([b, c, nm])
+        return L.ReadFiles(len(groups), b, c, len(names), nm)
+
+
+@dataclass
+class ErrorCountResult:
+    histograms: list
+    counts: np.ndarray
+    log10_prob: np.ndarray
+    read_set_max: int
+    quality_max: int
+    texts: dict = field(default_factory=dict)
+
+
+class Session:
+    """A device-resident sample on one GPU (bsq_session)."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        st = _status()
+        L.check(L.lib().bsq_session_create(device, C.byref(self._h), C.byref(st)), st)
+        self.sample = None
+
+    def load(self, sample):
+        st = _status()
+        rf = sample.read_files()
+        L.check(L.lib().bsq_session_load(self._h, sample.ref, sample.aln, C.byref(rf), C.byref(st)), st)
+        self.sample = sample
+        return self
+
+    def timing(self):
+        t = L.Timing()
+        L.lib().bsq_session_timing(self._h, C.byref(t))
+        return {k: getattr(t, k) for k, _ in L.Timing._fields_}
+
+    def close(self):
+        if self._h:
+            L.lib().bsq_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error_count(self, **kw):
+        return error_count(self.sample, session=self, **kw)
+
+    def identify_mutations(self, **kw):
+        return identify_mutations(self.sample, session=self, **kw)
+
+
+def _target_arrays(sample, call_targets):
+    n = sample.n_targets
+    call = (C.c_uint8 * n)(*[1 if (call_targets is None or i in call_targets) else 0 for i in range(n)])
+    return call
+
+
+def make_error_count_args(sample, read_set_max=None, quality_max=None, coverage_groups=None, call_targets=None,
+                          output_dir=None, error_rates_file=None, do_coverage=True, do_errors=True):
+    n = sample.n_targets
+    if quality_max is None:
+        quality_max = sample.max_quality() + 1
+    rf = sample.read_files()
+    if read_set_max is None:
+        read_set_max = rf.n_files
+    groups = coverage_groups if coverage_groups is not None else [0] * n
+    g = (C.c_int32 * n)(*groups)
+    call = _target_arrays(sample, call_targets)
+    args = L.ErrorCountArgs()
+    args.ref = sample.ref
+    args.aln = sample.aln
+    args.read_files = rf
+    args.coverage_group = g
+    args.n_coverage_groups = max(groups) + 1 if groups else 1
+    args.call_target = call
+    args.read_set_max = read_set_max
+    args.quality_max = quality_max
+    args.do_coverage = 1 if do_coverage else 0
+    args.do_errors = 1 if do_errors else 0
+    args.output_dir = output_dir.encode() if output_dir else None
+    args.error_rates_file = error_rates_file.encode() if error_rates_file else None
+    args._keep = (g, call, rf)
+    return args
+
+
+def error_count(sample, session=None, **kw):
+    """error_count (error_count.cpp:50-68) on the GPU.  Returns ErrorCountResult."""
+    args = make_error_count_args(sample, **kw)
+    lib = L.lib()
+    res = C.c_void_p()
+    st = _status()
+    if session is not None:
+        L.check(lib.bsq_session_error_count(session._h, C.byref(args), C.byref(res), C.byref(st)), st)
+    else:
+        L.check(lib.bsq_error_count(C.byref(args), C.byref(res), C.byref(st)), st)
+    try:
+        hists = []
+        for gi in range(args.n_coverage_groups):
+            p = L.c_u32p()
+            n = C.c_int64()
+            lib.bsq_ec_result_histogram(res, gi, C.byref(p), C.byref(n))
+            hists.append(_np(p, n.value, np.uint32).copy())
+        p64 = C.POINTER(C.c_uint64)()
+        n = C.c_int64()
+        lib.bsq_ec_result_counts(res, C.byref(p64), C.byref(n))
+        counts = _np(p64, n.value, np.uint64).copy()
+        pd = L.c_f64p()
+        lib.bsq_ec_result_log10_prob(res, C.byref(pd), C.byref(n))
+        log10p = _np(pd, n.value, np.float64).copy()
+    finally:
+        lib.bsq_ec_result_free(res)
+    return ErrorCountResult(hists, counts, log10p, args.read_set_max, args.quality_max)
+
+
+def mode_settings(mode):
+    """(mutation_cutoff, polymorphism_cutoff, precision_decimal, precision_places) of a mode
+    (settings.cpp:857-950)."""
+    if mode == POLYMORPHISM:
+        return 10.0, 2.0, 1e-6, 8
+    return 10.0, 10.0, 1e-6, 3
+
+
+def make_identify_args(sample, log10_prob, read_set_max, quality_max, propagation_cutoffs, seed_cutoffs=None,
+                       mode=CONSENSUS, call_targets=None, base_quality_cutoff=3, skip_mc=False, gd_file=None):
+    n = sample.n_targets
+    mut, poly, dec, places = mode_settings(mode)
+    table = np.ascontiguousarray(log10_prob, dtype=np.float64)
+    prop = (C.c_double * n)(*[float(x) for x in propagation_cutoffs])
+    seed = (C.c_double * n)(*[float(x) for x in (seed_cutoffs if seed_cutoffs is not None else [0.0] * n)])
+    call = _target_arrays(sample, call_targets)
+    rf = sample.read_files()
+    args = L.IdentifyArgs()
+    args.ref = sample.ref
+    args.aln = sample.aln
+    args.read_files = rf
+    args.call_target = call
+    args.log10_prob_table = table.ctypes.data_as(L.c_f64p)
+    args.n_bins = table.size
+    args.read_set_max = read_set_max
+    args.quality_max = quality_max
+    args.deletion_propagation_cutoff = prop
+    args.deletion_seed_cutoff = seed
+    args.mutation_cutoff = mut
+    args.polymorphism_cutoff = poly
+    args.polymorphism_precision_decimal = dec
+    args.polymorphism_precision_places = places
+    args.base_quality_cutoff = base_quality_cutoff
+    args.skip_missing_coverage_prediction = 1 if skip_mc else 0
+    args.gd_file = gd_file.encode() if gd_file else None
+    args._keep = (table, prop, seed, call, rf)
+    return args
+
+
+def identify_mutations(sample, log10_prob, read_set_max, quality_max, propagation_cutoffs, session=None,
+                       screen=True, **kw):
+    """identify_mutations (identify_mutations.cpp:48-88) on the GPU.  Returns ra_mc_evidence.gd text."""
+    args = make_identify_args(sample, log10_prob, read_set_max, quality_max, propagation_cutoffs, **kw)
+    lib = L.lib()
+    res = C.c_void_p()
+    st = _status()
+    if session is not None:
+        fn = lib.bsq_session_identify if screen else lib.bsq_session_identify_unscreened
+        L.check(fn(session._h, C.byref(args), C.byref(res), C.byref(st)), st)
+    else:
+        if not screen:
+            raise ValueError("screen=False needs a session")
+        L.check(lib.bsq_identify_mutations(C.byref(args), C.byref(res), C.byref(st)), st)
+    try:
+        p = C.c_char_p()
+        n = C.c_int64()
+        lib.bsq_im_result_gd_text(res, C.byref(p), C.byref(n))
+        text = C.string_at(p, n.value).decode()
+    finally:
+        lib.bsq_im_result_free(res)
+    return text
+
+
+def ra_test_settings(polymorphism_prediction=False, **overrides):
+    s = L.RaTestSettings()
+    L.lib().bsq_ra_test_settings_default(C.byref(s), 1 if polymorphism_prediction else 0)
+    for k, v in overrides.items():
+        setattr(s, k, v)
+    return s
+
+
+def test_RA_evidence(gd_text, sample=None, settings=None, polymorphism_prediction=False):
+    """test_RA_evidence (identify_mutations.cpp:687-749) on GD text; returns the new GD text."""
+    s = settings or ra_test_settings(polymorphism_prediction)
+    out = C.c_void_p()
+    n = C.c_int64()
+    st = _status()
+    ref = sample.ref if sample is not None else None
+    L.check(L.lib().bsq_test_ra_evidence(gd_text.encode(), ref, C.byref(s), C.byref(out), C.byref(n), C.byref(st)), st)
+    try:
+        return C.string_at(out, n.value).decode()
+    finally:
+        L.lib().bsq_free_text(out)
+
+
+def calculate_trims(seq):
+    """SequenceTrims of one sequence on the GPU: (left, right) uint8 arrays."""
+    b = seq.encode() if isinstance(seq, str) else bytes(seq)
+    out = np.zeros(2 * len(b), dtype=np.uint8)
+    st = _status()
+    L.check(L.lib().bsq_sequence_trims(b, len(b), out.ctypes.data_as(L.c_u8p), C.byref(st)), st)
+    return out[: len(b)], out[len(b):]
+
+
+def fit_coverage_distribution(hist, sequence_length):
+    """Negative-binomial fit of a unique-only coverage histogram and the deletion propagation
+    cutoff (coverage_distribution.cpp:345-399)."""
+    h = np.ascontiguousarray(hist, dtype=np.uint32)
+    out = L.CoverageFit()
+    st = _status()
+    L.check(L.lib().bsq_fit_coverage_distribution(h.ctypes.data_as(L.c_u32p), h.size, float(sequence_length),
+                                                  C.byref(out), C.byref(st)), st)
+    return {k: getattr(out, k) for k, _ in L.CoverageFit._fields_}
+
+
+def error_rates_text_to_table(text):
+    """read_log10_prob_table (error_count.cpp:629-654): the values identify_mutations uses."""
+    lines = text.split("\n")
+    vals = []
+    for ln in lines[2:]:
+        if not ln:
+            continue
+        vals.append(float(ln.split("\t")[-1]))
+    return np.array(vals, dtype=np.float64)

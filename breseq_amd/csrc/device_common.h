@@ -1,0 +1,199 @@
+// device_common.h -- device-side building blocks shared by the pileup kernels: base encodings
+// (common.h:202-316), the pileup entry of one alignment at one reference column (htslib
+// resolve_cigar2 as breseq consumes it, pileup_base.cpp:141-210), and the per-slot read filters
+// of identify_mutations_pileup::pileup_callback (identify_mutations.cpp:1561-1657).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+
+namespace bsq {
+
+enum { OP_M = 0, OP_I = 1, OP_D = 2, OP_N = 3, OP_S = 4, OP_H = 5, OP_P = 6, OP_EQ = 7, OP_X = 8 };
+
+// error bits reported back to the host (each is an ASSERT in the reference)
+enum : uint32_t {
+  kErrBaseCode = 1,      // read base is not A/C/G/T/N (basebam2index, common.h:271-283)
+  kErrRefChar = 2,       // reference base is not A/C/G/T/N (basechar2index)
+  kErrQuality = 4,       // quality >= quality covariate maximum (error_count.cpp:484-491)
+  kErrReadSet = 8,       // read file index >= read_set covariate maximum
+  kErrQueryIndex = 16,   // quality index outside the read (error_count.cpp:914,940,967)
+  kErrCigar = 32,        // CIGAR walk did not reach the column
+  kErrCapacity = 64      // a fixed device buffer was too small (host retries or reports)
+};
+
+__device__ __forceinline__ bool is_ref_op(uint32_t op) {
+  return op == OP_M || op == OP_D || op == OP_N || op == OP_EQ || op == OP_X;
+}
+
+__device__ __forceinline__ uint8_t comp_bam(uint8_t b) {
+  // 1<->8, 2<->4, 15->15, '.'->'.'
+  switch (b) {
+    case 1: return 8;
+    case 2: return 4;
+    case 4: return 2;
+    case 8: return 1;
+    default: return b;
+  }
+}
+
+__device__ __forceinline__ char comp_char(char c) {
+  switch (c) {
+    case 'A': return 'T';
+    case 'C': return 'G';
+    case 'G': return 'C';
+    case 'T': return 'A';
+    default: return c;
+  }
+}
+
+// BAM 4-bit code -> index A0 C1 G2 T3; '.' -> 4; anything else -> -1
+__device__ __forceinline__ int bam_index(uint8_t b) {
+  switch (b) {
+    case 1: return 0;
+    case 2: return 1;
+    case 4: return 2;
+    case 8: return 3;
+    case '.': return 4;
+    default: return -1;
+  }
+}
+
+__device__ __forceinline__ int char_index(char c) {
+  switch (c) {
+    case 'A': return 0;
+    case 'C': return 1;
+    case 'G': return 2;
+    case 'T': return 3;
+    case '.': return 4;
+    case 'N': return 5;
+    default: return -1;
+  }
+}
+
+__device__ __forceinline__ uint8_t comp_index(uint8_t i) { return i < 4 ? (uint8_t)(3 - i) : (uint8_t)4; }
+
+// Pileup entry of read R at global column p: returns false if R does not cover p.
+__device__ __forceinline__ bool entry_at(const DRead& R, const uint32_t* __restrict__ cigar, int p, int& qpos,
+                                         int& indel, bool& is_del, uint32_t& err) {
+  if (p < R.gpos || p >= R.gend) return false;
+  if (R.flags & kReadSimple) {
+    qpos = (int)R.y0 + (p - R.gpos);
+    indel = 0;
+    is_del = false;
+    return true;
+  }
+  const uint32_t* c = cigar + R.cig_off;
+  const int n = R.n_cigar;
+  int x = R.gpos, y = 0, k = 0;
+  for (; k < n; ++k) {
+    uint32_t op = c[k] & 15;
+    if (is_ref_op(op)) break;
+    if (op == OP_I || op == OP_S) y += (int)(c[k] >> 4);
+  }
+  for (;;) {
+    if (k >= n) {
+      err |= kErrCigar;
+      return false;
+    }
+    uint32_t w = c[k];
+    int l = (int)(w >> 4);
+    if (p - x < l) break;
+    uint32_t op = w & 15;
+    if (op == OP_M || op == OP_EQ || op == OP_X) y += l;
+    x += l;
+    ++k;
+    for (; k < n; ++k) {
+      uint32_t op2 = c[k] & 15;
+      if (is_ref_op(op2)) break;
+      if (op2 == OP_I || op2 == OP_S) y += (int)(c[k] >> 4);
+    }
+  }
+  uint32_t op = c[k] & 15;
+  int l = (int)(c[k] >> 4);
+  indel = 0;
+  if (x + l - 1 == p && k + 1 < n) {
+    uint32_t op2 = c[k + 1] & 15;
+    int l2 = (int)(c[k + 1] >> 4);
+    if (op2 == OP_D && op != OP_D) {
+      indel = -l2;
+      for (int kk = k + 2; kk < n; ++kk) {
+        if ((c[kk] & 15) == OP_D) indel -= (int)(c[kk] >> 4);
+        else break;
+      }
+    } else if (op2 == OP_I) {
+      indel = l2;
+      for (int kk = k + 2; kk < n; ++kk) {
+        uint32_t o3 = c[kk] & 15;
+        if (o3 == OP_I) indel += (int)(c[kk] >> 4);
+        else if (o3 != OP_P) break;
+      }
+    } else if (op2 == OP_P && k + 2 < n) {
+      int l3 = 0;
+      for (int kk = k + 2; kk < n; ++kk) {
+        uint32_t o3 = c[kk] & 15;
+        if (o3 == OP_I) l3 += (int)(c[kk] >> 4);
+        else if (is_ref_op(o3)) break;
+      }
+      if (l3 > 0) indel = l3;
+    }
+  }
+  if (op == OP_M || op == OP_EQ || op == OP_X) {
+    qpos = y + (p - x);
+    is_del = false;
+  } else {
+    qpos = y;
+    is_del = true;
+  }
+  return true;
+}
+
+// is_trimmed (alignment.h:389-410)
+__device__ __forceinline__ bool is_trimmed(const DRead& R, int qpos, bool past) {
+  uint32_t q1 = (uint32_t)qpos + 1;
+  if (R.xl >= 0 && q1 <= (uint32_t)R.xl) return true;
+  if (R.xr >= 0) {
+    uint32_t rem = (uint32_t)R.l_qseq - q1;
+    if (rem + 1 <= (uint32_t)R.xr) return true;
+    if (past && rem == (uint32_t)R.xr) return true;
+  }
+  return false;
+}
+
+// Observation a unique, untrimmed read contributes at (column, insert slot), after
+// alignment_position_to_covariates (error_count.cpp:1049-1105).  Returns false when the read is
+// skipped there.  obs: 0..4; q: quality of the base the reference reads it from.
+__device__ __forceinline__ bool slot_observation(const DRead& R, const uint8_t* __restrict__ bases,
+                                                 const uint8_t* __restrict__ quals, int qpos, int ondel,
+                                                 int slot, uint8_t read_base, int& obs, int& q, uint32_t& err) {
+  const int rev = (R.flags & kReadReversed) ? 1 : 0;
+  int qq = qpos;
+  if (ondel == -1) {
+    qq += 1 - rev;
+    if (qq < 0 || qq >= (int)R.l_qseq) {
+      err |= kErrQueryIndex;
+      return false;
+    }
+    if (bases[R.seq_off + qq] == 15) return false;
+  } else if (slot > 0) {
+    int mo = slot;
+    if (ondel < mo) mo = ondel;
+    qq += mo + 1 - rev;
+    if ((uint32_t)qq > (uint32_t)R.qbe0) return false;
+    if (qq < 0 || qq >= (int)R.l_qseq) {
+      err |= kErrQueryIndex;
+      return false;
+    }
+    if (bases[R.seq_off + qq] == 15) return false;
+  }
+  obs = bam_index(read_base);
+  if (obs < 0) {
+    err |= kErrBaseCode;
+    return false;
+  }
+  q = quals[R.seq_off + qq];
+  return true;
+}
+
+}  // namespace bsq

@@ -1,0 +1,593 @@
+// evidence.cpp -- see evidence.h.
+#include "evidence.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <sstream>
+
+#include "host_util.h"
+
+namespace bsq {
+
+static const char kIdx[6] = {'A', 'C', 'G', 'T', '.', 'N'};
+
+void counts_to_log10_prob(int S, int Q, const std::vector<uint64_t>& counts, std::vector<double>& out) {
+  const size_t nb = (size_t)S * 25 * Q;
+  // summed sub-table over obs_base (covariates read_set, ref_base, quality)
+  std::vector<double> sums((size_t)S * 5 * Q, 0.0);
+  for (size_t i = 0; i < nb; i++) {
+    const size_t rs = i % S, rb = (i / S) % 5, q = i / (25 * (size_t)S);
+    sums[rs + (size_t)S * rb + (size_t)5 * S * q] += (double)counts[i];
+  }
+  out.resize(nb);
+  for (size_t i = 0; i < nb; i++) {
+    const size_t rs = i % S, rb = (i / S) % 5, q = i / (25 * (size_t)S);
+    out[i] = std::log10((double)counts[i] + 1.0) - std::log10(sums[rs + (size_t)S * rb + (size_t)5 * S * q] + 5.0);
+  }
+}
+
+std::string error_rates_text(int S, int Q, const std::vector<double>& log10p) {
+  std::string s;
+  s.reserve(log10p.size() * 24 + 128);
+  s += "read_set=" + std::to_string(S) + ",ref_base,obs_base,quality=" + std::to_string(Q) + "\n";
+  s += "read_set\tref_base\tobs_base\tquality\tlog10_probability\n";
+  char buf[96];
+  for (size_t i = 0; i < log10p.size(); i++) {
+    const size_t rs = i % S, rb = (i / S) % 5, ob = (i / (5 * (size_t)S)) % 5, q = i / (25 * (size_t)S);
+    snprintf(buf, sizeof(buf), "%zu\t%c\t%c\t%zu\t%g\n", rs, kIdx[rb], kIdx[ob], q, log10p[i]);
+    s += buf;
+  }
+  return s;
+}
+
+std::string coverage_text(const std::vector<uint32_t>& hist) {
+  std::string s = "coverage\tn\n";
+  char buf[48];
+  for (size_t j = 1; j < hist.size(); j++) {
+    snprintf(buf, sizeof(buf), "%zu\t%u\n", j, hist[j]);
+    s += buf;
+  }
+  return s;
+}
+
+std::vector<std::string> base_qual_texts(int S, int Q, const std::vector<uint64_t>& counts) {
+  const size_t blk = (size_t)25 * Q;
+  std::vector<double> ot((size_t)S * blk, 0.0);
+  for (size_t i = 0; i < counts.size(); i++) {
+    const size_t rs = i % S, rb = (i / S) % 5, ob = (i / (5 * (size_t)S)) % 5, q = i / (25 * (size_t)S);
+    ot[rs * blk + rb + 5 * ob + 25 * q] += (double)counts[i];
+  }
+  std::vector<std::string> texts;
+  double running = 0;
+  for (int r = 0; r < S; r++) {
+    for (size_t k = 0; k < blk; k++) {
+      const size_t i = (size_t)r * blk + k;
+      running += ot[i];
+      if (i % 5 == 4) {
+        for (size_t j = i - 4; j <= i; j++) {
+          if (running > 0) ot[j] /= running;
+          if (ot[j] == 0) ot[j] = NAN;
+        }
+        running = 0;
+      }
+    }
+    std::string s = "quality";
+    for (int b1 = 0; b1 < 5; b1++)
+      for (int b2 = 0; b2 < 5; b2++) {
+        s += "\t";
+        s += kIdx[b1];
+        s += kIdx[b2];
+      }
+    s += "\n";
+    for (int q = 0; q < Q; q++) {
+      s += std::to_string(q);
+      for (int b1 = 0; b1 < 5; b1++)
+        for (int b2 = 0; b2 < 5; b2++) {
+          const double v = ot[(size_t)r * blk + (size_t)q * 25 + b1 * 5 + b2];
+          s += "\t";
+          s += std::isnan(v) ? std::string("NA") : fmt_g(v);
+        }
+      s += "\n";
+    }
+    texts.push_back(s);
+  }
+  return texts;
+}
+
+bool parse_error_rates(const std::string& text, std::vector<double>& out, int& S, int& Q) {
+  std::istringstream in(text);
+  std::string line;
+  if (!std::getline(in, line)) return false;
+  S = Q = 0;
+  {
+    std::stringstream cs(line);
+    std::string part;
+    bool rs = false, rb = false, ob = false, q = false;
+    while (std::getline(cs, part, ',')) {
+      size_t eq = part.find('=');
+      std::string name = part.substr(0, eq);
+      if (name == "read_set") { rs = true; S = atoi(part.substr(eq + 1).c_str()); }
+      else if (name == "ref_base") rb = true;
+      else if (name == "obs_base") ob = true;
+      else if (name == "quality") { q = true; Q = atoi(part.substr(eq + 1).c_str()); }
+      else return false;   // only the covariates of breseq's main pipeline are supported
+    }
+    if (!(rs && rb && ob && q) || S <= 0 || Q <= 0) return false;
+  }
+  if (!std::getline(in, line)) return false;
+  const size_t nb = (size_t)S * 25 * Q;
+  out.assign(nb, 0.0);
+  for (size_t i = 0; i < nb; i++) {
+    if (!std::getline(in, line)) return false;
+    size_t tab = line.rfind('\t');
+    std::string last = (tab == std::string::npos) ? line : line.substr(tab + 1);
+    out[i] = strtod(last.c_str(), NULL);
+  }
+  return true;
+}
+
+/* ---------------------------------------------------------------- genome diff */
+
+static double reported(const RaOut& r, int b) {
+  if (r.n == 0 || b >= 5) return 0.0;
+  return (r.f[b] < 0.5 / (double)r.n) ? 0.0 : r.f[b];
+}
+
+void ra_entry_from_result(const RaOut& r, const std::string& seq_id, uint32_t position, uint32_t places, GdEntry& e) {
+  e.type = 0;
+  e.seq_id = seq_id;
+  e.pos = position;
+  e.ins = r.slot;
+  e.ref_base = std::string(1, kIdx[r.ref_idx]);
+  e.new_base = std::string(1, kIdx[r.variant]);
+  std::map<std::string, std::string> m;
+  m["score"] = fmt_double(r.variant_score, 1, false);
+  m["major_base"] = std::string(1, kIdx[r.major]);
+  m["minor_base"] = std::string(1, kIdx[r.minor]);
+  m["major_frequency"] = fmt_double(reported(r, r.major), places, true);
+  m["frequency"] = fmt_double(reported(r, r.variant), places, true);
+  std::string spec;
+  for (int b = 0; b < 5; b++) {
+    const double fr = reported(r, b);
+    if (fr <= 0.0) continue;
+    if (!spec.empty()) spec += ",";
+    spec += std::string(1, kIdx[b]) + ":" + fmt_double(fr, places, true);
+  }
+  m["allele_frequencies"] = spec;
+  m["frequency_lower"] = fmt_double(r.lower, places, true);
+  m["frequency_upper"] = fmt_double(r.upper, places, true);
+  m["ks_quality_p_value"] = fmt_double(r.ks_p, 5, true);
+  m["fisher_strand_p_value"] = fmt_double(r.fisher_p, 5, true);
+  auto cov = [&](int b) -> std::string {
+    if (b >= 5) return "0/0";
+    return std::to_string((int32_t)r.cov[b][0]) + "/" + std::to_string((int32_t)r.cov[b][1]);
+  };
+  m["ref_cov"] = cov(r.ref_idx);
+  m["new_cov"] = cov(r.variant);
+  m["major_cov"] = cov(r.major);
+  m["minor_cov"] = cov(r.minor);
+  m["total_cov"] = std::to_string(r.total_top) + "/" + std::to_string(r.total_bot);
+  e.fields.assign(m.begin(), m.end());
+}
+
+static int type_group(int t) { return t == 0 ? 3 : (t == 1 ? 4 : 7); }
+
+std::string write_gd(std::vector<GdEntry>& entries) {
+  // unique ids in add() order
+  std::vector<size_t> idx(entries.size());
+  for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+    const GdEntry &x = entries[a], &y = entries[b];
+    if (x.order_target != y.order_target) return x.order_target < y.order_target;
+    if (x.order_pos != y.order_pos) return x.order_pos < y.order_pos;
+    return x.order_sub < y.order_sub;
+  });
+  for (size_t k = 0; k < idx.size(); k++) entries[idx[k]].id = (uint32_t)(k + 1);
+  // cGenomeDiff::sort (genome_diff_entry.cpp:566-700)
+  std::vector<const GdEntry*> v;
+  for (const GdEntry& e : entries) v.push_back(&e);
+  std::stable_sort(v.begin(), v.end(), [](const GdEntry* a, const GdEntry* b) {
+    if (type_group(a->type) != type_group(b->type)) return type_group(a->type) < type_group(b->type);
+    if (a->seq_id != b->seq_id) return a->seq_id < b->seq_id;
+    if (a->pos != b->pos) return a->pos < b->pos;
+    if (a->type == 0) {
+      if (a->ins != b->ins) return a->ins < b->ins;
+      if (a->ref_base != b->ref_base) return a->ref_base < b->ref_base;
+      if (a->new_base != b->new_base) return a->new_base < b->new_base;
+    } else {
+      if (a->end != b->end) return a->end < b->end;
+      if (a->type == 1) {
+        if (a->start_range != b->start_range) return a->start_range < b->start_range;
+        if (a->end_range != b->end_range) return a->end_range < b->end_range;
+      }
+    }
+    return a->id < b->id;
+  });
+  std::string out = "#=GENOME_DIFF\t1.0\n";
+  static const char* tn[3] = {"RA", "MC", "UN"};
+  for (const GdEntry* e : v) {
+    out += tn[e->type];
+    out += "\t" + std::to_string(e->id) + "\t.\t" + e->seq_id + "\t";
+    if (e->type == 0) {
+      out += std::to_string(e->pos) + "\t" + std::to_string(e->ins) + "\t" + e->ref_base + "\t" + e->new_base;
+    } else if (e->type == 1) {
+      out += std::to_string(e->pos) + "\t" + std::to_string(e->end) + "\t" + std::to_string(e->start_range) + "\t" +
+             std::to_string(e->end_range);
+    } else {
+      out += std::to_string(e->pos) + "\t" + std::to_string(e->end);
+    }
+    for (const auto& kv : e->fields) {
+      if (kv.second.empty() || kv.first[0] == '_') continue;
+      out += "\t" + kv.first + "=" + kv.second;
+    }
+    out += "\n";
+  }
+  return out;
+}
+
+/* ---------------------------------------------------------------- test_RA_evidence */
+
+namespace {
+
+struct Line {
+  bool is_entry = false;
+  std::string raw;                 // header / comment lines
+  std::string type, id, evidence;
+  std::vector<std::string> spec;   // positional fields
+  std::map<std::string, std::string> kv;
+};
+
+double from_string_d(const std::string& s) {   // from_string<double>: 0 when not a number
+  std::istringstream is(s);
+  double t = 0.0;
+  is >> t;
+  if (is.fail()) return 0.0;
+  return t;
+}
+
+double double_from_string(const std::string& in) {   // common.h:939-950
+  std::string u = in;
+  for (char& c : u) c = (char)toupper((unsigned char)c);
+  if (u == "NA" || u == "#NA" || u == "NAN") return NAN;
+  if (u == "INF") return INFINITY;
+  if (u == "-INF") return -INFINITY;
+  return from_string_d(in);
+}
+
+std::vector<std::string> split(const std::string& s, char d) {
+  std::vector<std::string> v;
+  std::string cur;
+  for (char c : s) {
+    if (c == d) { v.push_back(cur); cur.clear(); }
+    else cur.push_back(c);
+  }
+  v.push_back(cur);
+  return v;
+}
+
+struct Ra {
+  std::map<std::string, std::string>& m;
+  bool has(const std::string& k) const { return m.count(k) != 0; }
+  std::string get(const std::string& k) const {
+    auto it = m.find(k);
+    return it == m.end() ? std::string() : it->second;
+  }
+  void add_reject(const std::string& reason, const std::string& field = "reject") {
+    if (!has(field)) { m[field] = reason; return; }
+    std::vector<std::string> cur = split(m[field], ',');
+    for (auto& c : cur) if (c == reason) return;
+    m[field] += "," + reason;
+  }
+};
+
+struct RefAccess {
+  const bsq_reference* ref;
+  int target(const std::string& seq_id) const {
+    for (int i = 0; ref && i < ref->n_targets; i++) if (seq_id == ref->names[i]) return i;
+    return -1;
+  }
+  std::string base1(int t, int64_t pos1) const {   // get_sequence_1(pos, pos); "" when outside
+    if (t < 0 || pos1 < 1 || pos1 > ref->lengths[t]) return "";
+    return std::string(1, ref->seqs[t][pos1 - 1]);
+  }
+  int64_t length(int t) const { return t < 0 ? 0 : ref->lengths[t]; }
+};
+
+double score_of(Ra& ra) {
+  if (ra.has("score")) return double_from_string(ra.get("score"));
+  double legacy = NAN;
+  if (ra.has("consensus_score")) legacy = double_from_string(ra.get("consensus_score"));
+  if (ra.has("polymorphism_score")) {
+    double p = double_from_string(ra.get("polymorphism_score"));
+    if (std::isnan(legacy) || p > legacy) legacy = p;
+  }
+  return legacy;
+}
+
+double bd_lower(double k, double n);
+double bd_upper(double k, double n);
+
+bool freq_bounds(Ra& ra, double& lo, double& hi) {
+  if (ra.has("frequency_lower") && ra.has("frequency_upper")) {
+    lo = from_string_d(ra.get("frequency_lower"));
+    hi = from_string_d(ra.get("frequency_upper"));
+    return true;
+  }
+  if (!ra.has("total_cov") || !ra.has("frequency")) return false;
+  std::vector<std::string> tb = split(ra.get("total_cov"), '/');
+  if (tb.size() < 2) return false;
+  double n = from_string_d(tb[0]) + from_string_d(tb[1]);
+  if (!(n > 0.0)) return false;
+  double k = from_string_d(ra.get("frequency")) * n;
+  lo = bd_lower(k, n);
+  hi = bd_upper(k, n);
+  return true;
+}
+
+void bias(Ra& ra, const bsq_ra_test_settings& s) {   // rejected_RA_polymorphism_bias
+  if (s.polymorphism_ks_quality_p_value_cutoff != 0 && ra.has("ks_quality_p_value"))
+    if (from_string_d(ra.get("ks_quality_p_value")) < s.polymorphism_ks_quality_p_value_cutoff) ra.add_reject("KS_BASE_QUALITY");
+  if (s.polymorphism_fisher_strand_p_value_cutoff != 0 && ra.has("fisher_strand_p_value"))
+    if (from_string_d(ra.get("fisher_strand_p_value")) < s.polymorphism_fisher_strand_p_value_cutoff) ra.add_reject("FISHER_STRAND");
+}
+
+void poly_coverage(Ra& ra, const bsq_ra_test_settings& s) {   // rejected_RA_polymorphism_coverage
+  if (s.polymorphism_minimum_variant_coverage_each_strand > 0) {
+    bool passed = true;
+    std::string ref_base = ra.has("ref_base") ? ra.get("ref_base") : "";
+    bool major_var = ra.has("major_base") && ra.get("major_base") != ref_base;
+    bool minor_var = ra.has("minor_base") && ra.get("minor_base") != ref_base;
+    if (!major_var && !minor_var) major_var = minor_var = true;
+    const double c = s.polymorphism_minimum_variant_coverage_each_strand;
+    if (major_var && ra.has("major_cov")) {
+      auto tb = split(ra.get("major_cov"), '/');
+      passed = passed && from_string_d(tb[0]) >= c;
+      passed = passed && from_string_d(tb.size() > 1 ? tb[1] : "") >= c;
+    }
+    if (minor_var && ra.has("minor_cov")) {
+      auto tb = split(ra.get("minor_cov"), '/');
+      passed = passed && from_string_d(tb[0]) >= c;
+      passed = passed && from_string_d(tb.size() > 1 ? tb[1] : "") >= c;
+    }
+    if (!passed) ra.add_reject("VARIANT_STRAND_COVERAGE");
+  }
+  if (s.polymorphism_minimum_total_coverage_each_strand > 0) {
+    auto tb = split(ra.get("total_cov"), '/');
+    double top = from_string_d(tb[0]), bot = from_string_d(tb.size() > 1 ? tb[1] : "");
+    if (!(top >= s.polymorphism_minimum_total_coverage_each_strand && bot >= s.polymorphism_minimum_total_coverage_each_strand))
+      ra.add_reject("TOTAL_STRAND_COVERAGE");
+  }
+  if (s.polymorphism_minimum_variant_coverage > 0) {
+    bool passed = true;
+    auto tb = split(ra.get("major_cov"), '/');
+    passed = passed && (from_string_d(tb[0]) + from_string_d(tb.size() > 1 ? tb[1] : "") >= s.polymorphism_minimum_variant_coverage);
+    tb = split(ra.get("minor_cov"), '/');
+    passed = passed && (from_string_d(tb[0]) + from_string_d(tb.size() > 1 ? tb[1] : "") >= s.polymorphism_minimum_variant_coverage);
+    if (!passed) ra.add_reject("VARIANT_COVERAGE");
+  }
+  if (s.polymorphism_minimum_total_coverage > 0) {
+    auto tb = split(ra.get("total_cov"), '/');
+    if (from_string_d(tb[0]) + from_string_d(tb.size() > 1 ? tb[1] : "") < s.polymorphism_minimum_total_coverage) ra.add_reject("TOTAL_COVERAGE");
+  }
+}
+
+void cons_coverage(Ra& ra, const bsq_ra_test_settings& s) {   // rejected_RA_consensus_coverage
+  auto two = [&](const std::string& k, double& a, double& b) {
+    auto tb = split(ra.get(k), '/');
+    a = from_string_d(tb[0]);
+    b = from_string_d(tb.size() > 1 ? tb[1] : "");
+  };
+  double a, b;
+  if (s.consensus_minimum_variant_coverage_each_strand > 0) {
+    two("major_cov", a, b);
+    if (a < s.consensus_minimum_variant_coverage_each_strand || b < s.consensus_minimum_variant_coverage_each_strand) ra.add_reject("VARIANT_STRAND_COVERAGE");
+  }
+  if (s.consensus_minimum_total_coverage_each_strand > 0) {
+    two("total_cov", a, b);
+    if (a < s.consensus_minimum_total_coverage_each_strand || b < s.consensus_minimum_total_coverage_each_strand) ra.add_reject("TOTAL_STRAND_COVERAGE");
+  }
+  if (s.consensus_minimum_variant_coverage > 0) {
+    two("major_cov", a, b);
+    if (a + b < s.consensus_minimum_variant_coverage) ra.add_reject("VARIANT_COVERAGE");
+  }
+  if (s.consensus_minimum_total_coverage > 0) {
+    two("total_cov", a, b);
+    if (a + b < s.consensus_minimum_total_coverage) ra.add_reject("TOTAL_COVERAGE");
+  }
+}
+
+void homopolymer(Ra& ra, const RefAccess& R, int32_t indel_len, int32_t surround_len, bool no_indels) {   // :400-487
+  const std::string rb = ra.get("ref_base"), nb = ra.get("new_base");
+  if (indel_len && (rb == "." || nb == ".")) {
+    const std::string seq_id = ra.get("seq_id");
+    const int t = R.target(seq_id);
+    int32_t mut_pos = (int32_t)from_string_d(ra.get("position"));
+    bool is_ins = from_string_d(ra.get("insert_position")) > 0;
+    std::string mb = (rb == ".") ? nb : rb;
+    int32_t hl = 0;
+    bool no_match = false;
+    if (is_ins) {
+      if (R.base1(t, mut_pos) != mb) {
+        mut_pos++;
+        if (mut_pos > (int32_t)R.length(t)) no_match = true;
+        else if (R.base1(t, mut_pos) != mb) no_match = true;
+      }
+    }
+    if (!no_match) {
+      int32_t sp = mut_pos - 1;
+      while (R.base1(t, sp) == mb && sp > 0) sp--;
+      sp++;
+      int32_t ep = mut_pos + 1;
+      while (R.base1(t, ep) == mb && ep <= (int32_t)R.length(t)) ep++;
+      ep--;
+      hl = ep - sp + 1;
+    }
+    if (hl >= indel_len) ra.add_reject("INDEL_HOMOPOLYMER");
+  }
+  if (surround_len && rb != "." && nb != ".") {
+    const int t = R.target(ra.get("seq_id"));
+    int32_t mut_pos = (int32_t)from_string_d(ra.get("position"));
+    int32_t sp = mut_pos - 1;
+    while (sp >= 1) {
+      if (R.base1(t, sp) != nb) break;
+      sp--;
+    }
+    sp++;
+    int32_t ep = mut_pos + 1;
+    while (ep <= (int32_t)R.length(t)) {
+      if (R.base1(t, ep) != nb) break;
+      ep++;
+    }
+    ep--;
+    if (sp < mut_pos && ep > mut_pos && ep - sp + 1 >= surround_len) ra.add_reject("SURROUNDING_HOMOPOLYMER");
+  }
+  if (no_indels && (rb == "." || nb == ".")) ra.add_reject("POLYMORPHIC_INDEL");
+}
+
+bool test_mode(Ra& ra, const RefAccess& R, const bsq_ra_test_settings& s, bool poly_mode) {   // :522-685
+  const double score = score_of(ra);
+  double lower, upper;
+  if (!freq_bounds(ra, lower, upper)) lower = upper = from_string_d(ra.get("frequency"));
+  if (score < s.mutation_log10_e_value_cutoff) ra.add_reject("SCORE_CUTOFF");
+  const double fcons = poly_mode ? upper : lower;
+  if (s.consensus_frequency_cutoff > 0.0 && fcons < s.consensus_frequency_cutoff) ra.add_reject("FREQUENCY_CUTOFF");
+  cons_coverage(ra, s);
+  homopolymer(ra, R, s.consensus_reject_indel_homopolymer_length, s.consensus_reject_surrounding_homopolymer_length, false);
+  if (!ra.has("reject")) {
+    ra.m["prediction"] = "consensus";
+    return ra.get("ref_base") == ra.get("major_base");
+  }
+  ra.m["consensus_reject"] = ra.m["reject"];
+  ra.m.erase("reject");
+  if (score < s.polymorphism_log10_e_value_cutoff) ra.add_reject("SCORE_CUTOFF");
+  if (s.polymorphism_frequency_cutoff > 0.0 && lower < s.polymorphism_frequency_cutoff) ra.add_reject("FREQUENCY_CUTOFF");
+  bias(ra, s);
+  poly_coverage(ra, s);
+  homopolymer(ra, R, s.polymorphism_reject_indel_homopolymer_length, s.polymorphism_reject_surrounding_homopolymer_length,
+              s.polymorphism_no_indels != 0);
+  if (!ra.has("reject")) {
+    ra.m["prediction"] = "polymorphism";
+    return false;
+  }
+  ra.m["polymorphism_reject"] = ra.m["reject"];
+  ra.m.erase("reject");
+  if (!poly_mode) return true;
+  ra.m["prediction"] = "polymorphism";
+  ra.m["reject"] = ra.m["polymorphism_reject"];
+  ra.m.erase("polymorphism_reject");
+  return false;
+}
+
+// Clopper-Pearson bounds for evidence written without recorded bounds (stats.cpp:2395-2415):
+// lower = incbi(k, n-k+1, 0.05), upper = incbi(k+1, n-k, 0.95) via bisection on the regularized
+// incomplete beta function.
+double ibeta(double a, double b, double x) {   // continued fraction (Numerical Recipes betacf form)
+  if (x <= 0.0) return 0.0;
+  if (x >= 1.0) return 1.0;
+  const double lbt = std::lgamma(a + b) - std::lgamma(a) - std::lgamma(b) + a * std::log(x) + b * std::log(1.0 - x);
+  auto cf = [](double aa, double bb, double xx) {
+    const double tiny = 1e-300;
+    double qab = aa + bb, qap = aa + 1.0, qam = aa - 1.0, c = 1.0, d = 1.0 - qab * xx / qap;
+    if (std::fabs(d) < tiny) d = tiny;
+    d = 1.0 / d;
+    double h = d;
+    for (int m = 1; m < 400; m++) {
+      int m2 = 2 * m;
+      double an = m * (bb - m) * xx / ((qam + m2) * (aa + m2));
+      d = 1.0 + an * d; if (std::fabs(d) < tiny) d = tiny;
+      c = 1.0 + an / c; if (std::fabs(c) < tiny) c = tiny;
+      d = 1.0 / d; h *= d * c;
+      an = -(aa + m) * (qab + m) * xx / ((aa + m2) * (qap + m2));
+      d = 1.0 + an * d; if (std::fabs(d) < tiny) d = tiny;
+      c = 1.0 + an / c; if (std::fabs(c) < tiny) c = tiny;
+      d = 1.0 / d;
+      double del = d * c;
+      h *= del;
+      if (std::fabs(del - 1.0) < 1e-15) break;
+    }
+    return h;
+  };
+  if (x < (a + 1.0) / (a + b + 2.0)) return std::exp(lbt) * cf(a, b, x) / a;
+  return 1.0 - std::exp(lbt) * cf(b, a, 1.0 - x) / b;
+}
+
+double ibeta_inv(double a, double b, double y) {
+  double lo = 0.0, hi = 1.0;
+  for (int i = 0; i < 200; i++) {
+    double mid = 0.5 * (lo + hi);
+    if (ibeta(a, b, mid) < y) lo = mid;
+    else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+double bd_lower(double k, double n) { return (k <= 0) ? 0.0 : ibeta_inv(k, n - k + 1, 0.05); }
+double bd_upper(double k, double n) { return (k >= n) ? 1.0 : ibeta_inv(k + 1, n - k, 0.95); }
+
+}  // namespace
+
+std::string test_ra_evidence_text(const std::string& text, const bsq_reference* ref, const bsq_ra_test_settings& s) {
+  std::vector<Line> lines;
+  std::istringstream in(text);
+  std::string l;
+  while (std::getline(in, l)) {
+    Line L;
+    if (l.empty() || l[0] == '#') {
+      L.raw = l;
+      lines.push_back(L);
+      continue;
+    }
+    std::vector<std::string> f = split(l, '\t');
+    if (f.size() < 3) throw BsqError(BSQ_ERR_FORMAT, "malformed genome diff line: " + l);
+    L.is_entry = true;
+    L.type = f[0];
+    L.id = f[1];
+    L.evidence = f[2];
+    for (size_t i = 3; i < f.size(); i++) {
+      size_t eq = f[i].find('=');
+      if (eq == std::string::npos) L.spec.push_back(f[i]);
+      else L.kv[f[i].substr(0, eq)] = f[i].substr(eq + 1);
+    }
+    lines.push_back(L);
+  }
+  RefAccess R{ref};
+  std::string out;
+  for (Line& L : lines) {
+    if (!L.is_entry) {
+      out += L.raw + "\n";
+      continue;
+    }
+    if (L.type == "RA") {
+      if (L.spec.size() < 5) throw BsqError(BSQ_ERR_FORMAT, "RA entry without its 5 positional fields");
+      std::map<std::string, std::string> m = L.kv;
+      m["seq_id"] = L.spec[0];
+      m["position"] = L.spec[1];
+      m["insert_position"] = L.spec[2];
+      m["ref_base"] = L.spec[3];
+      m["new_base"] = L.spec[4];
+      Ra ra{m};
+      if (!ra.has("score") && !ra.has("consensus_score") && !ra.has("polymorphism_score"))
+        throw BsqError(BSQ_ERR_FORMAT, "Expected field 'score' in evidence item");
+      if (!ra.has("frequency")) throw BsqError(BSQ_ERR_FORMAT, "Expected field 'frequency' in evidence item");
+      bool del = test_mode(ra, R, s, s.polymorphism_prediction != 0);
+      if (ra.has("user_defined")) del = false;
+      if (del) continue;
+      for (const char* k : {"seq_id", "position", "insert_position", "ref_base", "new_base"}) m.erase(k);
+      L.kv = m;
+    }
+    out += L.type + "\t" + L.id + "\t" + L.evidence;
+    for (const auto& sp : L.spec) out += "\t" + sp;
+    for (const auto& kv : L.kv) {
+      if (kv.second.empty() || kv.first[0] == '_') continue;
+      out += "\t" + kv.first + "=" + kv.second;
+    }
+    out += "\n";
+  }
+  return out;
+}
+
+}  // namespace bsq

@@ -1,0 +1,97 @@
+// extra.hip -- per-position reference trims on the GPU (SequenceTrims, calculate_trims.cpp:67-138).
+//
+// The reference scans each position for perfect tandem repeats of unit 1..18 and max-scatters the
+// resulting trim lengths forward.  Here the scatter is turned into a gather so every output byte is
+// written by exactly one thread: k_repeat_extent computes each position's maximal repeat extent m[p]
+// (capped at 255), and k_trim_gather computes for each x
+//   right[x] = max over p in (x-255, x] with p + m[p] > x of (x - p + 1)
+//   left[x]  = max over the same p of (m[p] - (x - p)).
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "host_util.h"
+
+namespace bsq {
+
+__global__ void __launch_bounds__(256) k_repeat_extent(const char* __restrict__ s, int64_t len, uint8_t* __restrict__ m) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= len) return;
+  uint32_t mx = 1;
+  for (int size = 1; size <= 18; size++) {
+    // repeat_match(pos, size, num): s[p+i] == s[p+i+size*num] for i < size, all indexes < len
+    uint32_t num = 1;
+    for (;;) {
+      const int64_t off = (int64_t)size * num;
+      if (p + size - 1 + off >= len) break;
+      bool ok = true;
+      for (int i = 0; i < size; i++)
+        if (s[p + i] != s[p + i + off]) {
+          ok = false;
+          break;
+        }
+      if (!ok) break;
+      num++;
+      if ((uint64_t)num * size > 255) break;   // capped below; longer runs change nothing
+    }
+    if (num > 1 && num * size > mx) mx = num * size;
+  }
+  if (mx > 255) mx = 255;
+  m[p] = (uint8_t)mx;
+}
+
+__global__ void __launch_bounds__(256) k_trim_gather(const uint8_t* __restrict__ m, int64_t len, uint8_t* __restrict__ out) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= len) return;
+  uint32_t right = 0, left = 0;
+  const int64_t p0 = (x >= 254) ? x - 254 : 0;
+  for (int64_t p = p0; p <= x; p++) {
+    const uint32_t mp = m[p];
+    const uint32_t o = (uint32_t)(x - p);
+    if (o < mp) {
+      right = max(right, o + 1);
+      left = max(left, mp - o);
+    }
+  }
+  out[x] = (uint8_t)left;
+  out[len + x] = (uint8_t)right;
+}
+
+}  // namespace bsq
+
+using namespace bsq;
+
+extern "C" int bsq_sequence_trims(const char* seq, int64_t len, uint8_t* out, bsq_status* status) {
+  char* d_s = nullptr;
+  uint8_t *d_m = nullptr, *d_o = nullptr;
+  auto cleanup = [&]() {
+    if (d_s) (void)hipFree(d_s);
+    if (d_m) (void)hipFree(d_m);
+    if (d_o) (void)hipFree(d_o);
+  };
+  try {
+    if (!seq || !out || len < 0) throw BsqError(BSQ_ERR_ARG, "bad arguments");
+    if (len == 0) {
+      set_status(status, BSQ_OK, "");
+      return BSQ_OK;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw BsqError(BSQ_ERR_DEVICE, "no HIP device available");
+    if (hipMalloc((void**)&d_s, (size_t)len) != hipSuccess || hipMalloc((void**)&d_m, (size_t)len) != hipSuccess ||
+        hipMalloc((void**)&d_o, 2 * (size_t)len) != hipSuccess)
+      throw BsqError(BSQ_ERR_DEVICE, "hipMalloc failed");
+    if (hipMemcpy(d_s, seq, (size_t)len, hipMemcpyHostToDevice) != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, "copy failed");
+    const int64_t blocks = (len + 255) / 256;
+    hipLaunchKernelGGL(k_repeat_extent, dim3((unsigned)blocks), dim3(256), 0, 0, d_s, len, d_m);
+    hipLaunchKernelGGL(k_trim_gather, dim3((unsigned)blocks), dim3(256), 0, 0, d_m, len, d_o);
+    if (hipGetLastError() != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, "kernel launch failed");
+    if (hipMemcpy(out, d_o, 2 * (size_t)len, hipMemcpyDeviceToHost) != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, "copy failed");
+    cleanup();
+    set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& e) {
+    cleanup();
+    set_status(status, e.code, e.what());
+    return e.code;
+  }
+}

@@ -1,0 +1,90 @@
+// host_util.cpp -- see host_util.h.
+#include "host_util.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace bsq {
+
+void set_status(bsq_status* st, int code, const std::string& msg) {
+  if (!st) return;
+  st->code = code;
+  snprintf(st->message, sizeof(st->message), "%s", msg.c_str());
+}
+
+std::string fmt_double(double v, unsigned precision, bool scientific) {
+  if (std::isnan(v)) return "NA";
+  char buf[512];
+  snprintf(buf, sizeof(buf), scientific ? "%.*e" : "%.*f", (int)precision, v);
+  std::string s(buf);
+  // a 3-digit exponent with a leading zero is shortened (only non-glibc runtimes print one)
+  if (scientific && s.size() >= 3 && s[s.size() - 3] == '0') s.erase(s.size() - 3, 1);
+  return s;
+}
+
+std::string fmt_g(double v) {
+  char buf[64];
+  snprintf(buf, sizeof(buf), "%g", v);
+  return buf;
+}
+
+static bool repeat_at(const char* s, int64_t len, int64_t pos, int64_t size, int64_t num) {
+  const int64_t off = size * num;
+  for (int64_t i = 0; i < size; i++) {
+    if (pos + i >= len || pos + i + off >= len) return false;
+    if (s[pos + i] != s[pos + i + off]) return false;
+  }
+  return true;
+}
+
+void host_sequence_trims(const char* s, int64_t len, uint8_t* t) {
+  memset(t, 0, 2 * (size_t)len);
+  for (int64_t p = 0; p < len; p++) {
+    uint32_t mx = 1;
+    for (int64_t size = 1; size <= 18; size++) {
+      int64_t num = 1;
+      while (repeat_at(s, len, p, size, num)) num++;
+      if (num > 1) mx = std::max<uint32_t>(mx, (uint32_t)(num * size));
+    }
+    if (mx > 255) mx = 255;
+    for (uint32_t o = 0; o < mx; o++) {
+      if (p + o >= len) break;
+      uint8_t v = (uint8_t)(o + 1);
+      if (v > t[len + p + o]) t[len + p + o] = v;
+    }
+    for (uint32_t o = 0; o < mx; o++) {
+      if (p + o >= len) break;
+      uint8_t v = (uint8_t)(mx - o);
+      if (v > t[p + o]) t[p + o] = v;
+    }
+  }
+}
+
+static char comp(char c) {
+  switch (c) {
+    case 'A': return 'T';
+    case 'C': return 'G';
+    case 'G': return 'C';
+    case 'T': return 'A';
+  }
+  return c;
+}
+
+void host_edge_trims(const std::string& seq, uint32_t& left, uint32_t& right) {
+  left = right = 1;
+  const int64_t n = (int64_t)seq.size();
+  std::string rc(seq.rbegin(), seq.rend());
+  for (char& c : rc) c = comp(c);
+  for (int64_t size = 1; size <= 18; size++) {
+    int64_t num = 1;
+    while (repeat_at(seq.data(), n, 0, size, num)) num++;
+    if (num > 1) left = std::max<uint32_t>(left, (uint32_t)(num * size));
+    num = 1;
+    while (repeat_at(rc.data(), n, 0, size, num)) num++;
+    if (num > 1) right = std::max<uint32_t>(right, (uint32_t)(num * size));
+  }
+}
+
+}  // namespace bsq

@@ -1,0 +1,868 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the pileup-scoring engine.
+//
+//  k_error_count      pass 1: unique-only coverage histogram + covariate count table
+//                     (error_count_pileup::pileup_callback, error_count.cpp:125-199, 854-997)
+//  k_identify_columns pass 2, insert slot 0 of every column: coverage tallies, per-base counts,
+//                     summed per-genotype log10 likelihoods in read order, pure-genotype call, and
+//                     a conservative screen that routes columns to the exact FP64 scorer
+//                     (identify_mutations.cpp:1309-1821)
+//  k_insert_slots     insert slots >= 1 of columns whose unique reads carry insertions
+//  k_score_candidates wave per candidate: allele-frequency EM, presence score, profile-likelihood
+//                     bounds, Fisher strand and KS quality tests (identify_mutations.cpp:3009-3318)
+//  k_runs / k_mc_detail  MC and UN interval detection (identify_mutations.cpp:2262-2344, 2972-3007)
+//
+// Column kernels map one 64-wide wavefront onto 64 consecutive reference columns and walk the
+// alignments overlapping that window in file order: every lane sees the same alignment in the
+// same iteration (scalar loads of the 40-byte record), and the bases/qualities of consecutive
+// columns are consecutive bytes of that alignment, so the loads coalesce.  Each lane accumulates
+// its column's sums sequentially in file order, which is the reference's summation order.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace bsq {
+
+#define WAVE 64
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Pass 1: error_count                                                                         */
+
+__global__ void __launch_bounds__(256) k_error_count(EcParams P) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* table = lds;                         // P.n_bins counters
+  uint32_t* hist = lds + P.n_bins;               // P.n_groups_lds * P.hist_lds bins
+  const int tid = threadIdx.x;
+  const int nlds = P.n_bins + P.n_groups_lds * P.hist_lds;
+  for (int i = tid; i < nlds; i += blockDim.x) lds[i] = 0;
+  __syncthreads();
+
+  const int wave = tid / WAVE, lane = tid % WAVE;
+  const int waves_per_block = blockDim.x / WAVE;
+  const int64_t chunk = (P.n_windows + gridDim.x - 1) / gridDim.x;
+  const int64_t w_begin = (int64_t)blockIdx.x * chunk;
+  const int64_t w_end = min<int64_t>(w_begin + chunk, P.n_windows);
+  uint32_t err = 0;
+  const int S = P.read_set_max, Q = P.quality_max;
+
+  for (int64_t w = w_begin + wave; w < w_end; w += waves_per_block) {
+    const int p = (int)(w * WAVE + lane);
+    const bool lane_ok = (p < P.n_pos) && (P.pos_target[p] >= 0);
+    const int rb = uniform_i(P.win_rb[w]), re = uniform_i(P.win_re[w]);
+    int uniq = 0;
+    bool has_red = false;
+    for (int r = rb; r < re; r++) {
+      const DRead R = P.reads[r];
+      int qpos, indel;
+      bool is_del;
+      if (!lane_ok || !entry_at(R, P.cigar, p, qpos, indel, is_del, err)) continue;
+      if (is_del) continue;
+      if (R.x1 > 1) {
+        has_red = true;
+        continue;
+      }
+      ++uniq;
+      const int rev = (R.flags & kReadReversed) ? 1 : 0;
+      const int rs = R.rfile;
+      if (rs >= S) {
+        err |= kErrReadSet;
+        continue;
+      }
+      const uint8_t* sq = P.bases + R.seq_off;
+      const uint8_t* ql = P.quals + R.seq_off;
+      // (1) base substitution / match, read-strand space (error_count.cpp:877-903)
+      {
+        uint8_t ob = sq[qpos];
+        char rc = P.ref[p];
+        if (ob != 15 && rc != 'N') {
+          if (rev) {
+            ob = comp_bam(ob);
+            rc = comp_char(rc);
+          }
+          const int oi = bam_index(ob), ri = char_index(rc);
+          const int q = ql[qpos];
+          if (oi < 0 || oi > 3) err |= kErrBaseCode;
+          else if (ri < 0 || ri > 3) err |= kErrRefChar;
+          else if (q >= Q) err |= kErrQuality;
+          else atomicAdd(&table[rs + S * ri + 5 * S * oi + 25 * S * q], 1u);
+        }
+      }
+      if (indel == 0) {   // '..' pair (:909-928)
+        if (qpos < (int)R.qe0) {
+          const int mq = qpos + 1 - rev;
+          if (mq < 0 || mq >= (int)R.l_qseq) {
+            err |= kErrQueryIndex;
+          } else {
+            const uint8_t ob = sq[mq];
+            const char rc = P.ref[p + 1 - rev];
+            const int q = ql[mq];
+            if (ob != 15 && rc != 'N') {
+              if (q >= Q) err |= kErrQuality;
+              else atomicAdd(&table[rs + S * 4 + 5 * S * 4 + 25 * S * q], 1u);
+            }
+          }
+        }
+      } else if (indel == -1) {   // deletion of one base (:934-956)
+        const int mq = qpos + 1 - rev;
+        if (mq < 0 || mq >= (int)R.l_qseq) {
+          err |= kErrQueryIndex;
+        } else {
+          const uint8_t ob = sq[mq];
+          char rc = P.ref[p + 1];
+          if (ob != 15 && rc != 'N') {
+            if (rev) rc = comp_char(rc);
+            const int ri = char_index(rc);
+            const int q = ql[mq];
+            if (ri < 0 || ri > 4) err |= kErrRefChar;
+            else if (q >= Q) err |= kErrQuality;
+            else atomicAdd(&table[rs + S * ri + 5 * S * 4 + 25 * S * q], 1u);
+          }
+        }
+      } else if (indel == 1) {   // insertion of one base (:963-982)
+        const int mq = qpos + 1;
+        if (mq < 0 || mq >= (int)R.l_qseq) {
+          err |= kErrQueryIndex;
+        } else if (mq <= (int)R.qe0 && mq >= (int)R.qs0) {
+          uint8_t ob = sq[mq];
+          if (ob != 15) {
+            if (rev) ob = comp_bam(ob);
+            const int oi = bam_index(ob);
+            const int q = ql[mq];
+            if (oi < 0 || oi > 3) err |= kErrBaseCode;
+            else if (q >= Q) err |= kErrQuality;
+            else atomicAdd(&table[rs + S * 4 + 5 * S * oi + 25 * S * q], 1u);
+          }
+        }
+      }
+    }
+    if (lane_ok && !has_red) {
+      const int t = P.pos_target[p];
+      const int g = P.target_group[t];
+      if (g < P.n_groups_lds && uniq < P.hist_lds) atomicAdd(&hist[g * P.hist_lds + uniq], 1u);
+      else if (uniq < P.hist_len) atomicAdd(&P.hist[(int64_t)g * P.hist_len + uniq], 1ull);
+      else err |= kErrCapacity;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < P.n_bins; i += blockDim.x)
+    if (table[i]) atomicAdd(&P.counts[i], (unsigned long long)table[i]);
+  for (int i = tid; i < P.n_groups_lds * P.hist_lds; i += blockDim.x) {
+    if (hist[i]) {
+      const int g = i / P.hist_lds, c = i % P.hist_lds;
+      atomicAdd(&P.hist[(int64_t)g * P.hist_len + c], (unsigned long long)hist[i]);
+    }
+  }
+  if (err) atomicOr(P.err, err);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Pass 2: per-column tallies at insert slot 0                                                 */
+
+// pure_genotype_call (identify_mutations.cpp:3398-3433)
+__device__ __forceinline__ void pure_genotype(const double lps[5], uint32_t n, int& best, double& score) {
+  best = 5;
+  score = __longlong_as_double(0x7ff8000000000000LL);
+  if (n == 0) return;
+  best = 0;
+  for (int b = 1; b < 5; b++)
+    if (lps[b] > lps[best]) best = b;
+  double off = -1.7976931348623157e308;
+  for (int b = 0; b < 5; b++)
+    if (b != best) off = fmax(off, lps[b]);
+  double tot = 0.0;
+  for (int b = 0; b < 5; b++)
+    if (b != best) tot += pow(10.0, lps[b] - off);
+  double lt = log10(tot);
+  lt += off;
+  score = lps[best] - lt;
+}
+
+// Conservative screen: true unless no allele can reach the presence-score cutoff.  The full-model
+// log likelihood is at most Sum_i max_b log10 P_i(b); the null model's EM starts from the
+// Laplace-smoothed counts and never decreases its likelihood, so its reported value is at least
+// the likelihood at that start, bounded below per read by its observed (or best other) allele.
+__device__ __forceinline__ bool screen_passes(const ColumnAcc& A, int ref_idx, double log10_ref_len, double cutoff,
+                                              double margin) {
+  if (A.n == 0) return false;
+  for (int v = 0; v < 5; v++) {
+    if (v == ref_idx) continue;
+    const float init_total = 2.0f + (float)(A.n - A.cnt[v]);
+    float lb = 0.0f;
+    for (int b = 0; b < 5; b++) {
+      if (b == v) continue;
+      const float f0 = (0.5f + (float)A.cnt[b]) / init_total;
+      lb += (float)A.cnt[b] * log10f(f0) + A.S[b * 5 + b];
+    }
+    if (A.cnt[v] > 0) {
+      float best = -3.0e38f;
+      for (int c = 0; c < 5; c++) {
+        if (c == v) continue;
+        const float f0 = (0.5f + (float)A.cnt[c]) / init_total;
+        best = fmaxf(best, (float)A.cnt[v] * log10f(f0) + A.S[v * 5 + c]);
+      }
+      lb += best;
+    }
+    const double ub = (double)A.SL - (double)lb - log10_ref_len;
+    if (ub >= cutoff - margin) return true;
+  }
+  return false;
+}
+
+__global__ void __launch_bounds__(256) k_identify_columns(ImParams P) {
+  const int tid = threadIdx.x;
+  const int wave = tid / WAVE, lane = tid % WAVE;
+  const int waves_per_block = blockDim.x / WAVE;
+  const int64_t chunk = (P.n_windows + gridDim.x - 1) / gridDim.x;
+  const int64_t w_begin = (int64_t)blockIdx.x * chunk;
+  const int64_t w_end = min<int64_t>(w_begin + chunk, P.n_windows);
+  uint32_t err = 0;
+  const int S = P.read_set_max, Q = P.quality_max;
+
+  for (int64_t w = w_begin + wave; w < w_end; w += waves_per_block) {
+    const int p = (int)(w * WAVE + lane);
+    const int t = (p < P.n_pos) ? P.pos_target[p] : -1;
+    const bool lane_ok = (t >= 0) && P.targets[t].scored && (P.targets[t].prop_cutoff >= 0.0);
+    const int rb = uniform_i(P.win_rb[w]), re = uniform_i(P.win_re[w]);
+    double lps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double red_bot = 0.0, red_top = 0.0;
+    int uniq_bot = 0, uniq_top = 0;
+    bool insert_slot = false;
+    ColumnAcc A;
+    A.reset();
+    for (int r = rb; r < re; r++) {
+      const DRead R = P.reads[r];
+      int qpos, indel;
+      bool is_del;
+      if (!lane_ok || !entry_at(R, P.cigar, p, qpos, indel, is_del, err)) continue;
+      const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
+      uint8_t read_base = '.';
+      bool past = true;
+      if (ondel >= 0) {
+        read_base = P.bases[R.seq_off + qpos];
+        past = false;
+      }
+      if (read_base == 15) continue;
+      const bool top = !(R.flags & kReadReversed);
+      const bool trimmed = is_trimmed(R, qpos, past);
+      if (R.x1 == 1) {
+        if (top) ++uniq_top;
+        else ++uniq_bot;
+        if (ondel > 0) insert_slot = true;
+      } else {
+        if (top) red_top += 1.0 / (double)R.x1;
+        else red_bot += 1.0 / (double)R.x1;
+      }
+      if (R.x1 > 1 || trimmed) continue;
+      int obs, q;
+      if (!slot_observation(R, P.bases, P.quals, qpos, ondel, 0, read_base, obs, q, err)) continue;
+      if (q < P.base_quality_cutoff) continue;
+      if ((int)R.rfile >= S) {
+        err |= kErrReadSet;
+        continue;
+      }
+      if (q >= Q) {
+        err |= kErrQuality;
+        continue;
+      }
+      const uint32_t key = ((((uint32_t)R.mslot * S + R.rfile) * Q + q) * 10u) + obs * 2u + (top ? 0u : 1u);
+      const double* lp = P.tab_lp + (size_t)key * 5;
+      double l[5];
+      for (int b = 0; b < 5; b++) l[b] = lp[b];
+      for (int b = 0; b < 5; b++) lps[b] += l[b];
+      A.add(obs, top, l, (float)P.tab_r[(size_t)key * 6 + 5]);
+    }
+    if (!lane_ok) {
+      if (p < P.n_pos) {
+        P.col_flags[p] = 0;
+        P.col_unique[p] = 0;
+        P.col_total[p] = 0;
+      }
+      continue;
+    }
+    // position_coverage::sum (identify_mutations.h:115-120)
+    const double u1 = (double)uniq_bot + (double)uniq_top;
+    const double r1 = red_bot + red_top;
+    const int total = (int)round(u1) + (int)round(r1);
+    int best;
+    double score;
+    pure_genotype(lps, A.n, best, score);
+    const double cbs = score - P.log10_ref_length;
+    const char refc = P.ref[p];
+    const int ref_idx = char_index(refc);
+    if (ref_idx < 0) err |= kErrRefChar;
+    uint8_t fl = kPosValid;
+    if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
+    if (r1 > 0.0) fl |= kPosRedundant;
+    if (insert_slot) fl |= kPosInsert;
+    if (!isnan(cbs) && (double)total <= P.targets[t].seed_cutoff) fl |= kPosSeed;
+    const bool cons = (best != ref_idx) && !isnan(cbs) && (cbs > 0.0);
+    if (cons) fl |= kPosConsensus;
+    bool cand = cons || P.screen_off ||
+                screen_passes(A, ref_idx, P.log10_ref_length, P.polymorphism_cutoff, P.screen_margin);
+    if (P.screen_off && A.n == 0) cand = false;
+    if (cand) {
+      fl |= kPosCandidate;
+      const int slot = atomicAdd(P.n_cand, 1);
+      if (slot < P.cand_cap) {
+        P.cand[slot] = make_int2(p, 0);
+        P.cand_cons[slot] = cons ? 1 : 0;
+      } else {
+        err |= kErrCapacity;
+      }
+    }
+    if (insert_slot) {
+      const int slot = atomicAdd(P.n_ins, 1);
+      if (slot < P.ins_cap) P.ins[slot] = p;
+      else err |= kErrCapacity;
+    }
+    P.col_flags[p] = fl;
+    P.col_unique[p] = uniq_bot + uniq_top;
+    P.col_total[p] = total;
+  }
+  if (err) atomicOr(P.err, err);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Insert slots >= 1: one thread per column carrying an insertion                              */
+
+__global__ void __launch_bounds__(64) k_insert_slots(ImParams P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= min(*P.n_ins, P.ins_cap)) return;
+  const int p = P.ins[i];
+  const int64_t w = p / WAVE;
+  const int rb = P.win_rb[w], re = P.win_re[w];
+  const int S = P.read_set_max, Q = P.quality_max;
+  uint32_t err = 0;
+  bool next = true;
+  for (int slot = 1; next && slot < 65536; slot++) {
+    next = false;
+    double lps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    ColumnAcc A;
+    A.reset();
+    for (int r = rb; r < re; r++) {
+      const DRead R = P.reads[r];
+      int qpos, indel;
+      bool is_del;
+      if (!entry_at(R, P.cigar, p, qpos, indel, is_del, err)) continue;
+      const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
+      uint8_t read_base = '.';
+      bool past = true;
+      if (ondel >= slot) {
+        read_base = P.bases[R.seq_off + qpos + slot];
+        past = false;
+      }
+      if (read_base == 15) continue;
+      const bool top = !(R.flags & kReadReversed);
+      const bool trimmed = is_trimmed(R, qpos, past);
+      if (R.x1 == 1 && ondel > slot) next = true;
+      if (R.x1 != 1 || trimmed) continue;
+      int obs, q;
+      if (!slot_observation(R, P.bases, P.quals, qpos, ondel, slot, read_base, obs, q, err)) continue;
+      if (q < P.base_quality_cutoff) continue;
+      if ((int)R.rfile >= S) {
+        err |= kErrReadSet;
+        continue;
+      }
+      if (q >= Q) {
+        err |= kErrQuality;
+        continue;
+      }
+      const uint32_t key = ((((uint32_t)R.mslot * S + R.rfile) * Q + q) * 10u) + obs * 2u + (top ? 0u : 1u);
+      const double* lp = P.tab_lp + (size_t)key * 5;
+      double l[5];
+      for (int b = 0; b < 5; b++) l[b] = lp[b];
+      for (int b = 0; b < 5; b++) lps[b] += l[b];
+      A.add(obs, top, l, (float)P.tab_r[(size_t)key * 6 + 5]);
+    }
+    int best;
+    double score;
+    pure_genotype(lps, A.n, best, score);
+    const double cbs = score - P.log10_ref_length;
+    const bool cons = (best != 4) && !isnan(cbs) && (cbs > 0.0);   // reference base is '.'
+    bool cand = cons || (P.screen_off && A.n > 0) ||
+                screen_passes(A, 4, P.log10_ref_length, P.polymorphism_cutoff, P.screen_margin);
+    if (cand) {
+      const int c = atomicAdd(P.n_cand, 1);
+      if (c < P.cand_cap) {
+        P.cand[c] = make_int2(p, slot);
+        P.cand_cons[c] = cons ? 1 : 0;
+      } else {
+        err |= kErrCapacity;
+      }
+    }
+  }
+  if (err) atomicOr(P.err, err);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Exact FP64 scorer: one wavefront per candidate (position, slot)                             */
+
+struct WaveScratch {
+  double term[6][WAVE];   // per-read terms of one chunk: log10 s_i + lpmax_i, w_i(b)
+  double sums[6];
+  double f[5];
+  int flag;
+};
+
+// One EM-type pass over the candidate's reads with frequencies f (uniform across the wave).
+// Per-read terms are computed in parallel; the six sums are accumulated in read order by six
+// lanes, which is the reference's order, so the results are bit-identical to a sequential loop.
+__device__ void em_pass(const uint32_t* keys, int n, const double* __restrict__ tab_r, const double f[5],
+                        const bool allowed[5], WaveScratch& W, double sums[6]) {
+  const int lane = threadIdx.x % WAVE;
+  double acc = 0.0;
+  for (int c0 = 0; c0 < n; c0 += WAVE) {
+    const int i = c0 + lane;
+    if (i < n) {
+      const double* tr = tab_r + (size_t)keys[i] * 6;
+      double rr[5];
+      for (int b = 0; b < 5; b++) rr[b] = tr[b];
+      const double lpmax = tr[5];
+      double s = 0.0;
+      for (int b = 0; b < 5; b++)
+        if (allowed[b]) s += f[b] * rr[b];
+      if (s > 0.0) {
+        W.term[0][lane] = log10(s) + lpmax;
+        for (int b = 0; b < 5; b++) W.term[1 + b][lane] = allowed[b] ? f[b] * rr[b] / s : 0.0;
+      } else {
+        W.term[0][lane] = 0.0;
+        for (int b = 0; b < 5; b++) W.term[1 + b][lane] = allowed[b] ? f[b] : 0.0;
+      }
+    }
+    __syncthreads();
+    if (lane < 6) {
+      const int m = min(WAVE, n - c0);
+      for (int j = 0; j < m; j++) acc += W.term[lane][j];
+    }
+    __syncthreads();
+  }
+  if (lane < 6) W.sums[lane] = acc;
+  __syncthreads();
+  for (int k = 0; k < 6; k++) sums[k] = W.sums[k];
+  __syncthreads();
+}
+
+struct AlleleFit {
+  double f[5];
+  double ll;
+  uint32_t n;
+};
+
+// fit_allele_frequencies (identify_mutations.cpp:3240-3318)
+__device__ AlleleFit fit_alleles_counts(const uint32_t* keys, const uint32_t obs_of[5], int n,
+                                        const double* __restrict__ tab_r, const bool allowed[5], double tol,
+                                        WaveScratch& W) {
+  AlleleFit m;
+  for (int b = 0; b < 5; b++) m.f[b] = 0.0;
+  m.ll = 0.0;
+  m.n = (uint32_t)n;
+  if (n == 0) return m;
+  int n_allowed = 0;
+  for (int b = 0; b < 5; b++)
+    if (allowed[b]) {
+      n_allowed++;
+      m.f[b] = 0.5;
+    }
+  if (n_allowed == 0) return m;
+  // m.f[obs] += 1.0 once per read in the reference: exact for integer counts below 2^53
+  for (int b = 0; b < 5; b++)
+    if (allowed[b]) m.f[b] += (double)obs_of[b];
+  double init_total = 0.0;
+  for (int b = 0; b < 5; b++) init_total += m.f[b];
+  for (int b = 0; b < 5; b++) m.f[b] /= init_total;
+  double sums[6];
+  for (int it = 1; it <= 50; it++) {
+    em_pass(keys, n, tab_r, m.f, allowed, W, sums);
+    double max_delta = 0.0;
+    for (int b = 0; b < 5; b++) {
+      if (!allowed[b]) continue;
+      const double fn = sums[1 + b] / (double)n;
+      max_delta = fmax(max_delta, fabs(fn - m.f[b]));
+      m.f[b] = fn;
+    }
+    m.ll = sums[0];
+    if (max_delta < tol) break;
+  }
+  return m;
+}
+
+// profile_log10_likelihood (identify_mutations.cpp:3094-3143)
+__device__ double profile_ll(const uint32_t* keys, int n, const double* __restrict__ tab_r, const AlleleFit& full,
+                             int v, double f_fixed, double tol, WaveScratch& W) {
+  double f[5];
+  double other = 0.0;
+  for (int b = 0; b < 5; b++)
+    if (b != v) other += full.f[b];
+  for (int b = 0; b < 5; b++) {
+    if (b == v) f[b] = f_fixed;
+    else f[b] = (other > 0.0) ? (1.0 - f_fixed) * full.f[b] / other : (1.0 - f_fixed) / 4.0;
+  }
+  const bool all[5] = {true, true, true, true, true};
+  double ll = 0.0;
+  double sums[6];
+  for (int it = 0; it < 50; it++) {
+    em_pass(keys, n, tab_r, f, all, W, sums);
+    ll = sums[0];
+    double others = 0.0;
+    for (int b = 0; b < 5; b++)
+      if (b != v) others += sums[1 + b];
+    double max_delta = 0.0;
+    for (int b = 0; b < 5; b++) {
+      if (b == v) continue;
+      const double fn = (others > 0.0) ? (1.0 - f_fixed) * sums[1 + b] / others : (1.0 - f_fixed) / 4.0;
+      max_delta = fmax(max_delta, fabs(fn - f[b]));
+      f[b] = fn;
+    }
+    if (max_delta < tol) break;
+  }
+  return ll;
+}
+
+__device__ __forceinline__ double log_choose(double n, double k) { return lgamma(n + 1) - lgamma(k + 1) - lgamma(n - k + 1); }
+
+// fisher_exact_test_2x2 (stats.cpp:2144-2169)
+__device__ double fisher_2x2(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+  const uint32_t row1 = a + b, row2 = c + d, col1 = a + c, col2 = b + d;
+  const uint32_t n = row1 + row2;
+  const uint32_t a_min = (row1 > col2) ? (row1 - col2) : 0;
+  const uint32_t a_max = min(row1, col1);
+  const double log_denom = log_choose(n, row1);
+  const double lpo = log_choose(col1, a) + log_choose(col2, row1 - a) - log_denom;
+  const double lre = log(1.0 + 1e-7);
+  double total = 0.0;
+  for (uint32_t ai = a_min; ai <= a_max; ai++) {
+    const double lp = log_choose(col1, ai) + log_choose(col2, row1 - ai) - log_denom;
+    if (lp <= lpo + lre) total += exp(lp);
+  }
+  return fmin(total, 1.0);
+}
+
+// ks_test_two_sample_less (stats.cpp:2190-2284) from per-quality histograms of x (minor) and y
+// (major).  Pooled ties are whole quality values, so the sorted pooled sample is determined by the
+// two histograms.
+__device__ double ks_less_hist(const uint16_t* hx, const uint16_t* hy, uint32_t nx, uint32_t ny, double* row) {
+  uint32_t cx = 0, cy = 0, bx = 0, by = 0;
+  double min_z = 0.0;
+  bool have = false;
+  for (int v = 0; v < 128; v++) {
+    if (hx[v] == 0 && hy[v] == 0) continue;
+    cx += hx[v];
+    cy += hy[v];
+    const double z = (double)cx / nx - (double)cy / ny;
+    if (!have || z < min_z) {
+      min_z = z;
+      have = true;
+      bx = cx;
+      by = cy;
+    }
+  }
+  const double stat = -min_z;
+  if ((double)nx * ny < 10000) {
+    const uint32_t m = ny, n = nx;
+    const int64_t thr = (int64_t)by * n - (int64_t)bx * m;
+    // boundary[k]: k is 0 or a cumulative pooled count at the end of a tie group
+    auto boundary = [&](uint32_t k) -> bool {
+      if (k == 0) return true;
+      uint32_t cum = 0;
+      for (int v = 0; v < 128; v++) {
+        const uint32_t g = (uint32_t)hx[v] + hy[v];
+        if (!g) continue;
+        if (k <= cum) return false;
+        cum += g;
+        if (k == cum) return true;
+        if (k < cum) return false;
+      }
+      return true;
+    };
+    // u[i][j], i over m (y), j over n (x); keep one row over the smaller dimension
+    if (m <= n) {
+      // iterate j outer, i inner: row[i] = u[i][j]
+      for (uint32_t j = 0; j <= n; j++) {
+        for (uint32_t i = 0; i <= m; i++) {
+          double val;
+          if (i == 0 && j == 0) val = 1.0;
+          else val = (i > 0 ? row[i - 1] : 0.0) + (j > 0 ? row[i] : 0.0);
+          const int64_t lvl = (int64_t)i * n - (int64_t)j * m;
+          if (lvl >= thr && boundary(i + j)) val = 0.0;
+          row[i] = val;
+        }
+      }
+      const double total = exp(log_choose(m + n, m));
+      const double pv = 1.0 - row[m] / total;
+      return fmin(fmax(pv, 0.0), 1.0);
+    } else {
+      for (uint32_t i = 0; i <= m; i++) {
+        for (uint32_t j = 0; j <= n; j++) {
+          double val;
+          if (i == 0 && j == 0) val = 1.0;
+          else val = (i > 0 ? row[j] : 0.0) + (j > 0 ? row[j - 1] : 0.0);
+          const int64_t lvl = (int64_t)i * n - (int64_t)j * m;
+          if (lvl >= thr && boundary(i + j)) val = 0.0;
+          row[j] = val;
+        }
+      }
+      const double total = exp(log_choose(m + n, m));
+      const double pv = 1.0 - row[n] / total;
+      return fmin(fmax(pv, 0.0), 1.0);
+    }
+  }
+  const double neff = ((double)nx * ny) / (nx + ny);
+  return exp(-2.0 * stat * stat * neff);
+}
+
+__global__ void __launch_bounds__(64) k_score_candidates(ImParams P, int cand_begin, int cand_end) {
+  extern __shared__ uint32_t keys[];   // P.key_cap entries
+  __shared__ WaveScratch W;
+  __shared__ uint32_t cov[6][3];
+  __shared__ uint16_t qh[2][128];
+  __shared__ int n_keys;
+  __shared__ double ks_row[128];
+  const int lane = threadIdx.x;
+  const int S = P.read_set_max, Q = P.quality_max;
+  for (int ci = cand_begin + blockIdx.x; ci < cand_end; ci += gridDim.x) {
+    const int2 cd = P.cand[ci];
+    const int p = cd.x, slot = cd.y;
+    const int64_t w = p / WAVE;
+    const int rb = P.win_rb[w], re = P.win_re[w];
+    uint32_t err = 0;
+    if (lane < 18) cov[lane / 3][lane % 3] = 0;
+    if (lane == 0) n_keys = 0;
+    __syncthreads();
+    // gather pdata keys in file order (wave-parallel, order-preserving compaction)
+    for (int r0 = rb; r0 < re; r0 += WAVE) {
+      const int r = r0 + lane;
+      bool use = false;
+      uint32_t key = 0;
+      int obs = 0, strand_idx = 0;
+      if (r < re) {
+        const DRead R = P.reads[r];
+        int qpos, indel;
+        bool is_del;
+        if (entry_at(R, P.cigar, p, qpos, indel, is_del, err)) {
+          const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
+          uint8_t read_base = '.';
+          bool past = true;
+          if (ondel >= slot) {
+            read_base = P.bases[R.seq_off + qpos + slot];
+            past = false;
+          }
+          if (read_base != 15 && R.x1 == 1 && !is_trimmed(R, qpos, past)) {
+            int q;
+            if (slot_observation(R, P.bases, P.quals, qpos, ondel, slot, read_base, obs, q, err) &&
+                q >= P.base_quality_cutoff && (int)R.rfile < S && q < Q) {
+              const bool top = !(R.flags & kReadReversed);
+              key = ((((uint32_t)R.mslot * S + R.rfile) * Q + q) * 10u) + obs * 2u + (top ? 0u : 1u);
+              strand_idx = top ? 2 : 0;
+              use = true;
+            }
+          }
+        }
+      }
+      const unsigned long long mask = __ballot(use);
+      const int before = __popcll(mask & ((1ull << lane) - 1ull));
+      const int base = n_keys;
+      if (use) {
+        const int idx = base + before;
+        if (idx < P.key_cap) keys[idx] = key;
+        else err |= kErrCapacity;
+        atomicAdd(&cov[obs][strand_idx], 1u);
+      }
+      __syncthreads();
+      if (lane == 0) n_keys = base + __popcll(mask);
+      __syncthreads();
+    }
+    const int n = min(n_keys, P.key_cap);
+    uint32_t obs_count[5];
+    for (int b = 0; b < 5; b++) obs_count[b] = cov[b][0] + cov[b][2];
+    const double tol = P.precision_decimal;
+    const char refc = (slot == 0) ? P.ref[p] : '.';
+    const int ref_idx = char_index(refc) < 0 ? 5 : char_index(refc);
+    const bool cons = P.cand_cons[ci] != 0;
+    // full fit
+    const bool all[5] = {true, true, true, true, true};
+    const AlleleFit full = fit_alleles_counts(keys, obs_count, n, P.tab_r, all, tol, W);
+    // allele_model::major_index / next_index (identify_mutations.cpp:3061-3086)
+    auto major_index = [&](const AlleleFit& m) -> int {
+      if (m.n == 0) return 5;
+      int best = 0;
+      for (int b = 1; b < 5; b++)
+        if (m.f[b] > m.f[best]) best = b;
+      return (m.f[best] > 0.0) ? best : 5;
+    };
+    auto next_index = [&](const AlleleFit& m, int excl) -> int {
+      if (m.n == 0) return 5;
+      const double thr = 0.5 / (double)m.n;
+      int best = 5;
+      for (int b = 0; b < 5; b++) {
+        if (b == excl) continue;
+        if (m.f[b] < thr) continue;
+        if (best == 5 || m.f[b] > m.f[best]) best = b;
+      }
+      return best;
+    };
+    const int major = major_index(full);
+    const int minor = next_index(full, major);
+    const int variant = next_index(full, ref_idx);
+    double vscore = __longlong_as_double(0x7ff8000000000000LL);
+    bool poly = false;
+    if (variant != 5 && full.n > 0) {
+      bool without[5];
+      for (int b = 0; b < 5; b++) without[b] = (b != variant);
+      AlleleFit nf = fit_alleles_counts(keys, obs_count, n, P.tab_r, without, tol, W);
+      vscore = (full.ll - nf.ll) - P.log10_ref_length;
+      poly = vscore >= P.polymorphism_cutoff;
+    }
+    const bool emitted = cons || poly;
+    RaOut out;
+    out.gpos = p;
+    out.slot = slot;
+    out.emitted = emitted ? 1 : 0;
+    out.ref_idx = (uint8_t)ref_idx;
+    out.major = (uint8_t)major;
+    out.minor = (uint8_t)minor;
+    out.variant = (uint8_t)variant;
+    out.n = (uint32_t)n;
+    for (int b = 0; b < 5; b++) out.f[b] = full.f[b];
+    out.variant_score = vscore;
+    out.lower = 0.0;
+    out.upper = 1.0;
+    out.ks_p = 1.0;
+    out.fisher_p = 1.0;
+    for (int b = 0; b < 5; b++) {
+      out.cov[b][0] = cov[b][2];
+      out.cov[b][1] = cov[b][0];
+    }
+    int tt = 0, tb = 0;
+    for (int b = 0; b < 5; b++) {
+      tt += (int)cov[b][2];
+      tb += (int)cov[b][0];
+    }
+    out.total_top = tt;
+    out.total_bot = tb;
+    if (emitted) {
+      // write_RA_frequency_bounds (identify_mutations.cpp:3175-3217)
+      if (full.n > 0 && variant < 5) {
+        const double kDrop = 0.587566;
+        const double fh = full.f[variant];
+        const double plmax = profile_ll(keys, n, P.tab_r, full, variant, fh, tol, W);
+        const double target = plmax - kDrop;
+        if (profile_ll(keys, n, P.tab_r, full, variant, 0.0, tol, W) >= target) {
+          out.lower = 0.0;
+        } else {
+          double lo = 0.0, hi = fh;
+          for (int i = 0; (i < 40) && ((hi - lo) > tol); i++) {
+            const double mid = 0.5 * (lo + hi);
+            if (profile_ll(keys, n, P.tab_r, full, variant, mid, tol, W) >= target) hi = mid;
+            else lo = mid;
+          }
+          out.lower = hi;
+        }
+        if (profile_ll(keys, n, P.tab_r, full, variant, 1.0, tol, W) >= target) {
+          out.upper = 1.0;
+        } else {
+          double lo = fh, hi = 1.0;
+          for (int i = 0; (i < 40) && ((hi - lo) > tol); i++) {
+            const double mid = 0.5 * (lo + hi);
+            if (profile_ll(keys, n, P.tab_r, full, variant, mid, tol, W) >= target) lo = mid;
+            else hi = mid;
+          }
+          out.upper = lo;
+        }
+      }
+      // annotate_polymorphism_statistics (identify_mutations.cpp:3009-3033)
+      for (int i = lane; i < 256; i += WAVE) (&qh[0][0])[i] = 0;
+      __syncthreads();
+      const int mj = major, mn = minor;
+      if (lane == 0) {
+        uint32_t nx = 0, ny = 0;
+        for (int i = 0; i < n; i++) {
+          const uint32_t k = keys[i];
+          const int ob = (int)((k % 10u) / 2u);
+          const int q = (int)((k / 10u) % (uint32_t)Q);
+          if (ob == mj && q < 128) {
+            qh[1][q]++;
+            ny++;
+          }
+          if (ob == mn && q < 128) {
+            qh[0][q]++;
+            nx++;
+          }
+        }
+        double ks = 1.0;
+        if (nx > 0 && ny > 0) ks = ks_less_hist(qh[0], qh[1], nx, ny, ks_row);
+        const uint32_t mjt = (mj < 5) ? cov[mj][2] : 0, mjb = (mj < 5) ? cov[mj][0] : 0;
+        const uint32_t mnt = (mn < 5) ? cov[mn][2] : 0, mnb = (mn < 5) ? cov[mn][0] : 0;
+        out.ks_p = ks;
+        out.fisher_p = fisher_2x2(mnt, mnb, mjt, mjb);
+      }
+    }
+    if (lane == 0) P.ra_out[ci] = out;
+    if (err) atomicOr(P.err, err);
+    __syncthreads();
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* MC / UN runs                                                                                */
+
+// Marks run starts and ends of the two interval types; appends (position, kind) events.
+// kind: 0 MC start, 1 MC end, 2 UN start, 3 UN end.
+__global__ void __launch_bounds__(256) k_runs(RunParams P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n_pos) return;
+  const int t = P.pos_target[p];
+  if (t < 0) return;
+  const TargetInfo T = P.targets[t];
+  if (!T.scored || T.prop_cutoff < 0.0) return;
+  const int first = T.off, last = T.off + T.len - 1;
+  auto in_mc = [&](int q) -> bool { return (double)P.col_unique[q] <= T.prop_cutoff; };
+  auto in_un = [&](int q) -> bool { return !(P.col_flags[q] & kPosPredicted); };
+  if (P.do_mc) {
+    const bool a = in_mc(p);
+    if (a && (p == first || !in_mc(p - 1))) {
+      const int k = atomicAdd(P.n_events, 1);
+      if (k < P.event_cap) P.events[k] = make_int2(p, 0);
+    }
+    if (a && (p == last || !in_mc(p + 1))) {
+      const int k = atomicAdd(P.n_events, 1);
+      if (k < P.event_cap) P.events[k] = make_int2(p, 1);
+    }
+  }
+  const bool u = in_un(p);
+  if (u && (p == first || !in_un(p - 1))) {
+    const int k = atomicAdd(P.n_events, 1);
+    if (k < P.event_cap) P.events[k] = make_int2(p, 2);
+  }
+  if (u && (p == last || !in_un(p + 1))) {
+    const int k = atomicAdd(P.n_events, 1);
+    if (k < P.event_cap) P.events[k] = make_int2(p, 3);
+  }
+}
+
+// Per MC run [s, e]: seed reached in the run, redundant sub-range (identify_mutations.cpp:2280,
+// 2326-2343).  One thread per run, sequential over the run's columns.
+__global__ void __launch_bounds__(64) k_mc_detail(RunParams P, int n_runs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_runs) return;
+  const int s = P.runs[i].x, e = P.runs[i].y;
+  bool seed = false, reached_zero = false;
+  int red_start = -1, red_end = -1;
+  for (int p = s; p <= e; p++) {
+    const uint8_t fl = P.col_flags[p];
+    if (fl & kPosSeed) seed = true;
+    if (!(fl & kPosRedundant)) {
+      reached_zero = true;
+      red_end = -1;
+    } else {
+      if (!reached_zero) red_start = p;
+      else if (red_end < 0) red_end = p;
+    }
+  }
+  // the closing column (e + 1) is checked against the seed before the run closes
+  if (e + 1 < P.n_pos && P.pos_target[e + 1] == P.pos_target[s] && (P.col_flags[e + 1] & kPosSeed)) seed = true;
+  P.run_out[i] = make_int4(seed ? 1 : 0, red_start, red_end, 0);
+}
+
+}  // namespace bsq

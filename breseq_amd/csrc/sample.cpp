@@ -1,0 +1,801 @@
+// sample.cpp -- bsq_sample: container, deterministic synthetic sample generator, and BAM/FASTA
+// I/O on zlib (the parts of htslib breseq's pileup path relies on: sam_read1 record decoding,
+// BGZF blocks, fai_fetch whole-sequence loads).
+#include "sample.h"
+
+#include <zlib.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+#include "host_util.h"
+
+namespace bsq {
+
+void Sample::finalize() {
+  name_ptrs.clear();
+  seq_ptrs.clear();
+  for (size_t i = 0; i < names.size(); i++) {
+    name_ptrs.push_back(names[i].c_str());
+    seq_ptrs.push_back(seqs[i].c_str());
+  }
+  lengths.resize(seqs.size());
+  for (size_t i = 0; i < seqs.size(); i++) lengths[i] = (int64_t)seqs[i].size();
+  ref_view.n_targets = (int32_t)names.size();
+  ref_view.names = name_ptrs.data();
+  ref_view.lengths = lengths.data();
+  ref_view.seqs = seq_ptrs.data();
+  aln_view.n_reads = (int64_t)pos.size();
+  aln_view.tid = tid.data();
+  aln_view.pos = pos.data();
+  aln_view.flag = flag.data();
+  aln_view.mapq = mapq.data();
+  aln_view.cigar_off = cigar_off.data();
+  aln_view.cigar = cigar.data();
+  aln_view.seq_off = seq_off.data();
+  aln_view.seq = seq.data();
+  aln_view.qual = qual.data();
+  aln_view.x1 = x1.data();
+  aln_view.xl = xl.data();
+  aln_view.xr = xr.data();
+  aln_view.rg = rg.data();
+  rf_name_ptrs.clear();
+  for (auto& n : rf_names) rf_name_ptrs.push_back(n.c_str());
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Deterministic RNG: xoshiro256** seeded by splitmix64; one stream per read index so the      */
+/* output does not depend on thread count.                                                    */
+
+struct Rng {
+  uint64_t s[4];
+  static uint64_t splitmix(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }
+  explicit Rng(uint64_t seed) {
+    uint64_t x = seed;
+    for (int i = 0; i < 4; i++) s[i] = splitmix(x);
+  }
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  uint64_t next() {
+    const uint64_t r = rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+  uint64_t below(uint64_t n) { return n ? next() % n : 0; }
+  double normal() {   // Box-Muller
+    double u1 = uniform(), u2 = uniform();
+    if (u1 < 1e-300) u1 = 1e-300;
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+};
+
+static const uint8_t kBam[4] = {1, 2, 4, 8};
+static const char kChr[4] = {'A', 'C', 'G', 'T'};
+
+static inline int chr_index(char c) {
+  switch (c) {
+    case 'A': return 0;
+    case 'C': return 1;
+    case 'G': return 2;
+    case 'T': return 3;
+  }
+  return -1;
+}
+
+struct Variant {
+  int64_t pos;       // 0-based reference position (SNP: base; DEL: first deleted base; INS: base after which)
+  int type;          // 0 SNP, 1 INS, 2 DEL
+  std::string alt;   // SNP base / inserted bases
+  int32_t del_len;
+  double freq;
+};
+
+// read one fragment's aligned read of query length `len` starting at reference `start`,
+// applying the carried variants; returns false if the read would run off the reference.
+struct Built {
+  std::vector<uint32_t> cigar;
+  std::string bases;
+  int64_t pos;
+};
+
+static void push_op(std::vector<uint32_t>& cig, uint32_t op, uint32_t n) {
+  if (n == 0) return;
+  if (!cig.empty() && (cig.back() & 0xf) == op) cig.back() += n << 4;
+  else cig.push_back((n << 4) | op);
+}
+
+static bool build_read(const std::string& ref, const std::vector<Variant>& vars, const std::vector<char>& carry,
+                       size_t vfirst, int64_t start, int32_t len, int32_t edge_guard, Built& out) {
+  out.cigar.clear();
+  out.bases.clear();
+  out.pos = start;
+  const int64_t L = (int64_t)ref.size();
+  int64_t p = start;
+  size_t vi = vfirst;
+  while ((int32_t)out.bases.size() < len) {
+    if (p >= L) return false;
+    while (vi < vars.size() && vars[vi].pos < p) vi++;
+    const Variant* v = (vi < vars.size() && vars[vi].pos == p && carry[vi]) ? &vars[vi] : nullptr;
+    int32_t q = (int32_t)out.bases.size();
+    bool near_edge = (q < edge_guard) || (q > len - 1 - edge_guard - 8);
+    if (v && v->type == 2 && !near_edge && q > 0) {            // deletion starting at p
+      push_op(out.cigar, 2, (uint32_t)v->del_len);
+      p += v->del_len;
+      vi++;
+      continue;
+    }
+    char b = ref[p];
+    if (v && v->type == 0) b = v->alt[0];
+    push_op(out.cigar, 0, 1);
+    out.bases.push_back(b);
+    if (v && v->type == 1 && !near_edge) {                      // insertion after p
+      int32_t room = len - (int32_t)out.bases.size();
+      int32_t n = std::min<int32_t>((int32_t)v->alt.size(), room);
+      if (n == (int32_t)v->alt.size() && room - n > edge_guard) {
+        push_op(out.cigar, 1, (uint32_t)n);
+        out.bases += v->alt;
+      }
+    }
+    p++;
+  }
+  return true;
+}
+
+}  // namespace bsq
+
+using namespace bsq;
+
+/* ------------------------------------------------------------------------------------------ */
+
+extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_status* status) {
+  try {
+    if (!cfg || !out) throw BsqError(BSQ_ERR_ARG, "null argument");
+    if (cfg->ref_length < 1000) throw BsqError(BSQ_ERR_ARG, "ref_length must be >= 1000");
+    if (cfg->read_length < 20 || cfg->read_length > 1000) throw BsqError(BSQ_ERR_ARG, "read_length out of range");
+    if (cfg->qual_min < 0 || cfg->qual_max > 93 || cfg->qual_min > cfg->qual_max) throw BsqError(BSQ_ERR_ARG, "bad quality range");
+    bsq_sample* smp = new bsq_sample();
+    Sample& S = smp->s;
+    Rng rng(cfg->seed * 0x100000001B3ULL + 17);
+    // reference
+    const int64_t L = cfg->ref_length;
+    std::string ref(L, 'A');
+    for (int64_t i = 0; i < L; i++) {
+      double u = rng.uniform();
+      bool gc = u < cfg->gc;
+      uint64_t k = rng.next() & 1;
+      ref[i] = gc ? (k ? 'G' : 'C') : (k ? 'T' : 'A');
+    }
+    S.names.push_back("synthetic_1");
+    // planted variants: distinct positions, >= 20 bp apart, away from the ends
+    std::vector<Variant> vars;
+    {
+      int32_t nv = cfg->n_snps + cfg->n_indels;
+      std::vector<int64_t> sites;
+      int guard = 0;
+      while ((int32_t)sites.size() < nv && guard < nv * 100) {
+        guard++;
+        int64_t p = 200 + (int64_t)rng.below((uint64_t)std::max<int64_t>(1, L - 400));
+        bool ok = true;
+        for (int64_t q : sites) if (std::llabs(q - p) < 20) { ok = false; break; }
+        if (ok) sites.push_back(p);
+      }
+      for (size_t i = 0; i < sites.size(); i++) {
+        Variant v;
+        v.pos = sites[i];
+        v.del_len = 0;
+        double fmin = cfg->variant_freq_min, fmax = cfg->variant_freq_max;
+        if (fmin <= 0 || fmax <= 0) { fmin = fmax = 1.0; }
+        v.freq = (fmin == fmax) ? fmin : std::exp(std::log(fmin) + rng.uniform() * (std::log(fmax) - std::log(fmin)));
+        if ((int32_t)i < cfg->n_snps) {
+          v.type = 0;
+          int r0 = chr_index(ref[v.pos]);
+          int alt = (r0 + 1 + (int)rng.below(3)) & 3;
+          v.alt = std::string(1, kChr[alt]);
+        } else {
+          int32_t len = 1 + (int32_t)rng.below((uint64_t)std::max(1, cfg->indel_max));
+          if (rng.below(2)) {
+            v.type = 1;
+            for (int32_t k = 0; k < len; k++) v.alt.push_back(kChr[rng.below(4)]);
+          } else {
+            v.type = 2;
+            v.del_len = len;
+          }
+        }
+        vars.push_back(v);
+      }
+      std::sort(vars.begin(), vars.end(), [](const Variant& a, const Variant& b) { return a.pos < b.pos; });
+    }
+    // quality distribution: clipped normal, discretized; cumulative table over [qmin, qmax]
+    const int qmin = cfg->qual_min, qmax = cfg->qual_max;
+    std::vector<double> qcdf;
+    {
+      std::vector<double> w;
+      for (int q = qmin; q <= qmax; q++) {
+        double lo = (q == qmin) ? -1e300 : (q - 0.5 - cfg->qual_mean) / cfg->qual_sd;
+        double hi = (q == qmax) ? 1e300 : (q + 0.5 - cfg->qual_mean) / cfg->qual_sd;
+        w.push_back(0.5 * (std::erfc(-hi / std::sqrt(2.0)) - std::erfc(-lo / std::sqrt(2.0))));
+      }
+      double tot = std::accumulate(w.begin(), w.end(), 0.0);
+      double c = 0;
+      for (double x : w) { c += x / tot; qcdf.push_back(c); }
+      qcdf.back() = 1.0;
+    }
+    std::vector<uint64_t> err_thr(94, 0);
+    for (int q = 0; q <= 93; q++) err_thr[q] = (uint64_t)(std::pow(10.0, -q / 10.0) * 4294967296.0);
+
+    const int32_t RL = cfg->read_length;
+    const bool paired = cfg->paired != 0;
+    const double cov = cfg->coverage;
+    int64_t n_frag = (int64_t)std::llround(cov * (double)L / (double)(RL * (paired ? 2 : 1)));
+    if (n_frag < 1) n_frag = 1;
+
+    struct RawRead {
+      int64_t pos;
+      int64_t order;
+      std::vector<uint32_t> cigar;
+      std::string bases;
+      std::vector<uint8_t> q;
+      uint16_t flag;
+      uint8_t mapq;
+      int32_t x1;
+      int64_t mate_pos;
+      int32_t tlen;
+      int32_t file;
+    };
+    std::vector<RawRead> reads;
+    reads.reserve((size_t)n_frag * (paired ? 2 : 1));
+    std::vector<char> carry(vars.size(), 0);
+    Built b1, b2;
+    for (int64_t f = 0; f < n_frag; f++) {
+      Rng fr(cfg->seed ^ (0xA0761D6478BD642FULL * (uint64_t)(f + 1)));
+      int64_t flen = RL;
+      if (paired) {
+        double x = cfg->insert_mean + cfg->insert_sd * fr.normal();
+        flen = (int64_t)std::llround(x);
+        if (flen < RL) flen = RL;
+      }
+      if (flen + 16 >= L) continue;
+      int64_t start = (int64_t)fr.below((uint64_t)(L - flen - 8));
+      bool frag_rev = (fr.next() & 1) != 0;
+      // haplotype: each variant in the fragment carried with its frequency
+      size_t vfirst = std::lower_bound(vars.begin(), vars.end(), start, [](const Variant& v, int64_t p) { return v.pos < p; }) - vars.begin();
+      for (size_t vi = vfirst; vi < vars.size() && vars[vi].pos < start + flen + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
+      const int32_t guard = 4;
+      int n_mates = paired ? 2 : 1;
+      bool ok = build_read(ref, vars, carry, vfirst, start, RL, guard, b1);
+      if (ok && paired) {
+        // mate 2 ends at start + flen - 1 on the reference (approximately, before indels)
+        int64_t s2 = start + flen - RL;
+        size_t v2 = std::lower_bound(vars.begin(), vars.end(), s2, [](const Variant& v, int64_t p) { return v.pos < p; }) - vars.begin();
+        ok = build_read(ref, vars, carry, v2, s2, RL, guard, b2);
+      }
+      for (size_t vi = vfirst; vi < vars.size() && vars[vi].pos < start + flen + 16; vi++) carry[vi] = 0;
+      if (!ok) continue;
+      bool redundant = fr.uniform() < cfg->redundant_fraction;
+      int32_t x1v = redundant ? (2 + (int32_t)fr.below(2)) : 1;
+      for (int m = 0; m < n_mates; m++) {
+        const Built& B = (m == 0) ? b1 : b2;
+        RawRead rr;
+        rr.pos = B.pos;
+        rr.order = (int64_t)reads.size();
+        rr.cigar = B.cigar;
+        rr.bases = B.bases;
+        rr.q.resize(RL);
+        bool rev = paired ? ((m == 0) ? frag_rev : !frag_rev) : frag_rev;
+        for (int32_t i = 0; i < RL; i++) {
+          uint64_t u = fr.next();
+          double uq = (double)(u >> 40) * (1.0 / 16777216.0);
+          int q = qmin + (int)(std::lower_bound(qcdf.begin(), qcdf.end(), uq) - qcdf.begin());
+          if (q > qmax) q = qmax;
+          rr.q[i] = (uint8_t)q;
+          uint64_t ue = u & 0xFFFFFFFFULL;
+          if (ue < err_thr[q]) {
+            int c = chr_index(rr.bases[i]);
+            int alt = (c + 1 + (int)((u >> 32) % 3)) & 3;
+            rr.bases[i] = kChr[alt];
+          }
+          if ((u >> 24 & 0xFFFF) == 7) { rr.bases[i] = 'N'; rr.q[i] = (uint8_t)std::max(qmin, 2); }
+        }
+        uint16_t flag = rev ? 0x10 : 0;
+        if (paired) {
+          flag |= 0x1 | 0x2 | ((m == 0) ? 0x40 : 0x80);
+          bool mrev = (m == 0) ? !frag_rev : frag_rev;
+          if (mrev) flag |= 0x20;
+        }
+        rr.flag = flag;
+        rr.mapq = redundant ? 1 : (uint8_t)cfg->mapq;
+        rr.x1 = x1v;
+        rr.file = paired ? m : 0;
+        rr.mate_pos = paired ? ((m == 0) ? b2.pos : b1.pos) : -1;
+        rr.tlen = paired ? (int32_t)((m == 0) ? flen : -flen) : 0;
+        reads.push_back(std::move(rr));
+      }
+    }
+    std::stable_sort(reads.begin(), reads.end(), [](const RawRead& a, const RawRead& b) { return a.pos < b.pos; });
+    // trims for XL/XR (calculate_trims.cpp:239-285 + alignment.cpp:920-946)
+    std::vector<uint8_t> gtrims(2 * (size_t)L);
+    host_sequence_trims(ref.data(), L, gtrims.data());
+    S.seqs.push_back(ref);
+    size_t nr = reads.size();
+    S.tid.assign(nr, 0);
+    S.pos.resize(nr);
+    S.flag.resize(nr);
+    S.mapq.resize(nr);
+    S.x1.resize(nr);
+    S.xl.resize(nr);
+    S.xr.resize(nr);
+    S.rg.assign(nr, 0);
+    S.mate_tid.assign(nr, paired ? 0 : -1);
+    S.mate_pos.resize(nr);
+    S.tlen.resize(nr);
+    S.read_names.resize(nr);
+    S.cigar_off.assign(1, 0);
+    S.seq_off.assign(1, 0);
+    S.seq.reserve(nr * RL);
+    S.qual.reserve(nr * RL);
+    for (size_t i = 0; i < nr; i++) {
+      RawRead& r = reads[i];
+      S.pos[i] = (int32_t)r.pos;
+      S.flag[i] = r.flag;
+      S.mapq[i] = r.mapq;
+      S.x1[i] = r.x1;
+      S.mate_pos[i] = (int32_t)r.mate_pos;
+      S.tlen[i] = r.tlen;
+      S.read_names[i] = std::to_string(r.file + 1) + ":" + std::to_string(r.order / (paired ? 2 : 1) + 1);
+      for (uint32_t c : r.cigar) S.cigar.push_back(c);
+      S.cigar_off.push_back((int64_t)S.cigar.size());
+      for (int32_t k = 0; k < RL; k++) {
+        int c = chr_index(r.bases[k]);
+        S.seq.push_back(c < 0 ? 15 : kBam[c]);
+        S.qual.push_back(r.q[k]);
+      }
+      S.seq_off.push_back((int64_t)S.seq.size());
+      // alignment trims: max(genomic, read-edge) + query offset
+      int64_t rend = r.pos;
+      for (uint32_t c : r.cigar) {
+        uint32_t op = c & 0xf;
+        if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) rend += c >> 4;
+      }
+      uint32_t etl = 1, etr = 1;
+      host_edge_trims(r.bases, etl, etr);
+      uint32_t gl = gtrims[r.pos], gr = gtrims[L + (rend - 1)];
+      S.xl[i] = (int32_t)std::max(gl, etl);      // query_start_0 == 0 (no soft clips)
+      S.xr[i] = (int32_t)std::max(gr, etr);      // read_length - query_end_1 == 0
+      S.rg[i] = 0;
+    }
+    S.rg_ids.push_back("0");
+    S.rg_libs.push_back(paired ? "reads" : "reads_1");
+    S.rf_base.push_back(0);
+    S.rf_count.push_back(paired ? 2 : 1);
+    S.rf_names.push_back("reads_1");
+    if (paired) S.rf_names.push_back("reads_2");
+    S.finalize();
+    *out = smp;
+    set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& e) {
+    set_status(status, e.code, e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_status(status, BSQ_ERR_ARG, e.what());
+    return BSQ_ERR_ARG;
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* FASTA                                                                                       */
+
+static void read_fasta(const std::string& path, Sample& S) {
+  std::ifstream in(path);
+  if (!in) throw BsqError(BSQ_ERR_IO, "could not open FASTA " + path);
+  std::string line;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (line.empty()) continue;
+    if (line[0] == '>') {
+      std::string name = line.substr(1);
+      size_t sp = name.find_first_of(" \t");
+      if (sp != std::string::npos) name = name.substr(0, sp);
+      S.names.push_back(name);
+      S.seqs.emplace_back();
+    } else {
+      if (S.seqs.empty()) throw BsqError(BSQ_ERR_FORMAT, "FASTA sequence before header");
+      S.seqs.back() += line;
+    }
+  }
+}
+
+extern "C" int bsq_sample_write_fasta(const bsq_sample* s, const char* path, bsq_status* status) {
+  try {
+    std::ofstream out(path);
+    if (!out) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + path);
+    for (size_t i = 0; i < s->s.names.size(); i++) {
+      out << ">" << s->s.names[i] << "\n";
+      const std::string& q = s->s.seqs[i];
+      for (size_t p = 0; p < q.size(); p += 60) out << q.substr(p, 60) << "\n";
+    }
+    set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& e) {
+    set_status(status, e.code, e.what());
+    return e.code;
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* BAM (SAMv1 section 4): BGZF blocks of raw-deflate data with a BC extra field.              */
+
+namespace {
+
+struct BgzfWriter {
+  FILE* f = nullptr;
+  std::string buf;
+  explicit BgzfWriter(const char* path) {
+    f = fopen(path, "wb");
+    if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + path);
+  }
+  void block(const char* data, size_t n) {
+    std::vector<unsigned char> comp(n + 1024);
+    z_stream zs;
+    memset(&zs, 0, sizeof(zs));
+    if (deflateInit2(&zs, 6, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK) throw BsqError(BSQ_ERR_IO, "deflateInit2");
+    zs.next_in = (Bytef*)data;
+    zs.avail_in = (uInt)n;
+    zs.next_out = comp.data();
+    zs.avail_out = (uInt)comp.size();
+    if (deflate(&zs, Z_FINISH) != Z_STREAM_END) throw BsqError(BSQ_ERR_IO, "deflate");
+    size_t clen = zs.total_out;
+    deflateEnd(&zs);
+    uint32_t crc = (uint32_t)crc32(0L, (const Bytef*)data, (uInt)n);
+    size_t bsize = 18 + clen + 8;
+    unsigned char hdr[18] = {31, 139, 8, 4, 0, 0, 0, 0, 0, 255, 6, 0, 'B', 'C', 2, 0, 0, 0};
+    hdr[16] = (unsigned char)((bsize - 1) & 0xff);
+    hdr[17] = (unsigned char)(((bsize - 1) >> 8) & 0xff);
+    fwrite(hdr, 1, 18, f);
+    fwrite(comp.data(), 1, clen, f);
+    unsigned char tail[8];
+    for (int i = 0; i < 4; i++) tail[i] = (unsigned char)(crc >> (8 * i));
+    for (int i = 0; i < 4; i++) tail[4 + i] = (unsigned char)(n >> (8 * i));
+    fwrite(tail, 1, 8, f);
+  }
+  void write(const void* p, size_t n) {
+    buf.append((const char*)p, n);
+    while (buf.size() >= 65280) {
+      block(buf.data(), 65280);
+      buf.erase(0, 65280);
+    }
+  }
+  void close() {
+    if (!buf.empty()) block(buf.data(), buf.size());
+    buf.clear();
+    block("", 0);   // EOF marker block
+    fclose(f);
+    f = nullptr;
+  }
+};
+
+template <class T>
+void put(std::string& s, T v) {
+  s.append((const char*)&v, sizeof(T));
+}
+
+int reg2bin(int64_t beg, int64_t end) {
+  --end;
+  if (beg >> 14 == end >> 14) return (int)(((1 << 15) - 1) / 7 + (beg >> 14));
+  if (beg >> 17 == end >> 17) return (int)(((1 << 12) - 1) / 7 + (beg >> 17));
+  if (beg >> 20 == end >> 20) return (int)(((1 << 9) - 1) / 7 + (beg >> 20));
+  if (beg >> 23 == end >> 23) return (int)(((1 << 6) - 1) / 7 + (beg >> 23));
+  if (beg >> 26 == end >> 26) return (int)(((1 << 3) - 1) / 7 + (beg >> 26));
+  return 0;
+}
+
+void put_int_tag(std::string& s, const char* tag, int32_t v) {
+  s.append(tag, 2);
+  if (v >= 0 && v <= 255) { s.push_back('C'); put<uint8_t>(s, (uint8_t)v); }
+  else if (v >= 0 && v <= 65535) { s.push_back('S'); put<uint16_t>(s, (uint16_t)v); }
+  else { s.push_back('i'); put<int32_t>(s, v); }
+}
+
+}  // namespace
+
+extern "C" int bsq_sample_write_bam(const bsq_sample* smp, const char* path, bsq_status* status) {
+  try {
+    const Sample& S = smp->s;
+    BgzfWriter w(path);
+    std::string text = "@HD\tVN:1.6\tSO:coordinate\n";
+    for (size_t i = 0; i < S.names.size(); i++) text += "@SQ\tSN:" + S.names[i] + "\tLN:" + std::to_string(S.seqs[i].size()) + "\n";
+    for (size_t i = 0; i < S.rg_ids.size(); i++) text += "@RG\tID:" + S.rg_ids[i] + "\tLB:" + S.rg_libs[i] + "\n";
+    std::string h = "BAM\1";
+    put<int32_t>(h, (int32_t)text.size());
+    h += text;
+    put<int32_t>(h, (int32_t)S.names.size());
+    for (size_t i = 0; i < S.names.size(); i++) {
+      put<int32_t>(h, (int32_t)S.names[i].size() + 1);
+      h += S.names[i];
+      h.push_back('\0');
+      put<int32_t>(h, (int32_t)S.seqs[i].size());
+    }
+    w.write(h.data(), h.size());
+    std::string rec;
+    for (int64_t r = 0; r < S.n_reads(); r++) {
+      rec.clear();
+      std::string name = (r < (int64_t)S.read_names.size() && !S.read_names[r].empty()) ? S.read_names[r] : ("r" + std::to_string(r));
+      int64_t c0 = S.cigar_off[r], c1 = S.cigar_off[r + 1];
+      int64_t s0 = S.seq_off[r], s1 = S.seq_off[r + 1];
+      int32_t lseq = (int32_t)(s1 - s0);
+      int64_t end = S.pos[r];
+      for (int64_t k = c0; k < c1; k++) {
+        uint32_t op = S.cigar[k] & 0xf;
+        if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) end += S.cigar[k] >> 4;
+      }
+      if (end == S.pos[r]) end++;
+      put<int32_t>(rec, 0);   // block_size placeholder
+      put<int32_t>(rec, S.tid[r]);
+      put<int32_t>(rec, S.pos[r]);
+      put<uint8_t>(rec, (uint8_t)(name.size() + 1));
+      put<uint8_t>(rec, S.mapq[r]);
+      put<uint16_t>(rec, (uint16_t)reg2bin(S.pos[r], end));
+      put<uint16_t>(rec, (uint16_t)(c1 - c0));
+      put<uint16_t>(rec, S.flag[r]);
+      put<int32_t>(rec, lseq);
+      put<int32_t>(rec, r < (int64_t)S.mate_tid.size() ? S.mate_tid[r] : -1);
+      put<int32_t>(rec, r < (int64_t)S.mate_pos.size() ? S.mate_pos[r] : -1);
+      put<int32_t>(rec, r < (int64_t)S.tlen.size() ? S.tlen[r] : 0);
+      rec += name;
+      rec.push_back('\0');
+      for (int64_t k = c0; k < c1; k++) put<uint32_t>(rec, S.cigar[k]);
+      for (int32_t i = 0; i < lseq; i += 2) {
+        uint8_t hi = S.seq[s0 + i] & 0xf;
+        uint8_t lo = (i + 1 < lseq) ? (S.seq[s0 + i + 1] & 0xf) : 0;
+        rec.push_back((char)((hi << 4) | lo));
+      }
+      rec.append((const char*)&S.qual[s0], (size_t)lseq);
+      if (S.x1[r] >= 0) put_int_tag(rec, "X1", S.x1[r]);
+      if (S.xl[r] >= 0) put_int_tag(rec, "XL", S.xl[r]);
+      if (S.xr[r] >= 0) put_int_tag(rec, "XR", S.xr[r]);
+      if (S.rg[r] >= 0 && S.rg[r] < (int32_t)S.rg_ids.size()) {
+        rec += "RGZ";
+        rec += S.rg_ids[S.rg[r]];
+        rec.push_back('\0');
+      }
+      int32_t bs = (int32_t)rec.size() - 4;
+      memcpy(&rec[0], &bs, 4);
+      w.write(rec.data(), rec.size());
+    }
+    w.close();
+    set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& e) {
+    set_status(status, e.code, e.what());
+    return e.code;
+  }
+}
+
+namespace {
+
+struct Reader {
+  const unsigned char* p;
+  const unsigned char* e;
+  template <class T>
+  T get() {
+    if ((size_t)(e - p) < sizeof(T)) throw BsqError(BSQ_ERR_FORMAT, "truncated BAM");
+    T v;
+    memcpy(&v, p, sizeof(T));
+    p += sizeof(T);
+    return v;
+  }
+  void need(size_t n) const {
+    if ((size_t)(e - p) < n) throw BsqError(BSQ_ERR_FORMAT, "truncated BAM");
+  }
+};
+
+// integer value of an aux tag payload of the given type; advances past it
+bool aux_int(char t, const unsigned char* v, int64_t& out) {
+  switch (t) {
+    case 'c': out = (int8_t)v[0]; return true;
+    case 'C': out = v[0]; return true;
+    case 's': { int16_t x; memcpy(&x, v, 2); out = x; return true; }
+    case 'S': { uint16_t x; memcpy(&x, v, 2); out = x; return true; }
+    case 'i': { int32_t x; memcpy(&x, v, 4); out = x; return true; }
+    case 'I': { uint32_t x; memcpy(&x, v, 4); out = x; return true; }
+  }
+  return false;
+}
+
+size_t aux_size(char t, const unsigned char* v, const unsigned char* end) {
+  switch (t) {
+    case 'A': case 'c': case 'C': return 1;
+    case 's': case 'S': return 2;
+    case 'i': case 'I': case 'f': return 4;
+    case 'Z': case 'H': {
+      const unsigned char* q = v;
+      while (q < end && *q) q++;
+      if (q >= end) throw BsqError(BSQ_ERR_FORMAT, "unterminated aux string");
+      return (size_t)(q - v) + 1;
+    }
+    case 'B': {
+      if (end - v < 5) throw BsqError(BSQ_ERR_FORMAT, "truncated aux array");
+      char st = (char)v[0];
+      uint32_t n;
+      memcpy(&n, v + 1, 4);
+      size_t es = (st == 'c' || st == 'C') ? 1 : (st == 's' || st == 'S') ? 2 : 4;
+      return 5 + es * n;
+    }
+  }
+  throw BsqError(BSQ_ERR_FORMAT, std::string("unknown aux type ") + t);
+}
+
+}  // namespace
+
+extern "C" int bsq_sample_read(const char* bam_path, const char* fasta_path, bsq_sample** out, bsq_status* status) {
+  bsq_sample* smp = nullptr;
+  try {
+    smp = new bsq_sample();
+    Sample& S = smp->s;
+    read_fasta(fasta_path, S);
+    gzFile gz = gzopen(bam_path, "rb");
+    if (!gz) throw BsqError(BSQ_ERR_IO, std::string("could not open BAM ") + bam_path);
+    std::vector<unsigned char> data;
+    {
+      std::vector<unsigned char> chunk(1 << 22);
+      for (;;) {
+        int n = gzread(gz, chunk.data(), (unsigned)chunk.size());
+        if (n < 0) { gzclose(gz); throw BsqError(BSQ_ERR_FORMAT, "BGZF decompression failed"); }
+        if (n == 0) break;
+        data.insert(data.end(), chunk.begin(), chunk.begin() + n);
+      }
+      gzclose(gz);
+    }
+    Reader R{data.data(), data.data() + data.size()};
+    R.need(4);
+    if (memcmp(R.p, "BAM\1", 4) != 0) throw BsqError(BSQ_ERR_FORMAT, "not a BAM file");
+    R.p += 4;
+    int32_t l_text = R.get<int32_t>();
+    R.need((size_t)l_text);
+    std::string text((const char*)R.p, (size_t)l_text);
+    R.p += l_text;
+    // @RG lines
+    {
+      std::istringstream ts(text);
+      std::string line;
+      while (std::getline(ts, line)) {
+        if (line.compare(0, 3, "@RG") != 0) continue;
+        std::string id, lb;
+        std::istringstream ls(line);
+        std::string f;
+        while (std::getline(ls, f, '\t')) {
+          if (f.compare(0, 3, "ID:") == 0) id = f.substr(3);
+          if (f.compare(0, 3, "LB:") == 0) lb = f.substr(3);
+        }
+        S.rg_ids.push_back(id);
+        S.rg_libs.push_back(lb);
+      }
+    }
+    int32_t n_ref = R.get<int32_t>();
+    std::vector<std::string> bam_names;
+    std::vector<int64_t> bam_lens;
+    for (int32_t i = 0; i < n_ref; i++) {
+      int32_t ln = R.get<int32_t>();
+      R.need((size_t)ln);
+      bam_names.emplace_back((const char*)R.p, strnlen((const char*)R.p, (size_t)ln));
+      R.p += ln;
+      bam_lens.push_back(R.get<int32_t>());
+    }
+    // order FASTA sequences as the BAM header targets (pileup_base.cpp:80-86)
+    {
+      std::map<std::string, size_t> idx;
+      for (size_t i = 0; i < S.names.size(); i++) idx[S.names[i]] = i;
+      std::vector<std::string> seqs;
+      for (int32_t i = 0; i < n_ref; i++) {
+        auto it = idx.find(bam_names[i]);
+        if (it == idx.end()) throw BsqError(BSQ_ERR_FORMAT, "FASTA lacks target " + bam_names[i]);
+        if ((int64_t)S.seqs[it->second].size() != bam_lens[i]) throw BsqError(BSQ_ERR_FORMAT, "FASTA length differs from BAM header for " + bam_names[i]);
+        seqs.push_back(S.seqs[it->second]);
+      }
+      S.names = bam_names;
+      S.seqs = seqs;
+    }
+    S.cigar_off.assign(1, 0);
+    S.seq_off.assign(1, 0);
+    while (R.p < R.e) {
+      int32_t bs = R.get<int32_t>();
+      R.need((size_t)bs);
+      const unsigned char* rec_end = R.p + bs;
+      Reader Q{R.p, rec_end};
+      int32_t tid = Q.get<int32_t>();
+      int32_t pos = Q.get<int32_t>();
+      uint8_t lrn = Q.get<uint8_t>();
+      uint8_t mq = Q.get<uint8_t>();
+      (void)Q.get<uint16_t>();
+      uint16_t nc = Q.get<uint16_t>();
+      uint16_t flag = Q.get<uint16_t>();
+      int32_t lseq = Q.get<int32_t>();
+      int32_t mtid = Q.get<int32_t>();
+      int32_t mpos = Q.get<int32_t>();
+      int32_t tlen = Q.get<int32_t>();
+      Q.need(lrn);
+      S.read_names.emplace_back((const char*)Q.p, strnlen((const char*)Q.p, lrn));
+      Q.p += lrn;
+      for (uint16_t k = 0; k < nc; k++) S.cigar.push_back(Q.get<uint32_t>());
+      Q.need((size_t)(lseq + 1) / 2 + (size_t)lseq);
+      for (int32_t i = 0; i < lseq; i++) {
+        uint8_t byte = Q.p[i >> 1];
+        S.seq.push_back((i & 1) ? (byte & 0xf) : (byte >> 4));
+      }
+      Q.p += (lseq + 1) / 2;
+      S.qual.insert(S.qual.end(), Q.p, Q.p + lseq);
+      Q.p += lseq;
+      int32_t x1 = -1, xl = -1, xr = -1, rgi = -1;
+      while (Q.p < rec_end) {
+        Q.need(3);
+        char t0 = (char)Q.p[0], t1 = (char)Q.p[1], ty = (char)Q.p[2];
+        const unsigned char* v = Q.p + 3;
+        size_t sz = aux_size(ty, v, rec_end);
+        if ((size_t)(rec_end - v) < sz) throw BsqError(BSQ_ERR_FORMAT, "truncated aux field");
+        int64_t iv;
+        if (t0 == 'X' && t1 == '1' && aux_int(ty, v, iv)) x1 = (int32_t)iv;
+        else if (t0 == 'X' && t1 == 'L' && aux_int(ty, v, iv)) xl = (int32_t)iv;
+        else if (t0 == 'X' && t1 == 'R' && aux_int(ty, v, iv)) xr = (int32_t)iv;
+        else if (t0 == 'R' && t1 == 'G' && ty == 'Z') {
+          std::string id((const char*)v);
+          for (size_t g = 0; g < S.rg_ids.size(); g++)
+            if (S.rg_ids[g] == id) { rgi = (int32_t)g; break; }
+        }
+        Q.p = v + sz;
+      }
+      S.tid.push_back(tid);
+      S.pos.push_back(pos);
+      S.flag.push_back(flag);
+      S.mapq.push_back(mq);
+      S.mate_tid.push_back(mtid);
+      S.mate_pos.push_back(mpos);
+      S.tlen.push_back(tlen);
+      S.x1.push_back(x1);
+      S.xl.push_back(xl);
+      S.xr.push_back(xr);
+      S.rg.push_back(rgi);
+      S.cigar_off.push_back((int64_t)S.cigar.size());
+      S.seq_off.push_back((int64_t)S.seq.size());
+      R.p = rec_end;
+    }
+    S.finalize();
+    *out = smp;
+    set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& e) {
+    delete smp;
+    set_status(status, e.code, e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    delete smp;
+    set_status(status, BSQ_ERR_FORMAT, e.what());
+    return BSQ_ERR_FORMAT;
+  }
+}
+
+extern "C" const bsq_reference* bsq_sample_reference(const bsq_sample* s) { return &s->s.ref_view; }
+extern "C" const bsq_alignments* bsq_sample_alignments(const bsq_sample* s) { return &s->s.aln_view; }
+extern "C" int32_t bsq_sample_n_read_groups(const bsq_sample* s) { return (int32_t)s->s.rg_ids.size(); }
+extern "C" const char* bsq_sample_read_group(const bsq_sample* s, int32_t i) {
+  if (i < 0 || i >= (int32_t)s->s.rg_ids.size()) return nullptr;
+  return s->s.rg_ids[i].c_str();
+}
+extern "C" void bsq_sample_free(bsq_sample* s) { delete s; }

@@ -1,0 +1,939 @@
+// session.hip -- device-resident sessions: packs a sample's alignments once into the device CSR,
+// runs the pileup kernels, and assembles breseq's output files from their results.
+//
+// Entry points (include/bsq.h) replace:
+//   bsq_session_error_count -> error_count()            error_count.cpp:50-68
+//   bsq_session_identify    -> identify_mutations()     identify_mutations.cpp:48-88
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "evidence.h"
+#include "host_util.h"
+#include "kernels.h"
+#include "sample.h"
+
+namespace bsq {
+
+#define HIP_OK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) count = 1;
+    HIP_OK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+};
+
+static const int kWave = 64;
+
+struct Session {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bsq_timing timing{};
+  bool loaded = false;
+  // packed metadata (host)
+  int n_targets = 0;
+  std::vector<std::string> names;
+  std::vector<int64_t> lengths;
+  std::vector<int32_t> tgt_off;
+  int32_t n_pos = 0;        // global columns incl. pads
+  int64_t n_windows = 0;
+  int32_t max_window_reads = 0;
+  int64_t n_dreads = 0;
+  int64_t aligned_bases = 0;
+  std::vector<int32_t> mapq_values;   // mslot -> MAPQ
+  int32_t n_read_files = 1;
+  std::vector<int32_t> h_pos_target;
+  std::string h_ref;                  // padded reference (for host-side formatting)
+  // device buffers
+  DBuf<DRead> d_reads;
+  DBuf<uint32_t> d_cigar;
+  DBuf<uint8_t> d_bases, d_quals;
+  DBuf<char> d_ref;
+  DBuf<int32_t> d_pos_target, d_win_rb, d_win_re, d_target_group;
+  DBuf<TargetInfo> d_targets;
+  DBuf<unsigned long long> d_counts, d_hist;
+  DBuf<uint32_t> d_err;
+  DBuf<double> d_tab_lp, d_tab_r;
+  DBuf<uint8_t> d_col_flags;
+  DBuf<int32_t> d_col_unique, d_col_total;
+  DBuf<int2> d_cand;
+  DBuf<uint8_t> d_cand_cons;
+  DBuf<int32_t> d_counters;   // [0] n_cand, [1] n_ins, [2] n_events
+  DBuf<int32_t> d_ins;
+  DBuf<RaOut> d_ra;
+  DBuf<int2> d_events, d_runs;
+  DBuf<int4> d_run_out;
+
+  ~Session() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static uint32_t read_file_of(const bsq_read_files* rf, const bsq_alignments* a, int64_t r) {
+  if (!rf || rf->n_groups <= 0) return 0;
+  uint32_t g = 0;
+  if (rf->n_groups > 1) {
+    int32_t rg = a->rg ? a->rg[r] : -1;
+    g = (rg < 0 || rg >= rf->n_groups) ? 0u : (uint32_t)rg;
+  }
+  const bool read2 = (a->flag[r] & 0x80) != 0;
+  return rf->base[g] + ((read2 && rf->count[g] > 1) ? 1u : 0u);
+}
+
+static bool ref_consuming(uint32_t op) { return op == 0 || op == 2 || op == 3 || op == 7 || op == 8; }
+
+// Pack the alignments into the device CSR (DRead records + window index) and upload.
+static void session_load(Session& S, const bsq_reference* ref, const bsq_alignments* a, const bsq_read_files* rf) {
+  const double t0 = now_ms();
+  if (!ref || !a) throw BsqError(BSQ_ERR_ARG, "null reference or alignments");
+  S.n_targets = ref->n_targets;
+  S.names.clear();
+  S.lengths.clear();
+  S.tgt_off.clear();
+  int64_t off = 0;
+  for (int t = 0; t < ref->n_targets; t++) {
+    S.names.push_back(ref->names[t]);
+    S.lengths.push_back(ref->lengths[t]);
+    S.tgt_off.push_back((int32_t)off);
+    off += ref->lengths[t] + 1;
+  }
+  if (off >= ((int64_t)1 << 31) - 4096) throw BsqError(BSQ_ERR_LIMIT, "total reference length exceeds 2^31");
+  S.n_pos = (int32_t)off;
+  S.h_ref.assign((size_t)off + kWave, '\0');
+  S.h_pos_target.assign((size_t)off, -1);
+  for (int t = 0; t < ref->n_targets; t++) {
+    memcpy(&S.h_ref[S.tgt_off[t]], ref->seqs[t], (size_t)ref->lengths[t]);
+    std::fill(S.h_pos_target.begin() + S.tgt_off[t], S.h_pos_target.begin() + S.tgt_off[t] + ref->lengths[t], t);
+  }
+  // reads
+  std::vector<DRead> dr;
+  dr.reserve((size_t)a->n_reads);
+  std::vector<uint32_t> cig;
+  std::map<int32_t, uint8_t> mslots;
+  std::vector<uint8_t> mq_of;
+  int32_t maxspan = 1;
+  int64_t last_g = -1;
+  int64_t nab = 0;
+  uint32_t max_rfile = 0;
+  for (int64_t r = 0; r < a->n_reads; r++) {
+    if (a->flag[r] & 0x4) continue;
+    const int32_t tid = a->tid[r];
+    if (tid < 0) continue;
+    if (tid >= ref->n_targets) throw BsqError(BSQ_ERR_FORMAT, "alignment target id out of range");
+    const int64_t c0 = a->cigar_off[r], c1 = a->cigar_off[r + 1];
+    const int64_t nc = c1 - c0;
+    if (nc == 0) continue;
+    if (nc > 255) throw BsqError(BSQ_ERR_LIMIT, "alignment with more than 255 CIGAR operations");
+    const int64_t lq = a->seq_off[r + 1] - a->seq_off[r];
+    if (lq > 65535) throw BsqError(BSQ_ERR_LIMIT, "read longer than 65535 bases");
+    DRead d;
+    memset(&d, 0, sizeof(d));
+    d.seq_off = a->seq_off[r];
+    const int64_t gp = (int64_t)S.tgt_off[tid] + a->pos[r];
+    int64_t span = 0, qlen = 0;
+    int n_ref_ops = 0, n_ins = 0;
+    uint32_t first_ref_op = 0;
+    int64_t y0 = 0;
+    bool seen_ref = false;
+    for (int64_t k = c0; k < c1; k++) {
+      const uint32_t op = a->cigar[k] & 15, len = a->cigar[k] >> 4;
+      if (ref_consuming(op)) {
+        span += len;
+        if (!seen_ref) first_ref_op = op;
+        seen_ref = true;
+        n_ref_ops++;
+      }
+      if (op == 0 || op == 1 || op == 4 || op == 7 || op == 8) qlen += len;
+      if (op == 1 || op == 6) n_ins++;
+      if (!seen_ref && (op == 1 || op == 4)) y0 += len;
+    }
+    if (span == 0) continue;
+    if (a->pos[r] < 0 || a->pos[r] + span > ref->lengths[tid]) throw BsqError(BSQ_ERR_FORMAT, "alignment extends past its reference");
+    if (qlen != lq) throw BsqError(BSQ_ERR_FORMAT, "CIGAR query length differs from SEQ length");
+    if (gp < last_g) throw BsqError(BSQ_ERR_FORMAT, "alignments are not sorted by coordinate");
+    last_g = gp;
+    d.gpos = (int32_t)gp;
+    d.gend = (int32_t)(gp + span);
+    d.l_qseq = (uint16_t)lq;
+    d.n_cigar = (uint8_t)nc;
+    d.cig_off = (uint32_t)cig.size();
+    for (int64_t k = c0; k < c1; k++) cig.push_back(a->cigar[k]);
+    const bool simple = (n_ref_ops == 1) && (n_ins == 0) && (first_ref_op == 0 || first_ref_op == 7 || first_ref_op == 8);
+    d.flags = (uint8_t)(((a->flag[r] & 0x10) ? kReadReversed : 0) | (simple ? kReadSimple : 0));
+    d.y0 = (uint16_t)y0;
+    // query_start_1 / query_end_1 / query_bounds_1 (alignment.cpp:125-289)
+    {
+      uint32_t qs = 1;
+      for (int64_t k = c0; k < c1; k++) {
+        if ((a->cigar[k] & 15) != 4) break;
+        qs += a->cigar[k] >> 4;
+      }
+      int32_t qe = (int32_t)qlen;
+      for (int64_t k = c1 - 1; k > c0; k--) {
+        if ((a->cigar[k] & 15) != 4) break;
+        qe -= (int32_t)(a->cigar[k] >> 4);
+      }
+      int32_t be = (int32_t)qlen;
+      for (int64_t k = c1 - 1; k > c0; k--) {
+        const uint32_t op = a->cigar[k] & 15;
+        if (op != 4 && op != 5 && op != 3) break;
+        if (op == 4) be -= (int32_t)(a->cigar[k] >> 4);
+      }
+      d.qs0 = (uint16_t)(qs - 1);
+      d.qe0 = (uint16_t)(qe - 1);
+      d.qbe0 = (uint16_t)(be - 1);
+    }
+    const int32_t x1 = (a->x1 && a->x1[r] >= 0) ? a->x1[r] : 1;
+    d.x1 = (uint16_t)std::min(x1, 65535);
+    d.xl = (int16_t)((a->xl && a->xl[r] >= 0) ? std::min(a->xl[r], 32767) : -1);
+    d.xr = (int16_t)((a->xr && a->xr[r] >= 0) ? std::min(a->xr[r], 32767) : -1);
+    const uint32_t rfi = read_file_of(rf, a, r);
+    if (rfi > 255) throw BsqError(BSQ_ERR_LIMIT, "more than 256 read files");
+    max_rfile = std::max(max_rfile, rfi);
+    d.rfile = (uint8_t)rfi;
+    auto it = mslots.find(a->mapq[r]);
+    if (it == mslots.end()) it = mslots.emplace(a->mapq[r], (uint8_t)0).first;
+    d.mslot = a->mapq[r];   // remapped below
+    maxspan = std::max<int32_t>(maxspan, (int32_t)span);
+    nab += span;
+    dr.push_back(d);
+  }
+  // MAPQ slots in ascending MAPQ order
+  S.mapq_values.clear();
+  {
+    uint8_t k = 0;
+    for (auto& kv : mslots) {
+      kv.second = k++;
+      S.mapq_values.push_back(kv.first);
+    }
+    for (DRead& d : dr) d.mslot = mslots[d.mslot];
+  }
+  S.n_read_files = (int32_t)max_rfile + 1;
+  S.n_dreads = (int64_t)dr.size();
+  if (S.n_dreads >= ((int64_t)1 << 31)) throw BsqError(BSQ_ERR_LIMIT, "more than 2^31 alignments");
+  S.aligned_bases = nab;
+  // window index: alignments overlapping [64w, 64w+64)
+  S.n_windows = ((int64_t)S.n_pos + kWave - 1) / kWave;
+  std::vector<int32_t> rb(S.n_windows), re(S.n_windows);
+  {
+    size_t lo = 0, hi = 0;
+    int32_t mx = 0;
+    for (int64_t w = 0; w < S.n_windows; w++) {
+      const int64_t w0 = w * kWave, w1 = w0 + kWave;
+      while (lo < dr.size() && (int64_t)dr[lo].gpos < w0 - maxspan) lo++;
+      while (hi < dr.size() && (int64_t)dr[hi].gpos < w1) hi++;
+      rb[w] = (int32_t)lo;
+      re[w] = (int32_t)hi;
+      mx = std::max<int32_t>(mx, (int32_t)(hi - lo));
+    }
+    S.max_window_reads = mx;
+  }
+  const double t1 = now_ms();
+  // upload
+  const double t2 = now_ms();
+  S.d_reads.alloc(dr.size());
+  HIP_OK(hipMemcpyAsync(S.d_reads.p, dr.data(), dr.size() * sizeof(DRead), hipMemcpyHostToDevice, S.stream));
+  S.d_cigar.alloc(cig.size());
+  HIP_OK(hipMemcpyAsync(S.d_cigar.p, cig.data(), cig.size() * sizeof(uint32_t), hipMemcpyHostToDevice, S.stream));
+  const size_t nbases = (size_t)a->seq_off[a->n_reads];
+  S.d_bases.alloc(nbases + 16);
+  S.d_quals.alloc(nbases + 16);
+  if (nbases) {
+    HIP_OK(hipMemcpyAsync(S.d_bases.p, a->seq, nbases, hipMemcpyHostToDevice, S.stream));
+    HIP_OK(hipMemcpyAsync(S.d_quals.p, a->qual, nbases, hipMemcpyHostToDevice, S.stream));
+  }
+  S.d_ref.alloc(S.h_ref.size());
+  HIP_OK(hipMemcpyAsync(S.d_ref.p, S.h_ref.data(), S.h_ref.size(), hipMemcpyHostToDevice, S.stream));
+  S.d_pos_target.alloc(S.h_pos_target.size());
+  HIP_OK(hipMemcpyAsync(S.d_pos_target.p, S.h_pos_target.data(), S.h_pos_target.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
+  S.d_win_rb.alloc(rb.size());
+  S.d_win_re.alloc(re.size());
+  HIP_OK(hipMemcpyAsync(S.d_win_rb.p, rb.data(), rb.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
+  HIP_OK(hipMemcpyAsync(S.d_win_re.p, re.data(), re.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
+  S.d_err.alloc(1);
+  S.d_counters.alloc(4);
+  HIP_OK(hipStreamSynchronize(S.stream));
+  S.timing.pack_ms = t1 - t0;
+  S.timing.h2d_ms = now_ms() - t2;
+  S.timing.aligned_bases = S.aligned_bases;
+  S.loaded = true;
+}
+
+static std::vector<int32_t> target_order(const Session& S, const uint8_t* call) {
+  std::vector<int32_t> t;
+  for (int i = 0; i < S.n_targets; i++)
+    if (!call || call[i]) t.push_back(i);
+  std::sort(t.begin(), t.end(), [&](int32_t a, int32_t b) { return S.names[a] < S.names[b]; });
+  return t;
+}
+
+static void check_err(Session& S) {
+  uint32_t err = 0;
+  HIP_OK(hipMemcpyAsync(&err, S.d_err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, S.stream));
+  HIP_OK(hipStreamSynchronize(S.stream));
+  if (!err) return;
+  std::string m = "alignment data violates a breseq invariant:";
+  if (err & 1) m += " read base not A/C/G/T/N;";
+  if (err & 2) m += " reference base not A/C/G/T/N;";
+  if (err & 4) m += " quality above the 'quality' covariate maximum;";
+  if (err & 8) m += " read file index above the 'read_set' covariate maximum;";
+  if (err & 16) m += " quality index outside the read;";
+  if (err & 32) m += " CIGAR does not cover its alignment span;";
+  if (err & 64) m += " device buffer capacity exceeded;";
+  throw BsqError(BSQ_ERR_DATA, m);
+}
+
+static int grid_for(int64_t n_windows) {
+  int64_t g = (n_windows + 3) / 4;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace bsq
+
+using namespace bsq;
+
+struct bsq_session {
+  Session s;
+};
+
+struct bsq_ec_result {
+  std::vector<std::vector<uint32_t>> hist;
+  std::vector<uint64_t> counts;
+  std::vector<double> log10p;
+};
+
+struct bsq_im_result {
+  std::string text;
+  int64_t n_ra = 0, n_mc = 0, n_un = 0;
+};
+
+namespace bsq {
+
+static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec_result& out) {
+  if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
+  const int Sm = args->read_set_max, Q = args->quality_max;
+  if (Sm <= 0 || Q <= 0 || Sm > 255 || Q > 256) throw BsqError(BSQ_ERR_ARG, "covariate maxima out of range");
+  const int n_bins = Sm * 25 * Q;
+  const int G = std::max(1, args->n_coverage_groups);
+  // scored columns: targets of the call set only
+  std::vector<int32_t> pos_target(S.h_pos_target);
+  std::vector<int32_t> group(S.n_targets, 0);
+  for (int t = 0; t < S.n_targets; t++) {
+    const bool call = !args->call_target || args->call_target[t];
+    int g = args->coverage_group ? args->coverage_group[t] : 0;
+    if (call && (g < 0 || g >= G)) throw BsqError(BSQ_ERR_ARG, "coverage group out of range");
+    group[t] = std::max(0, std::min(g, G - 1));
+    if (!call) std::fill(pos_target.begin() + S.tgt_off[t], pos_target.begin() + S.tgt_off[t] + S.lengths[t], -1);
+  }
+  DBuf<int32_t> d_pt;
+  d_pt.alloc(pos_target.size());
+  HIP_OK(hipMemcpyAsync(d_pt.p, pos_target.data(), pos_target.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
+  S.d_target_group.alloc(group.size());
+  HIP_OK(hipMemcpyAsync(S.d_target_group.p, group.data(), group.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
+  const int hist_len = S.max_window_reads + 2;
+  S.d_counts.alloc(n_bins);
+  S.d_hist.alloc((size_t)G * hist_len);
+  HIP_OK(hipMemsetAsync(S.d_counts.p, 0, n_bins * sizeof(unsigned long long), S.stream));
+  HIP_OK(hipMemsetAsync(S.d_hist.p, 0, (size_t)G * hist_len * sizeof(unsigned long long), S.stream));
+  HIP_OK(hipMemsetAsync(S.d_err.p, 0, sizeof(uint32_t), S.stream));
+  EcParams P;
+  P.reads = S.d_reads.p;
+  P.cigar = S.d_cigar.p;
+  P.bases = S.d_bases.p;
+  P.quals = S.d_quals.p;
+  P.ref = S.d_ref.p;
+  P.pos_target = d_pt.p;
+  P.target_group = S.d_target_group.p;
+  P.win_rb = S.d_win_rb.p;
+  P.win_re = S.d_win_re.p;
+  P.n_windows = S.n_windows;
+  P.n_pos = S.n_pos;
+  P.read_set_max = Sm;
+  P.quality_max = Q;
+  P.n_bins = n_bins;
+  P.n_groups_lds = std::min(G, 4);
+  P.hist_lds = 1024;
+  P.hist_len = hist_len;
+  P.counts = S.d_counts.p;
+  P.hist = S.d_hist.p;
+  P.err = S.d_err.p;
+  const size_t lds = (size_t)(n_bins + P.n_groups_lds * P.hist_lds) * sizeof(uint32_t);
+  if (lds > 64 * 1024) throw BsqError(BSQ_ERR_LIMIT, "covariate table too large for LDS");
+  HIP_OK(hipEventRecord(S.ev0, S.stream));
+  hipLaunchKernelGGL(k_error_count, dim3(grid_for(S.n_windows)), dim3(256), lds, S.stream, P);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(S.ev1, S.stream));
+  HIP_OK(hipStreamSynchronize(S.stream));
+  float ms = 0;
+  HIP_OK(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+  S.timing.kernel_ms = ms;
+  check_err(S);
+  const double t0 = now_ms();
+  std::vector<unsigned long long> counts(n_bins), hist((size_t)G * hist_len);
+  HIP_OK(hipMemcpy(counts.data(), S.d_counts.p, n_bins * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(hist.data(), S.d_hist.p, hist.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  S.timing.d2h_ms = now_ms() - t0;
+  const double t1 = now_ms();
+  out.counts.assign(counts.begin(), counts.end());
+  out.hist.assign(G, std::vector<uint32_t>());
+  for (int g = 0; g < G; g++) {
+    int64_t last = -1;
+    for (int c = 0; c < hist_len; c++)
+      if (hist[(size_t)g * hist_len + c]) last = c;
+    // error_count resizes its table to the largest coverage seen, so only groups that were visited
+    // have entries
+    out.hist[g].assign((size_t)(last + 1), 0);
+    for (int64_t c = 0; c <= last; c++) out.hist[g][c] = (uint32_t)hist[(size_t)g * hist_len + c];
+  }
+  counts_to_log10_prob(Sm, Q, out.counts, out.log10p);
+  if (args->output_dir) {
+    const std::string dir = args->output_dir;
+    if (args->do_coverage) {
+      for (int g = 0; g < G; g++) {
+        std::ofstream f(dir + "/" + std::to_string(g) + ".unique_only_coverage_distribution.tab");
+        if (!f) throw BsqError(BSQ_ERR_IO, "could not write coverage distribution in " + dir);
+        f << coverage_text(out.hist[g]);
+      }
+    }
+    if (args->do_errors) {
+      const std::string rates = args->error_rates_file ? std::string(args->error_rates_file) : dir + "/error_rates.tab";
+      std::ofstream f(rates);
+      if (!f) throw BsqError(BSQ_ERR_IO, "could not write " + rates);
+      f << error_rates_text(Sm, Q, out.log10p);
+      std::vector<std::string> bq = base_qual_texts(Sm, Q, out.counts);
+      for (int r = 0; r < Sm; r++) {
+        std::string name = (r < args->read_files.n_files && args->read_files.names) ? args->read_files.names[r] : std::to_string(r);
+        std::ofstream fb(dir + "/base_qual_error_prob." + name + ".tab");
+        if (!fb) throw BsqError(BSQ_ERR_IO, "could not write base_qual_error_prob in " + dir);
+        fb << bq[r];
+      }
+    }
+  }
+  S.timing.host_post_ms = now_ms() - t1;
+}
+
+// fill_read_base_likelihoods (identify_mutations.cpp:3359-3384) for every (MAPQ, read file,
+// quality, observed base, strand): the per-read cache depends on nothing else, so it is computed
+// once per key on the host with the same libm calls and looked up by the kernels.
+static void build_likelihood_tables(const Session& S, const bsq_identify_args* a, std::vector<double>& lp,
+                                    std::vector<double>& rr) {
+  const int Sm = a->read_set_max, Q = a->quality_max;
+  const size_t nm = S.mapq_values.size();
+  const size_t nkeys = nm * Sm * Q * 10;
+  lp.assign(nkeys * 5, 0.0);
+  rr.assign(nkeys * 6, 0.0);
+  std::vector<double> prob(a->n_bins);
+  for (int64_t i = 0; i < a->n_bins; i++) prob[i] = pow(10, a->log10_prob_table[i]);
+  auto comp = [](int i) { return i < 4 ? 3 - i : 4; };
+  for (size_t m = 0; m < nm; m++) {
+    const double incorrect = pow(10, -static_cast<double>(S.mapq_values[m]) / 10);
+    const double correct = 1 - incorrect;
+    const double uniform = 1.0 / 5.0;
+    for (int rs = 0; rs < Sm; rs++)
+      for (int q = 0; q < Q; q++)
+        for (int obs = 0; obs < 5; obs++)
+          for (int sb = 0; sb < 2; sb++) {
+            const size_t key = (((m * Sm + rs) * Q + q) * 10) + obs * 2 + sb;
+            const bool top = (sb == 0);
+            const int o = top ? obs : comp(obs);
+            double l[5];
+            double mx = -std::numeric_limits<double>::max();
+            for (int b = 0; b < 5; b++) {
+              const int refb = top ? b : comp(b);
+              double pr = correct * prob[rs + Sm * refb + 5 * Sm * o + 25 * Sm * q] + incorrect * uniform;
+              if (pr < 0.0) pr = 0.0;
+              l[b] = log10(pr);
+              mx = std::max(mx, l[b]);
+            }
+            for (int b = 0; b < 5; b++) {
+              lp[key * 5 + b] = l[b];
+              rr[key * 6 + b] = pow(10, l[b] - mx);
+            }
+            rr[key * 6 + 5] = mx;
+          }
+  }
+}
+
+static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& out, int screen_off) {
+  if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
+  const int Sm = a->read_set_max, Q = a->quality_max;
+  if (Sm <= 0 || Q <= 0 || Sm > 255 || Q > 256) throw BsqError(BSQ_ERR_ARG, "covariate maxima out of range");
+  if (a->n_bins != (int64_t)Sm * 25 * Q) throw BsqError(BSQ_ERR_ARG, "error table size does not match its covariates");
+  if (!a->deletion_propagation_cutoff || !a->deletion_seed_cutoff) throw BsqError(BSQ_ERR_ARG, "missing per-target cutoffs");
+  // targets
+  std::vector<TargetInfo> T(S.n_targets);
+  for (int t = 0; t < S.n_targets; t++) {
+    T[t].off = S.tgt_off[t];
+    T[t].len = (int32_t)S.lengths[t];
+    T[t].group = 0;
+    T[t].scored = (!a->call_target || a->call_target[t]) ? 1 : 0;
+    T[t].prop_cutoff = a->deletion_propagation_cutoff[t];
+    T[t].seed_cutoff = a->deletion_seed_cutoff[t];
+    if (T[t].scored && !a->skip_missing_coverage_prediction && T[t].prop_cutoff >= 0.0 && T[t].seed_cutoff > T[t].prop_cutoff)
+      throw BsqError(BSQ_ERR_ARG, "deletion seed cutoff above the propagation cutoff is not supported");
+  }
+  S.d_targets.alloc(T.size());
+  HIP_OK(hipMemcpyAsync(S.d_targets.p, T.data(), T.size() * sizeof(TargetInfo), hipMemcpyHostToDevice, S.stream));
+  std::vector<double> lp, rr;
+  build_likelihood_tables(S, a, lp, rr);
+  S.d_tab_lp.alloc(lp.size());
+  S.d_tab_r.alloc(rr.size());
+  HIP_OK(hipMemcpyAsync(S.d_tab_lp.p, lp.data(), lp.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
+  HIP_OK(hipMemcpyAsync(S.d_tab_r.p, rr.data(), rr.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
+  double total_len = 0;
+  for (int t = 0; t < S.n_targets; t++) total_len += (double)S.lengths[t];
+  S.d_col_flags.alloc(S.n_pos);
+  S.d_col_unique.alloc(S.n_pos);
+  S.d_col_total.alloc(S.n_pos);
+  int cand_cap = screen_off ? S.n_pos + 1024 : std::max<int>(1 << 16, S.n_pos / 64);
+  int ins_cap = std::max<int>(1 << 14, S.n_pos / 256);
+  S.d_cand.alloc(cand_cap);
+  S.d_cand_cons.alloc(cand_cap);
+  S.d_ins.alloc(ins_cap);
+  ImParams P;
+  P.reads = S.d_reads.p;
+  P.cigar = S.d_cigar.p;
+  P.bases = S.d_bases.p;
+  P.quals = S.d_quals.p;
+  P.ref = S.d_ref.p;
+  P.pos_target = S.d_pos_target.p;
+  P.targets = S.d_targets.p;
+  P.win_rb = S.d_win_rb.p;
+  P.win_re = S.d_win_re.p;
+  P.n_windows = S.n_windows;
+  P.n_pos = S.n_pos;
+  P.read_set_max = Sm;
+  P.quality_max = Q;
+  P.base_quality_cutoff = a->base_quality_cutoff;
+  P.tab_lp = S.d_tab_lp.p;
+  P.tab_r = S.d_tab_r.p;
+  P.log10_ref_length = log10(total_len);
+  P.mutation_cutoff = a->mutation_cutoff;
+  P.polymorphism_cutoff = a->polymorphism_cutoff;
+  P.precision_decimal = a->polymorphism_precision_decimal;
+  P.screen_margin = 0.05;
+  P.screen_off = screen_off;
+  P.col_flags = S.d_col_flags.p;
+  P.col_unique = S.d_col_unique.p;
+  P.col_total = S.d_col_total.p;
+  P.cand = S.d_cand.p;
+  P.cand_cons = S.d_cand_cons.p;
+  P.n_cand = S.d_counters.p + 0;
+  P.cand_cap = cand_cap;
+  P.ins = S.d_ins.p;
+  P.n_ins = S.d_counters.p + 1;
+  P.ins_cap = ins_cap;
+  P.key_cap = std::max(64, S.max_window_reads);
+  P.ra_out = nullptr;
+  P.err = S.d_err.p;
+  HIP_OK(hipMemsetAsync(S.d_counters.p, 0, 4 * sizeof(int32_t), S.stream));
+  HIP_OK(hipMemsetAsync(S.d_err.p, 0, sizeof(uint32_t), S.stream));
+  HIP_OK(hipEventRecord(S.ev0, S.stream));
+  hipLaunchKernelGGL(k_identify_columns, dim3(grid_for(S.n_windows)), dim3(256), 0, S.stream, P);
+  HIP_OK(hipGetLastError());
+  int32_t counters[4];
+  HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
+  HIP_OK(hipStreamSynchronize(S.stream));
+  check_err(S);
+  if (counters[1] > 0) {
+    const int n_ins = std::min(counters[1], ins_cap);
+    hipLaunchKernelGGL(k_insert_slots, dim3((n_ins + 63) / 64), dim3(64), 0, S.stream, P);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
+    HIP_OK(hipStreamSynchronize(S.stream));
+    check_err(S);
+  }
+  const int n_cand = std::min(counters[0], cand_cap);
+  S.d_ra.alloc(std::max(1, n_cand));
+  P.ra_out = S.d_ra.p;
+  if (n_cand > 0) {
+    const size_t lds = (size_t)P.key_cap * sizeof(uint32_t);
+    if (lds > 48 * 1024) throw BsqError(BSQ_ERR_LIMIT, "column depth too large for the exact scorer's LDS list");
+    const int grid = std::min(n_cand, 8192);
+    hipLaunchKernelGGL(k_score_candidates, dim3(grid), dim3(64), lds, S.stream, P, 0, n_cand);
+    HIP_OK(hipGetLastError());
+  }
+  // MC / UN
+  RunParams R;
+  R.pos_target = S.d_pos_target.p;
+  R.targets = S.d_targets.p;
+  R.col_flags = S.d_col_flags.p;
+  R.col_unique = S.d_col_unique.p;
+  R.n_pos = S.n_pos;
+  R.do_mc = a->skip_missing_coverage_prediction ? 0 : 1;
+  const int event_cap = std::max(1 << 16, S.n_pos / 8);
+  S.d_events.alloc(event_cap);
+  R.events = S.d_events.p;
+  R.n_events = S.d_counters.p + 2;
+  R.event_cap = event_cap;
+  R.runs = nullptr;
+  R.run_out = nullptr;
+  hipLaunchKernelGGL(k_runs, dim3((S.n_pos + 255) / 256), dim3(256), 0, S.stream, R);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
+  HIP_OK(hipStreamSynchronize(S.stream));
+  const int n_events = counters[2];
+  if (n_events > event_cap) throw BsqError(BSQ_ERR_LIMIT, "too many MC/UN interval boundaries");
+  std::vector<int2> events(n_events);
+  if (n_events) HIP_OK(hipMemcpy(events.data(), S.d_events.p, n_events * sizeof(int2), hipMemcpyDeviceToHost));
+  std::vector<int> starts[2], ends[2];
+  for (const int2& e : events) {
+    const int kind = e.y;
+    if (kind == 0) starts[0].push_back(e.x);
+    else if (kind == 1) ends[0].push_back(e.x);
+    else if (kind == 2) starts[1].push_back(e.x);
+    else ends[1].push_back(e.x);
+  }
+  for (int k = 0; k < 2; k++) {
+    std::sort(starts[k].begin(), starts[k].end());
+    std::sort(ends[k].begin(), ends[k].end());
+    if (starts[k].size() != ends[k].size()) throw BsqError(BSQ_ERR_DATA, "unpaired interval boundaries");
+  }
+  std::vector<int2> mc_runs;
+  for (size_t i = 0; i < starts[0].size(); i++) mc_runs.push_back(make_int2(starts[0][i], ends[0][i]));
+  std::vector<int4> mc_out(mc_runs.size());
+  if (!mc_runs.empty()) {
+    S.d_runs.alloc(mc_runs.size());
+    S.d_run_out.alloc(mc_runs.size());
+    HIP_OK(hipMemcpyAsync(S.d_runs.p, mc_runs.data(), mc_runs.size() * sizeof(int2), hipMemcpyHostToDevice, S.stream));
+    R.runs = S.d_runs.p;
+    R.run_out = S.d_run_out.p;
+    hipLaunchKernelGGL(k_mc_detail, dim3(((int)mc_runs.size() + 63) / 64), dim3(64), 0, S.stream, R, (int)mc_runs.size());
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipEventRecord(S.ev1, S.stream));
+  HIP_OK(hipStreamSynchronize(S.stream));
+  float ms = 0;
+  HIP_OK(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+  S.timing.kernel_ms = ms;
+  S.timing.candidates = n_cand;
+  check_err(S);
+  const double t0 = now_ms();
+  std::vector<RaOut> ra(n_cand);
+  if (n_cand) HIP_OK(hipMemcpy(ra.data(), S.d_ra.p, n_cand * sizeof(RaOut), hipMemcpyDeviceToHost));
+  if (!mc_runs.empty()) HIP_OK(hipMemcpy(mc_out.data(), S.d_run_out.p, mc_out.size() * sizeof(int4), hipMemcpyDeviceToHost));
+  // unique coverage at MC boundary columns
+  std::vector<int32_t> col_unique;
+  if (!mc_runs.empty()) {
+    col_unique.resize(S.n_pos);
+    HIP_OK(hipMemcpy(col_unique.data(), S.d_col_unique.p, S.n_pos * sizeof(int32_t), hipMemcpyDeviceToHost));
+  }
+  S.timing.d2h_ms = now_ms() - t0;
+  const double t1 = now_ms();
+  // assemble entries
+  std::vector<int32_t> order = target_order(S, a->call_target);
+  std::vector<int64_t> rank(S.n_targets, -1);
+  for (size_t i = 0; i < order.size(); i++) rank[order[i]] = (int64_t)i;
+  std::vector<GdEntry> entries;
+  for (const RaOut& r : ra) {
+    if (!r.emitted) continue;
+    const int t = S.h_pos_target[r.gpos];
+    GdEntry e;
+    const uint32_t position = (uint32_t)(r.gpos - S.tgt_off[t] + 1);
+    ra_entry_from_result(r, S.names[t], position, a->polymorphism_precision_places, e);
+    e.order_target = rank[t];
+    e.order_pos = position;
+    e.order_sub = 2 + r.slot;
+    entries.push_back(std::move(e));
+    out.n_ra++;
+  }
+  for (size_t i = 0; i < mc_runs.size(); i++) {
+    if (!mc_out[i].x) continue;   // seed never reached
+    const int s = mc_runs[i].x, e_ = mc_runs[i].y;
+    const int t = S.h_pos_target[s];
+    const int32_t base = S.tgt_off[t];
+    const uint32_t start = (uint32_t)(s - base + 1), end = (uint32_t)(e_ - base + 1);
+    const uint32_t red_start = (mc_out[i].y >= 0) ? (uint32_t)(mc_out[i].y - base + 1) : start;
+    const uint32_t red_end = (mc_out[i].z >= 0) ? (uint32_t)(mc_out[i].z - base + 1) : end;
+    GdEntry e;
+    e.type = 1;
+    e.seq_id = S.names[t];
+    e.pos = start;
+    e.end = end;
+    e.start_range = red_start - start;
+    e.end_range = end - red_end;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    const double lo = (start == 1) ? nan : (double)col_unique[s - 1];
+    const double ro = (end == (uint32_t)S.lengths[t]) ? nan : (double)col_unique[e_ + 1];
+    e.fields = {{"left_inside_cov", fmt_double((double)col_unique[s], 0, false)},
+                {"left_outside_cov", fmt_double(lo, 0, false)},
+                {"right_inside_cov", fmt_double((double)col_unique[e_], 0, false)},
+                {"right_outside_cov", fmt_double(ro, 0, false)}};
+    e.order_target = rank[t];
+    e.order_pos = end + 1;
+    e.order_sub = 0;
+    entries.push_back(std::move(e));
+    out.n_mc++;
+  }
+  for (size_t i = 0; i < starts[1].size(); i++) {
+    const int s = starts[1][i], e_ = ends[1][i];
+    const int t = S.h_pos_target[s];
+    const int32_t base = S.tgt_off[t];
+    GdEntry e;
+    e.type = 2;
+    e.seq_id = S.names[t];
+    e.pos = (uint32_t)(s - base + 1);
+    e.end = (uint32_t)(e_ - base + 1);
+    e.order_target = rank[t];
+    e.order_pos = e.end + 1;
+    e.order_sub = 1;
+    entries.push_back(std::move(e));
+    out.n_un++;
+  }
+  if (!a->skip_missing_coverage_prediction) {
+    for (int t : order) {
+      if (a->deletion_propagation_cutoff[t] >= 0.0) continue;
+      GdEntry e;
+      e.type = 1;
+      e.seq_id = S.names[t];
+      e.pos = 1;
+      e.end = (uint32_t)S.lengths[t];
+      e.fields = {{"left_inside_cov", fmt_double(0.0, 0, false)},
+                  {"left_outside_cov", "NA"},
+                  {"right_inside_cov", fmt_double(0.0, 0, false)},
+                  {"right_outside_cov", "NA"}};
+      e.order_target = rank[t];
+      e.order_pos = S.lengths[t] + 2;
+      e.order_sub = 0;
+      entries.push_back(std::move(e));
+      out.n_mc++;
+    }
+  }
+  out.text = write_gd(entries);
+  if (a->gd_file) {
+    std::ofstream f(a->gd_file);
+    if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + a->gd_file);
+    f << out.text;
+  }
+  S.timing.host_post_ms = now_ms() - t1;
+}
+
+}  // namespace bsq
+
+/* ------------------------------------------------------------------------------------------ */
+/* C ABI                                                                                       */
+
+#define BSQ_TRY try {
+#define BSQ_CATCH(status)                          \
+  }                                                \
+  catch (const BsqError& e) {                      \
+    set_status(status, e.code, e.what());          \
+    return e.code;                                 \
+  }                                                \
+  catch (const std::bad_alloc&) {                  \
+    set_status(status, BSQ_ERR_LIMIT, "out of host memory"); \
+    return BSQ_ERR_LIMIT;                          \
+  }                                                \
+  catch (const std::exception& e) {                \
+    set_status(status, BSQ_ERR_ARG, e.what());     \
+    return BSQ_ERR_ARG;                            \
+  }
+
+extern "C" {
+
+int bsq_session_create(int32_t device, bsq_session** out, bsq_status* status) {
+  BSQ_TRY
+  if (!out) throw BsqError(BSQ_ERR_ARG, "null output pointer");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw BsqError(BSQ_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) throw BsqError(BSQ_ERR_DEVICE, "device index out of range");
+  std::unique_ptr<bsq_session> s(new bsq_session());
+  s->s.device = device;
+  HIP_OK(hipSetDevice(device));
+  HIP_OK(hipStreamCreateWithFlags(&s->s.stream, hipStreamNonBlocking));
+  HIP_OK(hipEventCreate(&s->s.ev0));
+  HIP_OK(hipEventCreate(&s->s.ev1));
+  *out = s.release();
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
+  BSQ_CATCH(status)
+}
+
+int bsq_session_load(bsq_session* s, const bsq_reference* ref, const bsq_alignments* aln, const bsq_read_files* rf,
+                     bsq_status* status) {
+  BSQ_TRY
+  if (!s) throw BsqError(BSQ_ERR_ARG, "null session");
+  HIP_OK(hipSetDevice(s->s.device));
+  session_load(s->s, ref, aln, rf);
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
+  BSQ_CATCH(status)
+}
+
+int bsq_session_error_count(bsq_session* s, const bsq_error_count_args* args, bsq_ec_result** out, bsq_status* status) {
+  BSQ_TRY
+  if (!s || !args || !out) throw BsqError(BSQ_ERR_ARG, "null argument");
+  HIP_OK(hipSetDevice(s->s.device));
+  std::unique_ptr<bsq_ec_result> r(new bsq_ec_result());
+  run_error_count(s->s, args, *r);
+  *out = r.release();
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
+  BSQ_CATCH(status)
+}
+
+int bsq_session_identify(bsq_session* s, const bsq_identify_args* args, bsq_im_result** out, bsq_status* status) {
+  BSQ_TRY
+  if (!s || !args || !out) throw BsqError(BSQ_ERR_ARG, "null argument");
+  HIP_OK(hipSetDevice(s->s.device));
+  std::unique_ptr<bsq_im_result> r(new bsq_im_result());
+  run_identify(s->s, args, *r, 0);
+  *out = r.release();
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
+  BSQ_CATCH(status)
+}
+
+// Same as bsq_session_identify, but every covered column is scored exactly (the screen is off).
+// Used by the tests to show that the screen never drops an entry.
+int bsq_session_identify_unscreened(bsq_session* s, const bsq_identify_args* args, bsq_im_result** out,
+                                    bsq_status* status) {
+  BSQ_TRY
+  if (!s || !args || !out) throw BsqError(BSQ_ERR_ARG, "null argument");
+  HIP_OK(hipSetDevice(s->s.device));
+  std::unique_ptr<bsq_im_result> r(new bsq_im_result());
+  run_identify(s->s, args, *r, 1);
+  *out = r.release();
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
+  BSQ_CATCH(status)
+}
+
+int bsq_session_timing(const bsq_session* s, bsq_timing* out) {
+  if (!s || !out) return BSQ_ERR_ARG;
+  *out = s->s.timing;
+  return BSQ_OK;
+}
+
+void bsq_session_destroy(bsq_session* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->s.device);
+  delete s;
+}
+
+int bsq_error_count(const bsq_error_count_args* args, bsq_ec_result** out, bsq_status* status) {
+  bsq_session* s = nullptr;
+  int rc = bsq_session_create(0, &s, status);
+  if (rc) return rc;
+  rc = bsq_session_load(s, args->ref, args->aln, &args->read_files, status);
+  if (!rc) rc = bsq_session_error_count(s, args, out, status);
+  bsq_session_destroy(s);
+  return rc;
+}
+
+int bsq_identify_mutations(const bsq_identify_args* args, bsq_im_result** out, bsq_status* status) {
+  bsq_session* s = nullptr;
+  int rc = bsq_session_create(0, &s, status);
+  if (rc) return rc;
+  rc = bsq_session_load(s, args->ref, args->aln, &args->read_files, status);
+  if (!rc) rc = bsq_session_identify(s, args, out, status);
+  bsq_session_destroy(s);
+  return rc;
+}
+
+int bsq_ec_result_histogram(const bsq_ec_result* r, int32_t group, const uint32_t** hist, int64_t* len) {
+  if (!r || group < 0 || group >= (int32_t)r->hist.size()) return BSQ_ERR_ARG;
+  *hist = r->hist[group].data();
+  *len = (int64_t)r->hist[group].size();
+  return BSQ_OK;
+}
+
+int bsq_ec_result_counts(const bsq_ec_result* r, const uint64_t** counts, int64_t* n) {
+  if (!r) return BSQ_ERR_ARG;
+  *counts = r->counts.data();
+  *n = (int64_t)r->counts.size();
+  return BSQ_OK;
+}
+
+int bsq_ec_result_log10_prob(const bsq_ec_result* r, const double** t, int64_t* n) {
+  if (!r) return BSQ_ERR_ARG;
+  *t = r->log10p.data();
+  *n = (int64_t)r->log10p.size();
+  return BSQ_OK;
+}
+
+void bsq_ec_result_free(bsq_ec_result* r) { delete r; }
+
+int bsq_im_result_gd_text(const bsq_im_result* r, const char** text, int64_t* len) {
+  if (!r) return BSQ_ERR_ARG;
+  *text = r->text.c_str();
+  *len = (int64_t)r->text.size();
+  return BSQ_OK;
+}
+
+int bsq_im_result_counts(const bsq_im_result* r, int64_t* n_ra, int64_t* n_mc, int64_t* n_un) {
+  if (!r) return BSQ_ERR_ARG;
+  *n_ra = r->n_ra;
+  *n_mc = r->n_mc;
+  *n_un = r->n_un;
+  return BSQ_OK;
+}
+
+void bsq_im_result_free(bsq_im_result* r) { delete r; }
+
+void bsq_ra_test_settings_default(bsq_ra_test_settings* s, int32_t poly) {
+  memset(s, 0, sizeof(*s));
+  s->polymorphism_prediction = poly;
+  s->mutation_log10_e_value_cutoff = 10;
+  s->consensus_frequency_cutoff = poly ? 0.95 : 0.50;
+  s->polymorphism_log10_e_value_cutoff = poly ? 2 : 10;
+  s->polymorphism_frequency_cutoff = poly ? 0.05 : 0.10;
+  s->polymorphism_minimum_variant_coverage_each_strand = 2;
+  s->polymorphism_fisher_strand_p_value_cutoff = 0.05;
+  s->polymorphism_ks_quality_p_value_cutoff = 0;
+}
+
+int bsq_test_ra_evidence(const char* gd_text, const bsq_reference* ref, const bsq_ra_test_settings* s, char** out_text,
+                         int64_t* out_len, bsq_status* status) {
+  BSQ_TRY
+  if (!gd_text || !s || !out_text) throw BsqError(BSQ_ERR_ARG, "null argument");
+  std::string r = test_ra_evidence_text(gd_text, ref, *s);
+  char* buf = (char*)malloc(r.size() + 1);
+  if (!buf) throw std::bad_alloc();
+  memcpy(buf, r.data(), r.size());
+  buf[r.size()] = '\0';
+  *out_text = buf;
+  if (out_len) *out_len = (int64_t)r.size();
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
+  BSQ_CATCH(status)
+}
+
+void bsq_free_text(char* text) { free(text); }
+
+}  // extern "C"

@@ -1,0 +1,290 @@
+/*
+ * bsq.h -- C ABI of the MI355X pileup-scoring engine (breseq_amd).
+ *
+ * Drop-in boundary for breseq's pileup-scoring path.  Every entry point takes plain C structs,
+ * pointers and sizes; nothing here names a torch or HIP type.  Each function cites the breseq
+ * interface (src/breseq/..., file:line) whose behaviour it replaces.
+ *
+ * Conventions
+ *   - Positions in structs are 0-based unless a name ends in _1.
+ *   - Every call returns BSQ_OK (0) or an error code and fills *status (may be NULL).  Errors are
+ *     returned, never raised with exit(): breseq's ASSERT/ERROR (common.h:111-193) ends the
+ *     process, which a serving loop cannot afford.
+ *   - Result objects are owned by the library and released with the matching *_free call.
+ */
+#ifndef BSQ_H
+#define BSQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSQ_ABI_VERSION 1
+
+enum bsq_code {
+  BSQ_OK = 0,
+  BSQ_ERR_ARG = 1,      /* malformed argument or unsupported option                    */
+  BSQ_ERR_IO = 2,       /* file could not be opened, read or written                   */
+  BSQ_ERR_FORMAT = 3,   /* BAM / FASTA / table text not in the expected format         */
+  BSQ_ERR_DEVICE = 4,   /* HIP runtime failure or no device                            */
+  BSQ_ERR_LIMIT = 5,    /* input exceeds a documented size limit                       */
+  BSQ_ERR_DATA = 6      /* data violates an invariant breseq asserts (e.g. covariate
+                           value above its declared maximum, error_count.cpp:484-491)  */
+};
+
+typedef struct bsq_status {
+  int32_t code;
+  char message[480];
+} bsq_status;
+
+/* ------------------------------------------------------------------------------------------ */
+/* Input data model                                                                           */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Reference sequences, in BAM header (target id) order.  Replaces the FASTA loaded by
+ * pileup_base::pileup_base (pileup_base.cpp:61-88) and reference_sequence (:36-44). */
+typedef struct bsq_reference {
+  int32_t n_targets;
+  const char* const* names;   /* seq_id of each target                                       */
+  const int64_t* lengths;     /* length of each target                                       */
+  const char* const* seqs;    /* upper-case bases of each target (lengths[i] chars)          */
+} bsq_reference;
+
+/* Sorted alignments, one record per BAM record in file order, as the fields breseq reads
+ * through alignment_wrapper / pileup_wrapper (alignment.h:43-416). */
+typedef struct bsq_alignments {
+  int64_t n_reads;
+  const int32_t* tid;         /* reference target id                                          */
+  const int32_t* pos;         /* 0-based leftmost aligned reference position (BAM POS)       */
+  const uint16_t* flag;       /* BAM FLAG                                                     */
+  const uint8_t* mapq;        /* MAPQ                                                         */
+  const int64_t* cigar_off;   /* n_reads+1 offsets into cigar                                 */
+  const uint32_t* cigar;      /* BAM CIGAR words (len<<4 | op)                                */
+  const int64_t* seq_off;     /* n_reads+1 offsets into seq / qual                            */
+  const uint8_t* seq;         /* one BAM 4-bit base code per byte (A=1 C=2 G=4 T=8 N=15)     */
+  const uint8_t* qual;        /* phred base qualities                                         */
+  const int32_t* x1;          /* X1:i redundancy tag, -1 when absent (alignment.cpp:52-57)   */
+  const int32_t* xl;          /* XL:i left trim tag, -1 when absent (alignment.h:389-410)    */
+  const int32_t* xr;          /* XR:i right trim tag, -1 when absent                         */
+  const int32_t* rg;          /* index of the RG:Z tag in the header @RG list, -1 when absent */
+} bsq_alignments;
+
+/* Read group -> read file resolution (alignment.h:533-591, read_group_index_map +
+ * read_file_partition).  n_groups == 0 resolves every record to read file 0. */
+typedef struct bsq_read_files {
+  int32_t n_groups;
+  const uint32_t* base;       /* flat index of each group's first read file                   */
+  const uint32_t* count;      /* number of read files in each group (1 or 2)                  */
+  int32_t n_files;
+  const char* const* names;   /* read file base names, for base_qual_error_prob.#.tab         */
+} bsq_read_files;
+
+/* ------------------------------------------------------------------------------------------ */
+/* error_count  (error_count.h:41-52, error_count.cpp:50-68)                                  */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct bsq_error_count_args {
+  const bsq_reference* ref;
+  const bsq_alignments* aln;
+  bsq_read_files read_files;
+  const int32_t* coverage_group;   /* per target; Settings::seq_id_to_coverage_group           */
+  int32_t n_coverage_groups;
+  const uint8_t* call_target;      /* per target 1/0: call_mutations_seq_id_set (NULL = all)   */
+  int32_t read_set_max;            /* covariate "read_set=N"                                   */
+  int32_t quality_max;             /* covariate "quality=max_qual+1"                           */
+  int32_t do_coverage;             /* write @.unique_only_coverage_distribution.tab            */
+  int32_t do_errors;               /* write error_rates.tab and base_qual_error_prob.#.tab     */
+  const char* output_dir;          /* NULL: compute only, write nothing                        */
+  const char* error_rates_file;    /* full path of error_rates.tab (Settings, settings.cpp:1555) */
+} bsq_error_count_args;
+
+typedef struct bsq_ec_result bsq_ec_result;
+
+/* Histogram of unique-only coverage of coverage group g: hist[c] = positions with c
+ * non-deletion, non-redundant reads and no redundant read (error_count.cpp:125-199). */
+int bsq_ec_result_histogram(const bsq_ec_result* r, int32_t group, const uint32_t** hist,
+                            int64_t* len);
+/* Covariate count table (error_count.cpp:854-997); index = read_set + S*ref + 5S*obs + 25S*q. */
+int bsq_ec_result_counts(const bsq_ec_result* r, const uint64_t** counts, int64_t* n_bins);
+/* log10 probability table after counts_to_log10_prob (error_count.cpp:1005-1026). */
+int bsq_ec_result_log10_prob(const bsq_ec_result* r, const double** table, int64_t* n_bins);
+void bsq_ec_result_free(bsq_ec_result* r);
+
+/* ------------------------------------------------------------------------------------------ */
+/* identify_mutations  (identify_mutations.h:46-60, identify_mutations.cpp:48-88)             */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct bsq_identify_args {
+  const bsq_reference* ref;
+  const bsq_alignments* aln;
+  bsq_read_files read_files;
+  const uint8_t* call_target;              /* per target (NULL = all)                         */
+  const double* log10_prob_table;          /* as re-read from error_rates.tab text            */
+  int64_t n_bins;
+  int32_t read_set_max;
+  int32_t quality_max;
+  const double* deletion_propagation_cutoff; /* per target; < 0 means the coverage fit failed */
+  const double* deletion_seed_cutoff;        /* per target                                   */
+  double mutation_cutoff;                  /* consensus score cutoff (10)                    */
+  double polymorphism_cutoff;              /* 10 consensus mode, 2 polymorphism mode         */
+  double polymorphism_precision_decimal;   /* 1e-6                                           */
+  uint32_t polymorphism_precision_places;  /* 3 consensus mode, 8 polymorphism mode          */
+  int32_t base_quality_cutoff;             /* Settings::base_quality_cutoff (3)              */
+  int32_t skip_missing_coverage_prediction;
+  const char* gd_file;                     /* NULL: do not write ra_mc_evidence.gd           */
+} bsq_identify_args;
+
+typedef struct bsq_im_result bsq_im_result;
+
+/* ra_mc_evidence.gd text exactly as cGenomeDiff::write would print it (genome_diff.cpp:685). */
+int bsq_im_result_gd_text(const bsq_im_result* r, const char** text, int64_t* len);
+/* Number of RA / MC / UN entries. */
+int bsq_im_result_counts(const bsq_im_result* r, int64_t* n_ra, int64_t* n_mc, int64_t* n_un);
+void bsq_im_result_free(bsq_im_result* r);
+
+/* ------------------------------------------------------------------------------------------ */
+/* test_RA_evidence  (identify_mutations.h:63-67, identify_mutations.cpp:687-749)            */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct bsq_ra_test_settings {
+  int32_t polymorphism_prediction;
+  double mutation_log10_e_value_cutoff;
+  double polymorphism_log10_e_value_cutoff;
+  double consensus_frequency_cutoff;
+  double polymorphism_frequency_cutoff;
+  double consensus_minimum_variant_coverage;
+  double consensus_minimum_total_coverage;
+  double consensus_minimum_variant_coverage_each_strand;
+  double consensus_minimum_total_coverage_each_strand;
+  double polymorphism_minimum_variant_coverage;
+  double polymorphism_minimum_total_coverage;
+  double polymorphism_minimum_variant_coverage_each_strand;
+  double polymorphism_minimum_total_coverage_each_strand;
+  double polymorphism_fisher_strand_p_value_cutoff;
+  double polymorphism_ks_quality_p_value_cutoff;
+  int32_t consensus_reject_indel_homopolymer_length;
+  int32_t consensus_reject_surrounding_homopolymer_length;
+  int32_t polymorphism_reject_indel_homopolymer_length;
+  int32_t polymorphism_reject_surrounding_homopolymer_length;
+  int32_t polymorphism_no_indels;
+} bsq_ra_test_settings;
+
+/* Mode defaults of Settings (settings.cpp:857-950). */
+void bsq_ra_test_settings_default(bsq_ra_test_settings* s, int32_t polymorphism_prediction);
+
+/* Classify the RA entries of a GD text (as written by identify_mutations) and return the new
+ * GD text.  ref is needed only by the homopolymer filters. */
+int bsq_test_ra_evidence(const char* gd_text, const bsq_reference* ref,
+                         const bsq_ra_test_settings* s, char** out_text, int64_t* out_len,
+                         bsq_status* status);
+void bsq_free_text(char* text);
+
+/* ------------------------------------------------------------------------------------------ */
+/* calculate_trims  (calculate_trims.h:34, calculate_trims.cpp:67-179)                        */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Per-position left/right trims of one reference sequence of length len:
+ * out[0..len) = left trims, out[len..2len) = right trims (SequenceTrims layout). */
+int bsq_sequence_trims(const char* seq, int64_t len, uint8_t* out, bsq_status* status);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Coverage distribution fit (coverage_distribution.cpp:345-399)                              */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct bsq_coverage_fit {
+  double nb_fit_size, nb_fit_mu, average, variance, relative_variance;
+  double deletion_coverage_propagation_cutoff;
+  uint32_t censor_start, censor_end;
+} bsq_coverage_fit;
+
+int bsq_fit_coverage_distribution(const uint32_t* hist, int64_t len, double sequence_length,
+                                  bsq_coverage_fit* out, bsq_status* status);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Device-resident sessions (MI355X path)                                                     */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct bsq_session bsq_session;
+
+typedef struct bsq_timing {
+  double pack_ms;          /* host CSR packing                                    */
+  double h2d_ms;           /* host -> device copies                               */
+  double kernel_ms;        /* kernels of the last pass, HIP events on the stream  */
+  double d2h_ms;           /* result copies back                                  */
+  double host_post_ms;     /* host formatting of results                          */
+  int64_t aligned_bases;   /* pileup entries (M/=/X bases + D positions)          */
+  int64_t candidates;      /* positions sent to the exact FP64 scorer             */
+} bsq_timing;
+
+/* Create a session bound to HIP device `device`.  Fails with BSQ_ERR_DEVICE when no GPU. */
+int bsq_session_create(int32_t device, bsq_session** out, bsq_status* status);
+/* Pack alignments into the device CSR and copy reference + reads to HBM (once). */
+int bsq_session_load(bsq_session* s, const bsq_reference* ref, const bsq_alignments* aln,
+                     const bsq_read_files* read_files, bsq_status* status);
+/* error_count on the loaded sample.  args->ref / args->aln are ignored (the loaded ones are
+ * used). */
+int bsq_session_error_count(bsq_session* s, const bsq_error_count_args* args,
+                            bsq_ec_result** out, bsq_status* status);
+/* identify_mutations on the loaded sample. */
+int bsq_session_identify(bsq_session* s, const bsq_identify_args* args, bsq_im_result** out,
+                         bsq_status* status);
+/* identify_mutations with the candidate screen disabled: every covered column goes through the
+ * exact FP64 scorer.  Same output as bsq_session_identify; used to show the screen drops nothing. */
+int bsq_session_identify_unscreened(bsq_session* s, const bsq_identify_args* args,
+                                    bsq_im_result** out, bsq_status* status);
+int bsq_session_timing(const bsq_session* s, bsq_timing* out);
+void bsq_session_destroy(bsq_session* s);
+
+/* One-shot conveniences: create a session on device 0, load, run, destroy. */
+int bsq_error_count(const bsq_error_count_args* args, bsq_ec_result** out, bsq_status* status);
+int bsq_identify_mutations(const bsq_identify_args* args, bsq_im_result** out,
+                           bsq_status* status);
+
+/* ------------------------------------------------------------------------------------------ */
+/* BAM / FASTA I/O (replaces htslib hts_open/sam_read1/fai_fetch on this path)                */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct bsq_sample bsq_sample;   /* owns a bsq_reference + bsq_alignments */
+
+int bsq_sample_read(const char* bam_path, const char* fasta_path, bsq_sample** out,
+                    bsq_status* status);
+int bsq_sample_write_bam(const bsq_sample* s, const char* bam_path, bsq_status* status);
+int bsq_sample_write_fasta(const bsq_sample* s, const char* fasta_path, bsq_status* status);
+const bsq_reference* bsq_sample_reference(const bsq_sample* s);
+const bsq_alignments* bsq_sample_alignments(const bsq_sample* s);
+/* Number of @RG lines and their IDs (header order). */
+int32_t bsq_sample_n_read_groups(const bsq_sample* s);
+const char* bsq_sample_read_group(const bsq_sample* s, int32_t i);
+void bsq_sample_free(bsq_sample* s);
+
+/* Synthetic sample generator (seeded, deterministic): random reference, planted variants,
+ * reads drawn from the mutated genome and aligned back by construction. */
+typedef struct bsq_sim_config {
+  uint64_t seed;
+  int64_t ref_length;
+  double gc;                 /* GC fraction of the reference                        */
+  int32_t read_length;
+  double coverage;
+  int32_t paired;            /* 1: two read files (R1/R2), insert ~ N(mean, sd)      */
+  double insert_mean, insert_sd;
+  double qual_mean, qual_sd;
+  int32_t qual_min, qual_max;
+  int32_t n_snps;
+  int32_t n_indels;
+  int32_t indel_max;
+  double variant_freq_min;   /* planted variant frequency range (1,1 = fixed)       */
+  double variant_freq_max;
+  double redundant_fraction; /* fraction of reads given X1>1                        */
+  int32_t mapq;
+} bsq_sim_config;
+
+int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_status* status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BSQ_H */

@@ -1,0 +1,59 @@
+/*
+ * oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Single-threaded CPU restatement of breseq's pileup-scoring path.  Only tests/, the smoke check
+ * of __graft_entry__ and the CPU-baseline leg of bench.py may load it, and only as the checker.
+ * The product (breseq_amd) never links or calls anything here.
+ *
+ * Parity pins (what fixes each function independently of this restatement) are listed in the
+ * header comment of oracle.cpp next to each function.
+ */
+#ifndef BSQ_ORACLE_H
+#define BSQ_ORACLE_H
+
+#include "../include/bsq.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ec_out orc_ec_out;
+typedef struct orc_im_out orc_im_out;
+
+/* error_count (error_count.cpp:50-68) -> histograms, counts, log10 table and the text of
+ * error_rates.tab / @.unique_only_coverage_distribution.tab / base_qual_error_prob.#.tab */
+int orc_error_count(const bsq_error_count_args* args, orc_ec_out** out, bsq_status* status);
+int orc_ec_histogram(const orc_ec_out* o, int32_t group, const uint32_t** hist, int64_t* len);
+int orc_ec_counts(const orc_ec_out* o, const uint64_t** counts, int64_t* n);
+int orc_ec_log10_prob(const orc_ec_out* o, const double** t, int64_t* n);
+/* which: 0 = error_rates.tab, 1 = coverage distribution of group `index`,
+ *        2 = base_qual_error_prob of read file `index` */
+int orc_ec_text(const orc_ec_out* o, int32_t which, int32_t index, const char** text, int64_t* len);
+void orc_ec_free(orc_ec_out* o);
+
+/* Parse error_rates.tab text as read_log10_prob_table (error_count.cpp:629-654). */
+int orc_read_log10_prob_table(const char* text, double* out, int64_t n_bins);
+
+/* identify_mutations (identify_mutations.cpp:48-88) -> ra_mc_evidence.gd text */
+int orc_identify_mutations(const bsq_identify_args* args, orc_im_out** out, bsq_status* status);
+int orc_im_gd_text(const orc_im_out* o, const char** text, int64_t* len);
+/* Per position (slot 0) diagnostics for error reporting: consensus score, unique cov, ... */
+int orc_im_position_scores(const orc_im_out* o, const double** score, int64_t* n);
+void orc_im_free(orc_im_out* o);
+
+/* SequenceTrims (calculate_trims.cpp:67-138) */
+int orc_sequence_trims(const char* seq, int64_t len, uint8_t* out);
+
+/* stats (stats.cpp:2144-2284) */
+double orc_fisher_exact_2x2(uint32_t a, uint32_t b, uint32_t c, uint32_t d);
+double orc_ks_test_less(const double* x, int64_t nx, const double* y, int64_t ny);
+
+/* Alignment pileup traversal diagnostics: entries of position pos0 of target tid, in order.
+ * Writes up to cap (read index, qpos, indel, is_del) quadruples; returns the count. */
+int64_t orc_pileup_column(const bsq_alignments* aln, int32_t tid, int64_t pos0, int32_t* out,
+                          int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,136 @@
+"""Test-side binding of the oracle (oracle/liboracle.so): the CPU restatement of breseq's
+pileup-scoring path, used only as the checker."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from breseq_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_O = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+
+
+def lib():
+    global _O
+    if _O is not None:
+        return _O
+    if not os.path.exists(ORACLE_LIB):
+        build()
+    O = C.CDLL(ORACLE_LIB)
+    P = C.POINTER
+    vp = C.c_void_p
+    st = P(L.Status)
+    O.orc_error_count.argtypes = [P(L.ErrorCountArgs), P(vp), st]
+    O.orc_ec_histogram.argtypes = [vp, C.c_int32, P(L.c_u32p), L.c_i64p]
+    O.orc_ec_counts.argtypes = [vp, P(P(C.c_uint64)), L.c_i64p]
+    O.orc_ec_log10_prob.argtypes = [vp, P(L.c_f64p), L.c_i64p]
+    O.orc_ec_text.argtypes = [vp, C.c_int32, C.c_int32, P(C.c_char_p), L.c_i64p]
+    O.orc_ec_free.argtypes = [vp]
+    O.orc_read_log10_prob_table.argtypes = [C.c_char_p, L.c_f64p, C.c_int64]
+    O.orc_identify_mutations.argtypes = [P(L.IdentifyArgs), P(vp), st]
+    O.orc_im_gd_text.argtypes = [vp, P(C.c_char_p), L.c_i64p]
+    O.orc_im_position_scores.argtypes = [vp, P(L.c_f64p), L.c_i64p]
+    O.orc_im_free.argtypes = [vp]
+    O.orc_sequence_trims.argtypes = [C.c_char_p, C.c_int64, L.c_u8p]
+    O.orc_fisher_exact_2x2.restype = C.c_double
+    O.orc_fisher_exact_2x2.argtypes = [C.c_uint32] * 4
+    O.orc_ks_test_less.restype = C.c_double
+    O.orc_ks_test_less.argtypes = [L.c_f64p, C.c_int64, L.c_f64p, C.c_int64]
+    O.orc_pileup_column.restype = C.c_int64
+    O.orc_pileup_column.argtypes = [P(L.Alignments), C.c_int32, C.c_int64, L.c_i32p, C.c_int64]
+    _O = O
+    return O
+
+
+def error_count(args):
+    O = lib()
+    h = C.c_void_p()
+    st = L.Status()
+    rc = O.orc_error_count(C.byref(args), C.byref(h), C.byref(st))
+    L.check(rc, st)
+    try:
+        hists = []
+        for g in range(args.n_coverage_groups):
+            p = L.c_u32p()
+            n = C.c_int64()
+            O.orc_ec_histogram(h, g, C.byref(p), C.byref(n))
+            hists.append(np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32))
+        p64 = C.POINTER(C.c_uint64)()
+        n = C.c_int64()
+        O.orc_ec_counts(h, C.byref(p64), C.byref(n))
+        counts = np.ctypeslib.as_array(p64, shape=(n.value,)).copy()
+        pd = L.c_f64p()
+        O.orc_ec_log10_prob(h, C.byref(pd), C.byref(n))
+        log10p = np.ctypeslib.as_array(pd, shape=(n.value,)).copy()
+        texts = {}
+        t = C.c_char_p()
+        O.orc_ec_text(h, 0, 0, C.byref(t), C.byref(n))
+        texts["error_rates"] = C.string_at(t, n.value).decode()
+        for g in range(args.n_coverage_groups):
+            O.orc_ec_text(h, 1, g, C.byref(t), C.byref(n))
+            texts["coverage_%d" % g] = C.string_at(t, n.value).decode()
+        for r in range(args.read_set_max):
+            if O.orc_ec_text(h, 2, r, C.byref(t), C.byref(n)) == 0:
+                texts["base_qual_%d" % r] = C.string_at(t, n.value).decode()
+    finally:
+        O.orc_ec_free(h)
+    return hists, counts, log10p, texts
+
+
+def identify_mutations(args):
+    O = lib()
+    h = C.c_void_p()
+    st = L.Status()
+    rc = O.orc_identify_mutations(C.byref(args), C.byref(h), C.byref(st))
+    L.check(rc, st)
+    try:
+        t = C.c_char_p()
+        n = C.c_int64()
+        O.orc_im_gd_text(h, C.byref(t), C.byref(n))
+        text = C.string_at(t, n.value).decode()
+        ps = L.c_f64p()
+        O.orc_im_position_scores(h, C.byref(ps), C.byref(n))
+        scores = np.ctypeslib.as_array(ps, shape=(n.value,)).copy() if n.value else np.zeros(0)
+    finally:
+        O.orc_im_free(h)
+    return text, scores
+
+
+def read_log10_prob_table(text, n_bins):
+    out = np.zeros(n_bins, dtype=np.float64)
+    rc = lib().orc_read_log10_prob_table(text.encode(), out.ctypes.data_as(L.c_f64p), n_bins)
+    assert rc == 0
+    return out
+
+
+def sequence_trims(seq):
+    b = seq.encode() if isinstance(seq, str) else bytes(seq)
+    out = np.zeros(2 * len(b), dtype=np.uint8)
+    lib().orc_sequence_trims(b, len(b), out.ctypes.data_as(L.c_u8p))
+    return out[: len(b)], out[len(b):]
+
+
+def fisher(a, b, c, d):
+    return lib().orc_fisher_exact_2x2(a, b, c, d)
+
+
+def ks_less(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().orc_ks_test_less(x.ctypes.data_as(L.c_f64p), x.size, y.ctypes.data_as(L.c_f64p), y.size)
+
+
+def pileup_column(aln_ptr, tid, pos0, cap=4096):
+    out = np.zeros(4 * cap, dtype=np.int32)
+    n = lib().orc_pileup_column(aln_ptr, tid, pos0, out.ctypes.data_as(L.c_i32p), cap)
+    assert n >= 0
+    return out[: 4 * min(n, cap)].reshape(-1, 4)

@@ -1,0 +1,182 @@
+#!/usr/bin/env python
+"""Benchmark of the MI355X pileup-scoring path (breseq error_count + identify_mutations).
+
+One step = one pass of the whole path over one sample already resident in HBM:
+  error_count (coverage histogram + covariate counts) -> error_rates.tab round trip ->
+  identify_mutations (RA / MC / UN scoring) -> ra_mc_evidence.gd text.
+Metric: aligned bases/s (pileup entries: M/=/X bases + D positions), plus p50/p99 step latency.
+Workload: 4.6 Mb reference (GC 50.8%), 100x of 2x150 bp paired reads, insert N(350,50),
+qualities N(34,5), 500 SNPs + 50 indels of 1-5 bp, consensus mode.  Synthetic, seeded.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+C2 = dict(seed=2, ref_length=4_600_000, gc=0.508, read_length=150, coverage=100.0, paired=True, insert_mean=350,
+          insert_sd=50, qual_mean=34, qual_sd=5, qual_min=2, qual_max=41, n_snps=500, n_indels=50, indel_max=5,
+          redundant_fraction=0.01, mapq=42)
+
+# Per-unit algorithmic bytes of the column kernels: 1 B base + 1 B quality per aligned base,
+# 24 B of per-read metadata (position, CIGAR, flags, tags), and per reference column 1 B of
+# reference plus the column kernel's output (identify: 1 B flags + 4 B unique + 4 B total).
+BYTES_PER_BASE = 2
+BYTES_PER_READ = 24
+BYTES_PER_COLUMN_EC = 1
+BYTES_PER_COLUMN_IM = 1 + 9
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return 0.0
+    k = (len(xs) - 1) * q / 100.0
+    f = int(k)
+    c = min(f + 1, len(xs) - 1)
+    return xs[f] + (xs[c] - xs[f]) * (k - f)
+
+
+def run_step(B, sess, mode):
+    ec = sess.error_count()
+    prop = [5.0] * sess.sample.n_targets
+    gd = sess.identify_mutations(log10_prob=ec.rates_table, read_set_max=ec.read_set_max,
+                                 quality_max=ec.quality_max, propagation_cutoffs=prop, mode=mode)
+    return ec, gd
+
+
+def cpu_baseline(B, cfg, target_seconds):
+    """The oracle (single-threaded C++ restatement of the reference path) on a bounded sample of
+    the same workload: same generator settings on a shorter reference."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from breseq_amd import api
+    length = 300_000
+    small = dict(cfg)
+    small["ref_length"] = length
+    small["n_snps"] = max(1, int(cfg["n_snps"] * length / cfg["ref_length"]))
+    small["n_indels"] = max(1, int(cfg["n_indels"] * length / cfg["ref_length"]))
+    s = B.Sample.simulate(**small)
+    nab = s.aligned_bases()
+    t0 = time.perf_counter()
+    args = api.make_error_count_args(s)
+    hists, counts, log10p, texts = O.error_count(args)
+    table = O.read_log10_prob_table(texts["error_rates"], len(log10p))
+    iargs = api.make_identify_args(s, table, args.read_set_max, args.quality_max, [5.0] * s.n_targets)
+    O.identify_mutations(iargs)
+    dt = time.perf_counter() - t0
+    return {"value": nab / dt, "unit": "aligned bases/s", "cores": 1, "kind": "port",
+            "sample": "%d bp reference, %.0fx 2x%d bp paired reads, same generator and mode (%d aligned bases, "
+                      "%.1f s single-threaded)" % (length, cfg["coverage"], cfg["read_length"], nab, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ref-length", type=int, default=C2["ref_length"])
+    ap.add_argument("--coverage", type=float, default=C2["coverage"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (the per-sample work "
+              "shards by genome window; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import breseq_amd as B
+
+    cfg = dict(C2)
+    cfg["ref_length"] = args.ref_length
+    cfg["coverage"] = args.coverage
+    log("[bench] simulating %d bp at %.0fx" % (cfg["ref_length"], cfg["coverage"]))
+    t = time.time()
+    sample = B.Sample.simulate(**cfg)
+    log("[bench] simulated in %.1f s" % (time.time() - t))
+    sess = B.Session().load(sample)
+    tm = sess.timing()
+    nab, n_reads, n_cols = tm["aligned_bases"], tm["n_reads"], tm["n_positions"]
+    log("[bench] loaded: %d aligned bases, %d reads, %d columns; pack %.0f ms, H2D %.0f ms"
+        % (nab, n_reads, n_cols, tm["pack_ms"], tm["h2d_ms"]))
+    for i in range(args.warmup):
+        run_step(B, sess, B.CONSENSUS)
+        log("[bench] warmup step %d done" % i)
+    step_ms = []
+    ec_k, im_k = [], []
+    detail = None
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        ec = sess.error_count()
+        ect = sess.timing()
+        prop = [5.0] * sample.n_targets
+        gd = sess.identify_mutations(log10_prob=ec.rates_table, read_set_max=ec.read_set_max,
+                                     quality_max=ec.quality_max, propagation_cutoffs=prop, mode=B.CONSENSUS)
+        t1 = time.perf_counter()
+        imt = sess.timing()
+        step_ms.append((t1 - t0) * 1000.0)
+        ec_k.append(ect["kernel_detail_ms"][0])
+        im_k.append(imt["kernel_detail_ms"])
+        detail = imt
+    total_s = sum(step_ms) / 1000.0
+    ms_per_step = sum(step_ms) / len(step_ms)
+    value = nab * len(step_ms) / total_s
+    n_ra = gd.count("\nRA\t")
+    # dominant kernel and its roofline
+    avg = lambda xs: sum(xs) / len(xs)
+    k_ec = avg(ec_k)
+    k_col = avg([d[1] for d in im_k])
+    k_ins = avg([d[2] for d in im_k])
+    k_score = avg([d[3] for d in im_k])
+    k_runs = avg([d[4] for d in im_k])
+    kernels = {"k_error_count": k_ec, "k_identify_columns": k_col, "k_insert_slots": k_ins,
+               "k_score_candidates": k_score, "k_runs+k_mc_detail": k_runs}
+    bytes_ec = BYTES_PER_BASE * nab + BYTES_PER_READ * n_reads + BYTES_PER_COLUMN_EC * n_cols
+    bytes_im = BYTES_PER_BASE * nab + BYTES_PER_READ * n_reads + BYTES_PER_COLUMN_IM * n_cols
+    col_kernels = {"k_error_count": (k_ec, bytes_ec), "k_identify_columns": (k_col, bytes_im)}
+    dom = max(col_kernels, key=lambda k: col_kernels[k][0])
+    dms, dbytes = col_kernels[dom]
+    achieved = dbytes / (dms / 1000.0) / 1e9
+    out = {
+        "metric": "aligned bases/sec through pileup scoring, 4.6Mb@100x; p50/p99 sample latency",
+        "value": value,
+        "unit": "aligned bases/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "p50_ms": percentile(step_ms, 50),
+        "p99_ms": percentile(step_ms, 99),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8 bases/quals, int32 counts, f64 likelihoods",
+        "data": "synthetic (seeded generator: random reference, planted variants, reads aligned by construction)",
+        "config": {"workload": "consensus-mode pileup scoring of one sample: %d bp reference at %.0fx, 2x150 bp "
+                               "paired reads, insert N(350,50), quals N(34,5), 500 SNPs + 50 indels"
+                               % (cfg["ref_length"], cfg["coverage"]),
+                   "aligned_bases": nab, "reads": n_reads, "ra_entries": n_ra,
+                   "candidates": detail["candidates"]},
+        "kernels_ms": kernels,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "algorithmic_bytes": dbytes},
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle, 1 core)")
+        out["cpu_baseline"] = cpu_baseline(B, cfg, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

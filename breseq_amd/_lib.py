@@ -142,6 +142,9 @@ class Timing(C.Structure):
         ("host_post_ms", C.c_double),
         ("aligned_bases", C.c_int64),
         ("candidates", C.c_int64),
+        ("n_reads", C.c_int64),
+        ("n_positions", C.c_int64),
+        ("kernel_detail_ms", C.c_double * 8),
     ]
 
 
@@ -219,6 +222,8 @@ def lib():
     _decl(L, "bsq_ec_result_histogram", C.c_int, [vp, C.c_int32, P(c_u32p), c_i64p])
     _decl(L, "bsq_ec_result_counts", C.c_int, [vp, P(C.POINTER(C.c_uint64)), c_i64p])
     _decl(L, "bsq_ec_result_log10_prob", C.c_int, [vp, P(c_f64p), c_i64p])
+    _decl(L, "bsq_ec_result_rates_table", C.c_int, [vp, P(c_f64p), c_i64p])
+    _decl(L, "bsq_ec_result_rates_text", C.c_int, [vp, P(C.c_char_p), c_i64p])
     _decl(L, "bsq_ec_result_free", None, [vp])
     _decl(L, "bsq_im_result_gd_text", C.c_int, [vp, P(C.c_char_p), c_i64p])
     _decl(L, "bsq_im_result_counts", C.c_int, [vp, c_i64p, c_i64p, c_i64p])

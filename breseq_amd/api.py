@@ -156,6 +156,8 @@ class ErrorCountResult:
     log10_prob: np.ndarray
     read_set_max: int
     quality_max: int
+    rates_table: np.ndarray = None    # the table identify_mutations reads back from error_rates.tab
+    rates_text: str = ""
     texts: dict = field(default_factory=dict)
 
 
@@ -178,7 +180,9 @@ class Session:
     def timing(self):
         t = L.Timing()
         L.lib().bsq_session_timing(self._h, C.byref(t))
-        return {k: getattr(t, k) for k, _ in L.Timing._fields_}
+        d = {k: getattr(t, k) for k, _ in L.Timing._fields_}
+        d["kernel_detail_ms"] = list(t.kernel_detail_ms)
+        return d
 
     def close(self):
         if self._h:
@@ -256,9 +260,14 @@ def error_count(sample, session=None, **kw):
         pd = L.c_f64p()
         lib.bsq_ec_result_log10_prob(res, C.byref(pd), C.byref(n))
         log10p = _np(pd, n.value, np.float64).copy()
+        lib.bsq_ec_result_rates_table(res, C.byref(pd), C.byref(n))
+        rates = _np(pd, n.value, np.float64).copy()
+        t = C.c_char_p()
+        lib.bsq_ec_result_rates_text(res, C.byref(t), C.byref(n))
+        text = C.string_at(t, n.value).decode()
     finally:
         lib.bsq_ec_result_free(res)
-    return ErrorCountResult(hists, counts, log10p, args.read_set_max, args.quality_max)
+    return ErrorCountResult(hists, counts, log10p, args.read_set_max, args.quality_max, rates, text)
 
 
 def mode_settings(mode):

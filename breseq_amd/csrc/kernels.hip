@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(256) k_identify_columns(ImParams P) {
     if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
     if (r1 > 0.0) fl |= kPosRedundant;
     if (insert_slot) fl |= kPosInsert;
-    if (!isnan(cbs) && (double)total <= P.targets[t].seed_cutoff) fl |= kPosSeed;
+    if ((double)total <= P.targets[t].seed_cutoff) fl |= kPosSeed;   // unique[1] is never NaN here
     const bool cons = (best != ref_idx) && !isnan(cbs) && (cbs > 0.0);
     if (cons) fl |= kPosConsensus;
     bool cand = cons || P.screen_off ||

@@ -59,6 +59,7 @@ struct Session {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evk[8] = {};
   bsq_timing timing{};
   bool loaded = false;
   // packed metadata (host)
@@ -98,6 +99,8 @@ struct Session {
   ~Session() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t e : evk)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -289,6 +292,8 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   S.timing.pack_ms = t1 - t0;
   S.timing.h2d_ms = now_ms() - t2;
   S.timing.aligned_bases = S.aligned_bases;
+  S.timing.n_reads = S.n_dreads;
+  S.timing.n_positions = S.n_pos;
   S.loaded = true;
 }
 
@@ -335,6 +340,8 @@ struct bsq_ec_result {
   std::vector<std::vector<uint32_t>> hist;
   std::vector<uint64_t> counts;
   std::vector<double> log10p;
+  std::string rates_text;
+  std::vector<double> rates_table;
 };
 
 struct bsq_im_result {
@@ -402,6 +409,8 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   float ms = 0;
   HIP_OK(hipEventElapsedTime(&ms, S.ev0, S.ev1));
   S.timing.kernel_ms = ms;
+  for (double& d : S.timing.kernel_detail_ms) d = 0.0;
+  S.timing.kernel_detail_ms[0] = ms;
   check_err(S);
   const double t0 = now_ms();
   std::vector<unsigned long long> counts(n_bins), hist((size_t)G * hist_len);
@@ -421,6 +430,11 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
     for (int64_t c = 0; c <= last; c++) out.hist[g][c] = (uint32_t)hist[(size_t)g * hist_len + c];
   }
   counts_to_log10_prob(Sm, Q, out.counts, out.log10p);
+  out.rates_text = error_rates_text(Sm, Q, out.log10p);
+  {
+    int s2 = 0, q2 = 0;
+    if (!parse_error_rates(out.rates_text, out.rates_table, s2, q2)) throw BsqError(BSQ_ERR_FORMAT, "error rate table round trip failed");
+  }
   if (args->output_dir) {
     const std::string dir = args->output_dir;
     if (args->do_coverage) {
@@ -434,7 +448,7 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
       const std::string rates = args->error_rates_file ? std::string(args->error_rates_file) : dir + "/error_rates.tab";
       std::ofstream f(rates);
       if (!f) throw BsqError(BSQ_ERR_IO, "could not write " + rates);
-      f << error_rates_text(Sm, Q, out.log10p);
+      f << out.rates_text;
       std::vector<std::string> bq = base_qual_texts(Sm, Q, out.counts);
       for (int r = 0; r < Sm; r++) {
         std::string name = (r < args->read_files.n_files && args->read_files.names) ? args->read_files.names[r] : std::to_string(r);
@@ -566,6 +580,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   HIP_OK(hipEventRecord(S.ev0, S.stream));
   hipLaunchKernelGGL(k_identify_columns, dim3(grid_for(S.n_windows)), dim3(256), 0, S.stream, P);
   HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(S.evk[0], S.stream));
   int32_t counters[4];
   HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
   HIP_OK(hipStreamSynchronize(S.stream));
@@ -578,6 +593,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
     HIP_OK(hipStreamSynchronize(S.stream));
     check_err(S);
   }
+  HIP_OK(hipEventRecord(S.evk[1], S.stream));
   const int n_cand = std::min(counters[0], cand_cap);
   S.d_ra.alloc(std::max(1, n_cand));
   P.ra_out = S.d_ra.p;
@@ -588,6 +604,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
     hipLaunchKernelGGL(k_score_candidates, dim3(grid), dim3(64), lds, S.stream, P, 0, n_cand);
     HIP_OK(hipGetLastError());
   }
+  HIP_OK(hipEventRecord(S.evk[2], S.stream));
   // MC / UN
   RunParams R;
   R.pos_target = S.d_pos_target.p;
@@ -642,6 +659,18 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   HIP_OK(hipEventElapsedTime(&ms, S.ev0, S.ev1));
   S.timing.kernel_ms = ms;
   S.timing.candidates = n_cand;
+  for (double& d : S.timing.kernel_detail_ms) d = 0.0;
+  {
+    float a = 0, b = 0, c = 0, d = 0;
+    HIP_OK(hipEventElapsedTime(&a, S.ev0, S.evk[0]));
+    HIP_OK(hipEventElapsedTime(&b, S.evk[0], S.evk[1]));
+    HIP_OK(hipEventElapsedTime(&c, S.evk[1], S.evk[2]));
+    HIP_OK(hipEventElapsedTime(&d, S.evk[2], S.ev1));
+    S.timing.kernel_detail_ms[1] = a;
+    S.timing.kernel_detail_ms[2] = b;
+    S.timing.kernel_detail_ms[3] = c;
+    S.timing.kernel_detail_ms[4] = d;
+  }
   check_err(S);
   const double t0 = now_ms();
   std::vector<RaOut> ra(n_cand);
@@ -778,6 +807,7 @@ int bsq_session_create(int32_t device, bsq_session** out, bsq_status* status) {
   HIP_OK(hipStreamCreateWithFlags(&s->s.stream, hipStreamNonBlocking));
   HIP_OK(hipEventCreate(&s->s.ev0));
   HIP_OK(hipEventCreate(&s->s.ev1));
+  for (hipEvent_t& e : s->s.evk) HIP_OK(hipEventCreate(&e));
   *out = s.release();
   set_status(status, BSQ_OK, "");
   return BSQ_OK;
@@ -884,6 +914,20 @@ int bsq_ec_result_log10_prob(const bsq_ec_result* r, const double** t, int64_t* 
   if (!r) return BSQ_ERR_ARG;
   *t = r->log10p.data();
   *n = (int64_t)r->log10p.size();
+  return BSQ_OK;
+}
+
+int bsq_ec_result_rates_table(const bsq_ec_result* r, const double** t, int64_t* n) {
+  if (!r) return BSQ_ERR_ARG;
+  *t = r->rates_table.data();
+  *n = (int64_t)r->rates_table.size();
+  return BSQ_OK;
+}
+
+int bsq_ec_result_rates_text(const bsq_ec_result* r, const char** text, int64_t* len) {
+  if (!r) return BSQ_ERR_ARG;
+  *text = r->rates_text.c_str();
+  *len = (int64_t)r->rates_text.size();
   return BSQ_OK;
 }
 

@@ -111,6 +111,11 @@ int bsq_ec_result_histogram(const bsq_ec_result* r, int32_t group, const uint32_
 int bsq_ec_result_counts(const bsq_ec_result* r, const uint64_t** counts, int64_t* n_bins);
 /* log10 probability table after counts_to_log10_prob (error_count.cpp:1005-1026). */
 int bsq_ec_result_log10_prob(const bsq_ec_result* r, const double** table, int64_t* n_bins);
+/* The same table as identify_mutations sees it: printed to error_rates.tab with 6 significant
+ * digits and parsed back with strtod (error_count.cpp:660-690 then 629-654). */
+int bsq_ec_result_rates_table(const bsq_ec_result* r, const double** table, int64_t* n_bins);
+/* Text of error_rates.tab. */
+int bsq_ec_result_rates_text(const bsq_ec_result* r, const char** text, int64_t* len);
 void bsq_ec_result_free(bsq_ec_result* r);
 
 /* ------------------------------------------------------------------------------------------ */
@@ -217,6 +222,12 @@ typedef struct bsq_timing {
   double host_post_ms;     /* host formatting of results                          */
   int64_t aligned_bases;   /* pileup entries (M/=/X bases + D positions)          */
   int64_t candidates;      /* positions sent to the exact FP64 scorer             */
+  int64_t n_reads;         /* mapped alignments in the device CSR                 */
+  int64_t n_positions;     /* reference columns (incl. one pad per target)        */
+  /* per-kernel times of the last pass, HIP events on the session stream:
+   * [0] k_error_count  [1] k_identify_columns  [2] k_insert_slots
+   * [3] k_score_candidates  [4] k_runs + k_mc_detail */
+  double kernel_detail_ms[8];
 } bsq_timing;
 
 /* Create a session bound to HIP device `device`.  Fails with BSQ_ERR_DEVICE when no GPU. */

@@ -42,6 +42,8 @@ def test_error_count_bit_exact(name):
         assert np.array_equal(res.histograms[g], h_o[g]), "histogram of group %d differs" % g
     assert np.array_equal(res.counts, c_o.astype(np.uint64))
     assert np.array_equal(res.log10_prob, l_o)
+    assert res.rates_text == texts["error_rates"]
+    assert np.array_equal(res.rates_table, O.read_log10_prob_table(texts["error_rates"], len(l_o)))
     assert res.counts.sum() > 0
 
 

@@ -140,6 +140,7 @@ class Timing(C.Structure):
         ("kernel_ms", C.c_double),
         ("d2h_ms", C.c_double),
         ("host_post_ms", C.c_double),
+        ("host_pre_ms", C.c_double),
         ("aligned_bases", C.c_int64),
         ("candidates", C.c_int64),
         ("n_reads", C.c_int64),

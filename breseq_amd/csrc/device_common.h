@@ -23,6 +23,19 @@ enum : uint32_t {
   kErrCapacity = 64      // a fixed device buffer was too small (host retries or reports)
 };
 
+// Alignment records are read-only for a kernel's lifetime; reading them through the constant
+// address space lets the compiler use scalar (SMEM) loads when the index is wave-uniform.
+typedef __attribute__((address_space(4))) const uint64_t ConstU64;
+__device__ __forceinline__ DRead load_read(const DRead* reads, int r) {
+  const ConstU64* c = (const ConstU64*)(reads + r);
+  uint64_t w[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++) w[i] = c[i];
+  DRead d;
+  __builtin_memcpy(&d, w, sizeof(DRead));
+  return d;
+}
+
 __device__ __forceinline__ bool is_ref_op(uint32_t op) {
   return op == OP_M || op == OP_D || op == OP_N || op == OP_EQ || op == OP_X;
 }

@@ -16,6 +16,7 @@ struct EcParams {
   const char* ref;
   const int32_t* pos_target;    // target of each global column, -1 for pads / unscored
   const int32_t* target_group;  // coverage group of each target
+  const int32_t* target_scored; // 1 if the target is in the call set
   const int32_t* win_rb;        // alignments overlapping window w: [win_rb[w], win_re[w])
   const int32_t* win_re;
   int64_t n_windows;
@@ -44,6 +45,10 @@ struct ImParams {
   int32_t base_quality_cutoff;
   const double* tab_lp;         // per key: log10 P(obs | true base b), b = 0..4
   const double* tab_r;          // per key: 10^(lp - lpmax) for b = 0..4, then lpmax
+  const double* fold_lp;        // strand-folded rows (mslot, read file, quality, read-strand obs) x 5
+  const float* fold_max;        // per folded row: max_b log10 P
+  int32_t fold_rows;
+  int32_t fold_lds;             // 1: stage the folded table in LDS
   double log10_ref_length;
   double mutation_cutoff;
   double polymorphism_cutoff;
@@ -67,6 +72,14 @@ struct ImParams {
   uint32_t* err;
 };
 
+// MC / UN interval found by k_runs (kind 0 MC, 1 UN); columns are global.
+struct RunRec {
+  int32_t kind, s, e;
+  int32_t seed;              // MC: seed cutoff reached in the run or at its closing column
+  int32_t red_start, red_end;   // MC: redundant sub-range columns, -1 when unset
+  int32_t lo, li, ri, ro;    // MC: unique coverage left-outside, left-inside, right-inside, right-outside
+};
+
 struct RunParams {
   const int32_t* pos_target;
   const TargetInfo* targets;
@@ -74,45 +87,47 @@ struct RunParams {
   const int32_t* col_unique;
   int32_t n_pos;
   int32_t do_mc;
-  int2* events;
-  int32_t* n_events;
-  int32_t event_cap;
-  const int2* runs;
-  int4* run_out;
+  RunRec* runs;
+  int32_t* n_runs;
+  int32_t run_cap;
+  uint32_t* err;
 };
 
-// Screening sums of one column (FP32 is enough: the screen compares an upper bound with a margin).
-struct ColumnAcc {
+// Screening sums of one column (FP32: the screen compares an upper bound against a margin).
+struct Screen {
   uint32_t n;
   uint32_t cnt[5];
-  float SL;        // Sum_i max_b log10 P_i(b)
-  float S[25];     // S[o*5+b] = Sum over reads observing o of log10 P_i(b)
+  float SL;     // Sum_i max_b log10 P_i(b)
+  float D[5];   // D[o] = Sum over reads observing o of log10 P_i(o)
+  float E[5];   // E[o] = Sum over reads observing o of log10 P_i(reference base)
   __device__ void reset() {
     n = 0;
     SL = 0.0f;
-    for (int i = 0; i < 5; i++) cnt[i] = 0;
-    for (int i = 0; i < 25; i++) S[i] = 0.0f;
+    for (int i = 0; i < 5; i++) {
+      cnt[i] = 0;
+      D[i] = 0.0f;
+      E[i] = 0.0f;
+    }
   }
-  __device__ void add(int obs, bool top, const double l[5], float lpmax) {
-    (void)top;
+  __device__ void add(int obs, const double l[5], float lpmax, int ref) {
     n++;
     SL += lpmax;
+    const float lr = (float)(ref == 0 ? l[0] : ref == 1 ? l[1] : ref == 2 ? l[2] : ref == 3 ? l[3] : l[4]);
 #pragma unroll
     for (int o = 0; o < 5; o++) {
-      if (o == obs) {
-        cnt[o]++;
-#pragma unroll
-        for (int b = 0; b < 5; b++) S[o * 5 + b] += (float)l[b];
-      }
+      const bool m = (o == obs);
+      cnt[o] += m ? 1u : 0u;
+      D[o] += m ? (float)l[o] : 0.0f;
+      E[o] += m ? lr : 0.0f;
     }
   }
 };
 
+
 __global__ void k_error_count(EcParams P);
 __global__ void k_identify_columns(ImParams P);
 __global__ void k_insert_slots(ImParams P);
-__global__ void k_score_candidates(ImParams P, int cand_begin, int cand_end);
+__global__ void k_score_candidates(ImParams P);
 __global__ void k_runs(RunParams P);
-__global__ void k_mc_detail(RunParams P, int n_runs);
 
 }  // namespace bsq

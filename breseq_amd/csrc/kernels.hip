@@ -9,7 +9,7 @@
 //  k_insert_slots     insert slots >= 1 of columns whose unique reads carry insertions
 //  k_score_candidates wave per candidate: allele-frequency EM, presence score, profile-likelihood
 //                     bounds, Fisher strand and KS quality tests (identify_mutations.cpp:3009-3318)
-//  k_runs / k_mc_detail  MC and UN interval detection (identify_mutations.cpp:2262-2344, 2972-3007)
+//  k_runs             MC and UN interval detection (identify_mutations.cpp:2262-2344, 2972-3007)
 //
 // Column kernels map one 64-wide wavefront onto 64 consecutive reference columns and walk the
 // alignments overlapping that window in file order: every lane sees the same alignment in the
@@ -49,12 +49,17 @@ __global__ void __launch_bounds__(256) k_error_count(EcParams P) {
 
   for (int64_t w = w_begin + wave; w < w_end; w += waves_per_block) {
     const int p = (int)(w * WAVE + lane);
-    const bool lane_ok = (p < P.n_pos) && (P.pos_target[p] >= 0);
+    const int tgt = (p < P.n_pos) ? P.pos_target[p] : -1;
+    const bool lane_ok = (tgt >= 0) && P.target_scored[tgt];
     const int rb = uniform_i(P.win_rb[w]), re = uniform_i(P.win_re[w]);
     int uniq = 0;
     bool has_red = false;
+    DRead Rn;
+    if (rb < re) Rn = load_read(P.reads, rb);
     for (int r = rb; r < re; r++) {
-      const DRead R = P.reads[r];
+      const DRead R = Rn;
+      const int rn = uniform_i(r + 1);   // keeps the record loads on the scalar unit
+      if (rn < re) Rn = load_read(P.reads, rn);     // prefetch the next alignment's record
       int qpos, indel;
       bool is_del;
       if (!lane_ok || !entry_at(R, P.cigar, p, qpos, indel, is_del, err)) continue;
@@ -138,8 +143,7 @@ __global__ void __launch_bounds__(256) k_error_count(EcParams P) {
       }
     }
     if (lane_ok && !has_red) {
-      const int t = P.pos_target[p];
-      const int g = P.target_group[t];
+      const int g = P.target_group[tgt];
       if (g < P.n_groups_lds && uniq < P.hist_lds) atomicAdd(&hist[g * P.hist_lds + uniq], 1u);
       else if (uniq < P.hist_len) atomicAdd(&P.hist[(int64_t)g * P.hist_len + uniq], 1ull);
       else err |= kErrCapacity;
@@ -179,38 +183,86 @@ __device__ __forceinline__ void pure_genotype(const double lps[5], uint32_t n, i
   score = lps[best] - lt;
 }
 
-// Conservative screen: true unless no allele can reach the presence-score cutoff.  The full-model
-// log likelihood is at most Sum_i max_b log10 P_i(b); the null model's EM starts from the
-// Laplace-smoothed counts and never decreases its likelihood, so its reported value is at least
-// the likelihood at that start, bounded below per read by its observed (or best other) allele.
-__device__ __forceinline__ bool screen_passes(const ColumnAcc& A, int ref_idx, double log10_ref_len, double cutoff,
+// Conservative screen: true unless no allele can reach the presence-score cutoff.
+//   * The full model's reported log likelihood is at most Sum_i max_b log10 P_i(b) (= SL).
+//   * The null model (allele v held out) starts EM from the Laplace-smoothed counts and EM never
+//     lowers the likelihood, so its reported value is at least the likelihood at that start, which
+//     per read is at least log10 f0(c) + log10 P_i(c) for any held-in allele c: the read's own
+//     observed allele (sums D[b]), or the reference allele for reads that observe v (sums E[v]).
+// Accumulated in FP32; the margin covers the rounding.  Columns whose reference base is N go to
+// the exact scorer whenever they have reads.
+__device__ __forceinline__ bool screen_passes(const Screen& A, int ref_idx, double log10_ref_len, double cutoff,
                                               double margin) {
   if (A.n == 0) return false;
+  if (ref_idx < 0 || ref_idx > 4) return true;
+  const float n = (float)A.n;
+#pragma unroll
   for (int v = 0; v < 5; v++) {
     if (v == ref_idx) continue;
-    const float init_total = 2.0f + (float)(A.n - A.cnt[v]);
-    float lb = 0.0f;
+    const float init_total = 2.0f + (n - (float)A.cnt[v]);
+    const float inv = 1.0f / init_total;
+    float lb = 0.0f, lf_ref = 0.0f;
+#pragma unroll
     for (int b = 0; b < 5; b++) {
+      const float lf = log10f((0.5f + (float)A.cnt[b]) * inv);
+      if (b == ref_idx) lf_ref = lf;
       if (b == v) continue;
-      const float f0 = (0.5f + (float)A.cnt[b]) / init_total;
-      lb += (float)A.cnt[b] * log10f(f0) + A.S[b * 5 + b];
+      lb += (float)A.cnt[b] * lf + A.D[b];
     }
-    if (A.cnt[v] > 0) {
-      float best = -3.0e38f;
-      for (int c = 0; c < 5; c++) {
-        if (c == v) continue;
-        const float f0 = (0.5f + (float)A.cnt[c]) / init_total;
-        best = fmaxf(best, (float)A.cnt[v] * log10f(f0) + A.S[v * 5 + c]);
-      }
-      lb += best;
-    }
+    if (A.cnt[v] > 0) lb += (float)A.cnt[v] * lf_ref + A.E[v];
     const double ub = (double)A.SL - (double)lb - log10_ref_len;
     if (ub >= cutoff - margin) return true;
   }
   return false;
 }
 
+// log10 P(observed base | true base b) for b = 0..4 and its maximum, from the strand-folded table:
+// rows are indexed by the read-strand observation, columns by the read-strand hypothesis.
+__device__ __forceinline__ void lookup_lp(const double* __restrict__ tab, const float* __restrict__ tmax, int S, int Q,
+                                          const DRead& R, int q, int obs, bool top, double l[5], float& lpmax) {
+  const int o_rs = top ? obs : (int)comp_index((uint8_t)obs);
+  const int row = (((int)R.mslot * S + R.rfile) * Q + q) * 5 + o_rs;
+  const double* T = tab + (size_t)row * 5;
+  if (top) {
+    l[0] = T[0];
+    l[1] = T[1];
+    l[2] = T[2];
+    l[3] = T[3];
+    l[4] = T[4];
+  } else {
+    l[0] = T[3];
+    l[1] = T[2];
+    l[2] = T[1];
+    l[3] = T[0];
+    l[4] = T[4];
+  }
+  lpmax = tmax[row];
+}
+
+// Stage the folded likelihood table in LDS when it fits; returns the pointers to use.
+__device__ __forceinline__ void stage_table(const ImParams& P, double* smem, const double*& tab, const float*& tmax) {
+  tab = P.fold_lp;
+  tmax = P.fold_max;
+  if (P.fold_lds) {
+    const int n_lp = P.fold_rows * 5;
+    float* smax = (float*)(smem + n_lp);
+    for (int i = threadIdx.x; i < n_lp; i += blockDim.x) smem[i] = P.fold_lp[i];
+    for (int i = threadIdx.x; i < P.fold_rows; i += blockDim.x) smax[i] = P.fold_max[i];
+    __syncthreads();
+    tab = smem;
+    tmax = smax;
+  }
+}
+
+__device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, int ref_idx, const double lps[5],
+                                           double red_bot, double red_top, int uniq_bot, int uniq_top,
+                                           bool insert_slot, const Screen& A, uint32_t& err);
+
 __global__ void __launch_bounds__(256) k_identify_columns(ImParams P) {
+  extern __shared__ double smem[];
+  const double* tab;
+  const float* tmax;
+  stage_table(P, smem, tab, tmax);
   const int tid = threadIdx.x;
   const int wave = tid / WAVE, lane = tid % WAVE;
   const int waves_per_block = blockDim.x / WAVE;
@@ -224,54 +276,77 @@ __global__ void __launch_bounds__(256) k_identify_columns(ImParams P) {
     const int p = (int)(w * WAVE + lane);
     const int t = (p < P.n_pos) ? P.pos_target[p] : -1;
     const bool lane_ok = (t >= 0) && P.targets[t].scored && (P.targets[t].prop_cutoff >= 0.0);
+    const int ref_idx = lane_ok ? char_index(P.ref[p]) : 0;
+    const int scr_ref = (ref_idx >= 0 && ref_idx < 5) ? ref_idx : 0;
     const int rb = uniform_i(P.win_rb[w]), re = uniform_i(P.win_re[w]);
     double lps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double red_bot = 0.0, red_top = 0.0;
     int uniq_bot = 0, uniq_top = 0;
     bool insert_slot = false;
-    ColumnAcc A;
+    Screen A;
     A.reset();
+    DRead R;
+    if (rb < re) R = load_read(P.reads, rb);
     for (int r = rb; r < re; r++) {
-      const DRead R = P.reads[r];
+      DRead Rn;
+      const int rn = uniform_i(r + 1);   // keeps the record loads on the scalar unit
+      if (rn < re) Rn = load_read(P.reads, rn);     // prefetch the next alignment's record
       int qpos, indel;
       bool is_del;
-      if (!lane_ok || !entry_at(R, P.cigar, p, qpos, indel, is_del, err)) continue;
-      const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
-      uint8_t read_base = '.';
-      bool past = true;
-      if (ondel >= 0) {
-        read_base = P.bases[R.seq_off + qpos];
-        past = false;
+      if (lane_ok && entry_at(R, P.cigar, p, qpos, indel, is_del, err)) {
+        const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
+        uint8_t base = '.', qv = 0;
+        if (ondel >= 0) {
+          base = P.bases[R.seq_off + qpos];
+          qv = P.quals[R.seq_off + qpos];
+        }
+        if (base != 15) {
+          const bool top = !(R.flags & kReadReversed);
+          const bool trimmed = is_trimmed(R, qpos, ondel < 0);
+          if (R.x1 == 1) {
+            if (top) ++uniq_top;
+            else ++uniq_bot;
+            if (ondel > 0) insert_slot = true;
+          } else {
+            if (top) red_top += 1.0 / (double)R.x1;
+            else red_bot += 1.0 / (double)R.x1;
+          }
+          if (R.x1 <= 1 && !trimmed) {
+            int obs = -1, q = qv;
+            bool use = true;
+            if (ondel < 0) {   // deletion: quality of the next read base (error_count.cpp:1075-1080)
+              const int qq = qpos + 1 - (top ? 0 : 1);
+              if (qq < 0 || qq >= (int)R.l_qseq) {
+                err |= kErrQueryIndex;
+                use = false;
+              } else if (P.bases[R.seq_off + qq] == 15) {
+                use = false;
+              } else {
+                obs = 4;
+                q = P.quals[R.seq_off + qq];
+              }
+            } else {
+              obs = bam_index(base);
+              if (obs < 0) {
+                err |= kErrBaseCode;
+                use = false;
+              }
+            }
+            if (use && q >= P.base_quality_cutoff) {
+              if ((int)R.rfile >= S) err |= kErrReadSet;
+              else if (q >= Q) err |= kErrQuality;
+              else {
+                double l[5];
+                float lpmax;
+                lookup_lp(tab, tmax, S, Q, R, q, obs, top, l, lpmax);
+                for (int b = 0; b < 5; b++) lps[b] += l[b];
+                A.add(obs, l, lpmax, scr_ref);
+              }
+            }
+          }
+        }
       }
-      if (read_base == 15) continue;
-      const bool top = !(R.flags & kReadReversed);
-      const bool trimmed = is_trimmed(R, qpos, past);
-      if (R.x1 == 1) {
-        if (top) ++uniq_top;
-        else ++uniq_bot;
-        if (ondel > 0) insert_slot = true;
-      } else {
-        if (top) red_top += 1.0 / (double)R.x1;
-        else red_bot += 1.0 / (double)R.x1;
-      }
-      if (R.x1 > 1 || trimmed) continue;
-      int obs, q;
-      if (!slot_observation(R, P.bases, P.quals, qpos, ondel, 0, read_base, obs, q, err)) continue;
-      if (q < P.base_quality_cutoff) continue;
-      if ((int)R.rfile >= S) {
-        err |= kErrReadSet;
-        continue;
-      }
-      if (q >= Q) {
-        err |= kErrQuality;
-        continue;
-      }
-      const uint32_t key = ((((uint32_t)R.mslot * S + R.rfile) * Q + q) * 10u) + obs * 2u + (top ? 0u : 1u);
-      const double* lp = P.tab_lp + (size_t)key * 5;
-      double l[5];
-      for (int b = 0; b < 5; b++) l[b] = lp[b];
-      for (int b = 0; b < 5; b++) lps[b] += l[b];
-      A.add(obs, top, l, (float)P.tab_r[(size_t)key * 6 + 5]);
+      R = Rn;
     }
     if (!lane_ok) {
       if (p < P.n_pos) {
@@ -279,8 +354,18 @@ __global__ void __launch_bounds__(256) k_identify_columns(ImParams P) {
         P.col_unique[p] = 0;
         P.col_total[p] = 0;
       }
-      continue;
+    } else {
+      finish_column(P, p, t, ref_idx, lps, red_bot, red_top, uniq_bot, uniq_top, insert_slot, A, err);
     }
+  }
+  if (err) atomicOr(P.err, err);
+}
+
+// Per-column tail of slot 0: coverage sums, pure-genotype call, screen, outputs.
+__device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, int ref_idx, const double lps[5],
+                                           double red_bot, double red_top, int uniq_bot, int uniq_top,
+                                           bool insert_slot, const Screen& A, uint32_t& err) {
+  {
     // position_coverage::sum (identify_mutations.h:115-120)
     const double u1 = (double)uniq_bot + (double)uniq_top;
     const double r1 = red_bot + red_top;
@@ -289,8 +374,6 @@ __global__ void __launch_bounds__(256) k_identify_columns(ImParams P) {
     double score;
     pure_genotype(lps, A.n, best, score);
     const double cbs = score - P.log10_ref_length;
-    const char refc = P.ref[p];
-    const int ref_idx = char_index(refc);
     if (ref_idx < 0) err |= kErrRefChar;
     uint8_t fl = kPosValid;
     if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
@@ -299,9 +382,8 @@ __global__ void __launch_bounds__(256) k_identify_columns(ImParams P) {
     if ((double)total <= P.targets[t].seed_cutoff) fl |= kPosSeed;   // unique[1] is never NaN here
     const bool cons = (best != ref_idx) && !isnan(cbs) && (cbs > 0.0);
     if (cons) fl |= kPosConsensus;
-    bool cand = cons || P.screen_off ||
+    bool cand = cons || (P.screen_off && A.n > 0) ||
                 screen_passes(A, ref_idx, P.log10_ref_length, P.polymorphism_cutoff, P.screen_margin);
-    if (P.screen_off && A.n == 0) cand = false;
     if (cand) {
       fl |= kPosCandidate;
       const int slot = atomicAdd(P.n_cand, 1);
@@ -321,75 +403,74 @@ __global__ void __launch_bounds__(256) k_identify_columns(ImParams P) {
     P.col_unique[p] = uniq_bot + uniq_top;
     P.col_total[p] = total;
   }
-  if (err) atomicOr(P.err, err);
 }
 
 /* ------------------------------------------------------------------------------------------ */
 /* Insert slots >= 1: one thread per column carrying an insertion                              */
 
 __global__ void __launch_bounds__(64) k_insert_slots(ImParams P) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= min(*P.n_ins, P.ins_cap)) return;
-  const int p = P.ins[i];
-  const int64_t w = p / WAVE;
-  const int rb = P.win_rb[w], re = P.win_re[w];
+  const int n_ins = min(*P.n_ins, P.ins_cap);
   const int S = P.read_set_max, Q = P.quality_max;
   uint32_t err = 0;
-  bool next = true;
-  for (int slot = 1; next && slot < 65536; slot++) {
-    next = false;
-    double lps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    ColumnAcc A;
-    A.reset();
-    for (int r = rb; r < re; r++) {
-      const DRead R = P.reads[r];
-      int qpos, indel;
-      bool is_del;
-      if (!entry_at(R, P.cigar, p, qpos, indel, is_del, err)) continue;
-      const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
-      uint8_t read_base = '.';
-      bool past = true;
-      if (ondel >= slot) {
-        read_base = P.bases[R.seq_off + qpos + slot];
-        past = false;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_ins; i += gridDim.x * blockDim.x) {
+    const int p = P.ins[i];
+    const int64_t w = p / WAVE;
+    const int rb = P.win_rb[w], re = P.win_re[w];
+    bool next = true;
+    for (int slot = 1; next && slot < 65536; slot++) {
+      next = false;
+      double lps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      Screen A;
+      A.reset();
+      for (int r = rb; r < re; r++) {
+        const DRead R = P.reads[r];
+        int qpos, indel;
+        bool is_del;
+        if (!entry_at(R, P.cigar, p, qpos, indel, is_del, err)) continue;
+        const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
+        uint8_t read_base = '.';
+        bool past = true;
+        if (ondel >= slot) {
+          read_base = P.bases[R.seq_off + qpos + slot];
+          past = false;
+        }
+        if (read_base == 15) continue;
+        const bool top = !(R.flags & kReadReversed);
+        const bool trimmed = is_trimmed(R, qpos, past);
+        if (R.x1 == 1 && ondel > slot) next = true;
+        if (R.x1 > 1 || trimmed) continue;
+        int obs, q;
+        if (!slot_observation(R, P.bases, P.quals, qpos, ondel, slot, read_base, obs, q, err)) continue;
+        if (q < P.base_quality_cutoff) continue;
+        if ((int)R.rfile >= S) {
+          err |= kErrReadSet;
+          continue;
+        }
+        if (q >= Q) {
+          err |= kErrQuality;
+          continue;
+        }
+        double l[5];
+        float lpmax;
+        lookup_lp(P.fold_lp, P.fold_max, S, Q, R, q, obs, top, l, lpmax);
+        for (int b = 0; b < 5; b++) lps[b] += l[b];
+        A.add(obs, l, lpmax, 4);
       }
-      if (read_base == 15) continue;
-      const bool top = !(R.flags & kReadReversed);
-      const bool trimmed = is_trimmed(R, qpos, past);
-      if (R.x1 == 1 && ondel > slot) next = true;
-      if (R.x1 != 1 || trimmed) continue;
-      int obs, q;
-      if (!slot_observation(R, P.bases, P.quals, qpos, ondel, slot, read_base, obs, q, err)) continue;
-      if (q < P.base_quality_cutoff) continue;
-      if ((int)R.rfile >= S) {
-        err |= kErrReadSet;
-        continue;
-      }
-      if (q >= Q) {
-        err |= kErrQuality;
-        continue;
-      }
-      const uint32_t key = ((((uint32_t)R.mslot * S + R.rfile) * Q + q) * 10u) + obs * 2u + (top ? 0u : 1u);
-      const double* lp = P.tab_lp + (size_t)key * 5;
-      double l[5];
-      for (int b = 0; b < 5; b++) l[b] = lp[b];
-      for (int b = 0; b < 5; b++) lps[b] += l[b];
-      A.add(obs, top, l, (float)P.tab_r[(size_t)key * 6 + 5]);
-    }
-    int best;
-    double score;
-    pure_genotype(lps, A.n, best, score);
-    const double cbs = score - P.log10_ref_length;
-    const bool cons = (best != 4) && !isnan(cbs) && (cbs > 0.0);   // reference base is '.'
-    bool cand = cons || (P.screen_off && A.n > 0) ||
-                screen_passes(A, 4, P.log10_ref_length, P.polymorphism_cutoff, P.screen_margin);
-    if (cand) {
-      const int c = atomicAdd(P.n_cand, 1);
-      if (c < P.cand_cap) {
-        P.cand[c] = make_int2(p, slot);
-        P.cand_cons[c] = cons ? 1 : 0;
-      } else {
-        err |= kErrCapacity;
+      int best;
+      double score;
+      pure_genotype(lps, A.n, best, score);
+      const double cbs = score - P.log10_ref_length;
+      const bool cons = (best != 4) && !isnan(cbs) && (cbs > 0.0);   // reference base is '.'
+      const bool cand = cons || (P.screen_off && A.n > 0) ||
+                        screen_passes(A, 4, P.log10_ref_length, P.polymorphism_cutoff, P.screen_margin);
+      if (cand) {
+        const int c = atomicAdd(P.n_cand, 1);
+        if (c < P.cand_cap) {
+          P.cand[c] = make_int2(p, slot);
+          P.cand_cons[c] = cons ? 1 : 0;
+        } else {
+          err |= kErrCapacity;
+        }
       }
     }
   }
@@ -612,7 +693,7 @@ __device__ double ks_less_hist(const uint16_t* hx, const uint16_t* hy, uint32_t 
   return exp(-2.0 * stat * stat * neff);
 }
 
-__global__ void __launch_bounds__(64) k_score_candidates(ImParams P, int cand_begin, int cand_end) {
+__global__ void __launch_bounds__(64) k_score_candidates(ImParams P) {
   extern __shared__ uint32_t keys[];   // P.key_cap entries
   __shared__ WaveScratch W;
   __shared__ uint32_t cov[6][3];
@@ -621,7 +702,8 @@ __global__ void __launch_bounds__(64) k_score_candidates(ImParams P, int cand_be
   __shared__ double ks_row[128];
   const int lane = threadIdx.x;
   const int S = P.read_set_max, Q = P.quality_max;
-  for (int ci = cand_begin + blockIdx.x; ci < cand_end; ci += gridDim.x) {
+  const int n_cand = min(*P.n_cand, P.cand_cap);
+  for (int ci = blockIdx.x; ci < n_cand; ci += gridDim.x) {
     const int2 cd = P.cand[ci];
     const int p = cd.x, slot = cd.y;
     const int64_t w = p / WAVE;
@@ -807,8 +889,12 @@ __global__ void __launch_bounds__(64) k_score_candidates(ImParams P, int cand_be
 /* ------------------------------------------------------------------------------------------ */
 /* MC / UN runs                                                                                */
 
-// Marks run starts and ends of the two interval types; appends (position, kind) events.
-// kind: 0 MC start, 1 MC end, 2 UN start, 3 UN end.
+// One thread per column; the thread at the first column of a run walks it to its end, so every
+// run is found and described by exactly one thread:
+//   MC: maximal runs with unique coverage <= the propagation cutoff; the seed is reached if any
+//       column of the run, or the column closing it, has total coverage <= the seed cutoff
+//       (identify_mutations.cpp:2270-2281); the redundant sub-range follows :2326-2343.
+//   UN: maximal runs of columns without a predicted base (:2972-3007).
 __global__ void __launch_bounds__(256) k_runs(RunParams P) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P.n_pos) return;
@@ -819,50 +905,59 @@ __global__ void __launch_bounds__(256) k_runs(RunParams P) {
   const int first = T.off, last = T.off + T.len - 1;
   auto in_mc = [&](int q) -> bool { return (double)P.col_unique[q] <= T.prop_cutoff; };
   auto in_un = [&](int q) -> bool { return !(P.col_flags[q] & kPosPredicted); };
-  if (P.do_mc) {
-    const bool a = in_mc(p);
-    if (a && (p == first || !in_mc(p - 1))) {
-      const int k = atomicAdd(P.n_events, 1);
-      if (k < P.event_cap) P.events[k] = make_int2(p, 0);
+  if (P.do_mc && in_mc(p) && (p == first || !in_mc(p - 1))) {
+    bool seed = false, reached_zero = false;
+    int red_start = -1, red_end = -1;
+    int q = p;
+    for (; q <= last && in_mc(q); q++) {
+      const uint8_t fl = P.col_flags[q];
+      if (fl & kPosSeed) seed = true;
+      if (!(fl & kPosRedundant)) {
+        reached_zero = true;
+        red_end = -1;
+      } else if (!reached_zero) {
+        red_start = q;
+      } else if (red_end < 0) {
+        red_end = q;
+      }
     }
-    if (a && (p == last || !in_mc(p + 1))) {
-      const int k = atomicAdd(P.n_events, 1);
-      if (k < P.event_cap) P.events[k] = make_int2(p, 1);
-    }
-  }
-  const bool u = in_un(p);
-  if (u && (p == first || !in_un(p - 1))) {
-    const int k = atomicAdd(P.n_events, 1);
-    if (k < P.event_cap) P.events[k] = make_int2(p, 2);
-  }
-  if (u && (p == last || !in_un(p + 1))) {
-    const int k = atomicAdd(P.n_events, 1);
-    if (k < P.event_cap) P.events[k] = make_int2(p, 3);
-  }
-}
-
-// Per MC run [s, e]: seed reached in the run, redundant sub-range (identify_mutations.cpp:2280,
-// 2326-2343).  One thread per run, sequential over the run's columns.
-__global__ void __launch_bounds__(64) k_mc_detail(RunParams P, int n_runs) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_runs) return;
-  const int s = P.runs[i].x, e = P.runs[i].y;
-  bool seed = false, reached_zero = false;
-  int red_start = -1, red_end = -1;
-  for (int p = s; p <= e; p++) {
-    const uint8_t fl = P.col_flags[p];
-    if (fl & kPosSeed) seed = true;
-    if (!(fl & kPosRedundant)) {
-      reached_zero = true;
-      red_end = -1;
+    const int e = q - 1;
+    if (e < last && (P.col_flags[e + 1] & kPosSeed)) seed = true;
+    const int k = atomicAdd(P.n_runs, 1);
+    if (k < P.run_cap) {
+      RunRec R;
+      R.kind = 0;
+      R.s = p;
+      R.e = e;
+      R.seed = seed ? 1 : 0;
+      R.red_start = red_start;
+      R.red_end = red_end;
+      R.lo = (p > first) ? P.col_unique[p - 1] : -1;
+      R.li = P.col_unique[p];
+      R.ri = P.col_unique[e];
+      R.ro = (e < last) ? P.col_unique[e + 1] : -1;
+      P.runs[k] = R;
     } else {
-      if (!reached_zero) red_start = p;
-      else if (red_end < 0) red_end = p;
+      atomicOr(P.err, kErrCapacity);
     }
   }
-  // the closing column (e + 1) is checked against the seed before the run closes
-  if (e + 1 < P.n_pos && P.pos_target[e + 1] == P.pos_target[s] && (P.col_flags[e + 1] & kPosSeed)) seed = true;
-  P.run_out[i] = make_int4(seed ? 1 : 0, red_start, red_end, 0);
+  if (in_un(p) && (p == first || !in_un(p - 1))) {
+    int q = p;
+    while (q <= last && in_un(q)) q++;
+    const int k = atomicAdd(P.n_runs, 1);
+    if (k < P.run_cap) {
+      RunRec R;
+      R.kind = 1;
+      R.s = p;
+      R.e = q - 1;
+      R.seed = 0;
+      R.red_start = R.red_end = -1;
+      R.lo = R.li = R.ri = R.ro = 0;
+      P.runs[k] = R;
+    } else {
+      atomicOr(P.err, kErrCapacity);
+    }
+  }
 }
 
 }  // namespace bsq

@@ -81,7 +81,10 @@ struct Session {
   DBuf<uint32_t> d_cigar;
   DBuf<uint8_t> d_bases, d_quals;
   DBuf<char> d_ref;
-  DBuf<int32_t> d_pos_target, d_win_rb, d_win_re, d_target_group;
+  DBuf<int32_t> d_pos_target, d_win_rb, d_win_re, d_target_group, d_target_scored;
+  DBuf<double> d_fold_lp;
+  DBuf<float> d_fold_max;
+  DBuf<RunRec> d_runrec;
   DBuf<TargetInfo> d_targets;
   DBuf<unsigned long long> d_counts, d_hist;
   DBuf<uint32_t> d_err;
@@ -90,7 +93,7 @@ struct Session {
   DBuf<int32_t> d_col_unique, d_col_total;
   DBuf<int2> d_cand;
   DBuf<uint8_t> d_cand_cons;
-  DBuf<int32_t> d_counters;   // [0] n_cand, [1] n_ins, [2] n_events
+  DBuf<int32_t> d_counters;   // [0] n_cand, [1] n_ins, [2] n_runs, [3] error bits
   DBuf<int32_t> d_ins;
   DBuf<RaOut> d_ra;
   DBuf<int2> d_events, d_runs;
@@ -305,10 +308,7 @@ static std::vector<int32_t> target_order(const Session& S, const uint8_t* call) 
   return t;
 }
 
-static void check_err(Session& S) {
-  uint32_t err = 0;
-  HIP_OK(hipMemcpyAsync(&err, S.d_err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, S.stream));
-  HIP_OK(hipStreamSynchronize(S.stream));
+static void raise_err_bits(uint32_t err) {
   if (!err) return;
   std::string m = "alignment data violates a breseq invariant:";
   if (err & 1) m += " read base not A/C/G/T/N;";
@@ -319,6 +319,13 @@ static void check_err(Session& S) {
   if (err & 32) m += " CIGAR does not cover its alignment span;";
   if (err & 64) m += " device buffer capacity exceeded;";
   throw BsqError(BSQ_ERR_DATA, m);
+}
+
+static void check_err(Session& S) {
+  uint32_t err = 0;
+  HIP_OK(hipMemcpyAsync(&err, S.d_err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, S.stream));
+  HIP_OK(hipStreamSynchronize(S.stream));
+  raise_err_bits(err);
 }
 
 static int grid_for(int64_t n_windows) {
@@ -357,21 +364,19 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   if (Sm <= 0 || Q <= 0 || Sm > 255 || Q > 256) throw BsqError(BSQ_ERR_ARG, "covariate maxima out of range");
   const int n_bins = Sm * 25 * Q;
   const int G = std::max(1, args->n_coverage_groups);
-  // scored columns: targets of the call set only
-  std::vector<int32_t> pos_target(S.h_pos_target);
-  std::vector<int32_t> group(S.n_targets, 0);
+  // scored targets: the call set only
+  std::vector<int32_t> group(S.n_targets, 0), scored(S.n_targets, 0);
   for (int t = 0; t < S.n_targets; t++) {
     const bool call = !args->call_target || args->call_target[t];
     int g = args->coverage_group ? args->coverage_group[t] : 0;
     if (call && (g < 0 || g >= G)) throw BsqError(BSQ_ERR_ARG, "coverage group out of range");
     group[t] = std::max(0, std::min(g, G - 1));
-    if (!call) std::fill(pos_target.begin() + S.tgt_off[t], pos_target.begin() + S.tgt_off[t] + S.lengths[t], -1);
+    scored[t] = call ? 1 : 0;
   }
-  DBuf<int32_t> d_pt;
-  d_pt.alloc(pos_target.size());
-  HIP_OK(hipMemcpyAsync(d_pt.p, pos_target.data(), pos_target.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
   S.d_target_group.alloc(group.size());
+  S.d_target_scored.alloc(scored.size());
   HIP_OK(hipMemcpyAsync(S.d_target_group.p, group.data(), group.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
+  HIP_OK(hipMemcpyAsync(S.d_target_scored.p, scored.data(), scored.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
   const int hist_len = S.max_window_reads + 2;
   S.d_counts.alloc(n_bins);
   S.d_hist.alloc((size_t)G * hist_len);
@@ -384,8 +389,9 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   P.bases = S.d_bases.p;
   P.quals = S.d_quals.p;
   P.ref = S.d_ref.p;
-  P.pos_target = d_pt.p;
+  P.pos_target = S.d_pos_target.p;
   P.target_group = S.d_target_group.p;
+  P.target_scored = S.d_target_scored.p;
   P.win_rb = S.d_win_rb.p;
   P.win_re = S.d_win_re.p;
   P.n_windows = S.n_windows;
@@ -464,13 +470,19 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
 // fill_read_base_likelihoods (identify_mutations.cpp:3359-3384) for every (MAPQ, read file,
 // quality, observed base, strand): the per-read cache depends on nothing else, so it is computed
 // once per key on the host with the same libm calls and looked up by the kernels.
+//   lp / rr: keyed by (mslot, read file, quality, top-strand obs, strand) -- the exact scorer
+//   fold / fmax: keyed by (mslot, read file, quality, read-strand obs), hypotheses in read-strand
+//                order; a bottom-strand read reads the row of the complemented observation with
+//                its hypotheses complemented (the reference complements both before the lookup).
 static void build_likelihood_tables(const Session& S, const bsq_identify_args* a, std::vector<double>& lp,
-                                    std::vector<double>& rr) {
+                                    std::vector<double>& rr, std::vector<double>& fold, std::vector<float>& fmax) {
   const int Sm = a->read_set_max, Q = a->quality_max;
   const size_t nm = S.mapq_values.size();
   const size_t nkeys = nm * Sm * Q * 10;
   lp.assign(nkeys * 5, 0.0);
   rr.assign(nkeys * 6, 0.0);
+  fold.assign(nm * Sm * Q * 5 * 5, 0.0);
+  fmax.assign(nm * Sm * Q * 5, 0.0f);
   std::vector<double> prob(a->n_bins);
   for (int64_t i = 0; i < a->n_bins; i++) prob[i] = pow(10, a->log10_prob_table[i]);
   auto comp = [](int i) { return i < 4 ? 3 - i : 4; };
@@ -480,7 +492,7 @@ static void build_likelihood_tables(const Session& S, const bsq_identify_args* a
     const double uniform = 1.0 / 5.0;
     for (int rs = 0; rs < Sm; rs++)
       for (int q = 0; q < Q; q++)
-        for (int obs = 0; obs < 5; obs++)
+        for (int obs = 0; obs < 5; obs++) {
           for (int sb = 0; sb < 2; sb++) {
             const size_t key = (((m * Sm + rs) * Q + q) * 10) + obs * 2 + sb;
             const bool top = (sb == 0);
@@ -499,7 +511,13 @@ static void build_likelihood_tables(const Session& S, const bsq_identify_args* a
               rr[key * 6 + b] = pow(10, l[b] - mx);
             }
             rr[key * 6 + 5] = mx;
+            if (top) {   // read-strand row == top-strand row for a top-strand read
+              const size_t row = ((m * Sm + rs) * Q + q) * 5 + obs;
+              for (int b = 0; b < 5; b++) fold[row * 5 + b] = l[b];
+              fmax[row] = (float)mx;
+            }
           }
+        }
   }
 }
 
@@ -509,6 +527,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   if (Sm <= 0 || Q <= 0 || Sm > 255 || Q > 256) throw BsqError(BSQ_ERR_ARG, "covariate maxima out of range");
   if (a->n_bins != (int64_t)Sm * 25 * Q) throw BsqError(BSQ_ERR_ARG, "error table size does not match its covariates");
   if (!a->deletion_propagation_cutoff || !a->deletion_seed_cutoff) throw BsqError(BSQ_ERR_ARG, "missing per-target cutoffs");
+  const double th0 = now_ms();
   // targets
   std::vector<TargetInfo> T(S.n_targets);
   for (int t = 0; t < S.n_targets; t++) {
@@ -523,22 +542,30 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   }
   S.d_targets.alloc(T.size());
   HIP_OK(hipMemcpyAsync(S.d_targets.p, T.data(), T.size() * sizeof(TargetInfo), hipMemcpyHostToDevice, S.stream));
-  std::vector<double> lp, rr;
-  build_likelihood_tables(S, a, lp, rr);
+  std::vector<double> lp, rr, fold;
+  std::vector<float> fmax;
+  build_likelihood_tables(S, a, lp, rr, fold, fmax);
   S.d_tab_lp.alloc(lp.size());
   S.d_tab_r.alloc(rr.size());
+  S.d_fold_lp.alloc(fold.size());
+  S.d_fold_max.alloc(fmax.size());
   HIP_OK(hipMemcpyAsync(S.d_tab_lp.p, lp.data(), lp.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
   HIP_OK(hipMemcpyAsync(S.d_tab_r.p, rr.data(), rr.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
+  HIP_OK(hipMemcpyAsync(S.d_fold_lp.p, fold.data(), fold.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
+  HIP_OK(hipMemcpyAsync(S.d_fold_max.p, fmax.data(), fmax.size() * sizeof(float), hipMemcpyHostToDevice, S.stream));
   double total_len = 0;
   for (int t = 0; t < S.n_targets; t++) total_len += (double)S.lengths[t];
   S.d_col_flags.alloc(S.n_pos);
   S.d_col_unique.alloc(S.n_pos);
   S.d_col_total.alloc(S.n_pos);
-  int cand_cap = screen_off ? S.n_pos + 1024 : std::max<int>(1 << 16, S.n_pos / 64);
-  int ins_cap = std::max<int>(1 << 14, S.n_pos / 256);
+  const int cand_cap = screen_off ? S.n_pos + 1024 : std::max<int>(1 << 16, S.n_pos / 64);
+  const int ins_cap = std::max<int>(1 << 14, S.n_pos / 256);
+  const int run_cap = std::max<int>(1 << 16, S.n_pos / 16);
   S.d_cand.alloc(cand_cap);
   S.d_cand_cons.alloc(cand_cap);
   S.d_ins.alloc(ins_cap);
+  S.d_ra.alloc(cand_cap);
+  S.d_runrec.alloc(run_cap);
   ImParams P;
   P.reads = S.d_reads.p;
   P.cigar = S.d_cigar.p;
@@ -556,11 +583,16 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   P.base_quality_cutoff = a->base_quality_cutoff;
   P.tab_lp = S.d_tab_lp.p;
   P.tab_r = S.d_tab_r.p;
+  P.fold_lp = S.d_fold_lp.p;
+  P.fold_max = S.d_fold_max.p;
+  P.fold_rows = (int32_t)fmax.size();
+  const size_t fold_bytes = fold.size() * sizeof(double) + fmax.size() * sizeof(float);
+  P.fold_lds = (fold_bytes <= 40 * 1024) ? 1 : 0;
   P.log10_ref_length = log10(total_len);
   P.mutation_cutoff = a->mutation_cutoff;
   P.polymorphism_cutoff = a->polymorphism_cutoff;
   P.precision_decimal = a->polymorphism_precision_decimal;
-  P.screen_margin = 0.05;
+  P.screen_margin = 0.1;
   P.screen_off = screen_off;
   P.col_flags = S.d_col_flags.p;
   P.col_unique = S.d_col_unique.p;
@@ -573,39 +605,10 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   P.n_ins = S.d_counters.p + 1;
   P.ins_cap = ins_cap;
   P.key_cap = std::max(64, S.max_window_reads);
-  P.ra_out = nullptr;
-  P.err = S.d_err.p;
-  HIP_OK(hipMemsetAsync(S.d_counters.p, 0, 4 * sizeof(int32_t), S.stream));
-  HIP_OK(hipMemsetAsync(S.d_err.p, 0, sizeof(uint32_t), S.stream));
-  HIP_OK(hipEventRecord(S.ev0, S.stream));
-  hipLaunchKernelGGL(k_identify_columns, dim3(grid_for(S.n_windows)), dim3(256), 0, S.stream, P);
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(S.evk[0], S.stream));
-  int32_t counters[4];
-  HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
-  HIP_OK(hipStreamSynchronize(S.stream));
-  check_err(S);
-  if (counters[1] > 0) {
-    const int n_ins = std::min(counters[1], ins_cap);
-    hipLaunchKernelGGL(k_insert_slots, dim3((n_ins + 63) / 64), dim3(64), 0, S.stream, P);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
-    HIP_OK(hipStreamSynchronize(S.stream));
-    check_err(S);
-  }
-  HIP_OK(hipEventRecord(S.evk[1], S.stream));
-  const int n_cand = std::min(counters[0], cand_cap);
-  S.d_ra.alloc(std::max(1, n_cand));
   P.ra_out = S.d_ra.p;
-  if (n_cand > 0) {
-    const size_t lds = (size_t)P.key_cap * sizeof(uint32_t);
-    if (lds > 48 * 1024) throw BsqError(BSQ_ERR_LIMIT, "column depth too large for the exact scorer's LDS list");
-    const int grid = std::min(n_cand, 8192);
-    hipLaunchKernelGGL(k_score_candidates, dim3(grid), dim3(64), lds, S.stream, P, 0, n_cand);
-    HIP_OK(hipGetLastError());
-  }
-  HIP_OK(hipEventRecord(S.evk[2], S.stream));
-  // MC / UN
+  P.err = (uint32_t*)(S.d_counters.p + 3);
+  const size_t key_lds = (size_t)P.key_cap * sizeof(uint32_t);
+  if (key_lds > 48 * 1024) throw BsqError(BSQ_ERR_LIMIT, "column depth too large for the exact scorer's LDS list");
   RunParams R;
   R.pos_target = S.d_pos_target.p;
   R.targets = S.d_targets.p;
@@ -613,75 +616,58 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   R.col_unique = S.d_col_unique.p;
   R.n_pos = S.n_pos;
   R.do_mc = a->skip_missing_coverage_prediction ? 0 : 1;
-  const int event_cap = std::max(1 << 16, S.n_pos / 8);
-  S.d_events.alloc(event_cap);
-  R.events = S.d_events.p;
-  R.n_events = S.d_counters.p + 2;
-  R.event_cap = event_cap;
-  R.runs = nullptr;
-  R.run_out = nullptr;
+  R.runs = S.d_runrec.p;
+  R.n_runs = S.d_counters.p + 2;
+  R.run_cap = run_cap;
+  R.err = P.err;
+  HIP_OK(hipMemsetAsync(S.d_counters.p, 0, 4 * sizeof(int32_t), S.stream));
+  const double th1 = now_ms();
+  // the whole pass is queued without a host round trip: later kernels read the counts on device
+  HIP_OK(hipEventRecord(S.ev0, S.stream));
+  hipLaunchKernelGGL(k_identify_columns, dim3(grid_for(S.n_windows)), dim3(256), P.fold_lds ? fold_bytes : 0, S.stream, P);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(S.evk[0], S.stream));
+  hipLaunchKernelGGL(k_insert_slots, dim3(64), dim3(64), 0, S.stream, P);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(S.evk[1], S.stream));
+  hipLaunchKernelGGL(k_score_candidates, dim3(4096), dim3(64), key_lds, S.stream, P);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(S.evk[2], S.stream));
   hipLaunchKernelGGL(k_runs, dim3((S.n_pos + 255) / 256), dim3(256), 0, S.stream, R);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
-  HIP_OK(hipStreamSynchronize(S.stream));
-  const int n_events = counters[2];
-  if (n_events > event_cap) throw BsqError(BSQ_ERR_LIMIT, "too many MC/UN interval boundaries");
-  std::vector<int2> events(n_events);
-  if (n_events) HIP_OK(hipMemcpy(events.data(), S.d_events.p, n_events * sizeof(int2), hipMemcpyDeviceToHost));
-  std::vector<int> starts[2], ends[2];
-  for (const int2& e : events) {
-    const int kind = e.y;
-    if (kind == 0) starts[0].push_back(e.x);
-    else if (kind == 1) ends[0].push_back(e.x);
-    else if (kind == 2) starts[1].push_back(e.x);
-    else ends[1].push_back(e.x);
-  }
-  for (int k = 0; k < 2; k++) {
-    std::sort(starts[k].begin(), starts[k].end());
-    std::sort(ends[k].begin(), ends[k].end());
-    if (starts[k].size() != ends[k].size()) throw BsqError(BSQ_ERR_DATA, "unpaired interval boundaries");
-  }
-  std::vector<int2> mc_runs;
-  for (size_t i = 0; i < starts[0].size(); i++) mc_runs.push_back(make_int2(starts[0][i], ends[0][i]));
-  std::vector<int4> mc_out(mc_runs.size());
-  if (!mc_runs.empty()) {
-    S.d_runs.alloc(mc_runs.size());
-    S.d_run_out.alloc(mc_runs.size());
-    HIP_OK(hipMemcpyAsync(S.d_runs.p, mc_runs.data(), mc_runs.size() * sizeof(int2), hipMemcpyHostToDevice, S.stream));
-    R.runs = S.d_runs.p;
-    R.run_out = S.d_run_out.p;
-    hipLaunchKernelGGL(k_mc_detail, dim3(((int)mc_runs.size() + 63) / 64), dim3(64), 0, S.stream, R, (int)mc_runs.size());
-    HIP_OK(hipGetLastError());
-  }
   HIP_OK(hipEventRecord(S.ev1, S.stream));
+  int32_t counters[4];
+  HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
   HIP_OK(hipStreamSynchronize(S.stream));
   float ms = 0;
   HIP_OK(hipEventElapsedTime(&ms, S.ev0, S.ev1));
   S.timing.kernel_ms = ms;
-  S.timing.candidates = n_cand;
   for (double& d : S.timing.kernel_detail_ms) d = 0.0;
   {
-    float a = 0, b = 0, c = 0, d = 0;
-    HIP_OK(hipEventElapsedTime(&a, S.ev0, S.evk[0]));
-    HIP_OK(hipEventElapsedTime(&b, S.evk[0], S.evk[1]));
-    HIP_OK(hipEventElapsedTime(&c, S.evk[1], S.evk[2]));
-    HIP_OK(hipEventElapsedTime(&d, S.evk[2], S.ev1));
-    S.timing.kernel_detail_ms[1] = a;
-    S.timing.kernel_detail_ms[2] = b;
-    S.timing.kernel_detail_ms[3] = c;
-    S.timing.kernel_detail_ms[4] = d;
+    float k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+    HIP_OK(hipEventElapsedTime(&k0, S.ev0, S.evk[0]));
+    HIP_OK(hipEventElapsedTime(&k1, S.evk[0], S.evk[1]));
+    HIP_OK(hipEventElapsedTime(&k2, S.evk[1], S.evk[2]));
+    HIP_OK(hipEventElapsedTime(&k3, S.evk[2], S.ev1));
+    S.timing.kernel_detail_ms[1] = k0;
+    S.timing.kernel_detail_ms[2] = k1;
+    S.timing.kernel_detail_ms[3] = k2;
+    S.timing.kernel_detail_ms[4] = k3;
   }
-  check_err(S);
+  const uint32_t errbits = (uint32_t)counters[3];
+  if (counters[0] > cand_cap) throw BsqError(BSQ_ERR_LIMIT, "too many candidate columns for the exact scorer");
+  if (counters[1] > ins_cap) throw BsqError(BSQ_ERR_LIMIT, "too many columns with insertions");
+  if (counters[2] > run_cap) throw BsqError(BSQ_ERR_LIMIT, "too many MC/UN intervals");
+  raise_err_bits(errbits);
+  const int n_cand = counters[0];
+  const int n_runs = counters[2];
+  S.timing.candidates = n_cand;
   const double t0 = now_ms();
   std::vector<RaOut> ra(n_cand);
-  if (n_cand) HIP_OK(hipMemcpy(ra.data(), S.d_ra.p, n_cand * sizeof(RaOut), hipMemcpyDeviceToHost));
-  if (!mc_runs.empty()) HIP_OK(hipMemcpy(mc_out.data(), S.d_run_out.p, mc_out.size() * sizeof(int4), hipMemcpyDeviceToHost));
-  // unique coverage at MC boundary columns
-  std::vector<int32_t> col_unique;
-  if (!mc_runs.empty()) {
-    col_unique.resize(S.n_pos);
-    HIP_OK(hipMemcpy(col_unique.data(), S.d_col_unique.p, S.n_pos * sizeof(int32_t), hipMemcpyDeviceToHost));
-  }
+  std::vector<RunRec> runs(n_runs);
+  if (n_cand) HIP_OK(hipMemcpyAsync(ra.data(), S.d_ra.p, n_cand * sizeof(RaOut), hipMemcpyDeviceToHost, S.stream));
+  if (n_runs) HIP_OK(hipMemcpyAsync(runs.data(), S.d_runrec.p, n_runs * sizeof(RunRec), hipMemcpyDeviceToHost, S.stream));
+  HIP_OK(hipStreamSynchronize(S.stream));
   S.timing.d2h_ms = now_ms() - t0;
   const double t1 = now_ms();
   // assemble entries
@@ -701,48 +687,35 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
     entries.push_back(std::move(e));
     out.n_ra++;
   }
-  for (size_t i = 0; i < mc_runs.size(); i++) {
-    if (!mc_out[i].x) continue;   // seed never reached
-    const int s = mc_runs[i].x, e_ = mc_runs[i].y;
-    const int t = S.h_pos_target[s];
-    const int32_t base = S.tgt_off[t];
-    const uint32_t start = (uint32_t)(s - base + 1), end = (uint32_t)(e_ - base + 1);
-    const uint32_t red_start = (mc_out[i].y >= 0) ? (uint32_t)(mc_out[i].y - base + 1) : start;
-    const uint32_t red_end = (mc_out[i].z >= 0) ? (uint32_t)(mc_out[i].z - base + 1) : end;
-    GdEntry e;
-    e.type = 1;
-    e.seq_id = S.names[t];
-    e.pos = start;
-    e.end = end;
-    e.start_range = red_start - start;
-    e.end_range = end - red_end;
-    const double nan = std::numeric_limits<double>::quiet_NaN();
-    const double lo = (start == 1) ? nan : (double)col_unique[s - 1];
-    const double ro = (end == (uint32_t)S.lengths[t]) ? nan : (double)col_unique[e_ + 1];
-    e.fields = {{"left_inside_cov", fmt_double((double)col_unique[s], 0, false)},
-                {"left_outside_cov", fmt_double(lo, 0, false)},
-                {"right_inside_cov", fmt_double((double)col_unique[e_], 0, false)},
-                {"right_outside_cov", fmt_double(ro, 0, false)}};
-    e.order_target = rank[t];
-    e.order_pos = end + 1;
-    e.order_sub = 0;
-    entries.push_back(std::move(e));
-    out.n_mc++;
-  }
-  for (size_t i = 0; i < starts[1].size(); i++) {
-    const int s = starts[1][i], e_ = ends[1][i];
-    const int t = S.h_pos_target[s];
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (const RunRec& rr_ : runs) {
+    const int t = S.h_pos_target[rr_.s];
     const int32_t base = S.tgt_off[t];
     GdEntry e;
-    e.type = 2;
     e.seq_id = S.names[t];
-    e.pos = (uint32_t)(s - base + 1);
-    e.end = (uint32_t)(e_ - base + 1);
+    e.pos = (uint32_t)(rr_.s - base + 1);
+    e.end = (uint32_t)(rr_.e - base + 1);
     e.order_target = rank[t];
     e.order_pos = e.end + 1;
-    e.order_sub = 1;
+    if (rr_.kind == 0) {
+      if (!rr_.seed) continue;   // seed never reached
+      const uint32_t red_start = (rr_.red_start >= 0) ? (uint32_t)(rr_.red_start - base + 1) : e.pos;
+      const uint32_t red_end = (rr_.red_end >= 0) ? (uint32_t)(rr_.red_end - base + 1) : e.end;
+      e.type = 1;
+      e.start_range = red_start - e.pos;
+      e.end_range = e.end - red_end;
+      e.fields = {{"left_inside_cov", fmt_double((double)rr_.li, 0, false)},
+                  {"left_outside_cov", fmt_double(rr_.lo < 0 ? nan : (double)rr_.lo, 0, false)},
+                  {"right_inside_cov", fmt_double((double)rr_.ri, 0, false)},
+                  {"right_outside_cov", fmt_double(rr_.ro < 0 ? nan : (double)rr_.ro, 0, false)}};
+      e.order_sub = 0;
+      out.n_mc++;
+    } else {
+      e.type = 2;
+      e.order_sub = 1;
+      out.n_un++;
+    }
     entries.push_back(std::move(e));
-    out.n_un++;
   }
   if (!a->skip_missing_coverage_prediction) {
     for (int t : order) {
@@ -770,6 +743,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
     f << out.text;
   }
   S.timing.host_post_ms = now_ms() - t1;
+  S.timing.host_pre_ms = th1 - th0;
 }
 
 }  // namespace bsq

@@ -220,6 +220,7 @@ typedef struct bsq_timing {
   double kernel_ms;        /* kernels of the last pass, HIP events on the stream  */
   double d2h_ms;           /* result copies back                                  */
   double host_post_ms;     /* host formatting of results                          */
+  double host_pre_ms;      /* host argument processing and table uploads          */
   int64_t aligned_bases;   /* pileup entries (M/=/X bases + D positions)          */
   int64_t candidates;      /* positions sent to the exact FP64 scorer             */
   int64_t n_reads;         /* mapped alignments in the device CSR                 */

@@ -119,8 +119,11 @@ class Sample:
         return [r.lengths[i] for i in range(r.n_targets)]
 
     def max_quality(self):
-        q = self.arrays()["qual"]
-        return int(q.max()) if q.size else 0
+        """Largest base quality in the sample (Summary max_qual); computed once."""
+        if getattr(self, "_max_qual", None) is None:
+            q = self.arrays()["qual"]
+            self._max_qual = int(q.max()) if q.size else 0
+        return self._max_qual
 
     def aligned_bases(self):
         a = self.arrays()

@@ -30,56 +30,30 @@ __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfi
 /* ------------------------------------------------------------------------------------------ */
 /* Pass 1: error_count                                                                         */
 
-__global__ void __launch_bounds__(256) k_error_count(EcParams P) {
-  extern __shared__ uint32_t lds[];
-  uint32_t* table = lds;                         // P.n_bins counters
-  uint32_t* hist = lds + P.n_bins;               // P.n_groups_lds * P.hist_lds bins
-  const int tid = threadIdx.x;
-  const int nlds = P.n_bins + P.n_groups_lds * P.hist_lds;
-  for (int i = tid; i < nlds; i += blockDim.x) lds[i] = 0;
-  __syncthreads();
+constexpr int kBatch = 4;
 
-  const int wave = tid / WAVE, lane = tid % WAVE;
-  const int waves_per_block = blockDim.x / WAVE;
-  const int64_t chunk = (P.n_windows + gridDim.x - 1) / gridDim.x;
-  const int64_t w_begin = (int64_t)blockIdx.x * chunk;
-  const int64_t w_end = min<int64_t>(w_begin + chunk, P.n_windows);
-  uint32_t err = 0;
-  const int S = P.read_set_max, Q = P.quality_max;
-
-  for (int64_t w = w_begin + wave; w < w_end; w += waves_per_block) {
-    const int p = (int)(w * WAVE + lane);
-    const int tgt = (p < P.n_pos) ? P.pos_target[p] : -1;
-    const bool lane_ok = (tgt >= 0) && P.target_scored[tgt];
-    const int rb = uniform_i(P.win_rb[w]), re = uniform_i(P.win_re[w]);
-    int uniq = 0;
-    bool has_red = false;
-    DRead Rn;
-    if (rb < re) Rn = load_read(P.reads, rb);
-    for (int r = rb; r < re; r++) {
-      const DRead R = Rn;
-      const int rn = uniform_i(r + 1);   // keeps the record loads on the scalar unit
-      if (rn < re) Rn = load_read(P.reads, rn);     // prefetch the next alignment's record
-      int qpos, indel;
-      bool is_del;
-      if (!lane_ok || !entry_at(R, P.cigar, p, qpos, indel, is_del, err)) continue;
-      if (is_del) continue;
+// One column entry of error_count_pileup::pileup_callback (error_count.cpp:137-175) with
+// cErrorTable::count_alignment_position (:854-986).  b0/q0: base and quality at qpos.
+__device__ __forceinline__ void count_entry(const EcParams& P, const DRead& R, int p, int qpos, int indel, bool is_del,
+                                            uint8_t b0, uint8_t q0, uint32_t* table, int S, int Q, int& uniq,
+                                            bool& has_red, uint32_t& err) {
+      if (is_del) return;
       if (R.x1 > 1) {
         has_red = true;
-        continue;
+        return;
       }
       ++uniq;
       const int rev = (R.flags & kReadReversed) ? 1 : 0;
       const int rs = R.rfile;
       if (rs >= S) {
         err |= kErrReadSet;
-        continue;
+        return;
       }
       const uint8_t* sq = P.bases + R.seq_off;
       const uint8_t* ql = P.quals + R.seq_off;
       // (1) base substitution / match, read-strand space (error_count.cpp:877-903)
       {
-        uint8_t ob = sq[qpos];
+        uint8_t ob = b0;
         char rc = P.ref[p];
         if (ob != 15 && rc != 'N') {
           if (rev) {
@@ -87,7 +61,7 @@ __global__ void __launch_bounds__(256) k_error_count(EcParams P) {
             rc = comp_char(rc);
           }
           const int oi = bam_index(ob), ri = char_index(rc);
-          const int q = ql[qpos];
+          const int q = q0;
           if (oi < 0 || oi > 3) err |= kErrBaseCode;
           else if (ri < 0 || ri > 3) err |= kErrRefChar;
           else if (q >= Q) err |= kErrQuality;
@@ -140,6 +114,63 @@ __global__ void __launch_bounds__(256) k_error_count(EcParams P) {
             else atomicAdd(&table[rs + S * 4 + 5 * S * oi + 25 * S * q], 1u);
           }
         }
+      }
+    }
+
+__global__ void __launch_bounds__(256) k_error_count(EcParams P) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* table = lds;                         // P.n_bins counters
+  uint32_t* hist = lds + P.n_bins;               // P.n_groups_lds * P.hist_lds bins
+  const int tid = threadIdx.x;
+  const int nlds = P.n_bins + P.n_groups_lds * P.hist_lds;
+  for (int i = tid; i < nlds; i += blockDim.x) lds[i] = 0;
+  __syncthreads();
+
+  const int wave = tid / WAVE, lane = tid % WAVE;
+  const int waves_per_block = blockDim.x / WAVE;
+  const int64_t chunk = (P.n_windows + gridDim.x - 1) / gridDim.x;
+  const int64_t w_begin = (int64_t)blockIdx.x * chunk;
+  const int64_t w_end = min<int64_t>(w_begin + chunk, P.n_windows);
+  uint32_t err = 0;
+  const int S = P.read_set_max, Q = P.quality_max;
+
+  for (int64_t w = w_begin + wave; w < w_end; w += waves_per_block) {
+    const int p = (int)(w * WAVE + lane);
+    const int tgt = (p < P.n_pos) ? P.pos_target[p] : -1;
+    const bool lane_ok = (tgt >= 0) && P.target_scored[tgt];
+    const int rb = uniform_i(P.win_rb[w]), re = uniform_i(P.win_re[w]);
+    int uniq = 0;
+    bool has_red = false;
+    // Alignments are processed in batches of kBatch: the records (scalar loads) and the column
+    // entries of the whole batch are resolved first, so the byte loads of all of them are in
+    // flight together; the counts are then updated in file order.
+    for (int r0 = rb; r0 < re; r0 += kBatch) {
+      DRead Rb[kBatch];
+      int qp[kBatch], ind[kBatch];
+      bool dl[kBatch], cv[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; k++) {
+        const int r = uniform_i(r0 + k);
+        cv[k] = false;
+        if (r < re) {
+          Rb[k] = load_read(P.reads, r);
+          cv[k] = lane_ok && entry_at(Rb[k], P.cigar, p, qp[k], ind[k], dl[k], err);
+        }
+      }
+      uint8_t b0[kBatch], q0[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; k++) {
+        b0[k] = 15;
+        q0[k] = 0;
+        if (cv[k] && !dl[k]) {
+          b0[k] = P.bases[Rb[k].seq_off + qp[k]];
+          q0[k] = P.quals[Rb[k].seq_off + qp[k]];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kBatch; k++) {
+        if (!cv[k]) continue;
+        count_entry(P, Rb[k], p, qp[k], ind[k], dl[k], b0[k], q0[k], table, S, Q, uniq, has_red, err);
       }
     }
     if (lane_ok && !has_red) {
@@ -285,68 +316,82 @@ __global__ void __launch_bounds__(256) k_identify_columns(ImParams P) {
     bool insert_slot = false;
     Screen A;
     A.reset();
-    DRead R;
-    if (rb < re) R = load_read(P.reads, rb);
-    for (int r = rb; r < re; r++) {
-      DRead Rn;
-      const int rn = uniform_i(r + 1);   // keeps the record loads on the scalar unit
-      if (rn < re) Rn = load_read(P.reads, rn);     // prefetch the next alignment's record
-      int qpos, indel;
-      bool is_del;
-      if (lane_ok && entry_at(R, P.cigar, p, qpos, indel, is_del, err)) {
-        const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
-        uint8_t base = '.', qv = 0;
-        if (ondel >= 0) {
-          base = P.bases[R.seq_off + qpos];
-          qv = P.quals[R.seq_off + qpos];
-        }
-        if (base != 15) {
-          const bool top = !(R.flags & kReadReversed);
-          const bool trimmed = is_trimmed(R, qpos, ondel < 0);
-          if (R.x1 == 1) {
-            if (top) ++uniq_top;
-            else ++uniq_bot;
-            if (ondel > 0) insert_slot = true;
-          } else {
-            if (top) red_top += 1.0 / (double)R.x1;
-            else red_bot += 1.0 / (double)R.x1;
-          }
-          if (R.x1 <= 1 && !trimmed) {
-            int obs = -1, q = qv;
-            bool use = true;
-            if (ondel < 0) {   // deletion: quality of the next read base (error_count.cpp:1075-1080)
-              const int qq = qpos + 1 - (top ? 0 : 1);
-              if (qq < 0 || qq >= (int)R.l_qseq) {
-                err |= kErrQueryIndex;
-                use = false;
-              } else if (P.bases[R.seq_off + qq] == 15) {
-                use = false;
-              } else {
-                obs = 4;
-                q = P.quals[R.seq_off + qq];
-              }
-            } else {
-              obs = bam_index(base);
-              if (obs < 0) {
-                err |= kErrBaseCode;
-                use = false;
-              }
-            }
-            if (use && q >= P.base_quality_cutoff) {
-              if ((int)R.rfile >= S) err |= kErrReadSet;
-              else if (q >= Q) err |= kErrQuality;
-              else {
-                double l[5];
-                float lpmax;
-                lookup_lp(tab, tmax, S, Q, R, q, obs, top, l, lpmax);
-                for (int b = 0; b < 5; b++) lps[b] += l[b];
-                A.add(obs, l, lpmax, scr_ref);
-              }
-            }
-          }
+    // Batches of kBatch alignments: records and entries first, then the byte loads of the whole
+    // batch, then the per-read updates in file order (the FP64 sums keep the reference order).
+    for (int r0 = rb; r0 < re; r0 += kBatch) {
+      DRead Rb[kBatch];
+      int qp[kBatch], ind[kBatch];
+      bool dl[kBatch], cv[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; k++) {
+        const int r = uniform_i(r0 + k);
+        cv[k] = false;
+        if (r < re) {
+          Rb[k] = load_read(P.reads, r);
+          cv[k] = lane_ok && entry_at(Rb[k], P.cigar, p, qp[k], ind[k], dl[k], err);
         }
       }
-      R = Rn;
+      uint8_t b0[kBatch], q0[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; k++) {
+        b0[k] = '.';
+        q0[k] = 0;
+        if (cv[k] && !dl[k]) {
+          b0[k] = P.bases[Rb[k].seq_off + qp[k]];
+          q0[k] = P.quals[Rb[k].seq_off + qp[k]];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kBatch; k++) {
+        if (!cv[k]) continue;
+        const DRead& R = Rb[k];
+        const int qpos = qp[k];
+        const int ondel = dl[k] ? -1 : (ind[k] < 0 ? 0 : ind[k]);
+        const uint8_t base = b0[k];
+        if (base == 15) continue;
+        const bool top = !(R.flags & kReadReversed);
+        const bool trimmed = is_trimmed(R, qpos, ondel < 0);
+        if (R.x1 == 1) {
+          if (top) ++uniq_top;
+          else ++uniq_bot;
+          if (ondel > 0) insert_slot = true;
+        } else {
+          if (top) red_top += 1.0 / (double)R.x1;
+          else red_bot += 1.0 / (double)R.x1;
+        }
+        if (R.x1 > 1 || trimmed) continue;
+        int obs = -1, q = q0[k];
+        if (ondel < 0) {   // deletion: quality of the next read base (error_count.cpp:1075-1080)
+          const int qq = qpos + 1 - (top ? 0 : 1);
+          if (qq < 0 || qq >= (int)R.l_qseq) {
+            err |= kErrQueryIndex;
+            continue;
+          }
+          if (P.bases[R.seq_off + qq] == 15) continue;
+          obs = 4;
+          q = P.quals[R.seq_off + qq];
+        } else {
+          obs = bam_index(base);
+          if (obs < 0) {
+            err |= kErrBaseCode;
+            continue;
+          }
+        }
+        if (q < P.base_quality_cutoff) continue;
+        if ((int)R.rfile >= S) {
+          err |= kErrReadSet;
+          continue;
+        }
+        if (q >= Q) {
+          err |= kErrQuality;
+          continue;
+        }
+        double l[5];
+        float lpmax;
+        lookup_lp(tab, tmax, S, Q, R, q, obs, top, l, lpmax);
+        for (int b = 0; b < 5; b++) lps[b] += l[b];
+        A.add(obs, l, lpmax, scr_ref);
+      }
     }
     if (!lane_ok) {
       if (p < P.n_pos) {

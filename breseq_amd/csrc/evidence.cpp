@@ -591,3 +591,10 @@ std::string test_ra_evidence_text(const std::string& text, const bsq_reference* 
 }
 
 }  // namespace bsq
+
+extern "C" int32_t bsq_format_double(double v, uint32_t precision, int32_t scientific, char* buf, int32_t buf_len) {
+  const std::string s = bsq::fmt_double(v, precision, scientific != 0);
+  if (!buf || buf_len <= (int32_t)s.size()) return -1;
+  memcpy(buf, s.c_str(), s.size() + 1);
+  return (int32_t)s.size();
+}

@@ -187,6 +187,11 @@ int bsq_test_ra_evidence(const char* gd_text, const bsq_reference* ref,
                          bsq_status* status);
 void bsq_free_text(char* text);
 
+/* breseq's number formatting for evidence fields, formatted_double / to_string(double, precision,
+ * scientific) (common.h:845-866, 1542-1563): "NA" for NaN, otherwise %.Nf or %.Ne.  Returns the
+ * length written (excluding NUL), or -1 if the buffer is too small. */
+int32_t bsq_format_double(double v, uint32_t precision, int32_t scientific, char* buf, int32_t buf_len);
+
 /* ------------------------------------------------------------------------------------------ */
 /* calculate_trims  (calculate_trims.h:34, calculate_trims.cpp:67-179)                        */
 /* ------------------------------------------------------------------------------------------ */

@@ -1355,6 +1355,37 @@ int orc_sequence_trims(const char* seq, int64_t len, uint8_t* out) {
   return BSQ_OK;
 }
 
+double orc_fit_alleles(const double* r, const double* lpmax, const int32_t* obs, int64_t n, const int32_t* allowed,
+                       double tol, double* f, int32_t* iterations) {
+  bsq_identify_args args;
+  memset(&args, 0, sizeof(args));
+  args.polymorphism_precision_decimal = tol;
+  orc::Identify im(&args);
+  vector<orc::PData> pd((size_t)n);
+  for (int64_t i = 0; i < n; i++) {
+    for (int b = 0; b < 5; b++) pd[i].r[b] = r[i * 5 + b];
+    pd[i].lpmax = lpmax[i];
+    pd[i].base = orc::kIndexChar[obs[i]];
+  }
+  bool al[5];
+  for (int b = 0; b < 5; b++) al[b] = allowed[b] != 0;
+  orc::AlleleModel m = im.fit(pd, al);
+  for (int b = 0; b < 5; b++) f[b] = m.f[b];
+  *iterations = (int32_t)m.iterations;
+  return m.log10_likelihood;
+}
+
+double orc_pure_genotype(const double* lps, uint32_t n, int32_t* best) {
+  bsq_identify_args args;
+  memset(&args, 0, sizeof(args));
+  orc::Identify im(&args);
+  char g;
+  double score;
+  im.pure_genotype_call(lps, n, g, score);
+  *best = (g == 'N') ? 5 : (int32_t)(std::string("ACGT.").find(g));
+  return score;
+}
+
 double orc_fisher_exact_2x2(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return orc::fisher_2x2(a, b, c, d); }
 
 double orc_ks_test_less(const double* x, int64_t nx, const double* y, int64_t ny) {

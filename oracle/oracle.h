@@ -44,6 +44,14 @@ void orc_im_free(orc_im_out* o);
 /* SequenceTrims (calculate_trims.cpp:67-138) */
 int orc_sequence_trims(const char* seq, int64_t len, uint8_t* out);
 
+/* fit_allele_frequencies on explicit per-read data (identify_mutations.cpp:3240-3318):
+ * r[n*5] (max-normalized likelihoods), lpmax[n], obs[n] (0..4); allowed[5]; tol.
+ * Writes f[5]; returns log10 likelihood; *iterations gets the iteration count. */
+double orc_fit_alleles(const double* r, const double* lpmax, const int32_t* obs, int64_t n, const int32_t* allowed,
+                       double tol, double* f, int32_t* iterations);
+/* pure_genotype_call (identify_mutations.cpp:3398-3433): returns score, *best gets the index. */
+double orc_pure_genotype(const double* lps, uint32_t n, int32_t* best);
+
 /* stats (stats.cpp:2144-2284) */
 double orc_fisher_exact_2x2(uint32_t a, uint32_t b, uint32_t c, uint32_t d);
 double orc_ks_test_less(const double* x, int64_t nx, const double* y, int64_t ny);

@@ -45,6 +45,10 @@ def lib():
     O.orc_fisher_exact_2x2.argtypes = [C.c_uint32] * 4
     O.orc_ks_test_less.restype = C.c_double
     O.orc_ks_test_less.argtypes = [L.c_f64p, C.c_int64, L.c_f64p, C.c_int64]
+    O.orc_fit_alleles.restype = C.c_double
+    O.orc_fit_alleles.argtypes = [L.c_f64p, L.c_f64p, L.c_i32p, C.c_int64, L.c_i32p, C.c_double, L.c_f64p, L.c_i32p]
+    O.orc_pure_genotype.restype = C.c_double
+    O.orc_pure_genotype.argtypes = [L.c_f64p, C.c_uint32, L.c_i32p]
     O.orc_pileup_column.restype = C.c_int64
     O.orc_pileup_column.argtypes = [P(L.Alignments), C.c_int32, C.c_int64, L.c_i32p, C.c_int64]
     _O = O
@@ -134,3 +138,22 @@ def pileup_column(aln_ptr, tid, pos0, cap=4096):
     n = lib().orc_pileup_column(aln_ptr, tid, pos0, out.ctypes.data_as(L.c_i32p), cap)
     assert n >= 0
     return out[: 4 * min(n, cap)].reshape(-1, 4)
+
+
+def fit_alleles(r, lpmax, obs, allowed=(1, 1, 1, 1, 1), tol=1e-6):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    lpmax = np.ascontiguousarray(lpmax, dtype=np.float64)
+    obs = np.ascontiguousarray(obs, dtype=np.int32)
+    al = np.ascontiguousarray(allowed, dtype=np.int32)
+    f = np.zeros(5)
+    it = C.c_int32()
+    ll = lib().orc_fit_alleles(r.ctypes.data_as(L.c_f64p), lpmax.ctypes.data_as(L.c_f64p), obs.ctypes.data_as(L.c_i32p),
+                               len(obs), al.ctypes.data_as(L.c_i32p), tol, f.ctypes.data_as(L.c_f64p), C.byref(it))
+    return f, ll, it.value
+
+
+def pure_genotype(lps, n):
+    lps = np.ascontiguousarray(lps, dtype=np.float64)
+    best = C.c_int32()
+    score = lib().orc_pure_genotype(lps.ctypes.data_as(L.c_f64p), n, C.byref(best))
+    return best.value, score

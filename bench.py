@@ -32,6 +32,26 @@ BYTES_PER_COLUMN_EC = 1
 BYTES_PER_COLUMN_IM = 1 + 9
 
 
+COUNTERS = os.path.join(ROOT, "profiles", "counters_4.6Mb_100x.json")
+
+
+def traffic_of(kernel, cfg):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs (FETCH_SIZE and
+    WRITE_SIZE in KiB per dispatch, each scaled by what the counter reads for a copy of known size),
+    when they were taken on this workload; None otherwise."""
+    try:
+        with open(COUNTERS) as f:
+            c = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if c.get("ref_length") != cfg["ref_length"] or c.get("coverage") != cfg["coverage"]:
+        return None
+    k = c.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return (k["fetch_kib"] * c["fetch_scale"] + k["write_kib"] * c["write_scale"]) * 1024.0
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -123,7 +143,7 @@ def main():
         t1 = time.perf_counter()
         imt = sess.timing()
         step_ms.append((t1 - t0) * 1000.0)
-        ec_k.append(ect["kernel_detail_ms"][0])
+        ec_k.append(ect["kernel_detail_ms"])
         im_k.append(imt["kernel_detail_ms"])
         detail = imt
     total_s = sum(step_ms) / 1000.0
@@ -132,16 +152,17 @@ def main():
     n_ra = gd.count("\nRA\t")
     # dominant kernel and its roofline
     avg = lambda xs: sum(xs) / len(xs)
-    k_ec = avg(ec_k)
+    k_ec = avg([d[0] for d in ec_k])
+    k_cov = avg([d[5] for d in ec_k])
     k_col = avg([d[1] for d in im_k])
     k_ins = avg([d[2] for d in im_k])
     k_score = avg([d[3] for d in im_k])
     k_runs = avg([d[4] for d in im_k])
-    kernels = {"k_error_count": k_ec, "k_identify_columns": k_col, "k_insert_slots": k_ins,
-               "k_score_candidates": k_score, "k_runs+k_mc_detail": k_runs}
+    kernels = {"k_ec_reads": k_ec, "k_cov_scan+hist": k_cov, "k_identify_columns": k_col,
+               "k_insert_slots": k_ins, "k_score_candidates": k_score, "k_runs": k_runs}
     bytes_ec = BYTES_PER_BASE * nab + BYTES_PER_READ * n_reads + BYTES_PER_COLUMN_EC * n_cols
     bytes_im = BYTES_PER_BASE * nab + BYTES_PER_READ * n_reads + BYTES_PER_COLUMN_IM * n_cols
-    col_kernels = {"k_error_count": (k_ec, bytes_ec), "k_identify_columns": (k_col, bytes_im)}
+    col_kernels = {"k_ec_reads": (k_ec, bytes_ec), "k_identify_columns": (k_col, bytes_im)}
     dom = max(col_kernels, key=lambda k: col_kernels[k][0])
     dms, dbytes = col_kernels[dom]
     achieved = dbytes / (dms / 1000.0) / 1e9
@@ -167,7 +188,7 @@ def main():
                    "candidates": detail["candidates"]},
         "kernels_ms": kernels,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic_of(dom, cfg),
                      "algorithmic_bytes": dbytes},
     }
     if not args.no_cpu_baseline:

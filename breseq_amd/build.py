@@ -13,6 +13,7 @@ LIB = os.path.join(HERE, "libbsq.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 
 SOURCES = [
+    "error_count.hip",
     "kernels.hip",
     "session.hip",
     "extra.hip",

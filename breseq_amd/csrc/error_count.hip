@@ -1,0 +1,346 @@
+// error_count.hip -- pass 1 of the pileup path on CDNA4 (gfx950): the covariate count table and the
+// unique-only coverage histogram (error_count_pileup::pileup_callback, error_count.cpp:125-199, with
+// cErrorTable::count_alignment_position, :854-986).
+//
+// Both outputs are integer sums, so they do not depend on the order in which pileup entries are
+// visited.  Pass 1 therefore walks each alignment once (read-parallel) instead of walking columns:
+//
+//   k_ec_reads       one wavefront per alignment at a time; lanes take consecutive aligned bases of
+//                    each M/=/X run, so base, quality and reference loads are 64-byte coalesced.
+//                    Covariates go to an LDS-privatized count table.  Lane 0 adds +1/-1 at the ends
+//                    of each aligned run to a per-column difference array (unique or redundant).
+//   k_cov_tile_sums  phase 1 of the prefix scan of the difference arrays (per 4096-column tile)
+//   k_cov_tile_scan  phase 2: exclusive scan of the tile sums (one block)
+//   k_cov_hist       phase 3: per-column unique and redundant coverage -> histogram of the unique
+//                    coverage of columns without redundant reads (LDS-privatized)
+//
+// Deleted reference bases (is_del entries) never count, so only M/=/X runs contribute to either
+// output; the entry of the last base of a run carries the run's indel (next CIGAR operations), as
+// htslib's resolve_cigar2 reports it.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace bsq {
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// BAM 4-bit base code -> A0 C1 G2 T3; every other code -> 15.
+__device__ __forceinline__ uint32_t base_idx(uint32_t b) {
+  const uint32_t i = (uint32_t)(0xFFFFFFF3FFF2F10FULL >> ((b & 15u) * 4u)) & 15u;
+  return b > 15u ? 15u : i;
+}
+
+// Complement of a base index when rv == 3 (A<->T, C<->G); other codes are unchanged.
+__device__ __forceinline__ uint32_t comp_if(uint32_t i, uint32_t rv) { return i < 4u ? (i ^ rv) : i; }
+
+typedef __attribute__((address_space(4))) const uint32_t ConstU32;
+typedef __attribute__((address_space(4))) const int32_t ConstI32;
+
+// Indel carried by the entry of the last base of the reference-consuming operation at index k
+// (htslib resolve_cigar2 as breseq reads it; same rule as entry_at).
+__device__ __forceinline__ int run_end_indel(const ConstU32* c, int k, int n) {
+  if (k + 1 >= n) return 0;
+  const uint32_t op2 = c[k + 1] & 15u;
+  int indel = 0;
+  if (op2 == OP_D) {
+    indel = -(int)(c[k + 1] >> 4);
+    for (int kk = k + 2; kk < n; ++kk) {
+      if ((c[kk] & 15u) != OP_D) break;
+      indel -= (int)(c[kk] >> 4);
+    }
+  } else if (op2 == OP_I) {
+    indel = (int)(c[k + 1] >> 4);
+    for (int kk = k + 2; kk < n; ++kk) {
+      const uint32_t o3 = c[kk] & 15u;
+      if (o3 == OP_I) indel += (int)(c[kk] >> 4);
+      else if (o3 != OP_P) break;
+    }
+  } else if (op2 == OP_P && k + 2 < n) {
+    int l3 = 0;
+    for (int kk = k + 2; kk < n; ++kk) {
+      const uint32_t o3 = c[kk] & 15u;
+      if (o3 == OP_I) l3 += (int)(c[kk] >> 4);
+      else if (is_ref_op(o3)) break;
+    }
+    if (l3 > 0) indel = l3;
+  }
+  return indel;
+}
+
+// count_alignment_position (error_count.cpp:854-986) for the aligned bases [x, x+len) <-> query
+// [y, y+len) of one unique read; the last base carries `last_indel`.
+__device__ __forceinline__ void ec_count_run(const EcParams& P, const uint8_t* __restrict__ sq,
+                                             const uint8_t* __restrict__ ql, const DRead& R, int x, int y, int len,
+                                             int last_indel, uint32_t* table, uint32_t& err) {
+  const int S = P.read_set_max, Q = P.quality_max;
+  const int rev = (R.flags & kReadReversed) ? 1 : 0;
+  const uint32_t rv = rev ? 3u : 0u;
+  const int rs = R.rfile;
+  const int lane = threadIdx.x & 63;
+  const uint8_t* __restrict__ rx = P.ref_idx + x;
+  for (int j0 = 0; j0 < len; j0 += 64) {
+    const int j = j0 + lane;
+    if (j >= len) break;
+    const int qpos = y + j;
+    const uint32_t b = sq[qpos], q = ql[qpos];
+    const uint32_t r0 = rx[j], r1 = rx[j + 1];
+    const int indel = (j == len - 1) ? last_indel : 0;
+    // (1) base against reference, both complemented on the reverse strand (:877-903)
+    if (b != 15u && r0 != kRefN) {
+      const uint32_t oi = comp_if(base_idx(b), rv), ri = comp_if(r0, rv);
+      if (oi > 3u) err |= kErrBaseCode;
+      else if (ri > 3u) err |= kErrRefChar;
+      else if (q >= (uint32_t)Q) err |= kErrQuality;
+      else atomicAdd(&table[rs + S * (int)ri + 5 * S * (int)oi + 25 * S * (int)q], 1u);
+    }
+    // (2) '..' pair and (3) one-base deletion read the base after qpos in read direction
+    const bool pair = (indel == 0) && (qpos < (int)R.qe0);
+    if (pair || indel == -1) {
+      const int mq = qpos + 1 - rev;
+      if (mq < 0 || mq >= (int)R.l_qseq) {
+        err |= kErrQueryIndex;
+      } else {
+        const uint32_t b2 = rev ? b : (uint32_t)sq[mq];
+        const uint32_t q2 = rev ? q : (uint32_t)ql[mq];
+        if (pair) {   // (:909-928): reference at qpos+1-rev in reference direction
+          const uint32_t rp = rev ? r0 : r1;
+          if (b2 != 15u && rp != kRefN) {
+            if (q2 >= (uint32_t)Q) err |= kErrQuality;
+            else atomicAdd(&table[rs + S * 4 + 5 * S * 4 + 25 * S * (int)q2], 1u);
+          }
+        } else if (b2 != 15u && r1 != kRefN) {   // (:934-956): reference base after the deletion
+          const uint32_t rd = comp_if(r1, rv);
+          if (rd > 4u) err |= kErrRefChar;
+          else if (q2 >= (uint32_t)Q) err |= kErrQuality;
+          else atomicAdd(&table[rs + S * (int)rd + 5 * S * 4 + 25 * S * (int)q2], 1u);
+        }
+      }
+    } else if (indel == 1) {   // (4) one-base insertion (:963-982)
+      const int mq = qpos + 1;
+      if (mq >= (int)R.l_qseq) {
+        err |= kErrQueryIndex;
+      } else if (mq <= (int)R.qe0 && mq >= (int)R.qs0) {
+        const uint32_t b3 = sq[mq];
+        if (b3 != 15u) {
+          const uint32_t oi = comp_if(base_idx(b3), rv);
+          const uint32_t q3 = ql[mq];
+          if (oi > 3u) err |= kErrBaseCode;
+          else if (q3 >= (uint32_t)Q) err |= kErrQuality;
+          else atomicAdd(&table[rs + S * 4 + 5 * S * (int)oi + 25 * S * (int)q3], 1u);
+        }
+      }
+    }
+  }
+}
+
+constexpr int kEcChunk = 16;   // consecutive alignments per wavefront work item
+
+__global__ void __launch_bounds__(256) k_ec_reads(EcParams P) {
+  extern __shared__ uint32_t table[];
+  for (int i = threadIdx.x; i < P.n_bins; i += blockDim.x) table[i] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const ConstI32* pos_target = (const ConstI32*)P.pos_target;
+  const ConstI32* scored = (const ConstI32*)P.target_scored;
+  int32_t* diff_u = P.cov_diff;
+  int32_t* diff_r = P.cov_diff + P.n_pos + 1;
+  uint32_t err = 0;
+  for (int64_t c0 = gw * kEcChunk; c0 < P.n_reads; c0 += nw * kEcChunk) {
+    const int64_t c1 = min<int64_t>(c0 + kEcChunk, P.n_reads);
+    for (int64_t r = c0; r < c1; r++) {
+      const DRead R = load_read(P.reads, (int)r);
+      const int t = pos_target[R.gpos];
+      if (t < 0 || !scored[t]) continue;
+      const bool red = R.x1 > 1;
+      int32_t* diff = red ? diff_r : diff_u;
+      bool count = !red && P.do_errors;
+      if (count && (int)R.rfile >= P.read_set_max) {   // covariate read_set out of range: ASSERT in the reference
+        err |= kErrReadSet;
+        count = false;
+      }
+      const uint8_t* sq = P.bases + R.seq_off;
+      const uint8_t* ql = P.quals + R.seq_off;
+      if (R.flags & kReadSimple) {
+        if (lane == 0) {
+          atomicAdd(&diff[R.gpos], 1);
+          atomicAdd(&diff[R.gend], -1);
+        }
+        if (count) ec_count_run(P, sq, ql, R, R.gpos, R.y0, R.gend - R.gpos, 0, table, err);
+        continue;
+      }
+      const ConstU32* c = (const ConstU32*)(P.cigar + R.cig_off);
+      const int n = R.n_cigar;
+      int x = R.gpos, y = 0;
+      for (int k = 0; k < n; k++) {
+        const uint32_t w = c[k];
+        const uint32_t op = w & 15u;
+        const int l = (int)(w >> 4);
+        if (op == OP_M || op == OP_EQ || op == OP_X) {
+          if (l > 0) {
+            if (lane == 0) {
+              atomicAdd(&diff[x], 1);
+              atomicAdd(&diff[x + l], -1);
+            }
+            if (count) ec_count_run(P, sq, ql, R, x, y, l, run_end_indel(c, k, n), table, err);
+          }
+          x += l;
+          y += l;
+        } else if (op == OP_I || op == OP_S) {
+          y += l;
+        } else if (op == OP_D || op == OP_N) {
+          x += l;
+        }
+      }
+      if (x != R.gend) err |= kErrCigar;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < P.n_bins; i += blockDim.x)
+    if (table[i]) atomicAdd(&P.counts[i], (unsigned long long)table[i]);
+  if (err) atomicOr(P.err, err);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Coverage histogram: prefix scan of the difference arrays                                    */
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Inclusive scan across the 64 lanes of a wavefront.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_cov_tile_sums(CovParams C) {
+  __shared__ int red[2][4];
+  const int64_t n = (int64_t)C.n_pos + 1;
+  const int64_t base = (int64_t)blockIdx.x * kCovTile;
+  const int32_t* du = C.cov_diff;
+  const int32_t* dr = C.cov_diff + n;
+  int su = 0, sr = 0;
+#pragma unroll 4
+  for (int i = 0; i < kCovTile / 256; i++) {
+    const int64_t idx = base + i * 256 + threadIdx.x;
+    if (idx < n) {
+      su += du[idx];
+      sr += dr[idx];
+    }
+  }
+  su = wave_sum(su);
+  sr = wave_sum(sr);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][wv] = su;
+    red[1][wv] = sr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) C.tile_sum[blockIdx.x] = make_int2(red[0][0] + red[0][1] + red[0][2] + red[0][3],
+                                                           red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+}
+
+// One block of 1024 threads: tile_sum becomes the exclusive prefix of the tile sums.
+__global__ void __launch_bounds__(1024) k_cov_tile_scan(CovParams C) {
+  __shared__ int wsum[2][16];
+  __shared__ int carry[2];
+  if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int b0 = 0; b0 < C.n_tiles; b0 += 1024) {
+    const int b = b0 + threadIdx.x;
+    const int2 v = (b < C.n_tiles) ? C.tile_sum[b] : make_int2(0, 0);
+    const int iu = wave_incl_scan(v.x), ir = wave_incl_scan(v.y);
+    if (lane == 63) {
+      wsum[0][wv] = iu;
+      wsum[1][wv] = ir;
+    }
+    __syncthreads();
+    int ou = carry[0], orr = carry[1];
+    for (int k = 0; k < wv; k++) {
+      ou += wsum[0][k];
+      orr += wsum[1][k];
+    }
+    if (b < C.n_tiles) C.tile_sum[b] = make_int2(ou + iu - v.x, orr + ir - v.y);
+    __syncthreads();
+    if (threadIdx.x == 1023) {
+      carry[0] = ou + iu;
+      carry[1] = orr + ir;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) k_cov_hist(CovParams C) {
+  extern __shared__ uint32_t hist[];   // n_groups_lds * hist_lds
+  __shared__ int wtot[2][4];
+  const int nh = C.n_groups_lds * C.hist_lds;
+  for (int i = threadIdx.x; i < nh; i += blockDim.x) hist[i] = 0;
+  const int64_t n = (int64_t)C.n_pos + 1;
+  const int64_t c0 = (int64_t)blockIdx.x * kCovTile + (int64_t)threadIdx.x * 16;
+  const int32_t* du = C.cov_diff;
+  const int32_t* dr = C.cov_diff + n;
+  int pu[16], pr[16];
+  int su = 0, sr = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int64_t idx = c0 + i;
+    const int a = (idx < n) ? du[idx] : 0;
+    const int b = (idx < n) ? dr[idx] : 0;
+    su += a;
+    sr += b;
+    pu[i] = su;
+    pr[i] = sr;
+  }
+  // exclusive prefix of the thread totals within the block
+  const int iu = wave_incl_scan(su), ir = wave_incl_scan(sr);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 63) {
+    wtot[0][wv] = iu;
+    wtot[1][wv] = ir;
+  }
+  __syncthreads();
+  const int2 tile = C.tile_sum[blockIdx.x];
+  int ou = tile.x + iu - su, orr = tile.y + ir - sr;
+  for (int k = 0; k < wv; k++) {
+    ou += wtot[0][k];
+    orr += wtot[1][k];
+  }
+  uint32_t err = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int64_t col = c0 + i;
+    if (col >= C.n_pos) break;
+    const int t = C.pos_target[col];
+    if (t < 0 || !C.target_scored[t]) continue;
+    const int cu = ou + pu[i], cr = orr + pr[i];
+    if (cr != 0) continue;   // a redundant read covers the column (error_count.cpp:172-175)
+    const int g = C.target_group[t];
+    if (g < C.n_groups_lds && cu < C.hist_lds) atomicAdd(&hist[g * C.hist_lds + cu], 1u);
+    else if (cu >= 0 && cu < C.hist_len) atomicAdd(&C.hist[(int64_t)g * C.hist_len + cu], 1ull);
+    else err |= kErrCapacity;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nh; i += blockDim.x) {
+    if (hist[i]) {
+      const int g = i / C.hist_lds, c = i % C.hist_lds;
+      atomicAdd(&C.hist[(int64_t)g * C.hist_len + c], (unsigned long long)hist[i]);
+    }
+  }
+  if (err) atomicOr(C.err, err);
+}
+
+}  // namespace bsq

@@ -8,26 +8,43 @@
 
 namespace bsq {
 
+// Reference base codes per global column (ref_idx[]): A C G T = 0..3, '.' = 4, N = 5, anything
+// else (including the pad after each target) = 7.
+enum : uint8_t { kRefDot = 4, kRefN = 5, kRefOther = 7 };
+
+// Pass 1, read-parallel: covariate counts and coverage difference arrays.
 struct EcParams {
   const DRead* reads;
   const uint32_t* cigar;
   const uint8_t* bases;
   const uint8_t* quals;
-  const char* ref;
-  const int32_t* pos_target;    // target of each global column, -1 for pads / unscored
-  const int32_t* target_group;  // coverage group of each target
+  const uint8_t* ref_idx;       // per global column (plus one pad), kRef* codes
+  const int32_t* pos_target;    // target of each global column, -1 for pads
   const int32_t* target_scored; // 1 if the target is in the call set
-  const int32_t* win_rb;        // alignments overlapping window w: [win_rb[w], win_re[w])
-  const int32_t* win_re;
-  int64_t n_windows;
+  int64_t n_reads;
   int32_t n_pos;
   int32_t read_set_max, quality_max, n_bins;
+  int32_t do_errors;
+  int32_t* cov_diff;            // [0, n_pos+1): unique reads, [n_pos+1, 2n_pos+2): redundant reads
+  unsigned long long* counts;   // n_bins
+  uint32_t* err;
+};
+
+// Pass 1, coverage histogram from the difference arrays (three-phase prefix scan).
+struct CovParams {
+  const int32_t* cov_diff;      // as EcParams::cov_diff
+  const int32_t* pos_target;
+  const int32_t* target_group;
+  const int32_t* target_scored;
+  int32_t n_pos;
+  int32_t n_tiles;
+  int2* tile_sum;               // per tile: (unique, redundant) sums, then exclusive prefixes
   int32_t n_groups_lds, hist_lds;   // LDS-privatized histogram shape
   int32_t hist_len;                 // global histogram length per group
-  unsigned long long* counts;       // n_bins
   unsigned long long* hist;         // n_groups * hist_len
   uint32_t* err;
 };
+constexpr int kCovTile = 4096;      // columns per block of the coverage scan (256 threads x 16)
 
 struct ImParams {
   const DRead* reads;
@@ -124,7 +141,11 @@ struct Screen {
 };
 
 
-__global__ void k_error_count(EcParams P);
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v);
+__global__ void k_ec_reads(EcParams P);
+__global__ void k_cov_tile_sums(CovParams C);
+__global__ void k_cov_tile_scan(CovParams C);
+__global__ void k_cov_hist(CovParams C);
 __global__ void k_identify_columns(ImParams P);
 __global__ void k_insert_slots(ImParams P);
 __global__ void k_score_candidates(ImParams P);

@@ -1,7 +1,5 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the pileup-scoring engine.
 //
-//  k_error_count      pass 1: unique-only coverage histogram + covariate count table
-//                     (error_count_pileup::pileup_callback, error_count.cpp:125-199, 854-997)
 //  k_identify_columns pass 2, insert slot 0 of every column: coverage tallies, per-base counts,
 //                     summed per-genotype log10 likelihoods in read order, pure-genotype call, and
 //                     a conservative screen that routes columns to the exact FP64 scorer
@@ -27,170 +25,7 @@ namespace bsq {
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-/* ------------------------------------------------------------------------------------------ */
-/* Pass 1: error_count                                                                         */
-
 constexpr int kBatch = 4;
-
-// One column entry of error_count_pileup::pileup_callback (error_count.cpp:137-175) with
-// cErrorTable::count_alignment_position (:854-986).  b0/q0: base and quality at qpos.
-__device__ __forceinline__ void count_entry(const EcParams& P, const DRead& R, int p, int qpos, int indel, bool is_del,
-                                            uint8_t b0, uint8_t q0, uint32_t* table, int S, int Q, int& uniq,
-                                            bool& has_red, uint32_t& err) {
-      if (is_del) return;
-      if (R.x1 > 1) {
-        has_red = true;
-        return;
-      }
-      ++uniq;
-      const int rev = (R.flags & kReadReversed) ? 1 : 0;
-      const int rs = R.rfile;
-      if (rs >= S) {
-        err |= kErrReadSet;
-        return;
-      }
-      const uint8_t* sq = P.bases + R.seq_off;
-      const uint8_t* ql = P.quals + R.seq_off;
-      // (1) base substitution / match, read-strand space (error_count.cpp:877-903)
-      {
-        uint8_t ob = b0;
-        char rc = P.ref[p];
-        if (ob != 15 && rc != 'N') {
-          if (rev) {
-            ob = comp_bam(ob);
-            rc = comp_char(rc);
-          }
-          const int oi = bam_index(ob), ri = char_index(rc);
-          const int q = q0;
-          if (oi < 0 || oi > 3) err |= kErrBaseCode;
-          else if (ri < 0 || ri > 3) err |= kErrRefChar;
-          else if (q >= Q) err |= kErrQuality;
-          else atomicAdd(&table[rs + S * ri + 5 * S * oi + 25 * S * q], 1u);
-        }
-      }
-      if (indel == 0) {   // '..' pair (:909-928)
-        if (qpos < (int)R.qe0) {
-          const int mq = qpos + 1 - rev;
-          if (mq < 0 || mq >= (int)R.l_qseq) {
-            err |= kErrQueryIndex;
-          } else {
-            const uint8_t ob = sq[mq];
-            const char rc = P.ref[p + 1 - rev];
-            const int q = ql[mq];
-            if (ob != 15 && rc != 'N') {
-              if (q >= Q) err |= kErrQuality;
-              else atomicAdd(&table[rs + S * 4 + 5 * S * 4 + 25 * S * q], 1u);
-            }
-          }
-        }
-      } else if (indel == -1) {   // deletion of one base (:934-956)
-        const int mq = qpos + 1 - rev;
-        if (mq < 0 || mq >= (int)R.l_qseq) {
-          err |= kErrQueryIndex;
-        } else {
-          const uint8_t ob = sq[mq];
-          char rc = P.ref[p + 1];
-          if (ob != 15 && rc != 'N') {
-            if (rev) rc = comp_char(rc);
-            const int ri = char_index(rc);
-            const int q = ql[mq];
-            if (ri < 0 || ri > 4) err |= kErrRefChar;
-            else if (q >= Q) err |= kErrQuality;
-            else atomicAdd(&table[rs + S * ri + 5 * S * 4 + 25 * S * q], 1u);
-          }
-        }
-      } else if (indel == 1) {   // insertion of one base (:963-982)
-        const int mq = qpos + 1;
-        if (mq < 0 || mq >= (int)R.l_qseq) {
-          err |= kErrQueryIndex;
-        } else if (mq <= (int)R.qe0 && mq >= (int)R.qs0) {
-          uint8_t ob = sq[mq];
-          if (ob != 15) {
-            if (rev) ob = comp_bam(ob);
-            const int oi = bam_index(ob);
-            const int q = ql[mq];
-            if (oi < 0 || oi > 3) err |= kErrBaseCode;
-            else if (q >= Q) err |= kErrQuality;
-            else atomicAdd(&table[rs + S * 4 + 5 * S * oi + 25 * S * q], 1u);
-          }
-        }
-      }
-    }
-
-__global__ void __launch_bounds__(256) k_error_count(EcParams P) {
-  extern __shared__ uint32_t lds[];
-  uint32_t* table = lds;                         // P.n_bins counters
-  uint32_t* hist = lds + P.n_bins;               // P.n_groups_lds * P.hist_lds bins
-  const int tid = threadIdx.x;
-  const int nlds = P.n_bins + P.n_groups_lds * P.hist_lds;
-  for (int i = tid; i < nlds; i += blockDim.x) lds[i] = 0;
-  __syncthreads();
-
-  const int wave = tid / WAVE, lane = tid % WAVE;
-  const int waves_per_block = blockDim.x / WAVE;
-  const int64_t chunk = (P.n_windows + gridDim.x - 1) / gridDim.x;
-  const int64_t w_begin = (int64_t)blockIdx.x * chunk;
-  const int64_t w_end = min<int64_t>(w_begin + chunk, P.n_windows);
-  uint32_t err = 0;
-  const int S = P.read_set_max, Q = P.quality_max;
-
-  for (int64_t w = w_begin + wave; w < w_end; w += waves_per_block) {
-    const int p = (int)(w * WAVE + lane);
-    const int tgt = (p < P.n_pos) ? P.pos_target[p] : -1;
-    const bool lane_ok = (tgt >= 0) && P.target_scored[tgt];
-    const int rb = uniform_i(P.win_rb[w]), re = uniform_i(P.win_re[w]);
-    int uniq = 0;
-    bool has_red = false;
-    // Alignments are processed in batches of kBatch: the records (scalar loads) and the column
-    // entries of the whole batch are resolved first, so the byte loads of all of them are in
-    // flight together; the counts are then updated in file order.
-    for (int r0 = rb; r0 < re; r0 += kBatch) {
-      DRead Rb[kBatch];
-      int qp[kBatch], ind[kBatch];
-      bool dl[kBatch], cv[kBatch];
-#pragma unroll
-      for (int k = 0; k < kBatch; k++) {
-        const int r = uniform_i(r0 + k);
-        cv[k] = false;
-        if (r < re) {
-          Rb[k] = load_read(P.reads, r);
-          cv[k] = lane_ok && entry_at(Rb[k], P.cigar, p, qp[k], ind[k], dl[k], err);
-        }
-      }
-      uint8_t b0[kBatch], q0[kBatch];
-#pragma unroll
-      for (int k = 0; k < kBatch; k++) {
-        b0[k] = 15;
-        q0[k] = 0;
-        if (cv[k] && !dl[k]) {
-          b0[k] = P.bases[Rb[k].seq_off + qp[k]];
-          q0[k] = P.quals[Rb[k].seq_off + qp[k]];
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < kBatch; k++) {
-        if (!cv[k]) continue;
-        count_entry(P, Rb[k], p, qp[k], ind[k], dl[k], b0[k], q0[k], table, S, Q, uniq, has_red, err);
-      }
-    }
-    if (lane_ok && !has_red) {
-      const int g = P.target_group[tgt];
-      if (g < P.n_groups_lds && uniq < P.hist_lds) atomicAdd(&hist[g * P.hist_lds + uniq], 1u);
-      else if (uniq < P.hist_len) atomicAdd(&P.hist[(int64_t)g * P.hist_len + uniq], 1ull);
-      else err |= kErrCapacity;
-    }
-  }
-  __syncthreads();
-  for (int i = tid; i < P.n_bins; i += blockDim.x)
-    if (table[i]) atomicAdd(&P.counts[i], (unsigned long long)table[i]);
-  for (int i = tid; i < P.n_groups_lds * P.hist_lds; i += blockDim.x) {
-    if (hist[i]) {
-      const int g = i / P.hist_lds, c = i % P.hist_lds;
-      atomicAdd(&P.hist[(int64_t)g * P.hist_len + c], (unsigned long long)hist[i]);
-    }
-  }
-  if (err) atomicOr(P.err, err);
-}
 
 /* ------------------------------------------------------------------------------------------ */
 /* Pass 2: per-column tallies at insert slot 0                                                 */

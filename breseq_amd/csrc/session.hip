@@ -81,6 +81,9 @@ struct Session {
   DBuf<uint32_t> d_cigar;
   DBuf<uint8_t> d_bases, d_quals;
   DBuf<char> d_ref;
+  DBuf<uint8_t> d_ref_idx;        // kRef* code per global column
+  DBuf<int32_t> d_cov_diff;       // pass-1 coverage difference arrays (unique, redundant)
+  DBuf<int2> d_tile_sum;          // pass-1 coverage scan tile sums
   DBuf<int32_t> d_pos_target, d_win_rb, d_win_re, d_target_group, d_target_scored;
   DBuf<double> d_fold_lp;
   DBuf<float> d_fold_max;
@@ -283,6 +286,23 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   }
   S.d_ref.alloc(S.h_ref.size());
   HIP_OK(hipMemcpyAsync(S.d_ref.p, S.h_ref.data(), S.h_ref.size(), hipMemcpyHostToDevice, S.stream));
+  {
+    std::vector<uint8_t> ri(S.h_ref.size());
+    for (size_t i = 0; i < ri.size(); i++) {
+      switch (S.h_ref[i]) {
+        case 'A': ri[i] = 0; break;
+        case 'C': ri[i] = 1; break;
+        case 'G': ri[i] = 2; break;
+        case 'T': ri[i] = 3; break;
+        case '.': ri[i] = kRefDot; break;
+        case 'N': ri[i] = kRefN; break;
+        default: ri[i] = kRefOther; break;
+      }
+    }
+    S.d_ref_idx.alloc(ri.size());
+    HIP_OK(hipMemcpyAsync(S.d_ref_idx.p, ri.data(), ri.size(), hipMemcpyHostToDevice, S.stream));
+    HIP_OK(hipStreamSynchronize(S.stream));
+  }
   S.d_pos_target.alloc(S.h_pos_target.size());
   HIP_OK(hipMemcpyAsync(S.d_pos_target.p, S.h_pos_target.data(), S.h_pos_target.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
   S.d_win_rb.alloc(rb.size());
@@ -383,40 +403,64 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   HIP_OK(hipMemsetAsync(S.d_counts.p, 0, n_bins * sizeof(unsigned long long), S.stream));
   HIP_OK(hipMemsetAsync(S.d_hist.p, 0, (size_t)G * hist_len * sizeof(unsigned long long), S.stream));
   HIP_OK(hipMemsetAsync(S.d_err.p, 0, sizeof(uint32_t), S.stream));
+  const int64_t n_diff = 2 * ((int64_t)S.n_pos + 1);
+  S.d_cov_diff.alloc((size_t)n_diff);
+  const int n_tiles = (int)(((int64_t)S.n_pos + 1 + kCovTile - 1) / kCovTile);
+  S.d_tile_sum.alloc((size_t)n_tiles);
   EcParams P;
   P.reads = S.d_reads.p;
   P.cigar = S.d_cigar.p;
   P.bases = S.d_bases.p;
   P.quals = S.d_quals.p;
-  P.ref = S.d_ref.p;
+  P.ref_idx = S.d_ref_idx.p;
   P.pos_target = S.d_pos_target.p;
-  P.target_group = S.d_target_group.p;
   P.target_scored = S.d_target_scored.p;
-  P.win_rb = S.d_win_rb.p;
-  P.win_re = S.d_win_re.p;
-  P.n_windows = S.n_windows;
+  P.n_reads = S.n_dreads;
   P.n_pos = S.n_pos;
   P.read_set_max = Sm;
   P.quality_max = Q;
   P.n_bins = n_bins;
-  P.n_groups_lds = std::min(G, 4);
-  P.hist_lds = 1024;
-  P.hist_len = hist_len;
+  P.do_errors = args->do_errors ? 1 : 0;
+  P.cov_diff = S.d_cov_diff.p;
   P.counts = S.d_counts.p;
-  P.hist = S.d_hist.p;
   P.err = S.d_err.p;
-  const size_t lds = (size_t)(n_bins + P.n_groups_lds * P.hist_lds) * sizeof(uint32_t);
-  if (lds > 64 * 1024) throw BsqError(BSQ_ERR_LIMIT, "covariate table too large for LDS");
+  CovParams C;
+  C.cov_diff = S.d_cov_diff.p;
+  C.pos_target = S.d_pos_target.p;
+  C.target_group = S.d_target_group.p;
+  C.target_scored = S.d_target_scored.p;
+  C.n_pos = S.n_pos;
+  C.n_tiles = n_tiles;
+  C.tile_sum = S.d_tile_sum.p;
+  C.n_groups_lds = std::min(G, 4);
+  C.hist_lds = 1024;
+  C.hist_len = hist_len;
+  C.hist = S.d_hist.p;
+  C.err = S.d_err.p;
+  const size_t lds_table = (size_t)n_bins * sizeof(uint32_t);
+  const size_t lds_hist = (size_t)C.n_groups_lds * C.hist_lds * sizeof(uint32_t);
+  if (lds_table > 64 * 1024) throw BsqError(BSQ_ERR_LIMIT, "covariate table too large for LDS");
+  // one block per 8 waves' worth of alignment chunks, capped at 8 blocks per CU
+  const int64_t chunks = (S.n_dreads + 15) / 16;
+  const int ec_grid = (int)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 2048));
   HIP_OK(hipEventRecord(S.ev0, S.stream));
-  hipLaunchKernelGGL(k_error_count, dim3(grid_for(S.n_windows)), dim3(256), lds, S.stream, P);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, S.stream, S.d_cov_diff.p, n_diff, 0);
+  hipLaunchKernelGGL(k_ec_reads, dim3(ec_grid), dim3(256), lds_table, S.stream, P);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(S.evk[0], S.stream));
+  hipLaunchKernelGGL(k_cov_tile_sums, dim3(n_tiles), dim3(256), 0, S.stream, C);
+  hipLaunchKernelGGL(k_cov_tile_scan, dim3(1), dim3(1024), 0, S.stream, C);
+  hipLaunchKernelGGL(k_cov_hist, dim3(n_tiles), dim3(256), lds_hist, S.stream, C);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.ev1, S.stream));
   HIP_OK(hipStreamSynchronize(S.stream));
-  float ms = 0;
+  float ms = 0, ms_reads = 0;
   HIP_OK(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+  HIP_OK(hipEventElapsedTime(&ms_reads, S.ev0, S.evk[0]));
   S.timing.kernel_ms = ms;
   for (double& d : S.timing.kernel_detail_ms) d = 0.0;
-  S.timing.kernel_detail_ms[0] = ms;
+  S.timing.kernel_detail_ms[0] = ms_reads;
+  S.timing.kernel_detail_ms[5] = ms - ms_reads;
   check_err(S);
   const double t0 = now_ms();
   std::vector<unsigned long long> counts(n_bins), hist((size_t)G * hist_len);

@@ -231,8 +231,9 @@ typedef struct bsq_timing {
   int64_t n_reads;         /* mapped alignments in the device CSR                 */
   int64_t n_positions;     /* reference columns (incl. one pad per target)        */
   /* per-kernel times of the last pass, HIP events on the session stream:
-   * [0] k_error_count  [1] k_identify_columns  [2] k_insert_slots
-   * [3] k_score_candidates  [4] k_runs + k_mc_detail */
+   * [0] k_ec_reads (with the difference-array clear)  [1] k_identify_columns
+   * [2] k_insert_slots  [3] k_score_candidates  [4] k_runs
+   * [5] k_cov_tile_sums + k_cov_tile_scan + k_cov_hist */
   double kernel_detail_ms[8];
 } bsq_timing;
 

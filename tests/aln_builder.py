@@ -79,3 +79,106 @@ def bin_index(S, rs, ref, obs, q):
 
 
 IDX = {"A": 0, "C": 1, "G": 2, "T": 3, ".": 4}
+
+
+class BuiltSample:
+    """Adapter giving a Built set the Sample interface the argument builders use."""
+
+    def __init__(self, built):
+        self.b = built
+        self.ref = C.pointer(built.ref)
+        self.aln = C.pointer(built.aln)
+
+    @property
+    def n_targets(self):
+        return len(self.b.names)
+
+    def target_lengths(self):
+        return [int(x) for x in self.b.lengths]
+
+    def max_quality(self):
+        return int(self.b.qual.max()) if self.b.qual.size else 0
+
+    def read_files(self):
+        return self.b.read_files
+
+
+COMP = {"A": "T", "C": "G", "G": "C", "T": "A", "N": "N"}
+
+
+def random_alignments(seed, n_targets=2, target_len=400, n_reads=600, max_len=60):
+    """Alignments with every CIGAR shape the pileup distinguishes: soft clips, insertions and
+    deletions (also adjacent ones), reference skips, pads before insertions, =/X runs, N read bases,
+    both strands, redundant reads and reads ending at a target end.  Sorted by (target, position)."""
+    rng = np.random.default_rng(seed)
+    refs = []
+    for t in range(n_targets):
+        s = rng.choice(list("ACGT"), size=target_len)
+        s[rng.random(target_len) < 0.01] = "N"
+        refs.append(("t%d" % t, "".join(s)))
+    reads = []
+    for _ in range(n_reads):
+        t = int(rng.integers(n_targets))
+        ref = refs[t][1]
+        ops = []
+        if rng.random() < 0.3:
+            ops.append(("S", int(rng.integers(1, 6))))
+        if rng.random() < 0.05:
+            ops.append(("I", int(rng.integers(1, 3))))
+        n_runs = 1 if rng.random() < 0.3 else int(rng.integers(2, 5))
+        for k in range(n_runs):
+            m = "M" if rng.random() < 0.8 else ("=" if rng.random() < 0.5 else "X")
+            ops.append((m, int(rng.integers(3, 20))))
+            if k == n_runs - 1:
+                break
+            u = rng.random()
+            if u < 0.35:
+                ops.append(("D", int(rng.integers(1, 4))))
+                if rng.random() < 0.2:
+                    ops.append(("D", int(rng.integers(1, 3))))
+            elif u < 0.7:
+                ops.append(("I", int(rng.integers(1, 4))))
+                if rng.random() < 0.2:
+                    ops.append(("I", int(rng.integers(1, 3))))
+            elif u < 0.8:
+                ops.append(("P", 1))
+                ops.append(("I", int(rng.integers(1, 3))))
+            elif u < 0.9:
+                ops.append(("N", int(rng.integers(2, 8))))
+            else:
+                ops.append(("I", 1))
+                ops.append(("D", 1))
+        if rng.random() < 0.3:
+            ops.append(("S", int(rng.integers(1, 6))))
+        span = sum(l for o, l in ops if o in "MDN=X")
+        if span >= target_len:
+            continue
+        pos = int(rng.integers(0, target_len - span + 1))
+        if rng.random() < 0.03:
+            pos = target_len - span   # ends at the last base of the target
+        seq = []
+        rp = pos
+        for o, l in ops:
+            if o in "M=X":
+                for k in range(l):
+                    b = ref[rp + k]
+                    if o == "X" or (o == "M" and rng.random() < 0.05):
+                        b = str(rng.choice([c for c in "ACGT" if c != b]))
+                    seq.append(b)
+                rp += l
+            elif o in "IS":
+                seq.extend(str(c) for c in rng.choice(list("ACGT"), size=l))
+            elif o in "DN":
+                rp += l
+        seq = ["N" if rng.random() < 0.01 else b for b in seq]
+        q = [int(x) for x in rng.integers(2, 42, size=len(seq))]
+        d = dict(tid=t, pos=pos, cigar="".join("%d%s" % (l, o) for o, l in ops), seq="".join(seq), qual=q,
+                 flag=16 if rng.random() < 0.5 else 0, mapq=42 if rng.random() < 0.9 else int(rng.integers(0, 42)))
+        if rng.random() < 0.08:
+            d["x1"] = int(rng.integers(2, 4))
+        if rng.random() < 0.5:
+            d["xl"] = int(rng.integers(0, 6))
+            d["xr"] = int(rng.integers(0, 6))
+        reads.append(d)
+    reads.sort(key=lambda r: (r["tid"], r["pos"]))
+    return refs, reads

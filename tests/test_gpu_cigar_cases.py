@@ -1,0 +1,55 @@
+"""GPU parity on hand-built alignment sets covering every CIGAR shape the pileup distinguishes (soft
+clips, adjacent insertions/deletions, reference skips, pads, =/X runs, N bases, both strands,
+redundant reads, reads ending at a target end), against the oracle.  Integer outputs bit-exact;
+GD lines identical up to one unit in the last printed digit of FP fields."""
+import sys
+
+import numpy as np
+import pytest
+
+import breseq_amd as B
+import oracle_lib as O
+from aln_builder import Built, BuiltSample, random_alignments
+from parity_util import compare_gd
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_error_count_cigar_shapes_bit_exact(seed):
+    say("[gpu] error_count cigar shapes seed", seed)
+    refs, reads = random_alignments(seed, n_reads=900)
+    b = Built(refs, reads)
+    s = BuiltSample(b)
+    args = B.api.make_error_count_args(s)
+    h_o, c_o, l_o, texts = O.error_count(args)
+    res = B.api.error_count(s)
+    assert len(res.histograms) == len(h_o)
+    for g in range(len(h_o)):
+        assert np.array_equal(res.histograms[g], h_o[g]), g
+    assert np.array_equal(res.counts, c_o.astype(np.uint64))
+    assert res.rates_text == texts["error_rates"]
+    assert res.counts.sum() > 0
+
+
+@pytest.mark.parametrize("seed,mode", [(1, B.CONSENSUS), (2, B.POLYMORPHISM), (3, B.CONSENSUS)])
+def test_identify_cigar_shapes_match_oracle(seed, mode):
+    say("[gpu] identify cigar shapes seed", seed, mode)
+    refs, reads = random_alignments(seed, n_reads=900)
+    b = Built(refs, reads)
+    s = BuiltSample(b)
+    args = B.api.make_error_count_args(s)
+    _, _, l_o, texts = O.error_count(args)
+    table = O.read_log10_prob_table(texts["error_rates"], len(l_o))
+    prop = [3.0] * s.n_targets
+    iargs = B.api.make_identify_args(s, table, args.read_set_max, args.quality_max, prop, mode=mode)
+    gd_o, _ = O.identify_mutations(iargs)
+    gd_g = B.api.identify_mutations(s, table, args.read_set_max, args.quality_max, prop, mode=mode)
+    n, same, errs = compare_gd(gd_g, gd_o)
+    say("[gpu]   entries", n, "identical", same, errs)
+    assert n > 10

@@ -14,6 +14,7 @@ OBJ_DIR = os.path.join(HERE, "_build")
 
 SOURCES = [
     "error_count.hip",
+    "identify_columns.hip",
     "kernels.hip",
     "session.hip",
     "extra.hip",

@@ -209,4 +209,25 @@ __device__ __forceinline__ bool slot_observation(const DRead& R, const uint8_t* 
   return true;
 }
 
+// pure_genotype_call (identify_mutations.cpp:3398-3433)
+__device__ __forceinline__ void pure_genotype(const double lps[5], uint32_t n, int& best, double& score) {
+  best = 5;
+  score = __longlong_as_double(0x7ff8000000000000LL);
+  if (n == 0) return;
+  best = 0;
+  for (int b = 1; b < 5; b++)
+    if (lps[b] > lps[best]) best = b;
+  double off = -1.7976931348623157e308;
+  for (int b = 0; b < 5; b++)
+    if (b != best) off = fmax(off, lps[b]);
+  double tot = 0.0;
+  // kept rolled: one pow at a time keeps the register footprint of the callers' loops small
+#pragma unroll 1
+  for (int b = 0; b < 5; b++)
+    if (b != best) tot += pow(10.0, lps[b] - off);
+  double lt = log10(tot);
+  lt += off;
+  score = lps[best] - lt;
+}
+
 }  // namespace bsq

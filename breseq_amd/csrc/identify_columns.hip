@@ -1,0 +1,423 @@
+// identify_columns.hip -- pass 2, insert slot 0 of every reference column, on CDNA4 (gfx950):
+// coverage tallies, the per-genotype log10 likelihood sums, the pure-genotype call and a
+// conservative screen that routes columns to the exact FP64 scorer
+// (identify_mutations_pileup::pileup_callback, identify_mutations.cpp:1561-1821).
+//
+// One 64-wide wavefront owns 64 consecutive columns (lane = column) and walks the alignments that
+// overlap them in file order.  Every lane sees the same alignment in the same iteration, so the
+// 40-byte record is a scalar load and the bases and qualities of consecutive columns are
+// consecutive bytes of that alignment (one coalesced load each).  Each lane adds its column's FP64
+// likelihood terms in file order, which is the reference's summation order: the sums, the
+// pure-genotype score and the predicted/consensus decisions are bit-identical to the reference's.
+//
+// Per alignment the lane work is branch-light: filtered lanes (no entry, N base, trimmed, low
+// quality) read the all-zero table row instead of branching; the strand permutation of the
+// hypotheses is a wave-uniform branch; redundant alignments take a wave-uniform path that only
+// adds 1/X1.  The next alignment's record and bytes are loaded while the current one is added.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace bsq {
+
+namespace {
+
+constexpr int kWaveCols = 64;
+
+// BAM 4-bit base code -> A0 C1 G2 T3; every other code -> 15.
+__device__ __forceinline__ uint32_t base_idx4(uint32_t b) {
+  const uint32_t i = (uint32_t)(0xFFFFFFF3FFF2F10FULL >> ((b & 15u) * 4u)) & 15u;
+  return b > 15u ? 15u : i;
+}
+
+// Screening sums of one column (FP32).  Only observations other than the reference base update the
+// per-class sums; the reference class follows from n.
+struct Screen {
+  uint32_t n;          // reads with a likelihood term
+  uint32_t cnt[5];     // per observed class, reference class excluded
+  float SL;            // Sum_i max_b log10 P_i(b)
+  float Dall;          // Sum_i log10 P_i(observed class)
+  float G[5];          // per observed class v: Sum (log10 P_i(reference) - log10 P_i(v))
+  float Gabs;          // Sum |terms of G| (rounding bound)
+};
+
+// What one alignment contributes to one lane's column at insert slot 0.
+struct Entry {
+  bool cover;          // the alignment has a pileup entry at this column
+  bool is_del;
+  int qpos;            // clamped into the read when !cover
+  int ondel;           // -1 deletion, else max(indel, 0)
+  uint32_t b, q;       // base code and quality at qpos (b = '.' for deletions)
+  uint32_t qd;         // deletions: quality of the next read base in read direction
+  bool del_ok;         // deletions: that base exists and is not N
+};
+
+template <bool kLds>
+struct Table;
+
+template <>
+struct Table<true> {
+  const FoldRow* rows;   // in LDS
+  __device__ __forceinline__ const FoldRow& operator[](int i) const {
+    typedef __attribute__((address_space(3))) const FoldRow LdsRow;
+    return *(const FoldRow*)&((LdsRow*)rows)[i];
+  }
+};
+
+template <>
+struct Table<false> {
+  const FoldRow* __restrict__ rows;   // in HBM (L2-resident)
+  __device__ __forceinline__ const FoldRow& operator[](int i) const { return rows[i]; }
+};
+
+}  // namespace
+
+// Screen (identify_mutations.cpp:3240-3344 bounded from above): true unless no allele v can reach
+// the presence-score cutoff.
+//   * The full model's log likelihood is at most Sum_i max_b log10 P_i(b) = SL.
+//   * The null model (v held out) starts EM from Laplace-smoothed counts f0 and EM never lowers the
+//     likelihood, so its value is at least the likelihood at f0, which per read is at least
+//     log10 f0(c) + log10 P_i(c) for any held-in class c: the read's own class, or the reference
+//     class for reads that observe v.  Summed: Sum_{b!=v} cnt_b log10 f0_b + cnt_v log10 f0_ref
+//     + Dall + G[v].
+// The FP32 rounding of every sum is bounded explicitly and added to the margin.  Columns whose
+// reference base is N go to the exact scorer whenever they have reads.
+__device__ __forceinline__ bool screen_passes(const Screen& A, int ref_idx, double log10_ref_len, double cutoff,
+                                              double margin) {
+  if (A.n == 0) return false;
+  if (ref_idx < 0 || ref_idx > 4) return true;
+  // class counts; the reference class is what the other classes leave of n
+  float cnt[5], Lb[5];
+  uint32_t others = 0;
+#pragma unroll
+  for (int b = 0; b < 5; b++) others += (b == ref_idx) ? 0u : A.cnt[b];
+  float sumL = 0.0f, magL = 0.0f, Lref = 0.0f;
+#pragma unroll
+  for (int b = 0; b < 5; b++) {
+    cnt[b] = (float)((b == ref_idx) ? A.n - others : A.cnt[b]);
+    // log10 f0_b = log10(0.5 + cnt_b) - log10(init_total)
+    Lb[b] = log10f(0.5f + cnt[b]);
+    sumL += cnt[b] * Lb[b];
+    magL += cnt[b] * fabsf(Lb[b]);
+    if (b == ref_idx) Lref = Lb[b];
+  }
+  const float n = (float)A.n;
+  const float u = 5.9604645e-08f;   // 2^-24
+  const float mag0 = fabsf(A.SL) + fabsf(A.Dall) + A.Gabs + magL;
+#pragma unroll 1
+  for (int v = 0; v < 5; v++) {
+    if (v == ref_idx) continue;
+    const float Li = log10f(2.0f + (n - cnt[v]));
+    // Sum_{b!=v} cnt_b log10 f0_b + cnt_v log10 f0_ref, regrouped
+    const float lb = A.Dall + A.G[v] + (sumL - cnt[v] * Lb[v]) + cnt[v] * Lref - n * Li;
+    const float mag = mag0 + n * fabsf(Li) + cnt[v] * (fabsf(Lb[v]) + fabsf(Lref));
+    // every FP32 term carries at most (n + 16) u of relative error through its sums, the
+    // conversions from FP64 and log10f; doubled for safety
+    const double bound = 2.0 * (double)(n + 16.0f) * (double)u * (double)mag;
+    const double ub = (double)A.SL - (double)lb - log10_ref_len;
+    if (ub >= cutoff - margin - bound) return true;
+  }
+  return false;
+}
+
+// Entry of alignment R at column p for the general (non-simple) CIGAR path; loads issued here.
+__device__ __forceinline__ void prepare_general(const ImParams& P, const DRead& R, int p, Entry& e, uint32_t& err) {
+  int qpos = 0, indel = 0;
+  bool is_del = false;
+  e.cover = entry_at(R, P.cigar, p, qpos, indel, is_del, err);
+  e.is_del = e.cover && is_del;
+  e.ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
+  e.qpos = e.cover ? qpos : 0;
+  const uint8_t* sq = P.bases + R.seq_off;
+  const uint8_t* ql = P.quals + R.seq_off;
+  e.b = '.';
+  e.q = 0;
+  e.qd = 0;
+  e.del_ok = false;
+  if (e.cover && !is_del) {
+    e.b = sq[e.qpos];
+    e.q = ql[e.qpos];
+  } else if (e.is_del) {
+    // deletion: quality of the next read base in read direction (error_count.cpp:1075-1080)
+    const int rev = (R.flags & kReadReversed) ? 1 : 0;
+    const int qq = qpos + 1 - rev;
+    if (qq < 0 || qq >= (int)R.l_qseq) {
+      err |= kErrQueryIndex;
+    } else if (sq[qq] != 15) {
+      e.qd = ql[qq];
+      e.del_ok = true;
+    }
+  }
+}
+
+// Score of pure_genotype_call (identify_mutations.cpp:3398-3433) in FP32 transcendentals.  best is
+// exact (FP64 argmax, as the reference).  The reference's score is lps[best] - (log10(tot) + off)
+// with tot = Sum_{b != best} 10^(lps[b] - off) in [1, 4]; here each 10^d (d <= 0) is evaluated from
+// (float)d, so a term's relative error is at most ln(10) * 2^-24 * |d| + 2 ulp, below 6e-6 for the
+// terms that do not underflow (|d| < 40), and those that do are below 1e-38 of tot.  The score is
+// therefore within 3e-6 of the reference's; decisions closer than kScoreMargin to a threshold are
+// left to the exact FP64 evaluation of k_fix_columns.
+constexpr double kScoreMargin = 1e-4;
+
+__device__ __forceinline__ double genotype_score_approx(const double lps[5], int best) {
+  double off = -1.7976931348623157e308;
+#pragma unroll
+  for (int b = 0; b < 5; b++)
+    if (b != best) off = fmax(off, lps[b]);
+  float tot = 0.0f;
+#pragma unroll
+  for (int b = 0; b < 5; b++)
+    if (b != best) tot += exp10f((float)(lps[b] - off));
+  return (lps[best] - off) - (double)log10f(tot);
+}
+
+__device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, int ref_idx, const double lps[5],
+                                              double red_bot, double red_top, int uniq, bool insert_slot,
+                                              const Screen& A, uint32_t& err) {
+  // position_coverage::sum (identify_mutations.h:115-120)
+  const double r1 = red_bot + red_top;
+  const int total = (int)round((double)uniq) + (int)round(r1);
+  if (ref_idx < 0) err |= kErrRefChar;
+  uint8_t fl = kPosValid;
+  bool cons = false, cons_unsure = false;
+  if (A.n > 0) {
+    int best = 0;
+#pragma unroll
+    for (int b = 1; b < 5; b++)
+      if (lps[b] > lps[best]) best = b;
+    const double cbs = genotype_score_approx(lps, best) - P.log10_ref_length;
+    const bool pred_unsure = fabs(cbs - P.mutation_cutoff) <= kScoreMargin;
+    if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
+    if (best != ref_idx) {
+      cons = cbs > 0.0;
+      cons_unsure = fabs(cbs) <= kScoreMargin;
+    }
+    if (cons) fl |= kPosConsensus;
+    if (pred_unsure || cons_unsure) {
+      const int k = atomicAdd(P.n_fix, 1);
+      if (k < P.fix_cap) {
+        ColFix f;
+        f.p = p;
+        f.n = A.n;
+#pragma unroll
+        for (int b = 0; b < 5; b++) f.lps[b] = lps[b];
+        P.fix[k] = f;
+      } else {
+        err |= kErrCapacity;
+      }
+    }
+  }
+  if (r1 > 0.0) fl |= kPosRedundant;
+  if (insert_slot) fl |= kPosInsert;
+  if ((double)total <= P.targets[t].seed_cutoff) fl |= kPosSeed;
+  // an uncertain consensus call goes to the exact scorer, which makes the call itself
+  const bool cand = cons || cons_unsure || (P.screen_off && A.n > 0) ||
+                    screen_passes(A, ref_idx, P.log10_ref_length, P.polymorphism_cutoff, P.screen_margin);
+  if (cand) {
+    fl |= kPosCandidate;
+    const int slot = atomicAdd(P.n_cand, 1);
+    if (slot < P.cand_cap) P.cand[slot] = make_int2(p, 0);
+    else err |= kErrCapacity;
+  }
+  if (insert_slot) {
+    const int slot = atomicAdd(P.n_ins, 1);
+    if (slot < P.ins_cap) P.ins[slot] = p;
+    else err |= kErrCapacity;
+  }
+  P.col_flags[p] = fl;
+  P.col_unique[p] = uniq;
+  P.col_total[p] = total;
+}
+
+// Exact pure-genotype decisions for the columns the column kernel left uncertain (thread each).
+__global__ void __launch_bounds__(256) k_fix_columns(ImParams P) {
+  const int n = min(*P.n_fix, P.fix_cap);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const ColFix f = P.fix[i];
+    int best;
+    double score;
+    pure_genotype(f.lps, f.n, best, score);
+    const double cbs = score - P.log10_ref_length;
+    const int ref_idx = char_index(P.ref[f.p]);
+    uint8_t fl = P.col_flags[f.p] & (uint8_t)~(kPosPredicted | kPosConsensus);
+    if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
+    if ((best != ref_idx) && !isnan(cbs) && (cbs > 0.0)) fl |= kPosConsensus;
+    P.col_flags[f.p] = fl;
+  }
+}
+
+template <bool kLds>
+__global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams P) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Table<kLds> tab;
+  if constexpr (kLds) {
+    FoldRow* lds_rows = (FoldRow*)smem_raw;
+    const int n4 = (P.fold_rows + 1) * (int)(sizeof(FoldRow) / 16);
+    const uint4* src = (const uint4*)P.fold;
+    uint4* dst = (uint4*)lds_rows;
+    for (int i = threadIdx.x; i < n4; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+    tab.rows = lds_rows;
+  } else {
+    tab.rows = P.fold;
+  }
+  const int zero_row = P.fold_rows;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int64_t chunk = (P.n_windows + gridDim.x - 1) / gridDim.x;
+  const int64_t w_begin = (int64_t)blockIdx.x * chunk;
+  const int64_t w_end = min<int64_t>(w_begin + chunk, P.n_windows);
+  uint32_t err = 0;
+  const int S = P.read_set_max, Q = P.quality_max;
+  const int qcut = P.base_quality_cutoff;
+
+  for (int64_t w = w_begin + wave; w < w_end; w += kColWaves) {
+    const int p = (int)(w * kWaveCols + lane);
+    const int t = (p < P.n_pos) ? P.pos_target[p] : -1;
+    const bool lane_ok = (t >= 0) && P.targets[t].scored && (P.targets[t].prop_cutoff >= 0.0);
+    const int ref_idx = lane_ok ? char_index(P.ref[p]) : 0;
+    // hypothesis index of the reference base for a forward / reverse read
+    const int e_top = (ref_idx >= 0 && ref_idx < 5) ? ref_idx : 0;
+    const int e_bot = (e_top < 4) ? 3 - e_top : e_top;
+    const int rb = __builtin_amdgcn_readfirstlane(P.win_rb[w]);
+    const int re = __builtin_amdgcn_readfirstlane(P.win_re[w]);
+    double lps0 = 0.0, lps1 = 0.0, lps2 = 0.0, lps3 = 0.0, lps4 = 0.0;
+    double red_bot = 0.0, red_top = 0.0;
+    int uniq = 0;
+    bool insert_slot = false;
+    Screen A;
+    A.n = 0;
+    A.SL = 0.0f;
+    A.Dall = 0.0f;
+    A.Gabs = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      A.cnt[i] = 0;
+      A.G[i] = 0.0f;
+    }
+
+    // pipeline: the record and bytes of alignment r+1 are requested before alignment r is added
+    DRead Rn;
+    Entry En;
+    auto prepare = [&](int r) {
+      Rn = load_read(P.reads, r);
+      if (Rn.flags & kReadSimple) {
+        const int dp = p - Rn.gpos;
+        En.cover = lane_ok && (unsigned)dp < (unsigned)(Rn.gend - Rn.gpos);
+        En.is_del = false;
+        En.ondel = 0;
+        En.qpos = En.cover ? (int)Rn.y0 + dp : 0;
+        En.b = P.bases[Rn.seq_off + En.qpos];
+        En.q = P.quals[Rn.seq_off + En.qpos];
+        En.qd = 0;
+        En.del_ok = false;
+      } else {
+        prepare_general(P, Rn, p, En, err);
+        En.cover = En.cover && lane_ok;
+      }
+    };
+    if (rb < re) prepare(rb);
+    for (int r = rb; r < re; r++) {
+      const DRead R = Rn;
+      const Entry E = En;
+      if (r + 1 < re) prepare(r + 1);
+      const bool top = !(R.flags & kReadReversed);
+      const uint32_t base = E.is_del ? (uint32_t)'.' : E.b;
+      const bool valid = E.cover && base != 15u;
+      if (R.x1 != 1) {
+        // coverage of a read with X1 != 1 is 1/X1 (identify_mutations.cpp:1606-1614); a redundant
+        // read (X1 > 1) adds nothing else
+        const double inv = 1.0 / (double)R.x1;
+        if (top) red_top += valid ? inv : 0.0;
+        else red_bot += valid ? inv : 0.0;
+        if (R.x1 > 1) continue;
+      } else {
+        uniq += valid ? 1 : 0;
+        insert_slot = insert_slot || (valid && E.ondel > 0);
+      }
+      // is_trimmed (alignment.h:389-410)
+      bool trimmed = false;
+      {
+        const uint32_t q1 = (uint32_t)E.qpos + 1u;
+        if (R.xl >= 0 && q1 <= (uint32_t)R.xl) trimmed = true;
+        if (R.xr >= 0) {
+          const uint32_t rem = (uint32_t)R.l_qseq - q1;
+          if (rem + 1u <= (uint32_t)R.xr) trimmed = true;
+          if (E.ondel < 0 && rem == (uint32_t)R.xr) trimmed = true;
+        }
+      }
+      // observed class (reference strand) and quality (error_count.cpp:1049-1105)
+      uint32_t obs, q;
+      bool ok = valid && !trimmed;
+      if (E.is_del) {
+        obs = 4u;
+        q = E.qd;
+        ok = ok && E.del_ok;
+      } else {
+        obs = base_idx4(E.b);
+        q = E.q;
+        if (ok && obs > 3u) {
+          err |= kErrBaseCode;
+          ok = false;
+        }
+      }
+      ok = ok && q >= (uint32_t)qcut;
+      if (ok && (int)R.rfile >= S) {
+        err |= kErrReadSet;
+        ok = false;
+      }
+      if (ok && q >= (uint32_t)Q) {
+        err |= kErrQuality;
+        ok = false;
+      }
+      const uint32_t o_rs = (obs < 4u && !top) ? (obs ^ 3u) : obs;
+      const int row = ok ? ((((int)R.mslot * S + (int)R.rfile) * Q + (int)q) * 5 + (int)o_rs) : zero_row;
+      const FoldRow& T = tab[row];
+      const double t0 = T.t[0], t1 = T.t[1], t2 = T.t[2], t3 = T.t[3], t4 = T.t[4];
+      if (top) {
+        lps0 += t0;
+        lps1 += t1;
+        lps2 += t2;
+        lps3 += t3;
+      } else {
+        lps0 += t3;
+        lps1 += t2;
+        lps2 += t1;
+        lps3 += t0;
+      }
+      lps4 += t4;
+      const float dg = T.diag;
+      A.SL += T.lpmax;
+      A.Dall += dg;
+      A.n += ok ? 1u : 0u;
+      if (ok && (int)obs != ref_idx) {
+        const float e = (float)T.t[top ? e_top : e_bot];
+        const float g = e - dg;
+        A.Gabs += fabsf(e) + fabsf(dg);
+#pragma unroll
+        for (int o = 0; o < 5; o++) {
+          if ((int)obs == o) {
+            A.G[o] += g;
+            A.cnt[o] += 1u;
+          }
+        }
+      }
+    }
+    if (!lane_ok) {
+      if (p < P.n_pos) {
+        P.col_flags[p] = 0;
+        P.col_unique[p] = 0;
+        P.col_total[p] = 0;
+      }
+    } else {
+      const double lps[5] = {lps0, lps1, lps2, lps3, lps4};
+      finish_column(P, p, t, ref_idx, lps, red_bot, red_top, uniq, insert_slot, A, err);
+    }
+  }
+  if (err) atomicOr(P.err, err);
+}
+
+template __global__ void k_identify_columns_t<true>(ImParams P);
+template __global__ void k_identify_columns_t<false>(ImParams P);
+
+}  // namespace bsq

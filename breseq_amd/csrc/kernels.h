@@ -46,6 +46,24 @@ struct CovParams {
 };
 constexpr int kCovTile = 4096;      // columns per block of the coverage scan (256 threads x 16)
 
+// One row of the strand-folded likelihood table: a read of quality q from read file rs with
+// MAPQ slot m observing read-strand base o reads row ((m*S + rs)*Q + q)*5 + o.  Hypotheses are in
+// read-strand order; a reverse-strand read maps hypothesis b to t[comp(b)].
+struct FoldRow {
+  double t[5];    // log10 P(o | hypothesis), fill_read_base_likelihoods (identify_mutations.cpp:3359-3384)
+  float lpmax;    // max_b t[b]
+  float diag;     // t[o]: the read's own observation as the hypothesis
+};
+static_assert(sizeof(FoldRow) == 48, "FoldRow is three 16-byte LDS words");
+
+// A column whose pure-genotype decision the column kernel could not certify with its FP32
+// score: the exact FP64 call is made from these sums by k_fix_columns.
+struct ColFix {
+  int32_t p;
+  uint32_t n;
+  double lps[5];
+};
+
 struct ImParams {
   const DRead* reads;
   const uint32_t* cigar;
@@ -62,8 +80,8 @@ struct ImParams {
   int32_t base_quality_cutoff;
   const double* tab_lp;         // per key: log10 P(obs | true base b), b = 0..4
   const double* tab_r;          // per key: 10^(lp - lpmax) for b = 0..4, then lpmax
-  const double* fold_lp;        // strand-folded rows (mslot, read file, quality, read-strand obs) x 5
-  const float* fold_max;        // per folded row: max_b log10 P
+  const FoldRow* fold;          // strand-folded rows (mslot, read file, quality, read-strand obs);
+                                // row fold_rows is all zeros (the row of a read that adds nothing)
   int32_t fold_rows;
   int32_t fold_lds;             // 1: stage the folded table in LDS
   double log10_ref_length;
@@ -78,7 +96,6 @@ struct ImParams {
   int32_t* col_total;
   // candidates
   int2* cand;                   // (global column, insert slot)
-  uint8_t* cand_cons;           // consensus test passed (computed from the summed likelihoods)
   int32_t* n_cand;
   int32_t cand_cap;
   int32_t* ins;                 // columns with insert slots
@@ -86,6 +103,10 @@ struct ImParams {
   int32_t ins_cap;
   int32_t key_cap;              // LDS capacity of the scorer's read list
   RaOut* ra_out;
+  // columns whose predicted / consensus decision lies within the approximation margin
+  ColFix* fix;
+  int32_t* n_fix;
+  int32_t fix_cap;
   uint32_t* err;
 };
 
@@ -110,43 +131,16 @@ struct RunParams {
   uint32_t* err;
 };
 
-// Screening sums of one column (FP32: the screen compares an upper bound against a margin).
-struct Screen {
-  uint32_t n;
-  uint32_t cnt[5];
-  float SL;     // Sum_i max_b log10 P_i(b)
-  float D[5];   // D[o] = Sum over reads observing o of log10 P_i(o)
-  float E[5];   // E[o] = Sum over reads observing o of log10 P_i(reference base)
-  __device__ void reset() {
-    n = 0;
-    SL = 0.0f;
-    for (int i = 0; i < 5; i++) {
-      cnt[i] = 0;
-      D[i] = 0.0f;
-      E[i] = 0.0f;
-    }
-  }
-  __device__ void add(int obs, const double l[5], float lpmax, int ref) {
-    n++;
-    SL += lpmax;
-    const float lr = (float)(ref == 0 ? l[0] : ref == 1 ? l[1] : ref == 2 ? l[2] : ref == 3 ? l[3] : l[4]);
-#pragma unroll
-    for (int o = 0; o < 5; o++) {
-      const bool m = (o == obs);
-      cnt[o] += m ? 1u : 0u;
-      D[o] += m ? (float)l[o] : 0.0f;
-      E[o] += m ? lr : 0.0f;
-    }
-  }
-};
-
-
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v);
 __global__ void k_ec_reads(EcParams P);
 __global__ void k_cov_tile_sums(CovParams C);
 __global__ void k_cov_tile_scan(CovParams C);
 __global__ void k_cov_hist(CovParams C);
-__global__ void k_identify_columns(ImParams P);
+constexpr int kColWaves = 8;                 // wavefronts per block of the column kernel
+constexpr size_t kFoldLdsBytes = 48 * 1024;  // largest likelihood table staged in LDS
+template <bool kLds>
+__global__ void k_identify_columns_t(ImParams P);
+__global__ void k_fix_columns(ImParams P);
 __global__ void k_insert_slots(ImParams P);
 __global__ void k_score_candidates(ImParams P);
 __global__ void k_runs(RunParams P);

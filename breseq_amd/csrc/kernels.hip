@@ -1,10 +1,6 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the pileup-scoring engine.
 //
-//  k_identify_columns pass 2, insert slot 0 of every column: coverage tallies, per-base counts,
-//                     summed per-genotype log10 likelihoods in read order, pure-genotype call, and
-//                     a conservative screen that routes columns to the exact FP64 scorer
-//                     (identify_mutations.cpp:1309-1821)
-//  k_insert_slots     insert slots >= 1 of columns whose unique reads carry insertions
+//  k_insert_slots     enumerates insert slots >= 1 of columns whose unique reads carry insertions
 //  k_score_candidates wave per candidate: allele-frequency EM, presence score, profile-likelihood
 //                     bounds, Fisher strand and KS quality tests (identify_mutations.cpp:3009-3318)
 //  k_runs             MC and UN interval detection (identify_mutations.cpp:2262-2344, 2972-3007)
@@ -25,333 +21,40 @@ namespace bsq {
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-constexpr int kBatch = 4;
-
 /* ------------------------------------------------------------------------------------------ */
-/* Pass 2: per-column tallies at insert slot 0                                                 */
+/* Insert slots >= 1                                                                           */
 
-// pure_genotype_call (identify_mutations.cpp:3398-3433)
-__device__ __forceinline__ void pure_genotype(const double lps[5], uint32_t n, int& best, double& score) {
-  best = 5;
-  score = __longlong_as_double(0x7ff8000000000000LL);
-  if (n == 0) return;
-  best = 0;
-  for (int b = 1; b < 5; b++)
-    if (lps[b] > lps[best]) best = b;
-  double off = -1.7976931348623157e308;
-  for (int b = 0; b < 5; b++)
-    if (b != best) off = fmax(off, lps[b]);
-  double tot = 0.0;
-  for (int b = 0; b < 5; b++)
-    if (b != best) tot += pow(10.0, lps[b] - off);
-  double lt = log10(tot);
-  lt += off;
-  score = lps[best] - lt;
-}
-
-// Conservative screen: true unless no allele can reach the presence-score cutoff.
-//   * The full model's reported log likelihood is at most Sum_i max_b log10 P_i(b) (= SL).
-//   * The null model (allele v held out) starts EM from the Laplace-smoothed counts and EM never
-//     lowers the likelihood, so its reported value is at least the likelihood at that start, which
-//     per read is at least log10 f0(c) + log10 P_i(c) for any held-in allele c: the read's own
-//     observed allele (sums D[b]), or the reference allele for reads that observe v (sums E[v]).
-// Accumulated in FP32; the margin covers the rounding.  Columns whose reference base is N go to
-// the exact scorer whenever they have reads.
-__device__ __forceinline__ bool screen_passes(const Screen& A, int ref_idx, double log10_ref_len, double cutoff,
-                                              double margin) {
-  if (A.n == 0) return false;
-  if (ref_idx < 0 || ref_idx > 4) return true;
-  const float n = (float)A.n;
-#pragma unroll
-  for (int v = 0; v < 5; v++) {
-    if (v == ref_idx) continue;
-    const float init_total = 2.0f + (n - (float)A.cnt[v]);
-    const float inv = 1.0f / init_total;
-    float lb = 0.0f, lf_ref = 0.0f;
-#pragma unroll
-    for (int b = 0; b < 5; b++) {
-      const float lf = log10f((0.5f + (float)A.cnt[b]) * inv);
-      if (b == ref_idx) lf_ref = lf;
-      if (b == v) continue;
-      lb += (float)A.cnt[b] * lf + A.D[b];
-    }
-    if (A.cnt[v] > 0) lb += (float)A.cnt[v] * lf_ref + A.E[v];
-    const double ub = (double)A.SL - (double)lb - log10_ref_len;
-    if (ub >= cutoff - margin) return true;
-  }
-  return false;
-}
-
-// log10 P(observed base | true base b) for b = 0..4 and its maximum, from the strand-folded table:
-// rows are indexed by the read-strand observation, columns by the read-strand hypothesis.
-__device__ __forceinline__ void lookup_lp(const double* __restrict__ tab, const float* __restrict__ tmax, int S, int Q,
-                                          const DRead& R, int q, int obs, bool top, double l[5], float& lpmax) {
-  const int o_rs = top ? obs : (int)comp_index((uint8_t)obs);
-  const int row = (((int)R.mslot * S + R.rfile) * Q + q) * 5 + o_rs;
-  const double* T = tab + (size_t)row * 5;
-  if (top) {
-    l[0] = T[0];
-    l[1] = T[1];
-    l[2] = T[2];
-    l[3] = T[3];
-    l[4] = T[4];
-  } else {
-    l[0] = T[3];
-    l[1] = T[2];
-    l[2] = T[1];
-    l[3] = T[0];
-    l[4] = T[4];
-  }
-  lpmax = tmax[row];
-}
-
-// Stage the folded likelihood table in LDS when it fits; returns the pointers to use.
-__device__ __forceinline__ void stage_table(const ImParams& P, double* smem, const double*& tab, const float*& tmax) {
-  tab = P.fold_lp;
-  tmax = P.fold_max;
-  if (P.fold_lds) {
-    const int n_lp = P.fold_rows * 5;
-    float* smax = (float*)(smem + n_lp);
-    for (int i = threadIdx.x; i < n_lp; i += blockDim.x) smem[i] = P.fold_lp[i];
-    for (int i = threadIdx.x; i < P.fold_rows; i += blockDim.x) smax[i] = P.fold_max[i];
-    __syncthreads();
-    tab = smem;
-    tmax = smax;
-  }
-}
-
-__device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, int ref_idx, const double lps[5],
-                                           double red_bot, double red_top, int uniq_bot, int uniq_top,
-                                           bool insert_slot, const Screen& A, uint32_t& err);
-
-__global__ void __launch_bounds__(256) k_identify_columns(ImParams P) {
-  extern __shared__ double smem[];
-  const double* tab;
-  const float* tmax;
-  stage_table(P, smem, tab, tmax);
-  const int tid = threadIdx.x;
-  const int wave = tid / WAVE, lane = tid % WAVE;
-  const int waves_per_block = blockDim.x / WAVE;
-  const int64_t chunk = (P.n_windows + gridDim.x - 1) / gridDim.x;
-  const int64_t w_begin = (int64_t)blockIdx.x * chunk;
-  const int64_t w_end = min<int64_t>(w_begin + chunk, P.n_windows);
-  uint32_t err = 0;
-  const int S = P.read_set_max, Q = P.quality_max;
-
-  for (int64_t w = w_begin + wave; w < w_end; w += waves_per_block) {
-    const int p = (int)(w * WAVE + lane);
-    const int t = (p < P.n_pos) ? P.pos_target[p] : -1;
-    const bool lane_ok = (t >= 0) && P.targets[t].scored && (P.targets[t].prop_cutoff >= 0.0);
-    const int ref_idx = lane_ok ? char_index(P.ref[p]) : 0;
-    const int scr_ref = (ref_idx >= 0 && ref_idx < 5) ? ref_idx : 0;
-    const int rb = uniform_i(P.win_rb[w]), re = uniform_i(P.win_re[w]);
-    double lps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double red_bot = 0.0, red_top = 0.0;
-    int uniq_bot = 0, uniq_top = 0;
-    bool insert_slot = false;
-    Screen A;
-    A.reset();
-    // Batches of kBatch alignments: records and entries first, then the byte loads of the whole
-    // batch, then the per-read updates in file order (the FP64 sums keep the reference order).
-    for (int r0 = rb; r0 < re; r0 += kBatch) {
-      DRead Rb[kBatch];
-      int qp[kBatch], ind[kBatch];
-      bool dl[kBatch], cv[kBatch];
-#pragma unroll
-      for (int k = 0; k < kBatch; k++) {
-        const int r = uniform_i(r0 + k);
-        cv[k] = false;
-        if (r < re) {
-          Rb[k] = load_read(P.reads, r);
-          cv[k] = lane_ok && entry_at(Rb[k], P.cigar, p, qp[k], ind[k], dl[k], err);
-        }
-      }
-      uint8_t b0[kBatch], q0[kBatch];
-#pragma unroll
-      for (int k = 0; k < kBatch; k++) {
-        b0[k] = '.';
-        q0[k] = 0;
-        if (cv[k] && !dl[k]) {
-          b0[k] = P.bases[Rb[k].seq_off + qp[k]];
-          q0[k] = P.quals[Rb[k].seq_off + qp[k]];
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < kBatch; k++) {
-        if (!cv[k]) continue;
-        const DRead& R = Rb[k];
-        const int qpos = qp[k];
-        const int ondel = dl[k] ? -1 : (ind[k] < 0 ? 0 : ind[k]);
-        const uint8_t base = b0[k];
-        if (base == 15) continue;
-        const bool top = !(R.flags & kReadReversed);
-        const bool trimmed = is_trimmed(R, qpos, ondel < 0);
-        if (R.x1 == 1) {
-          if (top) ++uniq_top;
-          else ++uniq_bot;
-          if (ondel > 0) insert_slot = true;
-        } else {
-          if (top) red_top += 1.0 / (double)R.x1;
-          else red_bot += 1.0 / (double)R.x1;
-        }
-        if (R.x1 > 1 || trimmed) continue;
-        int obs = -1, q = q0[k];
-        if (ondel < 0) {   // deletion: quality of the next read base (error_count.cpp:1075-1080)
-          const int qq = qpos + 1 - (top ? 0 : 1);
-          if (qq < 0 || qq >= (int)R.l_qseq) {
-            err |= kErrQueryIndex;
-            continue;
-          }
-          if (P.bases[R.seq_off + qq] == 15) continue;
-          obs = 4;
-          q = P.quals[R.seq_off + qq];
-        } else {
-          obs = bam_index(base);
-          if (obs < 0) {
-            err |= kErrBaseCode;
-            continue;
-          }
-        }
-        if (q < P.base_quality_cutoff) continue;
-        if ((int)R.rfile >= S) {
-          err |= kErrReadSet;
-          continue;
-        }
-        if (q >= Q) {
-          err |= kErrQuality;
-          continue;
-        }
-        double l[5];
-        float lpmax;
-        lookup_lp(tab, tmax, S, Q, R, q, obs, top, l, lpmax);
-        for (int b = 0; b < 5; b++) lps[b] += l[b];
-        A.add(obs, l, lpmax, scr_ref);
-      }
-    }
-    if (!lane_ok) {
-      if (p < P.n_pos) {
-        P.col_flags[p] = 0;
-        P.col_unique[p] = 0;
-        P.col_total[p] = 0;
-      }
-    } else {
-      finish_column(P, p, t, ref_idx, lps, red_bot, red_top, uniq_bot, uniq_top, insert_slot, A, err);
-    }
-  }
-  if (err) atomicOr(P.err, err);
-}
-
-// Per-column tail of slot 0: coverage sums, pure-genotype call, screen, outputs.
-__device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, int ref_idx, const double lps[5],
-                                           double red_bot, double red_top, int uniq_bot, int uniq_top,
-                                           bool insert_slot, const Screen& A, uint32_t& err) {
-  {
-    // position_coverage::sum (identify_mutations.h:115-120)
-    const double u1 = (double)uniq_bot + (double)uniq_top;
-    const double r1 = red_bot + red_top;
-    const int total = (int)round(u1) + (int)round(r1);
-    int best;
-    double score;
-    pure_genotype(lps, A.n, best, score);
-    const double cbs = score - P.log10_ref_length;
-    if (ref_idx < 0) err |= kErrRefChar;
-    uint8_t fl = kPosValid;
-    if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
-    if (r1 > 0.0) fl |= kPosRedundant;
-    if (insert_slot) fl |= kPosInsert;
-    if ((double)total <= P.targets[t].seed_cutoff) fl |= kPosSeed;   // unique[1] is never NaN here
-    const bool cons = (best != ref_idx) && !isnan(cbs) && (cbs > 0.0);
-    if (cons) fl |= kPosConsensus;
-    bool cand = cons || (P.screen_off && A.n > 0) ||
-                screen_passes(A, ref_idx, P.log10_ref_length, P.polymorphism_cutoff, P.screen_margin);
-    if (cand) {
-      fl |= kPosCandidate;
-      const int slot = atomicAdd(P.n_cand, 1);
-      if (slot < P.cand_cap) {
-        P.cand[slot] = make_int2(p, 0);
-        P.cand_cons[slot] = cons ? 1 : 0;
-      } else {
-        err |= kErrCapacity;
-      }
-    }
-    if (insert_slot) {
-      const int slot = atomicAdd(P.n_ins, 1);
-      if (slot < P.ins_cap) P.ins[slot] = p;
-      else err |= kErrCapacity;
-    }
-    P.col_flags[p] = fl;
-    P.col_unique[p] = uniq_bot + uniq_top;
-    P.col_total[p] = total;
-  }
-}
-
-/* ------------------------------------------------------------------------------------------ */
-/* Insert slots >= 1: one thread per column carrying an insertion                              */
-
+// One wavefront per column whose unique reads carry insertions enumerates the insert slots the
+// reference visits (identify_mutations.cpp:1359-1361, 1598-1599): slot 1 always, and slot s+1 while
+// some unique read with a longer insertion has a base other than N at slot s.  Lanes take the
+// window's alignments.  Every visited slot goes to the exact scorer, which makes the consensus and
+// polymorphism calls itself.
 __global__ void __launch_bounds__(64) k_insert_slots(ImParams P) {
   const int n_ins = min(*P.n_ins, P.ins_cap);
-  const int S = P.read_set_max, Q = P.quality_max;
+  const int lane = threadIdx.x;
   uint32_t err = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_ins; i += gridDim.x * blockDim.x) {
+  for (int i = blockIdx.x; i < n_ins; i += gridDim.x) {
     const int p = P.ins[i];
     const int64_t w = p / WAVE;
     const int rb = P.win_rb[w], re = P.win_re[w];
-    bool next = true;
-    for (int slot = 1; next && slot < 65536; slot++) {
-      next = false;
-      double lps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      Screen A;
-      A.reset();
-      for (int r = rb; r < re; r++) {
+    for (int slot = 1; slot < 65536; slot++) {
+      if (lane == 0) {
+        const int c = atomicAdd(P.n_cand, 1);
+        if (c < P.cand_cap) P.cand[c] = make_int2(p, slot);
+        else err |= kErrCapacity;
+      }
+      bool next = false;
+      for (int r0 = rb; r0 < re; r0 += WAVE) {
+        const int r = r0 + lane;
+        if (r >= re) continue;
         const DRead R = P.reads[r];
+        if (R.x1 != 1) continue;
         int qpos, indel;
         bool is_del;
-        if (!entry_at(R, P.cigar, p, qpos, indel, is_del, err)) continue;
-        const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
-        uint8_t read_base = '.';
-        bool past = true;
-        if (ondel >= slot) {
-          read_base = P.bases[R.seq_off + qpos + slot];
-          past = false;
-        }
-        if (read_base == 15) continue;
-        const bool top = !(R.flags & kReadReversed);
-        const bool trimmed = is_trimmed(R, qpos, past);
-        if (R.x1 == 1 && ondel > slot) next = true;
-        if (R.x1 > 1 || trimmed) continue;
-        int obs, q;
-        if (!slot_observation(R, P.bases, P.quals, qpos, ondel, slot, read_base, obs, q, err)) continue;
-        if (q < P.base_quality_cutoff) continue;
-        if ((int)R.rfile >= S) {
-          err |= kErrReadSet;
-          continue;
-        }
-        if (q >= Q) {
-          err |= kErrQuality;
-          continue;
-        }
-        double l[5];
-        float lpmax;
-        lookup_lp(P.fold_lp, P.fold_max, S, Q, R, q, obs, top, l, lpmax);
-        for (int b = 0; b < 5; b++) lps[b] += l[b];
-        A.add(obs, l, lpmax, 4);
+        if (!entry_at(R, P.cigar, p, qpos, indel, is_del, err) || is_del || indel <= slot) continue;
+        if (P.bases[R.seq_off + qpos + slot] != 15) next = true;
       }
-      int best;
-      double score;
-      pure_genotype(lps, A.n, best, score);
-      const double cbs = score - P.log10_ref_length;
-      const bool cons = (best != 4) && !isnan(cbs) && (cbs > 0.0);   // reference base is '.'
-      const bool cand = cons || (P.screen_off && A.n > 0) ||
-                        screen_passes(A, 4, P.log10_ref_length, P.polymorphism_cutoff, P.screen_margin);
-      if (cand) {
-        const int c = atomicAdd(P.n_cand, 1);
-        if (c < P.cand_cap) {
-          P.cand[c] = make_int2(p, slot);
-          P.cand_cons[c] = cons ? 1 : 0;
-        } else {
-          err |= kErrCapacity;
-        }
-      }
+      if (!__any(next)) break;
     }
   }
   if (err) atomicOr(P.err, err);
@@ -610,7 +313,7 @@ __global__ void __launch_bounds__(64) k_score_candidates(ImParams P) {
             read_base = P.bases[R.seq_off + qpos + slot];
             past = false;
           }
-          if (read_base != 15 && R.x1 == 1 && !is_trimmed(R, qpos, past)) {
+          if (read_base != 15 && R.x1 <= 1 && !is_trimmed(R, qpos, past)) {   // X1 > 1: redundant
             int q;
             if (slot_observation(R, P.bases, P.quals, qpos, ondel, slot, read_base, obs, q, err) &&
                 q >= P.base_quality_cutoff && (int)R.rfile < S && q < Q) {
@@ -641,7 +344,22 @@ __global__ void __launch_bounds__(64) k_score_candidates(ImParams P) {
     const double tol = P.precision_decimal;
     const char refc = (slot == 0) ? P.ref[p] : '.';
     const int ref_idx = char_index(refc) < 0 ? 5 : char_index(refc);
-    const bool cons = P.cand_cons[ci] != 0;
+    // pure_genotype_call on the summed likelihoods of the same reads, in read order
+    // (identify_mutations.cpp:1669-1689, 3398-3433): five lanes each add one hypothesis' terms.
+    if (lane < 5) {
+      double acc = 0.0;
+      for (int i = 0; i < n; i++) acc += P.tab_lp[(size_t)keys[i] * 5 + lane];
+      W.sums[lane] = acc;
+    }
+    __syncthreads();
+    double lps[5];
+    for (int b = 0; b < 5; b++) lps[b] = W.sums[b];
+    __syncthreads();
+    int best;
+    double gscore;
+    pure_genotype(lps, (uint32_t)n, best, gscore);
+    const double cbs = gscore - P.log10_ref_length;
+    const bool cons = (best != ref_idx) && !isnan(cbs) && (cbs > 0.0);
     // full fit
     const bool all[5] = {true, true, true, true, true};
     const AlleleFit full = fit_alleles_counts(keys, obs_count, n, P.tab_r, all, tol, W);

@@ -85,8 +85,7 @@ struct Session {
   DBuf<int32_t> d_cov_diff;       // pass-1 coverage difference arrays (unique, redundant)
   DBuf<int2> d_tile_sum;          // pass-1 coverage scan tile sums
   DBuf<int32_t> d_pos_target, d_win_rb, d_win_re, d_target_group, d_target_scored;
-  DBuf<double> d_fold_lp;
-  DBuf<float> d_fold_max;
+  DBuf<FoldRow> d_fold;
   DBuf<RunRec> d_runrec;
   DBuf<TargetInfo> d_targets;
   DBuf<unsigned long long> d_counts, d_hist;
@@ -95,8 +94,8 @@ struct Session {
   DBuf<uint8_t> d_col_flags;
   DBuf<int32_t> d_col_unique, d_col_total;
   DBuf<int2> d_cand;
-  DBuf<uint8_t> d_cand_cons;
-  DBuf<int32_t> d_counters;   // [0] n_cand, [1] n_ins, [2] n_runs, [3] error bits
+  DBuf<int32_t> d_counters;   // [0] n_cand, [1] n_ins, [2] n_runs, [3] error bits, [4] n_fix
+  DBuf<ColFix> d_fix;
   DBuf<int32_t> d_ins;
   DBuf<RaOut> d_ra;
   DBuf<int2> d_events, d_runs;
@@ -310,7 +309,7 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   HIP_OK(hipMemcpyAsync(S.d_win_rb.p, rb.data(), rb.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
   HIP_OK(hipMemcpyAsync(S.d_win_re.p, re.data(), re.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
   S.d_err.alloc(1);
-  S.d_counters.alloc(4);
+  S.d_counters.alloc(8);
   HIP_OK(hipStreamSynchronize(S.stream));
   S.timing.pack_ms = t1 - t0;
   S.timing.h2d_ms = now_ms() - t2;
@@ -348,12 +347,6 @@ static void check_err(Session& S) {
   raise_err_bits(err);
 }
 
-static int grid_for(int64_t n_windows) {
-  int64_t g = (n_windows + 3) / 4;
-  if (g > 4096) g = 4096;
-  if (g < 1) g = 1;
-  return (int)g;
-}
 
 }  // namespace bsq
 
@@ -515,18 +508,18 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
 // quality, observed base, strand): the per-read cache depends on nothing else, so it is computed
 // once per key on the host with the same libm calls and looked up by the kernels.
 //   lp / rr: keyed by (mslot, read file, quality, top-strand obs, strand) -- the exact scorer
-//   fold / fmax: keyed by (mslot, read file, quality, read-strand obs), hypotheses in read-strand
+//   fold: keyed by (mslot, read file, quality, read-strand obs), hypotheses in read-strand
 //                order; a bottom-strand read reads the row of the complemented observation with
 //                its hypotheses complemented (the reference complements both before the lookup).
 static void build_likelihood_tables(const Session& S, const bsq_identify_args* a, std::vector<double>& lp,
-                                    std::vector<double>& rr, std::vector<double>& fold, std::vector<float>& fmax) {
+                                    std::vector<double>& rr, std::vector<FoldRow>& fold) {
   const int Sm = a->read_set_max, Q = a->quality_max;
   const size_t nm = S.mapq_values.size();
   const size_t nkeys = nm * Sm * Q * 10;
   lp.assign(nkeys * 5, 0.0);
   rr.assign(nkeys * 6, 0.0);
-  fold.assign(nm * Sm * Q * 5 * 5, 0.0);
-  fmax.assign(nm * Sm * Q * 5, 0.0f);
+  // one extra all-zero row: the row a filtered read reads
+  fold.assign(nm * Sm * Q * 5 + 1, FoldRow{{0.0, 0.0, 0.0, 0.0, 0.0}, 0.0f, 0.0f});
   std::vector<double> prob(a->n_bins);
   for (int64_t i = 0; i < a->n_bins; i++) prob[i] = pow(10, a->log10_prob_table[i]);
   auto comp = [](int i) { return i < 4 ? 3 - i : 4; };
@@ -557,8 +550,9 @@ static void build_likelihood_tables(const Session& S, const bsq_identify_args* a
             rr[key * 6 + 5] = mx;
             if (top) {   // read-strand row == top-strand row for a top-strand read
               const size_t row = ((m * Sm + rs) * Q + q) * 5 + obs;
-              for (int b = 0; b < 5; b++) fold[row * 5 + b] = l[b];
-              fmax[row] = (float)mx;
+              for (int b = 0; b < 5; b++) fold[row].t[b] = l[b];
+              fold[row].lpmax = (float)mx;
+              fold[row].diag = (float)l[obs];
             }
           }
         }
@@ -586,17 +580,15 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   }
   S.d_targets.alloc(T.size());
   HIP_OK(hipMemcpyAsync(S.d_targets.p, T.data(), T.size() * sizeof(TargetInfo), hipMemcpyHostToDevice, S.stream));
-  std::vector<double> lp, rr, fold;
-  std::vector<float> fmax;
-  build_likelihood_tables(S, a, lp, rr, fold, fmax);
+  std::vector<double> lp, rr;
+  std::vector<FoldRow> fold;
+  build_likelihood_tables(S, a, lp, rr, fold);
   S.d_tab_lp.alloc(lp.size());
   S.d_tab_r.alloc(rr.size());
-  S.d_fold_lp.alloc(fold.size());
-  S.d_fold_max.alloc(fmax.size());
+  S.d_fold.alloc(fold.size());
   HIP_OK(hipMemcpyAsync(S.d_tab_lp.p, lp.data(), lp.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
   HIP_OK(hipMemcpyAsync(S.d_tab_r.p, rr.data(), rr.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
-  HIP_OK(hipMemcpyAsync(S.d_fold_lp.p, fold.data(), fold.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
-  HIP_OK(hipMemcpyAsync(S.d_fold_max.p, fmax.data(), fmax.size() * sizeof(float), hipMemcpyHostToDevice, S.stream));
+  HIP_OK(hipMemcpyAsync(S.d_fold.p, fold.data(), fold.size() * sizeof(FoldRow), hipMemcpyHostToDevice, S.stream));
   double total_len = 0;
   for (int t = 0; t < S.n_targets; t++) total_len += (double)S.lengths[t];
   S.d_col_flags.alloc(S.n_pos);
@@ -606,10 +598,11 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   const int ins_cap = std::max<int>(1 << 14, S.n_pos / 256);
   const int run_cap = std::max<int>(1 << 16, S.n_pos / 16);
   S.d_cand.alloc(cand_cap);
-  S.d_cand_cons.alloc(cand_cap);
   S.d_ins.alloc(ins_cap);
   S.d_ra.alloc(cand_cap);
   S.d_runrec.alloc(run_cap);
+  const int fix_cap = std::max<int>(1 << 16, S.n_pos / 256);
+  S.d_fix.alloc(fix_cap);
   ImParams P;
   P.reads = S.d_reads.p;
   P.cigar = S.d_cigar.p;
@@ -627,11 +620,10 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   P.base_quality_cutoff = a->base_quality_cutoff;
   P.tab_lp = S.d_tab_lp.p;
   P.tab_r = S.d_tab_r.p;
-  P.fold_lp = S.d_fold_lp.p;
-  P.fold_max = S.d_fold_max.p;
-  P.fold_rows = (int32_t)fmax.size();
-  const size_t fold_bytes = fold.size() * sizeof(double) + fmax.size() * sizeof(float);
-  P.fold_lds = (fold_bytes <= 40 * 1024) ? 1 : 0;
+  P.fold = S.d_fold.p;
+  P.fold_rows = (int32_t)fold.size() - 1;
+  const size_t fold_bytes = fold.size() * sizeof(FoldRow);
+  P.fold_lds = (fold_bytes <= kFoldLdsBytes) ? 1 : 0;
   P.log10_ref_length = log10(total_len);
   P.mutation_cutoff = a->mutation_cutoff;
   P.polymorphism_cutoff = a->polymorphism_cutoff;
@@ -642,7 +634,6 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   P.col_unique = S.d_col_unique.p;
   P.col_total = S.d_col_total.p;
   P.cand = S.d_cand.p;
-  P.cand_cons = S.d_cand_cons.p;
   P.n_cand = S.d_counters.p + 0;
   P.cand_cap = cand_cap;
   P.ins = S.d_ins.p;
@@ -651,6 +642,9 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   P.key_cap = std::max(64, S.max_window_reads);
   P.ra_out = S.d_ra.p;
   P.err = (uint32_t*)(S.d_counters.p + 3);
+  P.fix = S.d_fix.p;
+  P.n_fix = S.d_counters.p + 4;
+  P.fix_cap = fix_cap;
   const size_t key_lds = (size_t)P.key_cap * sizeof(uint32_t);
   if (key_lds > 48 * 1024) throw BsqError(BSQ_ERR_LIMIT, "column depth too large for the exact scorer's LDS list");
   RunParams R;
@@ -664,14 +658,23 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   R.n_runs = S.d_counters.p + 2;
   R.run_cap = run_cap;
   R.err = P.err;
-  HIP_OK(hipMemsetAsync(S.d_counters.p, 0, 4 * sizeof(int32_t), S.stream));
+  HIP_OK(hipMemsetAsync(S.d_counters.p, 0, 8 * sizeof(int32_t), S.stream));
   const double th1 = now_ms();
   // the whole pass is queued without a host round trip: later kernels read the counts on device
   HIP_OK(hipEventRecord(S.ev0, S.stream));
-  hipLaunchKernelGGL(k_identify_columns, dim3(grid_for(S.n_windows)), dim3(256), P.fold_lds ? fold_bytes : 0, S.stream, P);
+  {
+    const int64_t g = std::min<int64_t>(std::max<int64_t>(1, (S.n_windows + kColWaves - 1) / kColWaves), 2048);
+    if (P.fold_lds)
+      hipLaunchKernelGGL(k_identify_columns_t<true>, dim3((int)g), dim3(kColWaves * 64), fold_bytes, S.stream, P);
+    else
+      hipLaunchKernelGGL(k_identify_columns_t<false>, dim3((int)g), dim3(kColWaves * 64), 0, S.stream, P);
+  }
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[0], S.stream));
-  hipLaunchKernelGGL(k_insert_slots, dim3(64), dim3(64), 0, S.stream, P);
+  hipLaunchKernelGGL(k_fix_columns, dim3(64), dim3(256), 0, S.stream, P);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(S.evk[3], S.stream));
+  hipLaunchKernelGGL(k_insert_slots, dim3(1024), dim3(64), 0, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[1], S.stream));
   hipLaunchKernelGGL(k_score_candidates, dim3(4096), dim3(64), key_lds, S.stream, P);
@@ -680,7 +683,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   hipLaunchKernelGGL(k_runs, dim3((S.n_pos + 255) / 256), dim3(256), 0, S.stream, R);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.ev1, S.stream));
-  int32_t counters[4];
+  int32_t counters[8];
   HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
   HIP_OK(hipStreamSynchronize(S.stream));
   float ms = 0;
@@ -688,20 +691,24 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   S.timing.kernel_ms = ms;
   for (double& d : S.timing.kernel_detail_ms) d = 0.0;
   {
-    float k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+    float k0 = 0, k1 = 0, k2 = 0, k3 = 0, kf = 0;
     HIP_OK(hipEventElapsedTime(&k0, S.ev0, S.evk[0]));
-    HIP_OK(hipEventElapsedTime(&k1, S.evk[0], S.evk[1]));
+    HIP_OK(hipEventElapsedTime(&kf, S.evk[0], S.evk[3]));
+    HIP_OK(hipEventElapsedTime(&k1, S.evk[3], S.evk[1]));
     HIP_OK(hipEventElapsedTime(&k2, S.evk[1], S.evk[2]));
     HIP_OK(hipEventElapsedTime(&k3, S.evk[2], S.ev1));
     S.timing.kernel_detail_ms[1] = k0;
     S.timing.kernel_detail_ms[2] = k1;
     S.timing.kernel_detail_ms[3] = k2;
     S.timing.kernel_detail_ms[4] = k3;
+    S.timing.kernel_detail_ms[6] = kf;
+    S.timing.kernel_detail_ms[7] = (double)counters[4];
   }
   const uint32_t errbits = (uint32_t)counters[3];
   if (counters[0] > cand_cap) throw BsqError(BSQ_ERR_LIMIT, "too many candidate columns for the exact scorer");
   if (counters[1] > ins_cap) throw BsqError(BSQ_ERR_LIMIT, "too many columns with insertions");
   if (counters[2] > run_cap) throw BsqError(BSQ_ERR_LIMIT, "too many MC/UN intervals");
+  if (counters[4] > fix_cap) throw BsqError(BSQ_ERR_LIMIT, "too many columns near a pure-genotype threshold");
   raise_err_bits(errbits);
   const int n_cand = counters[0];
   const int n_runs = counters[2];

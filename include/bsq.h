@@ -233,7 +233,8 @@ typedef struct bsq_timing {
   /* per-kernel times of the last pass, HIP events on the session stream:
    * [0] k_ec_reads (with the difference-array clear)  [1] k_identify_columns
    * [2] k_insert_slots  [3] k_score_candidates  [4] k_runs
-   * [5] k_cov_tile_sums + k_cov_tile_scan + k_cov_hist */
+   * [5] k_cov_tile_sums + k_cov_tile_scan + k_cov_hist  [6] k_fix_columns
+   * [7] number of columns k_fix_columns re-decided (a count, not a time) */
   double kernel_detail_ms[8];
 } bsq_timing;
 

@@ -15,6 +15,7 @@ OBJ_DIR = os.path.join(HERE, "_build")
 SOURCES = [
     "error_count.hip",
     "identify_columns.hip",
+    "score_candidates.hip",
     "kernels.hip",
     "session.hip",
     "extra.hip",

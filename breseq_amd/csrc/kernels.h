@@ -102,6 +102,7 @@ struct ImParams {
   int32_t* n_ins;
   int32_t ins_cap;
   int32_t key_cap;              // LDS capacity of the scorer's read list
+  int32_t rows_cap;             // reads whose table rows the scorer caches in LDS
   RaOut* ra_out;
   // columns whose predicted / consensus decision lies within the approximation margin
   ColFix* fix;
@@ -142,6 +143,8 @@ template <bool kLds>
 __global__ void k_identify_columns_t(ImParams P);
 __global__ void k_fix_columns(ImParams P);
 __global__ void k_insert_slots(ImParams P);
+constexpr int kScoreWaves = 4;               // wavefronts per candidate in the exact scorer
+constexpr size_t kScoreStaticLds = 30 * 1024;   // bound on the scorer's static LDS
 __global__ void k_score_candidates(ImParams P);
 __global__ void k_runs(RunParams P);
 

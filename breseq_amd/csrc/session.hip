@@ -640,13 +640,16 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   P.n_ins = S.d_counters.p + 1;
   P.ins_cap = ins_cap;
   P.key_cap = std::max(64, S.max_window_reads);
+  const size_t key_bytes = ((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15;
+  const size_t score_dyn_budget = 64 * 1024 - kScoreStaticLds;
+  if (key_bytes > score_dyn_budget) throw BsqError(BSQ_ERR_LIMIT, "column depth too large for the exact scorer's LDS list");
+  P.rows_cap = (int32_t)((score_dyn_budget - key_bytes) / (6 * sizeof(double)));
+  const size_t score_lds = key_bytes + (size_t)P.rows_cap * 6 * sizeof(double);
   P.ra_out = S.d_ra.p;
   P.err = (uint32_t*)(S.d_counters.p + 3);
   P.fix = S.d_fix.p;
   P.n_fix = S.d_counters.p + 4;
   P.fix_cap = fix_cap;
-  const size_t key_lds = (size_t)P.key_cap * sizeof(uint32_t);
-  if (key_lds > 48 * 1024) throw BsqError(BSQ_ERR_LIMIT, "column depth too large for the exact scorer's LDS list");
   RunParams R;
   R.pos_target = S.d_pos_target.p;
   R.targets = S.d_targets.p;
@@ -677,7 +680,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   hipLaunchKernelGGL(k_insert_slots, dim3(1024), dim3(64), 0, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[1], S.stream));
-  hipLaunchKernelGGL(k_score_candidates, dim3(4096), dim3(64), key_lds, S.stream, P);
+  hipLaunchKernelGGL(k_score_candidates, dim3(2048), dim3(kScoreWaves * 64), score_lds, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[2], S.stream));
   hipLaunchKernelGGL(k_runs, dim3((S.n_pos + 255) / 256), dim3(256), 0, S.stream, R);

@@ -132,10 +132,13 @@ def main():
         log("[bench] warmup step %d done" % i)
     step_ms = []
     ec_k, im_k = [], []
+    host = {"ec_host_pre": [], "ec_d2h": [], "ec_host_post": [], "im_host_pre": [], "im_d2h": [], "im_host_post": [],
+            "ec_call": [], "im_call": []}
     detail = None
     for i in range(args.steps):
         t0 = time.perf_counter()
         ec = sess.error_count()
+        tm = time.perf_counter()
         ect = sess.timing()
         prop = [5.0] * sample.n_targets
         gd = sess.identify_mutations(log10_prob=ec.rates_table, read_set_max=ec.read_set_max,
@@ -144,6 +147,12 @@ def main():
         imt = sess.timing()
         step_ms.append((t1 - t0) * 1000.0)
         ec_k.append(ect["kernel_detail_ms"])
+        for k, tmd in (("ec", ect), ("im", imt)):
+            host[k + "_host_pre"].append(tmd["host_pre_ms"])
+            host[k + "_d2h"].append(tmd["d2h_ms"])
+            host[k + "_host_post"].append(tmd["host_post_ms"])
+        host["ec_call"].append((tm - t0) * 1000.0)
+        host["im_call"].append((t1 - tm) * 1000.0)
         im_k.append(imt["kernel_detail_ms"])
         detail = imt
     total_s = sum(step_ms) / 1000.0
@@ -187,6 +196,7 @@ def main():
                    "aligned_bases": nab, "reads": n_reads, "ra_entries": n_ra,
                    "candidates": detail["candidates"]},
         "kernels_ms": kernels,
+        "host_ms": {k: sum(v) / len(v) for k, v in host.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic_of(dom, cfg),
                      "algorithmic_bytes": dbytes},

@@ -209,6 +209,43 @@ __device__ __forceinline__ bool slot_observation(const DRead& R, const uint8_t* 
   return true;
 }
 
+// pdata entry of alignment R at (global column p, insert slot): the per-read filters of
+// identify_mutations_pileup::pileup_callback (identify_mutations.cpp:1561-1657).  On success key is
+// ((mslot*S + read file)*Q + quality)*10 + obs*2 + (reverse strand), the row of the likelihood
+// tables; strand_idx is 2 for the forward strand and 0 for the reverse strand.
+__device__ __forceinline__ bool pdata_key(const DRead& R, const uint32_t* __restrict__ cigar,
+                                          const uint8_t* __restrict__ bases, const uint8_t* __restrict__ quals,
+                                          int p, int slot, int S, int Q, int qcut, uint32_t& key, int& obs,
+                                          int& strand_idx, uint32_t& err) {
+  int qpos, indel;
+  bool is_del;
+  if (!entry_at(R, cigar, p, qpos, indel, is_del, err)) return false;
+  const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
+  uint8_t read_base = '.';
+  bool past = true;
+  if (ondel >= slot) {
+    read_base = bases[R.seq_off + qpos + slot];
+    past = false;
+  }
+  // X1 > 1: redundant, no pdata
+  if (read_base == 15 || R.x1 > 1 || is_trimmed(R, qpos, past)) return false;
+  int q;
+  if (!slot_observation(R, bases, quals, qpos, ondel, slot, read_base, obs, q, err)) return false;
+  if (q < qcut) return false;
+  if ((int)R.rfile >= S) {
+    err |= kErrReadSet;
+    return false;
+  }
+  if (q >= Q) {
+    err |= kErrQuality;
+    return false;
+  }
+  const bool top = !(R.flags & kReadReversed);
+  key = ((((uint32_t)R.mslot * S + R.rfile) * Q + q) * 10u) + obs * 2u + (top ? 0u : 1u);
+  strand_idx = top ? 2 : 0;
+  return true;
+}
+
 // pure_genotype_call (identify_mutations.cpp:3398-3433)
 __device__ __forceinline__ void pure_genotype(const double lps[5], uint32_t n, int& best, double& score) {
   best = 5;

@@ -10,7 +10,7 @@ namespace bsq {
 // [tgt_off[t], tgt_off[t] + L_t) and is followed by one NUL pad byte in the reference array,
 // which plays the part of the terminator of the fai_fetch string (pileup_base.cpp:36-44).
 struct DRead {
-  int64_t seq_off;    // first base of the read in bases[] / quals[]
+  int64_t seq_off;    // first base of the read in the device bases[] / quals[] (8-byte aligned)
   int32_t gpos;       // global position of the first aligned reference base
   int32_t gend;       // one past the last aligned reference base (bam_endpos)
   uint32_t cig_off;   // first CIGAR word in cigar[]

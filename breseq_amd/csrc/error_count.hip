@@ -5,8 +5,9 @@
 // Both outputs are integer sums, so they do not depend on the order in which pileup entries are
 // visited.  Pass 1 therefore walks each alignment once (read-parallel) instead of walking columns:
 //
-//   k_ec_reads       one wavefront per alignment at a time; lanes take consecutive aligned bases of
-//                    each M/=/X run, so base, quality and reference loads are 64-byte coalesced.
+//   k_ec_reads       one wavefront per alignment at a time; each lane takes four consecutive query
+//                    positions of an M/=/X run (dword loads of bases and qualities from 8-byte
+//                    aligned read starts, aligned dword loads of reference codes).
 //                    Covariates go to an LDS-privatized count table.  Lane 0 adds +1/-1 at the ends
 //                    of each aligned run to a per-column difference array (unique or redundant).
 //   k_cov_tile_sums  phase 1 of the prefix scan of the difference arrays (per 4096-column tile)
@@ -23,6 +24,22 @@
 #include "kernels.h"
 
 namespace bsq {
+
+// Copies each alignment's bases and qualities to its 8-byte aligned offset (DRead::seq_off) in the
+// device layout; one wavefront per alignment.
+__global__ void k_repack_reads(const DRead* reads, const int64_t* raw_off, int64_t n_reads, const uint8_t* src_b,
+                               const uint8_t* src_q, uint8_t* dst_b, uint8_t* dst_q) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < n_reads; r += nw) {
+    const DRead R = reads[r];
+    const int64_t s = raw_off[r];
+    for (int j = lane; j < (int)R.l_qseq; j += 64) {
+      dst_b[R.seq_off + j] = src_b[s + j];
+      dst_q[R.seq_off + j] = src_q[s + j];
+    }
+  }
+}
 
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -71,8 +88,13 @@ __device__ __forceinline__ int run_end_indel(const ConstU32* c, int k, int n) {
   return indel;
 }
 
+__device__ __forceinline__ uint32_t byte_of(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
+
 // count_alignment_position (error_count.cpp:854-986) for the aligned bases [x, x+len) <-> query
-// [y, y+len) of one unique read; the last base carries `last_indel`.
+// [y, y+len) of one unique read; the last base carries `last_indel`.  Each lane takes four
+// consecutive query positions of a 4-aligned group: reads start at 8-byte aligned offsets, so the
+// bases and qualities of the group and of the next four positions are two dword loads each; the
+// five reference codes the group needs come from two aligned dwords of ref_idx.
 __device__ __forceinline__ void ec_count_run(const EcParams& P, const uint8_t* __restrict__ sq,
                                              const uint8_t* __restrict__ ql, const DRead& R, int x, int y, int len,
                                              int last_indel, uint32_t* table, uint32_t& err) {
@@ -81,56 +103,66 @@ __device__ __forceinline__ void ec_count_run(const EcParams& P, const uint8_t* _
   const uint32_t rv = rev ? 3u : 0u;
   const int rs = R.rfile;
   const int lane = threadIdx.x & 63;
-  const uint8_t* __restrict__ rx = P.ref_idx + x;
-  for (int j0 = 0; j0 < len; j0 += 64) {
-    const int j = j0 + lane;
-    if (j >= len) break;
-    const int qpos = y + j;
-    const uint32_t b = sq[qpos], q = ql[qpos];
-    const uint32_t r0 = rx[j], r1 = rx[j + 1];
-    const int indel = (j == len - 1) ? last_indel : 0;
-    // (1) base against reference, both complemented on the reverse strand (:877-903)
-    if (b != 15u && r0 != kRefN) {
-      const uint32_t oi = comp_if(base_idx(b), rv), ri = comp_if(r0, rv);
-      if (oi > 3u) err |= kErrBaseCode;
-      else if (ri > 3u) err |= kErrRefChar;
-      else if (q >= (uint32_t)Q) err |= kErrQuality;
-      else atomicAdd(&table[rs + S * (int)ri + 5 * S * (int)oi + 25 * S * (int)q], 1u);
-    }
-    // (2) '..' pair and (3) one-base deletion read the base after qpos in read direction
-    const bool pair = (indel == 0) && (qpos < (int)R.qe0);
-    if (pair || indel == -1) {
-      const int mq = qpos + 1 - rev;
-      if (mq < 0 || mq >= (int)R.l_qseq) {
-        err |= kErrQueryIndex;
-      } else {
-        const uint32_t b2 = rev ? b : (uint32_t)sq[mq];
-        const uint32_t q2 = rev ? q : (uint32_t)ql[mq];
-        if (pair) {   // (:909-928): reference at qpos+1-rev in reference direction
-          const uint32_t rp = rev ? r0 : r1;
-          if (b2 != 15u && rp != kRefN) {
-            if (q2 >= (uint32_t)Q) err |= kErrQuality;
-            else atomicAdd(&table[rs + S * 4 + 5 * S * 4 + 25 * S * (int)q2], 1u);
-          }
-        } else if (b2 != 15u && r1 != kRefN) {   // (:934-956): reference base after the deletion
-          const uint32_t rd = comp_if(r1, rv);
-          if (rd > 4u) err |= kErrRefChar;
-          else if (q2 >= (uint32_t)Q) err |= kErrQuality;
-          else atomicAdd(&table[rs + S * (int)rd + 5 * S * 4 + 25 * S * (int)q2], 1u);
-        }
+  const int yend = y + len;
+  const int last = yend - 1;
+  for (int q0 = (y & ~3) + 4 * lane; q0 < yend; q0 += 256) {
+    const uint32_t bw0 = *(const uint32_t*)(sq + q0), bw1 = *(const uint32_t*)(sq + q0 + 4);
+    const uint32_t qw0 = *(const uint32_t*)(ql + q0), qw1 = *(const uint32_t*)(ql + q0 + 4);
+    const int c0 = x + (q0 - y);   // column of query position q0 (>= x - 3; ref_idx is front-padded)
+    const uint32_t* rp = (const uint32_t*)(P.ref_idx + (c0 & ~3));
+    const uint32_t rw0 = rp[0], rw1 = rp[1];
+    const int sh = c0 & 3;
+    const uint32_t rlo = __builtin_amdgcn_alignbyte(rw1, rw0, sh);   // codes of columns c0 .. c0+3
+    const uint32_t r4 = (rw1 >> (8 * sh)) & 0xFFu;                   // code of column c0+4
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int qpos = q0 + k;
+      if (qpos < y || qpos > last) continue;
+      const uint32_t b = byte_of(bw0, k), q = byte_of(qw0, k);
+      const uint32_t nb = (k < 3) ? byte_of(bw0, k + 1) : byte_of(bw1, 0);   // query position qpos+1
+      const uint32_t nq = (k < 3) ? byte_of(qw0, k + 1) : byte_of(qw1, 0);
+      const uint32_t r0 = byte_of(rlo, k);
+      const uint32_t r1 = (k < 3) ? byte_of(rlo, k + 1) : r4;               // column p+1
+      const int indel = (qpos == last) ? last_indel : 0;
+      // (1) base against reference, both complemented on the reverse strand (:877-903)
+      if (b != 15u && r0 != kRefN) {
+        const uint32_t oi = comp_if(base_idx(b), rv), ri = comp_if(r0, rv);
+        if (oi > 3u) err |= kErrBaseCode;
+        else if (ri > 3u) err |= kErrRefChar;
+        else if (q >= (uint32_t)Q) err |= kErrQuality;
+        else atomicAdd(&table[rs + S * (int)ri + 5 * S * (int)oi + 25 * S * (int)q], 1u);
       }
-    } else if (indel == 1) {   // (4) one-base insertion (:963-982)
-      const int mq = qpos + 1;
-      if (mq >= (int)R.l_qseq) {
-        err |= kErrQueryIndex;
-      } else if (mq <= (int)R.qe0 && mq >= (int)R.qs0) {
-        const uint32_t b3 = sq[mq];
-        if (b3 != 15u) {
-          const uint32_t oi = comp_if(base_idx(b3), rv);
-          const uint32_t q3 = ql[mq];
+      // (2) '..' pair and (3) one-base deletion read the base after qpos in read direction
+      // (qpos+1 forward, qpos itself reverse)
+      const bool pair = (indel == 0) && (qpos < (int)R.qe0);
+      if (pair || indel == -1) {
+        if (!rev && qpos + 1 >= (int)R.l_qseq) {
+          err |= kErrQueryIndex;
+        } else {
+          const uint32_t b2 = rev ? b : nb;
+          const uint32_t q2 = rev ? q : nq;
+          if (pair) {   // (:909-928): reference at qpos+1-rev in reference direction
+            const uint32_t rp2 = rev ? r0 : r1;
+            if (b2 != 15u && rp2 != kRefN) {
+              if (q2 >= (uint32_t)Q) err |= kErrQuality;
+              else atomicAdd(&table[rs + S * 4 + 5 * S * 4 + 25 * S * (int)q2], 1u);
+            }
+          } else if (b2 != 15u && r1 != kRefN) {   // (:934-956): reference base after the deletion
+            const uint32_t rd = comp_if(r1, rv);
+            if (rd > 4u) err |= kErrRefChar;
+            else if (q2 >= (uint32_t)Q) err |= kErrQuality;
+            else atomicAdd(&table[rs + S * (int)rd + 5 * S * 4 + 25 * S * (int)q2], 1u);
+          }
+        }
+      } else if (indel == 1) {   // (4) one-base insertion (:963-982): the inserted base at qpos+1
+        const int mq = qpos + 1;
+        if (mq >= (int)R.l_qseq) {
+          err |= kErrQueryIndex;
+        } else if (mq <= (int)R.qe0 && mq >= (int)R.qs0 && nb != 15u) {
+          const uint32_t oi = comp_if(base_idx(nb), rv);
           if (oi > 3u) err |= kErrBaseCode;
-          else if (q3 >= (uint32_t)Q) err |= kErrQuality;
-          else atomicAdd(&table[rs + S * 4 + 5 * S * (int)oi + 25 * S * (int)q3], 1u);
+          else if (nq >= (uint32_t)Q) err |= kErrQuality;
+          else atomicAdd(&table[rs + S * 4 + 5 * S * (int)oi + 25 * S * (int)nq], 1u);
         }
       }
     }

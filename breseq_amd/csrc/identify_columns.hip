@@ -6,9 +6,13 @@
 // One 64-wide wavefront owns 64 consecutive columns (lane = column) and walks the alignments that
 // overlap them in file order.  Every lane sees the same alignment in the same iteration, so the
 // 40-byte record is a scalar load and the bases and qualities of consecutive columns are
-// consecutive bytes of that alignment (one coalesced load each).  Each lane adds its column's FP64
-// likelihood terms in file order, which is the reference's summation order: the sums, the
-// pure-genotype score and the predicted/consensus decisions are bit-identical to the reference's.
+// consecutive bytes of that alignment (one coalesced load each).
+//
+// The likelihood sums are FP32 (one 32-byte table row per read, two LDS reads); the column's
+// predicted and consensus decisions are certified against an explicit bound on the difference to
+// the reference's FP64 sums, and the rare columns within that bound of a threshold are decided by
+// k_fix_columns from exact FP64 sums in read order.  Coverage counts are integers and the 1/X1
+// redundancy sums stay FP64 in read order, so those outputs are the reference's exactly.
 //
 // Per alignment the lane work is branch-light: filtered lanes (no entry, N base, trimmed, low
 // quality) read the all-zero table row instead of branching; the strand permutation of the
@@ -105,7 +109,7 @@ __device__ __forceinline__ bool screen_passes(const Screen& A, int ref_idx, doub
   const float n = (float)A.n;
   const float u = 5.9604645e-08f;   // 2^-24
   const float mag0 = fabsf(A.SL) + fabsf(A.Dall) + A.Gabs + magL;
-#pragma unroll 1
+#pragma unroll
   for (int v = 0; v < 5; v++) {
     if (v == ref_idx) continue;
     const float Li = log10f(2.0f + (n - cnt[v]));
@@ -151,28 +155,31 @@ __device__ __forceinline__ void prepare_general(const ImParams& P, const DRead& 
   }
 }
 
-// Score of pure_genotype_call (identify_mutations.cpp:3398-3433) in FP32 transcendentals.  best is
-// exact (FP64 argmax, as the reference).  The reference's score is lps[best] - (log10(tot) + off)
-// with tot = Sum_{b != best} 10^(lps[b] - off) in [1, 4]; here each 10^d (d <= 0) is evaluated from
-// (float)d, so a term's relative error is at most ln(10) * 2^-24 * |d| + 2 ulp, below 6e-6 for the
-// terms that do not underflow (|d| < 40), and those that do are below 1e-38 of tot.  The score is
-// therefore within 3e-6 of the reference's; decisions closer than kScoreMargin to a threshold are
-// left to the exact FP64 evaluation of k_fix_columns.
-constexpr double kScoreMargin = 1e-4;
+// Certified pure_genotype_call (identify_mutations.cpp:3398-3433) from FP32 sums.
+//   * Each FP32 sum lps[b] adds n terms of one sign, each rounded once from FP64: it is within
+//     (n + 2) 2^-24 |lps[b]| of the reference's FP64 sum (which is itself exact to ~1e-13 relative).
+//     E is the largest such bound.
+//   * The reference's best is certain when the top sum leads the next by more than 2E.
+//   * Its score lps[best] - log10 Sum_{b != best} 10^lps[b] moves by at most 2E when every sum moves
+//     by at most E; here each 10^d (d <= 0) is evaluated from (float)d, adding at most 3e-6 (terms
+//     below 10^-38 of the total underflow, which changes the total by less than that).
+// A decision within 2E + kScoreSlack of its threshold is left to k_fix_columns, which recomputes
+// the column's sums exactly in read order.
+constexpr double kScoreSlack = 1e-4;
 
-__device__ __forceinline__ double genotype_score_approx(const double lps[5], int best) {
-  double off = -1.7976931348623157e308;
+__device__ __forceinline__ double genotype_score_approx(const float lps[5], int best) {
+  float off = -3.4028235e38f;
 #pragma unroll
   for (int b = 0; b < 5; b++)
-    if (b != best) off = fmax(off, lps[b]);
+    if (b != best) off = fmaxf(off, lps[b]);
   float tot = 0.0f;
 #pragma unroll
   for (int b = 0; b < 5; b++)
-    if (b != best) tot += exp10f((float)(lps[b] - off));
-  return (lps[best] - off) - (double)log10f(tot);
+    if (b != best) tot += exp10f((float)((double)lps[b] - (double)off));
+  return ((double)lps[best] - (double)off) - (double)log10f(tot);
 }
 
-__device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, int ref_idx, const double lps[5],
+__device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, int ref_idx, const float lps[5],
                                               double red_bot, double red_top, int uniq, bool insert_slot,
                                               const Screen& A, uint32_t& err) {
   // position_coverage::sum (identify_mutations.h:115-120)
@@ -183,29 +190,29 @@ __device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, i
   bool cons = false, cons_unsure = false;
   if (A.n > 0) {
     int best = 0;
+    float E = 0.0f;
 #pragma unroll
-    for (int b = 1; b < 5; b++)
+    for (int b = 0; b < 5; b++) {
       if (lps[b] > lps[best]) best = b;
-    const double cbs = genotype_score_approx(lps, best) - P.log10_ref_length;
-    const bool pred_unsure = fabs(cbs - P.mutation_cutoff) <= kScoreMargin;
-    if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
-    if (best != ref_idx) {
-      cons = cbs > 0.0;
-      cons_unsure = fabs(cbs) <= kScoreMargin;
+      E = fmaxf(E, fabsf(lps[b]));
     }
+    const double err_sum = ((double)A.n + 2.0) * 5.9604644775390625e-08 * (double)E * 1.0001;
+    float second = -3.4028235e38f;
+#pragma unroll
+    for (int b = 0; b < 5; b++)
+      if (b != best) second = fmaxf(second, lps[b]);
+    const bool best_unsure = ((double)lps[best] - (double)second) <= 2.0 * err_sum;
+    const double margin = 2.0 * err_sum + kScoreSlack;
+    const double cbs = genotype_score_approx(lps, best) - P.log10_ref_length;
+    const bool pred_unsure = best_unsure || fabs(cbs - P.mutation_cutoff) <= margin;
+    if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
+    if (best != ref_idx) cons = cbs > 0.0;
+    cons_unsure = best_unsure || (best != ref_idx && fabs(cbs) <= margin);
     if (cons) fl |= kPosConsensus;
     if (pred_unsure || cons_unsure) {
       const int k = atomicAdd(P.n_fix, 1);
-      if (k < P.fix_cap) {
-        ColFix f;
-        f.p = p;
-        f.n = A.n;
-#pragma unroll
-        for (int b = 0; b < 5; b++) f.lps[b] = lps[b];
-        P.fix[k] = f;
-      } else {
-        err |= kErrCapacity;
-      }
+      if (k < P.fix_cap) P.fix[k] = p;
+      else err |= kErrCapacity;
     }
   }
   if (r1 > 0.0) fl |= kPosRedundant;
@@ -230,21 +237,60 @@ __device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, i
   P.col_total[p] = total;
 }
 
-// Exact pure-genotype decisions for the columns the column kernel left uncertain (thread each).
-__global__ void __launch_bounds__(256) k_fix_columns(ImParams P) {
-  const int n = min(*P.n_fix, P.fix_cap);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const ColFix f = P.fix[i];
+// Exact pure-genotype decisions for the columns the column kernel could not certify: one wavefront
+// per column gathers its pdata in read order (order-preserving compaction), five lanes add the FP64
+// likelihoods in that order, and the reference's call is made from those sums.
+__global__ void __launch_bounds__(64) k_fix_columns(ImParams P) {
+  extern __shared__ uint32_t fix_keys[];   // P.key_cap
+  __shared__ double fix_lps[5];
+  const int lane = threadIdx.x;
+  const int n_fix = min(*P.n_fix, P.fix_cap);
+  const int S = P.read_set_max, Q = P.quality_max;
+  uint32_t err = 0;
+  for (int i = blockIdx.x; i < n_fix; i += gridDim.x) {
+    const int p = P.fix[i];
+    const int64_t w = p / kWaveCols;
+    const int rb = P.win_rb[w], re = P.win_re[w];
+    int n = 0;
+    for (int r0 = rb; r0 < re; r0 += 64) {
+      const int r = r0 + lane;
+      bool use = false;
+      uint32_t key = 0;
+      int obs = 0, strand_idx = 0;
+      if (r < re) use = pdata_key(P.reads[r], P.cigar, P.bases, P.quals, p, 0, S, Q, P.base_quality_cutoff, key, obs,
+                                  strand_idx, err);
+      const unsigned long long mask = __ballot(use);
+      if (use) {
+        const int idx = n + __popcll(mask & ((1ull << lane) - 1ull));
+        if (idx < P.key_cap) fix_keys[idx] = key;
+        else err |= kErrCapacity;
+      }
+      n += __popcll(mask);
+    }
+    n = min(n, P.key_cap);
+    __syncthreads();
+    if (lane < 5) {
+      double acc = 0.0;
+      for (int j = 0; j < n; j++) acc += P.tab_lp[(size_t)fix_keys[j] * 5 + lane];
+      fix_lps[lane] = acc;
+    }
+    __syncthreads();
+    double lps[5];
+    for (int b = 0; b < 5; b++) lps[b] = fix_lps[b];
     int best;
     double score;
-    pure_genotype(f.lps, f.n, best, score);
+    pure_genotype(lps, (uint32_t)n, best, score);
     const double cbs = score - P.log10_ref_length;
-    const int ref_idx = char_index(P.ref[f.p]);
-    uint8_t fl = P.col_flags[f.p] & (uint8_t)~(kPosPredicted | kPosConsensus);
-    if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
-    if ((best != ref_idx) && !isnan(cbs) && (cbs > 0.0)) fl |= kPosConsensus;
-    P.col_flags[f.p] = fl;
+    const int ref_idx = char_index(P.ref[p]);
+    if (lane == 0) {
+      uint8_t fl = P.col_flags[p] & (uint8_t)~(kPosPredicted | kPosConsensus);
+      if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
+      if ((best != ref_idx) && !isnan(cbs) && (cbs > 0.0)) fl |= kPosConsensus;
+      P.col_flags[p] = fl;
+    }
+    __syncthreads();
   }
+  if (err) atomicOr(P.err, err);
 }
 
 template <bool kLds>
@@ -281,7 +327,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
     const int e_bot = (e_top < 4) ? 3 - e_top : e_top;
     const int rb = __builtin_amdgcn_readfirstlane(P.win_rb[w]);
     const int re = __builtin_amdgcn_readfirstlane(P.win_re[w]);
-    double lps0 = 0.0, lps1 = 0.0, lps2 = 0.0, lps3 = 0.0, lps4 = 0.0;
+    float lps0 = 0.0f, lps1 = 0.0f, lps2 = 0.0f, lps3 = 0.0f, lps4 = 0.0f;
     double red_bot = 0.0, red_top = 0.0;
     int uniq = 0;
     bool insert_slot = false;
@@ -373,7 +419,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
       const uint32_t o_rs = (obs < 4u && !top) ? (obs ^ 3u) : obs;
       const int row = ok ? ((((int)R.mslot * S + (int)R.rfile) * Q + (int)q) * 5 + (int)o_rs) : zero_row;
       const FoldRow& T = tab[row];
-      const double t0 = T.t[0], t1 = T.t[1], t2 = T.t[2], t3 = T.t[3], t4 = T.t[4];
+      const float t0 = T.t[0], t1 = T.t[1], t2 = T.t[2], t3 = T.t[3], t4 = T.t[4];
       if (top) {
         lps0 += t0;
         lps1 += t1;
@@ -391,7 +437,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
       A.Dall += dg;
       A.n += ok ? 1u : 0u;
       if (ok && (int)obs != ref_idx) {
-        const float e = (float)T.t[top ? e_top : e_bot];
+        const float e = T.t[top ? e_top : e_bot];
         const float g = e - dg;
         A.Gabs += fabsf(e) + fabsf(dg);
 #pragma unroll
@@ -410,7 +456,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
         P.col_total[p] = 0;
       }
     } else {
-      const double lps[5] = {lps0, lps1, lps2, lps3, lps4};
+      const float lps[5] = {lps0, lps1, lps2, lps3, lps4};
       finish_column(P, p, t, ref_idx, lps, red_bot, red_top, uniq, insert_slot, A, err);
     }
   }

@@ -18,7 +18,7 @@ struct EcParams {
   const uint32_t* cigar;
   const uint8_t* bases;
   const uint8_t* quals;
-  const uint8_t* ref_idx;       // per global column (plus one pad), kRef* codes
+  const uint8_t* ref_idx;       // per global column (plus pads), kRef* codes; 16-byte aligned
   const int32_t* pos_target;    // target of each global column, -1 for pads
   const int32_t* target_scored; // 1 if the target is in the call set
   int64_t n_reads;
@@ -48,21 +48,16 @@ constexpr int kCovTile = 4096;      // columns per block of the coverage scan (2
 
 // One row of the strand-folded likelihood table: a read of quality q from read file rs with
 // MAPQ slot m observing read-strand base o reads row ((m*S + rs)*Q + q)*5 + o.  Hypotheses are in
-// read-strand order; a reverse-strand read maps hypothesis b to t[comp(b)].
+// read-strand order; a reverse-strand read maps hypothesis b to t[comp(b)].  FP32: the column pass
+// sums these with an explicit error bound and certifies its decisions (exact FP64 values are used
+// by the scorer and k_fix_columns).
 struct FoldRow {
-  double t[5];    // log10 P(o | hypothesis), fill_read_base_likelihoods (identify_mutations.cpp:3359-3384)
+  float t[5];     // log10 P(o | hypothesis), fill_read_base_likelihoods (identify_mutations.cpp:3359-3384)
   float lpmax;    // max_b t[b]
   float diag;     // t[o]: the read's own observation as the hypothesis
+  float pad;
 };
-static_assert(sizeof(FoldRow) == 48, "FoldRow is three 16-byte LDS words");
-
-// A column whose pure-genotype decision the column kernel could not certify with its FP32
-// score: the exact FP64 call is made from these sums by k_fix_columns.
-struct ColFix {
-  int32_t p;
-  uint32_t n;
-  double lps[5];
-};
+static_assert(sizeof(FoldRow) == 32, "FoldRow is two 16-byte LDS words");
 
 struct ImParams {
   const DRead* reads;
@@ -104,8 +99,8 @@ struct ImParams {
   int32_t key_cap;              // LDS capacity of the scorer's read list
   int32_t rows_cap;             // reads whose table rows the scorer caches in LDS
   RaOut* ra_out;
-  // columns whose predicted / consensus decision lies within the approximation margin
-  ColFix* fix;
+  // columns whose predicted / consensus decision the column pass could not certify
+  int32_t* fix;
   int32_t* n_fix;
   int32_t fix_cap;
   uint32_t* err;
@@ -133,12 +128,15 @@ struct RunParams {
 };
 
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v);
+__global__ void k_repack_reads(const DRead* reads, const int64_t* raw_off, int64_t n_reads, const uint8_t* src_b,
+                               const uint8_t* src_q, uint8_t* dst_b, uint8_t* dst_q);
+constexpr int kRefIdxFrontPad = 16;   // ref_idx bytes before column 0 (aligned loads may start at -4)
 __global__ void k_ec_reads(EcParams P);
 __global__ void k_cov_tile_sums(CovParams C);
 __global__ void k_cov_tile_scan(CovParams C);
 __global__ void k_cov_hist(CovParams C);
 constexpr int kColWaves = 8;                 // wavefronts per block of the column kernel
-constexpr size_t kFoldLdsBytes = 48 * 1024;  // largest likelihood table staged in LDS
+constexpr size_t kFoldLdsBytes = 48 * 1024;  // largest likelihood table staged in LDS (1535 rows)
 template <bool kLds>
 __global__ void k_identify_columns_t(ImParams P);
 __global__ void k_fix_columns(ImParams P);

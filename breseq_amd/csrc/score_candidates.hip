@@ -384,28 +384,7 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
       int obs = 0, strand_idx = 0;
       if (r < re) {
         const DRead R = P.reads[r];
-        int qpos, indel;
-        bool is_del;
-        if (entry_at(R, P.cigar, p, qpos, indel, is_del, err)) {
-          const int ondel = is_del ? -1 : (indel < 0 ? 0 : indel);
-          uint8_t read_base = '.';
-          bool past = true;
-          if (ondel >= slot) {
-            read_base = P.bases[R.seq_off + qpos + slot];
-            past = false;
-          }
-          // X1 > 1: redundant, no pdata
-          if (read_base != 15 && R.x1 <= 1 && !is_trimmed(R, qpos, past)) {
-            int q;
-            if (slot_observation(R, P.bases, P.quals, qpos, ondel, slot, read_base, obs, q, err) &&
-                q >= P.base_quality_cutoff && (int)R.rfile < S && q < Q) {
-              const bool top = !(R.flags & kReadReversed);
-              key = ((((uint32_t)R.mslot * S + R.rfile) * Q + q) * 10u) + obs * 2u + (top ? 0u : 1u);
-              strand_idx = top ? 2 : 0;
-              use = true;
-            }
-          }
-        }
+        use = pdata_key(R, P.cigar, P.bases, P.quals, p, slot, S, Q, P.base_quality_cutoff, key, obs, strand_idx, err);
       }
       const unsigned long long mask = __ballot(use);
       if (lane == 0) SS.wcount[wave] = (uint32_t)__popcll(mask);

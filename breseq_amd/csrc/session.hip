@@ -95,7 +95,7 @@ struct Session {
   DBuf<int32_t> d_col_unique, d_col_total;
   DBuf<int2> d_cand;
   DBuf<int32_t> d_counters;   // [0] n_cand, [1] n_ins, [2] n_runs, [3] error bits, [4] n_fix
-  DBuf<ColFix> d_fix;
+  DBuf<int32_t> d_fix;
   DBuf<int32_t> d_ins;
   DBuf<RaOut> d_ra;
   DBuf<int2> d_events, d_runs;
@@ -149,6 +149,9 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   // reads
   std::vector<DRead> dr;
   dr.reserve((size_t)a->n_reads);
+  std::vector<int64_t> raw_off;
+  raw_off.reserve((size_t)a->n_reads);
+  int64_t dev_bytes = 0;
   std::vector<uint32_t> cig;
   std::map<int32_t, uint8_t> mslots;
   std::vector<uint8_t> mq_of;
@@ -169,7 +172,10 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
     if (lq > 65535) throw BsqError(BSQ_ERR_LIMIT, "read longer than 65535 bases");
     DRead d;
     memset(&d, 0, sizeof(d));
-    d.seq_off = a->seq_off[r];
+    // device layout: every read starts at an 8-byte aligned offset (k_ec_reads loads dwords)
+    d.seq_off = dev_bytes;
+    dev_bytes += (lq + 7) & ~(int64_t)7;
+    raw_off.push_back(a->seq_off[r]);
     const int64_t gp = (int64_t)S.tgt_off[tid] + a->pos[r];
     int64_t span = 0, qlen = 0;
     int n_ref_ops = 0, n_ins = 0;
@@ -276,12 +282,25 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   HIP_OK(hipMemcpyAsync(S.d_reads.p, dr.data(), dr.size() * sizeof(DRead), hipMemcpyHostToDevice, S.stream));
   S.d_cigar.alloc(cig.size());
   HIP_OK(hipMemcpyAsync(S.d_cigar.p, cig.data(), cig.size() * sizeof(uint32_t), hipMemcpyHostToDevice, S.stream));
+  // bases and qualities: uploaded as given, then repacked on the device to the aligned layout
   const size_t nbases = (size_t)a->seq_off[a->n_reads];
-  S.d_bases.alloc(nbases + 16);
-  S.d_quals.alloc(nbases + 16);
-  if (nbases) {
-    HIP_OK(hipMemcpyAsync(S.d_bases.p, a->seq, nbases, hipMemcpyHostToDevice, S.stream));
-    HIP_OK(hipMemcpyAsync(S.d_quals.p, a->qual, nbases, hipMemcpyHostToDevice, S.stream));
+  S.d_bases.alloc((size_t)dev_bytes + 16);
+  S.d_quals.alloc((size_t)dev_bytes + 16);
+  HIP_OK(hipMemsetAsync(S.d_bases.p, 0, (size_t)dev_bytes + 16, S.stream));
+  HIP_OK(hipMemsetAsync(S.d_quals.p, 0, (size_t)dev_bytes + 16, S.stream));
+  if (nbases && !dr.empty()) {
+    DBuf<uint8_t> raw_b, raw_q;
+    DBuf<int64_t> d_raw_off;
+    raw_b.alloc(nbases);
+    raw_q.alloc(nbases);
+    d_raw_off.alloc(raw_off.size());
+    HIP_OK(hipMemcpyAsync(raw_b.p, a->seq, nbases, hipMemcpyHostToDevice, S.stream));
+    HIP_OK(hipMemcpyAsync(raw_q.p, a->qual, nbases, hipMemcpyHostToDevice, S.stream));
+    HIP_OK(hipMemcpyAsync(d_raw_off.p, raw_off.data(), raw_off.size() * sizeof(int64_t), hipMemcpyHostToDevice, S.stream));
+    hipLaunchKernelGGL(k_repack_reads, dim3(2048), dim3(256), 0, S.stream, S.d_reads.p, d_raw_off.p, (int64_t)dr.size(),
+                       raw_b.p, raw_q.p, S.d_bases.p, S.d_quals.p);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(S.stream));
   }
   S.d_ref.alloc(S.h_ref.size());
   HIP_OK(hipMemcpyAsync(S.d_ref.p, S.h_ref.data(), S.h_ref.size(), hipMemcpyHostToDevice, S.stream));
@@ -298,8 +317,9 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
         default: ri[i] = kRefOther; break;
       }
     }
-    S.d_ref_idx.alloc(ri.size());
-    HIP_OK(hipMemcpyAsync(S.d_ref_idx.p, ri.data(), ri.size(), hipMemcpyHostToDevice, S.stream));
+    S.d_ref_idx.alloc(ri.size() + kRefIdxFrontPad);
+    HIP_OK(hipMemsetAsync(S.d_ref_idx.p, kRefOther, ri.size() + kRefIdxFrontPad, S.stream));
+    HIP_OK(hipMemcpyAsync(S.d_ref_idx.p + kRefIdxFrontPad, ri.data(), ri.size(), hipMemcpyHostToDevice, S.stream));
     HIP_OK(hipStreamSynchronize(S.stream));
   }
   S.d_pos_target.alloc(S.h_pos_target.size());
@@ -373,6 +393,7 @@ namespace bsq {
 
 static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec_result& out) {
   if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
+  const double th0 = now_ms();
   const int Sm = args->read_set_max, Q = args->quality_max;
   if (Sm <= 0 || Q <= 0 || Sm > 255 || Q > 256) throw BsqError(BSQ_ERR_ARG, "covariate maxima out of range");
   const int n_bins = Sm * 25 * Q;
@@ -405,7 +426,7 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   P.cigar = S.d_cigar.p;
   P.bases = S.d_bases.p;
   P.quals = S.d_quals.p;
-  P.ref_idx = S.d_ref_idx.p;
+  P.ref_idx = S.d_ref_idx.p + kRefIdxFrontPad;
   P.pos_target = S.d_pos_target.p;
   P.target_scored = S.d_target_scored.p;
   P.n_reads = S.n_dreads;
@@ -436,6 +457,7 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   // one block per 8 waves' worth of alignment chunks, capped at 8 blocks per CU
   const int64_t chunks = (S.n_dreads + 15) / 16;
   const int ec_grid = (int)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 2048));
+  S.timing.host_pre_ms = now_ms() - th0;
   HIP_OK(hipEventRecord(S.ev0, S.stream));
   hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, S.stream, S.d_cov_diff.p, n_diff, 0);
   hipLaunchKernelGGL(k_ec_reads, dim3(ec_grid), dim3(256), lds_table, S.stream, P);
@@ -519,7 +541,7 @@ static void build_likelihood_tables(const Session& S, const bsq_identify_args* a
   lp.assign(nkeys * 5, 0.0);
   rr.assign(nkeys * 6, 0.0);
   // one extra all-zero row: the row a filtered read reads
-  fold.assign(nm * Sm * Q * 5 + 1, FoldRow{{0.0, 0.0, 0.0, 0.0, 0.0}, 0.0f, 0.0f});
+  fold.assign(nm * Sm * Q * 5 + 1, FoldRow{{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, 0.0f, 0.0f, 0.0f});
   std::vector<double> prob(a->n_bins);
   for (int64_t i = 0; i < a->n_bins; i++) prob[i] = pow(10, a->log10_prob_table[i]);
   auto comp = [](int i) { return i < 4 ? 3 - i : 4; };
@@ -550,7 +572,7 @@ static void build_likelihood_tables(const Session& S, const bsq_identify_args* a
             rr[key * 6 + 5] = mx;
             if (top) {   // read-strand row == top-strand row for a top-strand read
               const size_t row = ((m * Sm + rs) * Q + q) * 5 + obs;
-              for (int b = 0; b < 5; b++) fold[row].t[b] = l[b];
+              for (int b = 0; b < 5; b++) fold[row].t[b] = (float)l[b];
               fold[row].lpmax = (float)mx;
               fold[row].diag = (float)l[obs];
             }
@@ -674,7 +696,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[0], S.stream));
-  hipLaunchKernelGGL(k_fix_columns, dim3(64), dim3(256), 0, S.stream, P);
+  hipLaunchKernelGGL(k_fix_columns, dim3(256), dim3(64), key_bytes, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[3], S.stream));
   hipLaunchKernelGGL(k_insert_slots, dim3(1024), dim3(64), 0, S.stream, P);

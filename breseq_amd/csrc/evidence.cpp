@@ -1,6 +1,8 @@
 // evidence.cpp -- see evidence.h.
 #include "evidence.h"
 
+#include <charconv>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -144,34 +146,37 @@ void ra_entry_from_result(const RaOut& r, const std::string& seq_id, uint32_t po
   e.ins = r.slot;
   e.ref_base = std::string(1, kIdx[r.ref_idx]);
   e.new_base = std::string(1, kIdx[r.variant]);
-  std::map<std::string, std::string> m;
-  m["score"] = fmt_double(r.variant_score, 1, false);
-  m["major_base"] = std::string(1, kIdx[r.major]);
-  m["minor_base"] = std::string(1, kIdx[r.minor]);
-  m["major_frequency"] = fmt_double(reported(r, r.major), places, true);
-  m["frequency"] = fmt_double(reported(r, r.variant), places, true);
+  auto cov = [&](int b) -> std::string {
+    if (b >= 5) return "0/0";
+    return std::to_string((int32_t)r.cov[b][0]) + "/" + std::to_string((int32_t)r.cov[b][1]);
+  };
   std::string spec;
   for (int b = 0; b < 5; b++) {
     const double fr = reported(r, b);
     if (fr <= 0.0) continue;
     if (!spec.empty()) spec += ",";
-    spec += std::string(1, kIdx[b]) + ":" + fmt_double(fr, places, true);
+    spec += kIdx[b];
+    spec += ":";
+    spec += fmt_double(fr, places, true);
   }
-  m["allele_frequencies"] = spec;
-  m["frequency_lower"] = fmt_double(r.lower, places, true);
-  m["frequency_upper"] = fmt_double(r.upper, places, true);
-  m["ks_quality_p_value"] = fmt_double(r.ks_p, 5, true);
-  m["fisher_strand_p_value"] = fmt_double(r.fisher_p, 5, true);
-  auto cov = [&](int b) -> std::string {
-    if (b >= 5) return "0/0";
-    return std::to_string((int32_t)r.cov[b][0]) + "/" + std::to_string((int32_t)r.cov[b][1]);
-  };
-  m["ref_cov"] = cov(r.ref_idx);
-  m["new_cov"] = cov(r.variant);
-  m["major_cov"] = cov(r.major);
-  m["minor_cov"] = cov(r.minor);
-  m["total_cov"] = std::to_string(r.total_top) + "/" + std::to_string(r.total_bot);
-  e.fields.assign(m.begin(), m.end());
+  // the fields in the order of their names (the entry's std::map order when written)
+  e.fields.clear();
+  e.fields.reserve(15);
+  e.fields.emplace_back("allele_frequencies", std::move(spec));
+  e.fields.emplace_back("fisher_strand_p_value", fmt_double(r.fisher_p, 5, true));
+  e.fields.emplace_back("frequency", fmt_double(reported(r, r.variant), places, true));
+  e.fields.emplace_back("frequency_lower", fmt_double(r.lower, places, true));
+  e.fields.emplace_back("frequency_upper", fmt_double(r.upper, places, true));
+  e.fields.emplace_back("ks_quality_p_value", fmt_double(r.ks_p, 5, true));
+  e.fields.emplace_back("major_base", std::string(1, kIdx[r.major]));
+  e.fields.emplace_back("major_cov", cov(r.major));
+  e.fields.emplace_back("major_frequency", fmt_double(reported(r, r.major), places, true));
+  e.fields.emplace_back("minor_base", std::string(1, kIdx[r.minor]));
+  e.fields.emplace_back("minor_cov", cov(r.minor));
+  e.fields.emplace_back("new_cov", cov(r.variant));
+  e.fields.emplace_back("ref_cov", cov(r.ref_idx));
+  e.fields.emplace_back("score", fmt_double(r.variant_score, 1, false));
+  e.fields.emplace_back("total_cov", std::to_string(r.total_top) + "/" + std::to_string(r.total_bot));
 }
 
 static int type_group(int t) { return t == 0 ? 3 : (t == 1 ? 4 : 7); }
@@ -207,24 +212,47 @@ std::string write_gd(std::vector<GdEntry>& entries) {
     }
     return a->id < b->id;
   });
-  std::string out = "#=GENOME_DIFF\t1.0\n";
+  std::string out;
+  out.reserve(64 + v.size() * 400);
+  out += "#=GENOME_DIFF\t1.0\n";
   static const char* tn[3] = {"RA", "MC", "UN"};
+  char num[16];
+  auto put_u = [&](int64_t x) {
+    const std::to_chars_result r = std::to_chars(num, num + sizeof(num), x);
+    out.append(num, (size_t)(r.ptr - num));
+  };
   for (const GdEntry* e : v) {
     out += tn[e->type];
-    out += "\t" + std::to_string(e->id) + "\t.\t" + e->seq_id + "\t";
+    out += '\t';
+    put_u(e->id);
+    out += "\t.\t";
+    out += e->seq_id;
+    out += '\t';
+    put_u(e->pos);
+    out += '\t';
     if (e->type == 0) {
-      out += std::to_string(e->pos) + "\t" + std::to_string(e->ins) + "\t" + e->ref_base + "\t" + e->new_base;
+      put_u(e->ins);
+      out += '\t';
+      out += e->ref_base;
+      out += '\t';
+      out += e->new_base;
     } else if (e->type == 1) {
-      out += std::to_string(e->pos) + "\t" + std::to_string(e->end) + "\t" + std::to_string(e->start_range) + "\t" +
-             std::to_string(e->end_range);
+      put_u(e->end);
+      out += '\t';
+      put_u(e->start_range);
+      out += '\t';
+      put_u(e->end_range);
     } else {
-      out += std::to_string(e->pos) + "\t" + std::to_string(e->end);
+      put_u(e->end);
     }
     for (const auto& kv : e->fields) {
       if (kv.second.empty() || kv.first[0] == '_') continue;
-      out += "\t" + kv.first + "=" + kv.second;
+      out += '\t';
+      out += kv.first;
+      out += '=';
+      out += kv.second;
     }
-    out += "\n";
+    out += '\n';
   }
   return out;
 }

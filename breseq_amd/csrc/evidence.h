@@ -33,7 +33,7 @@ struct GdEntry {
   uint32_t end = 0;      // MC/UN end
   uint32_t start_range = 0, end_range = 0;
   std::string ref_base, new_base;
-  std::vector<std::pair<std::string, std::string>> fields;   // other key=value pairs
+  std::vector<std::pair<const char*, std::string>> fields;   // other key=value pairs (static names)
   // traversal order key (target rank, emission column, kind) used to number entries as add() does
   int64_t order_target = 0, order_pos = 0, order_sub = 0;
 };

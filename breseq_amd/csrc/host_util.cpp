@@ -1,6 +1,8 @@
 // host_util.cpp -- see host_util.h.
 #include "host_util.h"
 
+#include <charconv>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -15,13 +17,34 @@ void set_status(bsq_status* st, int code, const std::string& msg) {
 }
 
 std::string fmt_double(double v, unsigned precision, bool scientific) {
-  if (std::isnan(v)) return "NA";
-  char buf[512];
-  snprintf(buf, sizeof(buf), scientific ? "%.*e" : "%.*f", (int)precision, v);
-  std::string s(buf);
+  char buf[400];
+  const int n = fmt_double_buf(v, precision, scientific, buf, sizeof(buf));
+  return std::string(buf, (size_t)(n > 0 ? n : 0));
+}
+
+// printf("%.*e") / printf("%.*f") digits through std::to_chars, which is specified to produce the
+// same correctly rounded decimal form; NaN prints as NA.
+int fmt_double_buf(double v, unsigned precision, bool scientific, char* buf, size_t len) {
+  if (std::isnan(v)) {
+    if (len < 3) return -1;
+    buf[0] = 'N';
+    buf[1] = 'A';
+    buf[2] = 0;
+    return 2;
+  }
+  const std::to_chars_result r =
+      std::to_chars(buf, buf + len - 1, v, scientific ? std::chars_format::scientific : std::chars_format::fixed,
+                    (int)precision);
+  if (r.ec != std::errc()) return -1;
+  int n = (int)(r.ptr - buf);
   // a 3-digit exponent with a leading zero is shortened (only non-glibc runtimes print one)
-  if (scientific && s.size() >= 3 && s[s.size() - 3] == '0') s.erase(s.size() - 3, 1);
-  return s;
+  if (scientific && n >= 3 && buf[n - 3] == '0') {
+    buf[n - 3] = buf[n - 2];
+    buf[n - 2] = buf[n - 1];
+    n--;
+  }
+  buf[n] = 0;
+  return n;
 }
 
 std::string fmt_g(double v) {

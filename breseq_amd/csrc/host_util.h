@@ -20,6 +20,8 @@ void set_status(bsq_status* st, int code, const std::string& msg);
 // breseq to_string(double, precision, scientific) (common.h:845-866): "NA" for NaN, fixed or
 // scientific with `precision` digits after the point.
 std::string fmt_double(double v, unsigned precision, bool scientific);
+// fmt_double into buf (NUL-terminated); returns the length, or -1 if buf is too small.
+int fmt_double_buf(double v, unsigned precision, bool scientific, char* buf, size_t len);
 // iostream default formatting of a double (%g, 6 significant digits).
 std::string fmt_g(double v);
 

@@ -129,3 +129,28 @@ def test_number_formatting_rules():
     assert f(0.5, 3, 1) == "5.000e-01"
     assert f(55.25, 1, 0) == "55.2"
     assert f(1e-05, 5, 1) == "1.00000e-05"
+
+
+def test_number_formatting_matches_printf_rounding():
+    """bsq_format_double prints %.*e / %.*f digits; checked against Python's correctly rounded
+    formatting (the same digits glibc printf gives) on random values, halfway cases and extremes."""
+    lib = L.lib()
+    buf = C.create_string_buffer(512)
+    rng = np.random.default_rng(7)
+    vals = list(rng.uniform(-1, 1, 3000) * 10.0 ** rng.integers(-300, 300, 3000))
+    vals += [0.5, 1.5, 2.5, 0.125, 1e-5, 9.9995e-1, 0.0, -0.0, 1e300, 5e-324, 123456.5, 0.05, 0.15, 0.25]
+    vals += list(rng.uniform(0, 1, 2000))
+    n = 0
+    for v in vals:
+        for places in (0, 1, 3, 5, 8):
+            for sci in (0, 1):
+                if not sci and abs(v) > 1e30:
+                    continue
+                got = lib.bsq_format_double(float(v), places, sci, buf, 512)
+                assert got > 0
+                want = ("%.*e" if sci else "%.*f") % (places, v)
+                if sci and len(want) >= 3 and want[-3] == "0" and want[-4] in "+-":
+                    want = want[:-3] + want[-2:]
+                assert buf.value.decode() == want, (v, places, sci, buf.value, want)
+                n += 1
+    assert n > 40000

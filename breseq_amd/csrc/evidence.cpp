@@ -37,11 +37,24 @@ std::string error_rates_text(int S, int Q, const std::vector<double>& log10p) {
   s.reserve(log10p.size() * 24 + 128);
   s += "read_set=" + std::to_string(S) + ",ref_base,obs_base,quality=" + std::to_string(Q) + "\n";
   s += "read_set\tref_base\tobs_base\tquality\tlog10_probability\n";
-  char buf[96];
+  char num[32];
+  auto put_u = [&](size_t x) {
+    const std::to_chars_result r = std::to_chars(num, num + sizeof(num), x);
+    s.append(num, (size_t)(r.ptr - num));
+  };
   for (size_t i = 0; i < log10p.size(); i++) {
     const size_t rs = i % S, rb = (i / S) % 5, ob = (i / (5 * (size_t)S)) % 5, q = i / (25 * (size_t)S);
-    snprintf(buf, sizeof(buf), "%zu\t%c\t%c\t%zu\t%g\n", rs, kIdx[rb], kIdx[ob], q, log10p[i]);
-    s += buf;
+    // "%zu\t%c\t%c\t%zu\t%g\n"
+    put_u(rs);
+    s += '\t';
+    s += kIdx[rb];
+    s += '\t';
+    s += kIdx[ob];
+    s += '\t';
+    put_u(q);
+    s += '\t';
+    s += fmt_g(log10p[i]);
+    s += '\n';
   }
   return s;
 }
@@ -621,7 +634,7 @@ std::string test_ra_evidence_text(const std::string& text, const bsq_reference* 
 }  // namespace bsq
 
 extern "C" int32_t bsq_format_double(double v, uint32_t precision, int32_t scientific, char* buf, int32_t buf_len) {
-  const std::string s = bsq::fmt_double(v, precision, scientific != 0);
+  const std::string s = (scientific == 2) ? bsq::fmt_g(v, (int)precision) : bsq::fmt_double(v, precision, scientific != 0);
   if (!buf || buf_len <= (int32_t)s.size()) return -1;
   memcpy(buf, s.c_str(), s.size() + 1);
   return (int32_t)s.size();

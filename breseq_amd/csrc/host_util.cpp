@@ -47,10 +47,15 @@ int fmt_double_buf(double v, unsigned precision, bool scientific, char* buf, siz
   return n;
 }
 
-std::string fmt_g(double v) {
+// printf("%.*g") through std::to_chars (general format with a precision is specified as %g)
+std::string fmt_g(double v, int precision) {
   char buf[64];
-  snprintf(buf, sizeof(buf), "%g", v);
-  return buf;
+  const std::to_chars_result r = std::to_chars(buf, buf + sizeof(buf), v, std::chars_format::general, precision);
+  if (r.ec != std::errc()) {
+    snprintf(buf, sizeof(buf), "%.*g", precision, v);
+    return buf;
+  }
+  return std::string(buf, (size_t)(r.ptr - buf));
 }
 
 static bool repeat_at(const char* s, int64_t len, int64_t pos, int64_t size, int64_t num) {

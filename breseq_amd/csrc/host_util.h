@@ -23,7 +23,7 @@ std::string fmt_double(double v, unsigned precision, bool scientific);
 // fmt_double into buf (NUL-terminated); returns the length, or -1 if buf is too small.
 int fmt_double_buf(double v, unsigned precision, bool scientific, char* buf, size_t len);
 // iostream default formatting of a double (%g, 6 significant digits).
-std::string fmt_g(double v);
+std::string fmt_g(double v, int precision = 6);
 
 // SequenceTrims (calculate_trims.cpp:67-138): out[0..len) left, out[len..2len) right.
 void host_sequence_trims(const char* seq, int64_t len, uint8_t* out);

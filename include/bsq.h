@@ -188,8 +188,10 @@ int bsq_test_ra_evidence(const char* gd_text, const bsq_reference* ref,
 void bsq_free_text(char* text);
 
 /* breseq's number formatting for evidence fields, formatted_double / to_string(double, precision,
- * scientific) (common.h:845-866, 1542-1563): "NA" for NaN, otherwise %.Nf or %.Ne.  Returns the
- * length written (excluding NUL), or -1 if the buffer is too small. */
+ * scientific) (common.h:845-866, 1542-1563): "NA" for NaN, otherwise %.Nf (scientific 0) or %.Ne
+ * (scientific 1).  scientific 2 gives %.Ng, the default ostream form error_rates.tab is written in
+ * (error_count.cpp:660-690, N = 6).  Returns the length written (excluding NUL), or -1 if the
+ * buffer is too small. */
 int32_t bsq_format_double(double v, uint32_t precision, int32_t scientific, char* buf, int32_t buf_len);
 
 /* ------------------------------------------------------------------------------------------ */

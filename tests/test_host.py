@@ -154,3 +154,17 @@ def test_number_formatting_matches_printf_rounding():
                 assert buf.value.decode() == want, (v, places, sci, buf.value, want)
                 n += 1
     assert n > 40000
+
+
+def test_general_formatting_matches_printf_g():
+    """error_rates.tab values are printed as %g (6 significant digits); the %.*g mode of
+    bsq_format_double is checked against Python's %g on random log10 probabilities and extremes."""
+    lib = L.lib()
+    buf = C.create_string_buffer(128)
+    rng = np.random.default_rng(11)
+    vals = list(-rng.uniform(0, 8, 5000)) + list(np.log10(rng.uniform(1e-9, 1, 3000)))
+    vals += [0.0, -0.0, 1e-5, 1e-4, 123456.0, 1234567.0, 1e16, 5e-324, -2.5, 0.30103, 1e-300]
+    for v in vals:
+        for prec in (6, 3, 1):
+            assert lib.bsq_format_double(float(v), prec, 2, buf, 128) > 0
+            assert buf.value.decode() == "%.*g" % (prec, v), (v, prec, buf.value)

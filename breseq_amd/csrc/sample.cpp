@@ -14,6 +14,7 @@
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 
 #include "host_util.h"
 
@@ -643,6 +644,133 @@ size_t aux_size(char t, const unsigned char* v, const unsigned char* end) {
   throw BsqError(BSQ_ERR_FORMAT, std::string("unknown aux type ") + t);
 }
 
+int host_threads() {
+  const unsigned hc = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hc == 0 ? 4u : hc, 16u));
+}
+
+// Runs fn(t, begin, end) over [0, n) split into contiguous ranges, one per thread.
+template <class F>
+void parallel_ranges(int64_t n, int threads, F fn) {
+  threads = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n));
+  std::vector<std::thread> pool;
+  std::vector<std::string> errors(threads);
+  std::vector<int> codes(threads, 0);
+  for (int t = 0; t < threads; t++) {
+    const int64_t b = n * t / threads, e = n * (t + 1) / threads;
+    pool.emplace_back([&, t, b, e] {
+      try {
+        fn(t, b, e);
+      } catch (const BsqError& ex) {
+        codes[t] = ex.code;
+        errors[t] = ex.what();
+      } catch (const std::exception& ex) {
+        codes[t] = BSQ_ERR_FORMAT;
+        errors[t] = ex.what();
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  for (int t = 0; t < threads; t++)
+    if (codes[t]) throw BsqError(codes[t], errors[t]);
+}
+
+std::vector<unsigned char> read_file_bytes(const char* path) {
+  FILE* f = fopen(path, "rb");
+  if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not open ") + path);
+  std::vector<unsigned char> buf;
+  if (fseek(f, 0, SEEK_END) == 0) {
+    const long sz = ftell(f);
+    if (sz > 0) buf.resize((size_t)sz);
+    fseek(f, 0, SEEK_SET);
+  }
+  const size_t got = buf.empty() ? 0 : fread(buf.data(), 1, buf.size(), f);
+  fclose(f);
+  if (got != buf.size()) throw BsqError(BSQ_ERR_IO, std::string("short read of ") + path);
+  return buf;
+}
+
+// BGZF (SAMv1 section 4.1): a series of gzip members, each with a BC extra subfield holding the
+// member size.  Members are independent deflate streams, so they are inflated in parallel into
+// their offsets in the output (prefix sum of the ISIZE trailers), each one's CRC32 checked.
+// Returns false when the file is not BGZF-framed (the caller then reads it as one gzip stream).
+bool bgzf_inflate_parallel(const std::vector<unsigned char>& raw, std::vector<unsigned char>& out) {
+  struct Block {
+    size_t cdata, clen, uoff;
+    uint32_t usize, crc;
+  };
+  std::vector<Block> blocks;
+  size_t p = 0, uoff = 0;
+  while (p < raw.size()) {
+    if (raw.size() - p < 18 || raw[p] != 31 || raw[p + 1] != 139 || raw[p + 2] != 8 || !(raw[p + 3] & 4)) return false;
+    const uint16_t xlen = (uint16_t)(raw[p + 10] | (raw[p + 11] << 8));
+    size_t x = p + 12;
+    const size_t xe = x + xlen;
+    if (xe > raw.size()) return false;
+    int bsize = -1;
+    while (x + 4 <= xe) {
+      const uint16_t slen = (uint16_t)(raw[x + 2] | (raw[x + 3] << 8));
+      if (raw[x] == 'B' && raw[x + 1] == 'C' && slen == 2 && x + 6 <= xe) bsize = raw[x + 4] | (raw[x + 5] << 8);
+      x += 4 + (size_t)slen;
+    }
+    if (bsize < 0) return false;
+    const size_t end = p + (size_t)bsize + 1;
+    if (end > raw.size() || end < xe + 8) return false;
+    Block b;
+    b.cdata = xe;
+    b.clen = end - 8 - xe;
+    memcpy(&b.crc, &raw[end - 8], 4);
+    memcpy(&b.usize, &raw[end - 4], 4);
+    b.uoff = uoff;
+    uoff += b.usize;
+    blocks.push_back(b);
+    p = end;
+  }
+  out.resize(uoff);
+  parallel_ranges((int64_t)blocks.size(), host_threads(), [&](int, int64_t b0, int64_t b1) {
+    z_stream zs;
+    memset(&zs, 0, sizeof(zs));
+    if (inflateInit2(&zs, -15) != Z_OK) throw BsqError(BSQ_ERR_FORMAT, "zlib initialisation failed");
+    for (int64_t i = b0; i < b1; i++) {
+      const Block& b = blocks[(size_t)i];
+      inflateReset(&zs);
+      zs.next_in = const_cast<unsigned char*>(&raw[b.cdata]);
+      zs.avail_in = (uInt)b.clen;
+      zs.next_out = out.data() + b.uoff;
+      zs.avail_out = b.usize;
+      const int rc = inflate(&zs, Z_FINISH);
+      if (rc != Z_STREAM_END || zs.total_out != b.usize) {
+        inflateEnd(&zs);
+        throw BsqError(BSQ_ERR_FORMAT, "BGZF block does not inflate to its recorded size");
+      }
+      if ((uint32_t)crc32(0L, out.data() + b.uoff, b.usize) != b.crc) {
+        inflateEnd(&zs);
+        throw BsqError(BSQ_ERR_FORMAT, "BGZF block CRC mismatch");
+      }
+    }
+    inflateEnd(&zs);
+  });
+  return true;
+}
+
+std::vector<unsigned char> read_gzip_stream(const char* path) {
+  gzFile gz = gzopen(path, "rb");
+  if (!gz) throw BsqError(BSQ_ERR_IO, std::string("could not open BAM ") + path);
+  std::vector<unsigned char> data;
+  std::vector<unsigned char> chunk(1 << 22);
+  for (;;) {
+    const int n = gzread(gz, chunk.data(), (unsigned)chunk.size());
+    if (n < 0) {
+      gzclose(gz);
+      throw BsqError(BSQ_ERR_FORMAT, "BGZF decompression failed");
+    }
+    if (n == 0) break;
+    data.insert(data.end(), chunk.begin(), chunk.begin() + n);
+  }
+  gzclose(gz);
+  return data;
+}
+
 }  // namespace
 
 extern "C" int bsq_sample_read(const char* bam_path, const char* fasta_path, bsq_sample** out, bsq_status* status) {
@@ -651,18 +779,10 @@ extern "C" int bsq_sample_read(const char* bam_path, const char* fasta_path, bsq
     smp = new bsq_sample();
     Sample& S = smp->s;
     read_fasta(fasta_path, S);
-    gzFile gz = gzopen(bam_path, "rb");
-    if (!gz) throw BsqError(BSQ_ERR_IO, std::string("could not open BAM ") + bam_path);
     std::vector<unsigned char> data;
     {
-      std::vector<unsigned char> chunk(1 << 22);
-      for (;;) {
-        int n = gzread(gz, chunk.data(), (unsigned)chunk.size());
-        if (n < 0) { gzclose(gz); throw BsqError(BSQ_ERR_FORMAT, "BGZF decompression failed"); }
-        if (n == 0) break;
-        data.insert(data.end(), chunk.begin(), chunk.begin() + n);
-      }
-      gzclose(gz);
+      const std::vector<unsigned char> raw = read_file_bytes(bam_path);
+      if (!bgzf_inflate_parallel(raw, data)) data = read_gzip_stream(bam_path);
     }
     Reader R{data.data(), data.data() + data.size()};
     R.need(4);
@@ -713,69 +833,111 @@ extern "C" int bsq_sample_read(const char* bam_path, const char* fasta_path, bsq
       S.names = bam_names;
       S.seqs = seqs;
     }
+    // record boundaries and the CIGAR / sequence offsets of every record (sequential, cheap)
+    std::vector<size_t> rec_at;
     S.cigar_off.assign(1, 0);
     S.seq_off.assign(1, 0);
     while (R.p < R.e) {
-      int32_t bs = R.get<int32_t>();
+      const int32_t bs = R.get<int32_t>();
+      if (bs < 32) throw BsqError(BSQ_ERR_FORMAT, "BAM record shorter than its fixed fields");
       R.need((size_t)bs);
-      const unsigned char* rec_end = R.p + bs;
-      Reader Q{R.p, rec_end};
-      int32_t tid = Q.get<int32_t>();
-      int32_t pos = Q.get<int32_t>();
-      uint8_t lrn = Q.get<uint8_t>();
-      uint8_t mq = Q.get<uint8_t>();
-      (void)Q.get<uint16_t>();
-      uint16_t nc = Q.get<uint16_t>();
-      uint16_t flag = Q.get<uint16_t>();
-      int32_t lseq = Q.get<int32_t>();
-      int32_t mtid = Q.get<int32_t>();
-      int32_t mpos = Q.get<int32_t>();
-      int32_t tlen = Q.get<int32_t>();
-      Q.need(lrn);
-      S.read_names.emplace_back((const char*)Q.p, strnlen((const char*)Q.p, lrn));
-      Q.p += lrn;
-      for (uint16_t k = 0; k < nc; k++) S.cigar.push_back(Q.get<uint32_t>());
-      Q.need((size_t)(lseq + 1) / 2 + (size_t)lseq);
-      for (int32_t i = 0; i < lseq; i++) {
-        uint8_t byte = Q.p[i >> 1];
-        S.seq.push_back((i & 1) ? (byte & 0xf) : (byte >> 4));
-      }
-      Q.p += (lseq + 1) / 2;
-      S.qual.insert(S.qual.end(), Q.p, Q.p + lseq);
-      Q.p += lseq;
-      int32_t x1 = -1, xl = -1, xr = -1, rgi = -1;
-      while (Q.p < rec_end) {
-        Q.need(3);
-        char t0 = (char)Q.p[0], t1 = (char)Q.p[1], ty = (char)Q.p[2];
-        const unsigned char* v = Q.p + 3;
-        size_t sz = aux_size(ty, v, rec_end);
-        if ((size_t)(rec_end - v) < sz) throw BsqError(BSQ_ERR_FORMAT, "truncated aux field");
-        int64_t iv;
-        if (t0 == 'X' && t1 == '1' && aux_int(ty, v, iv)) x1 = (int32_t)iv;
-        else if (t0 == 'X' && t1 == 'L' && aux_int(ty, v, iv)) xl = (int32_t)iv;
-        else if (t0 == 'X' && t1 == 'R' && aux_int(ty, v, iv)) xr = (int32_t)iv;
-        else if (t0 == 'R' && t1 == 'G' && ty == 'Z') {
-          std::string id((const char*)v);
-          for (size_t g = 0; g < S.rg_ids.size(); g++)
-            if (S.rg_ids[g] == id) { rgi = (int32_t)g; break; }
-        }
-        Q.p = v + sz;
-      }
-      S.tid.push_back(tid);
-      S.pos.push_back(pos);
-      S.flag.push_back(flag);
-      S.mapq.push_back(mq);
-      S.mate_tid.push_back(mtid);
-      S.mate_pos.push_back(mpos);
-      S.tlen.push_back(tlen);
-      S.x1.push_back(x1);
-      S.xl.push_back(xl);
-      S.xr.push_back(xr);
-      S.rg.push_back(rgi);
-      S.cigar_off.push_back((int64_t)S.cigar.size());
-      S.seq_off.push_back((int64_t)S.seq.size());
-      R.p = rec_end;
+      uint16_t nc;
+      int32_t lseq;
+      memcpy(&nc, R.p + 12, 2);
+      memcpy(&lseq, R.p + 16, 4);
+      if (lseq < 0) throw BsqError(BSQ_ERR_FORMAT, "negative BAM sequence length");
+      rec_at.push_back((size_t)(R.p - data.data()));
+      S.cigar_off.push_back(S.cigar_off.back() + nc);
+      S.seq_off.push_back(S.seq_off.back() + lseq);
+      R.p += bs;
     }
+    const size_t nr = rec_at.size();
+    S.tid.resize(nr);
+    S.pos.resize(nr);
+    S.flag.resize(nr);
+    S.mapq.resize(nr);
+    S.mate_tid.resize(nr);
+    S.mate_pos.resize(nr);
+    S.tlen.resize(nr);
+    S.x1.resize(nr);
+    S.xl.resize(nr);
+    S.xr.resize(nr);
+    S.rg.resize(nr);
+    S.read_names.resize(nr);
+    S.cigar.resize((size_t)S.cigar_off.back());
+    S.seq.resize((size_t)S.seq_off.back());
+    S.qual.resize((size_t)S.seq_off.back());
+    // decode the records in parallel, each into its own slots
+    parallel_ranges((int64_t)nr, host_threads(), [&](int, int64_t r0, int64_t r1) {
+      for (int64_t r = r0; r < r1; r++) {
+        const size_t ri = (size_t)r;
+        const unsigned char* rec = data.data() + rec_at[ri];
+        int32_t bs;
+        memcpy(&bs, rec - 4, 4);
+        const unsigned char* rec_end = rec + bs;
+        Reader Q{rec, rec_end};
+        const int32_t tid = Q.get<int32_t>();
+        const int32_t pos = Q.get<int32_t>();
+        const uint8_t lrn = Q.get<uint8_t>();
+        const uint8_t mq = Q.get<uint8_t>();
+        (void)Q.get<uint16_t>();
+        const uint16_t nc = Q.get<uint16_t>();
+        const uint16_t flag = Q.get<uint16_t>();
+        const int32_t lseq = Q.get<int32_t>();
+        const int32_t mtid = Q.get<int32_t>();
+        const int32_t mpos = Q.get<int32_t>();
+        const int32_t tlen = Q.get<int32_t>();
+        Q.need(lrn);
+        S.read_names[ri].assign((const char*)Q.p, strnlen((const char*)Q.p, lrn));
+        Q.p += lrn;
+        Q.need((size_t)nc * 4);
+        if (nc) memcpy(&S.cigar[(size_t)S.cigar_off[ri]], Q.p, (size_t)nc * 4);
+        Q.p += (size_t)nc * 4;
+        Q.need((size_t)(lseq + 1) / 2 + (size_t)lseq);
+        if (lseq) {
+          uint8_t* sq = &S.seq[(size_t)S.seq_off[ri]];
+          for (int32_t i = 0; i < lseq; i++) {
+            const uint8_t byte = Q.p[i >> 1];
+            sq[i] = (i & 1) ? (byte & 0xf) : (byte >> 4);
+          }
+        }
+        Q.p += (lseq + 1) / 2;
+        if (lseq) memcpy(&S.qual[(size_t)S.seq_off[ri]], Q.p, (size_t)lseq);
+        Q.p += lseq;
+        int32_t x1 = -1, xl = -1, xr = -1, rgi = -1;
+        while (Q.p < rec_end) {
+          Q.need(3);
+          const char t0 = (char)Q.p[0], t1 = (char)Q.p[1], ty = (char)Q.p[2];
+          const unsigned char* v = Q.p + 3;
+          const size_t sz = aux_size(ty, v, rec_end);
+          if ((size_t)(rec_end - v) < sz) throw BsqError(BSQ_ERR_FORMAT, "truncated aux field");
+          int64_t iv;
+          if (t0 == 'X' && t1 == '1' && aux_int(ty, v, iv)) x1 = (int32_t)iv;
+          else if (t0 == 'X' && t1 == 'L' && aux_int(ty, v, iv)) xl = (int32_t)iv;
+          else if (t0 == 'X' && t1 == 'R' && aux_int(ty, v, iv)) xr = (int32_t)iv;
+          else if (t0 == 'R' && t1 == 'G' && ty == 'Z') {
+            const char* id = (const char*)v;
+            for (size_t g = 0; g < S.rg_ids.size(); g++)
+              if (S.rg_ids[g] == id) {
+                rgi = (int32_t)g;
+                break;
+              }
+          }
+          Q.p = v + sz;
+        }
+        S.tid[ri] = tid;
+        S.pos[ri] = pos;
+        S.flag[ri] = flag;
+        S.mapq[ri] = mq;
+        S.mate_tid[ri] = mtid;
+        S.mate_pos[ri] = mpos;
+        S.tlen[ri] = tlen;
+        S.x1[ri] = x1;
+        S.xl[ri] = xl;
+        S.xr[ri] = xr;
+        S.rg[ri] = rgi;
+      }
+    });
     S.finalize();
     *out = smp;
     set_status(status, BSQ_OK, "");

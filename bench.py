@@ -3,10 +3,12 @@
 
 One step = one pass of the whole path over one sample already resident in HBM:
   error_count (coverage histogram + covariate counts) -> error_rates.tab round trip ->
-  identify_mutations (RA / MC / UN scoring) -> ra_mc_evidence.gd text.
+  identify_mutations (RA / MC / UN scoring) -> ra_mc_evidence.gd text, with the discordant-pair
+  accumulator -> DP_candidate_regions.csv text (breseq runs it on every paired sample).
 Metric: aligned bases/s (pileup entries: M/=/X bases + D positions), plus p50/p99 step latency.
 Workload: 4.6 Mb reference (GC 50.8%), 100x of 2x150 bp paired reads, insert N(350,50),
-qualities N(34,5), 500 SNPs + 50 indels of 1-5 bp, consensus mode.  Synthetic, seeded.
+qualities N(34,5), 500 SNPs + 50 indels of 1-5 bp, 0.2% discordant pairs at 40 junctions,
+consensus mode.  Synthetic, seeded.
 """
 import argparse
 import json
@@ -21,7 +23,10 @@ HBM_PEAK_GBPS = 8000.0
 
 C2 = dict(seed=2, ref_length=4_600_000, gc=0.508, read_length=150, coverage=100.0, paired=True, insert_mean=350,
           insert_sd=50, qual_mean=34, qual_sd=5, qual_min=2, qual_max=41, n_snps=500, n_indels=50, indel_max=5,
-          redundant_fraction=0.01, mapq=42)
+          redundant_fraction=0.01, mapq=42, discordant_fraction=0.002, n_junctions=40)
+
+# paired-mapping-distance fit of the one read file set: median and MAD of N(350, 50)
+DP_SET = dict(median=350.0, mad=50.0 * 0.6745, r1_prefix=1, r2_prefix=2)
 
 # Per-unit algorithmic bytes of the column kernels: 1 B base + 1 B quality per aligned base,
 # 24 B of per-read metadata (position, CIGAR, flags, tags), and per reference column 1 B of
@@ -66,11 +71,12 @@ def percentile(xs, q):
     return xs[f] + (xs[c] - xs[f]) * (k - f)
 
 
-def run_step(B, sess, mode):
+def run_step(B, sess, mode, dp):
     ec = sess.error_count()
     prop = [5.0] * sess.sample.n_targets
     gd = sess.identify_mutations(log10_prob=ec.rates_table, read_set_max=ec.read_set_max,
-                                 quality_max=ec.quality_max, propagation_cutoffs=prop, mode=mode)
+                                 quality_max=ec.quality_max, propagation_cutoffs=prop, mode=mode, dp=dp,
+                                 with_dp=True)
     return ec, gd
 
 
@@ -91,8 +97,9 @@ def cpu_baseline(B, cfg, target_seconds):
     args = api.make_error_count_args(s)
     hists, counts, log10p, texts = O.error_count(args)
     table = O.read_log10_prob_table(texts["error_rates"], len(log10p))
-    iargs = api.make_identify_args(s, table, args.read_set_max, args.quality_max, [5.0] * s.n_targets)
-    O.identify_mutations(iargs)
+    iargs = api.make_identify_args(s, table, args.read_set_max, args.quality_max, [5.0] * s.n_targets,
+                                   dp=api.dp_settings([DP_SET], seed=3))
+    O.identify_with_dp(iargs)
     dt = time.perf_counter() - t0
     return {"value": nab / dt, "unit": "aligned bases/s", "cores": 1, "kind": "port",
             "sample": "%d bp reference, %.0fx 2x%d bp paired reads, same generator and mode (%d aligned bases, "
@@ -123,12 +130,13 @@ def main():
     sample = B.Sample.simulate(**cfg)
     log("[bench] simulated in %.1f s" % (time.time() - t))
     sess = B.Session().load(sample)
+    dp = B.api.dp_settings([DP_SET], seed=3)
     tm = sess.timing()
     nab, n_reads, n_cols = tm["aligned_bases"], tm["n_reads"], tm["n_positions"]
     log("[bench] loaded: %d aligned bases, %d reads, %d columns; pack %.0f ms, H2D %.0f ms"
         % (nab, n_reads, n_cols, tm["pack_ms"], tm["h2d_ms"]))
     for i in range(args.warmup):
-        run_step(B, sess, B.CONSENSUS)
+        run_step(B, sess, B.CONSENSUS, dp)
         log("[bench] warmup step %d done" % i)
     step_ms = []
     ec_k, im_k = [], []
@@ -141,8 +149,9 @@ def main():
         tm = time.perf_counter()
         ect = sess.timing()
         prop = [5.0] * sample.n_targets
-        gd = sess.identify_mutations(log10_prob=ec.rates_table, read_set_max=ec.read_set_max,
-                                     quality_max=ec.quality_max, propagation_cutoffs=prop, mode=B.CONSENSUS)
+        gd, dp_text = sess.identify_mutations(log10_prob=ec.rates_table, read_set_max=ec.read_set_max,
+                                              quality_max=ec.quality_max, propagation_cutoffs=prop, mode=B.CONSENSUS,
+                                              dp=dp, with_dp=True)
         t1 = time.perf_counter()
         imt = sess.timing()
         step_ms.append((t1 - t0) * 1000.0)
@@ -191,9 +200,11 @@ def main():
         "dtype": "u8 bases/quals, int32 counts, f64 likelihoods",
         "data": "synthetic (seeded generator: random reference, planted variants, reads aligned by construction)",
         "config": {"workload": "consensus-mode pileup scoring of one sample: %d bp reference at %.0fx, 2x150 bp "
-                               "paired reads, insert N(350,50), quals N(34,5), 500 SNPs + 50 indels"
-                               % (cfg["ref_length"], cfg["coverage"]),
+                               "paired reads, insert N(350,50), quals N(34,5), 500 SNPs + 50 indels, "
+                               "%.1f%% discordant pairs" % (cfg["ref_length"], cfg["coverage"],
+                                                            100.0 * cfg["discordant_fraction"]),
                    "aligned_bases": nab, "reads": n_reads, "ra_entries": n_ra,
+                   "dp_regions": dp_text.count("\n") - 1,
                    "candidates": detail["candidates"]},
         "kernels_ms": kernels,
         "host_ms": {k: sum(v) / len(v) for k, v in host.items()},

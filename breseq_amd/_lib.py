@@ -12,6 +12,7 @@ c_i32p = C.POINTER(C.c_int32)
 c_u32p = C.POINTER(C.c_uint32)
 c_i64p = C.POINTER(C.c_int64)
 c_u8p = C.POINTER(C.c_uint8)
+c_i8p = C.POINTER(C.c_int8)
 c_u16p = C.POINTER(C.c_uint16)
 c_f64p = C.POINTER(C.c_double)
 c_charpp = C.POINTER(C.c_char_p)
@@ -43,6 +44,10 @@ class Alignments(C.Structure):
         ("xl", c_i32p),
         ("xr", c_i32p),
         ("rg", c_i32p),
+        ("tlen", c_i32p),
+        ("xp", c_i8p),
+        ("name_off", c_i64p),
+        ("names", C.c_char_p),
     ]
 
 
@@ -73,6 +78,18 @@ class ErrorCountArgs(C.Structure):
     ]
 
 
+class DpSettings(C.Structure):
+    _fields_ = [
+        ("n_sets", C.c_int32),
+        ("paired", c_u8p),
+        ("median", c_f64p),
+        ("mad", c_f64p),
+        ("r1_prefix", c_u32p),
+        ("r2_prefix", c_u32p),
+        ("seed", C.c_int32),
+    ]
+
+
 class IdentifyArgs(C.Structure):
     _fields_ = [
         ("ref", C.POINTER(Reference)),
@@ -92,6 +109,8 @@ class IdentifyArgs(C.Structure):
         ("base_quality_cutoff", C.c_int32),
         ("skip_missing_coverage_prediction", C.c_int32),
         ("gd_file", C.c_char_p),
+        ("dp", C.POINTER(DpSettings)),
+        ("dp_file", C.c_char_p),
     ]
 
 
@@ -170,6 +189,8 @@ class SimConfig(C.Structure):
         ("variant_freq_max", C.c_double),
         ("redundant_fraction", C.c_double),
         ("mapq", C.c_int32),
+        ("discordant_fraction", C.c_double),
+        ("n_junctions", C.c_int32),
     ]
 
 
@@ -228,6 +249,9 @@ def lib():
     _decl(L, "bsq_ec_result_free", None, [vp])
     _decl(L, "bsq_im_result_gd_text", C.c_int, [vp, P(C.c_char_p), c_i64p])
     _decl(L, "bsq_im_result_counts", C.c_int, [vp, c_i64p, c_i64p, c_i64p])
+    _decl(L, "bsq_im_result_dp_text", C.c_int, [vp, P(C.c_char_p), c_i64p])
+    _decl(L, "bsq_dp_candidate_regions", C.c_int,
+          [P(Reference), P(Alignments), c_u8p, c_f64p, P(DpSettings), P(C.c_void_p), c_i64p, st])
     _decl(L, "bsq_im_result_free", None, [vp])
     _decl(L, "bsq_ra_test_settings_default", None, [P(RaTestSettings), C.c_int32])
     _decl(L, "bsq_test_ra_evidence", C.c_int, [C.c_char_p, P(Reference), P(RaTestSettings), P(C.c_void_p), c_i64p, st])

@@ -4,6 +4,7 @@ Function names and argument meanings follow the reference entry points:
   error_count(...)            error_count.h:41-52
   identify_mutations(...)     identify_mutations.h:46-60
   test_RA_evidence(...)       identify_mutations.h:63-67
+  dp_candidate_regions(...)   identify_mutations.cpp:883-908, 1395-1457, 2358-2422, 2550-2573
   calculate_trims(...)        calculate_trims.h:34 (SequenceTrims, calculate_trims.cpp:67-138)
   fit_coverage_distribution   coverage_distribution.cpp:345-399
 Errors raise BsqError with the C ABI's code instead of ending the process.
@@ -42,10 +43,10 @@ class Sample:
     def simulate(cls, seed=1, ref_length=48502, gc=0.5, read_length=100, coverage=100.0, paired=False,
                  insert_mean=350.0, insert_sd=50.0, qual_mean=32.0, qual_sd=6.0, qual_min=2, qual_max=41,
                  n_snps=20, n_indels=2, indel_max=5, variant_freq_min=1.0, variant_freq_max=1.0,
-                 redundant_fraction=0.01, mapq=42):
+                 redundant_fraction=0.01, mapq=42, discordant_fraction=0.0, n_junctions=0):
         cfg = L.SimConfig(seed, ref_length, gc, read_length, coverage, 1 if paired else 0, insert_mean, insert_sd,
                           qual_mean, qual_sd, qual_min, qual_max, n_snps, n_indels, indel_max, variant_freq_min,
-                          variant_freq_max, redundant_fraction, mapq)
+                          variant_freq_max, redundant_fraction, mapq, discordant_fraction, n_junctions)
         h = C.c_void_p()
         st = _status()
         L.check(L.lib().bsq_simulate(C.byref(cfg), C.byref(h), C.byref(st)), st)
@@ -101,7 +102,15 @@ class Sample:
             mapq=_np(a.mapq, n, np.uint8), cigar_off=_np(a.cigar_off, n + 1, np.int64),
             cigar=_np(a.cigar, nc, np.uint32), seq_off=_np(a.seq_off, n + 1, np.int64),
             seq=_np(a.seq, nb, np.uint8), qual=_np(a.qual, nb, np.uint8), x1=_np(a.x1, n, np.int32),
-            xl=_np(a.xl, n, np.int32), xr=_np(a.xr, n, np.int32), rg=_np(a.rg, n, np.int32))
+            xl=_np(a.xl, n, np.int32), xr=_np(a.xr, n, np.int32), rg=_np(a.rg, n, np.int32),
+            tlen=_np(a.tlen, n, np.int32), xp=_np(a.xp, n, np.int8))
+
+    def read_names(self):
+        a = self.aln.contents
+        n = a.n_reads
+        off = _np(a.name_off, n + 1, np.int64)
+        raw = C.string_at(a.names, int(off[-1])) if n else b""
+        return [raw[off[i]:off[i + 1]].decode() for i in range(n)]
 
     def reference_sequences(self):
         r = self.ref.contents
@@ -205,6 +214,37 @@ class Session:
         return identify_mutations(self.sample, session=self, **kw)
 
 
+def dp_settings(sets, seed=3):
+    """bsq_dp_settings from one dict per read file set: paired, median, mad, r1_prefix, r2_prefix
+    (identify_mutations.cpp:893-908; seed = Settings::discordant_pair_seed)."""
+    n = len(sets)
+    paired = (C.c_uint8 * n)(*[1 if x.get("paired", True) else 0 for x in sets])
+    med = (C.c_double * n)(*[float(x["median"]) for x in sets])
+    mad = (C.c_double * n)(*[float(x["mad"]) for x in sets])
+    r1 = (C.c_uint32 * n)(*[int(x.get("r1_prefix", 2 * i + 1)) for i, x in enumerate(sets)])
+    r2 = (C.c_uint32 * n)(*[int(x.get("r2_prefix", 2 * i + 2)) for i, x in enumerate(sets)])
+    d = L.DpSettings(n, paired, med, mad, r1, r2, seed)
+    d._keep = (paired, med, mad, r1, r2)
+    return d
+
+
+def dp_candidate_regions(sample, dp, call_targets=None, propagation_cutoffs=None):
+    """DP_candidate_regions.csv text for a sample (host accumulator, no GPU)."""
+    call = _target_arrays(sample, call_targets)
+    prop = None
+    if propagation_cutoffs is not None:
+        prop = (C.c_double * sample.n_targets)(*[float(x) for x in propagation_cutoffs])
+    out = C.c_void_p()
+    n = C.c_int64()
+    st = _status()
+    L.check(L.lib().bsq_dp_candidate_regions(sample.ref, sample.aln, call, prop, C.byref(dp) if dp is not None else None,
+                                             C.byref(out), C.byref(n), C.byref(st)), st)
+    try:
+        return C.string_at(out, n.value).decode()
+    finally:
+        L.lib().bsq_free_text(out)
+
+
 def _target_arrays(sample, call_targets):
     n = sample.n_targets
     call = (C.c_uint8 * n)(*[1 if (call_targets is None or i in call_targets) else 0 for i in range(n)])
@@ -282,7 +322,8 @@ def mode_settings(mode):
 
 
 def make_identify_args(sample, log10_prob, read_set_max, quality_max, propagation_cutoffs, seed_cutoffs=None,
-                       mode=CONSENSUS, call_targets=None, base_quality_cutoff=3, skip_mc=False, gd_file=None):
+                       mode=CONSENSUS, call_targets=None, base_quality_cutoff=3, skip_mc=False, gd_file=None,
+                       dp=None, dp_file=None):
     n = sample.n_targets
     mut, poly, dec, places = mode_settings(mode)
     table = np.ascontiguousarray(log10_prob, dtype=np.float64)
@@ -308,13 +349,16 @@ def make_identify_args(sample, log10_prob, read_set_max, quality_max, propagatio
     args.base_quality_cutoff = base_quality_cutoff
     args.skip_missing_coverage_prediction = 1 if skip_mc else 0
     args.gd_file = gd_file.encode() if gd_file else None
-    args._keep = (table, prop, seed, call, rf)
+    args.dp = C.pointer(dp) if dp is not None else None
+    args.dp_file = dp_file.encode() if dp_file else None
+    args._keep = (table, prop, seed, call, rf, dp)
     return args
 
 
 def identify_mutations(sample, log10_prob, read_set_max, quality_max, propagation_cutoffs, session=None,
-                       screen=True, **kw):
-    """identify_mutations (identify_mutations.cpp:48-88) on the GPU.  Returns ra_mc_evidence.gd text."""
+                       screen=True, with_dp=False, **kw):
+    """identify_mutations (identify_mutations.cpp:48-88) on the GPU.  Returns ra_mc_evidence.gd text,
+    or (gd text, DP_candidate_regions.csv text) with with_dp."""
     args = make_identify_args(sample, log10_prob, read_set_max, quality_max, propagation_cutoffs, **kw)
     lib = L.lib()
     res = C.c_void_p()
@@ -331,9 +375,11 @@ def identify_mutations(sample, log10_prob, read_set_max, quality_max, propagatio
         n = C.c_int64()
         lib.bsq_im_result_gd_text(res, C.byref(p), C.byref(n))
         text = C.string_at(p, n.value).decode()
+        lib.bsq_im_result_dp_text(res, C.byref(p), C.byref(n))
+        dp_text = C.string_at(p, n.value).decode()
     finally:
         lib.bsq_im_result_free(res)
-    return text
+    return (text, dp_text) if with_dp else text
 
 
 def ra_test_settings(polymorphism_prediction=False, **overrides):

@@ -22,7 +22,7 @@ SOURCES = [
     "sample.cpp",
     "host_util.cpp",
     "evidence.cpp",
-    "covfit.cpp",
+    "covfit.cpp", "dp_regions.cpp",
 ]
 
 # -ffp-contract=off: the FP64 scorer must round every multiply and add separately, as the

@@ -209,6 +209,17 @@ __device__ __forceinline__ bool slot_observation(const DRead& R, const uint8_t* 
   return true;
 }
 
+// Workgroups are dispatched to the eight XCDs round robin (block b runs on XCD b % 8).  This
+// renumbering gives each XCD one contiguous range of logical block indices, so work assigned by
+// logical index in contiguous chunks keeps neighbouring chunks in the same L2.  Grids are launched
+// with a multiple of 8 blocks; any remainder keeps its index.
+__device__ __forceinline__ int xcd_major_block() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int per = nb / 8;
+  if (b >= per * 8) return b;
+  return (b % 8) * per + b / 8;
+}
+
 // pdata entry of alignment R at (global column p, insert slot): the per-read filters of
 // identify_mutations_pileup::pileup_callback (identify_mutations.cpp:1561-1657).  On success key is
 // ((mslot*S + read file)*Q + quality)*10 + obs*2 + (reverse strand), the row of the likelihood

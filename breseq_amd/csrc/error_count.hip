@@ -90,21 +90,37 @@ __device__ __forceinline__ int run_end_indel(const ConstU32* c, int k, int n) {
 
 __device__ __forceinline__ uint32_t byte_of(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
 
+// Strand-folded base indices as 4-bit lookup tables: read bases by BAM code (A1 C2 G4 T8), reference
+// bases by kRef* code (A0 C1 G2 T3 '.'4 N5).  The reverse-strand tables complement A<->T, C<->G, as
+// count_alignment_position does before indexing (error_count.cpp:887-903).  15 = not a base.
+constexpr uint64_t kReadLutFwd = 0xFFFFFFF3FFF2F10FULL;
+constexpr uint64_t kReadLutRev = 0xFFFFFFF0FFF1F23FULL;
+constexpr uint32_t kRefLutFwd = 0xFF543210u;
+constexpr uint32_t kRefLutRev = 0xFF540123u;
+
 // count_alignment_position (error_count.cpp:854-986) for the aligned bases [x, x+len) <-> query
 // [y, y+len) of one unique read; the last base carries `last_indel`.  Each lane takes four
 // consecutive query positions of a 4-aligned group: reads start at 8-byte aligned offsets, so the
 // bases and qualities of the group and of the next four positions are two dword loads each; the
-// five reference codes the group needs come from two aligned dwords of ref_idx.
+// five reference codes the group needs come from two aligned dwords of ref_idx.  Per base: the
+// base/reference observation (1) and the '..' pair observation (2) are one LDS atomic each; the
+// one-base deletion (3) and insertion (4) observations can only occur at the run's last base and
+// are handled there.
+template <bool kRev>
 __device__ __forceinline__ void ec_count_run(const EcParams& P, const uint8_t* __restrict__ sq,
                                              const uint8_t* __restrict__ ql, const DRead& R, int x, int y, int len,
                                              int last_indel, uint32_t* table, uint32_t& err) {
-  const int S = P.read_set_max, Q = P.quality_max;
-  const int rev = (R.flags & kReadReversed) ? 1 : 0;
-  const uint32_t rv = rev ? 3u : 0u;
-  const int rs = R.rfile;
+  const uint32_t S = (uint32_t)P.read_set_max, Q = (uint32_t)P.quality_max;
+  const uint64_t rlut = kRev ? kReadLutRev : kReadLutFwd;
+  const uint32_t flut = kRev ? kRefLutRev : kRefLutFwd;
+  const uint32_t rs = R.rfile;
+  const uint32_t pair_base = rs + 24u * S;   // ref '.', obs '.'
   const int lane = threadIdx.x & 63;
   const int yend = y + len;
   const int last = yend - 1;
+  const int qe0 = (int)R.qe0;
+  // pair observations end at q_end, and at the last base when a deletion or insertion follows it
+  const int pair_end = (last_indel != 0) ? min(qe0, last) : qe0;
   for (int q0 = (y & ~3) + 4 * lane; q0 < yend; q0 += 256) {
     const uint32_t bw0 = *(const uint32_t*)(sq + q0), bw1 = *(const uint32_t*)(sq + q0 + 4);
     const uint32_t qw0 = *(const uint32_t*)(ql + q0), qw1 = *(const uint32_t*)(ql + q0 + 4);
@@ -117,55 +133,58 @@ __device__ __forceinline__ void ec_count_run(const EcParams& P, const uint8_t* _
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int qpos = q0 + k;
-      if (qpos < y || qpos > last) continue;
+      const bool in = qpos >= y && qpos <= last;
       const uint32_t b = byte_of(bw0, k), q = byte_of(qw0, k);
-      const uint32_t nb = (k < 3) ? byte_of(bw0, k + 1) : byte_of(bw1, 0);   // query position qpos+1
-      const uint32_t nq = (k < 3) ? byte_of(qw0, k + 1) : byte_of(qw1, 0);
       const uint32_t r0 = byte_of(rlo, k);
-      const uint32_t r1 = (k < 3) ? byte_of(rlo, k + 1) : r4;               // column p+1
-      const int indel = (qpos == last) ? last_indel : 0;
-      // (1) base against reference, both complemented on the reverse strand (:877-903)
-      if (b != 15u && r0 != kRefN) {
-        const uint32_t oi = comp_if(base_idx(b), rv), ri = comp_if(r0, rv);
-        if (oi > 3u) err |= kErrBaseCode;
-        else if (ri > 3u) err |= kErrRefChar;
-        else if (q >= (uint32_t)Q) err |= kErrQuality;
-        else atomicAdd(&table[rs + S * (int)ri + 5 * S * (int)oi + 25 * S * (int)q], 1u);
-      }
-      // (2) '..' pair and (3) one-base deletion read the base after qpos in read direction
-      // (qpos+1 forward, qpos itself reverse)
-      const bool pair = (indel == 0) && (qpos < (int)R.qe0);
-      if (pair || indel == -1) {
-        if (!rev && qpos + 1 >= (int)R.l_qseq) {
-          err |= kErrQueryIndex;
-        } else {
-          const uint32_t b2 = rev ? b : nb;
-          const uint32_t q2 = rev ? q : nq;
-          if (pair) {   // (:909-928): reference at qpos+1-rev in reference direction
-            const uint32_t rp2 = rev ? r0 : r1;
-            if (b2 != 15u && rp2 != kRefN) {
-              if (q2 >= (uint32_t)Q) err |= kErrQuality;
-              else atomicAdd(&table[rs + S * 4 + 5 * S * 4 + 25 * S * (int)q2], 1u);
-            }
-          } else if (b2 != 15u && r1 != kRefN) {   // (:934-956): reference base after the deletion
-            const uint32_t rd = comp_if(r1, rv);
-            if (rd > 4u) err |= kErrRefChar;
-            else if (q2 >= (uint32_t)Q) err |= kErrQuality;
-            else atomicAdd(&table[rs + S * (int)rd + 5 * S * 4 + 25 * S * (int)q2], 1u);
-          }
-        }
-      } else if (indel == 1) {   // (4) one-base insertion (:963-982): the inserted base at qpos+1
-        const int mq = qpos + 1;
-        if (mq >= (int)R.l_qseq) {
-          err |= kErrQueryIndex;
-        } else if (mq <= (int)R.qe0 && mq >= (int)R.qs0 && nb != 15u) {
-          const uint32_t oi = comp_if(base_idx(nb), rv);
-          if (oi > 3u) err |= kErrBaseCode;
-          else if (nq >= (uint32_t)Q) err |= kErrQuality;
-          else atomicAdd(&table[rs + S * 4 + 5 * S * (int)oi + 25 * S * (int)nq], 1u);
-        }
-      }
+      // (1) base against reference (:877-903)
+      const uint32_t oi = (b > 15u) ? 15u : (uint32_t)(rlut >> (4u * b)) & 15u;
+      const uint32_t ri = (flut >> (4u * r0)) & 15u;
+      const bool v1 = in && b != 15u && r0 != kRefN;
+      const bool good1 = v1 && oi < 4u && ri < 4u && q < Q;
+      if (v1 && !good1) err |= (oi > 3u) ? kErrBaseCode : (ri > 3u) ? kErrRefChar : kErrQuality;
+      if (good1) atomicAdd(&table[rs + S * ri + 5u * S * oi + 25u * S * q], 1u);
+      // (2) '..' pair (:909-928): the base after qpos in read direction (qpos+1 forward, qpos
+      // itself reverse) against the reference after it in reference direction
+      const uint32_t b2 = kRev ? b : ((k < 3) ? byte_of(bw0, k + 1) : byte_of(bw1, 0));
+      const uint32_t q2 = kRev ? q : ((k < 3) ? byte_of(qw0, k + 1) : byte_of(qw1, 0));
+      const uint32_t rp2 = kRev ? r0 : ((k < 3) ? byte_of(rlo, k + 1) : r4);
+      const bool v2 = in && qpos < pair_end && b2 != 15u && rp2 != kRefN;
+      if (v2 && q2 >= Q) err |= kErrQuality;
+      if (v2 && q2 < Q) atomicAdd(&table[pair_base + 25u * S * q2], 1u);
     }
+  }
+  if (last_indel != -1 && last_indel != 1) return;
+  // (3) one-base deletion (:934-956) / (4) one-base insertion (:963-982) after the last base
+  if (lane != 0) return;
+  const int qpos = last, p = x + len - 1;
+  const uint32_t b = sq[qpos], q = ql[qpos];
+  const uint32_t r1 = P.ref_idx[p + 1];
+  if (last_indel == -1) {
+    if (!kRev && qpos + 1 >= (int)R.l_qseq) {
+      err |= kErrQueryIndex;
+      return;
+    }
+    const uint32_t b2 = kRev ? b : (uint32_t)sq[qpos + 1];
+    const uint32_t q2 = kRev ? q : (uint32_t)ql[qpos + 1];
+    if (b2 == 15u || r1 == kRefN) return;
+    const uint32_t rd = (flut >> (4u * r1)) & 15u;
+    if (rd > 4u) err |= kErrRefChar;
+    else if (q2 >= Q) err |= kErrQuality;
+    else atomicAdd(&table[rs + S * rd + 20u * S + 25u * S * q2], 1u);
+  } else {
+    const int mq = qpos + 1;
+    if (mq >= (int)R.l_qseq) {
+      err |= kErrQueryIndex;
+      return;
+    }
+    if (mq > qe0 || mq < (int)R.qs0) return;
+    const uint32_t b3 = sq[mq];
+    if (b3 == 15u) return;
+    const uint32_t oi = (b3 > 15u) ? 15u : (uint32_t)(rlut >> (4u * b3)) & 15u;
+    const uint32_t q3 = ql[mq];
+    if (oi > 3u) err |= kErrBaseCode;
+    else if (q3 >= Q) err |= kErrQuality;
+    else atomicAdd(&table[rs + 4u * S + 5u * S * oi + 25u * S * q3], 1u);
   }
 }
 
@@ -176,7 +195,7 @@ __global__ void __launch_bounds__(256) k_ec_reads(EcParams P) {
   for (int i = threadIdx.x; i < P.n_bins; i += blockDim.x) table[i] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t gw = ((int64_t)xcd_major_block() * blockDim.x + threadIdx.x) >> 6;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const ConstI32* pos_target = (const ConstI32*)P.pos_target;
   const ConstI32* scored = (const ConstI32*)P.target_scored;
@@ -203,7 +222,10 @@ __global__ void __launch_bounds__(256) k_ec_reads(EcParams P) {
           atomicAdd(&diff[R.gpos], 1);
           atomicAdd(&diff[R.gend], -1);
         }
-        if (count) ec_count_run(P, sq, ql, R, R.gpos, R.y0, R.gend - R.gpos, 0, table, err);
+        if (count) {
+          if (R.flags & kReadReversed) ec_count_run<true>(P, sq, ql, R, R.gpos, R.y0, R.gend - R.gpos, 0, table, err);
+          else ec_count_run<false>(P, sq, ql, R, R.gpos, R.y0, R.gend - R.gpos, 0, table, err);
+        }
         continue;
       }
       const ConstU32* c = (const ConstU32*)(P.cigar + R.cig_off);
@@ -219,7 +241,11 @@ __global__ void __launch_bounds__(256) k_ec_reads(EcParams P) {
               atomicAdd(&diff[x], 1);
               atomicAdd(&diff[x + l], -1);
             }
-            if (count) ec_count_run(P, sq, ql, R, x, y, l, run_end_indel(c, k, n), table, err);
+            if (count) {
+              const int li = run_end_indel(c, k, n);
+              if (R.flags & kReadReversed) ec_count_run<true>(P, sq, ql, R, x, y, l, li, table, err);
+              else ec_count_run<false>(P, sq, ql, R, x, y, l, li, table, err);
+            }
           }
           x += l;
           y += l;

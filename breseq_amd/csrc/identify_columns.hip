@@ -55,6 +55,7 @@ struct Entry {
   uint32_t b, q;       // base code and quality at qpos (b = '.' for deletions)
   uint32_t qd;         // deletions: quality of the next read base in read direction
   bool del_ok;         // deletions: that base exists and is not N
+  bool trimmed;        // is_trimmed at slot 0 (alignment.h:389-410)
 };
 
 template <bool kLds>
@@ -139,6 +140,7 @@ __device__ __forceinline__ void prepare_general(const ImParams& P, const DRead& 
   e.q = 0;
   e.qd = 0;
   e.del_ok = false;
+  e.trimmed = is_trimmed(R, e.qpos, e.ondel < 0);
   if (e.cover && !is_del) {
     e.b = sq[e.qpos];
     e.q = ql[e.qpos];
@@ -152,6 +154,26 @@ __device__ __forceinline__ void prepare_general(const ImParams& P, const DRead& 
       e.qd = ql[qq];
       e.del_ok = true;
     }
+  }
+}
+
+// The per-call alignment view of the column pass (thread per alignment).
+__global__ void __launch_bounds__(256) k_prep_reads(const DRead* reads, int64_t n, int32_t S, int32_t Q, IdRec* out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const DRead R = reads[i];
+    IdRec c;
+    c.gpos = R.gpos;
+    c.gend = R.gend;
+    c.seq_lo = (int32_t)(uint32_t)((uint64_t)R.seq_off & 0xffffffffu);
+    c.seq_hi = (int32_t)(uint32_t)((uint64_t)R.seq_off >> 32);
+    c.qoff = (int32_t)R.y0 - R.gpos;
+    // q1 = qpos + 1 <= XL  <=>  qpos < XL;  l_qseq - q1 + 1 <= XR  <=>  qpos >= l_qseq - XR
+    c.trim_lo = (R.xl >= 0) ? (int32_t)R.xl : 0;
+    c.trim_hi = (R.xr >= 0) ? (int32_t)R.l_qseq - (int32_t)R.xr : 0x7fffffff;
+    c.row_base = ((int)R.rfile < S) ? (((uint32_t)R.mslot * (uint32_t)S + R.rfile) * (uint32_t)Q) * 5u : 0xffffffffu;
+    c.x1 = R.x1;
+    c.flags = R.flags;
+    out[i] = c;
   }
 }
 
@@ -311,7 +333,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
   const int zero_row = P.fold_rows;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int64_t chunk = (P.n_windows + gridDim.x - 1) / gridDim.x;
-  const int64_t w_begin = (int64_t)blockIdx.x * chunk;
+  const int64_t w_begin = (int64_t)xcd_major_block() * chunk;
   const int64_t w_end = min<int64_t>(w_begin + chunk, P.n_windows);
   uint32_t err = 0;
   const int S = P.read_set_max, Q = P.quality_max;
@@ -343,58 +365,56 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
     }
 
     // pipeline: the record and bytes of alignment r+1 are requested before alignment r is added
-    DRead Rn;
+    typedef __attribute__((address_space(4))) const uint32_t ConstU32;
+    uint32_t cn[10];
     Entry En;
     auto prepare = [&](int r) {
-      Rn = load_read(P.reads, r);
-      if (Rn.flags & kReadSimple) {
-        const int dp = p - Rn.gpos;
-        En.cover = lane_ok && (unsigned)dp < (unsigned)(Rn.gend - Rn.gpos);
+      const ConstU32* c = (const ConstU32*)(P.idrec + r);
+#pragma unroll
+      for (int k = 0; k < 10; k++) cn[k] = c[k];
+      const int gpos = (int)cn[0], gend = (int)cn[1];
+      const uint8_t* bb = P.bases + (int64_t)(((uint64_t)cn[3] << 32) | cn[2]);
+      const uint8_t* qb = P.quals + (int64_t)(((uint64_t)cn[3] << 32) | cn[2]);
+      if (cn[9] & kReadSimple) {
+        const int dp = p - gpos;
+        En.cover = lane_ok && (unsigned)dp < (unsigned)(gend - gpos);
         En.is_del = false;
         En.ondel = 0;
-        En.qpos = En.cover ? (int)Rn.y0 + dp : 0;
-        En.b = P.bases[Rn.seq_off + En.qpos];
-        En.q = P.quals[Rn.seq_off + En.qpos];
+        const int qpos = p + (int)cn[4];
+        En.qpos = En.cover ? qpos : 0;
+        En.b = bb[(uint32_t)En.qpos];
+        En.q = qb[(uint32_t)En.qpos];
         En.qd = 0;
         En.del_ok = false;
+        En.trimmed = qpos < (int)cn[5] || qpos >= (int)cn[6];
       } else {
+        const DRead Rn = load_read(P.reads, r);
         prepare_general(P, Rn, p, En, err);
         En.cover = En.cover && lane_ok;
       }
     };
     if (rb < re) prepare(rb);
     for (int r = rb; r < re; r++) {
-      const DRead R = Rn;
+      const uint32_t row_base = cn[7], x1 = cn[8], rflags = cn[9];
       const Entry E = En;
       if (r + 1 < re) prepare(r + 1);
-      const bool top = !(R.flags & kReadReversed);
+      const bool top = !(rflags & kReadReversed);
       const uint32_t base = E.is_del ? (uint32_t)'.' : E.b;
       const bool valid = E.cover && base != 15u;
-      if (R.x1 != 1) {
+      if (x1 != 1u) {
         // coverage of a read with X1 != 1 is 1/X1 (identify_mutations.cpp:1606-1614); a redundant
         // read (X1 > 1) adds nothing else
-        const double inv = 1.0 / (double)R.x1;
+        const double inv = 1.0 / (double)x1;
         if (top) red_top += valid ? inv : 0.0;
         else red_bot += valid ? inv : 0.0;
-        if (R.x1 > 1) continue;
+        if (x1 > 1u) continue;
       } else {
         uniq += valid ? 1 : 0;
         insert_slot = insert_slot || (valid && E.ondel > 0);
       }
-      // is_trimmed (alignment.h:389-410)
-      bool trimmed = false;
-      {
-        const uint32_t q1 = (uint32_t)E.qpos + 1u;
-        if (R.xl >= 0 && q1 <= (uint32_t)R.xl) trimmed = true;
-        if (R.xr >= 0) {
-          const uint32_t rem = (uint32_t)R.l_qseq - q1;
-          if (rem + 1u <= (uint32_t)R.xr) trimmed = true;
-          if (E.ondel < 0 && rem == (uint32_t)R.xr) trimmed = true;
-        }
-      }
       // observed class (reference strand) and quality (error_count.cpp:1049-1105)
       uint32_t obs, q;
-      bool ok = valid && !trimmed;
+      bool ok = valid && !E.trimmed;
       if (E.is_del) {
         obs = 4u;
         q = E.qd;
@@ -408,7 +428,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
         }
       }
       ok = ok && q >= (uint32_t)qcut;
-      if (ok && (int)R.rfile >= S) {
+      if (ok && row_base == 0xffffffffu) {
         err |= kErrReadSet;
         ok = false;
       }
@@ -417,7 +437,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
         ok = false;
       }
       const uint32_t o_rs = (obs < 4u && !top) ? (obs ^ 3u) : obs;
-      const int row = ok ? ((((int)R.mslot * S + (int)R.rfile) * Q + (int)q) * 5 + (int)o_rs) : zero_row;
+      const int row = ok ? (int)(row_base + q * 5u + o_rs) : zero_row;
       const FoldRow& T = tab[row];
       const float t0 = T.t[0], t1 = T.t[1], t2 = T.t[2], t3 = T.t[3], t4 = T.t[4];
       if (top) {

@@ -59,8 +59,22 @@ struct FoldRow {
 };
 static_assert(sizeof(FoldRow) == 32, "FoldRow is two 16-byte LDS words");
 
+// Per-call view of one alignment for the column pass: flat 32-bit fields (two scalar loads, no
+// unpacking).  Built by k_prep_reads from the DRead and the call's covariate shape.
+struct IdRec {
+  int32_t gpos, gend;       // aligned span [gpos, gend) in global columns
+  int32_t seq_lo, seq_hi;   // DRead::seq_off
+  int32_t qoff;             // simple alignments: query position = column + qoff
+  int32_t trim_lo, trim_hi; // slot 0: trimmed when qpos < trim_lo or qpos >= trim_hi (alignment.h:389-410)
+  uint32_t row_base;        // ((mslot*S + read file)*Q)*5, 0xFFFFFFFF when the read file is >= S
+  uint32_t x1;
+  uint32_t flags;           // kReadReversed | kReadSimple
+};
+static_assert(sizeof(IdRec) == 40, "IdRec is ten dwords");
+
 struct ImParams {
   const DRead* reads;
+  const IdRec* idrec;           // per alignment, built for this call
   const uint32_t* cigar;
   const uint8_t* bases;
   const uint8_t* quals;
@@ -139,6 +153,7 @@ constexpr int kColWaves = 8;                 // wavefronts per block of the colu
 constexpr size_t kFoldLdsBytes = 48 * 1024;  // largest likelihood table staged in LDS (1535 rows)
 template <bool kLds>
 __global__ void k_identify_columns_t(ImParams P);
+__global__ void k_prep_reads(const DRead* reads, int64_t n, int32_t S, int32_t Q, IdRec* out);
 __global__ void k_fix_columns(ImParams P);
 __global__ void k_insert_slots(ImParams P);
 constexpr int kScoreWaves = 4;               // wavefronts per candidate in the exact scorer

@@ -47,6 +47,18 @@ void Sample::finalize() {
   aln_view.xl = xl.data();
   aln_view.xr = xr.data();
   aln_view.rg = rg.data();
+  const size_t nr = pos.size();
+  if (tlen.size() != nr) tlen.assign(nr, 0);
+  if (xp.size() != nr) xp.assign(nr, (int8_t)-1);
+  name_off.assign(nr + 1, 0);
+  for (size_t i = 0; i < nr; i++) name_off[i + 1] = name_off[i] + (int64_t)(i < read_names.size() ? read_names[i].size() : 0);
+  name_chars.resize((size_t)name_off[nr]);
+  for (size_t i = 0; i < nr && i < read_names.size(); i++)
+    if (!read_names[i].empty()) memcpy(&name_chars[(size_t)name_off[i]], read_names[i].data(), read_names[i].size());
+  aln_view.tlen = tlen.data();
+  aln_view.xp = xp.data();
+  aln_view.name_off = name_off.data();
+  aln_view.names = name_chars.data();
   rf_name_ptrs.clear();
   for (auto& n : rf_names) rf_name_ptrs.push_back(n.c_str());
 }
@@ -244,6 +256,22 @@ extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_sta
 
     const int32_t RL = cfg->read_length;
     const bool paired = cfg->paired != 0;
+    // discordant-pair junctions: two far sites and the orientation of the pairs spanning them
+    struct Junction {
+      int64_t a, b;
+      int8_t orient;
+    };
+    std::vector<Junction> junctions;
+    if (paired && cfg->discordant_fraction > 0 && cfg->n_junctions > 0) {
+      const uint64_t span = (uint64_t)std::max<int64_t>(1, L - 2 * (int64_t)RL - 1000);
+      for (int32_t j = 0; j < cfg->n_junctions; j++) {
+        Junction J;
+        J.a = 500 + (int64_t)rng.below(span);
+        J.b = 500 + (int64_t)rng.below(span);
+        J.orient = (int8_t)rng.below(3);
+        junctions.push_back(J);
+      }
+    }
     const double cov = cfg->coverage;
     int64_t n_frag = (int64_t)std::llround(cov * (double)L / (double)(RL * (paired ? 2 : 1)));
     if (n_frag < 1) n_frag = 1;
@@ -260,6 +288,7 @@ extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_sta
       int64_t mate_pos;
       int32_t tlen;
       int32_t file;
+      int8_t xp;
     };
     std::vector<RawRead> reads;
     reads.reserve((size_t)n_frag * (paired ? 2 : 1));
@@ -276,20 +305,53 @@ extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_sta
       if (flen + 16 >= L) continue;
       int64_t start = (int64_t)fr.below((uint64_t)(L - flen - 8));
       bool frag_rev = (fr.next() & 1) != 0;
-      // haplotype: each variant in the fragment carried with its frequency
-      size_t vfirst = std::lower_bound(vars.begin(), vars.end(), start, [](const Variant& v, int64_t p) { return v.pos < p; }) - vars.begin();
-      for (size_t vi = vfirst; vi < vars.size() && vars[vi].pos < start + flen + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
       const int32_t guard = 4;
       int n_mates = paired ? 2 : 1;
-      bool ok = build_read(ref, vars, carry, vfirst, start, RL, guard, b1);
-      if (ok && paired) {
-        // mate 2 ends at start + flen - 1 on the reference (approximately, before indels)
-        int64_t s2 = start + flen - RL;
-        size_t v2 = std::lower_bound(vars.begin(), vars.end(), s2, [](const Variant& v, int64_t p) { return v.pos < p; }) - vars.begin();
-        ok = build_read(ref, vars, carry, v2, s2, RL, guard, b2);
+      auto first_var = [&](int64_t p) {
+        return (size_t)(std::lower_bound(vars.begin(), vars.end(), p, [](const Variant& v, int64_t x) { return v.pos < x; }) - vars.begin());
+      };
+      // discordant pair: R1 just before junction site a, R2 just after site b
+      const Junction* J = nullptr;
+      if (!junctions.empty() && fr.uniform() < cfg->discordant_fraction) J = &junctions[fr.below(junctions.size())];
+      bool ok;
+      bool rev0 = frag_rev, rev1 = !frag_rev;
+      if (J) {
+        const uint64_t spread = (uint64_t)std::max<int64_t>(1, (int64_t)cfg->insert_mean - RL);
+        const int64_t hi = L - RL - 64;
+        const int64_t s1 = std::min<int64_t>(hi, std::max<int64_t>(0, J->a - RL - (int64_t)fr.below(spread)));
+        const int64_t s2 = std::min<int64_t>(hi, std::max<int64_t>(0, J->b + (int64_t)fr.below(spread)));
+        if (J->orient == 0) { rev0 = false; rev1 = true; }
+        else if (J->orient == 1) { rev0 = true; rev1 = false; }
+        else { rev0 = rev1 = (fr.next() & 1) != 0; }
+        const size_t v1 = first_var(s1), v2 = first_var(s2);
+        for (size_t vi = v1; vi < vars.size() && vars[vi].pos < s1 + RL + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
+        ok = build_read(ref, vars, carry, v1, s1, RL, guard, b1);
+        for (size_t vi = v1; vi < vars.size() && vars[vi].pos < s1 + RL + 16; vi++) carry[vi] = 0;
+        for (size_t vi = v2; vi < vars.size() && vars[vi].pos < s2 + RL + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
+        ok = ok && build_read(ref, vars, carry, v2, s2, RL, guard, b2);
+        for (size_t vi = v2; vi < vars.size() && vars[vi].pos < s2 + RL + 16; vi++) carry[vi] = 0;
+      } else {
+        // haplotype: each variant in the fragment carried with its frequency
+        size_t vfirst = first_var(start);
+        for (size_t vi = vfirst; vi < vars.size() && vars[vi].pos < start + flen + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
+        ok = build_read(ref, vars, carry, vfirst, start, RL, guard, b1);
+        if (ok && paired) {
+          // mate 2 ends at start + flen - 1 on the reference (approximately, before indels)
+          int64_t s2 = start + flen - RL;
+          ok = build_read(ref, vars, carry, first_var(s2), s2, RL, guard, b2);
+        }
+        for (size_t vi = vfirst; vi < vars.size() && vars[vi].pos < start + flen + 16; vi++) carry[vi] = 0;
       }
-      for (size_t vi = vfirst; vi < vars.size() && vars[vi].pos < start + flen + 16; vi++) carry[vi] = 0;
       if (!ok) continue;
+      auto ref_end = [](const Built& B) {
+        int64_t e = B.pos;
+        for (uint32_t c : B.cigar)
+          if ((c & 0xf) == 0 || (c & 0xf) == 2) e += c >> 4;
+        return e;
+      };
+      // TLEN as mark_pair_info sets it: +distance on the leftmost mate, -distance on the other
+      const int64_t dist = paired ? (J ? std::max(ref_end(b1), ref_end(b2)) - std::min(b1.pos, b2.pos) : flen) : 0;
+      const bool m0_left = !paired || b1.pos <= b2.pos;
       bool redundant = fr.uniform() < cfg->redundant_fraction;
       int32_t x1v = redundant ? (2 + (int32_t)fr.below(2)) : 1;
       for (int m = 0; m < n_mates; m++) {
@@ -300,7 +362,7 @@ extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_sta
         rr.cigar = B.cigar;
         rr.bases = B.bases;
         rr.q.resize(RL);
-        bool rev = paired ? ((m == 0) ? frag_rev : !frag_rev) : frag_rev;
+        bool rev = paired ? ((m == 0) ? rev0 : rev1) : frag_rev;
         for (int32_t i = 0; i < RL; i++) {
           uint64_t u = fr.next();
           double uq = (double)(u >> 40) * (1.0 / 16777216.0);
@@ -317,8 +379,8 @@ extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_sta
         }
         uint16_t flag = rev ? 0x10 : 0;
         if (paired) {
-          flag |= 0x1 | 0x2 | ((m == 0) ? 0x40 : 0x80);
-          bool mrev = (m == 0) ? !frag_rev : frag_rev;
+          flag |= 0x1 | (J ? 0 : 0x2) | ((m == 0) ? 0x40 : 0x80);
+          bool mrev = (m == 0) ? rev1 : rev0;
           if (mrev) flag |= 0x20;
         }
         rr.flag = flag;
@@ -326,7 +388,8 @@ extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_sta
         rr.x1 = x1v;
         rr.file = paired ? m : 0;
         rr.mate_pos = paired ? ((m == 0) ? b2.pos : b1.pos) : -1;
-        rr.tlen = paired ? (int32_t)((m == 0) ? flen : -flen) : 0;
+        rr.tlen = paired ? (int32_t)(((m == 0) == m0_left) ? dist : -dist) : 0;
+        rr.xp = paired ? (J ? J->orient : (int8_t)0) : (int8_t)-1;
         reads.push_back(std::move(rr));
       }
     }
@@ -347,6 +410,7 @@ extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_sta
     S.mate_tid.assign(nr, paired ? 0 : -1);
     S.mate_pos.resize(nr);
     S.tlen.resize(nr);
+    S.xp.resize(nr);
     S.read_names.resize(nr);
     S.cigar_off.assign(1, 0);
     S.seq_off.assign(1, 0);
@@ -360,6 +424,7 @@ extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_sta
       S.x1[i] = r.x1;
       S.mate_pos[i] = (int32_t)r.mate_pos;
       S.tlen[i] = r.tlen;
+      S.xp[i] = r.xp;
       S.read_names[i] = std::to_string(r.file + 1) + ":" + std::to_string(r.order / (paired ? 2 : 1) + 1);
       for (uint32_t c : r.cigar) S.cigar.push_back(c);
       S.cigar_off.push_back((int64_t)S.cigar.size());
@@ -572,6 +637,12 @@ extern "C" int bsq_sample_write_bam(const bsq_sample* smp, const char* path, bsq
       if (S.x1[r] >= 0) put_int_tag(rec, "X1", S.x1[r]);
       if (S.xl[r] >= 0) put_int_tag(rec, "XL", S.xl[r]);
       if (S.xr[r] >= 0) put_int_tag(rec, "XR", S.xr[r]);
+      if (r < (int64_t)S.xp.size() && S.xp[r] >= 0 && S.xp[r] <= 2) {
+        static const char* kXp[3] = {"FR", "RF", "FF"};
+        rec += "XPZ";
+        rec += kXp[S.xp[r]];
+        rec.push_back('\0');
+      }
       if (S.rg[r] >= 0 && S.rg[r] < (int32_t)S.rg_ids.size()) {
         rec += "RGZ";
         rec += S.rg_ids[S.rg[r]];
@@ -859,6 +930,7 @@ extern "C" int bsq_sample_read(const char* bam_path, const char* fasta_path, bsq
     S.mate_tid.resize(nr);
     S.mate_pos.resize(nr);
     S.tlen.resize(nr);
+    S.xp.resize(nr);
     S.x1.resize(nr);
     S.xl.resize(nr);
     S.xr.resize(nr);
@@ -905,6 +977,7 @@ extern "C" int bsq_sample_read(const char* bam_path, const char* fasta_path, bsq
         if (lseq) memcpy(&S.qual[(size_t)S.seq_off[ri]], Q.p, (size_t)lseq);
         Q.p += lseq;
         int32_t x1 = -1, xl = -1, xr = -1, rgi = -1;
+        int8_t xpc = -1;
         while (Q.p < rec_end) {
           Q.need(3);
           const char t0 = (char)Q.p[0], t1 = (char)Q.p[1], ty = (char)Q.p[2];
@@ -915,7 +988,12 @@ extern "C" int bsq_sample_read(const char* bam_path, const char* fasta_path, bsq
           if (t0 == 'X' && t1 == '1' && aux_int(ty, v, iv)) x1 = (int32_t)iv;
           else if (t0 == 'X' && t1 == 'L' && aux_int(ty, v, iv)) xl = (int32_t)iv;
           else if (t0 == 'X' && t1 == 'R' && aux_int(ty, v, iv)) xr = (int32_t)iv;
-          else if (t0 == 'R' && t1 == 'G' && ty == 'Z') {
+          else if (t0 == 'X' && t1 == 'P' && ty == 'Z') {
+            const char* o = (const char*)v;
+            if (sz == 3 && o[0] == 'F' && o[1] == 'R') xpc = 0;
+            else if (sz == 3 && o[0] == 'R' && o[1] == 'F') xpc = 1;
+            else if (sz == 3 && o[0] == 'F' && o[1] == 'F') xpc = 2;
+          } else if (t0 == 'R' && t1 == 'G' && ty == 'Z') {
             const char* id = (const char*)v;
             for (size_t g = 0; g < S.rg_ids.size(); g++)
               if (S.rg_ids[g] == id) {
@@ -932,6 +1010,7 @@ extern "C" int bsq_sample_read(const char* bam_path, const char* fasta_path, bsq
         S.mate_tid[ri] = mtid;
         S.mate_pos[ri] = mpos;
         S.tlen[ri] = tlen;
+        S.xp[ri] = xpc;
         S.x1[ri] = x1;
         S.xl[ri] = xl;
         S.xr[ri] = xr;

@@ -26,6 +26,9 @@ struct Sample {
   std::vector<int32_t> x1, xl, xr, rg;
   std::vector<std::string> read_names;   // optional (BAM round trip)
   std::vector<int32_t> mate_tid, mate_pos, tlen;
+  std::vector<int8_t> xp;                // XP:Z pair orientation code (0 FR, 1 RF, 2 FF, -1 none)
+  std::vector<int64_t> name_off;         // CSR copy of read_names for the C view
+  std::string name_chars;
   // read groups (@RG ID / LB) and read files
   std::vector<std::string> rg_ids, rg_libs;
   std::vector<uint32_t> rf_base, rf_count;

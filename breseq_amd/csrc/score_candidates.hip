@@ -5,15 +5,20 @@
 //
 // One 4-wavefront block per candidate.  Every sum the reference forms over its per-read data
 // (pdata) is accumulated in read order, so results are bit-identical to a sequential loop:
-//   * EM passes: per-read terms (log10 s_i + lpmax_i and the five responsibilities) are computed
-//     two reads per lane from a per-candidate LDS cache of the reads' table rows; six lanes then
-//     add the six term columns in read order.
-//   * The candidate's EM work is a chain of ~1000 passes (null fit, profile likelihoods, bisection);
-//     it is spread over the four wavefronts:
-//       stage 1: null fit | profile at the ML frequency | profile at 0 | profile at 1
-//       stage 2: bisection for the lower, then the upper bound; three wavefronts evaluate the
-//                midpoint and both possible next midpoints, which is exactly two sequential
-//                bisection steps per round.
+//   * EM passes: per-read responsibilities are computed one read per lane from a per-candidate LDS
+//     cache of the reads' table rows; five lanes then add the five term columns in read order.  The
+//     log likelihood, which the reference only reports from the last pass of each loop, is formed
+//     once per loop from that pass's frequencies.
+//   * The candidate's EM work is a chain of ~1000 passes (null fit, profile likelihoods,
+//     bisections).  It runs as phases of one loop; in each phase every wavefront has at most one EM
+//     task:
+//       phase 0: full fit | pure-genotype sums
+//       phase 1: null fit | profile at the ML frequency | profile at 0 | profile at 1, then
+//                Fisher and KS
+//       phase 2+: bisection rounds for both bounds together.  Three wavefronts on one bound
+//                 evaluate the midpoint and both possible next midpoints, which is exactly two
+//                 sequential bisection steps per round; when both bounds still run, the upper
+//                 one advances one step per round on the fourth wavefront.
 //   * Fisher terms are evaluated across lanes and added in order by one lane; the exact KS lattice
 //     is filled one anti-diagonal at a time (each cell is one addition of two neighbours, so the
 //     order of the cells does not change any value).
@@ -27,16 +32,21 @@ namespace bsq {
 namespace {
 
 constexpr int kSW = kScoreWaves;
-constexpr int kTermCols = 128;   // reads per EM chunk (two per lane)
+static_assert(kSW == 4, "phase roles are laid out for four wavefronts");
+constexpr int kTermCols = 64;    // reads per EM chunk (one per lane)
+constexpr int kTermRow = 65;     // LDS row stride of a term column: the summing lanes hit different banks
 constexpr int kKsDiag = 100;     // exact KS runs when m*n < 10000, so min(m, n) <= 99
 
+// result slots of the phase tasks
+enum { kResFull = 0, kResNull = 1, kResPlmax = 2, kResP0 = 3, kResP1 = 4, kResBisect = 5 };   // + side*3 + k
+
 struct ScoreShared {
-  double term[kSW][6][kTermCols];
+  double term[kSW][5][kTermRow];
   double full_f[5];
-  double full_ll;
-  double res[kSW];
   double lps[5];
-  double lo, hi;
+  double res[kResBisect + 6];
+  double lo[2], hi[2];
+  double target, lower, upper, ks_p, fisher_p;
   double ks_a[kKsDiag + 1], ks_b[kKsDiag + 1];
   double fisher[64];
   uint32_t ks_bnd[(10000 + 2 + 31) / 32 + 1];
@@ -44,7 +54,10 @@ struct ScoreShared {
   uint32_t cov[6][3];
   uint32_t wcount[kSW];
   int32_t n_keys;
-  int32_t iter;
+  int32_t iter[2];
+  int32_t active[2];
+  int32_t emitted;
+  int32_t done;
 };
 
 static_assert(sizeof(ScoreShared) <= kScoreStaticLds, "scorer static LDS above its declared bound");
@@ -107,41 +120,45 @@ __device__ __forceinline__ double sum_in_order(double acc, const double* t, int 
   return acc;
 }
 
-// One EM pass with frequencies f (wave-uniform): sums[0] = Sum_i (log10 s_i + lpmax_i) over reads
-// with s_i > 0, sums[1+b] = Sum_i w_i(b); every sum in read order (identify_mutations.cpp:3268-3300).
+// One EM pass with frequencies f (wave-uniform).  with_ll == false: sums[1+b] = Sum_i w_i(b), the
+// responsibilities the update and the stopping test use.  with_ll == true: sums[0] = Sum_i
+// (log10 s_i + lpmax_i) over reads with s_i > 0, the log likelihood the reference returns.  Every
+// sum is in read order (identify_mutations.cpp:3268-3300).  The log likelihood is only needed for
+// the last pass of an EM loop, so the loops compute it once, after they stop, from the
+// frequencies of that pass: the same terms in the same order as the reference's last pass.
 __device__ __forceinline__ void em_pass(const Rows& R, int n, const double f[5], uint32_t allowed, double* term,
-                                        double sums[6]) {
+                                        bool with_ll, double sums[6]) {
   const int lane = threadIdx.x & 63;
   double acc = 0.0;
   for (int c0 = 0; c0 < n; c0 += kTermCols) {
-#pragma unroll 1
-    for (int k = 0; k < 2; k++) {
-      const int j = lane + 64 * k;
-      const int i = c0 + j;
-      if (i < n) {
-        double r[6];
-        R.get(i, r);
-        double s = 0.0;
+    const int i = c0 + lane;
+    if (i < n) {
+      double r[6];
+      R.get(i, r);
+      double s = 0.0;
 #pragma unroll
-        for (int b = 0; b < 5; b++)
-          if (allowed & (1u << b)) s += f[b] * r[b];
-        if (s > 0.0) {
-          term[j] = log10(s) + r[5];
+      for (int b = 0; b < 5; b++)
+        if (allowed & (1u << b)) s += f[b] * r[b];
+      if (with_ll) {
+        term[lane] = (s > 0.0) ? log10(s) + r[5] : 0.0;
+      } else if (s > 0.0) {
 #pragma unroll
-          for (int b = 0; b < 5; b++) term[(1 + b) * kTermCols + j] = (allowed & (1u << b)) ? f[b] * r[b] / s : 0.0;
-        } else {
-          term[j] = 0.0;
+        for (int b = 0; b < 5; b++) term[b * kTermRow + lane] = (allowed & (1u << b)) ? f[b] * r[b] / s : 0.0;
+      } else {
 #pragma unroll
-          for (int b = 0; b < 5; b++) term[(1 + b) * kTermCols + j] = (allowed & (1u << b)) ? f[b] : 0.0;
-        }
+        for (int b = 0; b < 5; b++) term[b * kTermRow + lane] = (allowed & (1u << b)) ? f[b] : 0.0;
       }
     }
     wave_sync();
-    if (lane < 6) acc = sum_in_order(acc, term + lane * kTermCols, min(kTermCols, n - c0));
+    if (lane < (with_ll ? 1 : 5)) acc = sum_in_order(acc, term + lane * kTermRow, min(kTermCols, n - c0));
     wave_sync();
   }
+  if (with_ll) {
+    sums[0] = readlane_d(acc, 0);
+  } else {
 #pragma unroll
-  for (int k = 0; k < 6; k++) sums[k] = readlane_d(acc, k);
+    for (int b = 0; b < 5; b++) sums[1 + b] = readlane_d(acc, b);
+  }
 }
 
 // The EM loops of the reference, one wavefront:
@@ -182,13 +199,14 @@ __device__ __noinline__ double run_em(const Rows& R, int n, const uint32_t* obs_
     }
     allowed = 31u;
   }
-  double ll = 0.0;
   double sums[6];
+  double f_last[5];
   const int first = (v < 0) ? 1 : 0;
 #pragma unroll 1
   for (int it = first; it < first + 50; it++) {
-    em_pass(R, n, f, allowed, term, sums);
-    ll = sums[0];
+    em_pass(R, n, f, allowed, term, false, sums);
+#pragma unroll
+    for (int b = 0; b < 5; b++) f_last[b] = f[b];
     double max_delta = 0.0;
     if (v < 0) {
 #pragma unroll
@@ -213,6 +231,9 @@ __device__ __noinline__ double run_em(const Rows& R, int n, const uint32_t* obs_
     }
     if (max_delta < tol) break;
   }
+  // log10 likelihood of the last pass
+  em_pass(R, n, f_last, allowed, term, true, sums);
+  const double ll = sums[0];
   if (f_out)
     for (int b = 0; b < 5; b++) f_out[b] = f[b];
   return ll;
@@ -332,6 +353,14 @@ __device__ __noinline__ double ks_less_w(ScoreShared& SS, uint32_t nx, uint32_t 
   return fmin(fmax(pv, 0.0), 1.0);
 }
 
+// pure_genotype_call kept out of line: its FP64 pow/log10 would otherwise set the register
+// footprint of the whole kernel.
+__device__ __noinline__ void pure_genotype_call(const double* lps, uint32_t n, int* best, double* score) {
+  double l[5];
+  for (int b = 0; b < 5; b++) l[b] = lps[b];
+  pure_genotype(l, n, *best, *score);
+}
+
 // allele_model::major_index / next_index (identify_mutations.cpp:3061-3086)
 __device__ __forceinline__ int major_index(const double f[5], uint32_t n) {
   if (n == 0) return 5;
@@ -365,6 +394,7 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
   double* term = &SS.term[wave][0][0];
   const int S = P.read_set_max, Q = P.quality_max;
   const double tol = P.precision_decimal;
+  const double kDrop = 0.587566;   // kProfileLikelihoodLog10Drop (identify_mutations.cpp:3173)
   const int n_cand = min(*P.n_cand, P.cand_cap);
   uint32_t err = 0;
   for (int ci = blockIdx.x; ci < n_cand; ci += gridDim.x) {
@@ -375,8 +405,7 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
     if (tid < 18) SS.cov[tid / 3][tid % 3] = 0;
     if (tid == 0) SS.n_keys = 0;
     __syncthreads();
-    // ---- pdata in file order (identify_mutations.cpp:1561-1657): order-preserving compaction over
-    //      256 alignments per round
+    // ---- pdata in file order (identify_mutations.cpp:1561-1657): order-preserving compaction
     for (int r0 = rb; r0 < re; r0 += kSW * 64) {
       const int r = r0 + tid;
       bool use = false;
@@ -398,7 +427,11 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
         atomicAdd(&SS.cov[obs][strand_idx], 1u);
       }
       __syncthreads();
-      if (tid == 0) SS.n_keys += (int)(SS.wcount[0] + SS.wcount[1] + SS.wcount[2] + SS.wcount[3]);
+      if (tid == 0) {
+        int tot = 0;
+        for (int k = 0; k < kSW; k++) tot += (int)SS.wcount[k];
+        SS.n_keys += tot;
+      }
       __syncthreads();
     }
     const int n = min(SS.n_keys, P.key_cap);
@@ -408,6 +441,15 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
     const bool cached = n <= P.rows_cap;
     if (cached)
       for (int i = tid; i < n * 6; i += kSW * 64) rows[i] = P.tab_r[(size_t)keys[i / 6] * 6 + i % 6];
+    if (tid == 0) {
+      SS.done = 0;
+      SS.emitted = 0;
+      SS.active[0] = SS.active[1] = 0;
+      SS.lower = 0.0;
+      SS.upper = 1.0;
+      SS.ks_p = 1.0;
+      SS.fisher_p = 1.0;
+    }
     __syncthreads();
     Rows RR;
     RR.lds = cached ? rows : nullptr;
@@ -415,155 +457,194 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
     RR.tab_r = P.tab_r;
     const char refc = (slot == 0) ? P.ref[p] : '.';
     const int ref_idx = char_index(refc) < 0 ? 5 : char_index(refc);
-    // ---- full fit (wave 0) | pure-genotype sums (wave 1, lanes 0..4)
-    if (wave == 0) {
-      double f[5];
-      const double ll = run_em(RR, n, obs_count, 31u, nullptr, -1, 0.0, tol, term, f);
-      if (lane == 0) {
-        for (int b = 0; b < 5; b++) SS.full_f[b] = f[b];
-        SS.full_ll = ll;
-      }
-    } else if (wave == 1) {
-      // pure_genotype_call on the summed likelihoods of the same reads, in read order
-      // (identify_mutations.cpp:1669-1689, 3398-3433): 128 reads' terms staged in LDS, then five
-      // lanes add them in order
-      double acc = 0.0;
-      for (int c0 = 0; c0 < n; c0 += kTermCols) {
-        for (int j = lane; j < kTermCols && c0 + j < n; j += 64) {
-          const double* lp = P.tab_lp + (size_t)keys[c0 + j] * 5;
-#pragma unroll
-          for (int b = 0; b < 5; b++) term[b * kTermCols + j] = lp[b];
+    int major = 5, minor = 5, variant = 5;
+    bool cons = false;
+    // ---- phases: one EM task per wavefront and phase, a single EM call site
+    for (int ph = 0;; ph++) {
+      int mode = 0;   // 0 none, 1 fit, 2 profile
+      uint32_t allowed = 31u;
+      double x = 0.0;
+      int res_slot = -1;
+      if (ph == 0) {
+        if (wave == 0) {
+          mode = 1;
+          res_slot = kResFull;
         }
-        wave_sync();
-        if (lane < 5) acc = sum_in_order(acc, term + lane * kTermCols, min(kTermCols, n - c0));
-        wave_sync();
-      }
-      if (lane < 5) SS.lps[lane] = acc;
-    }
-    __syncthreads();
-    double full_f[5], lps[5];
-    for (int b = 0; b < 5; b++) {
-      full_f[b] = SS.full_f[b];
-      lps[b] = SS.lps[b];
-    }
-    const double full_ll = SS.full_ll;
-    int best;
-    double gscore;
-    pure_genotype(lps, (uint32_t)n, best, gscore);
-    const double cbs = gscore - P.log10_ref_length;
-    const bool cons = (best != ref_idx) && !isnan(cbs) && (cbs > 0.0);
-    const int major = major_index(full_f, (uint32_t)n);
-    const int minor = next_index(full_f, (uint32_t)n, major);
-    const int variant = next_index(full_f, (uint32_t)n, ref_idx);
-    const bool bounds_apply = (n > 0) && (variant < 5);
-    // ---- stage 1: null fit | profile at f_ML | profile at 0 | profile at 1
-    if (wave == 0) {
-      if (variant != 5 && n > 0) {
-        const double ll = run_em(RR, n, obs_count, 31u & ~(1u << variant), nullptr, -1, 0.0, tol, term, nullptr);
-        if (lane == 0) SS.res[0] = ll;
-      }
-    } else if (bounds_apply) {
-      const double x = (wave == 1) ? full_f[variant] : (wave == 2) ? 0.0 : 1.0;
-      const double ll = run_em(RR, n, nullptr, 31u, SS.full_f, variant, x, tol, term, nullptr);
-      if (lane == 0) SS.res[wave] = ll;
-    }
-    __syncthreads();
-    double vscore = __longlong_as_double(0x7ff8000000000000LL);
-    bool poly = false;
-    if (variant != 5 && n > 0) {
-      vscore = (full_ll - SS.res[0]) - P.log10_ref_length;
-      poly = vscore >= P.polymorphism_cutoff;
-    }
-    const bool emitted = cons || poly;
-    double lower = 0.0, upper = 1.0;
-    // ---- stage 2: write_RA_frequency_bounds (identify_mutations.cpp:3175-3217)
-    if (emitted && bounds_apply) {
-      const double kDrop = 0.587566;
-      const double fh = full_f[variant];
-      const double target = SS.res[1] - kDrop;
-      const bool lower_zero = SS.res[2] >= target;
-      const bool upper_one = SS.res[3] >= target;
-      for (int side = 0; side < 2; side++) {
-        if (side == 0 && lower_zero) continue;
-        if (side == 1 && upper_one) continue;
-        __syncthreads();
-        if (tid == 0) {
-          SS.lo = (side == 0) ? 0.0 : fh;
-          SS.hi = (side == 0) ? fh : 1.0;
-          SS.iter = 0;
-        }
-        __syncthreads();
-        for (;;) {
-          const double lo = SS.lo, hi = SS.hi;
-          const int it = SS.iter;
-          if (!(it < 40 && (hi - lo) > tol)) break;
-          const double mid = 0.5 * (lo + hi);
-          const double midL = 0.5 * (lo + mid), midR = 0.5 * (mid + hi);
-          if (wave > 0) {
-            const double x = (wave == 1) ? mid : (wave == 2) ? midL : midR;
-            const double ll = run_em(RR, n, nullptr, 31u, SS.full_f, variant, x, tol, term, nullptr);
-            if (lane == 0) SS.res[wave] = ll;
+      } else if (ph == 1) {
+        if (variant != 5 && n > 0) {
+          if (wave == 0) {
+            mode = 1;
+            allowed = 31u & ~(1u << variant);
+            res_slot = kResNull;
+          } else if (wave <= 3) {
+            mode = 2;
+            x = (wave == 1) ? SS.full_f[variant] : (wave == 2) ? 0.0 : 1.0;
+            res_slot = (wave == 1) ? kResPlmax : (wave == 2) ? kResP0 : kResP1;
           }
-          __syncthreads();
-          if (tid == 0) {
-            // two steps of the sequential bisection from the three evaluations
+        }
+      } else {
+        // both bounds running: the lower one on wavefronts 0-2, the upper one on wavefront 3;
+        // one running: wavefronts 0-2 on it
+        const bool a0 = SS.active[0] != 0, a1 = SS.active[1] != 0;
+        int side = -1, k = 0;
+        if (a0 && a1) {
+          side = (wave < 3) ? 0 : 1;
+          k = (wave < 3) ? wave : 0;
+        } else if ((a0 || a1) && wave < 3) {
+          side = a0 ? 0 : 1;
+          k = wave;
+        }
+        if (side >= 0) {
+          const double lo = SS.lo[side], hi = SS.hi[side];
+          const double mid = 0.5 * (lo + hi);
+          x = (k == 0) ? mid : (k == 1) ? 0.5 * (lo + mid) : 0.5 * (mid + hi);
+          mode = 2;
+          res_slot = kResBisect + side * 3 + k;
+        }
+      }
+      if (mode != 0) {
+        double f[5];
+        const double ll = run_em(RR, n, obs_count, allowed, SS.full_f, (mode == 2) ? variant : -1, x, tol, term,
+                                 (ph == 0) ? f : nullptr);
+        if (lane == 0) {
+          SS.res[res_slot] = ll;
+          if (ph == 0)
+            for (int b = 0; b < 5; b++) SS.full_f[b] = f[b];
+        }
+      }
+      if (ph == 0 && wave == 1) {
+        // pure_genotype_call on the summed likelihoods of the same reads, in read order
+        // (identify_mutations.cpp:1669-1689, 3398-3433)
+        double acc = 0.0;
+        for (int c0 = 0; c0 < n; c0 += kTermCols) {
+          if (c0 + lane < n) {
+            const double* lp = P.tab_lp + (size_t)keys[c0 + lane] * 5;
+#pragma unroll
+            for (int b = 0; b < 5; b++) term[b * kTermRow + lane] = lp[b];
+          }
+          wave_sync();
+          if (lane < 5) acc = sum_in_order(acc, term + lane * kTermRow, min(kTermCols, n - c0));
+          wave_sync();
+        }
+        if (lane < 5) SS.lps[lane] = acc;
+      }
+      if (ph == 1 && wave == 3) {
+        // annotate_polymorphism_statistics (identify_mutations.cpp:3009-3033); used if emitted
+        for (int i = lane; i < 256; i += 64) (&SS.qh[0][0])[i] = 0;
+        wave_sync();
+        uint32_t nx = 0, ny = 0;
+        for (int i = lane; i < n; i += 64) {
+          const uint32_t k = keys[i];
+          const int ob = (int)((k % 10u) / 2u);
+          const int q = (int)((k / 10u) % (uint32_t)Q);
+          if (ob == major && q < 128) {
+            atomicAdd(&SS.qh[1][q], 1u);
+            ny++;
+          }
+          if (ob == minor && q < 128) {
+            atomicAdd(&SS.qh[0][q], 1u);
+            nx++;
+          }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          nx += __shfl_xor(nx, o, 64);
+          ny += __shfl_xor(ny, o, 64);
+        }
+        wave_sync();
+        const double ks = (nx > 0 && ny > 0) ? ks_less_w(SS, nx, ny) : 1.0;
+        const uint32_t mjt = (major < 5) ? SS.cov[major][2] : 0, mjb = (major < 5) ? SS.cov[major][0] : 0;
+        const uint32_t mnt = (minor < 5) ? SS.cov[minor][2] : 0, mnb = (minor < 5) ? SS.cov[minor][0] : 0;
+        const double fs = fisher_2x2_w(mnt, mnb, mjt, mjb, SS.fisher);
+        if (lane == 0) {
+          SS.ks_p = ks;
+          SS.fisher_p = fs;
+        }
+      }
+      __syncthreads();
+      if (ph == 0) {
+        // every thread derives the same calls from the shared sums
+        double full_f[5], lps[5];
+        for (int b = 0; b < 5; b++) {
+          full_f[b] = SS.full_f[b];
+          lps[b] = SS.lps[b];
+        }
+        int best;
+        double gscore;
+        pure_genotype_call(lps, (uint32_t)n, &best, &gscore);
+        const double cbs = gscore - P.log10_ref_length;
+        cons = (best != ref_idx) && !isnan(cbs) && (cbs > 0.0);
+        major = major_index(full_f, (uint32_t)n);
+        minor = next_index(full_f, (uint32_t)n, major);
+        variant = next_index(full_f, (uint32_t)n, ref_idx);
+        continue;   // phase 1 reads the derived calls; no shared state changes here
+      }
+      if (tid == 0) {
+        if (ph == 1) {
+          bool poly = false;
+          if (variant != 5 && n > 0) poly = ((SS.res[kResFull] - SS.res[kResNull]) - P.log10_ref_length) >= P.polymorphism_cutoff;
+          SS.emitted = (cons || poly) ? 1 : 0;
+          // write_RA_frequency_bounds (identify_mutations.cpp:3175-3217)
+          if (SS.emitted && variant < 5 && n > 0) {
+            const double fh = SS.full_f[variant];
+            const double target = SS.res[kResPlmax] - kDrop;
+            SS.target = target;
+            if (SS.res[kResP0] < target) {
+              SS.lo[0] = 0.0;
+              SS.hi[0] = fh;
+              SS.iter[0] = 0;
+              SS.active[0] = (fh - 0.0) > tol ? 1 : 0;
+              SS.lower = fh;
+            }
+            if (SS.res[kResP1] < target) {
+              SS.lo[1] = fh;
+              SS.hi[1] = 1.0;
+              SS.iter[1] = 0;
+              SS.active[1] = (1.0 - fh) > tol ? 1 : 0;
+              SS.upper = fh;
+            }
+          }
+        } else {
+          // each running bisection advances two steps from three evaluations (midpoint and both
+          // possible next midpoints), or one step from one evaluation
+          const bool both = SS.active[0] && SS.active[1];
+          for (int side = 0; side < 2; side++) {
+            if (!SS.active[side]) continue;
+            const bool three = !(both && side == 1);
+            const double lo = SS.lo[side], hi = SS.hi[side];
+            const double mid = 0.5 * (lo + hi);
+            const double* r3 = &SS.res[kResBisect + side * 3];
             double l = lo, h = hi;
-            int i = it;
-            // lower bound: profile >= target moves hi down; upper bound: moves lo up
-            const bool ge = SS.res[1] >= target;
+            int i = SS.iter[side];
+            // lower bound: a profile >= target moves hi down; upper bound: moves lo up
+            const bool ge = r3[0] >= SS.target;
             const bool left = (side == 0) ? ge : !ge;   // the interval kept is [lo, mid]
             if (left) h = mid;
             else l = mid;
             i++;
-            if (i < 40 && (h - l) > tol) {
-              const double m2 = 0.5 * (l + h);   // == midL when left, midR otherwise
-              const bool ge2 = (left ? SS.res[2] : SS.res[3]) >= target;
+            if (three && i < 40 && (h - l) > tol) {
+              const double m2 = 0.5 * (l + h);   // == the left or right next midpoint evaluated
+              const bool ge2 = (left ? r3[1] : r3[2]) >= SS.target;
               const bool left2 = (side == 0) ? ge2 : !ge2;
               if (left2) h = m2;
               else l = m2;
               i++;
             }
-            SS.lo = l;
-            SS.hi = h;
-            SS.iter = i;
+            SS.lo[side] = l;
+            SS.hi[side] = h;
+            SS.iter[side] = i;
+            SS.active[side] = (i < 40 && (h - l) > tol) ? 1 : 0;
+            if (side == 0) SS.lower = h;
+            else SS.upper = l;
           }
-          __syncthreads();
         }
-        if (side == 0) lower = SS.hi;
-        else upper = SS.lo;
+        SS.done = (SS.active[0] || SS.active[1]) ? 0 : 1;
       }
-    }
-    // ---- annotate_polymorphism_statistics (identify_mutations.cpp:3009-3033), wave 0
-    double ks_p = 1.0, fisher_p = 1.0;
-    if (emitted && wave == 0) {
-      for (int i = lane; i < 256; i += 64) (&SS.qh[0][0])[i] = 0;
-      wave_sync();
-      uint32_t nx = 0, ny = 0;
-      for (int i = lane; i < n; i += 64) {
-        const uint32_t k = keys[i];
-        const int ob = (int)((k % 10u) / 2u);
-        const int q = (int)((k / 10u) % (uint32_t)Q);
-        if (ob == major && q < 128) {
-          atomicAdd(&SS.qh[1][q], 1u);
-          ny++;
-        }
-        if (ob == minor && q < 128) {
-          atomicAdd(&SS.qh[0][q], 1u);
-          nx++;
-        }
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        nx += __shfl_xor(nx, o, 64);
-        ny += __shfl_xor(ny, o, 64);
-      }
-      wave_sync();
-      if (nx > 0 && ny > 0) ks_p = ks_less_w(SS, nx, ny);
-      const uint32_t mjt = (major < 5) ? SS.cov[major][2] : 0, mjb = (major < 5) ? SS.cov[major][0] : 0;
-      const uint32_t mnt = (minor < 5) ? SS.cov[minor][2] : 0, mnb = (minor < 5) ? SS.cov[minor][0] : 0;
-      fisher_p = fisher_2x2_w(mnt, mnb, mjt, mjb, SS.fisher);
+      __syncthreads();
+      if (SS.done) break;
     }
     if (tid == 0) {
       RaOut out;
+      const bool emitted = SS.emitted != 0;
       out.gpos = p;
       out.slot = slot;
       out.emitted = emitted ? 1 : 0;
@@ -573,12 +654,13 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
       out.variant = (uint8_t)variant;
       out.pad[0] = out.pad[1] = out.pad[2] = 0;
       out.n = (uint32_t)n;
-      for (int b = 0; b < 5; b++) out.f[b] = full_f[b];
-      out.variant_score = vscore;
-      out.lower = lower;
-      out.upper = upper;
-      out.ks_p = ks_p;
-      out.fisher_p = fisher_p;
+      for (int b = 0; b < 5; b++) out.f[b] = SS.full_f[b];
+      out.variant_score = (variant != 5 && n > 0) ? (SS.res[kResFull] - SS.res[kResNull]) - P.log10_ref_length
+                                                  : __longlong_as_double(0x7ff8000000000000LL);
+      out.lower = emitted ? SS.lower : 0.0;
+      out.upper = emitted ? SS.upper : 1.0;
+      out.ks_p = emitted ? SS.ks_p : 1.0;
+      out.fisher_p = emitted ? SS.fisher_p : 1.0;
       int tt = 0, tb = 0;
       for (int b = 0; b < 5; b++) {
         out.cov[b][0] = SS.cov[b][2];

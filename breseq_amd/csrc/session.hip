@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "dp_regions.h"
 #include "engine.h"
 #include "evidence.h"
 #include "host_util.h"
@@ -76,6 +77,7 @@ struct Session {
   int32_t n_read_files = 1;
   std::vector<int32_t> h_pos_target;
   std::string h_ref;                  // padded reference (for host-side formatting)
+  std::vector<DpRead> dp_reads;       // discordant-pair candidates, file order
   // device buffers
   DBuf<DRead> d_reads;
   DBuf<uint32_t> d_cigar;
@@ -96,6 +98,7 @@ struct Session {
   DBuf<int2> d_cand;
   DBuf<int32_t> d_counters;   // [0] n_cand, [1] n_ins, [2] n_runs, [3] error bits, [4] n_fix
   DBuf<int32_t> d_fix;
+  DBuf<IdRec> d_idrec;
   DBuf<int32_t> d_ins;
   DBuf<RaOut> d_ra;
   DBuf<int2> d_events, d_runs;
@@ -257,6 +260,7 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   }
   S.n_read_files = (int32_t)max_rfile + 1;
   S.n_dreads = (int64_t)dr.size();
+  dp_select(a, ref->n_targets, S.dp_reads);
   if (S.n_dreads >= ((int64_t)1 << 31)) throw BsqError(BSQ_ERR_LIMIT, "more than 2^31 alignments");
   S.aligned_bases = nab;
   // window index: alignments overlapping [64w, 64w+64)
@@ -386,6 +390,7 @@ struct bsq_ec_result {
 
 struct bsq_im_result {
   std::string text;
+  std::string dp_text;
   int64_t n_ra = 0, n_mc = 0, n_un = 0;
 };
 
@@ -456,7 +461,8 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   if (lds_table > 64 * 1024) throw BsqError(BSQ_ERR_LIMIT, "covariate table too large for LDS");
   // one block per 8 waves' worth of alignment chunks, capped at 8 blocks per CU
   const int64_t chunks = (S.n_dreads + 15) / 16;
-  const int ec_grid = (int)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 2048));
+  int ec_grid = (int)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 2048));
+  if (ec_grid >= 8) ec_grid &= ~7;   // a multiple of the 8 XCDs
   S.timing.host_pre_ms = now_ms() - th0;
   HIP_OK(hipEventRecord(S.ev0, S.stream));
   hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, S.stream, S.d_cov_diff.p, n_diff, 0);
@@ -625,8 +631,10 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   S.d_runrec.alloc(run_cap);
   const int fix_cap = std::max<int>(1 << 16, S.n_pos / 256);
   S.d_fix.alloc(fix_cap);
+  S.d_idrec.alloc((size_t)std::max<int64_t>(1, S.n_dreads));
   ImParams P;
   P.reads = S.d_reads.p;
+  P.idrec = S.d_idrec.p;
   P.cigar = S.d_cigar.p;
   P.bases = S.d_bases.p;
   P.quals = S.d_quals.p;
@@ -687,8 +695,10 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   const double th1 = now_ms();
   // the whole pass is queued without a host round trip: later kernels read the counts on device
   HIP_OK(hipEventRecord(S.ev0, S.stream));
+  hipLaunchKernelGGL(k_prep_reads, dim3(1024), dim3(256), 0, S.stream, S.d_reads.p, S.n_dreads, Sm, Q, S.d_idrec.p);
   {
-    const int64_t g = std::min<int64_t>(std::max<int64_t>(1, (S.n_windows + kColWaves - 1) / kColWaves), 2048);
+    int64_t g = std::min<int64_t>(std::max<int64_t>(1, (S.n_windows + kColWaves - 1) / kColWaves), 2048);
+    if (g >= 8) g &= ~(int64_t)7;   // a multiple of the 8 XCDs
     if (P.fold_lds)
       hipLaunchKernelGGL(k_identify_columns_t<true>, dim3((int)g), dim3(kColWaves * 64), fold_bytes, S.stream, P);
     else
@@ -817,6 +827,17 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
     std::ofstream f(a->gd_file);
     if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + a->gd_file);
     f << out.text;
+  }
+  {
+    std::vector<const char*> tn;
+    for (const std::string& n : S.names) tn.push_back(n.c_str());
+    out.dp_text = dp_regions_csv(S.n_targets, tn.data(), S.lengths.data(), a->call_target,
+                                 a->deletion_propagation_cutoff, a->dp, S.dp_reads);
+  }
+  if (a->dp_file) {
+    std::ofstream f(a->dp_file);
+    if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + a->dp_file);
+    f << out.dp_text;
   }
   S.timing.host_post_ms = now_ms() - t1;
   S.timing.host_pre_ms = th1 - th0;
@@ -987,6 +1008,13 @@ int bsq_im_result_gd_text(const bsq_im_result* r, const char** text, int64_t* le
   if (!r) return BSQ_ERR_ARG;
   *text = r->text.c_str();
   *len = (int64_t)r->text.size();
+  return BSQ_OK;
+}
+
+int bsq_im_result_dp_text(const bsq_im_result* r, const char** text, int64_t* len) {
+  if (!r) return BSQ_ERR_ARG;
+  *text = r->dp_text.c_str();
+  *len = (int64_t)r->dp_text.size();
   return BSQ_OK;
 }
 

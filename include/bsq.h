@@ -70,6 +70,13 @@ typedef struct bsq_alignments {
   const int32_t* xl;          /* XL:i left trim tag, -1 when absent (alignment.h:389-410)    */
   const int32_t* xr;          /* XR:i right trim tag, -1 when absent                         */
   const int32_t* rg;          /* index of the RG:Z tag in the header @RG list, -1 when absent */
+  /* Pair fields, read only by the discordant-pair accumulator (NULL when no read group is
+   * paired; identify_mutations.cpp:1399-1457). */
+  const int32_t* tlen;        /* BAM TLEN (alignment.h:284 insert_size)                       */
+  const int8_t* xp;           /* XP:Z pair orientation (resolve_alignments.cpp:1272):
+                                 0 "FR", 1 "RF", 2 "FF", -1 absent or any other value          */
+  const int64_t* name_off;    /* n_reads+1 offsets into names                                 */
+  const char* names;          /* read names, concatenated without terminators                 */
 } bsq_alignments;
 
 /* Read group -> read file resolution (alignment.h:533-591, read_group_index_map +
@@ -122,6 +129,19 @@ void bsq_ec_result_free(bsq_ec_result* r);
 /* identify_mutations  (identify_mutations.h:46-60, identify_mutations.cpp:48-88)             */
 /* ------------------------------------------------------------------------------------------ */
 
+/* Discordant-pair (DP) candidate-region detection, one group per paired read file set
+ * (identify_mutations.cpp:883-908).  Set i is read group i (the @RG index alignments carry). */
+typedef struct bsq_dp_settings {
+  int32_t n_sets;             /* Settings::read_file_sets.size()                               */
+  const uint8_t* paired;      /* 1: cReadFileSet::is_paired() and a paired-mapping-distance
+                                 fit exists for the set (Summary, pmdd.find != end)             */
+  const double* median;       /* fitted pair distance median; <= 0 disables the set            */
+  const double* mad;          /* its median absolute deviation; window = median + 2.42 * mad   */
+  const uint32_t* r1_prefix;  /* read-name prefix of the set's R1 file: m_files[0].m_id + 1     */
+  const uint32_t* r2_prefix;  /* and of its R2 file: m_files[1].m_id + 1                       */
+  int32_t seed;               /* Settings::discordant_pair_seed (default 3, settings.cpp:1340)  */
+} bsq_dp_settings;
+
 typedef struct bsq_identify_args {
   const bsq_reference* ref;
   const bsq_alignments* aln;
@@ -140,6 +160,8 @@ typedef struct bsq_identify_args {
   int32_t base_quality_cutoff;             /* Settings::base_quality_cutoff (3)              */
   int32_t skip_missing_coverage_prediction;
   const char* gd_file;                     /* NULL: do not write ra_mc_evidence.gd           */
+  const struct bsq_dp_settings* dp;        /* discordant-pair groups; NULL = none             */
+  const char* dp_file;                     /* NULL: do not write DP_candidate_regions.csv    */
 } bsq_identify_args;
 
 typedef struct bsq_im_result bsq_im_result;
@@ -148,7 +170,22 @@ typedef struct bsq_im_result bsq_im_result;
 int bsq_im_result_gd_text(const bsq_im_result* r, const char** text, int64_t* len);
 /* Number of RA / MC / UN entries. */
 int bsq_im_result_counts(const bsq_im_result* r, int64_t* n_ra, int64_t* n_mc, int64_t* n_un);
+/* DP_candidate_regions.csv text (identify_mutations.cpp:2550-2573); header only when args->dp
+ * is NULL or names no paired set. */
+int bsq_im_result_dp_text(const bsq_im_result* r, const char** text, int64_t* len);
 void bsq_im_result_free(bsq_im_result* r);
+
+/* The discordant-pair accumulator on its own (host; it reads no pileup column): the sliding
+ * windows and region state machine of check_discordant_completion (identify_mutations.cpp:
+ * 2358-2422), fed at each read's start column as in pileup_callback (:1395-1457), over the
+ * targets of call_target (NULL = all) whose deletion_propagation_cutoff is not negative (NULL =
+ * none is; pileup_callback returns before the accumulator on those, :1318-1334).  dp->n_sets is
+ * also the number of @RG lines read group indices resolve against.  Returns the CSV text;
+ * release it with bsq_free_text. */
+int bsq_dp_candidate_regions(const bsq_reference* ref, const bsq_alignments* aln,
+                             const uint8_t* call_target, const double* deletion_propagation_cutoff,
+                             const bsq_dp_settings* dp, char** out_text, int64_t* out_len,
+                             bsq_status* status);
 
 /* ------------------------------------------------------------------------------------------ */
 /* test_RA_evidence  (identify_mutations.h:63-67, identify_mutations.cpp:687-749)            */
@@ -300,6 +337,10 @@ typedef struct bsq_sim_config {
   double variant_freq_max;
   double redundant_fraction; /* fraction of reads given X1>1                        */
   int32_t mapq;
+  /* Discordant pairs: this fraction of fragments (paired only) becomes a pair without the
+   * proper-pair flag whose R2 lies at one of n_junctions far sites, in orientation FR, RF or FF. */
+  double discordant_fraction;
+  int32_t n_junctions;
 } bsq_sim_config;
 
 int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_status* status);

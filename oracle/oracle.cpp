@@ -26,6 +26,8 @@
  * update_unknown_intervals (UN)              identify_mutations.cpp:2972-3007    hand-built runs
  * GD sort / write                            genome_diff.cpp:685-830             expected.gd format round trip
  * SequenceTrims                              calculate_trims.cpp:67-138          brute force on tiny strings
+ * discordant-pair regions (DP enter step,    identify_mutations.cpp:883-908,     hand-computed windows in
+ *   check_discordant_completion, CSV)        1395-1457, 2358-2422, 2550-2573     tests/test_dp_regions.py
  *
  * Not pinned bit-for-bit: lgamma (Cephes lngamma in the reference, glibc lgamma here) -- Fisher /
  * KS p-values agree with the reference's printed 6 significant digits on the expected.gd vectors.
@@ -36,6 +38,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <map>
 #include <sstream>
@@ -721,7 +724,187 @@ struct Identify {
   Coverage left_outside, left_inside, last_cov;
   uint32_t un_start = UNDEF;
 
-  explicit Identify(const bsq_identify_args* a) : args(a), A{a->aln} {}
+  // Discordant-pair (DP) candidate regions (identify_mutations.h:244-290), per bin = strand*3 + orient
+  struct DpRead {
+    uint32_t start_pos, end_pos;
+    string key;
+    bool redundant;
+  };
+  struct DpGroup {
+    double window_width;
+    uint32_t r1_prefix, r2_prefix;
+    std::deque<DpRead> reads[6];
+  };
+  struct DpRegion {
+    string seq_id;
+    uint32_t start, end, max_count;
+    char strand;
+    string orientation;
+    bool redundant;
+    string pairs;
+  };
+  vector<DpGroup> dp_groups;
+  vector<int> rg_to_dp;
+  int32_t dp_metric[6] = {0, 0, 0, 0, 0, 0};
+  uint32_t dp_start[6], dp_max[6] = {0, 0, 0, 0, 0, 0}, dp_red[6] = {0, 0, 0, 0, 0, 0};
+  vector<string> dp_desc[6];
+  vector<DpRegion> dp_regions;
+
+  explicit Identify(const bsq_identify_args* a) : args(a), A{a->aln} {
+    for (int b = 0; b < 6; b++) dp_start[b] = UNDEF;
+    // identify_mutations.cpp:893-908
+    const bsq_dp_settings* dp = a->dp;
+    if (!dp) return;
+    rg_to_dp.assign((size_t)dp->n_sets, -1);
+    for (int32_t i = 0; i < dp->n_sets; i++) {
+      if (!dp->paired || !dp->paired[i]) continue;
+      if (dp->median[i] <= 0.0) continue;
+      DpGroup g;
+      g.window_width = dp->median[i] + 2.42 * dp->mad[i];
+      g.r1_prefix = dp->r1_prefix[i];
+      g.r2_prefix = dp->r2_prefix[i];
+      rg_to_dp[(size_t)i] = (int)dp_groups.size();
+      dp_groups.push_back(g);
+    }
+  }
+
+  // read_group_index_map::index (alignment.h:545-559); the @RG count is the number of read file sets
+  uint32_t read_group_index(int64_t r) const {
+    const int32_t n = args->dp->n_sets;
+    if (n <= 1) return 0;
+    const int32_t rg = A.a->rg ? A.a->rg[r] : -1;
+    if (rg < 0 || rg >= n) return 0;
+    return (uint32_t)rg;
+  }
+
+  static string dp_descriptor(const DpRead& r) {   // :1289-1292
+    return r.key + "__" + std::to_string(r.start_pos) + "__" + std::to_string(r.end_pos);
+  }
+
+  // DP "enter" step for one pileup entry at insert slot 0 (:1399-1457).  Returns false where breseq
+  // leaves the per-read loop (`continue`) for this read.
+  bool dp_enter(uint32_t position, int64_t r) {
+    if (dp_groups.empty()) return true;
+    if ((uint32_t)A.a->pos[r] + 1 != position) return true;
+    const uint16_t flag = A.a->flag[r];
+    const bool is_paired = flag & 0x1, unmapped = flag & 0x4, proper = flag & 0x2;
+    if (!(is_paired && !unmapped && !proper && !(flag & (0x100 | 0x800 | 0x8)))) return true;
+    const uint32_t rg = read_group_index(r);
+    if (rg >= rg_to_dp.size()) return true;
+    const int gi = rg_to_dp[rg];
+    if (gi < 0) return true;
+    const DpGroup& g = dp_groups[(size_t)gi];
+    const int s = A.reversed(r) ? 1 : 0;
+    const int8_t xp = A.a->xp ? A.a->xp[r] : -1;
+    int o;
+    if (xp == 0) o = 0;
+    else if (xp == 1) o = 1;
+    else if (xp == 2) o = 2;
+    else return false;
+    const int bin = s * 3 + o;
+    const bool focal_is_r1 = !(flag & 0x80);
+    const string fname(A.a->names + A.a->name_off[r], A.a->names + A.a->name_off[r + 1]);
+    const size_t colon = fname.find(':');
+    string read1_name, read2_name;
+    if (colon != string::npos) {
+      const string counter = fname.substr(colon + 1);
+      read1_name = std::to_string(g.r1_prefix) + ":" + counter;
+      read2_name = std::to_string(g.r2_prefix) + ":" + counter;
+    } else {
+      read1_name = focal_is_r1 ? fname : "";
+      read2_name = focal_is_r1 ? "" : fname;
+    }
+    const int32_t isize = A.a->tlen ? A.a->tlen[r] : 0;
+    const int64_t r1_insert_size = focal_is_r1 ? (int64_t)isize : -(int64_t)isize;
+    DpRead dr;
+    dr.start_pos = position;
+    dr.end_pos = (uint32_t)A.endpos(r);   // reference_end_1 = bam_endpos
+    dr.key = read1_name + "__" + read2_name + "__" + std::to_string(r1_insert_size);
+    dr.redundant = A.redundancy(r) > 1;
+    dp_groups[(size_t)gi].reads[bin].push_back(dr);
+    ++dp_metric[bin];
+    if (dp_start[bin] != UNDEF) {
+      dp_desc[bin].push_back(dp_descriptor(dr));
+      if (dr.redundant) ++dp_red[bin];
+    }
+    return true;
+  }
+
+  // check_discordant_completion (:2358-2422)
+  void dp_check(int32_t tid, uint32_t position) {
+    static const char* kOrientName[3] = {"FR", "RF", "FF"};
+    const uint32_t L = (uint32_t)args->ref->lengths[tid];
+    for (int bin = 0; bin < 6; bin++) {
+      for (DpGroup& g : dp_groups) {
+        while (!g.reads[bin].empty() &&
+               (static_cast<double>(position) - static_cast<double>(g.reads[bin].front().start_pos) >= g.window_width)) {
+          g.reads[bin].pop_front();
+          --dp_metric[bin];
+        }
+      }
+      const bool above = (dp_metric[bin] >= args->dp->seed) && (position <= L);
+      if (above) {
+        if (dp_start[bin] == UNDEF) {
+          dp_start[bin] = position;
+          dp_max[bin] = (uint32_t)dp_metric[bin];
+          dp_desc[bin].clear();
+          dp_red[bin] = 0;
+          for (DpGroup& g : dp_groups)
+            for (const DpRead& r : g.reads[bin]) {
+              dp_desc[bin].push_back(dp_descriptor(r));
+              if (r.redundant) ++dp_red[bin];
+            }
+        } else if ((uint32_t)dp_metric[bin] > dp_max[bin]) {
+          dp_max[bin] = (uint32_t)dp_metric[bin];
+        }
+      } else if (dp_start[bin] != UNDEF) {
+        DpRegion R;
+        R.seq_id = args->ref->names[tid];
+        R.start = dp_start[bin];
+        R.end = position - 1;
+        R.strand = (bin / 3 == 0) ? 'F' : 'R';
+        R.orientation = kOrientName[bin % 3];
+        R.max_count = dp_max[bin];
+        R.redundant = (2 * dp_red[bin] > dp_desc[bin].size());
+        for (size_t k = 0; k < dp_desc[bin].size(); k++) {
+          if (k) R.pairs += ";";
+          R.pairs += dp_desc[bin][k];
+        }
+        dp_regions.push_back(R);
+        dp_start[bin] = UNDEF;
+        dp_max[bin] = 0;
+        dp_desc[bin].clear();
+        dp_red[bin] = 0;
+      }
+    }
+  }
+
+  void dp_reset() {   // at_target_start (:2062-2068)
+    for (int bin = 0; bin < 6; bin++) {
+      for (DpGroup& g : dp_groups) g.reads[bin].clear();
+      dp_metric[bin] = 0;
+      dp_start[bin] = UNDEF;
+      dp_max[bin] = 0;
+      dp_desc[bin].clear();
+    }
+  }
+
+  // write_dp_candidate_regions (:2550-2573)
+  string write_dp() const {
+    vector<DpRegion> v = dp_regions;
+    std::sort(v.begin(), v.end(), [](const DpRegion& a, const DpRegion& b) {
+      if (a.seq_id != b.seq_id) return a.seq_id < b.seq_id;
+      if (a.start != b.start) return a.start < b.start;
+      if (a.strand != b.strand) return a.strand < b.strand;
+      return a.orientation < b.orientation;
+    });
+    std::ostringstream os;
+    os << "seq_id,start,end,strand,orientation,length,max_discordant_count,redundant,discordant_pairs" << "\n";
+    for (const DpRegion& r : v)
+      os << r.seq_id << "," << r.start << "," << r.end << "," << r.strand << "," << r.orientation << ","
+         << (r.end - r.start + 1) << "," << r.max_count << "," << (r.redundant ? 1 : 0) << "," << r.pairs << "\n";
+    return os.str();
+  }
 
   void add(GdEntry e) {
     e.id = ++next_id;
@@ -1004,6 +1187,7 @@ struct Identify {
       vector<PData> pdata;
       double lps[5] = {0, 0, 0, 0, 0};
       for (const Entry& e : col) {
+        if (insert_count == 0 && !dp_enter(position, e.read)) continue;
         int64_t r = e.read;
         int indel = e.indel < 0 ? 0 : e.indel;
         if (e.is_del) indel = -1;
@@ -1110,6 +1294,7 @@ struct Identify {
         add(e);
       }
     }
+    if (!dp_groups.empty()) dp_check(tid, position);
   }
 
   void run() {
@@ -1124,7 +1309,9 @@ struct Identify {
       del_start = del_end = red_start = red_end = UNDEF;
       un_start = UNDEF;
       const int64_t L = ref->lengths[tid];
+      dp_reset();
       for_each_column(A, tid, L, [&](int64_t p, const vector<Entry>& col) { callback(tid, p, col); });
+      if (!dp_groups.empty()) dp_check(tid, (uint32_t)L + 1);
       // at_target_end (:2117-2164)
       const string seq_id = ref->names[tid];
       prop_cutoff = args->deletion_propagation_cutoff[tid];
@@ -1233,7 +1420,7 @@ static void sequence_trims(const char* s, uint32_t len, uint8_t* t) {
 /* ================================================================ C API */
 
 struct orc_ec_out { orc::EcOut o; };
-struct orc_im_out { string text; vector<double> scores; };
+struct orc_im_out { string text; vector<double> scores; string dp_text; };
 
 static void set_status(bsq_status* st, int code, const char* msg) {
   if (!st) return;
@@ -1322,6 +1509,7 @@ int orc_identify_mutations(const bsq_identify_args* args, orc_im_out** out, bsq_
     im.run();
     orc_im_out* o = new orc_im_out();
     o->text = im.write();
+    o->dp_text = im.write_dp();
     o->scores = std::move(im.pos_scores);
     *out = o;
     set_status(st, BSQ_OK, "");
@@ -1333,6 +1521,12 @@ int orc_identify_mutations(const bsq_identify_args* args, orc_im_out** out, bsq_
     set_status(st, BSQ_ERR_DATA, e.what());
     return BSQ_ERR_DATA;
   }
+}
+
+int orc_im_dp_text(const orc_im_out* o, const char** text, int64_t* len) {
+  *text = o->dp_text.c_str();
+  *len = (int64_t)o->dp_text.size();
+  return BSQ_OK;
 }
 
 int orc_im_gd_text(const orc_im_out* o, const char** text, int64_t* len) {

@@ -37,6 +37,8 @@ int orc_read_log10_prob_table(const char* text, double* out, int64_t n_bins);
 /* identify_mutations (identify_mutations.cpp:48-88) -> ra_mc_evidence.gd text */
 int orc_identify_mutations(const bsq_identify_args* args, orc_im_out** out, bsq_status* status);
 int orc_im_gd_text(const orc_im_out* o, const char** text, int64_t* len);
+/* DP_candidate_regions.csv text (identify_mutations.cpp:2550-2573) */
+int orc_im_dp_text(const orc_im_out* o, const char** text, int64_t* len);
 /* Per position (slot 0) diagnostics for error reporting: consensus score, unique cov, ... */
 int orc_im_position_scores(const orc_im_out* o, const double** score, int64_t* n);
 void orc_im_free(orc_im_out* o);

@@ -6,6 +6,7 @@ import numpy as np
 
 from breseq_amd import _lib as L
 
+XP = {"FR": 0, "RF": 1, "FF": 2}
 OPS = {"M": 0, "I": 1, "D": 2, "N": 3, "S": 4, "H": 5, "P": 6, "=": 7, "X": 8}
 BAM = {"A": 1, "C": 2, "G": 4, "T": 8, "N": 15}
 
@@ -45,11 +46,18 @@ class Built:
         self.xl = np.array([r.get("xl", -1) for r in reads], dtype=np.int32)
         self.xr = np.array([r.get("xr", -1) for r in reads], dtype=np.int32)
         self.rg = np.array([r.get("rg", -1) for r in reads], dtype=np.int32)
+        self.tlen = np.array([r.get("tlen", 0) for r in reads], dtype=np.int32)
+        self.xp = np.array([XP.get(r.get("xp"), -1) for r in reads], dtype=np.int8)
+        names = [r.get("name", "r%d" % i).encode() for i, r in enumerate(reads)]
+        self.name_off = np.concatenate([[0], np.cumsum([len(x) for x in names])]).astype(np.int64)
+        self.name_chars = b"".join(names)
         p = lambda a, t: a.ctypes.data_as(t)
         self.aln = L.Alignments(n, p(self.tid, L.c_i32p), p(self.pos, L.c_i32p), p(self.flag, L.c_u16p),
                                 p(self.mapq, L.c_u8p), p(self.cigar_off, L.c_i64p), p(self.cigar, L.c_u32p),
                                 p(self.seq_off, L.c_i64p), p(self.seq, L.c_u8p), p(self.qual, L.c_u8p),
-                                p(self.x1, L.c_i32p), p(self.xl, L.c_i32p), p(self.xr, L.c_i32p), p(self.rg, L.c_i32p))
+                                p(self.x1, L.c_i32p), p(self.xl, L.c_i32p), p(self.xr, L.c_i32p), p(self.rg, L.c_i32p),
+                                p(self.tlen, L.c_i32p), p(self.xp, L.c_i8p), p(self.name_off, L.c_i64p),
+                                self.name_chars)
         self.base = (C.c_uint32 * 1)(0)
         self.count = (C.c_uint32 * 1)(1)
         self.rf_names = (C.c_char_p * 1)(b"reads")

@@ -38,6 +38,7 @@ def lib():
     O.orc_read_log10_prob_table.argtypes = [C.c_char_p, L.c_f64p, C.c_int64]
     O.orc_identify_mutations.argtypes = [P(L.IdentifyArgs), P(vp), st]
     O.orc_im_gd_text.argtypes = [vp, P(C.c_char_p), L.c_i64p]
+    O.orc_im_dp_text.argtypes = [vp, P(C.c_char_p), L.c_i64p]
     O.orc_im_position_scores.argtypes = [vp, P(L.c_f64p), L.c_i64p]
     O.orc_im_free.argtypes = [vp]
     O.orc_sequence_trims.argtypes = [C.c_char_p, C.c_int64, L.c_u8p]
@@ -107,6 +108,24 @@ def identify_mutations(args):
     finally:
         O.orc_im_free(h)
     return text, scores
+
+
+def identify_with_dp(args):
+    """(ra_mc_evidence.gd text, DP_candidate_regions.csv text) of the oracle."""
+    O = lib()
+    h = C.c_void_p()
+    st = L.Status()
+    L.check(O.orc_identify_mutations(C.byref(args), C.byref(h), C.byref(st)), st)
+    try:
+        t = C.c_char_p()
+        n = C.c_int64()
+        O.orc_im_gd_text(h, C.byref(t), C.byref(n))
+        gd = C.string_at(t, n.value).decode()
+        O.orc_im_dp_text(h, C.byref(t), C.byref(n))
+        dp = C.string_at(t, n.value).decode()
+    finally:
+        O.orc_im_free(h)
+    return gd, dp
 
 
 def read_log10_prob_table(text, n_bins):

@@ -19,10 +19,12 @@ SOURCES = [
     "kernels.hip",
     "session.hip",
     "extra.hip",
+    "selftest.hip",
     "sample.cpp",
     "host_util.cpp",
     "evidence.cpp",
-    "covfit.cpp", "dp_regions.cpp",
+    "covfit.cpp",
+    "dp_regions.cpp",
 ]
 
 # -ffp-contract=off: the FP64 scorer must round every multiply and add separately, as the

@@ -86,9 +86,9 @@ void append_i(std::string& s, int64_t v) {
   s.append(b, std::to_chars(b, b + sizeof(b), v).ptr);
 }
 
-// <read1>__<read2>__<r1_insert_size>__<start>__<end> (dp_descriptor, identify_mutations.cpp:1289-1292)
-std::string descriptor(const DpRead& r, const Group& g) {
-  std::string s;
+// Appends <read1>__<read2>__<r1_insert_size>__<start>__<end> (dp_descriptor,
+// identify_mutations.cpp:1289-1292).
+void append_descriptor(std::string& s, const DpRead& r, const Group& g) {
   if (r.has_colon) {
     append_u(s, g.r1_prefix);
     s += ':';
@@ -108,7 +108,6 @@ std::string descriptor(const DpRead& r, const Group& g) {
   append_u(s, r.start_1);
   s += "__";
   append_u(s, r.end_1);
-  return s;
 }
 
 // First column at which a read entering at s has left a window of width w: the smallest p >= s
@@ -174,7 +173,7 @@ std::string dp_regions_csv(int32_t n_targets, const char* const* names, const in
   };
   std::vector<Ev> ev;
   std::vector<std::deque<int32_t>> window(groups.size());
-  std::vector<std::string> desc;
+  std::vector<int32_t> desc;   // the open region's reads, formatted when it closes
   for (int32_t t = 0; t < n_targets; t++) {
     if (!scored(t)) continue;
     const uint64_t L = (uint64_t)lengths[t];
@@ -208,7 +207,7 @@ std::string dp_regions_csv(int32_t n_targets, const char* const* names, const in
           window[(size_t)g].push_back(ev[e].idx);
           metric++;
           if (open) {
-            desc.push_back(descriptor(r, groups[(size_t)g]));
+            desc.push_back(ev[e].idx);
             if (r.redundant) r_redundant++;
           }
           e++;
@@ -232,7 +231,7 @@ std::string dp_regions_csv(int32_t n_targets, const char* const* names, const in
             r_redundant = 0;
             for (size_t g = 0; g < groups.size(); g++)
               for (int32_t i : window[g]) {
-                desc.push_back(descriptor(reads[(size_t)i], groups[g]));
+                desc.push_back(i);
                 if (reads[(size_t)i].redundant) r_redundant++;
               }
           } else if ((uint32_t)metric > r_max) {
@@ -247,9 +246,11 @@ std::string dp_regions_csv(int32_t n_targets, const char* const* names, const in
           R.orientation = kOrientName[bin % 3];
           R.max_count = r_max;
           R.redundant = 2 * (uint64_t)r_redundant > desc.size();
+          R.pairs.reserve(desc.size() * 48);
           for (size_t k = 0; k < desc.size(); k++) {
             if (k) R.pairs += ';';
-            R.pairs += desc[k];
+            const DpRead& dr = reads[(size_t)desc[k]];
+            append_descriptor(R.pairs, dr, groups[(size_t)group_of(dr)]);
           }
           regions.push_back(std::move(R));
           open = false;

@@ -29,6 +29,10 @@ namespace {
 
 constexpr int kWaveCols = 64;
 
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, int j) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, j);
+}
+
 // BAM 4-bit base code -> A0 C1 G2 T3; every other code -> 15.
 __device__ __forceinline__ uint32_t base_idx4(uint32_t b) {
   const uint32_t i = (uint32_t)(0xFFFFFFF3FFF2F10FULL >> ((b & 15u) * 4u)) & 15u;
@@ -164,15 +168,16 @@ __global__ void __launch_bounds__(256) k_prep_reads(const DRead* reads, int64_t 
     IdRec c;
     c.gpos = R.gpos;
     c.gend = R.gend;
-    c.seq_lo = (int32_t)(uint32_t)((uint64_t)R.seq_off & 0xffffffffu);
-    c.seq_hi = (int32_t)(uint32_t)((uint64_t)R.seq_off >> 32);
+    c.seq_off = (uint32_t)(uint64_t)R.seq_off;
     c.qoff = (int32_t)R.y0 - R.gpos;
-    // q1 = qpos + 1 <= XL  <=>  qpos < XL;  l_qseq - q1 + 1 <= XR  <=>  qpos >= l_qseq - XR
-    c.trim_lo = (R.xl >= 0) ? (int32_t)R.xl : 0;
-    c.trim_hi = (R.xr >= 0) ? (int32_t)R.l_qseq - (int32_t)R.xr : 0x7fffffff;
+    // q1 = qpos + 1 <= XL  <=>  qpos < XL;  l_qseq - q1 + 1 <= XR  <=>  qpos >= l_qseq - XR.
+    // Query positions are below 65535, so 0xFFFF stands for "no right trim".
+    const uint32_t lo = (R.xl >= 0) ? (uint32_t)R.xl : 0u;
+    const uint32_t hi = (R.xr >= 0) ? (uint32_t)max(0, (int32_t)R.l_qseq - (int32_t)R.xr) : 0xFFFFu;
+    c.trim = lo | (hi << 16);
     c.row_base = ((int)R.rfile < S) ? (((uint32_t)R.mslot * (uint32_t)S + R.rfile) * (uint32_t)Q) * 5u : 0xffffffffu;
-    c.x1 = R.x1;
-    c.flags = R.flags;
+    c.x1_flags = (uint32_t)R.x1 | ((uint32_t)R.flags << 16);
+    c.pad = 0;
     out[i] = c;
   }
 }
@@ -364,107 +369,113 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
       A.G[i] = 0.0f;
     }
 
-    // pipeline: the record and bytes of alignment r+1 are requested before alignment r is added
-    typedef __attribute__((address_space(4))) const uint32_t ConstU32;
-    uint32_t cn[10];
-    Entry En;
-    auto prepare = [&](int r) {
-      const ConstU32* c = (const ConstU32*)(P.idrec + r);
-#pragma unroll
-      for (int k = 0; k < 10; k++) cn[k] = c[k];
-      const int gpos = (int)cn[0], gend = (int)cn[1];
-      const uint8_t* bb = P.bases + (int64_t)(((uint64_t)cn[3] << 32) | cn[2]);
-      const uint8_t* qb = P.quals + (int64_t)(((uint64_t)cn[3] << 32) | cn[2]);
-      if (cn[9] & kReadSimple) {
-        const int dp = p - gpos;
-        En.cover = lane_ok && (unsigned)dp < (unsigned)(gend - gpos);
-        En.is_del = false;
-        En.ondel = 0;
-        const int qpos = p + (int)cn[4];
-        En.qpos = En.cover ? qpos : 0;
-        En.b = bb[(uint32_t)En.qpos];
-        En.q = qb[(uint32_t)En.qpos];
-        En.qd = 0;
-        En.del_ok = false;
-        En.trimmed = qpos < (int)cn[5] || qpos >= (int)cn[6];
-      } else {
-        const DRead Rn = load_read(P.reads, r);
-        prepare_general(P, Rn, p, En, err);
-        En.cover = En.cover && lane_ok;
+    // The window's alignment records come in batches of 64 (lane i holds alignment r0 + i) and
+    // their fields are read out with v_readlane, so the loop never waits on a scalar load (whose
+    // counter it would share with the LDS table reads).  The bytes of alignment j + 1 are requested
+    // before alignment j is added.
+    for (int r0 = rb; r0 < re; r0 += 64) {
+      const int m = min(64, re - r0);
+      uint4 ra = make_uint4(0u, 0u, 0u, 0u), rc = make_uint4(0u, 0u, 0u, 0u);
+      if (lane < m) {
+        const uint4* src = (const uint4*)(P.idrec + r0 + lane);
+        ra = src[0];
+        rc = src[1];
       }
-    };
-    if (rb < re) prepare(rb);
-    for (int r = rb; r < re; r++) {
-      const uint32_t row_base = cn[7], x1 = cn[8], rflags = cn[9];
-      const Entry E = En;
-      if (r + 1 < re) prepare(r + 1);
-      const bool top = !(rflags & kReadReversed);
-      const uint32_t base = E.is_del ? (uint32_t)'.' : E.b;
-      const bool valid = E.cover && base != 15u;
-      if (x1 != 1u) {
-        // coverage of a read with X1 != 1 is 1/X1 (identify_mutations.cpp:1606-1614); a redundant
-        // read (X1 > 1) adds nothing else
-        const double inv = 1.0 / (double)x1;
-        if (top) red_top += valid ? inv : 0.0;
-        else red_bot += valid ? inv : 0.0;
-        if (x1 > 1u) continue;
-      } else {
-        uniq += valid ? 1 : 0;
-        insert_slot = insert_slot || (valid && E.ondel > 0);
-      }
-      // observed class (reference strand) and quality (error_count.cpp:1049-1105)
-      uint32_t obs, q;
-      bool ok = valid && !E.trimmed;
-      if (E.is_del) {
-        obs = 4u;
-        q = E.qd;
-        ok = ok && E.del_ok;
-      } else {
-        obs = base_idx4(E.b);
-        q = E.q;
-        if (ok && obs > 3u) {
-          err |= kErrBaseCode;
-          ok = false;
+      // ra = {gpos, gend, seq_off, qoff}, rc = {trim, row_base, x1_flags, pad}
+      uint32_t nb = 0u, nq = 0u;
+      auto load_bytes = [&](int j) {
+        const int gpos = (int)rdlane(ra.x, j), gend = (int)rdlane(ra.y, j);
+        const uint32_t soff = rdlane(ra.z, j);
+        const int qoff = (int)rdlane(ra.w, j);
+        const bool in = (unsigned)(p - gpos) < (unsigned)(gend - gpos);
+        const uint32_t at = soff + (in ? (uint32_t)(p + qoff) : 0u);
+        nb = P.bases[at];
+        nq = P.quals[at];
+      };
+      if (rdlane(rc.z, 0) & (kReadSimple << 16)) load_bytes(0);
+      for (int j = 0; j < m; j++) {
+        const uint32_t xf = rdlane(rc.z, j);
+        const uint32_t x1 = xf & 0xFFFFu, rflags = xf >> 16;
+        const uint32_t row_base = rdlane(rc.y, j);
+        uint32_t bq = nb, qq = nq;
+        if (j + 1 < m && (rdlane(rc.z, j + 1) & (kReadSimple << 16))) load_bytes(j + 1);
+        // entry bits: 1 cover, 2 deletion, 4 deletion quality usable, 8 trimmed, 16 insertion follows
+        uint32_t eb;
+        uint32_t qd = 0u;
+        if (rflags & kReadSimple) {
+          const int gpos = (int)rdlane(ra.x, j), gend = (int)rdlane(ra.y, j);
+          const int qpos = p + (int)rdlane(ra.w, j);
+          const uint32_t tr = rdlane(rc.x, j);
+          const bool cover = lane_ok && (unsigned)(p - gpos) < (unsigned)(gend - gpos);
+          const bool trimmed = qpos < (int)(tr & 0xFFFFu) || qpos >= (int)(tr >> 16);
+          eb = (cover ? 1u : 0u) | (trimmed ? 8u : 0u);
+        } else {
+          Entry E;
+          prepare_general(P, load_read(P.reads, r0 + j), p, E, err);
+          eb = ((E.cover && lane_ok) ? 1u : 0u) | (E.is_del ? 2u : 0u) | (E.del_ok ? 4u : 0u) |
+               (E.trimmed ? 8u : 0u) | (E.ondel > 0 ? 16u : 0u);
+          bq = E.b;
+          qq = E.q;
+          qd = E.qd;
         }
-      }
-      ok = ok && q >= (uint32_t)qcut;
-      if (ok && row_base == 0xffffffffu) {
-        err |= kErrReadSet;
-        ok = false;
-      }
-      if (ok && q >= (uint32_t)Q) {
-        err |= kErrQuality;
-        ok = false;
-      }
-      const uint32_t o_rs = (obs < 4u && !top) ? (obs ^ 3u) : obs;
-      const int row = ok ? (int)(row_base + q * 5u + o_rs) : zero_row;
-      const FoldRow& T = tab[row];
-      const float t0 = T.t[0], t1 = T.t[1], t2 = T.t[2], t3 = T.t[3], t4 = T.t[4];
-      if (top) {
-        lps0 += t0;
-        lps1 += t1;
-        lps2 += t2;
-        lps3 += t3;
-      } else {
-        lps0 += t3;
-        lps1 += t2;
-        lps2 += t1;
-        lps3 += t0;
-      }
-      lps4 += t4;
-      const float dg = T.diag;
-      A.SL += T.lpmax;
-      A.Dall += dg;
-      A.n += ok ? 1u : 0u;
-      if (ok && (int)obs != ref_idx) {
-        const float e = T.t[top ? e_top : e_bot];
-        const float g = e - dg;
-        A.Gabs += fabsf(e) + fabsf(dg);
+        const bool is_del = (eb & 2u) != 0u;
+        const bool top = !(rflags & kReadReversed);
+        const uint32_t base = is_del ? (uint32_t)'.' : bq;
+        const bool valid = (eb & 1u) && base != 15u;
+        if (x1 != 1u) {
+          // coverage of a read with X1 != 1 is 1/X1 (identify_mutations.cpp:1606-1614); a redundant
+          // read (X1 > 1) adds nothing else
+          const double inv = 1.0 / (double)x1;
+          if (top) red_top += valid ? inv : 0.0;
+          else red_bot += valid ? inv : 0.0;
+          if (x1 > 1u) continue;
+        } else {
+          uniq += valid ? 1 : 0;
+          insert_slot = insert_slot || (valid && (eb & 16u));
+        }
+        // observed class (reference strand) and quality (error_count.cpp:1049-1105), as selects;
+        // a base code other than A C G T N, a read file beyond the table or a quality beyond its
+        // maximum is reported (the reference asserts)
+        const uint32_t o4 = base_idx4(bq);
+        const uint32_t obs = is_del ? 4u : o4;
+        const uint32_t q = is_del ? qd : qq;
+        const bool cand = valid && !(eb & 8u) && (!is_del || (eb & 4u));
+        const bool bad_base = cand && !is_del && o4 > 3u;
+        bool ok = cand && !bad_base && q >= (uint32_t)qcut;
+        const bool bad_rs = ok && row_base == 0xffffffffu;
+        const bool bad_q = ok && !bad_rs && q >= (uint32_t)Q;
+        err |= (bad_base ? kErrBaseCode : 0u) | (bad_rs ? kErrReadSet : 0u) | (bad_q ? kErrQuality : 0u);
+        ok = ok && !bad_rs && !bad_q;
+        const uint32_t o_rs = (obs < 4u && !top) ? (obs ^ 3u) : obs;
+        const int row = ok ? (int)(row_base + q * 5u + o_rs) : zero_row;
+        const FoldRow& T = tab[row];
+        const float t0 = T.t[0], t1 = T.t[1], t2 = T.t[2], t3 = T.t[3], t4 = T.t[4];
+        if (top) {
+          lps0 += t0;
+          lps1 += t1;
+          lps2 += t2;
+          lps3 += t3;
+        } else {
+          lps0 += t3;
+          lps1 += t2;
+          lps2 += t1;
+          lps3 += t0;
+        }
+        lps4 += t4;
+        const float dg = T.diag;
+        A.SL += T.lpmax;
+        A.Dall += dg;
+        A.n += ok ? 1u : 0u;
+        if (ok && (int)obs != ref_idx) {
+          const float e = T.t[top ? e_top : e_bot];
+          const float g = e - dg;
+          A.Gabs += fabsf(e) + fabsf(dg);
 #pragma unroll
-        for (int o = 0; o < 5; o++) {
-          if ((int)obs == o) {
-            A.G[o] += g;
-            A.cnt[o] += 1u;
+          for (int o = 0; o < 5; o++) {
+            if ((int)obs == o) {
+              A.G[o] += g;
+              A.cnt[o] += 1u;
+            }
           }
         }
       }

@@ -63,14 +63,15 @@ static_assert(sizeof(FoldRow) == 32, "FoldRow is two 16-byte LDS words");
 // unpacking).  Built by k_prep_reads from the DRead and the call's covariate shape.
 struct IdRec {
   int32_t gpos, gend;       // aligned span [gpos, gend) in global columns
-  int32_t seq_lo, seq_hi;   // DRead::seq_off
+  uint32_t seq_off;         // DRead::seq_off (the device byte arrays are below 4 GiB)
   int32_t qoff;             // simple alignments: query position = column + qoff
-  int32_t trim_lo, trim_hi; // slot 0: trimmed when qpos < trim_lo or qpos >= trim_hi (alignment.h:389-410)
+  uint32_t trim;            // slot 0: trimmed when qpos < (trim & 0xFFFF) or qpos >= (trim >> 16)
+                            // (alignment.h:389-410)
   uint32_t row_base;        // ((mslot*S + read file)*Q)*5, 0xFFFFFFFF when the read file is >= S
-  uint32_t x1;
-  uint32_t flags;           // kReadReversed | kReadSimple
+  uint32_t x1_flags;        // X1 | (kReadReversed | kReadSimple) << 16
+  uint32_t pad;
 };
-static_assert(sizeof(IdRec) == 40, "IdRec is ten dwords");
+static_assert(sizeof(IdRec) == 32, "IdRec is eight dwords");
 
 struct ImParams {
   const DRead* reads;
@@ -157,7 +158,8 @@ __global__ void k_prep_reads(const DRead* reads, int64_t n, int32_t S, int32_t Q
 __global__ void k_fix_columns(ImParams P);
 __global__ void k_insert_slots(ImParams P);
 constexpr int kScoreWaves = 4;               // wavefronts per candidate in the exact scorer
-constexpr size_t kScoreStaticLds = 30 * 1024;   // bound on the scorer's static LDS
+constexpr size_t kScoreStaticLds = 28 * 1024;   // bound on the scorer's static LDS
+constexpr size_t kScoreLdsBudget = 52 * 1024;   // static + dynamic LDS of a scorer block: three per CU
 __global__ void k_score_candidates(ImParams P);
 __global__ void k_runs(RunParams P);
 

@@ -5,20 +5,22 @@
 //
 // One 4-wavefront block per candidate.  Every sum the reference forms over its per-read data
 // (pdata) is accumulated in read order, so results are bit-identical to a sequential loop:
-//   * EM passes: per-read responsibilities are computed one read per lane from a per-candidate LDS
-//     cache of the reads' table rows; five lanes then add the five term columns in read order.  The
-//     log likelihood, which the reference only reports from the last pass of each loop, is formed
-//     once per loop from that pass's frequencies.
+//   * EM: a wavefront runs up to two EM loops ("tasks") in lockstep.  Per pass, each lane takes
+//     one read and computes that read's responsibilities for every running task from a
+//     per-candidate LDS cache of the reads' table rows; then lane 5t+b adds term column b of task
+//     t in read order.  The in-order additions are the pass's critical path and its largest
+//     instruction count, and one pass of them serves both tasks.  The log likelihood, which
+//     the reference reports only from the last pass of a loop, is formed once per loop from that
+//     pass's frequencies: after its last update a task makes one more pass in "likelihood" mode.
 //   * The candidate's EM work is a chain of ~1000 passes (null fit, profile likelihoods,
-//     bisections).  It runs as phases of one loop; in each phase every wavefront has at most one EM
-//     task:
+//     bisections), run as phases:
 //       phase 0: full fit | pure-genotype sums
 //       phase 1: null fit | profile at the ML frequency | profile at 0 | profile at 1, then
 //                Fisher and KS
-//       phase 2+: bisection rounds for both bounds together.  Three wavefronts on one bound
-//                 evaluate the midpoint and both possible next midpoints, which is exactly two
-//                 sequential bisection steps per round; when both bounds still run, the upper
-//                 one advances one step per round on the fourth wavefront.
+//       phase 2+: bisection rounds.  With one bound still open, the seven midpoints of the next
+//                 three bisection levels are evaluated together (two per wavefront), which is
+//                 exactly three sequential steps per round; with both open, three midpoints per
+//                 bound (two steps each).
 //   * Fisher terms are evaluated across lanes and added in order by one lane; the exact KS lattice
 //     is filled one anti-diagonal at a time (each cell is one addition of two neighbours, so the
 //     order of the cells does not change any value).
@@ -33,18 +35,41 @@ namespace {
 
 constexpr int kSW = kScoreWaves;
 static_assert(kSW == 4, "phase roles are laid out for four wavefronts");
+constexpr int kTW = 2;           // EM tasks per wavefront
 constexpr int kTermCols = 64;    // reads per EM chunk (one per lane)
 constexpr int kTermRow = 65;     // LDS row stride of a term column: the summing lanes hit different banks
 constexpr int kKsDiag = 100;     // exact KS runs when m*n < 10000, so min(m, n) <= 99
+constexpr int kBisectPts = 7;    // midpoints of three bisection levels
 
 // result slots of the phase tasks
-enum { kResFull = 0, kResNull = 1, kResPlmax = 2, kResP0 = 3, kResP1 = 4, kResBisect = 5 };   // + side*3 + k
+enum { kResFull = 0, kResNull = 1, kResPlmax = 2, kResP0 = 3, kResP1 = 4, kResBisect = 5 };   // + side*7 + node
+constexpr int kNumRes = kResBisect + 2 * kBisectPts;
+
+// EM task states
+enum { kTaskIdle = 0, kTaskIter = 1, kTaskLast = 2 };
+
+// The EM tasks of one wavefront (LDS).
+struct WaveTasks {
+  double term[kTW][5][kTermRow];   // per task: five term columns of the current chunk
+  double f[kTW][5];                // current frequencies
+  double fl[kTW][5];               // frequencies of the last pass (the likelihood pass uses them)
+  double sum[kTW][5];              // the pass's in-order sums
+  double delta[kTW][5];            // |new - old| frequency per allele of the last update
+  double x[kTW];                   // profile: pinned frequency
+  int32_t mode[kTW];               // 1 fit, 2 profile
+  uint32_t allowed[kTW];
+  int32_t v[kTW];                  // profile: pinned allele
+  int32_t state[kTW];
+  int32_t it[kTW];
+  int32_t res[kTW];                // result slot
+};
 
 struct ScoreShared {
-  double term[kSW][5][kTermRow];
+  WaveTasks wt[kSW];
   double full_f[5];
   double lps[5];
-  double res[kResBisect + 6];
+  double res[kNumRes];
+  double pts[2][kBisectPts];       // bisection midpoints of the round, per bound
   double lo[2], hi[2];
   double target, lower, upper, ks_p, fisher_p;
   double ks_a[kKsDiag + 1], ks_b[kKsDiag + 1];
@@ -52,13 +77,22 @@ struct ScoreShared {
   uint32_t ks_bnd[(10000 + 2 + 31) / 32 + 1];
   uint32_t qh[2][128];
   uint32_t cov[6][3];
+  uint32_t obs[5];                 // reads per observed class (both strands)
   uint32_t wcount[kSW];
   int32_t n_keys;
   int32_t iter[2];
   int32_t active[2];
+  int32_t depth[2];                // bisection levels evaluated this round, per bound
   int32_t emitted;
   int32_t done;
 };
+
+// LDS views.  Helpers that are not inlined reach shared memory through these, so their accesses
+// compile to LDS instructions rather than generic (flat) ones, whose waits also cover memory loads.
+typedef __attribute__((address_space(3))) double LdsDouble;
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+typedef __attribute__((address_space(3))) WaveTasks LdsTasks;
+typedef __attribute__((address_space(3))) ScoreShared LdsShared;
 
 static_assert(sizeof(ScoreShared) <= kScoreStaticLds, "scorer static LDS above its declared bound");
 
@@ -66,6 +100,13 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double readfirstlane_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 __device__ __forceinline__ double readlane_d(double v, int l) {
@@ -95,89 +136,105 @@ struct Rows {
   }
 };
 
-// acc + t[0] + t[1] + ... + t[m-1], added left to right (one lane).  The LDS reads run 8 terms
-// ahead of the additions so the dependent chain does not wait on LDS latency.
-__device__ __forceinline__ double sum_in_order(double acc, const double* t, int m) {
+// acc + t[0] + t[1] + ... + t[m-1], added left to right (one lane).  LDS latency (~120 cycles)
+// is 25 dependent FP64 additions, so the reads run three groups of eight (24 terms) ahead of
+// the additions.
+__device__ __forceinline__ double sum_in_order(double acc, const LdsDouble* t, int m) {
   int j = 0;
-  if (m >= 8) {
-    double c[8];
+  if (m >= 32) {
+    double c0[8], c1[8], c2[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) c[k] = t[k];
-    for (; j + 16 <= m; j += 8) {
+    for (int k = 0; k < 8; k++) {
+      c0[k] = t[k];
+      c1[k] = t[8 + k];
+      c2[k] = t[16 + k];
+    }
+    for (; j + 32 <= m; j += 8) {
       double nx[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) nx[k] = t[j + 8 + k];
+      for (int k = 0; k < 8; k++) nx[k] = t[j + 24 + k];
 #pragma unroll
-      for (int k = 0; k < 8; k++) acc += c[k];
+      for (int k = 0; k < 8; k++) acc += c0[k];
 #pragma unroll
-      for (int k = 0; k < 8; k++) c[k] = nx[k];
+      for (int k = 0; k < 8; k++) {
+        c0[k] = c1[k];
+        c1[k] = c2[k];
+        c2[k] = nx[k];
+      }
     }
 #pragma unroll
-    for (int k = 0; k < 8; k++) acc += c[k];
-    j += 8;
+    for (int k = 0; k < 8; k++) acc += c0[k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc += c1[k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc += c2[k];
+    j += 24;
   }
   for (; j < m; j++) acc += t[j];
   return acc;
 }
 
-// One EM pass with frequencies f (wave-uniform).  with_ll == false: sums[1+b] = Sum_i w_i(b), the
-// responsibilities the update and the stopping test use.  with_ll == true: sums[0] = Sum_i
-// (log10 s_i + lpmax_i) over reads with s_i > 0, the log likelihood the reference returns.  Every
-// sum is in read order (identify_mutations.cpp:3268-3300).  The log likelihood is only needed for
-// the last pass of an EM loop, so the loops compute it once, after they stop, from the
-// frequencies of that pass: the same terms in the same order as the reference's last pass.
-__device__ __forceinline__ void em_pass(const Rows& R, int n, const double f[5], uint32_t allowed, double* term,
-                                        bool with_ll, double sums[6]) {
-  const int lane = threadIdx.x & 63;
-  double acc = 0.0;
-  for (int c0 = 0; c0 < n; c0 += kTermCols) {
-    const int i = c0 + lane;
-    if (i < n) {
-      double r[6];
-      R.get(i, r);
-      double s = 0.0;
+// The same for exactly K terms (K a multiple of 8), unrolled in groups of eight, each group's LDS
+// reads issued two groups before it is added.
+template <int K>
+__device__ __forceinline__ double sum_in_order_k(double acc, const LdsDouble* t) {
+  static_assert(K % 8 == 0, "groups of eight");
+  constexpr int G = K / 8;
+  double g0[8], g1[8], g2[8];
 #pragma unroll
-      for (int b = 0; b < 5; b++)
-        if (allowed & (1u << b)) s += f[b] * r[b];
-      if (with_ll) {
-        term[lane] = (s > 0.0) ? log10(s) + r[5] : 0.0;
-      } else if (s > 0.0) {
+  for (int k = 0; k < 8; k++) g0[k] = t[k];
+  if (G > 1) {
 #pragma unroll
-        for (int b = 0; b < 5; b++) term[b * kTermRow + lane] = (allowed & (1u << b)) ? f[b] * r[b] / s : 0.0;
-      } else {
-#pragma unroll
-        for (int b = 0; b < 5; b++) term[b * kTermRow + lane] = (allowed & (1u << b)) ? f[b] : 0.0;
-      }
-    }
-    wave_sync();
-    if (lane < (with_ll ? 1 : 5)) acc = sum_in_order(acc, term + lane * kTermRow, min(kTermCols, n - c0));
-    wave_sync();
+    for (int k = 0; k < 8; k++) g1[k] = t[8 + k];
   }
-  if (with_ll) {
-    sums[0] = readlane_d(acc, 0);
-  } else {
 #pragma unroll
-    for (int b = 0; b < 5; b++) sums[1 + b] = readlane_d(acc, b);
+  for (int grp = 0; grp < G; grp++) {
+    double* cur = (grp % 3 == 0) ? g0 : (grp % 3 == 1) ? g1 : g2;
+    double* nxt = (grp % 3 == 0) ? g2 : (grp % 3 == 1) ? g0 : g1;
+    if (grp + 2 < G) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) nxt[k] = t[(grp + 2) * 8 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc += cur[k];
+  }
+  return acc;
+}
+
+// A chunk's column of m terms, padded in LDS with -0.0 up to the next multiple of 16: adding -0.0
+// leaves every double unchanged (+0.0 included), so the result is the in-order sum of the m terms.
+__device__ __forceinline__ double sum_chunk(double acc, const LdsDouble* t, int m) {
+  switch ((m + 15) >> 4) {
+    case 1: return sum_in_order_k<16>(acc, t);
+    case 2: return sum_in_order_k<32>(acc, t);
+    case 3: return sum_in_order_k<48>(acc, t);
+    default: return sum_in_order_k<64>(acc, t);
   }
 }
 
-// The EM loops of the reference, one wavefront:
-//   v < 0:  fit_allele_frequencies (identify_mutations.cpp:3240-3318) over the alleles in
-//           `allowed`, started from the Laplace-smoothed counts obs_of; returns the last pass's
-//           log10 likelihood and leaves the frequencies in f.
-//   v >= 0: profile_log10_likelihood (:3094-3143): f[v] pinned to f_fixed, the other alleles
-//           started from full_f rescaled; returns the last pass's log10 likelihood.
-__device__ __noinline__ double run_em(const Rows& R, int n, const uint32_t* obs_of, uint32_t allowed,
-                                      const double* full_f, int v, double f_fixed, double tol, double* term,
-                                      double* f_out) {
+// Starts EM task slot t of a wavefront (one lane): the starting point of the reference's loops.
+//   mode 1: fit_allele_frequencies (identify_mutations.cpp:3240-3318) over the alleles in
+//           `allowed`, from the Laplace-smoothed counts obs_of.
+//   mode 2: profile_log10_likelihood (:3094-3143): f[v] pinned to x, the other alleles started
+//           from full_f rescaled.
+// A fit with no reads or no allowed allele has likelihood 0 and frequencies 0 without a pass.
+__device__ __noinline__ void task_start(LdsTasks* W, int t, int mode, uint32_t allowed, int v, double x, int res,
+                           const LdsU32* obs_of, const LdsDouble* full_f, int n, LdsDouble* res_out) {
   double f[5];
-  if (v < 0) {
+  W->mode[t] = mode;
+  W->v[t] = v;
+  W->x[t] = x;
+  W->res[t] = res;
+  W->it[t] = 0;
+  if (mode == 1) {
 #pragma unroll
     for (int b = 0; b < 5; b++) f[b] = 0.0;
     if (n == 0 || allowed == 0) {
-      if (f_out)
-        for (int b = 0; b < 5; b++) f_out[b] = 0.0;
-      return 0.0;
+#pragma unroll
+      for (int b = 0; b < 5; b++) W->f[t][b] = 0.0;
+      W->state[t] = kTaskIdle;
+      res_out[res] = 0.0;
+      return;
     }
 #pragma unroll
     for (int b = 0; b < 5; b++)
@@ -194,55 +251,153 @@ __device__ __noinline__ double run_em(const Rows& R, int n, const uint32_t* obs_
       if (b != v) other += full_f[b];
 #pragma unroll
     for (int b = 0; b < 5; b++) {
-      if (b == v) f[b] = f_fixed;
-      else f[b] = (other > 0.0) ? (1.0 - f_fixed) * full_f[b] / other : (1.0 - f_fixed) / 4.0;
+      if (b == v) f[b] = x;
+      else f[b] = (other > 0.0) ? (1.0 - x) * full_f[b] / other : (1.0 - x) / 4.0;
     }
     allowed = 31u;
   }
-  double sums[6];
-  double f_last[5];
-  const int first = (v < 0) ? 1 : 0;
-#pragma unroll 1
-  for (int it = first; it < first + 50; it++) {
-    em_pass(R, n, f, allowed, term, false, sums);
+  W->allowed[t] = allowed;
 #pragma unroll
-    for (int b = 0; b < 5; b++) f_last[b] = f[b];
-    double max_delta = 0.0;
-    if (v < 0) {
+  for (int b = 0; b < 5; b++) W->f[t][b] = f[b];
+  W->state[t] = kTaskIter;
+}
+
+// Runs the started tasks of a wavefront to completion; each task's log10 likelihood goes to
+// res_out[its slot] and its final frequencies stay in W->f.  Per pass, for every running task:
+//   kTaskIter: sums[b] = Sum_i w_i(b), w_i(b) = f_b r_i(b) / s_i, s_i = Sum_{allowed b} f_b r_i(b)
+//   kTaskLast: sums[0] = Sum_i (log10 s_i + lpmax_i) over reads with s_i > 0
+// every sum in read order (identify_mutations.cpp:3268-3300); then the reference's loop body
+// (:3268-3318, 3094-3143) updates the frequencies: 50 passes at most, stop when the largest change
+// is below tol, then one likelihood pass from that last pass's frequencies.
+__device__ __noinline__ void run_tasks(const Rows R, int n, LdsTasks* W, double tol, LdsDouble* res_out) {
+  const int lane = threadIdx.x & 63;
+  const int my_t = min(lane / 5, kTW - 1), my_b = lane % 5;
+  const bool sum_lane = lane < 5 * kTW;
+  const LdsDouble* my_col = &W->term[my_t][my_b][0];
+  for (;;) {
+    // the pass's task states and frequencies, read once (wave-uniform)
+    int st[kTW];
+    uint32_t al[kTW];
+    double f[kTW][5];
+    bool any = false;
 #pragma unroll
-      for (int b = 0; b < 5; b++) {
-        if (!(allowed & (1u << b))) continue;
-        const double fn = sums[1 + b] / (double)n;
-        max_delta = fmax(max_delta, fabs(fn - f[b]));
-        f[b] = fn;
+    for (int t = 0; t < kTW; t++) {
+      st[t] = __builtin_amdgcn_readfirstlane(W->state[t]);
+      al[t] = (uint32_t)__builtin_amdgcn_readfirstlane((int)W->allowed[t]);
+      const LdsDouble* src = (st[t] == kTaskLast) ? W->fl[t] : W->f[t];
+#pragma unroll
+      for (int b = 0; b < 5; b++) f[t][b] = readfirstlane_d(src[b]);
+      any = any || (st[t] != kTaskIdle);
+    }
+    if (!any) break;
+    double acc = 0.0;
+    for (int c0 = 0; c0 < n; c0 += kTermCols) {
+      const int i = c0 + lane;
+      if (i >= n) {
+        // padding of the chunk's last partial group of 16
+#pragma unroll
+        for (int t = 0; t < kTW; t++) {
+          LdsDouble* tm = &W->term[t][0][0];
+#pragma unroll
+          for (int b = 0; b < 5; b++) tm[b * kTermRow + lane] = -0.0;
+        }
+      } else {
+        double r[6];
+        R.get(i, r);
+#pragma unroll
+        for (int t = 0; t < kTW; t++) {
+          if (st[t] == kTaskIdle) continue;
+          double x[5];
+          double s = 0.0;
+#pragma unroll
+          for (int b = 0; b < 5; b++) {
+            x[b] = f[t][b] * r[b];
+            if (al[t] & (1u << b)) s += x[b];
+          }
+          LdsDouble* tm = &W->term[t][0][0];
+          if (st[t] == kTaskLast) {
+            tm[lane] = (s > 0.0) ? log10(s) + r[5] : 0.0;
+            continue;
+          }
+          double q[5];
+          if (s > 0.0) {
+            // f_b r_b / s correctly rounded from one reciprocal (the SharedDiv step, with the
+            // out-of-range fallback taken per lane only when needed)
+            const double y = 1.0 / s;
+            bool slow = !(s >= 0x1p-500 && s <= 0x1p500);
+#pragma unroll
+            for (int b = 0; b < 5; b++) {
+              const double q0 = x[b] * y;
+              q[b] = fma(fma(-s, q0, x[b]), y, q0);
+              slow = slow || !(x[b] == 0.0 || (x[b] >= 0x1p-500 && x[b] <= 0x1p500));
+            }
+            if (slow) {
+#pragma unroll
+              for (int b = 0; b < 5; b++) q[b] = x[b] / s;
+            }
+          } else {
+#pragma unroll
+            for (int b = 0; b < 5; b++) q[b] = f[t][b];
+          }
+#pragma unroll
+          for (int b = 0; b < 5; b++) tm[b * kTermRow + lane] = (al[t] & (1u << b)) ? q[b] : 0.0;
+        }
       }
-    } else {
-      double others = 0.0;
+      wave_sync();
+      // lanes of idle tasks add stale columns; their sums are not used
+      if (sum_lane) acc = sum_chunk(acc, my_col, min(kTermCols, n - c0));
+      wave_sync();
+    }
+    if (sum_lane) W->sum[my_t][my_b] = acc;
+    wave_sync();
+    // update, lane 5t+b for allele b of task t
+    if (sum_lane) {
+      const int mst = W->state[my_t];
+      if (mst == kTaskLast) {
+        if (my_b == 0) {
+          res_out[W->res[my_t]] = acc;
+          W->state[my_t] = kTaskIdle;
+        }
+      } else if (mst == kTaskIter) {
+        const double fo = W->f[my_t][my_b];
+        const double mine = W->sum[my_t][my_b];
+        double fn = fo;
+        bool upd = false;
+        if (W->mode[my_t] == 1) {
+          upd = (W->allowed[my_t] >> my_b) & 1u;
+          if (upd) fn = SharedDiv((double)n)(mine);
+        } else {
+          const int v = W->v[my_t];
+          const double x = W->x[my_t];
+          double others = 0.0;
 #pragma unroll
-      for (int b = 0; b < 5; b++)
-        if (b != v) others += sums[1 + b];
-#pragma unroll
-      for (int b = 0; b < 5; b++) {
-        if (b == v) continue;
-        const double fn = (others > 0.0) ? (1.0 - f_fixed) * sums[1 + b] / others : (1.0 - f_fixed) / 4.0;
-        max_delta = fmax(max_delta, fabs(fn - f[b]));
-        f[b] = fn;
+          for (int b = 0; b < 5; b++)
+            if (b != v) others += W->sum[my_t][b];
+          upd = my_b != v;
+          if (upd) fn = (others > 0.0) ? SharedDiv(others)((1.0 - x) * mine) : (1.0 - x) / 4.0;
+        }
+        W->fl[my_t][my_b] = fo;
+        W->f[my_t][my_b] = fn;
+        W->delta[my_t][my_b] = upd ? fabs(fn - fo) : 0.0;
       }
     }
-    if (max_delta < tol) break;
+    wave_sync();
+    if (lane < kTW && W->state[lane] == kTaskIter) {
+      double max_delta = 0.0;
+#pragma unroll
+      for (int b = 0; b < 5; b++) max_delta = fmax(max_delta, W->delta[lane][b]);
+      const int it = W->it[lane] + 1;
+      W->it[lane] = it;
+      if (max_delta < tol || it >= 50) W->state[lane] = kTaskLast;
+    }
+    wave_sync();
   }
-  // log10 likelihood of the last pass
-  em_pass(R, n, f_last, allowed, term, true, sums);
-  const double ll = sums[0];
-  if (f_out)
-    for (int b = 0; b < 5; b++) f_out[b] = f[b];
-  return ll;
 }
 
 __device__ __noinline__ double log_choose(double n, double k) { return lgamma(n + 1) - lgamma(k + 1) - lgamma(n - k + 1); }
 
 // fisher_exact_test_2x2 (stats.cpp:2144-2169): terms across lanes, summed in order by lane 0.
-__device__ __noinline__ double fisher_2x2_w(uint32_t a, uint32_t b, uint32_t c, uint32_t d, double* buf) {
+__device__ __noinline__ double fisher_2x2_w(uint32_t a, uint32_t b, uint32_t c, uint32_t d, LdsDouble* buf) {
   const int lane = threadIdx.x & 63;
   const uint32_t row1 = a + b, row2 = c + d, col1 = a + c, col2 = b + d;
   const uint32_t n = row1 + row2;
@@ -272,10 +427,10 @@ __device__ __noinline__ double fisher_2x2_w(uint32_t a, uint32_t b, uint32_t c, 
 
 // ks_test_two_sample_less (stats.cpp:2190-2284) from per-quality histograms of x (minor allele) and
 // y (major allele); pooled ties are whole quality values.
-__device__ __noinline__ double ks_less_w(ScoreShared& SS, uint32_t nx, uint32_t ny) {
+__device__ __noinline__ double ks_less_w(LdsShared* SS, uint32_t nx, uint32_t ny) {
   const int lane = threadIdx.x & 63;
-  const uint32_t* hx = SS.qh[0];
-  const uint32_t* hy = SS.qh[1];
+  const LdsU32* hx = SS->qh[0];
+  const LdsU32* hy = SS->qh[1];
   // statistic and the counts at its minimum (lane-redundant: 128 steps)
   uint32_t cx = 0, cy = 0, bx = 0, by = 0;
   double min_z = 0.0;
@@ -303,27 +458,27 @@ __device__ __noinline__ double ks_less_w(ScoreShared& SS, uint32_t nx, uint32_t 
   const uint32_t D = m + n;
   // boundary(d): d is 0 or the pooled count at the end of a tie group
   const uint32_t nw = D / 32 + 1;
-  for (uint32_t k = lane; k < nw; k += 64) SS.ks_bnd[k] = 0;
+  for (uint32_t k = lane; k < nw; k += 64) SS->ks_bnd[k] = 0;
   wave_sync();
   if (lane == 0) {
-    SS.ks_bnd[0] |= 1u;
+    SS->ks_bnd[0] |= 1u;
     uint32_t cum = 0;
 #pragma unroll 1
     for (int v = 0; v < 128; v++) {
       const uint32_t g = hx[v] + hy[v];
       if (!g) continue;
       cum += g;
-      SS.ks_bnd[cum >> 5] |= 1u << (cum & 31);
+      SS->ks_bnd[cum >> 5] |= 1u << (cum & 31);
     }
   }
   wave_sync();
   // the diagonal index runs over the smaller dimension
   const bool by_i = (m <= n);
   const uint32_t small = by_i ? m : n, large = by_i ? n : m;
-  double* A = SS.ks_a;   // diagonal d-1
-  double* B = SS.ks_b;   // diagonal d
+  LdsDouble* A = SS->ks_a;   // diagonal d-1
+  LdsDouble* B = SS->ks_b;   // diagonal d
   for (uint32_t d = 0; d <= D; d++) {
-    const bool bnd = (SS.ks_bnd[d >> 5] >> (d & 31)) & 1u;
+    const bool bnd = (SS->ks_bnd[d >> 5] >> (d & 31)) & 1u;
     const uint32_t s_lo = (d > large) ? d - large : 0u;
     const uint32_t s_hi = min(small, d);
     for (uint32_t s = s_lo + lane; s <= s_hi; s += 64) {
@@ -343,7 +498,7 @@ __device__ __noinline__ double ks_less_w(ScoreShared& SS, uint32_t nx, uint32_t 
       B[s] = val;
     }
     wave_sync();
-    double* t = A;
+    LdsDouble* t = A;
     A = B;
     B = t;
   }
@@ -384,14 +539,15 @@ __device__ __forceinline__ int next_index(const double f[5], uint32_t n, int exc
 
 }  // namespace
 
-__global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
+__global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3))) k_score_candidates(ImParams P) {
   extern __shared__ __align__(16) unsigned char dyn[];
   uint32_t* keys = (uint32_t*)dyn;
   double* rows = (double*)(dyn + (((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15));
   __shared__ ScoreShared SS;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  double* term = &SS.term[wave][0][0];
+  LdsTasks* W = (LdsTasks*)&SS.wt[wave];
+  LdsDouble* term = &W->term[0][0][0];
   const int S = P.read_set_max, Q = P.quality_max;
   const double tol = P.precision_decimal;
   const double kDrop = 0.587566;   // kProfileLikelihoodLog10Drop (identify_mutations.cpp:3173)
@@ -435,8 +591,7 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
       __syncthreads();
     }
     const int n = min(SS.n_keys, P.key_cap);
-    uint32_t obs_count[5];
-    for (int b = 0; b < 5; b++) obs_count[b] = SS.cov[b][0] + SS.cov[b][2];
+    if (tid < 5) SS.obs[tid] = SS.cov[tid][0] + SS.cov[tid][2];
     // ---- table rows of the reads, cached in LDS
     const bool cached = n <= P.rows_cap;
     if (cached)
@@ -445,6 +600,7 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
       SS.done = 0;
       SS.emitted = 0;
       SS.active[0] = SS.active[1] = 0;
+      SS.depth[0] = SS.depth[1] = 0;
       SS.lower = 0.0;
       SS.upper = 1.0;
       SS.ks_p = 1.0;
@@ -459,59 +615,44 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
     const int ref_idx = char_index(refc) < 0 ? 5 : char_index(refc);
     int major = 5, minor = 5, variant = 5;
     bool cons = false;
-    // ---- phases: one EM task per wavefront and phase, a single EM call site
+    // ---- phases: each wavefront starts its tasks of the phase, then runs them to completion
     for (int ph = 0;; ph++) {
-      int mode = 0;   // 0 none, 1 fit, 2 profile
-      uint32_t allowed = 31u;
-      double x = 0.0;
-      int res_slot = -1;
-      if (ph == 0) {
-        if (wave == 0) {
-          mode = 1;
-          res_slot = kResFull;
-        }
-      } else if (ph == 1) {
-        if (variant != 5 && n > 0) {
-          if (wave == 0) {
-            mode = 1;
-            allowed = 31u & ~(1u << variant);
-            res_slot = kResNull;
-          } else if (wave <= 3) {
-            mode = 2;
-            x = (wave == 1) ? SS.full_f[variant] : (wave == 2) ? 0.0 : 1.0;
-            res_slot = (wave == 1) ? kResPlmax : (wave == 2) ? kResP0 : kResP1;
+      if (lane == 0) {
+        // task slots of this wavefront for this phase
+        int k = 0;
+        auto start = [&](int mode, uint32_t allowed, int v, double x, int res) {
+          task_start(W, k++, mode, allowed, v, x, res, (const LdsU32*)SS.obs, (const LdsDouble*)SS.full_f, n,
+                     (LdsDouble*)SS.res);
+        };
+        if (ph == 0) {
+          if (wave == 0) start(1, 31u, -1, 0.0, kResFull);
+        } else if (ph == 1) {
+          if (variant != 5 && n > 0) {
+            if (wave == 0) start(1, 31u & ~(1u << variant), -1, 0.0, kResNull);
+            else if (wave == 1) start(2, 31u, variant, SS.full_f[variant], kResPlmax);
+            else if (wave == 2) start(2, 31u, variant, 0.0, kResP0);
+            else start(2, 31u, variant, 1.0, kResP1);
+          }
+        } else {
+          // one bound open: its seven midpoints, two per wavefront (nodes 2w, 2w+1);
+          // both open: three midpoints per bound, the lower bound's on wavefronts 0-1 and the
+          // upper bound's on wavefronts 2-3
+          const bool a0 = SS.active[0] != 0, a1 = SS.active[1] != 0;
+          if (a0 && a1) {
+            const int side = wave / 2;
+            const int n0 = (wave % 2 == 0) ? 0 : 2, n1 = (wave % 2 == 0) ? 2 : 3;
+            for (int node = n0; node < n1; node++) start(2, 31u, variant, SS.pts[side][node], kResBisect + side * kBisectPts + node);
+          } else if (a0 || a1) {
+            const int side = a0 ? 0 : 1;
+            for (int node = 2 * wave; node < min(2 * wave + 2, kBisectPts); node++)
+              start(2, 31u, variant, SS.pts[side][node], kResBisect + side * kBisectPts + node);
           }
         }
-      } else {
-        // both bounds running: the lower one on wavefronts 0-2, the upper one on wavefront 3;
-        // one running: wavefronts 0-2 on it
-        const bool a0 = SS.active[0] != 0, a1 = SS.active[1] != 0;
-        int side = -1, k = 0;
-        if (a0 && a1) {
-          side = (wave < 3) ? 0 : 1;
-          k = (wave < 3) ? wave : 0;
-        } else if ((a0 || a1) && wave < 3) {
-          side = a0 ? 0 : 1;
-          k = wave;
-        }
-        if (side >= 0) {
-          const double lo = SS.lo[side], hi = SS.hi[side];
-          const double mid = 0.5 * (lo + hi);
-          x = (k == 0) ? mid : (k == 1) ? 0.5 * (lo + mid) : 0.5 * (mid + hi);
-          mode = 2;
-          res_slot = kResBisect + side * 3 + k;
-        }
+        for (; k < kTW; k++) W->state[k] = kTaskIdle;
       }
-      if (mode != 0) {
-        double f[5];
-        const double ll = run_em(RR, n, obs_count, allowed, SS.full_f, (mode == 2) ? variant : -1, x, tol, term,
-                                 (ph == 0) ? f : nullptr);
-        if (lane == 0) {
-          SS.res[res_slot] = ll;
-          if (ph == 0)
-            for (int b = 0; b < 5; b++) SS.full_f[b] = f[b];
-        }
-      }
+      wave_sync();
+      run_tasks(RR, n, W, tol, (LdsDouble*)SS.res);
+      if (ph == 0 && wave == 0 && lane < 5) SS.full_f[lane] = W->f[0][lane];
       if (ph == 0 && wave == 1) {
         // pure_genotype_call on the summed likelihoods of the same reads, in read order
         // (identify_mutations.cpp:1669-1689, 3398-3433)
@@ -551,10 +692,10 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
           ny += __shfl_xor(ny, o, 64);
         }
         wave_sync();
-        const double ks = (nx > 0 && ny > 0) ? ks_less_w(SS, nx, ny) : 1.0;
+        const double ks = (nx > 0 && ny > 0) ? ks_less_w((LdsShared*)&SS, nx, ny) : 1.0;
         const uint32_t mjt = (major < 5) ? SS.cov[major][2] : 0, mjb = (major < 5) ? SS.cov[major][0] : 0;
         const uint32_t mnt = (minor < 5) ? SS.cov[minor][2] : 0, mnb = (minor < 5) ? SS.cov[minor][0] : 0;
-        const double fs = fisher_2x2_w(mnt, mnb, mjt, mjb, SS.fisher);
+        const double fs = fisher_2x2_w(mnt, mnb, mjt, mjb, (LdsDouble*)SS.fisher);
         if (lane == 0) {
           SS.ks_p = ks;
           SS.fisher_p = fs;
@@ -604,30 +745,24 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
             }
           }
         } else {
-          // each running bisection advances two steps from three evaluations (midpoint and both
-          // possible next midpoints), or one step from one evaluation
-          const bool both = SS.active[0] && SS.active[1];
+          // walk each open bound down the evaluated levels: every level is one sequential
+          // bisection step (interval halved at 0.5 * (lo + hi), ≤ 40 steps, stop at width ≤ tol)
           for (int side = 0; side < 2; side++) {
             if (!SS.active[side]) continue;
-            const bool three = !(both && side == 1);
-            const double lo = SS.lo[side], hi = SS.hi[side];
-            const double mid = 0.5 * (lo + hi);
-            const double* r3 = &SS.res[kResBisect + side * 3];
-            double l = lo, h = hi;
+            const double* r = &SS.res[kResBisect + side * kBisectPts];
+            double l = SS.lo[side], h = SS.hi[side];
             int i = SS.iter[side];
-            // lower bound: a profile >= target moves hi down; upper bound: moves lo up
-            const bool ge = r3[0] >= SS.target;
-            const bool left = (side == 0) ? ge : !ge;   // the interval kept is [lo, mid]
-            if (left) h = mid;
-            else l = mid;
-            i++;
-            if (three && i < 40 && (h - l) > tol) {
-              const double m2 = 0.5 * (l + h);   // == the left or right next midpoint evaluated
-              const bool ge2 = (left ? r3[1] : r3[2]) >= SS.target;
-              const bool left2 = (side == 0) ? ge2 : !ge2;
-              if (left2) h = m2;
-              else l = m2;
+            int node = 0;
+            for (int level = 0; level < SS.depth[side]; level++) {
+              if (level > 0 && !(i < 40 && (h - l) > tol)) break;
+              const double m = 0.5 * (l + h);   // == SS.pts[side][node]
+              // lower bound: a profile >= target moves hi down; upper bound: moves lo up
+              const bool ge = r[node] >= SS.target;
+              const bool left = (side == 0) ? ge : !ge;   // the interval kept is [l, m]
+              if (left) h = m;
+              else l = m;
               i++;
+              node = 2 * node + (left ? 1 : 2);
             }
             SS.lo[side] = l;
             SS.hi[side] = h;
@@ -636,6 +771,24 @@ __global__ void __launch_bounds__(kSW * 64) k_score_candidates(ImParams P) {
             if (side == 0) SS.lower = h;
             else SS.upper = l;
           }
+        }
+        // midpoints of the next round: breadth-first over the levels evaluated, each the
+        // midpoint of the interval a sequential bisection would hold there
+        const bool both = SS.active[0] && SS.active[1];
+        for (int side = 0; side < 2; side++) {
+          SS.depth[side] = SS.active[side] ? (both ? 2 : 3) : 0;
+          if (!SS.active[side]) continue;
+          const double lo = SS.lo[side], hi = SS.hi[side];
+          double* pt = SS.pts[side];
+          const double m0 = 0.5 * (lo + hi);
+          const double m1 = 0.5 * (lo + m0), m2 = 0.5 * (m0 + hi);
+          pt[0] = m0;
+          pt[1] = m1;
+          pt[2] = m2;
+          pt[3] = 0.5 * (lo + m1);
+          pt[4] = 0.5 * (m1 + m0);
+          pt[5] = 0.5 * (m0 + m2);
+          pt[6] = 0.5 * (m2 + hi);
         }
         SS.done = (SS.active[0] || SS.active[1]) ? 0 : 1;
       }

@@ -288,6 +288,8 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   HIP_OK(hipMemcpyAsync(S.d_cigar.p, cig.data(), cig.size() * sizeof(uint32_t), hipMemcpyHostToDevice, S.stream));
   // bases and qualities: uploaded as given, then repacked on the device to the aligned layout
   const size_t nbases = (size_t)a->seq_off[a->n_reads];
+  // the column pass addresses a read's bytes with a 32-bit offset
+  if (dev_bytes + 16 >= ((int64_t)1 << 32)) throw BsqError(BSQ_ERR_LIMIT, "more than 4 GiB of read bases");
   S.d_bases.alloc((size_t)dev_bytes + 16);
   S.d_quals.alloc((size_t)dev_bytes + 16);
   HIP_OK(hipMemsetAsync(S.d_bases.p, 0, (size_t)dev_bytes + 16, S.stream));
@@ -671,7 +673,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   P.ins_cap = ins_cap;
   P.key_cap = std::max(64, S.max_window_reads);
   const size_t key_bytes = ((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15;
-  const size_t score_dyn_budget = 64 * 1024 - kScoreStaticLds;
+  const size_t score_dyn_budget = kScoreLdsBudget - kScoreStaticLds;
   if (key_bytes > score_dyn_budget) throw BsqError(BSQ_ERR_LIMIT, "column depth too large for the exact scorer's LDS list");
   P.rows_cap = (int32_t)((score_dyn_budget - key_bytes) / (6 * sizeof(double)));
   const size_t score_lds = key_bytes + (size_t)P.rows_cap * 6 * sizeof(double);

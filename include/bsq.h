@@ -296,6 +296,14 @@ int bsq_session_identify_unscreened(bsq_session* s, const bsq_identify_args* arg
 int bsq_session_timing(const bsq_session* s, bsq_timing* out);
 void bsq_session_destroy(bsq_session* s);
 
+/* Diagnostic without a counterpart in breseq: checks on device `device` that the scorer's
+ * shared-denominator division (one reciprocal, then an FMA correction per quotient) returns the
+ * correctly rounded quotient, bit for bit against ordinary division, on n random and adversarial
+ * operand pairs.  *mismatches must come back 0; *fast_path counts the pairs that took the
+ * shared-reciprocal path. */
+int bsq_selftest_division(int32_t device, uint64_t n, uint64_t seed, uint64_t* mismatches,
+                          uint64_t* fast_path, bsq_status* status);
+
 /* One-shot conveniences: create a session on device 0, load, run, destroy. */
 int bsq_error_count(const bsq_error_count_args* args, bsq_ec_result** out, bsq_status* status);
 int bsq_identify_mutations(const bsq_identify_args* args, bsq_im_result** out,

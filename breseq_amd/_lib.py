@@ -168,6 +168,21 @@ class Timing(C.Structure):
     ]
 
 
+class CoverageTableArgs(C.Structure):
+    _fields_ = [
+        ("target", C.c_int32),
+        ("start_1", C.c_int64),
+        ("end_1", C.c_int64),
+        ("resolution", C.c_uint32),
+        ("total_only", C.c_int32),
+        ("per_read_group", C.c_int32),
+        ("n_read_groups", C.c_int32),
+        ("csv", C.c_int32),
+        ("show_average", C.c_int32),
+        ("reference_average", C.c_double),
+    ]
+
+
 class SimConfig(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64),
@@ -237,6 +252,7 @@ def lib():
     _decl(L, "bsq_session_error_count", C.c_int, [vp, P(ErrorCountArgs), P(vp), st])
     _decl(L, "bsq_session_identify", C.c_int, [vp, P(IdentifyArgs), P(vp), st])
     _decl(L, "bsq_session_identify_unscreened", C.c_int, [vp, P(IdentifyArgs), P(vp), st])
+    _decl(L, "bsq_session_coverage_table", C.c_int, [vp, P(CoverageTableArgs), P(vp), c_i64p, st])
     _decl(L, "bsq_session_timing", C.c_int, [vp, P(Timing)])
     _decl(L, "bsq_session_destroy", None, [vp])
     _decl(L, "bsq_error_count", C.c_int, [P(ErrorCountArgs), P(vp), st])

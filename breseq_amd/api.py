@@ -77,6 +77,11 @@ class Sample:
         return L.lib().bsq_sample_reference(self._h)
 
     @property
+    def n_read_groups(self):
+        """@RG lines of the BAM header (0 for a simulated sample)."""
+        return int(L.lib().bsq_sample_n_read_groups(self._h))
+
+    @property
     def aln(self):
         return L.lib().bsq_sample_alignments(self._h)
 
@@ -212,6 +217,33 @@ class Session:
 
     def identify_mutations(self, **kw):
         return identify_mutations(self.sample, session=self, **kw)
+
+    def coverage_table(self, target=0, start=0, end=0, resolution=600, total_only=False, per_read_group=False,
+                       csv=False, reference_average=None, n_read_groups=None):
+        """BAM2COV table of one region (coverage_output::table, coverage_output.cpp:190-283) from the
+        loaded sample: start/end 1-based inclusive, both 0 for the whole target; reference_average
+        adds the reference_unique_average_cov line (--show-average)."""
+        args = coverage_table_args(self.sample, target, start, end, resolution, total_only, per_read_group, csv,
+                                   reference_average, n_read_groups)
+        out = C.c_void_p()
+        n = C.c_int64()
+        st = _status()
+        L.check(L.lib().bsq_session_coverage_table(self._h, C.byref(args), C.byref(out), C.byref(n), C.byref(st)), st)
+        try:
+            return C.string_at(out, n.value).decode()
+        finally:
+            L.lib().bsq_free_text(out)
+
+
+def coverage_table_args(sample, target=0, start=0, end=0, resolution=600, total_only=False, per_read_group=False,
+                        csv=False, reference_average=None, n_read_groups=None):
+    """bsq_coverage_table_args for a region of a sample; the read-group count defaults to the
+    sample's @RG lines."""
+    n_rg = sample.n_read_groups if n_read_groups is None else n_read_groups
+    return L.CoverageTableArgs(int(target), int(start), int(end), int(resolution), 1 if total_only else 0,
+                               1 if per_read_group else 0, int(n_rg), 1 if csv else 0,
+                               0 if reference_average is None else 1,
+                               0.0 if reference_average is None else float(reference_average))
 
 
 def dp_settings(sets, seed=3):

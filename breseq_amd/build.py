@@ -20,6 +20,7 @@ SOURCES = [
     "session.hip",
     "extra.hip",
     "selftest.hip",
+    "coverage.hip",
     "sample.cpp",
     "host_util.cpp",
     "evidence.cpp",

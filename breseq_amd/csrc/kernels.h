@@ -157,6 +157,26 @@ __global__ void k_identify_columns_t(ImParams P);
 __global__ void k_prep_reads(const DRead* reads, int64_t n, int32_t S, int32_t Q, IdRec* out);
 __global__ void k_fix_columns(ImParams P);
 __global__ void k_insert_slots(ImParams P);
+// BAM2COV per-column counts over a region (coverage_output.cpp:307-503): lane = column, the window's
+// reads in file order.
+struct CovTabParams {
+  const DRead* reads;
+  const uint8_t* rg;     // @RG index of each read, 255 when it has none
+  const uint32_t* cigar;
+  const int32_t* win_rb;
+  const int32_t* win_re;
+  int32_t g0, g1;        // global columns of the region, inclusive
+  int32_t rg_filter;     // -1: every read; else only reads of this read group index
+  int32_t n_rg;          // @RG lines of the header (read_group_index_map::index resolves against it)
+  uint32_t* uniq;        // [2][n]: top, bottom (reversed) strand
+  double* red;           // [2][n]: Sum 1/X1 of redundant reads, in read order
+  uint32_t* raw;         // [2][n]: redundant reads
+  uint32_t* begin;       // [2][n]: unique reads whose first base (in read direction) is here
+  int32_t* last_end;     // largest gend among reads overlapping the region
+  uint32_t* err;
+};
+__global__ void k_coverage_columns(CovTabParams C);
+
 constexpr int kScoreWaves = 4;               // wavefronts per candidate in the exact scorer
 constexpr size_t kScoreStaticLds = 28 * 1024;   // bound on the scorer's static LDS
 constexpr size_t kScoreLdsBudget = 52 * 1024;   // static + dynamic LDS of a scorer block: three per CU

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <charconv>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -99,6 +100,10 @@ struct Session {
   DBuf<int32_t> d_counters;   // [0] n_cand, [1] n_ins, [2] n_runs, [3] error bits, [4] n_fix
   DBuf<int32_t> d_fix;
   DBuf<IdRec> d_idrec;
+  DBuf<uint32_t> d_ct_u, d_ct_raw, d_ct_beg;   // BAM2COV column counts (coverage table)
+  DBuf<double> d_ct_red;
+  DBuf<int32_t> d_ct_misc;                    // [0] largest read end overlapping the region, [1] error bits
+  DBuf<uint8_t> d_rg;                         // @RG index of each read (255: none)
   DBuf<int32_t> d_ins;
   DBuf<RaOut> d_ra;
   DBuf<int2> d_events, d_runs;
@@ -152,6 +157,8 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   // reads
   std::vector<DRead> dr;
   dr.reserve((size_t)a->n_reads);
+  std::vector<uint8_t> h_rg;   // @RG index of each packed read (255: none)
+  h_rg.reserve((size_t)a->n_reads);
   std::vector<int64_t> raw_off;
   raw_off.reserve((size_t)a->n_reads);
   int64_t dev_bytes = 0;
@@ -234,13 +241,16 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
       d.qbe0 = (uint16_t)(be - 1);
     }
     const int32_t x1 = (a->x1 && a->x1[r] >= 0) ? a->x1[r] : 1;
-    d.x1 = (uint16_t)std::min(x1, 65535);
+    if (x1 > 65535) throw BsqError(BSQ_ERR_LIMIT, "X1 redundancy above 65535");
+    d.x1 = (uint16_t)x1;
     d.xl = (int16_t)((a->xl && a->xl[r] >= 0) ? std::min(a->xl[r], 32767) : -1);
     d.xr = (int16_t)((a->xr && a->xr[r] >= 0) ? std::min(a->xr[r], 32767) : -1);
     const uint32_t rfi = read_file_of(rf, a, r);
     if (rfi > 255) throw BsqError(BSQ_ERR_LIMIT, "more than 256 read files");
     max_rfile = std::max(max_rfile, rfi);
     d.rfile = (uint8_t)rfi;
+    if (a->rg && a->rg[r] >= 255) throw BsqError(BSQ_ERR_LIMIT, "more than 255 read groups");
+    h_rg.push_back((a->rg && a->rg[r] >= 0) ? (uint8_t)a->rg[r] : (uint8_t)255);
     auto it = mslots.find(a->mapq[r]);
     if (it == mslots.end()) it = mslots.emplace(a->mapq[r], (uint8_t)0).first;
     d.mslot = a->mapq[r];   // remapped below
@@ -284,6 +294,8 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   const double t2 = now_ms();
   S.d_reads.alloc(dr.size());
   HIP_OK(hipMemcpyAsync(S.d_reads.p, dr.data(), dr.size() * sizeof(DRead), hipMemcpyHostToDevice, S.stream));
+  S.d_rg.alloc(h_rg.size());
+  HIP_OK(hipMemcpyAsync(S.d_rg.p, h_rg.data(), h_rg.size(), hipMemcpyHostToDevice, S.stream));
   S.d_cigar.alloc(cig.size());
   HIP_OK(hipMemcpyAsync(S.d_cigar.p, cig.data(), cig.size() * sizeof(uint32_t), hipMemcpyHostToDevice, S.stream));
   // bases and qualities: uploaded as given, then repacked on the device to the aligned layout
@@ -845,6 +857,212 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   S.timing.host_pre_ms = th1 - th0;
 }
 
+// BAM2COV table of one region (coverage_output::table, coverage_output.cpp:190-283; rows of
+// pileup_callback, :307-503).  The counts of every column come from k_coverage_columns (one pass
+// for all reads, one more per read group with per_read_group); which positions get a row follows
+// the reference's traversal (pileup_base.cpp:118-210, 308-360):
+//   * a position is handled when it lies in the region and (position + start) is a multiple of the
+//     downsampling step floor(size / resolution) (at least 1);
+//   * up to the last column any read overlapping the region covers, positions are visited one by
+//     one, so each handled position prints its own row;
+//   * after it, only handled positions advance the traversal, so a handled position also prints
+//     zero rows for the unhandled positions since the previous one (from position 1 when no read
+//     overlaps the region); those rows count in the averages' denominator only.
+static std::string coverage_table(Session& S, const bsq_coverage_table_args* a) {
+  if (!a) throw BsqError(BSQ_ERR_ARG, "null argument");
+  if (a->target < 0 || a->target >= S.n_targets) throw BsqError(BSQ_ERR_ARG, "target id out of range");
+  const int64_t L = S.lengths[(size_t)a->target];
+  int64_t start = a->start_1, end = a->end_1;
+  if (start == 0 && end == 0) {
+    start = 1;
+    end = L;
+  }
+  if (start < 1 || end > L || start > end) throw BsqError(BSQ_ERR_ARG, "region outside its reference sequence");
+  const int64_t off = S.tgt_off[(size_t)a->target];
+  const int64_t n = end - start + 1;
+  const int n_rg = std::max(1, a->n_read_groups);
+  const int passes = a->per_read_group ? 1 + n_rg : 1;
+  S.d_ct_u.alloc((size_t)(2 * n));
+  S.d_ct_raw.alloc((size_t)(2 * n));
+  S.d_ct_beg.alloc((size_t)(2 * n));
+  S.d_ct_red.alloc((size_t)(2 * n));
+  S.d_ct_misc.alloc(2);
+  std::vector<std::vector<uint32_t>> hu(passes), hraw(passes), hbeg(passes);
+  std::vector<std::vector<double>> hred(passes);
+  const int32_t misc[2] = {-1, 0};
+  int32_t last_end = -1;
+  for (int k = 0; k < passes; k++) {
+    CovTabParams C;
+    C.reads = S.d_reads.p;
+    C.rg = S.d_rg.p;
+    C.cigar = S.d_cigar.p;
+    C.win_rb = S.d_win_rb.p;
+    C.win_re = S.d_win_re.p;
+    C.g0 = (int32_t)(off + start - 1);
+    C.g1 = (int32_t)(off + end - 1);
+    C.rg_filter = k - 1;
+    C.n_rg = a->n_read_groups;
+    C.uniq = S.d_ct_u.p;
+    C.red = S.d_ct_red.p;
+    C.raw = S.d_ct_raw.p;
+    C.begin = S.d_ct_beg.p;
+    C.last_end = S.d_ct_misc.p;
+    C.err = (uint32_t*)(S.d_ct_misc.p + 1);
+    HIP_OK(hipMemcpyAsync(S.d_ct_misc.p, misc, sizeof(misc), hipMemcpyHostToDevice, S.stream));
+    const int64_t nwin = C.g1 / 64 - C.g0 / 64 + 1;
+    const int blocks = (int)std::min<int64_t>((nwin + 3) / 4, 4096);
+    hipLaunchKernelGGL(k_coverage_columns, dim3(blocks), dim3(256), 0, S.stream, C);
+    HIP_OK(hipGetLastError());
+    hu[k].resize((size_t)(2 * n));
+    hraw[k].resize((size_t)(2 * n));
+    hbeg[k].resize((size_t)(2 * n));
+    hred[k].resize((size_t)(2 * n));
+    HIP_OK(hipMemcpyAsync(hu[k].data(), S.d_ct_u.p, (size_t)(2 * n) * 4, hipMemcpyDeviceToHost, S.stream));
+    HIP_OK(hipMemcpyAsync(hraw[k].data(), S.d_ct_raw.p, (size_t)(2 * n) * 4, hipMemcpyDeviceToHost, S.stream));
+    HIP_OK(hipMemcpyAsync(hbeg[k].data(), S.d_ct_beg.p, (size_t)(2 * n) * 4, hipMemcpyDeviceToHost, S.stream));
+    HIP_OK(hipMemcpyAsync(hred[k].data(), S.d_ct_red.p, (size_t)(2 * n) * 8, hipMemcpyDeviceToHost, S.stream));
+    int32_t got[2];
+    HIP_OK(hipMemcpyAsync(got, S.d_ct_misc.p, sizeof(got), hipMemcpyDeviceToHost, S.stream));
+    HIP_OK(hipStreamSynchronize(S.stream));
+    raise_err_bits((uint32_t)got[1]);
+    if (k == 0) last_end = got[0];   // all reads: the last column the region's reads cover
+  }
+  const int64_t last_pile = (last_end >= 0) ? (int64_t)last_end - off : 0;   // 1-based last covered column
+  uint32_t ds = 1;
+  if (a->resolution != 0) {
+    ds = (uint32_t)std::floor((double)(uint32_t)n / (double)a->resolution);
+    if (ds < 1) ds = 1;
+  }
+  const char d = a->csv ? ',' : '\t';
+  static const char* const kCols8[8] = {"unique_top_cov", "unique_bot_cov", "redundant_top_cov", "redundant_bot_cov",
+                                        "raw_redundant_top_cov", "raw_redundant_bot_cov", "unique_top_begin",
+                                        "unique_bot_begin"};
+  static const char* const kCols3[3] = {"unique_cov", "redundant_cov", "total_cov"};
+  const int ncols = a->total_only ? 3 : 8;
+  const char* const* cols = a->total_only ? kCols3 : kCols8;
+  std::string out;
+  out.reserve((size_t)(n / ds + 16) * (size_t)(24 + 8 * ncols * passes));
+  auto put_u = [&](uint64_t v) {
+    char b[24];
+    out.append(b, std::to_chars(b, b + sizeof(b), v).ptr);
+  };
+  out += "position";
+  out += d;
+  out += "ref_base";
+  for (int c = 0; c < ncols; c++) {
+    out += d;
+    out += cols[c];
+  }
+  for (int g = 0; a->per_read_group && g < n_rg; g++)
+    for (int c = 0; c < ncols; c++) {
+      out += d;
+      out += "RG-";
+      put_u((uint64_t)g);
+      out += '_';
+      out += cols[c];
+    }
+  out += '\n';
+  const char* ref = S.h_ref.data() + off;   // reference bases of the target
+  double avg_u = 0.0, avg_r = 0.0, avg_t = 0.0;
+  std::vector<double> rg_u((size_t)n_rg, 0.0), rg_r((size_t)n_rg, 0.0), rg_t((size_t)n_rg, 0.0);
+  uint32_t n_positions = 0;
+  auto zero_row = [&](int64_t pos) {
+    put_u((uint64_t)pos);
+    out += d;
+    out += ref[pos - 1];
+    for (int c = 0; c < ncols * passes; c++) {
+      out += d;
+      out += '0';
+    }
+    out += '\n';
+    n_positions++;
+  };
+  auto row = [&](int64_t pos) {
+    const size_t i = (size_t)(pos - start), j = (size_t)(n + pos - start);
+    put_u((uint64_t)pos);
+    out += d;
+    out += ref[pos - 1];
+    for (int k = 0; k < passes; k++) {
+      const uint32_t u0 = hu[k][i], u1 = hu[k][j];
+      const double r0 = hred[k][i], r1 = hred[k][j];
+      // the sums in the reference's expression order: (u0 + u1) unsigned, then + r0, + r1
+      const double tot = (double)(u0 + u1) + r0 + r1;
+      if (a->total_only) {
+        out += d;
+        put_u((uint64_t)(uint32_t)(u0 + u1));
+        out += d;
+        out += fmt_g(r0 + r1);
+        out += d;
+        out += fmt_g(tot);
+      } else {
+        out += d;
+        put_u(u0);
+        out += d;
+        put_u(u1);
+        out += d;
+        out += fmt_g(r0);
+        out += d;
+        out += fmt_g(r1);
+        out += d;
+        put_u(hraw[k][i]);
+        out += d;
+        put_u(hraw[k][j]);
+        out += d;
+        put_u(hbeg[k][i]);
+        out += d;
+        put_u(hbeg[k][j]);
+      }
+      if (k == 0) {
+        avg_u += (double)(uint32_t)(u0 + u1);
+        avg_r += r0 + r1;
+        avg_t += tot;
+      } else {
+        rg_u[(size_t)(k - 1)] += (double)(uint32_t)(u0 + u1);
+        rg_r[(size_t)(k - 1)] += r0 + r1;
+        rg_t[(size_t)(k - 1)] += tot;
+      }
+    }
+    out += '\n';
+    n_positions++;
+  };
+  auto handled = [&](int64_t pos) { return pos >= start && pos <= end && ((uint64_t)(pos + start) % ds) == 0; };
+  const int64_t covered_end = std::min<int64_t>(end, last_pile);
+  for (int64_t pos = start; pos <= covered_end; pos++)
+    if (handled(pos)) row(pos);
+  int64_t last = std::max<int64_t>(last_pile, 0);
+  for (int64_t pos = std::max<int64_t>(last_pile + 1, 1); pos <= end; pos++) {
+    if (!handled(pos)) continue;
+    for (int64_t g = last + 1; g < pos; g++) zero_row(g);
+    row(pos);
+    last = pos;
+  }
+  auto foot = [&](const std::string& name, const std::string& value) {
+    out += '#';
+    out += d;
+    out += name;
+    out += d;
+    out += value;
+    out += '\n';
+  };
+  if (a->show_average) foot("reference_unique_average_cov", fmt_g(a->reference_average));
+  foot("region_unique_average_cov", fmt_g(avg_u / (double)n_positions));
+  foot("region_repeat_average_cov", fmt_g(avg_r / (double)n_positions));
+  foot("region_average_cov", fmt_g(avg_t / (double)n_positions));
+  {
+    std::string v;
+    char b[24];
+    v.assign(b, std::to_chars(b, b + sizeof(b), (uint64_t)n_positions).ptr);
+    foot("number_of_positions", v);
+  }
+  for (int g = 0; a->per_read_group && g < n_rg; g++) {
+    const std::string p = "RG-" + std::to_string(g) + "_";
+    foot(p + "region_unique_average_cov", fmt_g(rg_u[(size_t)g] / (double)n_positions));
+    foot(p + "region_repeat_average_cov", fmt_g(rg_r[(size_t)g] / (double)n_positions));
+    foot(p + "region_average_cov", fmt_g(rg_t[(size_t)g] / (double)n_positions));
+  }
+  return out;
+}
+
 }  // namespace bsq
 
 /* ------------------------------------------------------------------------------------------ */
@@ -932,6 +1150,24 @@ int bsq_session_identify_unscreened(bsq_session* s, const bsq_identify_args* arg
   std::unique_ptr<bsq_im_result> r(new bsq_im_result());
   run_identify(s->s, args, *r, 1);
   *out = r.release();
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
+  BSQ_CATCH(status)
+}
+
+int bsq_session_coverage_table(bsq_session* s, const bsq_coverage_table_args* args, char** out_text,
+                               int64_t* out_len, bsq_status* status) {
+  BSQ_TRY
+  if (!s || !args || !out_text || !out_len) throw BsqError(BSQ_ERR_ARG, "null argument");
+  if (!s->s.loaded) throw BsqError(BSQ_ERR_ARG, "no sample loaded");
+  HIP_OK(hipSetDevice(s->s.device));
+  const std::string t = coverage_table(s->s, args);
+  char* buf = (char*)malloc(t.size() + 1);
+  if (!buf) throw BsqError(BSQ_ERR_LIMIT, "out of host memory");
+  memcpy(buf, t.data(), t.size());
+  buf[t.size()] = '\0';
+  *out_text = buf;
+  *out_len = (int64_t)t.size();
   set_status(status, BSQ_OK, "");
   return BSQ_OK;
   BSQ_CATCH(status)

@@ -296,6 +296,28 @@ int bsq_session_identify_unscreened(bsq_session* s, const bsq_identify_args* arg
 int bsq_session_timing(const bsq_session* s, bsq_timing* out);
 void bsq_session_destroy(bsq_session* s);
 
+/* BAM2COV coverage table of one region: coverage_output::table (coverage_output.cpp:190-283) with
+ * the rows of pileup_callback (:307-503) over do_pileup's region and downsampling
+ * (pileup_base.cpp:118-135, 141-210, 308-360).  Rows: position, reference base, then per strand the
+ * unique coverage, the redundant coverage (Sum 1/X1), the redundant read count and the unique reads
+ * starting there (or unique / redundant / total with total_only), repeated per read group with
+ * per_read_group; then the averages of the region.  *out_text is NUL-terminated, *out_len bytes
+ * long; release it with bsq_free_text. */
+typedef struct bsq_coverage_table_args {
+  int32_t target;             /* target id                                                    */
+  int64_t start_1, end_1;     /* region, 1-based inclusive; both 0: the whole target          */
+  uint32_t resolution;        /* --resolution (default 600); 0: every position                */
+  int32_t total_only;         /* --total-only                                                 */
+  int32_t per_read_group;     /* --per-read-group                                             */
+  int32_t n_read_groups;      /* @RG lines of the BAM header                                  */
+  int32_t csv;                /* --format CSV (comma-separated); else TSV                     */
+  int32_t show_average;       /* --show-average: the reference_unique_average_cov line        */
+  double reference_average;   /* its value: the target's coverage_average from summary.json   */
+} bsq_coverage_table_args;
+
+int bsq_session_coverage_table(bsq_session* s, const bsq_coverage_table_args* args, char** out_text,
+                               int64_t* out_len, bsq_status* status);
+
 /* Diagnostic without a counterpart in breseq: checks on device `device` that the scorer's
  * shared-denominator division (one reciprocal, then an FMA correction per quotient) returns the
  * correctly rounded quotient, bit for bit against ordinary division, on n random and adversarial

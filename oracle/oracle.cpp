@@ -28,6 +28,8 @@
  * SequenceTrims                              calculate_trims.cpp:67-138          brute force on tiny strings
  * discordant-pair regions (DP enter step,    identify_mutations.cpp:883-908,     hand-computed windows in
  *   check_discordant_completion, CSV)        1395-1457, 2358-2422, 2550-2573     tests/test_dp_regions.py
+ * BAM2COV coverage table (table, rows,       coverage_output.cpp:190-283,        hand-computed table in
+ *   region traversal and downsampling)       307-503; pileup_base.cpp:118-360    tests/test_coverage_table.py
  *
  * Not pinned bit-for-bit: lgamma (Cephes lngamma in the reference, glibc lgamma here) -- Fisher /
  * KS p-values agree with the reference's printed 6 significant digits on the expected.gd vectors.
@@ -1419,6 +1421,180 @@ static void sequence_trims(const char* s, uint32_t len, uint8_t* t) {
 
 /* ================================================================ C API */
 
+namespace orc {
+
+// BAM2COV table of one region, restating coverage_output::table (coverage_output.cpp:190-283),
+// coverage_output::pileup_callback (:307-503) and the traversal of pileup_base::do_pileup with its
+// first-level callback (pileup_base.cpp:118-210, 308-360), step by step.
+static string coverage_table(const bsq_reference* ref, const bsq_alignments* aln, const bsq_coverage_table_args* a) {
+  const Aln A{aln};
+  const int32_t tid = a->target;
+  if (tid < 0 || tid >= ref->n_targets) fail(BSQ_ERR_ARG, "target id out of range");
+  const int64_t L = ref->lengths[tid];
+  uint32_t start = (uint32_t)a->start_1, end = (uint32_t)a->end_1;
+  if (start == 0 && end == 0) {
+    start = 1;
+    end = (uint32_t)L;
+  }
+  if (start < 1 || (int64_t)end > L || start > end) fail(BSQ_ERR_ARG, "region outside its reference sequence");
+  const char* refseq = ref->seqs[tid];
+  // coverage_output::table: downsampling step from the resolution
+  const uint32_t size = end - start + 1;
+  uint32_t downsample = 1;
+  if (a->resolution != 0) {
+    downsample = static_cast<uint32_t>(floor(static_cast<double>(size) / static_cast<double>(a->resolution)));
+    if (downsample < 1) downsample = 1;
+  }
+  const string d = a->csv ? "," : "\t";
+  vector<string> cols;
+  if (a->total_only) cols = {"unique_cov", "redundant_cov", "total_cov"};
+  else
+    cols = {"unique_top_cov", "unique_bot_cov", "redundant_top_cov", "redundant_bot_cov", "raw_redundant_top_cov",
+            "raw_redundant_bot_cov", "unique_top_begin", "unique_bot_begin"};
+  const size_t n_rg_cols = (size_t)std::max(a->n_read_groups, 1);
+  std::ostringstream out;
+  out << "position" << d << "ref_base";
+  for (size_t c = 0; c < cols.size(); c++) out << d << cols[c];
+  if (a->per_read_group)
+    for (size_t g = 0; g < n_rg_cols; g++)
+      for (size_t c = 0; c < cols.size(); c++) out << d << "RG-" << g << "_" << cols[c];
+  out << std::endl;
+  double avg_unique = 0, avg_repeat = 0, avg_cov = 0;
+  uint32_t num_positions = 0;
+  vector<double> rg_unique(n_rg_cols, 0.0), rg_repeat(n_rg_cols, 0.0), rg_cov(n_rg_cols, 0.0);
+  // the reads sam_itr_queryi(tid, start-1, end) returns, and the columns they give the pileup
+  vector<int64_t> fetched;
+  for (int64_t r = 0; r < A.n(); r++) {
+    if (A.a->tid[r] != tid || (A.a->flag[r] & 0x4) || A.n_cigar(r) == 0) continue;
+    const int64_t p0 = A.a->pos[r], e0 = A.endpos(r);
+    if (e0 <= p0) continue;
+    if (p0 < (int64_t)end && e0 > (int64_t)start - 1) fetched.push_back(r);
+  }
+  vector<vector<Entry>> entries((size_t)L + 2);   // by 1-based position, fetched reads in file order
+  vector<char> covered((size_t)L + 2, 0);
+  for (int64_t r : fetched) {
+    ResolvedRead rr;
+    resolve_read(A, r, rr);
+    for (int64_t p = rr.pos; p < rr.end; p++) {
+      const int64_t j = p - rr.pos;
+      entries[(size_t)(p + 1)].push_back(Entry{r, rr.qpos[j], rr.indel[j], rr.is_del[j] != 0});
+      covered[(size_t)(p + 1)] = 1;
+    }
+  }
+  auto rg_index = [&](int64_t r) -> size_t {   // read_group_index_map::index (alignment.h:545-559)
+    if (a->n_read_groups <= 1) return 0;
+    const int32_t g = A.a->rg ? A.a->rg[r] : -1;
+    return (g < 0 || g >= a->n_read_groups) ? 0 : (size_t)g;
+  };
+  uint32_t last_position_1 = 0;   // pileup_base::m_last_position_1
+  auto handle_position = [&](uint32_t pos_1) {   // pileup_base.cpp:118-135 with clipping on
+    if (pos_1 < start) return false;
+    if (pos_1 > end) return false;
+    if ((pos_1 + start) % downsample != 0) return false;
+    return true;
+  };
+  const size_t ncc = cols.size();
+  auto callback = [&](uint32_t pos) {   // coverage_output::pileup_callback
+    if (pos == 0) return;
+    for (uint32_t i = last_position_1 + 1; i < pos; i++) {
+      out << i << d << refseq[i - 1];
+      const size_t zero_columns = ncc * (a->per_read_group ? (1 + n_rg_cols) : 1);
+      for (size_t c = 0; c < zero_columns; c++) out << d << 0;
+      out << std::endl;
+      num_positions++;
+    }
+    const uint8_t ref_base = (uint8_t)refseq[pos - 1];
+    uint32_t unique_cov[2] = {0, 0}, raw_redundant_cov[2] = {0, 0}, unique_begin_reads[2] = {0, 0};
+    double redundant_cov[2] = {0.0, 0.0};
+    const size_t num_rg = a->per_read_group ? n_rg_cols : 0;
+    vector<uint32_t> rg_unique_cov(num_rg * 2, 0), rg_raw(num_rg * 2, 0), rg_begin(num_rg * 2, 0);
+    vector<double> rg_red(num_rg * 2, 0.0);
+    for (const Entry& e : entries[pos]) {
+      if (e.is_del) continue;
+      const int64_t r = e.read;
+      const uint32_t redundancy = A.redundancy(r);
+      const uint32_t reversed = A.reversed(r) ? 1 : 0;
+      const size_t slot = a->per_read_group ? (rg_index(r) * 2 + reversed) : 0;
+      const uint32_t qpos_1 = (uint32_t)e.qpos + 1;
+      const bool this_is_first_base = !reversed ? (qpos_1 == 1) : (qpos_1 == A.len(r));
+      if (redundancy == 1) {
+        unique_cov[reversed]++;
+        if (this_is_first_base) unique_begin_reads[reversed]++;
+        if (a->per_read_group) {
+          rg_unique_cov[slot]++;
+          if (this_is_first_base) rg_begin[slot]++;
+        }
+      } else {
+        raw_redundant_cov[reversed]++;
+        redundant_cov[reversed] += 1.0 / redundancy;
+        if (a->per_read_group) {
+          rg_raw[slot]++;
+          rg_red[slot] += 1.0 / redundancy;
+        }
+      }
+    }
+    num_positions++;
+    avg_unique += unique_cov[0] + unique_cov[1];
+    avg_repeat += redundant_cov[0] + redundant_cov[1];
+    avg_cov += unique_cov[0] + unique_cov[1] + redundant_cov[0] + redundant_cov[1];
+    if (a->total_only) {
+      out << pos << d << ref_base << d << (unique_cov[0] + unique_cov[1]) << d << (redundant_cov[0] + redundant_cov[1])
+          << d << (unique_cov[0] + unique_cov[1] + redundant_cov[0] + redundant_cov[1]);
+    } else {
+      out << pos << d << ref_base << d << unique_cov[0] << d << unique_cov[1] << d << redundant_cov[0] << d
+          << redundant_cov[1] << d << raw_redundant_cov[0] << d << raw_redundant_cov[1] << d << unique_begin_reads[0]
+          << d << unique_begin_reads[1];
+    }
+    for (size_t g = 0; g < num_rg; g++) {
+      const size_t t = g * 2, b = g * 2 + 1;
+      if (a->total_only) {
+        out << d << (rg_unique_cov[t] + rg_unique_cov[b]) << d << (rg_red[t] + rg_red[b]) << d
+            << (rg_unique_cov[t] + rg_unique_cov[b] + rg_red[t] + rg_red[b]);
+      } else {
+        out << d << rg_unique_cov[t] << d << rg_unique_cov[b] << d << rg_red[t] << d << rg_red[b] << d << rg_raw[t] << d
+            << rg_raw[b] << d << rg_begin[t] << d << rg_begin[b];
+      }
+      rg_unique[g] += rg_unique_cov[t] + rg_unique_cov[b];
+      rg_repeat[g] += rg_red[t] + rg_red[b];
+      rg_cov[g] += rg_unique_cov[t] + rg_unique_cov[b] + rg_red[t] + rg_red[b];
+    }
+    out << std::endl;
+  };
+  // first_level_pileup_callback over the pileup's columns (those the fetched reads cover)
+  for (uint32_t pos = 1; pos <= (uint32_t)L; pos++) {
+    if (!covered[pos]) continue;
+    for (uint32_t on_pos_1 = last_position_1 + 1; on_pos_1 <= pos - 1; on_pos_1++) {
+      if (handle_position(on_pos_1)) callback(on_pos_1);
+      last_position_1 = on_pos_1;
+    }
+    if (handle_position(pos)) callback(pos);
+    last_position_1 = pos;
+  }
+  // do_pileup: positions after the last one the pileup handled
+  for (uint32_t on_pos_1 = last_position_1 + 1; on_pos_1 <= end; on_pos_1++) {
+    if (handle_position(on_pos_1)) {
+      callback(on_pos_1);
+      last_position_1 = on_pos_1;
+    }
+  }
+  if (a->show_average) out << "#" << d << "reference_unique_average_cov" << d << a->reference_average << std::endl;
+  out << "#" << d << "region_unique_average_cov" << d << (avg_unique / num_positions) << std::endl;
+  out << "#" << d << "region_repeat_average_cov" << d << (avg_repeat / num_positions) << std::endl;
+  out << "#" << d << "region_average_cov" << d << (avg_cov / num_positions) << std::endl;
+  out << "#" << d << "number_of_positions" << d << num_positions << std::endl;
+  if (a->per_read_group) {
+    for (size_t g = 0; g < n_rg_cols; g++) {
+      const string p = "RG-" + std::to_string(g) + "_";
+      out << "#" << d << p << "region_unique_average_cov" << d << (rg_unique[g] / num_positions) << std::endl;
+      out << "#" << d << p << "region_repeat_average_cov" << d << (rg_repeat[g] / num_positions) << std::endl;
+      out << "#" << d << p << "region_average_cov" << d << (rg_cov[g] / num_positions) << std::endl;
+    }
+  }
+  return out.str();
+}
+
+}  // namespace orc
+
 struct orc_ec_out { orc::EcOut o; };
 struct orc_im_out { string text; vector<double> scores; string dp_text; };
 
@@ -1542,6 +1718,25 @@ int orc_im_position_scores(const orc_im_out* o, const double** s, int64_t* n) {
 }
 
 void orc_im_free(orc_im_out* o) { delete o; }
+
+int orc_coverage_table(const bsq_reference* ref, const bsq_alignments* aln, const bsq_coverage_table_args* args,
+                       char** text, int64_t* len, bsq_status* st) {
+  try {
+    const string t = orc::coverage_table(ref, aln, args);
+    char* b = (char*)malloc(t.size() + 1);
+    memcpy(b, t.data(), t.size());
+    b[t.size()] = 0;
+    *text = b;
+    *len = (int64_t)t.size();
+    set_status(st, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const orc::orc_error& e) {
+    set_status(st, e.code, e.what());
+    return e.code;
+  }
+}
+
+void orc_free_text(char* t) { free(t); }
 
 int orc_sequence_trims(const char* seq, int64_t len, uint8_t* out) {
   if (len < 0 || len > 0xFFFFFFFFLL) return BSQ_ERR_ARG;

@@ -43,6 +43,11 @@ int orc_im_dp_text(const orc_im_out* o, const char** text, int64_t* len);
 int orc_im_position_scores(const orc_im_out* o, const double** score, int64_t* n);
 void orc_im_free(orc_im_out* o);
 
+/* BAM2COV coverage table of one region (coverage_output.cpp:190-503); release with orc_free_text */
+int orc_coverage_table(const bsq_reference* ref, const bsq_alignments* aln, const bsq_coverage_table_args* args,
+                       char** text, int64_t* len, bsq_status* status);
+void orc_free_text(char* text);
+
 /* SequenceTrims (calculate_trims.cpp:67-138) */
 int orc_sequence_trims(const char* seq, int64_t len, uint8_t* out);
 

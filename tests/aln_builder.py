@@ -92,10 +92,11 @@ IDX = {"A": 0, "C": 1, "G": 2, "T": 3, ".": 4}
 class BuiltSample:
     """Adapter giving a Built set the Sample interface the argument builders use."""
 
-    def __init__(self, built):
+    def __init__(self, built, n_read_groups=0):
         self.b = built
         self.ref = C.pointer(built.ref)
         self.aln = C.pointer(built.aln)
+        self.n_read_groups = n_read_groups
 
     @property
     def n_targets(self):

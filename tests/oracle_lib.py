@@ -42,6 +42,8 @@ def lib():
     O.orc_im_position_scores.argtypes = [vp, P(L.c_f64p), L.c_i64p]
     O.orc_im_free.argtypes = [vp]
     O.orc_sequence_trims.argtypes = [C.c_char_p, C.c_int64, L.c_u8p]
+    O.orc_coverage_table.argtypes = [P(L.Reference), P(L.Alignments), P(L.CoverageTableArgs), P(vp), L.c_i64p, st]
+    O.orc_free_text.argtypes = [vp]
     O.orc_fisher_exact_2x2.restype = C.c_double
     O.orc_fisher_exact_2x2.argtypes = [C.c_uint32] * 4
     O.orc_ks_test_less.restype = C.c_double
@@ -176,3 +178,16 @@ def pure_genotype(lps, n):
     best = C.c_int32()
     score = lib().orc_pure_genotype(lps.ctypes.data_as(L.c_f64p), n, C.byref(best))
     return best.value, score
+
+
+def coverage_table(ref, aln, args):
+    """BAM2COV table text of one region (coverage_output.cpp:190-503)."""
+    O = lib()
+    out = C.c_void_p()
+    n = C.c_int64()
+    st = L.Status()
+    L.check(O.orc_coverage_table(ref, aln, C.byref(args), C.byref(out), C.byref(n), C.byref(st)), st)
+    try:
+        return C.string_at(out, n.value).decode()
+    finally:
+        O.orc_free_text(out)

@@ -2,9 +2,12 @@
 // reproduces breseq's iostream output, and the host trims used when synthesizing XL/XR tags.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/bsq.h"
 
@@ -24,6 +27,39 @@ std::string fmt_double(double v, unsigned precision, bool scientific);
 int fmt_double_buf(double v, unsigned precision, bool scientific, char* buf, size_t len);
 // iostream default formatting of a double (%g, 6 significant digits).
 std::string fmt_g(double v, int precision = 6);
+
+// Host threads for parallel host work: the hardware threads, at most 16.
+inline int host_threads() {
+  const unsigned hc = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hc == 0 ? 4u : hc, 16u));
+}
+
+// Runs fn(t, begin, end) over [0, n) split into contiguous ranges, one per thread; an exception
+// thrown in a thread is rethrown here as a BsqError.
+template <class F>
+void parallel_ranges(int64_t n, int threads, F fn) {
+  threads = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n));
+  std::vector<std::thread> pool;
+  std::vector<std::string> errors(threads);
+  std::vector<int> codes(threads, 0);
+  for (int t = 0; t < threads; t++) {
+    const int64_t b = n * t / threads, e = n * (t + 1) / threads;
+    pool.emplace_back([&, t, b, e] {
+      try {
+        fn(t, b, e);
+      } catch (const BsqError& ex) {
+        codes[t] = ex.code;
+        errors[t] = ex.what();
+      } catch (const std::exception& ex) {
+        codes[t] = BSQ_ERR_FORMAT;
+        errors[t] = ex.what();
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  for (int t = 0; t < threads; t++)
+    if (codes[t]) throw BsqError(codes[t], errors[t]);
+}
 
 // SequenceTrims (calculate_trims.cpp:67-138): out[0..len) left, out[len..2len) right.
 void host_sequence_trims(const char* seq, int64_t len, uint8_t* out);

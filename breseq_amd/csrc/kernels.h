@@ -165,17 +165,21 @@ struct CovTabParams {
   const uint32_t* cigar;
   const int32_t* win_rb;
   const int32_t* win_re;
-  int32_t g0, g1;        // global columns of the region, inclusive
+  int32_t h0;            // global column of the first row
+  int32_t step;          // columns between rows (the downsampling step)
+  int64_t n;             // rows: columns h0 + j * step, j < n
   int32_t rg_filter;     // -1: every read; else only reads of this read group index
   int32_t n_rg;          // @RG lines of the header (read_group_index_map::index resolves against it)
   uint32_t* uniq;        // [2][n]: top, bottom (reversed) strand
   double* red;           // [2][n]: Sum 1/X1 of redundant reads, in read order
   uint32_t* raw;         // [2][n]: redundant reads
   uint32_t* begin;       // [2][n]: unique reads whose first base (in read direction) is here
-  int32_t* last_end;     // largest gend among reads overlapping the region
   uint32_t* err;
 };
+// Consecutive rows (step 1): one wavefront per 64 columns, lane = column.
 __global__ void k_coverage_columns(CovTabParams C);
+// Downsampled rows (step > 1): one wavefront per row, lanes over the row's reads.
+__global__ void k_coverage_rows(CovTabParams C);
 
 constexpr int kScoreWaves = 4;               // wavefronts per candidate in the exact scorer
 constexpr size_t kScoreStaticLds = 28 * 1024;   // bound on the scorer's static LDS

@@ -715,37 +715,6 @@ size_t aux_size(char t, const unsigned char* v, const unsigned char* end) {
   throw BsqError(BSQ_ERR_FORMAT, std::string("unknown aux type ") + t);
 }
 
-int host_threads() {
-  const unsigned hc = std::thread::hardware_concurrency();
-  return (int)std::max(1u, std::min(hc == 0 ? 4u : hc, 16u));
-}
-
-// Runs fn(t, begin, end) over [0, n) split into contiguous ranges, one per thread.
-template <class F>
-void parallel_ranges(int64_t n, int threads, F fn) {
-  threads = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n));
-  std::vector<std::thread> pool;
-  std::vector<std::string> errors(threads);
-  std::vector<int> codes(threads, 0);
-  for (int t = 0; t < threads; t++) {
-    const int64_t b = n * t / threads, e = n * (t + 1) / threads;
-    pool.emplace_back([&, t, b, e] {
-      try {
-        fn(t, b, e);
-      } catch (const BsqError& ex) {
-        codes[t] = ex.code;
-        errors[t] = ex.what();
-      } catch (const std::exception& ex) {
-        codes[t] = BSQ_ERR_FORMAT;
-        errors[t] = ex.what();
-      }
-    });
-  }
-  for (auto& th : pool) th.join();
-  for (int t = 0; t < threads; t++)
-    if (codes[t]) throw BsqError(codes[t], errors[t]);
-}
-
 std::vector<unsigned char> read_file_bytes(const char* path) {
   FILE* f = fopen(path, "rb");
   if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not open ") + path);

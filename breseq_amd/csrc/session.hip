@@ -79,6 +79,8 @@ struct Session {
   std::vector<int32_t> h_pos_target;
   std::string h_ref;                  // padded reference (for host-side formatting)
   std::vector<DpRead> dp_reads;       // discordant-pair candidates, file order
+  std::vector<int32_t> h_gpos;        // global first column of each packed read (sorted)
+  std::vector<int32_t> h_gend_max;    // running maximum of the packed reads' gend, in read order
   // device buffers
   DBuf<DRead> d_reads;
   DBuf<uint32_t> d_cigar;
@@ -102,7 +104,7 @@ struct Session {
   DBuf<IdRec> d_idrec;
   DBuf<uint32_t> d_ct_u, d_ct_raw, d_ct_beg;   // BAM2COV column counts (coverage table)
   DBuf<double> d_ct_red;
-  DBuf<int32_t> d_ct_misc;                    // [0] largest read end overlapping the region, [1] error bits
+  DBuf<int32_t> d_ct_misc;                    // error bits of the coverage-table kernels
   DBuf<uint8_t> d_rg;                         // @RG index of each read (255: none)
   DBuf<int32_t> d_ins;
   DBuf<RaOut> d_ra;
@@ -270,6 +272,12 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   }
   S.n_read_files = (int32_t)max_rfile + 1;
   S.n_dreads = (int64_t)dr.size();
+  S.h_gpos.resize(dr.size());
+  S.h_gend_max.resize(dr.size());
+  for (size_t i = 0; i < dr.size(); i++) {
+    S.h_gpos[i] = dr[i].gpos;
+    S.h_gend_max[i] = std::max(dr[i].gend, i ? S.h_gend_max[i - 1] : (int32_t)-1);
+  }
   dp_select(a, ref->n_targets, S.dp_reads);
   if (S.n_dreads >= ((int64_t)1 << 31)) throw BsqError(BSQ_ERR_LIMIT, "more than 2^31 alignments");
   S.aligned_bases = nab;
@@ -858,8 +866,9 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
 }
 
 // BAM2COV table of one region (coverage_output::table, coverage_output.cpp:190-283; rows of
-// pileup_callback, :307-503).  The counts of every column come from k_coverage_columns (one pass
-// for all reads, one more per read group with per_read_group); which positions get a row follows
+// pileup_callback, :307-503).  The counts come from k_coverage_columns (consecutive rows) or
+// k_coverage_rows (downsampled rows), computed only for the positions that print counts: one pass
+// for all reads, one more per read group with per_read_group.  Which positions get a row follows
 // the reference's traversal (pileup_base.cpp:118-210, 308-360):
 //   * a position is handled when it lies in the region and (position + start) is a multiple of the
 //     downsampling step floor(size / resolution) (at least 1);
@@ -868,6 +877,8 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
 //   * after it, only handled positions advance the traversal, so a handled position also prints
 //     zero rows for the unhandled positions since the previous one (from position 1 when no read
 //     overlaps the region); those rows count in the averages' denominator only.
+// So the table is: the handled positions up to that last column, with counts, then every position
+// after it up to the last handled one, as zero rows.
 static std::string coverage_table(Session& S, const bsq_coverage_table_args* a) {
   if (!a) throw BsqError(BSQ_ERR_ARG, "null argument");
   if (a->target < 0 || a->target >= S.n_targets) throw BsqError(BSQ_ERR_ARG, "target id out of range");
@@ -879,59 +890,78 @@ static std::string coverage_table(Session& S, const bsq_coverage_table_args* a) 
   }
   if (start < 1 || end > L || start > end) throw BsqError(BSQ_ERR_ARG, "region outside its reference sequence");
   const int64_t off = S.tgt_off[(size_t)a->target];
-  const int64_t n = end - start + 1;
   const int n_rg = std::max(1, a->n_read_groups);
   const int passes = a->per_read_group ? 1 + n_rg : 1;
-  S.d_ct_u.alloc((size_t)(2 * n));
-  S.d_ct_raw.alloc((size_t)(2 * n));
-  S.d_ct_beg.alloc((size_t)(2 * n));
-  S.d_ct_red.alloc((size_t)(2 * n));
-  S.d_ct_misc.alloc(2);
+  int64_t ds = 1;
+  if (a->resolution != 0) {
+    ds = (int64_t)std::floor((double)(uint32_t)(end - start + 1) / (double)a->resolution);
+    if (ds < 1) ds = 1;
+  }
+  // the last column covered by a read that overlaps the region (sam_itr_queryi(start-1, end)):
+  // reads are sorted by first column, so it is the running maximum of gend over the reads that
+  // begin at or before the region's last column, when that maximum reaches into the region
+  int64_t last_pile = 0;   // 1-based; 0: no read overlaps the region
+  {
+    const int32_t g0 = (int32_t)(off + start - 1), g1 = (int32_t)(off + end - 1);
+    const size_t hi = (size_t)(std::upper_bound(S.h_gpos.begin(), S.h_gpos.end(), g1) - S.h_gpos.begin());
+    if (hi > 0 && S.h_gend_max[hi - 1] > g0) last_pile = (int64_t)S.h_gend_max[hi - 1] - off;
+  }
+  // handled positions: (position + start) a multiple of the step
+  const int64_t rem = (2 * start) % ds;
+  const int64_t first = start + (rem ? ds - rem : 0);
+  const int64_t row_end = std::min(end, last_pile);
+  const int64_t n = row_end >= first ? (row_end - first) / ds + 1 : 0;   // rows with counts
+  const int64_t tail_end = end >= first ? first + (end - first) / ds * ds : 0;
+  const int64_t z0 = std::max<int64_t>(last_pile, 0) + 1;                // zero rows z0..tail_end
+  const int64_t nz = tail_end >= z0 ? tail_end - z0 + 1 : 0;
   std::vector<std::vector<uint32_t>> hu(passes), hraw(passes), hbeg(passes);
   std::vector<std::vector<double>> hred(passes);
-  const int32_t misc[2] = {-1, 0};
-  int32_t last_end = -1;
-  for (int k = 0; k < passes; k++) {
-    CovTabParams C;
-    C.reads = S.d_reads.p;
-    C.rg = S.d_rg.p;
-    C.cigar = S.d_cigar.p;
-    C.win_rb = S.d_win_rb.p;
-    C.win_re = S.d_win_re.p;
-    C.g0 = (int32_t)(off + start - 1);
-    C.g1 = (int32_t)(off + end - 1);
-    C.rg_filter = k - 1;
-    C.n_rg = a->n_read_groups;
-    C.uniq = S.d_ct_u.p;
-    C.red = S.d_ct_red.p;
-    C.raw = S.d_ct_raw.p;
-    C.begin = S.d_ct_beg.p;
-    C.last_end = S.d_ct_misc.p;
-    C.err = (uint32_t*)(S.d_ct_misc.p + 1);
-    HIP_OK(hipMemcpyAsync(S.d_ct_misc.p, misc, sizeof(misc), hipMemcpyHostToDevice, S.stream));
-    const int64_t nwin = C.g1 / 64 - C.g0 / 64 + 1;
-    const int blocks = (int)std::min<int64_t>((nwin + 3) / 4, 4096);
-    hipLaunchKernelGGL(k_coverage_columns, dim3(blocks), dim3(256), 0, S.stream, C);
-    HIP_OK(hipGetLastError());
-    hu[k].resize((size_t)(2 * n));
-    hraw[k].resize((size_t)(2 * n));
-    hbeg[k].resize((size_t)(2 * n));
-    hred[k].resize((size_t)(2 * n));
-    HIP_OK(hipMemcpyAsync(hu[k].data(), S.d_ct_u.p, (size_t)(2 * n) * 4, hipMemcpyDeviceToHost, S.stream));
-    HIP_OK(hipMemcpyAsync(hraw[k].data(), S.d_ct_raw.p, (size_t)(2 * n) * 4, hipMemcpyDeviceToHost, S.stream));
-    HIP_OK(hipMemcpyAsync(hbeg[k].data(), S.d_ct_beg.p, (size_t)(2 * n) * 4, hipMemcpyDeviceToHost, S.stream));
-    HIP_OK(hipMemcpyAsync(hred[k].data(), S.d_ct_red.p, (size_t)(2 * n) * 8, hipMemcpyDeviceToHost, S.stream));
-    int32_t got[2];
-    HIP_OK(hipMemcpyAsync(got, S.d_ct_misc.p, sizeof(got), hipMemcpyDeviceToHost, S.stream));
+  if (n > 0) {
+    S.d_ct_u.alloc((size_t)(2 * n));
+    S.d_ct_raw.alloc((size_t)(2 * n));
+    S.d_ct_beg.alloc((size_t)(2 * n));
+    S.d_ct_red.alloc((size_t)(2 * n));
+    S.d_ct_misc.alloc(1);
+    HIP_OK(hipMemsetAsync(S.d_ct_misc.p, 0, sizeof(int32_t), S.stream));
+    for (int k = 0; k < passes; k++) {
+      CovTabParams C;
+      C.reads = S.d_reads.p;
+      C.rg = S.d_rg.p;
+      C.cigar = S.d_cigar.p;
+      C.win_rb = S.d_win_rb.p;
+      C.win_re = S.d_win_re.p;
+      C.h0 = (int32_t)(off + first - 1);
+      C.step = (int32_t)ds;
+      C.n = n;
+      C.rg_filter = k - 1;
+      C.n_rg = a->n_read_groups;
+      C.uniq = S.d_ct_u.p;
+      C.red = S.d_ct_red.p;
+      C.raw = S.d_ct_raw.p;
+      C.begin = S.d_ct_beg.p;
+      C.err = (uint32_t*)S.d_ct_misc.p;
+      if (ds == 1) {
+        const int64_t nwin = (C.h0 + n - 1) / 64 - C.h0 / 64 + 1;
+        hipLaunchKernelGGL(k_coverage_columns, dim3((unsigned)std::min<int64_t>((nwin + 3) / 4, 4096)), dim3(256), 0,
+                           S.stream, C);
+      } else {
+        hipLaunchKernelGGL(k_coverage_rows, dim3((unsigned)std::min<int64_t>((n + 3) / 4, 4096)), dim3(256), 0,
+                           S.stream, C);
+      }
+      HIP_OK(hipGetLastError());
+      hu[k].resize((size_t)(2 * n));
+      hraw[k].resize((size_t)(2 * n));
+      hbeg[k].resize((size_t)(2 * n));
+      hred[k].resize((size_t)(2 * n));
+      HIP_OK(hipMemcpyAsync(hu[k].data(), S.d_ct_u.p, (size_t)(2 * n) * 4, hipMemcpyDeviceToHost, S.stream));
+      HIP_OK(hipMemcpyAsync(hraw[k].data(), S.d_ct_raw.p, (size_t)(2 * n) * 4, hipMemcpyDeviceToHost, S.stream));
+      HIP_OK(hipMemcpyAsync(hbeg[k].data(), S.d_ct_beg.p, (size_t)(2 * n) * 4, hipMemcpyDeviceToHost, S.stream));
+      HIP_OK(hipMemcpyAsync(hred[k].data(), S.d_ct_red.p, (size_t)(2 * n) * 8, hipMemcpyDeviceToHost, S.stream));
+    }
+    int32_t got = 0;
+    HIP_OK(hipMemcpyAsync(&got, S.d_ct_misc.p, sizeof(got), hipMemcpyDeviceToHost, S.stream));
     HIP_OK(hipStreamSynchronize(S.stream));
-    raise_err_bits((uint32_t)got[1]);
-    if (k == 0) last_end = got[0];   // all reads: the last column the region's reads cover
-  }
-  const int64_t last_pile = (last_end >= 0) ? (int64_t)last_end - off : 0;   // 1-based last covered column
-  uint32_t ds = 1;
-  if (a->resolution != 0) {
-    ds = (uint32_t)std::floor((double)(uint32_t)n / (double)a->resolution);
-    if (ds < 1) ds = 1;
+    raise_err_bits((uint32_t)got);
   }
   const char d = a->csv ? ',' : '\t';
   static const char* const kCols8[8] = {"unique_top_cov", "unique_bot_cov", "redundant_top_cov", "redundant_bot_cov",
@@ -940,12 +970,11 @@ static std::string coverage_table(Session& S, const bsq_coverage_table_args* a) 
   static const char* const kCols3[3] = {"unique_cov", "redundant_cov", "total_cov"};
   const int ncols = a->total_only ? 3 : 8;
   const char* const* cols = a->total_only ? kCols3 : kCols8;
-  std::string out;
-  out.reserve((size_t)(n / ds + 16) * (size_t)(24 + 8 * ncols * passes));
-  auto put_u = [&](uint64_t v) {
+  auto put_u = [](std::string& o, uint64_t v) {
     char b[24];
-    out.append(b, std::to_chars(b, b + sizeof(b), v).ptr);
+    o.append(b, std::to_chars(b, b + sizeof(b), v).ptr);
   };
+  std::string out;
   out += "position";
   out += d;
   out += "ref_base";
@@ -957,85 +986,97 @@ static std::string coverage_table(Session& S, const bsq_coverage_table_args* a) 
     for (int c = 0; c < ncols; c++) {
       out += d;
       out += "RG-";
-      put_u((uint64_t)g);
+      put_u(out, (uint64_t)g);
       out += '_';
       out += cols[c];
     }
   out += '\n';
   const char* ref = S.h_ref.data() + off;   // reference bases of the target
+  // averages: per-row sums in row order, as the reference accumulates them; zero rows add nothing
   double avg_u = 0.0, avg_r = 0.0, avg_t = 0.0;
   std::vector<double> rg_u((size_t)n_rg, 0.0), rg_r((size_t)n_rg, 0.0), rg_t((size_t)n_rg, 0.0);
-  uint32_t n_positions = 0;
-  auto zero_row = [&](int64_t pos) {
-    put_u((uint64_t)pos);
-    out += d;
-    out += ref[pos - 1];
-    for (int c = 0; c < ncols * passes; c++) {
-      out += d;
-      out += '0';
+  for (int k = 0; k < passes; k++) {
+    double su = 0.0, sr = 0.0, st = 0.0;
+    for (int64_t j = 0; j < n; j++) {
+      const uint32_t u = hu[k][(size_t)j] + hu[k][(size_t)(n + j)];
+      const double r0 = hred[k][(size_t)j], r1 = hred[k][(size_t)(n + j)];
+      su += (double)u;
+      sr += r0 + r1;
+      st += (double)u + r0 + r1;
     }
-    out += '\n';
-    n_positions++;
-  };
-  auto row = [&](int64_t pos) {
-    const size_t i = (size_t)(pos - start), j = (size_t)(n + pos - start);
-    put_u((uint64_t)pos);
-    out += d;
-    out += ref[pos - 1];
-    for (int k = 0; k < passes; k++) {
-      const uint32_t u0 = hu[k][i], u1 = hu[k][j];
-      const double r0 = hred[k][i], r1 = hred[k][j];
-      // the sums in the reference's expression order: (u0 + u1) unsigned, then + r0, + r1
-      const double tot = (double)(u0 + u1) + r0 + r1;
-      if (a->total_only) {
-        out += d;
-        put_u((uint64_t)(uint32_t)(u0 + u1));
-        out += d;
-        out += fmt_g(r0 + r1);
-        out += d;
-        out += fmt_g(tot);
-      } else {
-        out += d;
-        put_u(u0);
-        out += d;
-        put_u(u1);
-        out += d;
-        out += fmt_g(r0);
-        out += d;
-        out += fmt_g(r1);
-        out += d;
-        put_u(hraw[k][i]);
-        out += d;
-        put_u(hraw[k][j]);
-        out += d;
-        put_u(hbeg[k][i]);
-        out += d;
-        put_u(hbeg[k][j]);
-      }
-      if (k == 0) {
-        avg_u += (double)(uint32_t)(u0 + u1);
-        avg_r += r0 + r1;
-        avg_t += tot;
-      } else {
-        rg_u[(size_t)(k - 1)] += (double)(uint32_t)(u0 + u1);
-        rg_r[(size_t)(k - 1)] += r0 + r1;
-        rg_t[(size_t)(k - 1)] += tot;
-      }
+    if (k == 0) {
+      avg_u = su;
+      avg_r = sr;
+      avg_t = st;
+    } else {
+      rg_u[(size_t)(k - 1)] = su;
+      rg_r[(size_t)(k - 1)] = sr;
+      rg_t[(size_t)(k - 1)] = st;
     }
-    out += '\n';
-    n_positions++;
-  };
-  auto handled = [&](int64_t pos) { return pos >= start && pos <= end && ((uint64_t)(pos + start) % ds) == 0; };
-  const int64_t covered_end = std::min<int64_t>(end, last_pile);
-  for (int64_t pos = start; pos <= covered_end; pos++)
-    if (handled(pos)) row(pos);
-  int64_t last = std::max<int64_t>(last_pile, 0);
-  for (int64_t pos = std::max<int64_t>(last_pile + 1, 1); pos <= end; pos++) {
-    if (!handled(pos)) continue;
-    for (int64_t g = last + 1; g < pos; g++) zero_row(g);
-    row(pos);
-    last = pos;
   }
+  // rows: n with counts at first + j*ds, then zero rows z0..tail_end; formatted in parallel chunks
+  const int64_t n_rows = n + nz;
+  const int threads = (int)std::min<int64_t>(host_threads(), std::max<int64_t>(1, n_rows / 4096));
+  std::vector<std::string> parts((size_t)threads);
+  parallel_ranges(n_rows, threads, [&](int t, int64_t i0, int64_t i1) {
+    std::string& o = parts[(size_t)t];
+    o.reserve((size_t)(i1 - i0) * (size_t)(12 + 4 * ncols * passes));
+    for (int64_t i = i0; i < i1; i++) {
+      if (i >= n) {
+        const int64_t pos = z0 + (i - n);
+        put_u(o, (uint64_t)pos);
+        o += d;
+        o += ref[pos - 1];
+        for (int c = 0; c < ncols * passes; c++) {
+          o += d;
+          o += '0';
+        }
+        o += '\n';
+        continue;
+      }
+      const int64_t pos = first + i * ds;
+      const size_t jt = (size_t)i, jb = (size_t)(n + i);
+      put_u(o, (uint64_t)pos);
+      o += d;
+      o += ref[pos - 1];
+      for (int k = 0; k < passes; k++) {
+        const uint32_t u0 = hu[k][jt], u1 = hu[k][jb];
+        const double r0 = hred[k][jt], r1 = hred[k][jb];
+        if (a->total_only) {
+          // the reference's expression order: (u0 + u1) unsigned, then + r0, + r1
+          o += d;
+          put_u(o, (uint64_t)(uint32_t)(u0 + u1));
+          o += d;
+          o += fmt_g(r0 + r1);
+          o += d;
+          o += fmt_g((double)(uint32_t)(u0 + u1) + r0 + r1);
+        } else {
+          o += d;
+          put_u(o, u0);
+          o += d;
+          put_u(o, u1);
+          o += d;
+          o += fmt_g(r0);
+          o += d;
+          o += fmt_g(r1);
+          o += d;
+          put_u(o, hraw[k][jt]);
+          o += d;
+          put_u(o, hraw[k][jb]);
+          o += d;
+          put_u(o, hbeg[k][jt]);
+          o += d;
+          put_u(o, hbeg[k][jb]);
+        }
+      }
+      o += '\n';
+    }
+  });
+  size_t total = out.size() + 512 + 160 * (size_t)n_rg;
+  for (const std::string& p : parts) total += p.size();
+  out.reserve(total);
+  for (const std::string& p : parts) out += p;
+  const uint32_t n_positions = (uint32_t)n_rows;
   auto foot = [&](const std::string& name, const std::string& value) {
     out += '#';
     out += d;
@@ -1050,8 +1091,7 @@ static std::string coverage_table(Session& S, const bsq_coverage_table_args* a) 
   foot("region_average_cov", fmt_g(avg_t / (double)n_positions));
   {
     std::string v;
-    char b[24];
-    v.assign(b, std::to_chars(b, b + sizeof(b), (uint64_t)n_positions).ptr);
+    put_u(v, (uint64_t)n_positions);
     foot("number_of_positions", v);
   }
   for (int g = 0; a->per_read_group && g < n_rg; g++) {

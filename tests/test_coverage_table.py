@@ -5,6 +5,7 @@ import ctypes as C
 
 import pytest
 
+import bam2cov_golden
 import oracle_lib as O
 from aln_builder import Built, BuiltSample
 from breseq_amd import _lib as L
@@ -118,3 +119,13 @@ def test_region_checks():
         table(start=0, end=11)
     with pytest.raises(L.BsqError):
         table(start=5, end=4)
+
+
+@pytest.mark.parametrize("case", sorted(bam2cov_golden.CASES))
+def test_reference_golden_tables(case):
+    # breseq's own BAM2COV tables, over alignments rebuilt from their coverage columns
+    refs, reads, n_rg, kw, want = bam2cov_golden.CASES[case]()
+    b = Built(refs, reads)
+    s = BuiltSample(b, n_read_groups=n_rg)
+    got = O.coverage_table(C.pointer(b.ref), C.pointer(b.aln), api.coverage_table_args(s, **kw))
+    assert got == want

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import breseq_amd as B
+import bam2cov_golden
 import oracle_lib as O
 from aln_builder import Built, BuiltSample, random_alignments
 from test_coverage_table import READS, REF
@@ -87,3 +88,12 @@ def test_simulated_genome():
     for kw in [dict(), dict(resolution=0, start=1000, end=3000), dict(start=59000, end=60000, resolution=0)]:
         got, want = both(sess, s, **kw)
         assert got == want, kw
+
+
+@pytest.mark.parametrize("case", sorted(bam2cov_golden.CASES))
+def test_reference_golden_tables(case):
+    say("[gpu] coverage table, breseq golden", case)
+    refs, reads, n_rg, kw, want = bam2cov_golden.CASES[case]()
+    s = BuiltSample(Built(refs, reads), n_read_groups=n_rg)
+    sess = B.Session().load(s)
+    assert sess.coverage_table(**kw) == want

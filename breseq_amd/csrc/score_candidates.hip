@@ -17,10 +17,14 @@
 //       phase 0: full fit | pure-genotype sums
 //       phase 1: null fit | profile at the ML frequency | profile at 0 | profile at 1, then
 //                Fisher and KS
-//       phase 2+: bisection rounds.  With one bound still open, the seven midpoints of the next
-//                 three bisection levels are evaluated together (two per wavefront), which is
-//                 exactly three sequential steps per round; with both open, three midpoints per
-//                 bound (two steps each).
+//       phase 2+: bisection rounds.  Each round evaluates the midpoints a sequential bisection
+//                 would visit along one predicted path: eight levels with one bound open, four
+//                 per bound with both open (two midpoints per wavefront).  The path follows the
+//                 crossing point of a model of the profile likelihood (below); the round then
+//                 keeps every level up to the first one where the real decision leaves the path.
+//                 A bound whose prediction goes wrong within its first two levels switches to
+//                 full trees (seven midpoints = three levels, or three = two levels).  Only
+//                 midpoints the sequential bisection visits are used, so the bounds are its own.
 //   * Fisher terms are evaluated across lanes and added in order by one lane; the exact KS lattice
 //     is filled one anti-diagonal at a time (each cell is one addition of two neighbours, so the
 //     order of the cells does not change any value).
@@ -39,10 +43,10 @@ constexpr int kTW = 2;           // EM tasks per wavefront
 constexpr int kTermCols = 64;    // reads per EM chunk (one per lane)
 constexpr int kTermRow = 65;     // LDS row stride of a term column: the summing lanes hit different banks
 constexpr int kKsDiag = 100;     // exact KS runs when m*n < 10000, so min(m, n) <= 99
-constexpr int kBisectPts = 7;    // midpoints of three bisection levels
+constexpr int kBisectPts = 8;    // midpoints evaluated per bound and round (one per task slot)
 
 // result slots of the phase tasks
-enum { kResFull = 0, kResNull = 1, kResPlmax = 2, kResP0 = 3, kResP1 = 4, kResBisect = 5 };   // + side*7 + node
+enum { kResFull = 0, kResNull = 1, kResPlmax = 2, kResP0 = 3, kResP1 = 4, kResBisect = 5 };   // + side*8 + node
 constexpr int kNumRes = kResBisect + 2 * kBisectPts;
 
 // EM task states
@@ -70,6 +74,7 @@ struct ScoreShared {
   double lps[5];
   double res[kNumRes];
   double pts[2][kBisectPts];       // bisection midpoints of the round, per bound
+  double vlo[2], vhi[2];           // profile values at the ends of each bound's interval
   double lo[2], hi[2];
   double target, lower, upper, ks_p, fisher_p;
   double ks_a[kKsDiag + 1], ks_b[kKsDiag + 1];
@@ -83,6 +88,8 @@ struct ScoreShared {
   int32_t iter[2];
   int32_t active[2];
   int32_t depth[2];                // bisection levels evaluated this round, per bound
+  int32_t path[2];                 // this round's midpoints follow one predicted path (else: a tree)
+  int32_t tree_only[2];            // a predicted path went wrong early: trees from then on
   int32_t emitted;
   int32_t done;
 };
@@ -516,6 +523,55 @@ __device__ __noinline__ void pure_genotype_call(const double* lps, uint32_t n, i
   pure_genotype(l, n, *best, *score);
 }
 
+// Relative entropy KL(fh || f) of two-allele frequencies: the drop of a binomial log likelihood
+// from its maximum at fh, per read and in natural-log units.
+__device__ double kl_two(double fh, double f) {
+  double t = 0.0;
+  if (fh > 0.0) t += fh * log(fh / f);
+  if (fh < 1.0) t += (1.0 - fh) * log((1.0 - fh) / (1.0 - f));
+  return t;
+}
+
+// Predicted crossing point of the profile likelihood with the target inside [lo, hi] (bound
+// `side`: 0 lower, 1 upper), used only to choose which midpoints to evaluate.  Model:
+// profile(f) = vmax - A * KL(fh || f) + c, fitted to the interval ends whose values are known and
+// interior (two ends: A and c; one end: A; none: A = n / ln 10, the binomial curvature of n reads).
+__device__ __noinline__ double predict_crossing(int side, double lo, double hi, double vlo, double vhi, double fh,
+                                                double vmax, double target, int n) {
+  double xs[2], vs[2];
+  int k = 0;
+  if (lo != fh && lo > 0.0 && lo < 1.0 && isfinite(vlo)) {
+    xs[k] = lo;
+    vs[k++] = vlo;
+  }
+  if (hi != fh && hi > 0.0 && hi < 1.0 && isfinite(vhi)) {
+    xs[k] = hi;
+    vs[k++] = vhi;
+  }
+  double A = (double)n / 2.302585092994046, c = 0.0;
+  if (k == 2) {
+    const double d0 = kl_two(fh, xs[0]), d1 = kl_two(fh, xs[1]);
+    if (d1 != d0) {
+      A = (vs[0] - vs[1]) / (d1 - d0);
+      c = vs[0] + A * d0 - vmax;
+    }
+  } else if (k == 1) {
+    const double d = kl_two(fh, xs[0]);
+    if (d > 0.0) A = (vmax - vs[0]) / d;
+  }
+  const double want = (vmax - target + c) / A;
+  if (!(A > 0.0) || !isfinite(want)) return 0.5 * (lo + hi);
+  // KL grows away from fh: falls across the lower interval, rises across the upper one
+  double a = lo, b = hi;
+  for (int it = 0; it < 60; it++) {
+    const double m = 0.5 * (a + b);
+    const bool beyond = kl_two(fh, m) > want;   // m lies outside the crossing (farther from fh)
+    if ((side == 0) == beyond) a = m;
+    else b = m;
+  }
+  return 0.5 * (a + b);
+}
+
 // allele_model::major_index / next_index (identify_mutations.cpp:3061-3086)
 __device__ __forceinline__ int major_index(const double f[5], uint32_t n) {
   if (n == 0) return 5;
@@ -601,6 +657,8 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
       SS.emitted = 0;
       SS.active[0] = SS.active[1] = 0;
       SS.depth[0] = SS.depth[1] = 0;
+      SS.path[0] = SS.path[1] = 0;
+      SS.tree_only[0] = SS.tree_only[1] = 0;
       SS.lower = 0.0;
       SS.upper = 1.0;
       SS.ks_p = 1.0;
@@ -634,17 +692,14 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
             else start(2, 31u, variant, 1.0, kResP1);
           }
         } else {
-          // one bound open: its seven midpoints, two per wavefront (nodes 2w, 2w+1);
-          // both open: three midpoints per bound, the lower bound's on wavefronts 0-1 and the
-          // upper bound's on wavefronts 2-3
+          // one bound open: its midpoints on nodes 2w, 2w+1; both open: the lower bound's on
+          // wavefronts 0-1 and the upper bound's on wavefronts 2-3, nodes 2(w%2), 2(w%2)+1
           const bool a0 = SS.active[0] != 0, a1 = SS.active[1] != 0;
-          if (a0 && a1) {
-            const int side = wave / 2;
-            const int n0 = (wave % 2 == 0) ? 0 : 2, n1 = (wave % 2 == 0) ? 2 : 3;
-            for (int node = n0; node < n1; node++) start(2, 31u, variant, SS.pts[side][node], kResBisect + side * kBisectPts + node);
-          } else if (a0 || a1) {
-            const int side = a0 ? 0 : 1;
-            for (int node = 2 * wave; node < min(2 * wave + 2, kBisectPts); node++)
+          if (a0 || a1) {
+            const int side = (a0 && a1) ? wave / 2 : (a0 ? 0 : 1);
+            const int n0 = (a0 && a1) ? 2 * (wave % 2) : 2 * wave;
+            const int npts = SS.path[side] ? SS.depth[side] : (1 << SS.depth[side]) - 1;
+            for (int node = n0; node < min(n0 + 2, npts); node++)
               start(2, 31u, variant, SS.pts[side][node], kResBisect + side * kBisectPts + node);
           }
         }
@@ -732,6 +787,8 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
             if (SS.res[kResP0] < target) {
               SS.lo[0] = 0.0;
               SS.hi[0] = fh;
+              SS.vlo[0] = SS.res[kResP0];
+              SS.vhi[0] = SS.res[kResPlmax];
               SS.iter[0] = 0;
               SS.active[0] = (fh - 0.0) > tol ? 1 : 0;
               SS.lower = fh;
@@ -739,6 +796,8 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
             if (SS.res[kResP1] < target) {
               SS.lo[1] = fh;
               SS.hi[1] = 1.0;
+              SS.vlo[1] = SS.res[kResPlmax];
+              SS.vhi[1] = SS.res[kResP1];
               SS.iter[1] = 0;
               SS.active[1] = (1.0 - fh) > tol ? 1 : 0;
               SS.upper = fh;
@@ -750,36 +809,71 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
           for (int side = 0; side < 2; side++) {
             if (!SS.active[side]) continue;
             const double* r = &SS.res[kResBisect + side * kBisectPts];
+            const bool path = SS.path[side] != 0;
             double l = SS.lo[side], h = SS.hi[side];
+            double vl = SS.vlo[side], vh = SS.vhi[side];
             int i = SS.iter[side];
-            int node = 0;
+            int node = 0, advanced = 0;
             for (int level = 0; level < SS.depth[side]; level++) {
               if (level > 0 && !(i < 40 && (h - l) > tol)) break;
-              const double m = 0.5 * (l + h);   // == SS.pts[side][node]
+              const double m = 0.5 * (l + h);
+              // a path's midpoint at this level is the sequential one only while every earlier
+              // decision followed the path (off it, the interval and so its midpoint differ)
+              if (path && m != SS.pts[side][level]) break;
               // lower bound: a profile >= target moves hi down; upper bound: moves lo up
-              const bool ge = r[node] >= SS.target;
+              const double v = r[node];
+              const bool ge = v >= SS.target;
               const bool left = (side == 0) ? ge : !ge;   // the interval kept is [l, m]
-              if (left) h = m;
-              else l = m;
+              if (left) {
+                h = m;
+                vh = v;
+              } else {
+                l = m;
+                vl = v;
+              }
               i++;
-              node = 2 * node + (left ? 1 : 2);
+              advanced++;
+              node = path ? level + 1 : 2 * node + (left ? 1 : 2);
             }
+            if (path && advanced < 3) SS.tree_only[side] = 1;
             SS.lo[side] = l;
             SS.hi[side] = h;
+            SS.vlo[side] = vl;
+            SS.vhi[side] = vh;
             SS.iter[side] = i;
             SS.active[side] = (i < 40 && (h - l) > tol) ? 1 : 0;
             if (side == 0) SS.lower = h;
             else SS.upper = l;
           }
         }
-        // midpoints of the next round: breadth-first over the levels evaluated, each the
-        // midpoint of the interval a sequential bisection would hold there
+        // midpoints of the next round, each the midpoint of the interval a sequential bisection
+        // would hold there: along the predicted path, or breadth-first over a full tree
         const bool both = SS.active[0] && SS.active[1];
         for (int side = 0; side < 2; side++) {
-          SS.depth[side] = SS.active[side] ? (both ? 2 : 3) : 0;
-          if (!SS.active[side]) continue;
+          if (!SS.active[side]) {
+            SS.depth[side] = 0;
+            continue;
+          }
           const double lo = SS.lo[side], hi = SS.hi[side];
           double* pt = SS.pts[side];
+          if (!SS.tree_only[side]) {
+            const int depth = both ? kBisectPts / 2 : kBisectPts;
+            const double fs = predict_crossing(side, lo, hi, SS.vlo[side], SS.vhi[side], SS.full_f[variant],
+                                               SS.res[kResPlmax], SS.target, n);
+            double a = lo, b = hi;
+            for (int d = 0; d < depth; d++) {
+              const double m = 0.5 * (a + b);
+              pt[d] = m;
+              const bool left = (side == 0) ? (fs <= m) : (fs < m);
+              if (left) b = m;
+              else a = m;
+            }
+            SS.path[side] = 1;
+            SS.depth[side] = depth;
+            continue;
+          }
+          SS.path[side] = 0;
+          SS.depth[side] = both ? 2 : 3;
           const double m0 = 0.5 * (lo + hi);
           const double m1 = 0.5 * (lo + m0), m2 = 0.5 * (m0 + hi);
           pt[0] = m0;

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <future>
 #include <map>
 #include <memory>
 #include <string>
@@ -60,6 +61,7 @@ static const int kWave = 64;
 struct Session {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;   // side stream: work that runs beside the exact scorer
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evk[8] = {};
   bsq_timing timing{};
@@ -117,6 +119,7 @@ struct Session {
     for (hipEvent_t e : evk)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+    if (stream2) (void)hipStreamDestroy(stream2);
   }
 };
 
@@ -731,17 +734,74 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   hipLaunchKernelGGL(k_fix_columns, dim3(256), dim3(64), key_bytes, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[3], S.stream));
+  // MC/UN runs read only the final column flags and coverage, so they run on the side stream beside
+  // the insert slots and the scorer, and the host turns them into entries while the scorer runs
+  HIP_OK(hipStreamWaitEvent(S.stream2, S.evk[3], 0));
+  HIP_OK(hipEventRecord(S.evk[4], S.stream2));
+  hipLaunchKernelGGL(k_runs, dim3((S.n_pos + 255) / 256), dim3(256), 0, S.stream2, R);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(S.evk[5], S.stream2));
+  int32_t n_runs_seen = 0;
+  HIP_OK(hipMemcpyAsync(&n_runs_seen, S.d_counters.p + 2, sizeof(int32_t), hipMemcpyDeviceToHost, S.stream2));
   hipLaunchKernelGGL(k_insert_slots, dim3(1024), dim3(64), 0, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[1], S.stream));
   hipLaunchKernelGGL(k_score_candidates, dim3(2048), dim3(kScoreWaves * 64), score_lds, S.stream, P);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(S.evk[2], S.stream));
-  hipLaunchKernelGGL(k_runs, dim3((S.n_pos + 255) / 256), dim3(256), 0, S.stream, R);
-  HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.ev1, S.stream));
   int32_t counters[8];
   HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
+  // host work that needs no scorer output, done while the scorer runs: the DP candidate regions
+  // (their own thread) and the MC/UN entries
+  std::vector<const char*> tn;
+  for (const std::string& n : S.names) tn.push_back(n.c_str());
+  std::future<std::string> dp_text = std::async(std::launch::async, [&] {
+    return dp_regions_csv(S.n_targets, tn.data(), S.lengths.data(), a->call_target, a->deletion_propagation_cutoff,
+                          a->dp, S.dp_reads);
+  });
+  std::vector<int32_t> order = target_order(S, a->call_target);
+  std::vector<int64_t> rank(S.n_targets, -1);
+  for (size_t i = 0; i < order.size(); i++) rank[order[i]] = (int64_t)i;
+  std::vector<GdEntry> run_entries;
+  HIP_OK(hipStreamSynchronize(S.stream2));
+  const double tr0 = now_ms();
+  if (n_runs_seen <= run_cap) {   // past the cap the call fails below, with the other limit checks
+    std::vector<RunRec> runs(n_runs_seen);
+    if (n_runs_seen)
+      HIP_OK(hipMemcpyAsync(runs.data(), S.d_runrec.p, n_runs_seen * sizeof(RunRec), hipMemcpyDeviceToHost, S.stream2));
+    HIP_OK(hipStreamSynchronize(S.stream2));
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    for (const RunRec& rr_ : runs) {
+      const int t = S.h_pos_target[rr_.s];
+      const int32_t base = S.tgt_off[t];
+      GdEntry e;
+      e.seq_id = S.names[t];
+      e.pos = (uint32_t)(rr_.s - base + 1);
+      e.end = (uint32_t)(rr_.e - base + 1);
+      e.order_target = rank[t];
+      e.order_pos = e.end + 1;
+      if (rr_.kind == 0) {
+        if (!rr_.seed) continue;   // seed never reached
+        const uint32_t red_start = (rr_.red_start >= 0) ? (uint32_t)(rr_.red_start - base + 1) : e.pos;
+        const uint32_t red_end = (rr_.red_end >= 0) ? (uint32_t)(rr_.red_end - base + 1) : e.end;
+        e.type = 1;
+        e.start_range = red_start - e.pos;
+        e.end_range = e.end - red_end;
+        e.fields = {{"left_inside_cov", fmt_double((double)rr_.li, 0, false)},
+                    {"left_outside_cov", fmt_double(rr_.lo < 0 ? nan : (double)rr_.lo, 0, false)},
+                    {"right_inside_cov", fmt_double((double)rr_.ri, 0, false)},
+                    {"right_outside_cov", fmt_double(rr_.ro < 0 ? nan : (double)rr_.ro, 0, false)}};
+        e.order_sub = 0;
+        out.n_mc++;
+      } else {
+        e.type = 2;
+        e.order_sub = 1;
+        out.n_un++;
+      }
+      run_entries.push_back(std::move(e));
+    }
+  }
+  const double run_host_ms = now_ms() - tr0;
   HIP_OK(hipStreamSynchronize(S.stream));
   float ms = 0;
   HIP_OK(hipEventElapsedTime(&ms, S.ev0, S.ev1));
@@ -752,8 +812,8 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
     HIP_OK(hipEventElapsedTime(&k0, S.ev0, S.evk[0]));
     HIP_OK(hipEventElapsedTime(&kf, S.evk[0], S.evk[3]));
     HIP_OK(hipEventElapsedTime(&k1, S.evk[3], S.evk[1]));
-    HIP_OK(hipEventElapsedTime(&k2, S.evk[1], S.evk[2]));
-    HIP_OK(hipEventElapsedTime(&k3, S.evk[2], S.ev1));
+    HIP_OK(hipEventElapsedTime(&k2, S.evk[1], S.ev1));
+    HIP_OK(hipEventElapsedTime(&k3, S.evk[4], S.evk[5]));
     S.timing.kernel_detail_ms[1] = k0;
     S.timing.kernel_detail_ms[2] = k1;
     S.timing.kernel_detail_ms[3] = k2;
@@ -768,21 +828,16 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   if (counters[4] > fix_cap) throw BsqError(BSQ_ERR_LIMIT, "too many columns near a pure-genotype threshold");
   raise_err_bits(errbits);
   const int n_cand = counters[0];
-  const int n_runs = counters[2];
   S.timing.candidates = n_cand;
   const double t0 = now_ms();
   std::vector<RaOut> ra(n_cand);
-  std::vector<RunRec> runs(n_runs);
   if (n_cand) HIP_OK(hipMemcpyAsync(ra.data(), S.d_ra.p, n_cand * sizeof(RaOut), hipMemcpyDeviceToHost, S.stream));
-  if (n_runs) HIP_OK(hipMemcpyAsync(runs.data(), S.d_runrec.p, n_runs * sizeof(RunRec), hipMemcpyDeviceToHost, S.stream));
   HIP_OK(hipStreamSynchronize(S.stream));
   S.timing.d2h_ms = now_ms() - t0;
   const double t1 = now_ms();
-  // assemble entries
-  std::vector<int32_t> order = target_order(S, a->call_target);
-  std::vector<int64_t> rank(S.n_targets, -1);
-  for (size_t i = 0; i < order.size(); i++) rank[order[i]] = (int64_t)i;
+  // assemble entries: RA, then the MC/UN runs, then whole-target MC of failed fits
   std::vector<GdEntry> entries;
+  entries.reserve((size_t)n_cand + run_entries.size() + order.size());
   for (const RaOut& r : ra) {
     if (!r.emitted) continue;
     const int t = S.h_pos_target[r.gpos];
@@ -795,36 +850,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
     entries.push_back(std::move(e));
     out.n_ra++;
   }
-  const double nan = std::numeric_limits<double>::quiet_NaN();
-  for (const RunRec& rr_ : runs) {
-    const int t = S.h_pos_target[rr_.s];
-    const int32_t base = S.tgt_off[t];
-    GdEntry e;
-    e.seq_id = S.names[t];
-    e.pos = (uint32_t)(rr_.s - base + 1);
-    e.end = (uint32_t)(rr_.e - base + 1);
-    e.order_target = rank[t];
-    e.order_pos = e.end + 1;
-    if (rr_.kind == 0) {
-      if (!rr_.seed) continue;   // seed never reached
-      const uint32_t red_start = (rr_.red_start >= 0) ? (uint32_t)(rr_.red_start - base + 1) : e.pos;
-      const uint32_t red_end = (rr_.red_end >= 0) ? (uint32_t)(rr_.red_end - base + 1) : e.end;
-      e.type = 1;
-      e.start_range = red_start - e.pos;
-      e.end_range = e.end - red_end;
-      e.fields = {{"left_inside_cov", fmt_double((double)rr_.li, 0, false)},
-                  {"left_outside_cov", fmt_double(rr_.lo < 0 ? nan : (double)rr_.lo, 0, false)},
-                  {"right_inside_cov", fmt_double((double)rr_.ri, 0, false)},
-                  {"right_outside_cov", fmt_double(rr_.ro < 0 ? nan : (double)rr_.ro, 0, false)}};
-      e.order_sub = 0;
-      out.n_mc++;
-    } else {
-      e.type = 2;
-      e.order_sub = 1;
-      out.n_un++;
-    }
-    entries.push_back(std::move(e));
-  }
+  for (GdEntry& e : run_entries) entries.push_back(std::move(e));
   if (!a->skip_missing_coverage_prediction) {
     for (int t : order) {
       if (a->deletion_propagation_cutoff[t] >= 0.0) continue;
@@ -850,17 +876,13 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
     if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + a->gd_file);
     f << out.text;
   }
-  {
-    std::vector<const char*> tn;
-    for (const std::string& n : S.names) tn.push_back(n.c_str());
-    out.dp_text = dp_regions_csv(S.n_targets, tn.data(), S.lengths.data(), a->call_target,
-                                 a->deletion_propagation_cutoff, a->dp, S.dp_reads);
-  }
+  out.dp_text = dp_text.get();
   if (a->dp_file) {
     std::ofstream f(a->dp_file);
     if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + a->dp_file);
     f << out.dp_text;
   }
+  S.timing.kernel_detail_ms[5] = run_host_ms;
   S.timing.host_post_ms = now_ms() - t1;
   S.timing.host_pre_ms = th1 - th0;
 }
@@ -1136,6 +1158,7 @@ int bsq_session_create(int32_t device, bsq_session** out, bsq_status* status) {
   s->s.device = device;
   HIP_OK(hipSetDevice(device));
   HIP_OK(hipStreamCreateWithFlags(&s->s.stream, hipStreamNonBlocking));
+  HIP_OK(hipStreamCreateWithFlags(&s->s.stream2, hipStreamNonBlocking));
   HIP_OK(hipEventCreate(&s->s.ev0));
   HIP_OK(hipEventCreate(&s->s.ev1));
   for (hipEvent_t& e : s->s.evk) HIP_OK(hipEventCreate(&e));

@@ -186,6 +186,7 @@ def main():
     achieved = dbytes / (dms / 1000.0) / 1e9
     host_ms = {k: sum(v) / len(v) for k, v in host.items()}
     host_ms["im_runs_host_beside_scorer"] = avg([d[5] for d in im_k])
+    host_ms["im_dp_regions_thread"] = avg([d[0] for d in im_k])
     out = {
         "metric": "aligned bases/sec through pileup scoring, 4.6Mb@100x; p50/p99 sample latency",
         "value": value,

@@ -15,15 +15,16 @@
 //   * The candidate's EM work is a chain of ~1000 passes (null fit, profile likelihoods,
 //     bisections), run as phases:
 //       phase 0: full fit | pure-genotype sums
-//       phase 1: null fit | profile at the ML frequency | profile at 0 | profile at 1, then
-//                Fisher and KS
+//       phase 1: null fit + a speculative bisection midpoint | profile at the ML frequency and
+//                at 1, then Fisher and KS | profile at 0 + a midpoint | two more midpoints
 //       phase 2+: bisection rounds.  Each round evaluates the midpoints a sequential bisection
 //                 would visit along one predicted path: eight levels with one bound open, four
 //                 per bound with both open (two midpoints per wavefront).  The path follows the
 //                 crossing point of a model of the profile likelihood (below); the round then
 //                 keeps every level up to the first one where the real decision leaves the path.
 //                 A bound whose prediction goes wrong within its first two levels switches to
-//                 full trees (seven midpoints = three levels, or three = two levels).  Only
+//                 full trees (seven midpoints = three levels, or three = two levels).  The first
+//                 four path levels (two per bound with both open) run in phase 1.  Only
 //                 midpoints the sequential bisection visits are used, so the bounds are its own.
 //   * Fisher terms are evaluated across lanes and added in order by one lane; the exact KS lattice
 //     is filled one anti-diagonal at a time (each cell is one addition of two neighbours, so the
@@ -90,6 +91,7 @@ struct ScoreShared {
   int32_t depth[2];                // bisection levels evaluated this round, per bound
   int32_t path[2];                 // this round's midpoints follow one predicted path (else: a tree)
   int32_t tree_only[2];            // a predicted path went wrong early: trees from then on
+  int32_t spec[2];                 // first path levels per bound evaluated beside phase 1
   int32_t emitted;
   int32_t done;
 };
@@ -686,10 +688,25 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
           if (wave == 0) start(1, 31u, -1, 0.0, kResFull);
         } else if (ph == 1) {
           if (variant != 5 && n > 0) {
-            if (wave == 0) start(1, 31u & ~(1u << variant), -1, 0.0, kResNull);
-            else if (wave == 1) start(2, 31u, variant, SS.full_f[variant], kResPlmax);
-            else if (wave == 2) start(2, 31u, variant, 0.0, kResP0);
-            else start(2, 31u, variant, 1.0, kResP1);
+            // the first path levels of the bounds (lower bound's first) fill the slots that the
+            // four phase-1 tasks leave idle; plmax and p1 end after a pass or two and share a wave
+            auto spec_start = [&](int j) {
+              const int side = j < SS.spec[0] ? 0 : 1, level = side ? j - SS.spec[0] : j;
+              if (level < SS.spec[side]) start(2, 31u, variant, SS.pts[side][level], kResBisect + side * kBisectPts + level);
+            };
+            if (wave == 0) {
+              start(1, 31u & ~(1u << variant), -1, 0.0, kResNull);
+              spec_start(0);
+            } else if (wave == 1) {
+              start(2, 31u, variant, SS.full_f[variant], kResPlmax);
+              start(2, 31u, variant, 1.0, kResP1);
+            } else if (wave == 2) {
+              start(2, 31u, variant, 0.0, kResP0);
+              spec_start(1);
+            } else {
+              spec_start(2);
+              spec_start(3);
+            }
           }
         } else {
           // one bound open: its midpoints on nodes 2w, 2w+1; both open: the lower bound's on
@@ -724,7 +741,7 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
         }
         if (lane < 5) SS.lps[lane] = acc;
       }
-      if (ph == 1 && wave == 3) {
+      if (ph == 1 && wave == 1) {
         // annotate_polymorphism_statistics (identify_mutations.cpp:3009-3033); used if emitted
         for (int i = lane; i < 256; i += 64) (&SS.qh[0][0])[i] = 0;
         wave_sync();
@@ -772,7 +789,32 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
         major = major_index(full_f, (uint32_t)n);
         minor = next_index(full_f, (uint32_t)n, major);
         variant = next_index(full_f, (uint32_t)n, ref_idx);
-        continue;   // phase 1 reads the derived calls; no shared state changes here
+        if (tid == 0) {
+          // the first levels of each bound's predicted path, evaluated speculatively in phase 1:
+          // four for a bound whose interval alone is wide enough to be searched, else two each
+          // (the model needs only the ML frequency and n until profile values are known)
+          SS.spec[0] = SS.spec[1] = 0;
+          if (variant != 5 && n > 0) {
+            const double fh = SS.full_f[variant];
+            const bool open0 = fh > tol, open1 = (1.0 - fh) > tol;
+            SS.spec[0] = open0 ? (open1 ? 2 : 4) : 0;
+            SS.spec[1] = open1 ? (open0 ? 2 : 4) : 0;
+            for (int side = 0; side < 2; side++) {
+              const double lo = side ? fh : 0.0, hi = side ? 1.0 : fh;
+              const double fs = predict_crossing(side, lo, hi, 0.0, 0.0, fh, kDrop, 0.0, n);
+              double a = lo, b = hi;
+              for (int d = 0; d < SS.spec[side]; d++) {
+                const double m = 0.5 * (a + b);
+                SS.pts[side][d] = m;
+                const bool left = (side == 0) ? (fs <= m) : (fs < m);
+                if (left) b = m;
+                else a = m;
+              }
+            }
+          }
+        }
+        __syncthreads();
+        continue;   // phase 1 reads the derived calls and the speculative midpoints
       }
       if (tid == 0) {
         if (ph == 1) {
@@ -803,7 +845,13 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
               SS.upper = fh;
             }
           }
-        } else {
+          // the speculative levels evaluated in phase 1 are walked like a round's path
+          for (int side = 0; side < 2; side++) {
+            SS.path[side] = 1;
+            SS.depth[side] = SS.spec[side];
+          }
+        }
+        {
           // walk each open bound down the evaluated levels: every level is one sequential
           // bisection step (interval halved at 0.5 * (lo + hi), ≤ 40 steps, stop at width ≤ tol)
           for (int side = 0; side < 2; side++) {
@@ -835,7 +883,7 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
               advanced++;
               node = path ? level + 1 : 2 * node + (left ? 1 : 2);
             }
-            if (path && advanced < 3) SS.tree_only[side] = 1;
+            if (ph >= 2 && path && advanced < 3) SS.tree_only[side] = 1;
             SS.lo[side] = l;
             SS.hi[side] = h;
             SS.vlo[side] = vl;

@@ -755,9 +755,13 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   // (their own thread) and the MC/UN entries
   std::vector<const char*> tn;
   for (const std::string& n : S.names) tn.push_back(n.c_str());
+  double dp_ms = 0.0;
   std::future<std::string> dp_text = std::async(std::launch::async, [&] {
-    return dp_regions_csv(S.n_targets, tn.data(), S.lengths.data(), a->call_target, a->deletion_propagation_cutoff,
-                          a->dp, S.dp_reads);
+    const double d0 = now_ms();
+    std::string t = dp_regions_csv(S.n_targets, tn.data(), S.lengths.data(), a->call_target,
+                                   a->deletion_propagation_cutoff, a->dp, S.dp_reads);
+    dp_ms = now_ms() - d0;
+    return t;
   });
   std::vector<int32_t> order = target_order(S, a->call_target);
   std::vector<int64_t> rank(S.n_targets, -1);
@@ -877,6 +881,7 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
     f << out.text;
   }
   out.dp_text = dp_text.get();
+  S.timing.kernel_detail_ms[0] = dp_ms;
   if (a->dp_file) {
     std::ofstream f(a->dp_file);
     if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + a->dp_file);

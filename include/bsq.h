@@ -269,11 +269,13 @@ typedef struct bsq_timing {
   int64_t candidates;      /* positions sent to the exact FP64 scorer             */
   int64_t n_reads;         /* mapped alignments in the device CSR                 */
   int64_t n_positions;     /* reference columns (incl. one pad per target)        */
-  /* per-kernel times of the last pass, HIP events on the session stream:
-   * [0] k_ec_reads (with the difference-array clear)  [1] k_identify_columns
-   * [2] k_insert_slots  [3] k_score_candidates  [4] k_runs
-   * [5] k_cov_tile_sums + k_cov_tile_scan + k_cov_hist  [6] k_fix_columns
-   * [7] number of columns k_fix_columns re-decided (a count, not a time) */
+  /* per-kernel times of the last pass, HIP events on the session streams:
+   * error_count: [0] k_ec_reads (with the difference-array clear)
+   *              [5] k_cov_tile_sums + k_cov_tile_scan + k_cov_hist
+   * identify:    [1] k_identify_columns  [2] k_insert_slots  [3] k_score_candidates  [4] k_runs
+   *              [6] k_fix_columns  [7] number of columns k_fix_columns re-decided (a count)
+   *              host work beside the scorer: [0] DP candidate regions (own thread)
+   *              [5] MC/UN entries */
   double kernel_detail_ms[8];
 } bsq_timing;
 

@@ -271,6 +271,26 @@ __device__ __noinline__ void task_start(LdsTasks* W, int t, int mode, uint32_t a
   W->state[t] = kTaskIter;
 }
 
+// The likelihood pass of task t for one chunk of reads: term column 0 gets log10 s_i + lpmax_i
+// (0 for s_i = 0), s_i = Sum_{allowed b} f_b r_i(b) in allele order.  Kept out of line: the
+// inlined log10 would set the register budget of every EM pass, and runs once per EM loop.
+__device__ __noinline__ void likelihood_terms(const Rows R, int n, int c0, LdsTasks* W, int t, double f0, double f1,
+                                              double f2, double f3, double f4, uint32_t al) {
+  const int lane = threadIdx.x & 63;
+  const int i = c0 + lane;
+  if (i >= n) return;
+  double r[6];
+  R.get(i, r);
+  const double f[5] = {f0, f1, f2, f3, f4};
+  double s = 0.0;
+#pragma unroll
+  for (int b = 0; b < 5; b++) {
+    const double x = f[b] * r[b];
+    if (al & (1u << b)) s += x;
+  }
+  W->term[t][0][lane] = (s > 0.0) ? log10(s) + r[5] : 0.0;
+}
+
 // Runs the started tasks of a wavefront to completion; each task's log10 likelihood goes to
 // res_out[its slot] and its final frequencies stay in W->f.  Per pass, for every running task:
 //   kTaskIter: sums[b] = Sum_i w_i(b), w_i(b) = f_b r_i(b) / s_i, s_i = Sum_{allowed b} f_b r_i(b)
@@ -302,6 +322,9 @@ __device__ __noinline__ void run_tasks(const Rows R, int n, LdsTasks* W, double 
     double acc = 0.0;
     for (int c0 = 0; c0 < n; c0 += kTermCols) {
       const int i = c0 + lane;
+#pragma unroll
+      for (int t = 0; t < kTW; t++)
+        if (st[t] == kTaskLast) likelihood_terms(R, n, c0, W, t, f[t][0], f[t][1], f[t][2], f[t][3], f[t][4], al[t]);
       if (i >= n) {
         // padding of the chunk's last partial group of 16
 #pragma unroll
@@ -315,7 +338,7 @@ __device__ __noinline__ void run_tasks(const Rows R, int n, LdsTasks* W, double 
         R.get(i, r);
 #pragma unroll
         for (int t = 0; t < kTW; t++) {
-          if (st[t] == kTaskIdle) continue;
+          if (st[t] != kTaskIter) continue;
           double x[5];
           double s = 0.0;
 #pragma unroll
@@ -324,10 +347,6 @@ __device__ __noinline__ void run_tasks(const Rows R, int n, LdsTasks* W, double 
             if (al[t] & (1u << b)) s += x[b];
           }
           LdsDouble* tm = &W->term[t][0][0];
-          if (st[t] == kTaskLast) {
-            tm[lane] = (s > 0.0) ? log10(s) + r[5] : 0.0;
-            continue;
-          }
           double q[5];
           if (s > 0.0) {
             // f_b r_b / s correctly rounded from one reciprocal (the SharedDiv step, with the
@@ -374,7 +393,9 @@ __device__ __noinline__ void run_tasks(const Rows R, int n, LdsTasks* W, double 
         bool upd = false;
         if (W->mode[my_t] == 1) {
           upd = (W->allowed[my_t] >> my_b) & 1u;
-          if (upd) fn = SharedDiv((double)n)(mine);
+          // one correctly rounded division: a reciprocal hoisted out of the pass loop would be
+          // spilled to scratch under this function's register budget
+          if (upd) fn = mine / (double)n;
         } else {
           const int v = W->v[my_t];
           const double x = W->x[my_t];

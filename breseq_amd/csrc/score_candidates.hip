@@ -550,8 +550,8 @@ __device__ __noinline__ void pure_genotype_call(const double* lps, uint32_t n, i
 // from its maximum at fh, per read and in natural-log units.
 __device__ double kl_two(double fh, double f) {
   double t = 0.0;
-  if (fh > 0.0) t += fh * log(fh / f);
-  if (fh < 1.0) t += (1.0 - fh) * log((1.0 - fh) / (1.0 - f));
+  if (fh > 0.0) t += fh * (log(fh) - log(f));
+  if (fh < 1.0) t += (1.0 - fh) * (log1p(-fh) - log1p(-f));
   return t;
 }
 
@@ -584,9 +584,10 @@ __device__ __noinline__ double predict_crossing(int side, double lo, double hi, 
   }
   const double want = (vmax - target + c) / A;
   if (!(A > 0.0) || !isfinite(want)) return 0.5 * (lo + hi);
-  // KL grows away from fh: falls across the lower interval, rises across the upper one
+  // KL grows away from fh: falls across the lower interval, rises across the upper one.  20 halvings
+  // place the crossing within 2^-20 of the interval, far finer than the eight levels a round uses.
   double a = lo, b = hi;
-  for (int it = 0; it < 60; it++) {
+  for (int it = 0; it < 20; it++) {
     const double m = 0.5 * (a + b);
     const bool beyond = kl_two(fh, m) > want;   // m lies outside the crossing (farther from fh)
     if ((side == 0) == beyond) a = m;

@@ -81,6 +81,11 @@ struct Session {
   std::vector<int32_t> h_pos_target;
   std::string h_ref;                  // padded reference (for host-side formatting)
   std::vector<DpRead> dp_reads;       // discordant-pair candidates, file order
+  // smallest capacities of identify's result lists, raised when a pass overflows one (the pass
+  // is then run again with room for every entry)
+  struct {
+    int32_t cand = 0, ins = 0, runs = 0, fix = 0;
+  } id_caps;
   std::vector<int32_t> h_gpos;        // global first column of each packed read (sorted)
   std::vector<int32_t> h_gend_max;    // running maximum of the packed reads' gend, in read order
   // device buffers
@@ -612,8 +617,12 @@ static void build_likelihood_tables(const Session& S, const bsq_identify_args* a
   }
 }
 
-static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& out, int screen_off) {
+// One identify pass.  Returns false, with the session's minimum capacities raised, when one of the
+// device result lists (candidates, insert columns, MC/UN runs, columns to re-decide) was too small;
+// the caller runs the pass again.
+static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_result& out, int screen_off) {
   if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
+  out.n_ra = out.n_mc = out.n_un = 0;
   const int Sm = a->read_set_max, Q = a->quality_max;
   if (Sm <= 0 || Q <= 0 || Sm > 255 || Q > 256) throw BsqError(BSQ_ERR_ARG, "covariate maxima out of range");
   if (a->n_bins != (int64_t)Sm * 25 * Q) throw BsqError(BSQ_ERR_ARG, "error table size does not match its covariates");
@@ -647,14 +656,14 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   S.d_col_flags.alloc(S.n_pos);
   S.d_col_unique.alloc(S.n_pos);
   S.d_col_total.alloc(S.n_pos);
-  const int cand_cap = screen_off ? S.n_pos + 1024 : std::max<int>(1 << 16, S.n_pos / 64);
-  const int ins_cap = std::max<int>(1 << 14, S.n_pos / 256);
-  const int run_cap = std::max<int>(1 << 16, S.n_pos / 16);
+  const int cand_cap = std::max<int>(screen_off ? S.n_pos + 1024 : std::max<int>(1 << 16, S.n_pos / 64), S.id_caps.cand);
+  const int ins_cap = std::max<int>(std::max<int>(1 << 14, S.n_pos / 256), S.id_caps.ins);
+  const int run_cap = std::max<int>(std::max<int>(1 << 16, S.n_pos / 16), S.id_caps.runs);
   S.d_cand.alloc(cand_cap);
   S.d_ins.alloc(ins_cap);
   S.d_ra.alloc(cand_cap);
   S.d_runrec.alloc(run_cap);
-  const int fix_cap = std::max<int>(1 << 16, S.n_pos / 256);
+  const int fix_cap = std::max<int>(std::max<int>(1 << 16, S.n_pos / 256), S.id_caps.fix);
   S.d_fix.alloc(fix_cap);
   S.d_idrec.alloc((size_t)std::max<int64_t>(1, S.n_dreads));
   ImParams P;
@@ -826,10 +835,21 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
     S.timing.kernel_detail_ms[7] = (double)counters[4];
   }
   const uint32_t errbits = (uint32_t)counters[3];
-  if (counters[0] > cand_cap) throw BsqError(BSQ_ERR_LIMIT, "too many candidate columns for the exact scorer");
-  if (counters[1] > ins_cap) throw BsqError(BSQ_ERR_LIMIT, "too many columns with insertions");
-  if (counters[2] > run_cap) throw BsqError(BSQ_ERR_LIMIT, "too many MC/UN intervals");
-  if (counters[4] > fix_cap) throw BsqError(BSQ_ERR_LIMIT, "too many columns near a pure-genotype threshold");
+  // the list counters count every append, also those past the end: a list that overflowed gets
+  // room for all of them (and a quarter more) and the pass is repeated
+  bool grow = false;
+  auto need = [&](int32_t got, int cap, int32_t& minimum) {
+    if (got <= cap) return;
+    const int64_t want = (int64_t)got + got / 4 + 1024;
+    if (want > (int64_t)(1 << 30)) throw BsqError(BSQ_ERR_LIMIT, "identify result list beyond 2^30 entries");
+    minimum = (int32_t)want;
+    grow = true;
+  };
+  need(counters[0], cand_cap, S.id_caps.cand);
+  need(counters[1], ins_cap, S.id_caps.ins);
+  need(counters[2], run_cap, S.id_caps.runs);
+  need(counters[4], fix_cap, S.id_caps.fix);
+  if (grow) return false;
   raise_err_bits(errbits);
   const int n_cand = counters[0];
   S.timing.candidates = n_cand;
@@ -890,6 +910,13 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   S.timing.kernel_detail_ms[5] = run_host_ms;
   S.timing.host_post_ms = now_ms() - t1;
   S.timing.host_pre_ms = th1 - th0;
+  return true;
+}
+
+static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& out, int screen_off) {
+  for (int attempt = 0; attempt < 3; attempt++)
+    if (run_identify_once(S, a, out, screen_off)) return;
+  throw BsqError(BSQ_ERR_LIMIT, "identify result lists could not be sized for this sample");
 }
 
 // BAM2COV table of one region (coverage_output::table, coverage_output.cpp:190-283; rows of

@@ -632,6 +632,12 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
   const double tol = P.precision_decimal;
   const double kDrop = 0.587566;   // kProfileLikelihoodLog10Drop (identify_mutations.cpp:3173)
   const int n_cand = min(*P.n_cand, P.cand_cap);
+  // With many more candidates than resident blocks the pass is bound by total work, not by one
+  // candidate's chain: the null fit then runs alone first, and only candidates that are emitted go on
+  // to the profile likelihoods and bisections (one phase more on their chain).  With few, all of
+  // phase 1 runs together.
+  const bool two_stage = n_cand > 2 * (int)gridDim.x;
+  const int prof_ph = two_stage ? 2 : 1;   // the phase of the profile likelihoods and first path levels
   uint32_t err = 0;
   for (int ci = blockIdx.x; ci < n_cand; ci += gridDim.x) {
     const int2 cd = P.cand[ci];
@@ -708,7 +714,9 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
         };
         if (ph == 0) {
           if (wave == 0) start(1, 31u, -1, 0.0, kResFull);
-        } else if (ph == 1) {
+        } else if (ph == 1 && two_stage) {
+          if (variant != 5 && n > 0 && wave == 0) start(1, 31u & ~(1u << variant), -1, 0.0, kResNull);
+        } else if (ph == prof_ph) {
           if (variant != 5 && n > 0) {
             // the first path levels of the bounds (lower bound's first) fill the slots that the
             // four phase-1 tasks leave idle; plmax and p1 end after a pass or two and share a wave
@@ -717,7 +725,7 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
               if (level < SS.spec[side]) start(2, 31u, variant, SS.pts[side][level], kResBisect + side * kBisectPts + level);
             };
             if (wave == 0) {
-              start(1, 31u & ~(1u << variant), -1, 0.0, kResNull);
+              if (!two_stage) start(1, 31u & ~(1u << variant), -1, 0.0, kResNull);
               spec_start(0);
             } else if (wave == 1) {
               start(2, 31u, variant, SS.full_f[variant], kResPlmax);
@@ -763,7 +771,7 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
         }
         if (lane < 5) SS.lps[lane] = acc;
       }
-      if (ph == 1 && wave == 1) {
+      if (ph == prof_ph && wave == 1) {
         // annotate_polymorphism_statistics (identify_mutations.cpp:3009-3033); used if emitted
         for (int i = lane; i < 256; i += 64) (&SS.qh[0][0])[i] = 0;
         wave_sync();
@@ -843,6 +851,11 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
           bool poly = false;
           if (variant != 5 && n > 0) poly = ((SS.res[kResFull] - SS.res[kResNull]) - P.log10_ref_length) >= P.polymorphism_cutoff;
           SS.emitted = (cons || poly) ? 1 : 0;
+        }
+        if (ph == 1 && two_stage) {
+          // the profile phase runs only for an emitted entry with a variant allele
+          SS.done = (SS.emitted && variant < 5 && n > 0) ? 0 : 1;
+        } else if (ph == prof_ph) {
           // write_RA_frequency_bounds (identify_mutations.cpp:3175-3217)
           if (SS.emitted && variant < 5 && n > 0) {
             const double fh = SS.full_f[variant];
@@ -867,13 +880,13 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
               SS.upper = fh;
             }
           }
-          // the speculative levels evaluated in phase 1 are walked like a round's path
+          // the speculative levels evaluated with the profiles are walked like a round's path
           for (int side = 0; side < 2; side++) {
             SS.path[side] = 1;
             SS.depth[side] = SS.spec[side];
           }
         }
-        {
+        if (ph >= prof_ph) {
           // walk each open bound down the evaluated levels: every level is one sequential
           // bisection step (interval halved at 0.5 * (lo + hi), ≤ 40 steps, stop at width ≤ tol)
           for (int side = 0; side < 2; side++) {
@@ -905,7 +918,7 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
               advanced++;
               node = path ? level + 1 : 2 * node + (left ? 1 : 2);
             }
-            if (ph >= 2 && path && advanced < 3) SS.tree_only[side] = 1;
+            if (ph > prof_ph && path && advanced < 3) SS.tree_only[side] = 1;
             SS.lo[side] = l;
             SS.hi[side] = h;
             SS.vlo[side] = vl;
@@ -919,7 +932,7 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
         // midpoints of the next round, each the midpoint of the interval a sequential bisection
         // would hold there: along the predicted path, or breadth-first over a full tree
         const bool both = SS.active[0] && SS.active[1];
-        for (int side = 0; side < 2; side++) {
+        for (int side = 0; side < 2 && ph >= prof_ph; side++) {
           if (!SS.active[side]) {
             SS.depth[side] = 0;
             continue;
@@ -954,7 +967,7 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
           pt[5] = 0.5 * (m0 + m2);
           pt[6] = 0.5 * (m2 + hi);
         }
-        SS.done = (SS.active[0] || SS.active[1]) ? 0 : 1;
+        if (ph >= prof_ph) SS.done = (SS.active[0] || SS.active[1]) ? 0 : 1;
       }
       __syncthreads();
       if (SS.done) break;

@@ -86,7 +86,8 @@ def cpu_baseline(B, cfg, target_seconds):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from breseq_amd import api
-    length = 300_000
+    # the oracle scores about 6.6e6 aligned bases (66 kb of reference at 100x) per second on one core
+    length = int(min(max(60_000 * target_seconds, 100_000), cfg["ref_length"]))
     small = dict(cfg)
     small["ref_length"] = length
     small["n_snps"] = max(1, int(cfg["n_snps"] * length / cfg["ref_length"]))

@@ -968,6 +968,9 @@ static std::string coverage_table(Session& S, const bsq_coverage_table_args* a) 
   const int64_t tail_end = end >= first ? first + (end - first) / ds * ds : 0;
   const int64_t z0 = std::max<int64_t>(last_pile, 0) + 1;                // zero rows z0..tail_end
   const int64_t nz = tail_end >= z0 ? tail_end - z0 + 1 : 0;
+  // counts come back to the host for every pass: 24 bytes per row and pass
+  if ((double)n * passes > (double)(1 << 28))
+    throw BsqError(BSQ_ERR_LIMIT, "coverage table too large: split the region or lower the resolution");
   std::vector<std::vector<uint32_t>> hu(passes), hraw(passes), hbeg(passes);
   std::vector<std::vector<double>> hred(passes);
   if (n > 0) {

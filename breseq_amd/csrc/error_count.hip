@@ -188,7 +188,47 @@ __device__ __forceinline__ void ec_count_run(const EcParams& P, const uint8_t* _
   }
 }
 
+// Observations (1) and (2) of ec_count_run for one lane's four query positions [q0, q0 + 4) of an
+// alignment whose M/=/X run is [x, x + len) <-> [y, y + len) and whose last base carries no indel,
+// with the strand a run-time value (the three alignments of a shared wavefront differ in strand).
+__device__ __forceinline__ void ec_group4(const EcParams& P, const uint8_t* __restrict__ sq,
+                                          const uint8_t* __restrict__ ql, uint32_t rs, bool rev, int x, int y,
+                                          int last, int pair_end, int q0, uint32_t* table, uint32_t& err) {
+  const uint32_t S = (uint32_t)P.read_set_max, Q = (uint32_t)P.quality_max;
+  const uint64_t rlut = rev ? kReadLutRev : kReadLutFwd;
+  const uint32_t flut = rev ? kRefLutRev : kRefLutFwd;
+  const uint32_t pair_base = rs + 24u * S;   // ref '.', obs '.'
+  const uint32_t bw0 = *(const uint32_t*)(sq + q0), bw1 = *(const uint32_t*)(sq + q0 + 4);
+  const uint32_t qw0 = *(const uint32_t*)(ql + q0), qw1 = *(const uint32_t*)(ql + q0 + 4);
+  const int c0 = x + (q0 - y);   // column of query position q0 (>= x - 3; ref_idx is front-padded)
+  const uint32_t* rp = (const uint32_t*)(P.ref_idx + (c0 & ~3));
+  const uint32_t rw0 = rp[0], rw1 = rp[1];
+  const int sh = c0 & 3;
+  const uint32_t rlo = __builtin_amdgcn_alignbyte(rw1, rw0, sh);   // codes of columns c0 .. c0+3
+  const uint32_t r4 = (rw1 >> (8 * sh)) & 0xFFu;                   // code of column c0+4
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int qpos = q0 + k;
+    const bool in = qpos >= y && qpos <= last;
+    const uint32_t b = byte_of(bw0, k), q = byte_of(qw0, k);
+    const uint32_t r0 = byte_of(rlo, k);
+    const uint32_t oi = (b > 15u) ? 15u : (uint32_t)(rlut >> (4u * b)) & 15u;
+    const uint32_t ri = (flut >> (4u * r0)) & 15u;
+    const bool v1 = in && b != 15u && r0 != kRefN;
+    const bool good1 = v1 && oi < 4u && ri < 4u && q < Q;
+    if (v1 && !good1) err |= (oi > 3u) ? kErrBaseCode : (ri > 3u) ? kErrRefChar : kErrQuality;
+    if (good1) atomicAdd(&table[rs + S * ri + 5u * S * oi + 25u * S * q], 1u);
+    const uint32_t b2 = rev ? b : ((k < 3) ? byte_of(bw0, k + 1) : byte_of(bw1, 0));
+    const uint32_t q2 = rev ? q : ((k < 3) ? byte_of(qw0, k + 1) : byte_of(qw1, 0));
+    const uint32_t rp2 = rev ? r0 : ((k < 3) ? byte_of(rlo, k + 1) : r4);
+    const bool v2 = in && qpos < pair_end && b2 != 15u && rp2 != kRefN;
+    if (v2 && q2 >= Q) err |= kErrQuality;
+    if (v2 && q2 < Q) atomicAdd(&table[pair_base + 25u * S * q2], 1u);
+  }
+}
+
 constexpr int kEcChunk = 16;   // consecutive alignments per wavefront work item
+constexpr int kEcGroup = 21;   // lanes per alignment when three share a wavefront (8 positions each)
 
 __global__ void __launch_bounds__(256) k_ec_reads(EcParams P) {
   extern __shared__ uint32_t table[];
@@ -202,60 +242,99 @@ __global__ void __launch_bounds__(256) k_ec_reads(EcParams P) {
   int32_t* diff_u = P.cov_diff;
   int32_t* diff_r = P.cov_diff + P.n_pos + 1;
   uint32_t err = 0;
+  // one alignment at a time, wave-uniform (scalar loads; any CIGAR, redundant alignments)
+  auto one = [&](int64_t r) {
+    const DRead R = load_read(P.reads, (int)r);
+    const int t = pos_target[R.gpos];
+    if (t < 0 || !scored[t]) return;
+    const bool red = R.x1 > 1;
+    int32_t* diff = red ? diff_r : diff_u;
+    bool count = !red && P.do_errors;
+    if (count && (int)R.rfile >= P.read_set_max) {   // covariate read_set out of range: ASSERT in the reference
+      err |= kErrReadSet;
+      count = false;
+    }
+    const uint8_t* sq = P.bases + R.seq_off;
+    const uint8_t* ql = P.quals + R.seq_off;
+    if (R.flags & kReadSimple) {
+      if (lane == 0) {
+        atomicAdd(&diff[R.gpos], 1);
+        atomicAdd(&diff[R.gend], -1);
+      }
+      if (count) {
+        if (R.flags & kReadReversed) ec_count_run<true>(P, sq, ql, R, R.gpos, R.y0, R.gend - R.gpos, 0, table, err);
+        else ec_count_run<false>(P, sq, ql, R, R.gpos, R.y0, R.gend - R.gpos, 0, table, err);
+      }
+      return;
+    }
+    const ConstU32* c = (const ConstU32*)(P.cigar + R.cig_off);
+    const int n = R.n_cigar;
+    int x = R.gpos, y = 0;
+    for (int k = 0; k < n; k++) {
+      const uint32_t w = c[k];
+      const uint32_t op = w & 15u;
+      const int l = (int)(w >> 4);
+      if (op == OP_M || op == OP_EQ || op == OP_X) {
+        if (l > 0) {
+          if (lane == 0) {
+            atomicAdd(&diff[x], 1);
+            atomicAdd(&diff[x + l], -1);
+          }
+          if (count) {
+            const int li = run_end_indel(c, k, n);
+            if (R.flags & kReadReversed) ec_count_run<true>(P, sq, ql, R, x, y, l, li, table, err);
+            else ec_count_run<false>(P, sq, ql, R, x, y, l, li, table, err);
+          }
+        }
+        x += l;
+        y += l;
+      } else if (op == OP_I || op == OP_S) {
+        y += l;
+      } else if (op == OP_D || op == OP_N) {
+        x += l;
+      }
+    }
+    if (x != R.gend) err |= kErrCigar;
+  };
+  const int grp = lane / kEcGroup, gl = lane - kEcGroup * grp;
   for (int64_t c0 = gw * kEcChunk; c0 < P.n_reads; c0 += nw * kEcChunk) {
     const int64_t c1 = min<int64_t>(c0 + kEcChunk, P.n_reads);
-    for (int64_t r = c0; r < c1; r++) {
-      const DRead R = load_read(P.reads, (int)r);
-      const int t = pos_target[R.gpos];
-      if (t < 0 || !scored[t]) continue;
-      const bool red = R.x1 > 1;
-      int32_t* diff = red ? diff_r : diff_u;
-      bool count = !red && P.do_errors;
-      if (count && (int)R.rfile >= P.read_set_max) {   // covariate read_set out of range: ASSERT in the reference
-        err |= kErrReadSet;
-        count = false;
+    int64_t r = c0;
+    while (r < c1) {
+      // Up to three counted single-run alignments of at most 168 bases share the wavefront (lanes
+      // 21g .. 21g+20 take alignment r+g, eight query positions per lane), which fills the lanes
+      // that one 150-base alignment leaves idle; any other alignment goes alone.
+      const int avail = (int)min<int64_t>(3, c1 - r);
+      bool fit = false;
+      DRead R;
+      if (grp < avail) {
+        R = P.reads[r + grp];
+        const int t = pos_target[R.gpos];
+        fit = t >= 0 && scored[t] && (R.flags & kReadSimple) && R.x1 <= 1 && P.do_errors &&
+              (int)R.rfile < P.read_set_max && (R.y0 & 3) + (R.gend - R.gpos) <= 8 * kEcGroup;
       }
-      const uint8_t* sq = P.bases + R.seq_off;
-      const uint8_t* ql = P.quals + R.seq_off;
-      if (R.flags & kReadSimple) {
-        if (lane == 0) {
-          atomicAdd(&diff[R.gpos], 1);
-          atomicAdd(&diff[R.gend], -1);
+      // leading alignments that fit (group leaders sit at lanes 0, 21, 42)
+      const unsigned long long m = __ballot(fit && gl == 0);
+      const int k = !(m & 1ull) ? 0 : !((m >> kEcGroup) & 1ull) ? 1 : !((m >> (2 * kEcGroup)) & 1ull) ? 2 : 3;
+      if (k > 0) {
+        if (grp < k) {
+          if (gl == 0) {
+            atomicAdd(&diff_u[R.gpos], 1);
+            atomicAdd(&diff_u[R.gend], -1);
+          }
+          const uint8_t* sq = P.bases + R.seq_off;
+          const uint8_t* ql = P.quals + R.seq_off;
+          const int y = R.y0, yend = y + (R.gend - R.gpos), last = yend - 1;
+          const bool rev = (R.flags & kReadReversed) != 0;
+          const int q0 = (y & ~3) + 8 * gl;
+          if (q0 < yend) ec_group4(P, sq, ql, R.rfile, rev, R.gpos, y, last, (int)R.qe0, q0, table, err);
+          if (q0 + 4 < yend) ec_group4(P, sq, ql, R.rfile, rev, R.gpos, y, last, (int)R.qe0, q0 + 4, table, err);
         }
-        if (count) {
-          if (R.flags & kReadReversed) ec_count_run<true>(P, sq, ql, R, R.gpos, R.y0, R.gend - R.gpos, 0, table, err);
-          else ec_count_run<false>(P, sq, ql, R, R.gpos, R.y0, R.gend - R.gpos, 0, table, err);
-        }
+        r += k;
         continue;
       }
-      const ConstU32* c = (const ConstU32*)(P.cigar + R.cig_off);
-      const int n = R.n_cigar;
-      int x = R.gpos, y = 0;
-      for (int k = 0; k < n; k++) {
-        const uint32_t w = c[k];
-        const uint32_t op = w & 15u;
-        const int l = (int)(w >> 4);
-        if (op == OP_M || op == OP_EQ || op == OP_X) {
-          if (l > 0) {
-            if (lane == 0) {
-              atomicAdd(&diff[x], 1);
-              atomicAdd(&diff[x + l], -1);
-            }
-            if (count) {
-              const int li = run_end_indel(c, k, n);
-              if (R.flags & kReadReversed) ec_count_run<true>(P, sq, ql, R, x, y, l, li, table, err);
-              else ec_count_run<false>(P, sq, ql, R, x, y, l, li, table, err);
-            }
-          }
-          x += l;
-          y += l;
-        } else if (op == OP_I || op == OP_S) {
-          y += l;
-        } else if (op == OP_D || op == OP_N) {
-          x += l;
-        }
-      }
-      if (x != R.gend) err |= kErrCigar;
+      one(r);
+      r++;
     }
   }
   __syncthreads();

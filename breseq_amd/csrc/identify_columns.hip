@@ -343,6 +343,8 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
   uint32_t err = 0;
   const int S = P.read_set_max, Q = P.quality_max;
   const int qcut = P.base_quality_cutoff;
+  // counted qualities: qcut <= q < Q, i.e. (q - qcut) < qspan as unsigned
+  const uint32_t qspan = (Q > qcut) ? (uint32_t)(Q - qcut) : 0u;
 
   for (int64_t w = w_begin + wave; w < w_end; w += kColWaves) {
     const int p = (int)(w * kWaveCols + lane);
@@ -352,6 +354,9 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
     // hypothesis index of the reference base for a forward / reverse read
     const int e_top = (ref_idx >= 0 && ref_idx < 5) ? ref_idx : 0;
     const int e_bot = (e_top < 4) ? 3 - e_top : e_top;
+    // the reference base as a read-strand class (7 when it is not A, C, G or T)
+    const uint32_t ref_rs_top = (ref_idx >= 0 && ref_idx < 4) ? (uint32_t)ref_idx : 7u;
+    const uint32_t ref_rs_bot = (ref_idx >= 0 && ref_idx < 4) ? (uint32_t)(ref_idx ^ 3) : 7u;
     const int rb = __builtin_amdgcn_readfirstlane(P.win_rb[w]);
     const int re = __builtin_amdgcn_readfirstlane(P.win_re[w]);
     float lps0 = 0.0f, lps1 = 0.0f, lps2 = 0.0f, lps3 = 0.0f, lps4 = 0.0f;
@@ -382,23 +387,104 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
         rc = src[1];
       }
       // ra = {gpos, gend, seq_off, qoff}, rc = {trim, row_base, x1_flags, pad}
-      uint32_t nb = 0u, nq = 0u;
+      // Per-alignment view for the fast path (single-run alignments with X1 == 1 and a read file
+      // in the table): span length | untrimmed length << 16, the byte offset of the first aligned
+      // base, the first untrimmed column, and row_base << 2 | fast << 1 | reversed.  The span of
+      // a single-run alignment and the untrimmed length are below 65536.
+      const uint32_t tr_lo = rc.x & 0xFFFFu, tr_hi = rc.x >> 16;
+      const uint32_t f_len = (ra.y - ra.x) | ((tr_hi > tr_lo ? tr_hi - tr_lo : 0u) << 16);
+      const uint32_t f_at = ra.z + ra.w + ra.x;
+      const uint32_t f_clo = tr_lo - ra.w;
+      const bool f_fast = (rc.z & 0xFFFFu) == 1u && (rc.z & (kReadSimple << 16)) && rc.y < (1u << 29);
+      const uint32_t f_row = (f_fast ? ((rc.y << 2) | 2u) : 0u) | ((rc.z >> 16) & kReadReversed);
+      // bytes of a single-run alignment at this lane's column (its last aligned base when the
+      // column is outside it), and the column's offset into the span
+      uint32_t nb = 0u, nq = 0u, nd = 0u;
       auto load_bytes = [&](int j) {
-        const int gpos = (int)rdlane(ra.x, j), gend = (int)rdlane(ra.y, j);
-        const uint32_t soff = rdlane(ra.z, j);
-        const int qoff = (int)rdlane(ra.w, j);
-        const bool in = (unsigned)(p - gpos) < (unsigned)(gend - gpos);
-        const uint32_t at = soff + (in ? (uint32_t)(p + qoff) : 0u);
+        const uint32_t gpos = rdlane(ra.x, j);
+        const uint32_t len = rdlane(f_len, j) & 0xFFFFu;
+        nd = (uint32_t)p - gpos;
+        const uint32_t at = rdlane(f_at, j) + min(nd, len - 1u);
         nb = P.bases[at];
         nq = P.quals[at];
       };
-      if (rdlane(rc.z, 0) & (kReadSimple << 16)) load_bytes(0);
-      for (int j = 0; j < m; j++) {
+      // The unique single-run alignments of the batch go first, the others after them in file
+      // order: only the others add to the FP64 sums of 1/X1 (which keep the reference's read
+      // order); the FP32 sums are certified for any order and the counts are order-free.
+      const unsigned long long live = (m == 64) ? ~0ull : ((1ull << m) - 1ull);
+      unsigned long long fm = __ballot(f_fast) & live;
+      unsigned long long sm = live & ~fm;
+      if (fm) {
+        int j = __builtin_ctzll(fm);
+        load_bytes(j);
+        for (;;) {
+          fm &= fm - 1ull;
+          const int jn = fm ? __builtin_ctzll(fm) : -1;
+          const uint32_t bq = nb, qq = nq, dj = nd;
+          if (jn >= 0) load_bytes(jn);
+          const uint32_t fr = rdlane(f_row, j);
+          // Fast path: a unique single-run alignment.  No deletion or insertion entries; the
+          // observed class in read-strand terms comes from a 2-bit field per BAM base code
+          // (A1 C2 G4 T8 -> 0 1 2 3, complemented for reverse reads), valid only for those codes.
+          const uint32_t fl = rdlane(f_len, j);
+          const uint32_t clo = rdlane(f_clo, j);
+          const bool rev = (fr & 1u) != 0u;
+          const bool valid = lane_ok && dj < (fl & 0xFFFFu) && bq != 15u;
+          uniq += valid ? 1 : 0;
+          const bool cand = valid && ((uint32_t)p - clo) < (fl >> 16);
+          const bool acgt = ((0x116u >> min(bq, 16u)) & 1u) != 0u;
+          const uint32_t o_rs = ((rev ? 0x0000012Cu : 0x00030210u) >> ((2u * bq) & 31u)) & 3u;
+          const bool ok = cand && acgt && (qq - (uint32_t)qcut) < qspan;
+          if (cand && !ok) {
+            // not counted: quality below the cutoff, or an error (the reference asserts)
+            if (!acgt) err |= kErrBaseCode;
+            else if ((int)qq >= qcut && qq >= (uint32_t)Q) err |= kErrQuality;
+          }
+          const int row = ok ? (int)((fr >> 2) + qq * 5u + o_rs) : zero_row;
+          const FoldRow& T = tab[row];
+          const float t0 = T.t[0], t1 = T.t[1], t2 = T.t[2], t3 = T.t[3], t4 = T.t[4];
+          if (!rev) {
+            lps0 += t0;
+            lps1 += t1;
+            lps2 += t2;
+            lps3 += t3;
+          } else {
+            lps0 += t3;
+            lps1 += t2;
+            lps2 += t1;
+            lps3 += t0;
+          }
+          lps4 += t4;
+          const float dg = T.diag;
+          A.SL += T.lpmax;
+          A.Dall += dg;
+          A.n += ok ? 1u : 0u;
+          if (ok && o_rs != (rev ? ref_rs_bot : ref_rs_top)) {
+            const uint32_t obs = rev ? (o_rs ^ 3u) : o_rs;
+            const float e = T.t[rev ? e_bot : e_top];
+            const float g = e - dg;
+            A.Gabs += fabsf(e) + fabsf(dg);
+  #pragma unroll
+            for (int o = 0; o < 4; o++) {
+              if (obs == (uint32_t)o) {
+                A.G[o] += g;
+                A.cnt[o] += 1u;
+              }
+            }
+          }
+          if (jn < 0) break;
+          j = jn;
+        }
+      }
+      while (sm) {
+        const int j = __builtin_ctzll(sm);
+        sm &= sm - 1ull;
+        if (rdlane(rc.z, j) & (kReadSimple << 16)) load_bytes(j);
+        const uint32_t dj = nd;
+        uint32_t bq = nb, qq = nq;
         const uint32_t xf = rdlane(rc.z, j);
         const uint32_t x1 = xf & 0xFFFFu, rflags = xf >> 16;
         const uint32_t row_base = rdlane(rc.y, j);
-        uint32_t bq = nb, qq = nq;
-        if (j + 1 < m && (rdlane(rc.z, j + 1) & (kReadSimple << 16))) load_bytes(j + 1);
         // entry bits: 1 cover, 2 deletion, 4 deletion quality usable, 8 trimmed, 16 insertion follows
         uint32_t eb;
         uint32_t qd = 0u;
@@ -406,7 +492,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
           const int gpos = (int)rdlane(ra.x, j), gend = (int)rdlane(ra.y, j);
           const int qpos = p + (int)rdlane(ra.w, j);
           const uint32_t tr = rdlane(rc.x, j);
-          const bool cover = lane_ok && (unsigned)(p - gpos) < (unsigned)(gend - gpos);
+          const bool cover = lane_ok && dj < (uint32_t)(gend - gpos);
           const bool trimmed = qpos < (int)(tr & 0xFFFFu) || qpos >= (int)(tr >> 16);
           eb = (cover ? 1u : 0u) | (trimmed ? 8u : 0u);
         } else {

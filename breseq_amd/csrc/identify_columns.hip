@@ -20,6 +20,8 @@
 // adds 1/X1.  The next alignment's record and bytes are loaded while the current one is added.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -389,14 +391,14 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
       // ra = {gpos, gend, seq_off, qoff}, rc = {trim, row_base, x1_flags, pad}
       // Per-alignment view for the fast path (single-run alignments with X1 == 1 and a read file
       // in the table): span length | untrimmed length << 16, the byte offset of the first aligned
-      // base, the first untrimmed column, and row_base << 2 | fast << 1 | reversed.  The span of
+      // base, the first untrimmed column, and row_base << 2 | fast << 1.  The span of
       // a single-run alignment and the untrimmed length are below 65536.
       const uint32_t tr_lo = rc.x & 0xFFFFu, tr_hi = rc.x >> 16;
       const uint32_t f_len = (ra.y - ra.x) | ((tr_hi > tr_lo ? tr_hi - tr_lo : 0u) << 16);
       const uint32_t f_at = ra.z + ra.w + ra.x;
       const uint32_t f_clo = tr_lo - ra.w;
       const bool f_fast = (rc.z & 0xFFFFu) == 1u && (rc.z & (kReadSimple << 16)) && rc.y < (1u << 29);
-      const uint32_t f_row = (f_fast ? ((rc.y << 2) | 2u) : 0u) | ((rc.z >> 16) & kReadReversed);
+      const uint32_t f_row = f_fast ? ((rc.y << 2) | 2u) : 0u;
       // bytes of a single-run alignment at this lane's column (its last aligned base when the
       // column is outside it), and the column's offset into the span
       uint32_t nb = 0u, nq = 0u, nd = 0u;
@@ -408,13 +410,21 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
         nb = P.bases[at];
         nq = P.quals[at];
       };
-      // The unique single-run alignments of the batch go first, the others after them in file
-      // order: only the others add to the FP64 sums of 1/X1 (which keep the reference's read
-      // order); the FP32 sums are certified for any order and the counts are order-free.
+      // The unique single-run alignments of the batch go first (forward ones, then reverse
+      // ones), the others after them in file order: only the others add to the FP64 sums of
+      // 1/X1 (which keep the reference's read order); the FP32 sums are certified for any order
+      // and the counts are order-free.
       const unsigned long long live = (m == 64) ? ~0ull : ((1ull << m) - 1ull);
-      unsigned long long fm = __ballot(f_fast) & live;
-      unsigned long long sm = live & ~fm;
-      if (fm) {
+      const bool f_rev = (rc.z >> 16) & kReadReversed;
+      unsigned long long sm = live & ~__ballot(f_fast);
+      // one loop per strand, so that the strand is a compile-time constant
+      auto fast_loop = [&](auto rev_c, unsigned long long fm) {
+        constexpr bool rev = decltype(rev_c)::value;
+        // read-strand class per BAM base code (A1 C2 G4 T8), two bits each
+        constexpr uint32_t kLut = rev ? 0x0000012Cu : 0x00030210u;
+        const uint32_t ref_rs = rev ? ref_rs_bot : ref_rs_top;
+        const int e_ref = rev ? e_bot : e_top;
+        if (!fm) return;
         int j = __builtin_ctzll(fm);
         load_bytes(j);
         for (;;) {
@@ -422,18 +432,16 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
           const int jn = fm ? __builtin_ctzll(fm) : -1;
           const uint32_t bq = nb, qq = nq, dj = nd;
           if (jn >= 0) load_bytes(jn);
+          // No deletion or insertion entries in a single-run alignment; the class is valid only
+          // for the codes of A, C, G and T.
           const uint32_t fr = rdlane(f_row, j);
-          // Fast path: a unique single-run alignment.  No deletion or insertion entries; the
-          // observed class in read-strand terms comes from a 2-bit field per BAM base code
-          // (A1 C2 G4 T8 -> 0 1 2 3, complemented for reverse reads), valid only for those codes.
           const uint32_t fl = rdlane(f_len, j);
           const uint32_t clo = rdlane(f_clo, j);
-          const bool rev = (fr & 1u) != 0u;
           const bool valid = lane_ok && dj < (fl & 0xFFFFu) && bq != 15u;
           uniq += valid ? 1 : 0;
           const bool cand = valid && ((uint32_t)p - clo) < (fl >> 16);
           const bool acgt = ((0x116u >> min(bq, 16u)) & 1u) != 0u;
-          const uint32_t o_rs = ((rev ? 0x0000012Cu : 0x00030210u) >> ((2u * bq) & 31u)) & 3u;
+          const uint32_t o_rs = (kLut >> ((2u * bq) & 31u)) & 3u;
           const bool ok = cand && acgt && (qq - (uint32_t)qcut) < qspan;
           if (cand && !ok) {
             // not counted: quality below the cutoff, or an error (the reference asserts)
@@ -459,12 +467,12 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
           A.SL += T.lpmax;
           A.Dall += dg;
           A.n += ok ? 1u : 0u;
-          if (ok && o_rs != (rev ? ref_rs_bot : ref_rs_top)) {
+          if (ok && o_rs != ref_rs) {
             const uint32_t obs = rev ? (o_rs ^ 3u) : o_rs;
-            const float e = T.t[rev ? e_bot : e_top];
+            const float e = T.t[e_ref];
             const float g = e - dg;
             A.Gabs += fabsf(e) + fabsf(dg);
-  #pragma unroll
+#pragma unroll
             for (int o = 0; o < 4; o++) {
               if (obs == (uint32_t)o) {
                 A.G[o] += g;
@@ -475,7 +483,9 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
           if (jn < 0) break;
           j = jn;
         }
-      }
+      };
+      fast_loop(std::integral_constant<bool, false>{}, __ballot(f_fast && !f_rev) & live);
+      fast_loop(std::integral_constant<bool, true>{}, __ballot(f_fast && f_rev) & live);
       while (sm) {
         const int j = __builtin_ctzll(sm);
         sm &= sm - 1ull;

@@ -159,37 +159,70 @@ void ra_entry_from_result(const RaOut& r, const std::string& seq_id, uint32_t po
   e.ins = r.slot;
   e.ref_base = std::string(1, kIdx[r.ref_idx]);
   e.new_base = std::string(1, kIdx[r.variant]);
-  auto cov = [&](int b) -> std::string {
-    if (b >= 5) return "0/0";
-    return std::to_string((int32_t)r.cov[b][0]) + "/" + std::to_string((int32_t)r.cov[b][1]);
+  // the fields in the order of their names (the entry's std::map order when written), written
+  // straight into one string; a value that formats empty is left out, as the writer does
+  std::string& t = e.fields_text;
+  t.clear();
+  t.reserve(400);
+  char buf[64];
+  auto key = [&](const char* k) {
+    t += '\t';
+    t += k;
+    t += '=';
   };
-  std::string spec;
+  auto put_i = [&](int64_t x) {
+    const std::to_chars_result c = std::to_chars(buf, buf + sizeof(buf), x);
+    t.append(buf, (size_t)(c.ptr - buf));
+  };
+  auto put_d = [&](const char* k, double v, unsigned prec, bool sci) {
+    const int n = fmt_double_buf(v, prec, sci, buf, sizeof(buf));
+    if (n <= 0) return;
+    key(k);
+    t.append(buf, (size_t)n);
+  };
+  auto put_cov = [&](const char* k, int b) {
+    key(k);
+    if (b >= 5) {
+      t += "0/0";
+      return;
+    }
+    put_i((int32_t)r.cov[b][0]);
+    t += '/';
+    put_i((int32_t)r.cov[b][1]);
+  };
+  bool any = false;
   for (int b = 0; b < 5; b++) {
     const double fr = reported(r, b);
     if (fr <= 0.0) continue;
-    if (!spec.empty()) spec += ",";
-    spec += kIdx[b];
-    spec += ":";
-    spec += fmt_double(fr, places, true);
+    const int n = fmt_double_buf(fr, places, true, buf, sizeof(buf));
+    if (n <= 0) continue;
+    if (!any) key("allele_frequencies");
+    else t += ',';
+    any = true;
+    t += kIdx[b];
+    t += ':';
+    t.append(buf, (size_t)n);
   }
-  // the fields in the order of their names (the entry's std::map order when written)
+  put_d("fisher_strand_p_value", r.fisher_p, 5, true);
+  put_d("frequency", reported(r, r.variant), places, true);
+  put_d("frequency_lower", r.lower, places, true);
+  put_d("frequency_upper", r.upper, places, true);
+  put_d("ks_quality_p_value", r.ks_p, 5, true);
+  key("major_base");
+  t += kIdx[r.major];
+  put_cov("major_cov", r.major);
+  put_d("major_frequency", reported(r, r.major), places, true);
+  key("minor_base");
+  t += kIdx[r.minor];
+  put_cov("minor_cov", r.minor);
+  put_cov("new_cov", r.variant);
+  put_cov("ref_cov", r.ref_idx);
+  put_d("score", r.variant_score, 1, false);
+  key("total_cov");
+  put_i(r.total_top);
+  t += '/';
+  put_i(r.total_bot);
   e.fields.clear();
-  e.fields.reserve(15);
-  e.fields.emplace_back("allele_frequencies", std::move(spec));
-  e.fields.emplace_back("fisher_strand_p_value", fmt_double(r.fisher_p, 5, true));
-  e.fields.emplace_back("frequency", fmt_double(reported(r, r.variant), places, true));
-  e.fields.emplace_back("frequency_lower", fmt_double(r.lower, places, true));
-  e.fields.emplace_back("frequency_upper", fmt_double(r.upper, places, true));
-  e.fields.emplace_back("ks_quality_p_value", fmt_double(r.ks_p, 5, true));
-  e.fields.emplace_back("major_base", std::string(1, kIdx[r.major]));
-  e.fields.emplace_back("major_cov", cov(r.major));
-  e.fields.emplace_back("major_frequency", fmt_double(reported(r, r.major), places, true));
-  e.fields.emplace_back("minor_base", std::string(1, kIdx[r.minor]));
-  e.fields.emplace_back("minor_cov", cov(r.minor));
-  e.fields.emplace_back("new_cov", cov(r.variant));
-  e.fields.emplace_back("ref_cov", cov(r.ref_idx));
-  e.fields.emplace_back("score", fmt_double(r.variant_score, 1, false));
-  e.fields.emplace_back("total_cov", std::to_string(r.total_top) + "/" + std::to_string(r.total_bot));
 }
 
 static int type_group(int t) { return t == 0 ? 3 : (t == 1 ? 4 : 7); }
@@ -258,6 +291,7 @@ std::string write_gd(std::vector<GdEntry>& entries) {
     } else {
       put_u(e->end);
     }
+    out += e->fields_text;
     for (const auto& kv : e->fields) {
       if (kv.second.empty() || kv.first[0] == '_') continue;
       out += '\t';

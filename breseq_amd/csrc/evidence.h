@@ -34,6 +34,7 @@ struct GdEntry {
   uint32_t start_range = 0, end_range = 0;
   std::string ref_base, new_base;
   std::vector<std::pair<const char*, std::string>> fields;   // other key=value pairs (static names)
+  std::string fields_text;   // or those pairs already written as "\tkey=value..." (RA entries)
   // traversal order key (target rank, emission column, kind) used to number entries as add() does
   int64_t order_target = 0, order_pos = 0, order_sub = 0;
 };

@@ -298,43 +298,46 @@ __global__ void __launch_bounds__(256) k_ec_reads(EcParams P) {
   };
   const int grp = lane / kEcGroup, gl = lane - kEcGroup * grp;
   for (int64_t c0 = gw * kEcChunk; c0 < P.n_reads; c0 += nw * kEcChunk) {
-    const int64_t c1 = min<int64_t>(c0 + kEcChunk, P.n_reads);
-    int64_t r = c0;
-    while (r < c1) {
-      // Up to three counted single-run alignments of at most 168 bases share the wavefront (lanes
-      // 21g .. 21g+20 take alignment r+g, eight query positions per lane), which fills the lanes
-      // that one 150-base alignment leaves idle; any other alignment goes alone.
-      const int avail = (int)min<int64_t>(3, c1 - r);
-      bool fit = false;
-      DRead R;
-      if (grp < avail) {
-        R = P.reads[r + grp];
-        const int t = pos_target[R.gpos];
-        fit = t >= 0 && scored[t] && (R.flags & kReadSimple) && R.x1 <= 1 && P.do_errors &&
-              (int)R.rfile < P.read_set_max && (R.y0 & 3) + (R.gend - R.gpos) <= 8 * kEcGroup;
+    const int n = (int)min<int64_t>(kEcChunk, P.n_reads - c0);
+    // Counted single-run alignments of at most 168 bases go three to a wavefront (lanes
+    // 21g .. 21g+20 take one, eight query positions per lane), which fills the lanes that one
+    // 150-base alignment leaves idle; the chunk's other alignments then go one at a time.  The
+    // counts are integers, so the order does not matter.
+    bool fit = false;
+    if (lane < n) {
+      const DRead R = P.reads[c0 + lane];
+      const int t = pos_target[R.gpos];
+      fit = t >= 0 && scored[t] && (R.flags & kReadSimple) && R.x1 <= 1 && P.do_errors &&
+            (int)R.rfile < P.read_set_max && (R.y0 & 3) + (R.gend - R.gpos) <= 8 * kEcGroup;
+    }
+    unsigned long long fm = __ballot(fit);
+    unsigned long long sm = ((n == 64) ? ~0ull : ((1ull << n) - 1ull)) & ~fm;
+    while (fm) {
+      const int ja = __builtin_ctzll(fm);
+      fm &= fm - 1ull;
+      const int jb = fm ? __builtin_ctzll(fm) : -1;
+      if (jb >= 0) fm &= fm - 1ull;
+      const int jc = fm ? __builtin_ctzll(fm) : -1;
+      if (jc >= 0) fm &= fm - 1ull;
+      const int mine = (grp == 0) ? ja : (grp == 1) ? jb : (grp == 2) ? jc : -1;
+      if (mine < 0) continue;
+      const DRead R = P.reads[c0 + mine];
+      if (gl == 0) {
+        atomicAdd(&diff_u[R.gpos], 1);
+        atomicAdd(&diff_u[R.gend], -1);
       }
-      // leading alignments that fit (group leaders sit at lanes 0, 21, 42)
-      const unsigned long long m = __ballot(fit && gl == 0);
-      const int k = !(m & 1ull) ? 0 : !((m >> kEcGroup) & 1ull) ? 1 : !((m >> (2 * kEcGroup)) & 1ull) ? 2 : 3;
-      if (k > 0) {
-        if (grp < k) {
-          if (gl == 0) {
-            atomicAdd(&diff_u[R.gpos], 1);
-            atomicAdd(&diff_u[R.gend], -1);
-          }
-          const uint8_t* sq = P.bases + R.seq_off;
-          const uint8_t* ql = P.quals + R.seq_off;
-          const int y = R.y0, yend = y + (R.gend - R.gpos), last = yend - 1;
-          const bool rev = (R.flags & kReadReversed) != 0;
-          const int q0 = (y & ~3) + 8 * gl;
-          if (q0 < yend) ec_group4(P, sq, ql, R.rfile, rev, R.gpos, y, last, (int)R.qe0, q0, table, err);
-          if (q0 + 4 < yend) ec_group4(P, sq, ql, R.rfile, rev, R.gpos, y, last, (int)R.qe0, q0 + 4, table, err);
-        }
-        r += k;
-        continue;
-      }
-      one(r);
-      r++;
+      const uint8_t* sq = P.bases + R.seq_off;
+      const uint8_t* ql = P.quals + R.seq_off;
+      const int y = R.y0, yend = y + (R.gend - R.gpos), last = yend - 1;
+      const bool rev = (R.flags & kReadReversed) != 0;
+      const int q0 = (y & ~3) + 8 * gl;
+      if (q0 < yend) ec_group4(P, sq, ql, R.rfile, rev, R.gpos, y, last, (int)R.qe0, q0, table, err);
+      if (q0 + 4 < yend) ec_group4(P, sq, ql, R.rfile, rev, R.gpos, y, last, (int)R.qe0, q0 + 4, table, err);
+    }
+    while (sm) {
+      const int j = __builtin_ctzll(sm);
+      sm &= sm - 1ull;
+      one(c0 + j);
     }
   }
   __syncthreads();

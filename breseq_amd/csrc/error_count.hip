@@ -90,6 +90,13 @@ __device__ __forceinline__ int run_end_indel(const ConstU32* c, int k, int n) {
 
 __device__ __forceinline__ uint32_t byte_of(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
 
+// Bin of (read set rs, reference class ri, observed class oi, quality q) in the count table,
+// rs + S*(ri + 5*oi + 25*q), as full-rate 24-bit multiply-adds: every operand is below 2^16
+// (classes below 16, qualities below 256, S at most 256).
+__device__ __forceinline__ uint32_t ec_bin(uint32_t rs, uint32_t S, uint32_t ri, uint32_t oi, uint32_t q) {
+  return __umul24(__umul24(q, 25u) + __umul24(oi, 5u) + ri, S) + rs;
+}
+
 // Strand-folded base indices as 4-bit lookup tables: read bases by BAM code (A1 C2 G4 T8), reference
 // bases by kRef* code (A0 C1 G2 T3 '.'4 N5).  The reverse-strand tables complement A<->T, C<->G, as
 // count_alignment_position does before indexing (error_count.cpp:887-903).  15 = not a base.
@@ -142,7 +149,7 @@ __device__ __forceinline__ void ec_count_run(const EcParams& P, const uint8_t* _
       const bool v1 = in && b != 15u && r0 != kRefN;
       const bool good1 = v1 && oi < 4u && ri < 4u && q < Q;
       if (v1 && !good1) err |= (oi > 3u) ? kErrBaseCode : (ri > 3u) ? kErrRefChar : kErrQuality;
-      if (good1) atomicAdd(&table[rs + S * ri + 5u * S * oi + 25u * S * q], 1u);
+      if (good1) atomicAdd(&table[ec_bin(rs, S, ri, oi, q)], 1u);
       // (2) '..' pair (:909-928): the base after qpos in read direction (qpos+1 forward, qpos
       // itself reverse) against the reference after it in reference direction
       const uint32_t b2 = kRev ? b : ((k < 3) ? byte_of(bw0, k + 1) : byte_of(bw1, 0));
@@ -150,7 +157,7 @@ __device__ __forceinline__ void ec_count_run(const EcParams& P, const uint8_t* _
       const uint32_t rp2 = kRev ? r0 : ((k < 3) ? byte_of(rlo, k + 1) : r4);
       const bool v2 = in && qpos < pair_end && b2 != 15u && rp2 != kRefN;
       if (v2 && q2 >= Q) err |= kErrQuality;
-      if (v2 && q2 < Q) atomicAdd(&table[pair_base + 25u * S * q2], 1u);
+      if (v2 && q2 < Q) atomicAdd(&table[__umul24(q2, 25u * S) + pair_base], 1u);
     }
   }
   if (last_indel != -1 && last_indel != 1) return;
@@ -217,13 +224,13 @@ __device__ __forceinline__ void ec_group4(const EcParams& P, const uint8_t* __re
     const bool v1 = in && b != 15u && r0 != kRefN;
     const bool good1 = v1 && oi < 4u && ri < 4u && q < Q;
     if (v1 && !good1) err |= (oi > 3u) ? kErrBaseCode : (ri > 3u) ? kErrRefChar : kErrQuality;
-    if (good1) atomicAdd(&table[rs + S * ri + 5u * S * oi + 25u * S * q], 1u);
+    if (good1) atomicAdd(&table[ec_bin(rs, S, ri, oi, q)], 1u);
     const uint32_t b2 = rev ? b : ((k < 3) ? byte_of(bw0, k + 1) : byte_of(bw1, 0));
     const uint32_t q2 = rev ? q : ((k < 3) ? byte_of(qw0, k + 1) : byte_of(qw1, 0));
     const uint32_t rp2 = rev ? r0 : ((k < 3) ? byte_of(rlo, k + 1) : r4);
     const bool v2 = in && qpos < pair_end && b2 != 15u && rp2 != kRefN;
     if (v2 && q2 >= Q) err |= kErrQuality;
-    if (v2 && q2 < Q) atomicAdd(&table[pair_base + 25u * S * q2], 1u);
+    if (v2 && q2 < Q) atomicAdd(&table[__umul24(q2, 25u * S) + pair_base], 1u);
   }
 }
 

@@ -425,15 +425,29 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
         const uint32_t ref_rs = rev ? ref_rs_bot : ref_rs_top;
         const int e_ref = rev ? e_bot : e_top;
         if (!fm) return;
-        int j = __builtin_ctzll(fm);
-        load_bytes(j);
-        for (;;) {
+        struct Bytes {
+          uint32_t b, q, d;
+        };
+        // bytes of alignment j at this lane's column (its last aligned base when the column is
+        // outside it) and the column's offset into the span
+        auto fetch = [&](int j) {
+          const uint32_t gpos = rdlane(ra.x, j);
+          const uint32_t len = rdlane(f_len, j) & 0xFFFFu;
+          Bytes x;
+          x.d = (uint32_t)p - gpos;
+          const uint32_t at = rdlane(f_at, j) + min(x.d, len - 1u);
+          x.b = P.bases[at];
+          x.q = P.quals[at];
+          return x;
+        };
+        auto next = [&]() {
+          if (!fm) return -1;
+          const int j = __builtin_ctzll(fm);
           fm &= fm - 1ull;
-          const int jn = fm ? __builtin_ctzll(fm) : -1;
-          const uint32_t bq = nb, qq = nq, dj = nd;
-          if (jn >= 0) load_bytes(jn);
-          // No deletion or insertion entries in a single-run alignment; the class is valid only
-          // for the codes of A, C, G and T.
+          return j;
+        };
+        auto add = [&](int j, const Bytes& x) {
+          const uint32_t bq = x.b, qq = x.q, dj = x.d;
           const uint32_t fr = rdlane(f_row, j);
           const uint32_t fl = rdlane(f_len, j);
           const uint32_t clo = rdlane(f_clo, j);
@@ -441,7 +455,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
           uniq += valid ? 1 : 0;
           const bool cand = valid && ((uint32_t)p - clo) < (fl >> 16);
           const bool acgt = ((0x116u >> min(bq, 16u)) & 1u) != 0u;
-          const uint32_t o_rs = (kLut >> ((2u * bq) & 31u)) & 3u;
+          const uint32_t o_rs = __builtin_amdgcn_ubfe(kLut, 2u * min(bq, 16u), 2u);
           const bool ok = cand && acgt && (qq - (uint32_t)qcut) < qspan;
           if (cand && !ok) {
             // not counted: quality below the cutoff, or an error (the reference asserts)
@@ -480,8 +494,19 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
               }
             }
           }
-          if (jn < 0) break;
-          j = jn;
+        };
+        // unrolled by two so that the bytes in flight alternate between two register sets
+        int j = next();
+        Bytes xa = fetch(j), xb;
+        for (;;) {
+          const int j1 = next();
+          if (j1 >= 0) xb = fetch(j1);
+          add(j, xa);
+          if (j1 < 0) break;
+          j = next();
+          if (j >= 0) xa = fetch(j);
+          add(j1, xb);
+          if (j < 0) break;
         }
       };
       fast_loop(std::integral_constant<bool, false>{}, __ballot(f_fast && !f_rev) & live);

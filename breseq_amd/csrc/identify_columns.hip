@@ -331,13 +331,17 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
     const int n4 = (P.fold_rows + 1) * (int)(sizeof(FoldRow) / 16);
     const uint4* src = (const uint4*)P.fold;
     uint4* dst = (uint4*)lds_rows;
-    for (int i = threadIdx.x; i < n4; i += blockDim.x) dst[i] = src[i];
+    // the all-zero last row goes first, so that "no term" is row 0
+    constexpr int kRow4 = (int)(sizeof(FoldRow) / 16);
+    for (int i = threadIdx.x; i < n4; i += blockDim.x) dst[(i + kRow4) % n4] = src[i];
     __syncthreads();
     tab.rows = lds_rows;
   } else {
     tab.rows = P.fold;
   }
-  const int zero_row = P.fold_rows;
+  // LDS rows are shifted by one (the zero row first); rows in HBM keep the zero row last
+  constexpr uint32_t kRowOff = kLds ? 1u : 0u;
+  const int zero_row = kLds ? 0 : P.fold_rows;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int64_t chunk = (P.n_windows + gridDim.x - 1) / gridDim.x;
   const int64_t w_begin = (int64_t)xcd_major_block() * chunk;
@@ -398,7 +402,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
       const uint32_t f_at = ra.z + ra.w + ra.x;
       const uint32_t f_clo = tr_lo - ra.w;
       const bool f_fast = (rc.z & 0xFFFFu) == 1u && (rc.z & (kReadSimple << 16)) && rc.y < (1u << 29);
-      const uint32_t f_row = f_fast ? ((rc.y << 2) | 2u) : 0u;
+      const uint32_t f_row = f_fast ? (((rc.y + kRowOff) << 2) | 2u) : 0u;
       // bytes of a single-run alignment at this lane's column (its last aligned base when the
       // column is outside it), and the column's offset into the span
       uint32_t nb = 0u, nq = 0u, nd = 0u;
@@ -459,7 +463,8 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
           const bool ok = cand && acgt && (qq - (uint32_t)qcut) < qspan;
           if (cand && !ok) {
             // not counted: quality below the cutoff, or an error (the reference asserts)
-            if (!acgt) err |= kErrBaseCode;
+            // (the base-code test restated, so that the common path keeps one mask of it)
+            if (__popc(bq) != 1 || bq > 8u) err |= kErrBaseCode;
             else if ((int)qq >= qcut && qq >= (uint32_t)Q) err |= kErrQuality;
           }
           const int row = ok ? (int)((fr >> 2) + qq * 5u + o_rs) : zero_row;
@@ -568,7 +573,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
         err |= (bad_base ? kErrBaseCode : 0u) | (bad_rs ? kErrReadSet : 0u) | (bad_q ? kErrQuality : 0u);
         ok = ok && !bad_rs && !bad_q;
         const uint32_t o_rs = (obs < 4u && !top) ? (obs ^ 3u) : obs;
-        const int row = ok ? (int)(row_base + q * 5u + o_rs) : zero_row;
+        const int row = ok ? (int)(row_base + kRowOff + q * 5u + o_rs) : zero_row;
         const FoldRow& T = tab[row];
         const float t0 = T.t[0], t1 = T.t[1], t2 = T.t[2], t3 = T.t[3], t4 = T.t[4];
         if (top) {

@@ -1,7 +1,9 @@
 """GPU parity on hand-built alignment sets covering every CIGAR shape the pileup distinguishes (soft
 clips, adjacent insertions/deletions, reference skips, pads, =/X runs, N bases, both strands,
 redundant reads, reads ending at a target end), against the oracle.  Integer outputs bit-exact;
-GD lines identical up to one unit in the last printed digit of FP fields."""
+GD lines identical up to one unit in the last printed digit of FP fields.  One read in ten has a
+random MAPQ, so the likelihood table has ~40 MAPQ slots (~8,400 rows) and the column kernel reads
+it from HBM; the simulated samples of the other GPU tests take the LDS-staged table."""
 import sys
 
 import numpy as np

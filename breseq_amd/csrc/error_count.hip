@@ -5,11 +5,14 @@
 // Both outputs are integer sums, so they do not depend on the order in which pileup entries are
 // visited.  Pass 1 therefore walks each alignment once (read-parallel) instead of walking columns:
 //
-//   k_ec_reads       one wavefront per alignment at a time; each lane takes four consecutive query
-//                    positions of an M/=/X run (dword loads of bases and qualities from 8-byte
-//                    aligned read starts, aligned dword loads of reference codes).
-//                    Covariates go to an LDS-privatized count table.  Lane 0 adds +1/-1 at the ends
-//                    of each aligned run to a per-column difference array (unique or redundant).
+//   k_ec_reads       16 consecutive alignments per work item.  First the item's counted
+//                    single-run alignments of at most 168 bases, three to a wavefront (21 lanes
+//                    each, eight query positions per lane); then its other alignments one per
+//                    wavefront, each lane four consecutive query positions of an M/=/X run.  Bases
+//                    and qualities are dword loads from 8-byte aligned read starts, reference codes
+//                    aligned dword loads.  Covariates go to an LDS-privatized count table.  The
+//                    first lane of an alignment adds +1/-1 at the ends of each aligned run to a
+//                    per-column difference array (unique or redundant).
 //   k_cov_tile_sums  phase 1 of the prefix scan of the difference arrays (per 4096-column tile)
 //   k_cov_tile_scan  phase 2: exclusive scan of the tile sums (one block)
 //   k_cov_hist       phase 3: per-column unique and redundant coverage -> histogram of the unique

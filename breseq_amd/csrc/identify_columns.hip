@@ -18,6 +18,9 @@
 // quality) read the all-zero table row instead of branching; the strand permutation of the
 // hypotheses is a wave-uniform branch; redundant alignments take a wave-uniform path that only
 // adds 1/X1.  The next alignment's record and bytes are loaded while the current one is added.
+// Within each batch of 64 alignments, the unique single-run ones go first through a lean loop per
+// strand (class, trimming and quality checks as table lookups and range tests), and the others
+// follow in file order, so the FP64 1/X1 sums keep the read order.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>

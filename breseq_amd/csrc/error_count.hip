@@ -5,7 +5,7 @@
 // Both outputs are integer sums, so they do not depend on the order in which pileup entries are
 // visited.  Pass 1 therefore walks each alignment once (read-parallel) instead of walking columns:
 //
-//   k_ec_reads       16 consecutive alignments per work item.  First the item's counted
+//   k_ec_reads       32 consecutive alignments per work item.  First the item's counted
 //                    single-run alignments of at most 168 bases, three to a wavefront (21 lanes
 //                    each, eight query positions per lane); then its other alignments one per
 //                    wavefront, each lane four consecutive query positions of an M/=/X run.  Bases
@@ -237,7 +237,7 @@ __device__ __forceinline__ void ec_group4(const EcParams& P, const uint8_t* __re
   }
 }
 
-constexpr int kEcChunk = 16;   // consecutive alignments per wavefront work item
+constexpr int kEcChunk = 32;   // consecutive alignments per wavefront work item
 constexpr int kEcGroup = 21;   // lanes per alignment when three share a wavefront (8 positions each)
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_ec_reads(EcParams P) {

@@ -7,29 +7,48 @@
  * summation order, glibc log10/pow, no FMA contraction: built with -ffp-contract=off), so that
  * its FP64 results are what the reference computes on the same inputs.
  *
- * Function                                   reference                           pinned by
+ * Function                                   reference                           pinned by (tests/...)
  * ---------------------------------------------------------------------------------------------------
- * resolve_entries (pileup column)            htslib resolve_cigar2 via           tests/test_oracle_pileup.py:
- *                                            pileup_base.cpp:141-210,308-359     hand-built CIGAR known answers
- * error_count callback / count position      error_count.cpp:125-199, 854-997    brute-force recount in tests
- * counts_to_log10_prob, table text           error_count.cpp:660-690,1005-1026   closed form (Laplace ratio)
- * base_qual_error_prob text                  error_count.cpp:697-785             closed form, column layout test
- * identify callback (coverage, pos_info)     identify_mutations.cpp:1309-1660    brute-force recount in tests
- * fill_read_base_likelihoods                 identify_mutations.cpp:3359-3384    closed form in tests
- * pure_genotype_call                         identify_mutations.cpp:3398-3433    log-sum-exp closed form
- * fit_allele_frequencies (EM)                identify_mutations.cpp:3240-3318    KKT/fixed point vs scipy
- * variant_presence_score, profile bounds     identify_mutations.cpp:3094-3217    invariants (drop = 0.587566)
- * fisher_exact_test_2x2                      stats.cpp:2144-2169                 fisher_strand_p_value of every RA
- *                                                                                line in tests/<name>/expected.gd
- * ks_test_two_sample_less                    stats.cpp:2190-2284                 scipy ks_2samp exact (tie-free)
- * check_deletion_completion (MC)             identify_mutations.cpp:2262-2344    hand-built coverage runs
- * update_unknown_intervals (UN)              identify_mutations.cpp:2972-3007    hand-built runs
- * GD sort / write                            genome_diff.cpp:685-830             expected.gd format round trip
- * SequenceTrims                              calculate_trims.cpp:67-138          brute force on tiny strings
- * discordant-pair regions (DP enter step,    identify_mutations.cpp:883-908,     hand-computed windows in
- *   check_discordant_completion, CSV)        1395-1457, 2358-2422, 2550-2573     tests/test_dp_regions.py
- * BAM2COV coverage table (table, rows,       coverage_output.cpp:190-283,        hand-computed table in
- *   region traversal and downsampling)       307-503; pileup_base.cpp:118-360    tests/test_coverage_table.py
+ * resolve_read / pileup columns              htslib resolve_cigar2 via           test_oracle_known_answers.py
+ *                                            pileup_base.cpp:141-210,308-359     test_pileup_*: hand-built CIGARs
+ * error_count callback, count position       error_count.cpp:125-199, 854-997    test_oracle_known_answers.py
+ *                                                                                test_error_count_*: counts by hand
+ * counts_to_log10_prob, error_rates.tab      error_count.cpp:660-690,1005-1026   test_error_count_log10_table_is_
+ *                                                                                laplace_ratio (closed form, %g)
+ * base_qual_error_prob text                  error_count.cpp:697-785             test_oracle_pins.py
+ *                                                                                test_base_qual_error_prob_*: by hand
+ * identify callback (pos_info, coverage,     identify_mutations.cpp:1309-1700    test_oracle_pins.py
+ *   insert-slot loop, slot qualities)                                            test_identify_column_tallies_by_hand,
+ *                                                                                test_identify_insert_slots_by_hand
+ * fill_read_base_likelihoods                 identify_mutations.cpp:3359-3384    test_oracle_pins.py
+ *                                                                                test_read_base_likelihoods_limits_
+ *                                                                                and_strand (MAPQ 0 / high, strands)
+ * pure_genotype_call                         identify_mutations.cpp:3398-3433    test_pure_genotype_closed_form
+ * fit_allele_frequencies (EM)                identify_mutations.cpp:3240-3318    test_em_fixed_point_satisfies_kkt,
+ *                                                                                test_em_respects_held_out_allele
+ * variant_presence_score, profile bounds     identify_mutations.cpp:3094-3217,   test_oracle_pins.py
+ *                                            3329-3344                           test_presence_score_and_bounds_
+ *                                                                                closed_form (scipy brentq)
+ * fisher_exact_test_2x2                      stats.cpp:2144-2169                 test_fisher_against_scipy;
+ *                                                                                test_reference_vectors.py (printed
+ *                                                                                fisher_strand_p_value of expected.gd)
+ * ks_test_two_sample_less                    stats.cpp:2190-2284                 test_oracle_pins.py
+ *                                                                                test_ks_less_ties_brute_force,
+ *                                                                                test_ks_less_asymptotic_branch;
+ *                                                                                test_ks_less_against_scipy_exact_
+ *                                                                                tie_free
+ * check_deletion_completion (MC),            identify_mutations.cpp:2117-2164,   test_oracle_pins.py
+ *   at_target_end whole-target MC            2262-2344                           test_mc_runs_by_hand,
+ *                                                                                test_mc_seed_cutoff_by_hand
+ * update_unknown_intervals (UN)              identify_mutations.cpp:2972-3007    test_un_runs_by_hand
+ * GD sort / write, number formats            genome_diff.cpp:685-830,            test_reference_vectors.py,
+ *                                            common.h:845-866                    test_host.py (printf, expected.gd)
+ * SequenceTrims                              calculate_trims.cpp:67-138          test_trims_*: known answers and
+ *                                                                                brute force on tiny strings
+ * discordant-pair regions (DP enter step,    identify_mutations.cpp:883-908,     test_dp_regions.py: hand-computed
+ *   check_discordant_completion, CSV)        1395-1457, 2358-2422, 2550-2573     windows
+ * BAM2COV coverage table (table, rows,       coverage_output.cpp:190-283,        test_coverage_table.py: hand-
+ *   region traversal and downsampling)       307-503; pileup_base.cpp:118-360    computed tables; golden tables
  *
  * Not pinned bit-for-bit: lgamma (Cephes lngamma in the reference, glibc lgamma here) -- Fisher /
  * KS p-values agree with the reference's printed 6 significant digits on the expected.gd vectors.
@@ -1762,6 +1781,63 @@ double orc_fit_alleles(const double* r, const double* lpmax, const int32_t* obs,
   for (int b = 0; b < 5; b++) f[b] = m.f[b];
   *iterations = (int32_t)m.iterations;
   return m.log10_likelihood;
+}
+
+int orc_presence_and_bounds(const double* r, const double* lpmax, const int32_t* obs, int64_t n, int32_t v,
+                            double tol, double log10_ref_length, double* f, double* out) {
+  bsq_identify_args args;
+  memset(&args, 0, sizeof(args));
+  args.polymorphism_precision_decimal = tol;
+  args.polymorphism_precision_places = 17;
+  orc::Identify im(&args);
+  im.log10_ref_length = log10_ref_length;
+  vector<orc::PData> pd((size_t)n);
+  for (int64_t i = 0; i < n; i++) {
+    for (int b = 0; b < 5; b++) pd[i].r[b] = r[i * 5 + b];
+    pd[i].lpmax = lpmax[i];
+    pd[i].base = orc::kIndexChar[obs[i]];
+  }
+  bool al[5] = {true, true, true, true, true};
+  orc::AlleleModel m = im.fit(pd, al);
+  for (int b = 0; b < 5; b++) f[b] = m.f[b];
+  out[0] = m.log10_likelihood;
+  out[1] = im.variant_presence_score(pd, m, v);
+  orc::Fields fl;
+  im.frequency_bounds(fl, pd, m, v);
+  out[2] = strtod(fl["frequency_lower"].c_str(), nullptr);
+  out[3] = strtod(fl["frequency_upper"].c_str(), nullptr);
+  return BSQ_OK;
+}
+
+int orc_read_base_likelihoods(const double* log10_table, int64_t n_bins, int32_t S, int32_t Q, uint32_t rs,
+                              uint32_t q, uint32_t obs, int32_t strand, int32_t mapq, double* lp, double* r,
+                              double* lpmax) {
+  try {
+    bsq_identify_args args;
+    memset(&args, 0, sizeof(args));
+    args.log10_prob_table = log10_table;
+    args.n_bins = n_bins;
+    args.read_set_max = S;
+    args.quality_max = Q;
+    orc::Identify im(&args);
+    im.prob.resize((size_t)n_bins);
+    for (int64_t i = 0; i < n_bins; i++) im.prob[(size_t)i] = pow(10, log10_table[i]);
+    orc::PData pd;
+    pd.rs = rs;
+    pd.q = q;
+    pd.obs = obs;
+    pd.strand = strand;
+    pd.mq = mapq;
+    im.fill_likelihoods(pd);
+    for (int b = 0; b < 5; b++) {
+      lp[b] = pd.lp[b];
+      r[b] = pd.r[b];
+    }
+    *lpmax = pd.lpmax;
+    return BSQ_OK;
+  } catch (const orc::orc_error& e) {
+    return e.code;
+  }
 }
 
 double orc_pure_genotype(const double* lps, uint32_t n, int32_t* best) {

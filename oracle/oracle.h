@@ -56,6 +56,20 @@ int orc_sequence_trims(const char* seq, int64_t len, uint8_t* out);
  * Writes f[5]; returns log10 likelihood; *iterations gets the iteration count. */
 double orc_fit_alleles(const double* r, const double* lpmax, const int32_t* obs, int64_t n, const int32_t* allowed,
                        double tol, double* f, int32_t* iterations);
+/* The scoring of one column's explicit per-read data (r[n*5], lpmax[n], obs[n] as for
+ * orc_fit_alleles) for variant allele v: the five-allele fit (f[5]), then out[0] = its log10
+ * likelihood, out[1] = variant_presence_score (identify_mutations.cpp:3329-3344, with
+ * log10_ref_length), out[2], out[3] = the frequency_lower / frequency_upper of
+ * write_RA_frequency_bounds (:3094-3217) at full printed precision. */
+int orc_presence_and_bounds(const double* r, const double* lpmax, const int32_t* obs, int64_t n, int32_t v,
+                            double tol, double log10_ref_length, double* f, double* out);
+/* fill_read_base_likelihoods (identify_mutations.cpp:3359-3384) for one read base: read file rs,
+ * quality q, observed base index obs (0..4, reference-strand space), strand +1 forward / -1
+ * reverse, mapping quality mapq, against the log10 table (n_bins = S*25*Q).  Writes lp[5], r[5]
+ * and *lpmax. */
+int orc_read_base_likelihoods(const double* log10_table, int64_t n_bins, int32_t S, int32_t Q, uint32_t rs,
+                              uint32_t q, uint32_t obs, int32_t strand, int32_t mapq, double* lp, double* r,
+                              double* lpmax);
 /* pure_genotype_call (identify_mutations.cpp:3398-3433): returns score, *best gets the index. */
 double orc_pure_genotype(const double* lps, uint32_t n, int32_t* best);
 

@@ -191,3 +191,40 @@ def coverage_table(ref, aln, args):
         return C.string_at(out, n.value).decode()
     finally:
         O.orc_free_text(out)
+
+
+def _bind_extra(O):
+    if getattr(O, "_extra_bound", False):
+        return O
+    O.orc_read_base_likelihoods.restype = C.c_int
+    O.orc_read_base_likelihoods.argtypes = [L.c_f64p, C.c_int64, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32,
+                                            C.c_uint32, C.c_int32, C.c_int32, L.c_f64p, L.c_f64p, L.c_f64p]
+    O.orc_presence_and_bounds.restype = C.c_int
+    O.orc_presence_and_bounds.argtypes = [L.c_f64p, L.c_f64p, L.c_i32p, C.c_int64, C.c_int32, C.c_double, C.c_double,
+                                          L.c_f64p, L.c_f64p]
+    O._extra_bound = True
+    return O
+
+
+def read_base_likelihoods(table, S, Q, rs, q, obs, strand, mapq):
+    """fill_read_base_likelihoods of one read base: (lp[5], r[5], lpmax)."""
+    O = _bind_extra(lib())
+    t = np.ascontiguousarray(table, dtype=np.float64)
+    lp, r, mx = np.zeros(5), np.zeros(5), C.c_double()
+    rc = O.orc_read_base_likelihoods(t.ctypes.data_as(L.c_f64p), t.size, S, Q, rs, q, obs, strand, mapq,
+                                     lp.ctypes.data_as(L.c_f64p), r.ctypes.data_as(L.c_f64p), C.byref(mx))
+    assert rc == 0, rc
+    return lp, r, mx.value
+
+
+def presence_and_bounds(r, lpmax, obs, v, tol=1e-6, log10_ref_length=0.0):
+    """(f[5], full-fit log10 likelihood, presence score, lower bound, upper bound) of one column."""
+    O = _bind_extra(lib())
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    lpmax = np.ascontiguousarray(lpmax, dtype=np.float64)
+    obs = np.ascontiguousarray(obs, dtype=np.int32)
+    f, out = np.zeros(5), np.zeros(4)
+    O.orc_presence_and_bounds(r.ctypes.data_as(L.c_f64p), lpmax.ctypes.data_as(L.c_f64p), obs.ctypes.data_as(L.c_i32p),
+                              len(obs), v, tol, log10_ref_length, f.ctypes.data_as(L.c_f64p),
+                              out.ctypes.data_as(L.c_f64p))
+    return f, out[0], out[1], out[2], out[3]

@@ -206,6 +206,9 @@ class SimConfig(C.Structure):
         ("mapq", C.c_int32),
         ("discordant_fraction", C.c_double),
         ("n_junctions", C.c_int32),
+        ("n_missing", C.c_int32),
+        ("missing_length", C.c_int32),
+        ("missing_edge_reads", C.c_int32),
     ]
 
 

@@ -43,10 +43,12 @@ class Sample:
     def simulate(cls, seed=1, ref_length=48502, gc=0.5, read_length=100, coverage=100.0, paired=False,
                  insert_mean=350.0, insert_sd=50.0, qual_mean=32.0, qual_sd=6.0, qual_min=2, qual_max=41,
                  n_snps=20, n_indels=2, indel_max=5, variant_freq_min=1.0, variant_freq_max=1.0,
-                 redundant_fraction=0.01, mapq=42, discordant_fraction=0.0, n_junctions=0):
+                 redundant_fraction=0.01, mapq=42, discordant_fraction=0.0, n_junctions=0, n_missing=0,
+                 missing_length=0, missing_edge_reads=0):
         cfg = L.SimConfig(seed, ref_length, gc, read_length, coverage, 1 if paired else 0, insert_mean, insert_sd,
                           qual_mean, qual_sd, qual_min, qual_max, n_snps, n_indels, indel_max, variant_freq_min,
-                          variant_freq_max, redundant_fraction, mapq, discordant_fraction, n_junctions)
+                          variant_freq_max, redundant_fraction, mapq, discordant_fraction, n_junctions, n_missing,
+                          missing_length, missing_edge_reads)
         h = C.c_void_p()
         st = _status()
         L.check(L.lib().bsq_simulate(C.byref(cfg), C.byref(h), C.byref(st)), st)

@@ -67,27 +67,37 @@ static bool repeat_at(const char* s, int64_t len, int64_t pos, int64_t size, int
   return true;
 }
 
+// SequenceTrims (calculate_trims.cpp:67-138) as a gather: each position's perfect-repeat extent
+// m[p] (at least 1, capped at 255), then for every x the largest right trim x - p + 1 and left trim
+// m[p] - (x - p) over the p in (x - 255, x] whose extent reaches x.  Both passes are split over host
+// threads; every output byte is written by one thread.
 void host_sequence_trims(const char* s, int64_t len, uint8_t* t) {
-  memset(t, 0, 2 * (size_t)len);
-  for (int64_t p = 0; p < len; p++) {
-    uint32_t mx = 1;
-    for (int64_t size = 1; size <= 18; size++) {
-      int64_t num = 1;
-      while (repeat_at(s, len, p, size, num)) num++;
-      if (num > 1) mx = std::max<uint32_t>(mx, (uint32_t)(num * size));
+  std::vector<uint8_t> m((size_t)len);
+  parallel_ranges(len, host_threads(), [&](int, int64_t p0, int64_t p1) {
+    for (int64_t p = p0; p < p1; p++) {
+      uint32_t mx = 1;
+      for (int64_t size = 1; size <= 18; size++) {
+        int64_t num = 1;
+        while (num * size <= 255 && repeat_at(s, len, p, size, num)) num++;
+        if (num > 1) mx = std::max<uint32_t>(mx, (uint32_t)(num * size));
+      }
+      m[(size_t)p] = (uint8_t)std::min<uint32_t>(mx, 255);
     }
-    if (mx > 255) mx = 255;
-    for (uint32_t o = 0; o < mx; o++) {
-      if (p + o >= len) break;
-      uint8_t v = (uint8_t)(o + 1);
-      if (v > t[len + p + o]) t[len + p + o] = v;
+  });
+  parallel_ranges(len, host_threads(), [&](int, int64_t x0, int64_t x1) {
+    for (int64_t x = x0; x < x1; x++) {
+      uint32_t right = 0, left = 0;
+      for (int64_t p = std::max<int64_t>(0, x - 254); p <= x; p++) {
+        const uint32_t mp = m[(size_t)p], o = (uint32_t)(x - p);
+        if (o < mp) {
+          right = std::max(right, o + 1);
+          left = std::max(left, mp - o);
+        }
+      }
+      t[x] = (uint8_t)left;
+      t[len + x] = (uint8_t)right;
     }
-    for (uint32_t o = 0; o < mx; o++) {
-      if (p + o >= len) break;
-      uint8_t v = (uint8_t)(mx - o);
-      if (v > t[p + o]) t[p + o] = v;
-    }
-  }
+  });
 }
 
 static char comp(char c) {

@@ -272,133 +272,214 @@ extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_sta
         junctions.push_back(J);
       }
     }
+    // missing-coverage regions: no fragment overlapping one is sequenced (a deletion in the sample
+    // genome); a few repeat-mapped reads (X1 = 2) reach into each region from both edges
+    struct Missing {
+      int64_t a, b;   // [a, b)
+    };
+    std::vector<Missing> missing;
+    if (cfg->n_missing > 0) {
+      if (cfg->missing_length < 1 || cfg->missing_length * (int64_t)cfg->n_missing > L / 4)
+        throw BsqError(BSQ_ERR_ARG, "missing-coverage regions must be non-empty and cover at most a quarter of the reference");
+      int guard = 0;
+      while ((int32_t)missing.size() < cfg->n_missing && guard < cfg->n_missing * 100) {
+        guard++;
+        const int64_t a = 1000 + (int64_t)rng.below((uint64_t)std::max<int64_t>(1, L - 2000 - cfg->missing_length));
+        const int64_t b = a + cfg->missing_length;
+        bool ok = true;
+        for (const Missing& m : missing)
+          if (a < m.b + 2000 && m.a < b + 2000) ok = false;
+        if (ok) missing.push_back({a, b});
+      }
+      std::sort(missing.begin(), missing.end(), [](const Missing& x, const Missing& y) { return x.a < y.a; });
+    }
+    auto in_missing = [&](int64_t a, int64_t b) {   // [a, b) overlaps a missing region
+      for (const Missing& m : missing)
+        if (a < m.b && m.a < b) return true;
+      return false;
+    };
     const double cov = cfg->coverage;
     int64_t n_frag = (int64_t)std::llround(cov * (double)L / (double)(RL * (paired ? 2 : 1)));
     if (n_frag < 1) n_frag = 1;
 
-    struct RawRead {
-      int64_t pos;
-      int64_t order;
-      std::vector<uint32_t> cigar;
-      std::string bases;
+    // Reads of a range of fragments, in fragment order, in flat buffers.  Each fragment draws from
+    // its own generator, so the reads do not depend on how the fragments are split over threads.
+    struct Reads {
+      std::vector<int64_t> pos, mate_pos;
+      std::vector<uint32_t> cig_off{0}, cig;
+      std::vector<char> bases;
       std::vector<uint8_t> q;
-      uint16_t flag;
-      uint8_t mapq;
-      int32_t x1;
-      int64_t mate_pos;
-      int32_t tlen;
-      int32_t file;
-      int8_t xp;
+      std::vector<uint16_t> flag;
+      std::vector<uint8_t> mapq;
+      std::vector<int32_t> x1, tlen, file;
+      std::vector<int8_t> xp;
+      void add(const Built& B, int32_t RL_, uint16_t fl, uint8_t mq, int32_t x1v, int64_t mpos, int32_t tl, int32_t f,
+               int8_t x) {
+        pos.push_back(B.pos);
+        cig.insert(cig.end(), B.cigar.begin(), B.cigar.end());
+        cig_off.push_back((uint32_t)cig.size());
+        bases.insert(bases.end(), B.bases.begin(), B.bases.begin() + RL_);
+        q.resize(q.size() + (size_t)RL_);
+        flag.push_back(fl);
+        mapq.push_back(mq);
+        x1.push_back(x1v);
+        mate_pos.push_back(mpos);
+        tlen.push_back(tl);
+        file.push_back(f);
+        xp.push_back(x);
+      }
+      size_t size() const { return pos.size(); }
     };
-    std::vector<RawRead> reads;
-    reads.reserve((size_t)n_frag * (paired ? 2 : 1));
-    std::vector<char> carry(vars.size(), 0);
-    Built b1, b2;
-    for (int64_t f = 0; f < n_frag; f++) {
-      Rng fr(cfg->seed ^ (0xA0761D6478BD642FULL * (uint64_t)(f + 1)));
-      int64_t flen = RL;
-      if (paired) {
-        double x = cfg->insert_mean + cfg->insert_sd * fr.normal();
-        flen = (int64_t)std::llround(x);
-        if (flen < RL) flen = RL;
-      }
-      if (flen + 16 >= L) continue;
-      int64_t start = (int64_t)fr.below((uint64_t)(L - flen - 8));
-      bool frag_rev = (fr.next() & 1) != 0;
-      const int32_t guard = 4;
-      int n_mates = paired ? 2 : 1;
-      auto first_var = [&](int64_t p) {
-        return (size_t)(std::lower_bound(vars.begin(), vars.end(), p, [](const Variant& v, int64_t x) { return v.pos < x; }) - vars.begin());
-      };
-      // discordant pair: R1 just before junction site a, R2 just after site b
-      const Junction* J = nullptr;
-      if (!junctions.empty() && fr.uniform() < cfg->discordant_fraction) J = &junctions[fr.below(junctions.size())];
-      bool ok;
-      bool rev0 = frag_rev, rev1 = !frag_rev;
-      if (J) {
-        const uint64_t spread = (uint64_t)std::max<int64_t>(1, (int64_t)cfg->insert_mean - RL);
-        const int64_t hi = L - RL - 64;
-        const int64_t s1 = std::min<int64_t>(hi, std::max<int64_t>(0, J->a - RL - (int64_t)fr.below(spread)));
-        const int64_t s2 = std::min<int64_t>(hi, std::max<int64_t>(0, J->b + (int64_t)fr.below(spread)));
-        if (J->orient == 0) { rev0 = false; rev1 = true; }
-        else if (J->orient == 1) { rev0 = true; rev1 = false; }
-        else { rev0 = rev1 = (fr.next() & 1) != 0; }
-        const size_t v1 = first_var(s1), v2 = first_var(s2);
-        for (size_t vi = v1; vi < vars.size() && vars[vi].pos < s1 + RL + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
-        ok = build_read(ref, vars, carry, v1, s1, RL, guard, b1);
-        for (size_t vi = v1; vi < vars.size() && vars[vi].pos < s1 + RL + 16; vi++) carry[vi] = 0;
-        for (size_t vi = v2; vi < vars.size() && vars[vi].pos < s2 + RL + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
-        ok = ok && build_read(ref, vars, carry, v2, s2, RL, guard, b2);
-        for (size_t vi = v2; vi < vars.size() && vars[vi].pos < s2 + RL + 16; vi++) carry[vi] = 0;
-      } else {
-        // haplotype: each variant in the fragment carried with its frequency
-        size_t vfirst = first_var(start);
-        for (size_t vi = vfirst; vi < vars.size() && vars[vi].pos < start + flen + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
-        ok = build_read(ref, vars, carry, vfirst, start, RL, guard, b1);
-        if (ok && paired) {
-          // mate 2 ends at start + flen - 1 on the reference (approximately, before indels)
-          int64_t s2 = start + flen - RL;
-          ok = build_read(ref, vars, carry, first_var(s2), s2, RL, guard, b2);
+    // per-base qualities and sequencing errors of the last added read, from generator fr
+    auto finish_read = [&](Reads& R, Rng& fr) {
+      char* bs = R.bases.data() + R.bases.size() - (size_t)RL;
+      uint8_t* qs = R.q.data() + R.q.size() - (size_t)RL;
+      for (int32_t i = 0; i < RL; i++) {
+        uint64_t u = fr.next();
+        double uq = (double)(u >> 40) * (1.0 / 16777216.0);
+        int q = qmin + (int)(std::lower_bound(qcdf.begin(), qcdf.end(), uq) - qcdf.begin());
+        if (q > qmax) q = qmax;
+        qs[i] = (uint8_t)q;
+        uint64_t ue = u & 0xFFFFFFFFULL;
+        if (ue < err_thr[q]) {
+          int c = chr_index(bs[i]);
+          int alt = (c + 1 + (int)((u >> 32) % 3)) & 3;
+          bs[i] = kChr[alt];
         }
-        for (size_t vi = vfirst; vi < vars.size() && vars[vi].pos < start + flen + 16; vi++) carry[vi] = 0;
+        if ((u >> 24 & 0xFFFF) == 7) {
+          bs[i] = 'N';
+          qs[i] = (uint8_t)std::max(qmin, 2);
+        }
       }
-      if (!ok) continue;
-      auto ref_end = [](const Built& B) {
-        int64_t e = B.pos;
-        for (uint32_t c : B.cigar)
-          if ((c & 0xf) == 0 || (c & 0xf) == 2) e += c >> 4;
-        return e;
-      };
-      // TLEN as mark_pair_info sets it: +distance on the leftmost mate, -distance on the other
-      const int64_t dist = paired ? (J ? std::max(ref_end(b1), ref_end(b2)) - std::min(b1.pos, b2.pos) : flen) : 0;
-      const bool m0_left = !paired || b1.pos <= b2.pos;
-      bool redundant = fr.uniform() < cfg->redundant_fraction;
-      int32_t x1v = redundant ? (2 + (int32_t)fr.below(2)) : 1;
-      for (int m = 0; m < n_mates; m++) {
-        const Built& B = (m == 0) ? b1 : b2;
-        RawRead rr;
-        rr.pos = B.pos;
-        rr.order = (int64_t)reads.size();
-        rr.cigar = B.cigar;
-        rr.bases = B.bases;
-        rr.q.resize(RL);
-        bool rev = paired ? ((m == 0) ? rev0 : rev1) : frag_rev;
-        for (int32_t i = 0; i < RL; i++) {
-          uint64_t u = fr.next();
-          double uq = (double)(u >> 40) * (1.0 / 16777216.0);
-          int q = qmin + (int)(std::lower_bound(qcdf.begin(), qcdf.end(), uq) - qcdf.begin());
-          if (q > qmax) q = qmax;
-          rr.q[i] = (uint8_t)q;
-          uint64_t ue = u & 0xFFFFFFFFULL;
-          if (ue < err_thr[q]) {
-            int c = chr_index(rr.bases[i]);
-            int alt = (c + 1 + (int)((u >> 32) % 3)) & 3;
-            rr.bases[i] = kChr[alt];
+    };
+    auto first_var = [&](int64_t p) {
+      return (size_t)(std::lower_bound(vars.begin(), vars.end(), p, [](const Variant& v, int64_t x) { return v.pos < x; }) - vars.begin());
+    };
+    auto ref_end = [](const Built& B) {
+      int64_t e = B.pos;
+      for (uint32_t c : B.cigar)
+        if ((c & 0xf) == 0 || (c & 0xf) == 2) e += c >> 4;
+      return e;
+    };
+    const int n_chunks = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads() * 4, n_frag / 1024 + 1));
+    std::vector<Reads> chunks((size_t)n_chunks);
+    parallel_ranges(n_chunks, host_threads(), [&](int, int64_t c0, int64_t c1) {
+      std::vector<char> carry(vars.size(), 0);
+      Built b1, b2;
+      for (int64_t ch = c0; ch < c1; ch++) {
+        Reads& R = chunks[(size_t)ch];
+        const int64_t f0 = n_frag * ch / n_chunks, f1 = n_frag * (ch + 1) / n_chunks;
+        for (int64_t f = f0; f < f1; f++) {
+          Rng fr(cfg->seed ^ (0xA0761D6478BD642FULL * (uint64_t)(f + 1)));
+          int64_t flen = RL;
+          if (paired) {
+            double x = cfg->insert_mean + cfg->insert_sd * fr.normal();
+            flen = (int64_t)std::llround(x);
+            if (flen < RL) flen = RL;
           }
-          if ((u >> 24 & 0xFFFF) == 7) { rr.bases[i] = 'N'; rr.q[i] = (uint8_t)std::max(qmin, 2); }
+          if (flen + 16 >= L) continue;
+          int64_t start = (int64_t)fr.below((uint64_t)(L - flen - 8));
+          bool frag_rev = (fr.next() & 1) != 0;
+          const int32_t guard = 4;
+          int n_mates = paired ? 2 : 1;
+          // discordant pair: R1 just before junction site a, R2 just after site b
+          const Junction* J = nullptr;
+          if (!junctions.empty() && fr.uniform() < cfg->discordant_fraction) J = &junctions[fr.below(junctions.size())];
+          bool ok;
+          bool rev0 = frag_rev, rev1 = !frag_rev;
+          if (J) {
+            const uint64_t spread = (uint64_t)std::max<int64_t>(1, (int64_t)cfg->insert_mean - RL);
+            const int64_t hi = L - RL - 64;
+            const int64_t s1 = std::min<int64_t>(hi, std::max<int64_t>(0, J->a - RL - (int64_t)fr.below(spread)));
+            const int64_t s2 = std::min<int64_t>(hi, std::max<int64_t>(0, J->b + (int64_t)fr.below(spread)));
+            if (J->orient == 0) { rev0 = false; rev1 = true; }
+            else if (J->orient == 1) { rev0 = true; rev1 = false; }
+            else { rev0 = rev1 = (fr.next() & 1) != 0; }
+            const size_t v1 = first_var(s1), v2 = first_var(s2);
+            for (size_t vi = v1; vi < vars.size() && vars[vi].pos < s1 + RL + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
+            ok = build_read(ref, vars, carry, v1, s1, RL, guard, b1);
+            for (size_t vi = v1; vi < vars.size() && vars[vi].pos < s1 + RL + 16; vi++) carry[vi] = 0;
+            for (size_t vi = v2; vi < vars.size() && vars[vi].pos < s2 + RL + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
+            ok = ok && build_read(ref, vars, carry, v2, s2, RL, guard, b2);
+            for (size_t vi = v2; vi < vars.size() && vars[vi].pos < s2 + RL + 16; vi++) carry[vi] = 0;
+          } else {
+            // haplotype: each variant in the fragment carried with its frequency
+            size_t vfirst = first_var(start);
+            for (size_t vi = vfirst; vi < vars.size() && vars[vi].pos < start + flen + 16; vi++) carry[vi] = fr.uniform() < vars[vi].freq;
+            ok = build_read(ref, vars, carry, vfirst, start, RL, guard, b1);
+            if (ok && paired) {
+              // mate 2 ends at start + flen - 1 on the reference (approximately, before indels)
+              int64_t s2 = start + flen - RL;
+              ok = build_read(ref, vars, carry, first_var(s2), s2, RL, guard, b2);
+            }
+            for (size_t vi = vfirst; vi < vars.size() && vars[vi].pos < start + flen + 16; vi++) carry[vi] = 0;
+          }
+          if (!ok) continue;
+          if (!missing.empty()) {
+            if (in_missing(b1.pos, ref_end(b1)) || (paired && in_missing(b2.pos, ref_end(b2)))) continue;
+            if (!J && in_missing(start, start + flen)) continue;
+          }
+          // TLEN as mark_pair_info sets it: +distance on the leftmost mate, -distance on the other
+          const int64_t dist = paired ? (J ? std::max(ref_end(b1), ref_end(b2)) - std::min(b1.pos, b2.pos) : flen) : 0;
+          const bool m0_left = !paired || b1.pos <= b2.pos;
+          bool redundant = fr.uniform() < cfg->redundant_fraction;
+          int32_t x1v = redundant ? (2 + (int32_t)fr.below(2)) : 1;
+          for (int m = 0; m < n_mates; m++) {
+            const Built& B = (m == 0) ? b1 : b2;
+            bool rev = paired ? ((m == 0) ? rev0 : rev1) : frag_rev;
+            uint16_t flag = rev ? 0x10 : 0;
+            if (paired) {
+              flag |= 0x1 | (J ? 0 : 0x2) | ((m == 0) ? 0x40 : 0x80);
+              bool mrev = (m == 0) ? rev1 : rev0;
+              if (mrev) flag |= 0x20;
+            }
+            R.add(B, RL, flag, redundant ? (uint8_t)1 : (uint8_t)cfg->mapq, x1v,
+                  paired ? ((m == 0) ? b2.pos : b1.pos) : -1, paired ? (int32_t)(((m == 0) == m0_left) ? dist : -dist) : 0,
+                  paired ? m : 0, paired ? (J ? J->orient : (int8_t)0) : (int8_t)-1);
+            finish_read(R, fr);
+          }
         }
-        uint16_t flag = rev ? 0x10 : 0;
-        if (paired) {
-          flag |= 0x1 | (J ? 0 : 0x2) | ((m == 0) ? 0x40 : 0x80);
-          bool mrev = (m == 0) ? rev1 : rev0;
-          if (mrev) flag |= 0x20;
-        }
-        rr.flag = flag;
-        rr.mapq = redundant ? 1 : (uint8_t)cfg->mapq;
-        rr.x1 = x1v;
-        rr.file = paired ? m : 0;
-        rr.mate_pos = paired ? ((m == 0) ? b2.pos : b1.pos) : -1;
-        rr.tlen = paired ? (int32_t)(((m == 0) == m0_left) ? dist : -dist) : 0;
-        rr.xp = paired ? (J ? J->orient : (int8_t)0) : (int8_t)-1;
-        reads.push_back(std::move(rr));
+      }
+    });
+    // the repeat-mapped reads at the missing regions' edges: unpaired, X1 = 2, reference bases
+    Reads edge;
+    for (size_t k = 0; k < missing.size(); k++) {
+      Rng er(cfg->seed ^ (0x9FB21C651E98DF25ULL * (uint64_t)(k + 1)));
+      for (int32_t j = 0; j < 2 * cfg->missing_edge_reads; j++) {
+        const bool left = j < cfg->missing_edge_reads;
+        const int64_t reach = 10 + (int64_t)er.below(31);   // bases inside the region
+        int64_t s = left ? missing[k].a + reach - RL : missing[k].b - reach;
+        s = std::max<int64_t>(0, std::min<int64_t>(s, L - RL));
+        Built B;
+        B.pos = s;
+        B.cigar.assign(1, ((uint32_t)RL << 4) | 0u);
+        B.bases.assign(ref, (size_t)s, (size_t)RL);
+        edge.add(B, RL, (er.next() & 1) ? 0x10 : 0, 1, 2, -1, 0, 0, (int8_t)-1);
+        finish_read(edge, er);
       }
     }
-    std::stable_sort(reads.begin(), reads.end(), [](const RawRead& a, const RawRead& b) { return a.pos < b.pos; });
+    chunks.push_back(std::move(edge));
+    // concatenation in fragment order, then a stable counting sort by position
+    std::vector<int64_t> chunk_base(chunks.size() + 1, 0);
+    for (size_t c = 0; c < chunks.size(); c++) chunk_base[c + 1] = chunk_base[c] + (int64_t)chunks[c].size();
+    const size_t nr = (size_t)chunk_base.back();
+    std::vector<int32_t> src_chunk(nr), src_idx(nr);
+    std::vector<int64_t> bucket((size_t)L + 1, 0);
+    for (size_t c = 0; c < chunks.size(); c++)
+      for (int64_t p : chunks[c].pos) bucket[(size_t)p + 1]++;
+    for (int64_t p = 0; p < L; p++) bucket[(size_t)p + 1] += bucket[(size_t)p];
+    std::vector<int64_t> order_of(nr);   // file-order read index -> generation order
+    for (size_t c = 0; c < chunks.size(); c++)
+      for (size_t i = 0; i < chunks[c].size(); i++) {
+        const int64_t dst = bucket[(size_t)chunks[c].pos[i]]++;
+        src_chunk[(size_t)dst] = (int32_t)c;
+        src_idx[(size_t)dst] = (int32_t)i;
+        order_of[(size_t)dst] = chunk_base[c] + (int64_t)i;
+      }
     // trims for XL/XR (calculate_trims.cpp:239-285 + alignment.cpp:920-946)
     std::vector<uint8_t> gtrims(2 * (size_t)L);
     host_sequence_trims(ref.data(), L, gtrims.data());
     S.seqs.push_back(ref);
-    size_t nr = reads.size();
     S.tid.assign(nr, 0);
     S.pos.resize(nr);
     S.flag.resize(nr);
@@ -412,41 +493,50 @@ extern "C" int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_sta
     S.tlen.resize(nr);
     S.xp.resize(nr);
     S.read_names.resize(nr);
-    S.cigar_off.assign(1, 0);
-    S.seq_off.assign(1, 0);
-    S.seq.reserve(nr * RL);
-    S.qual.reserve(nr * RL);
+    S.cigar_off.assign(nr + 1, 0);
+    S.seq_off.assign(nr + 1, 0);
     for (size_t i = 0; i < nr; i++) {
-      RawRead& r = reads[i];
-      S.pos[i] = (int32_t)r.pos;
-      S.flag[i] = r.flag;
-      S.mapq[i] = r.mapq;
-      S.x1[i] = r.x1;
-      S.mate_pos[i] = (int32_t)r.mate_pos;
-      S.tlen[i] = r.tlen;
-      S.xp[i] = r.xp;
-      S.read_names[i] = std::to_string(r.file + 1) + ":" + std::to_string(r.order / (paired ? 2 : 1) + 1);
-      for (uint32_t c : r.cigar) S.cigar.push_back(c);
-      S.cigar_off.push_back((int64_t)S.cigar.size());
-      for (int32_t k = 0; k < RL; k++) {
-        int c = chr_index(r.bases[k]);
-        S.seq.push_back(c < 0 ? 15 : kBam[c]);
-        S.qual.push_back(r.q[k]);
-      }
-      S.seq_off.push_back((int64_t)S.seq.size());
-      // alignment trims: max(genomic, read-edge) + query offset
-      int64_t rend = r.pos;
-      for (uint32_t c : r.cigar) {
-        uint32_t op = c & 0xf;
-        if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) rend += c >> 4;
-      }
-      uint32_t etl = 1, etr = 1;
-      host_edge_trims(r.bases, etl, etr);
-      uint32_t gl = gtrims[r.pos], gr = gtrims[L + (rend - 1)];
-      S.xl[i] = (int32_t)std::max(gl, etl);      // query_start_0 == 0 (no soft clips)
-      S.xr[i] = (int32_t)std::max(gr, etr);      // read_length - query_end_1 == 0
-      S.rg[i] = 0;
+      const Reads& R = chunks[(size_t)src_chunk[i]];
+      const size_t k = (size_t)src_idx[i];
+      S.cigar_off[i + 1] = S.cigar_off[i] + (int64_t)(R.cig_off[k + 1] - R.cig_off[k]);
+      S.seq_off[i + 1] = S.seq_off[i] + RL;
     }
+    S.cigar.resize((size_t)S.cigar_off[nr]);
+    S.seq.resize((size_t)S.seq_off[nr]);
+    S.qual.resize((size_t)S.seq_off[nr]);
+    parallel_ranges((int64_t)nr, host_threads(), [&](int, int64_t i0, int64_t i1) {
+      std::string bases;
+      for (int64_t i = i0; i < i1; i++) {
+        const Reads& R = chunks[(size_t)src_chunk[(size_t)i]];
+        const size_t k = (size_t)src_idx[(size_t)i];
+        S.pos[i] = (int32_t)R.pos[k];
+        S.flag[i] = R.flag[k];
+        S.mapq[i] = R.mapq[k];
+        S.x1[i] = R.x1[k];
+        S.mate_pos[i] = (int32_t)R.mate_pos[k];
+        S.tlen[i] = R.tlen[k];
+        S.xp[i] = R.xp[k];
+        S.read_names[i] = std::to_string(R.file[k] + 1) + ":" + std::to_string(order_of[(size_t)i] / (paired ? 2 : 1) + 1);
+        std::copy(R.cig.begin() + R.cig_off[k], R.cig.begin() + R.cig_off[k + 1], S.cigar.begin() + S.cigar_off[i]);
+        bases.assign(R.bases.data() + k * (size_t)RL, (size_t)RL);
+        for (int32_t j = 0; j < RL; j++) {
+          int c = chr_index(bases[j]);
+          S.seq[(size_t)S.seq_off[i] + j] = c < 0 ? 15 : kBam[c];
+          S.qual[(size_t)S.seq_off[i] + j] = R.q[k * (size_t)RL + j];
+        }
+        // alignment trims: max(genomic, read-edge) + query offset
+        int64_t rend = R.pos[k];
+        for (uint32_t c = R.cig_off[k]; c < R.cig_off[k + 1]; c++) {
+          uint32_t op = R.cig[c] & 0xf;
+          if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) rend += R.cig[c] >> 4;
+        }
+        uint32_t etl = 1, etr = 1;
+        host_edge_trims(bases, etl, etr);
+        uint32_t gl = gtrims[R.pos[k]], gr = gtrims[L + (rend - 1)];
+        S.xl[i] = (int32_t)std::max(gl, etl);      // query_start_0 == 0 (no soft clips)
+        S.xr[i] = (int32_t)std::max(gr, etr);      // read_length - query_end_1 == 0
+      }
+    });
     S.rg_ids.push_back("0");
     S.rg_libs.push_back(paired ? "reads" : "reads_1");
     S.rf_base.push_back(0);

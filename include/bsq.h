@@ -373,6 +373,12 @@ typedef struct bsq_sim_config {
    * proper-pair flag whose R2 lies at one of n_junctions far sites, in orientation FR, RF or FF. */
   double discordant_fraction;
   int32_t n_junctions;
+  /* Missing coverage: n_missing regions of missing_length bp that no fragment covers (a deletion in
+   * the sample genome), each reached from both edges by missing_edge_reads repeat-mapped reads
+   * (X1 = 2) that extend 10-40 bp into it. */
+  int32_t n_missing;
+  int32_t missing_length;
+  int32_t missing_edge_reads;
 } bsq_sim_config;
 
 int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_status* status);

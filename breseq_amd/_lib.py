@@ -165,6 +165,29 @@ class Timing(C.Structure):
         ("n_reads", C.c_int64),
         ("n_positions", C.c_int64),
         ("kernel_detail_ms", C.c_double * 8),
+        ("fit_ms", C.c_double),
+        ("fit_wait_ms", C.c_double),
+        ("pipeline_ec_ms", C.c_double),
+        ("pipeline_ec_kernel_ms", C.c_double * 2),
+    ]
+
+
+class CoverageSummary(C.Structure):
+    _fields_ = [(k, C.c_double) for k in (
+        "deletion_coverage_propagation_cutoff", "nbinom_size_parameter", "nbinom_mean_parameter",
+        "nbinom_prob_parameter", "nbinom_variance", "nbinom_relative_variance", "average", "variance",
+        "relative_variance")]
+
+
+class CoverageStageArgs(C.Structure):
+    _fields_ = [
+        ("n_targets", C.c_int32),
+        ("target_lengths", C.POINTER(C.c_int64)),
+        ("coverage_group", C.POINTER(C.c_int32)),
+        ("n_coverage_groups", C.c_int32),
+        ("histograms", C.POINTER(C.POINTER(C.c_double))),
+        ("histogram_lengths", C.POINTER(C.c_int64)),
+        ("targeted_sequencing", C.c_int32),
     ]
 
 
@@ -280,6 +303,11 @@ def lib():
           [C.c_int32, C.c_uint64, C.c_uint64, P(C.c_uint64), P(C.c_uint64), st])
     _decl(L, "bsq_format_double", C.c_int32, [C.c_double, C.c_uint32, C.c_int32, C.c_char_p, C.c_int32])
     _decl(L, "bsq_fit_coverage_distribution", C.c_int, [c_u32p, C.c_int64, C.c_double, P(CoverageFit), st])
+    _decl(L, "bsq_analyze_unique_coverage_distributions", C.c_int, [P(CoverageStageArgs), P(CoverageSummary), st])
+    _decl(L, "bsq_read_coverage_histogram", C.c_int, [C.c_char_p, C.c_int64, c_f64p, C.c_int64, c_i64p, st])
+    _decl(L, "bsq_session_pipeline", C.c_int,
+          [vp, P(ErrorCountArgs), P(CoverageStageArgs), C.c_double, C.c_double, P(IdentifyArgs), P(vp),
+           P(CoverageSummary), P(vp), st])
     _LIB = L
     return L
 

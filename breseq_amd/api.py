@@ -220,6 +220,28 @@ class Session:
     def identify_mutations(self, **kw):
         return identify_mutations(self.sample, session=self, **kw)
 
+    def pipeline(self, mode=None, dp=None, coverage_groups=None, targeted_sequencing=False,
+                 propagation_cutoff_option=0.0, seed_cutoff_option=0.0, ec_kw=None, **im_kw):
+        """error_count -> coverage-distribution stage -> identify_mutations on the loaded sample, as
+        breseq runs stages 07-08 (bsq_session_pipeline).  Returns (ErrorCountResult, per-target
+        coverage summaries, GD text, DP_candidate_regions.csv text)."""
+        s = self.sample
+        n = s.n_targets
+        ec_args = make_error_count_args(s, coverage_groups=coverage_groups, **(ec_kw or {}))
+        cov = coverage_stage_args(s.target_lengths(), coverage_groups, None, targeted_sequencing)
+        im_args = make_identify_args(s, np.zeros(1), ec_args.read_set_max, ec_args.quality_max, [0.0] * n,
+                                     mode=mode if mode is not None else CONSENSUS, dp=dp, **im_kw)
+        summ = (L.CoverageSummary * n)()
+        ec_res = C.c_void_p()
+        im_res = C.c_void_p()
+        st = _status()
+        L.check(L.lib().bsq_session_pipeline(self._h, C.byref(ec_args), C.byref(cov), float(propagation_cutoff_option),
+                                             float(seed_cutoff_option), C.byref(im_args), C.byref(ec_res), summ,
+                                             C.byref(im_res), C.byref(st)), st)
+        ec = _ec_result(ec_res, ec_args)
+        gd, dp_text = _im_texts(im_res)
+        return ec, [_summary_dict(x) for x in summ], gd, dp_text
+
     def coverage_table(self, target=0, start=0, end=0, resolution=600, total_only=False, per_read_group=False,
                        csv=False, reference_average=None, n_read_groups=None):
         """BAM2COV table of one region (coverage_output::table, coverage_output.cpp:190-283) from the
@@ -323,6 +345,12 @@ def error_count(sample, session=None, **kw):
         L.check(lib.bsq_session_error_count(session._h, C.byref(args), C.byref(res), C.byref(st)), st)
     else:
         L.check(lib.bsq_error_count(C.byref(args), C.byref(res), C.byref(st)), st)
+    return _ec_result(res, args)
+
+
+def _ec_result(res, args):
+    """ErrorCountResult of a bsq_ec_result handle, which it releases."""
+    lib = L.lib()
     try:
         hists = []
         for gi in range(args.n_coverage_groups):
@@ -404,6 +432,13 @@ def identify_mutations(sample, log10_prob, read_set_max, quality_max, propagatio
         if not screen:
             raise ValueError("screen=False needs a session")
         L.check(lib.bsq_identify_mutations(C.byref(args), C.byref(res), C.byref(st)), st)
+    text, dp_text = _im_texts(res)
+    return (text, dp_text) if with_dp else text
+
+
+def _im_texts(res):
+    """(GD text, DP CSV text) of a bsq_im_result handle, which it releases."""
+    lib = L.lib()
     try:
         p = C.c_char_p()
         n = C.c_int64()
@@ -413,7 +448,7 @@ def identify_mutations(sample, log10_prob, read_set_max, quality_max, propagatio
         dp_text = C.string_at(p, n.value).decode()
     finally:
         lib.bsq_im_result_free(res)
-    return (text, dp_text) if with_dp else text
+    return text, dp_text
 
 
 def ra_test_settings(polymorphism_prediction=False, **overrides):
@@ -456,6 +491,53 @@ def fit_coverage_distribution(hist, sequence_length):
     L.check(L.lib().bsq_fit_coverage_distribution(h.ctypes.data_as(L.c_u32p), h.size, float(sequence_length),
                                                   C.byref(out), C.byref(st)), st)
     return {k: getattr(out, k) for k, _ in L.CoverageFit._fields_}
+
+
+def _summary_dict(x):
+    return {k: getattr(x, k) for k, _ in L.CoverageSummary._fields_}
+
+
+def coverage_stage_args(target_lengths, coverage_groups=None, histograms=None, targeted_sequencing=False):
+    n = len(target_lengths)
+    groups = list(coverage_groups) if coverage_groups is not None else [0] * n
+    G = max(groups) + 1 if groups else 1
+    lens = (C.c_int64 * n)(*[int(x) for x in target_lengths])
+    g = (C.c_int32 * n)(*groups)
+    hs = [np.ascontiguousarray(h, dtype=np.float64) for h in (histograms or [])]
+    hp = (L.c_f64p * max(G, 1))(*[h.ctypes.data_as(L.c_f64p) for h in hs]) if hs else None
+    hl = (C.c_int64 * max(G, 1))(*[h.size for h in hs]) if hs else None
+    a = L.CoverageStageArgs()
+    a.n_targets = n
+    a.target_lengths = lens
+    a.coverage_group = g
+    a.n_coverage_groups = G
+    a.histograms = hp
+    a.histogram_lengths = hl
+    a.targeted_sequencing = 1 if targeted_sequencing else 0
+    a._keep = (lens, g, hs, hp, hl)
+    return a
+
+
+def analyze_unique_coverage_distributions(target_lengths, histograms, coverage_groups=None, targeted_sequencing=False):
+    """CoverageDistribution::analyze_unique_coverage_distributions (coverage_distribution.cpp:510-633):
+    per-target summaries from per-group histograms (n[0..max], rows j >= 1)."""
+    a = coverage_stage_args(target_lengths, coverage_groups, histograms, targeted_sequencing)
+    out = (L.CoverageSummary * len(target_lengths))()
+    st = _status()
+    L.check(L.lib().bsq_analyze_unique_coverage_distributions(C.byref(a), out, C.byref(st)), st)
+    return [_summary_dict(x) for x in out]
+
+
+def read_coverage_histogram(text):
+    """read_coverage_histogram (coverage_distribution.cpp:34-65) of a coverage distribution file's text."""
+    b = text.encode()
+    n = C.c_int64()
+    st = _status()
+    L.check(L.lib().bsq_read_coverage_histogram(b, len(b), None, 0, C.byref(n), C.byref(st)), st)
+    out = np.zeros(n.value, dtype=np.float64)
+    L.check(L.lib().bsq_read_coverage_histogram(b, len(b), out.ctypes.data_as(L.c_f64p), out.size, C.byref(n),
+                                                C.byref(st)), st)
+    return out
 
 
 def error_rates_text_to_table(text):

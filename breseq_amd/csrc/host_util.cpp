@@ -4,11 +4,115 @@
 #include <charconv>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
 
 namespace bsq {
+
+namespace {
+
+// One submitted job: tasks are claimed through `next`; the submitter waits for `finished`.
+struct PoolJob {
+  int n = 0;
+  const std::function<void(int)>* fn = nullptr;
+  std::atomic<int> next{0};
+  std::atomic<int> finished{0};
+  std::mutex m;
+  std::condition_variable cv;
+  int err_code = 0;
+  std::string err_msg;
+
+  // Claims and runs tasks until none is left; returns the number run.
+  int drain() {
+    int ran = 0;
+    for (int i = next++; i < n; i = next++) {
+      try {
+        (*fn)(i);
+      } catch (const BsqError& e) {
+        std::lock_guard<std::mutex> g(m);
+        if (!err_code) {
+          err_code = e.code;
+          err_msg = e.what();
+        }
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> g(m);
+        if (!err_code) {
+          err_code = BSQ_ERR_FORMAT;
+          err_msg = e.what();
+        }
+      }
+      ran++;
+    }
+    return ran;
+  }
+  void add_finished(int k) {
+    if (k == 0) return;
+    if (finished.fetch_add(k) + k == n) {
+      std::lock_guard<std::mutex> g(m);
+      cv.notify_all();
+    }
+  }
+};
+
+class Pool {
+ public:
+  static Pool& get() {
+    static Pool* p = new Pool();   // never destroyed: workers may outlive static destructors
+    return *p;
+  }
+  void run(int n_tasks, const std::function<void(int)>& fn) {
+    if (n_tasks <= 0) return;
+    auto job = std::make_shared<PoolJob>();
+    job->n = n_tasks;
+    job->fn = &fn;
+    if (!workers_.empty() && n_tasks > 1) {
+      std::lock_guard<std::mutex> g(m_);
+      jobs_.push_back(job);
+      cv_.notify_all();
+    }
+    job->add_finished(job->drain());
+    {
+      std::unique_lock<std::mutex> g(job->m);
+      job->cv.wait(g, [&] { return job->finished.load() == job->n; });
+    }
+    if (job->err_code) throw BsqError(job->err_code, job->err_msg);
+  }
+
+ private:
+  Pool() {
+    const int n = host_threads() - 1;
+    for (int i = 0; i < n; i++) workers_.emplace_back([this] { loop(); });
+    for (auto& t : workers_) t.detach();
+  }
+  void loop() {
+    for (;;) {
+      std::shared_ptr<PoolJob> job;
+      {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] {
+          while (!jobs_.empty() && jobs_.front()->next.load() >= jobs_.front()->n) jobs_.pop_front();
+          return !jobs_.empty();
+        });
+        job = jobs_.front();
+      }
+      job->add_finished(job->drain());
+    }
+  }
+  std::mutex m_;
+  std::condition_variable cv_;
+  std::deque<std::shared_ptr<PoolJob>> jobs_;
+  std::vector<std::thread> workers_;
+};
+
+}  // namespace
+
+void pool_run(int n_tasks, const std::function<void(int)>& fn) { Pool::get().run(n_tasks, fn); }
 
 void set_status(bsq_status* st, int code, const std::string& msg) {
   if (!st) return;

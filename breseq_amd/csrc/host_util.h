@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -60,6 +61,12 @@ void parallel_ranges(int64_t n, int threads, F fn) {
   for (int t = 0; t < threads; t++)
     if (codes[t]) throw BsqError(codes[t], errors[t]);
 }
+
+// Runs fn(i) for every task i in [0, n_tasks) on a pool of host threads that lives for the
+// process (host_threads() - 1 workers plus the calling thread), and returns when every task has
+// ended.  Several callers may submit at once: each takes part in its own job, so a busy pool
+// never blocks one.  An exception in a task is rethrown here as a BsqError.
+void pool_run(int n_tasks, const std::function<void(int)>& fn);
 
 // SequenceTrims (calculate_trims.cpp:67-138): out[0..len) left, out[len..2len) right.
 void host_sequence_trims(const char* seq, int64_t len, uint8_t* out);

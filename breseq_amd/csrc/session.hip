@@ -21,6 +21,7 @@
 #include "dp_regions.h"
 #include "engine.h"
 #include "evidence.h"
+#include "covfit.h"
 #include "host_util.h"
 #include "kernels.h"
 #include "sample.h"
@@ -86,6 +87,7 @@ struct Session {
   struct {
     int32_t cand = 0, ins = 0, runs = 0, fix = 0;
   } id_caps;
+  std::vector<TargetInfo> h_targets;  // per-target parameters of the last identify (async copy source)
   std::vector<int32_t> h_gpos;        // global first column of each packed read (sorted)
   std::vector<int32_t> h_gend_max;    // running maximum of the packed reads' gend, in read order
   // device buffers
@@ -428,6 +430,8 @@ namespace bsq {
 
 static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec_result& out) {
   if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
+  S.timing.fit_ms = S.timing.fit_wait_ms = S.timing.pipeline_ec_ms = 0.0;
+  S.timing.pipeline_ec_kernel_ms[0] = S.timing.pipeline_ec_kernel_ms[1] = 0.0;
   const double th0 = now_ms();
   const int Sm = args->read_set_max, Q = args->quality_max;
   if (Sm <= 0 || Q <= 0 || Sm > 255 || Q > 256) throw BsqError(BSQ_ERR_ARG, "covariate maxima out of range");
@@ -620,9 +624,17 @@ static void build_likelihood_tables(const Session& S, const bsq_identify_args* a
 // One identify pass.  Returns false, with the session's minimum capacities raised, when one of the
 // device result lists (candidates, insert columns, MC/UN runs, columns to re-decide) was too small;
 // the caller runs the pass again.
-static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_result& out, int screen_off) {
+// When `late` is given, a->deletion_propagation_cutoff holds only each target's sign (whether the
+// coverage fit failed: the column pass needs no more); the values come from `late`, which is
+// waited for just before the MC runs are found, so a coverage fit running on host threads
+// overlaps the column pass.
+using LateCutoffs = std::shared_future<std::vector<double>>;
+
+static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_result& out, int screen_off,
+                              const LateCutoffs* late) {
   if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
   out.n_ra = out.n_mc = out.n_un = 0;
+  S.timing.fit_wait_ms = 0.0;
   const int Sm = a->read_set_max, Q = a->quality_max;
   if (Sm <= 0 || Q <= 0 || Sm > 255 || Q > 256) throw BsqError(BSQ_ERR_ARG, "covariate maxima out of range");
   if (a->n_bins != (int64_t)Sm * 25 * Q) throw BsqError(BSQ_ERR_ARG, "error table size does not match its covariates");
@@ -746,6 +758,20 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   // MC/UN runs read only the final column flags and coverage, so they run on the side stream beside
   // the insert slots and the scorer, and the host turns them into entries while the scorer runs
   HIP_OK(hipStreamWaitEvent(S.stream2, S.evk[3], 0));
+  std::vector<double> prop_final(a->deletion_propagation_cutoff, a->deletion_propagation_cutoff + S.n_targets);
+  if (late) {
+    const double w0 = now_ms();
+    const std::vector<double>& c = late->get();
+    S.timing.fit_wait_ms = now_ms() - w0;
+    for (int t = 0; t < S.n_targets; t++) {
+      if ((c[t] < 0.0) != (prop_final[t] < 0.0)) throw BsqError(BSQ_ERR_ARG, "coverage fit changed a cutoff's sign");
+      prop_final[t] = c[t];
+      T[t].prop_cutoff = c[t];
+    }
+    S.h_targets = T;
+    HIP_OK(hipMemcpyAsync(S.d_targets.p, S.h_targets.data(), T.size() * sizeof(TargetInfo), hipMemcpyHostToDevice,
+                          S.stream2));
+  }
   HIP_OK(hipEventRecord(S.evk[4], S.stream2));
   hipLaunchKernelGGL(k_runs, dim3((S.n_pos + 255) / 256), dim3(256), 0, S.stream2, R);
   HIP_OK(hipGetLastError());
@@ -877,7 +903,7 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   for (GdEntry& e : run_entries) entries.push_back(std::move(e));
   if (!a->skip_missing_coverage_prediction) {
     for (int t : order) {
-      if (a->deletion_propagation_cutoff[t] >= 0.0) continue;
+      if (prop_final[t] >= 0.0) continue;
       GdEntry e;
       e.type = 1;
       e.seq_id = S.names[t];
@@ -913,9 +939,10 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   return true;
 }
 
-static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& out, int screen_off) {
+static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& out, int screen_off,
+                         const LateCutoffs* late = nullptr) {
   for (int attempt = 0; attempt < 3; attempt++)
-    if (run_identify_once(S, a, out, screen_off)) return;
+    if (run_identify_once(S, a, out, screen_off, late)) return;
   throw BsqError(BSQ_ERR_LIMIT, "identify result lists could not be sized for this sample");
 }
 
@@ -1269,6 +1296,106 @@ int bsq_session_coverage_table(bsq_session* s, const bsq_coverage_table_args* ar
   set_status(status, BSQ_OK, "");
   return BSQ_OK;
   BSQ_CATCH(status)
+}
+
+int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const bsq_coverage_stage_args* cov,
+                         double prop_option, double seed_option, const bsq_identify_args* im, bsq_ec_result** ec_out,
+                         bsq_coverage_summary* per_target, bsq_im_result** im_out, bsq_status* status) {
+  std::unique_ptr<bsq_ec_result> e(new bsq_ec_result);
+  std::unique_ptr<bsq_im_result> r(new bsq_im_result);
+  try {
+    if (!s || !ec || !cov || !im || !ec_out || !per_target || !im_out) throw BsqError(BSQ_ERR_ARG, "null argument");
+    Session& S = s->s;
+    if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
+    if (cov->n_targets != S.n_targets || !cov->target_lengths) throw BsqError(BSQ_ERR_ARG, "coverage stage targets do not match the sample");
+    const double e0 = now_ms();
+    run_error_count(S, ec, *e);
+    const double ec_wall = now_ms() - e0;
+    const double ec_k0 = S.timing.kernel_detail_ms[0], ec_k1 = S.timing.kernel_detail_ms[5];
+    const int NT = S.n_targets;
+    const int G = (int)e->hist.size();
+    // the stage reads the histograms as its file holds them (rows j >= 1 as doubles)
+    auto H = std::make_shared<std::vector<std::vector<double>>>(G);
+    for (int g = 0; g < G; g++) (*H)[g].assign(e->hist[g].begin(), e->hist[g].end());
+    std::vector<double> avg(NT, 0.0);
+    std::vector<int32_t> grp(NT, 0);
+    for (int t = 0; t < NT; t++) {
+      grp[t] = cov->coverage_group ? cov->coverage_group[t] : 0;
+      if (grp[t] < 0 || grp[t] >= std::max(G, 1)) throw BsqError(BSQ_ERR_ARG, "coverage group of a target out of range");
+      if (grp[t] < G) avg[t] = coverage_histogram_mean((*H)[grp[t]].data(), (int64_t)(*H)[grp[t]].size());
+    }
+    const bool skip = im->skip_missing_coverage_prediction != 0;
+    const bool fit = !skip && !cov->targeted_sequencing;
+    // the overrides after the stage; with neither the fitted cutoff is negative only for a failed
+    // fit, which needs an average of at most 3
+    auto apply_options = [=](std::vector<bsq_coverage_summary>& sm) {
+      for (int t = 0; t < NT; t++) {
+        if (!skip && prop_option) {
+          sm[t].deletion_coverage_propagation_cutoff = (prop_option < 1) ? sm[t].average * prop_option : prop_option;
+        }
+      }
+    };
+    // the seed override reads the stage's average (0 when the stage fitted nothing)
+    std::vector<double> seed(NT, 0.0), sign(NT, 1.0);
+    for (int t = 0; t < NT; t++) {
+      if (!skip && seed_option) seed[t] = (seed_option < 1) ? (fit ? avg[t] : 0.0) * seed_option : seed_option;
+      if (!fit) sign[t] = 0.0;   // the summary keeps its default cutoff of 0
+      if (fit && !prop_option && !(avg[t] > 3)) sign[t] = -2.0;   // decided by the fit
+    }
+    auto summaries = std::make_shared<std::vector<bsq_coverage_summary>>(NT);
+    auto args = std::make_shared<bsq_coverage_stage_args>(*cov);
+    auto hp = std::make_shared<std::vector<const double*>>(G);
+    auto hl = std::make_shared<std::vector<int64_t>>(G);
+    for (int g = 0; g < G; g++) {
+      (*hp)[g] = (*H)[g].data();
+      (*hl)[g] = (int64_t)(*H)[g].size();
+    }
+    args->histograms = hp->data();
+    args->histogram_lengths = hl->data();
+    args->n_coverage_groups = G;
+    double* fit_ms = &S.timing.fit_ms;
+    S.timing.fit_ms = 0.0;
+    LateCutoffs late = std::async(std::launch::async, [=]() {
+      const double f0 = now_ms();
+      if (fit) coverage_stage(args.get(), summaries->data(), host_threads());
+      (void)H;
+      apply_options(*summaries);
+      std::vector<double> c(NT);
+      for (int t = 0; t < NT; t++) c[t] = (*summaries)[t].deletion_coverage_propagation_cutoff;
+      *fit_ms = now_ms() - f0;
+      return c;
+    }).share();
+    // targets whose sign the fit decides wait for it before the column pass
+    bool wait_first = false;
+    for (int t = 0; t < NT; t++) wait_first = wait_first || sign[t] < -1.0;
+    std::vector<double> prov(NT);
+    if (wait_first) {
+      const std::vector<double>& c = late.get();
+      for (int t = 0; t < NT; t++) prov[t] = c[t];
+    } else {
+      for (int t = 0; t < NT; t++) prov[t] = sign[t];
+    }
+    bsq_identify_args a2 = *im;
+    a2.log10_prob_table = e->rates_table.data();
+    a2.n_bins = (int64_t)e->rates_table.size();
+    a2.read_set_max = ec->read_set_max;
+    a2.quality_max = ec->quality_max;
+    a2.deletion_propagation_cutoff = prov.data();
+    a2.deletion_seed_cutoff = seed.data();
+    run_identify(S, &a2, *r, 0, &late);
+    late.wait();
+    for (int t = 0; t < NT; t++) per_target[t] = (*summaries)[t];
+    S.timing.pipeline_ec_ms = ec_wall;
+    S.timing.pipeline_ec_kernel_ms[0] = ec_k0;
+    S.timing.pipeline_ec_kernel_ms[1] = ec_k1;
+    *ec_out = e.release();
+    *im_out = r.release();
+    set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& ex) {
+    set_status(status, ex.code, ex.what());
+    return ex.code;
+  }
 }
 
 int bsq_session_timing(const bsq_session* s, bsq_timing* out) {

@@ -249,8 +249,45 @@ typedef struct bsq_coverage_fit {
   uint32_t censor_start, censor_end;
 } bsq_coverage_fit;
 
+/* fit_coverage_distribution (coverage_distribution.cpp:345-399) of one histogram hist[0..len),
+ * rows j >= 1 as read back from its file; sequence_length is the coverage group's summed length
+ * (its 0.05/sqrt is the quantile's probability). */
 int bsq_fit_coverage_distribution(const uint32_t* hist, int64_t len, double sequence_length,
                                   bsq_coverage_fit* out, bsq_status* status);
+
+/* Summary::unique_coverage[seq_id] as the coverage-distribution stage leaves it
+ * (summary.h:116-142, coverage_distribution.cpp:561-588). */
+typedef struct bsq_coverage_summary {
+  double deletion_coverage_propagation_cutoff;
+  double nbinom_size_parameter, nbinom_mean_parameter, nbinom_prob_parameter;
+  double nbinom_variance, nbinom_relative_variance;
+  double average, variance, relative_variance;
+} bsq_coverage_summary;
+
+typedef struct bsq_coverage_stage_args {
+  int32_t n_targets;
+  const int64_t* target_lengths;      /* ref_seq_info.get_sequence_length of each target           */
+  const int32_t* coverage_group;      /* per target: Settings::seq_id_to_coverage_group (NULL: 0)  */
+  int32_t n_coverage_groups;          /* Settings::seq_ids_by_coverage_group().size()              */
+  const double* const* histograms;    /* per group: n[0..len) as read back from
+                                         @.unique_only_coverage_distribution.tab (rows j >= 1)     */
+  const int64_t* histogram_lengths;
+  int32_t targeted_sequencing;        /* Settings::targeted_sequencing: no fit, summary untouched  */
+} bsq_coverage_stage_args;
+
+/* CoverageDistribution::analyze_unique_coverage_distributions (coverage_distribution.h:100-107,
+ * coverage_distribution.cpp:510-633): per coverage group, the NB fit of its histogram over the
+ * group's summed length, written into every member target's summary.  per_target[n_targets] is
+ * filled (all zero for targets the stage leaves alone).  The diagnostic plot is not drawn. */
+int bsq_analyze_unique_coverage_distributions(const bsq_coverage_stage_args* args,
+                                              bsq_coverage_summary* per_target, bsq_status* status);
+
+/* read_coverage_histogram (coverage_distribution.cpp:34-65) of the text of a
+ * @.unique_only_coverage_distribution.tab file: a header line, then "coverage<TAB>count" rows.
+ * Writes n[0..max_coverage] into out (cap entries) and the length needed into *len; call with
+ * cap 0 to learn it.  An empty table gives length 1 (n[0] = 0). */
+int bsq_read_coverage_histogram(const char* text, int64_t text_len, double* out, int64_t cap,
+                                int64_t* len, bsq_status* status);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Device-resident sessions (MI355X path)                                                     */
@@ -277,6 +314,14 @@ typedef struct bsq_timing {
    *              host work beside the scorer: [0] DP candidate regions (own thread)
    *              [5] MC/UN entries */
   double kernel_detail_ms[8];
+  /* bsq_session_pipeline: the coverage-distribution stage on host threads (it runs beside the
+   * identify column pass) and how long identify then waited for its cutoffs */
+  double fit_ms;
+  double fit_wait_ms;
+  /* bsq_session_pipeline: wall time of its error_count step and that step's kernels
+   * ([0] k_ec_reads, [1] coverage scan and histogram); the fields above then describe identify */
+  double pipeline_ec_ms;
+  double pipeline_ec_kernel_ms[2];
 } bsq_timing;
 
 /* Create a session bound to HIP device `device`.  Fails with BSQ_ERR_DEVICE when no GPU. */
@@ -295,6 +340,25 @@ int bsq_session_identify(bsq_session* s, const bsq_identify_args* args, bsq_im_r
  * exact FP64 scorer.  Same output as bsq_session_identify; used to show the screen drops nothing. */
 int bsq_session_identify_unscreened(bsq_session* s, const bsq_identify_args* args,
                                     bsq_im_result** out, bsq_status* status);
+/* breseq's stages 07-08 on the loaded sample, run back to back as breseq_cmdline.cpp:2293, 2341
+ * and 2471 run them:
+ *   1. error_count (ec);
+ *   2. the coverage-distribution stage on error_count's histograms (cov: lengths, groups and
+ *      targeted flag; its histogram fields are ignored), skipped when
+ *      im->skip_missing_coverage_prediction is set; then the Settings overrides
+ *      deletion_coverage_propagation_cutoff / deletion_coverage_seed_cutoff (0 = unset, < 1 a
+ *      fraction of the target's average coverage, >= 1 the value; breseq_cmdline.cpp:2350-2367,
+ *      2409-2427);
+ *   3. identify_mutations with error_count's table as re-read from error_rates.tab and each
+ *      target's cutoffs from step 2 (im's log10 table, covariate maxima and both cutoff arrays are
+ *      ignored).
+ * The coverage fit runs on host threads while the device runs identify's column pass: only the
+ * MC runs need its values, and whether a target's fit failed is known from its average first.
+ * per_target[n_targets] receives the stage's summaries (with the overrides applied). */
+int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const bsq_coverage_stage_args* cov,
+                         double deletion_coverage_propagation_cutoff, double deletion_coverage_seed_cutoff,
+                         const bsq_identify_args* im, bsq_ec_result** ec_out,
+                         bsq_coverage_summary* per_target, bsq_im_result** im_out, bsq_status* status);
 int bsq_session_timing(const bsq_session* s, bsq_timing* out);
 void bsq_session_destroy(bsq_session* s);
 

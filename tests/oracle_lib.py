@@ -228,3 +228,45 @@ def presence_and_bounds(r, lpmax, obs, v, tol=1e-6, log10_ref_length=0.0):
                               len(obs), v, tol, log10_ref_length, f.ctypes.data_as(L.c_f64p),
                               out.ctypes.data_as(L.c_f64p))
     return f, out[0], out[1], out[2], out[3]
+
+
+def _bind_fit(O):
+    if getattr(O, "_fit_bound", False):
+        return O
+    O.orc_fit_coverage_distribution.restype = C.c_int
+    O.orc_fit_coverage_distribution.argtypes = [L.c_u32p, C.c_int64, C.c_uint32, L.c_f64p]
+    O.orc_special.restype = C.c_double
+    O.orc_special.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double]
+    O.orc_nelder_mead_quadratic.restype = C.c_int
+    O.orc_nelder_mead_quadratic.argtypes = [C.c_double] * 5 + [L.c_f64p]
+    O._fit_bound = True
+    return O
+
+
+FIT_FIELDS = ("nb_fit_size", "nb_fit_mu", "average", "variance", "relative_variance",
+              "deletion_coverage_propagation_cutoff", "censor_start", "censor_end", "nb_fit_scale",
+              "restarts_converged")
+
+
+def fit_coverage_distribution(hist, sequence_length):
+    """fit_coverage_distribution of a unique-only coverage histogram (hist[j], rows j >= 1)."""
+    O = _bind_fit(lib())
+    h = np.ascontiguousarray(hist, dtype=np.uint32)
+    out = np.zeros(len(FIT_FIELDS))
+    O.orc_fit_coverage_distribution(h.ctypes.data_as(L.c_u32p), h.size, int(sequence_length),
+                                    out.ctypes.data_as(L.c_f64p))
+    return dict(zip(FIT_FIELDS, out.tolist()))
+
+
+SPECIAL = {"lngamma": 0, "gamma": 1, "lgam": 2, "incbet": 3, "nbdtr": 4, "dnbinom_mu": 5, "qnbinom_mu": 6}
+
+
+def special(name, a, b=0.0, x=0.0):
+    return _bind_fit(lib()).orc_special(SPECIAL[name], a, b, x)
+
+
+def nelder_mead_quadratic(c0, c1, w, x0, x1):
+    O = _bind_fit(lib())
+    est = np.zeros(2)
+    conv = O.orc_nelder_mead_quadratic(c0, c1, w, x0, x1, est.ctypes.data_as(L.c_f64p))
+    return est, bool(conv)

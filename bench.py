@@ -1,14 +1,18 @@
 #!/usr/bin/env python
-"""Benchmark of the MI355X pileup-scoring path (breseq error_count + identify_mutations).
+"""Benchmark of the MI355X pileup-scoring path (breseq error_count + coverage distribution +
+identify_mutations).
 
-One step = one pass of the whole path over one sample already resident in HBM:
+One step = one pass of the whole path over one sample already resident in HBM, as breseq runs its
+stages 07-08 (bsq_session_pipeline):
   error_count (coverage histogram + covariate counts) -> error_rates.tab round trip ->
+  coverage-distribution fit (deletion propagation cutoff; on host threads beside the next step) ->
   identify_mutations (RA / MC / UN scoring) -> ra_mc_evidence.gd text, with the discordant-pair
   accumulator -> DP_candidate_regions.csv text (breseq runs it on every paired sample).
 Metric: aligned bases/s (pileup entries: M/=/X bases + D positions), plus p50/p99 step latency.
 Workload: 4.6 Mb reference (GC 50.8%), 100x of 2x150 bp paired reads, insert N(350,50),
-qualities N(34,5), 500 SNPs + 50 indels of 1-5 bp, 0.2% discordant pairs at 40 junctions,
-consensus mode.  Synthetic, seeded.
+qualities N(34,5), 500 SNPs + 50 indels of 1-5 bp, 0.2% discordant pairs at 40 junctions, three
+400-bp deletions, consensus mode.  Synthetic, seeded (breseq_amd.workloads.C2); its results are
+checked against the oracle's by the GPU tests.
 """
 import argparse
 import json
@@ -21,12 +25,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 
-C2 = dict(seed=2, ref_length=4_600_000, gc=0.508, read_length=150, coverage=100.0, paired=True, insert_mean=350,
-          insert_sd=50, qual_mean=34, qual_sd=5, qual_min=2, qual_max=41, n_snps=500, n_indels=50, indel_max=5,
-          redundant_fraction=0.01, mapq=42, discordant_fraction=0.002, n_junctions=40)
+from breseq_amd import workloads  # noqa: E402
 
-# paired-mapping-distance fit of the one read file set: median and MAD of N(350, 50)
-DP_SET = dict(median=350.0, mad=50.0 * 0.6745, r1_prefix=1, r2_prefix=2)
+C2 = workloads.C2
+DP_SET = workloads.DP_SET
 
 # Per-unit algorithmic bytes of the column kernels: 1 B base + 1 B quality per aligned base,
 # 24 B of per-read metadata (position, CIGAR, flags, tags), and per reference column 1 B of
@@ -71,13 +73,8 @@ def percentile(xs, q):
     return xs[f] + (xs[c] - xs[f]) * (k - f)
 
 
-def run_step(B, sess, mode, dp):
-    ec = sess.error_count()
-    prop = [5.0] * sess.sample.n_targets
-    gd = sess.identify_mutations(log10_prob=ec.rates_table, read_set_max=ec.read_set_max,
-                                 quality_max=ec.quality_max, propagation_cutoffs=prop, mode=mode, dp=dp,
-                                 with_dp=True)
-    return ec, gd
+def run_step(sess, mode, dp):
+    return sess.pipeline(mode=mode, dp=dp)
 
 
 def cpu_baseline(B, cfg, target_seconds):
@@ -92,14 +89,17 @@ def cpu_baseline(B, cfg, target_seconds):
     small["ref_length"] = length
     small["n_snps"] = max(1, int(cfg["n_snps"] * length / cfg["ref_length"]))
     small["n_indels"] = max(1, int(cfg["n_indels"] * length / cfg["ref_length"]))
+    small["n_missing"] = 1
     s = B.Sample.simulate(**small)
     nab = s.aligned_bases()
     t0 = time.perf_counter()
     args = api.make_error_count_args(s)
     hists, counts, log10p, texts = O.error_count(args)
+    fit = O.fit_coverage_distribution(hists[0], sum(s.target_lengths()))
     table = O.read_log10_prob_table(texts["error_rates"], len(log10p))
-    iargs = api.make_identify_args(s, table, args.read_set_max, args.quality_max, [5.0] * s.n_targets,
-                                   dp=api.dp_settings([DP_SET], seed=3))
+    iargs = api.make_identify_args(s, table, args.read_set_max, args.quality_max,
+                                   [fit["deletion_coverage_propagation_cutoff"]] * s.n_targets,
+                                   dp=api.dp_settings([DP_SET], seed=workloads.DP_SEED))
     O.identify_with_dp(iargs)
     dt = time.perf_counter() - t0
     return {"value": nab / dt, "unit": "aligned bases/s", "cores": 1, "kind": "port",
@@ -131,39 +131,34 @@ def main():
     sample = B.Sample.simulate(**cfg)
     log("[bench] simulated in %.1f s" % (time.time() - t))
     sess = B.Session().load(sample)
-    dp = B.api.dp_settings([DP_SET], seed=3)
+    dp = B.api.dp_settings([DP_SET], seed=workloads.DP_SEED)
     tm = sess.timing()
     nab, n_reads, n_cols = tm["aligned_bases"], tm["n_reads"], tm["n_positions"]
     log("[bench] loaded: %d aligned bases, %d reads, %d columns; pack %.0f ms, H2D %.0f ms"
         % (nab, n_reads, n_cols, tm["pack_ms"], tm["h2d_ms"]))
     for i in range(args.warmup):
-        run_step(B, sess, B.CONSENSUS, dp)
+        run_step(sess, B.CONSENSUS, dp)
         log("[bench] warmup step %d done" % i)
     step_ms = []
     ec_k, im_k = [], []
-    host = {"ec_host_pre": [], "ec_d2h": [], "ec_host_post": [], "im_host_pre": [], "im_d2h": [], "im_host_post": [],
-            "ec_call": [], "im_call": []}
+    host = {"ec_call": [], "im_call": [], "im_host_pre": [], "im_d2h": [], "im_host_post": [], "fit": [],
+            "fit_wait": []}
     detail = None
     for i in range(args.steps):
         t0 = time.perf_counter()
-        ec = sess.error_count()
-        tm = time.perf_counter()
-        ect = sess.timing()
-        prop = [5.0] * sample.n_targets
-        gd, dp_text = sess.identify_mutations(log10_prob=ec.rates_table, read_set_max=ec.read_set_max,
-                                              quality_max=ec.quality_max, propagation_cutoffs=prop, mode=B.CONSENSUS,
-                                              dp=dp, with_dp=True)
+        ec, summaries, gd, dp_text = run_step(sess, B.CONSENSUS, dp)
         t1 = time.perf_counter()
         imt = sess.timing()
         step_ms.append((t1 - t0) * 1000.0)
-        ec_k.append(ect["kernel_detail_ms"])
-        for k, tmd in (("ec", ect), ("im", imt)):
-            host[k + "_host_pre"].append(tmd["host_pre_ms"])
-            host[k + "_d2h"].append(tmd["d2h_ms"])
-            host[k + "_host_post"].append(tmd["host_post_ms"])
-        host["ec_call"].append((tm - t0) * 1000.0)
-        host["im_call"].append((t1 - tm) * 1000.0)
+        ec_k.append(imt["pipeline_ec_kernel_ms"])
         im_k.append(imt["kernel_detail_ms"])
+        host["ec_call"].append(imt["pipeline_ec_ms"])
+        host["im_call"].append((t1 - t0) * 1000.0 - imt["pipeline_ec_ms"])
+        host["im_host_pre"].append(imt["host_pre_ms"])
+        host["im_d2h"].append(imt["d2h_ms"])
+        host["im_host_post"].append(imt["host_post_ms"])
+        host["fit"].append(imt["fit_ms"])
+        host["fit_wait"].append(imt["fit_wait_ms"])
         detail = imt
     total_s = sum(step_ms) / 1000.0
     ms_per_step = sum(step_ms) / len(step_ms)
@@ -172,7 +167,7 @@ def main():
     # dominant kernel and its roofline
     avg = lambda xs: sum(xs) / len(xs)
     k_ec = avg([d[0] for d in ec_k])
-    k_cov = avg([d[5] for d in ec_k])
+    k_cov = avg([d[1] for d in ec_k])
     k_col = avg([d[1] for d in im_k])
     k_ins = avg([d[2] for d in im_k])
     k_score = avg([d[3] for d in im_k])
@@ -188,6 +183,8 @@ def main():
     host_ms = {k: sum(v) / len(v) for k, v in host.items()}
     host_ms["im_runs_host_beside_scorer"] = avg([d[5] for d in im_k])
     host_ms["im_dp_regions_thread"] = avg([d[0] for d in im_k])
+    host_ms["fit_beside_identify"] = host_ms.pop("fit")
+    host_ms["identify_waited_for_fit"] = host_ms.pop("fit_wait")
     out = {
         "metric": "aligned bases/sec through pileup scoring, 4.6Mb@100x; p50/p99 sample latency",
         "value": value,
@@ -203,11 +200,14 @@ def main():
         "vs_baseline": None,
         "dtype": "u8 bases/quals, int32 counts, f64 likelihoods",
         "data": "synthetic (seeded generator: random reference, planted variants, reads aligned by construction)",
-        "config": {"workload": "consensus-mode pileup scoring of one sample: %d bp reference at %.0fx, 2x150 bp "
-                               "paired reads, insert N(350,50), quals N(34,5), 500 SNPs + 50 indels, "
-                               "%.1f%% discordant pairs" % (cfg["ref_length"], cfg["coverage"],
-                                                            100.0 * cfg["discordant_fraction"]),
+        "config": {"workload": "consensus-mode pileup scoring of one sample (error_count, coverage fit, "
+                               "identify_mutations): %d bp reference at %.0fx, 2x150 bp paired reads, insert "
+                               "N(350,50), quals N(34,5), 500 SNPs + 50 indels, %.1f%% discordant pairs, %d "
+                               "deletions" % (cfg["ref_length"], cfg["coverage"], 100.0 * cfg["discordant_fraction"],
+                                              cfg["n_missing"]),
                    "aligned_bases": nab, "reads": n_reads, "ra_entries": n_ra,
+                   "mc_entries": gd.count("\nMC\t"), "un_entries": gd.count("\nUN\t"),
+                   "deletion_propagation_cutoff": summaries[0]["deletion_coverage_propagation_cutoff"],
                    "dp_regions": dp_text.count("\n") - 1,
                    "candidates": detail["candidates"]},
         "kernels_ms": kernels,

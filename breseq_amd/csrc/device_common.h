@@ -20,7 +20,9 @@ enum : uint32_t {
   kErrReadSet = 8,       // read file index >= read_set covariate maximum
   kErrQueryIndex = 16,   // quality index outside the read (error_count.cpp:914,940,967)
   kErrCigar = 32,        // CIGAR walk did not reach the column
-  kErrCapacity = 64      // a fixed device buffer was too small (host retries or reports)
+  kErrCapacity = 64,     // a fixed device buffer was too small (host retries or reports)
+  kErrIndex = 128        // an index read from a device list fell outside the buffer it indexes;
+                         // the access was skipped (a library fault, reported rather than taken)
 };
 
 // Alignment records are read-only for a kernel's lifetime; reading them through the constant

@@ -237,7 +237,6 @@ __device__ __forceinline__ void ec_group4(const EcParams& P, const uint8_t* __re
   }
 }
 
-constexpr int kEcChunk = 32;   // consecutive alignments per wavefront work item
 constexpr int kEcGroup = 21;   // lanes per alignment when three share a wavefront (8 positions each)
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_ec_reads(EcParams P) {

@@ -188,9 +188,11 @@ __global__ void __launch_bounds__(256) k_prep_reads(const DRead* reads, int64_t 
 }
 
 // Certified pure_genotype_call (identify_mutations.cpp:3398-3433) from FP32 sums.
-//   * Each FP32 sum lps[b] adds n terms of one sign, each rounded once from FP64: it is within
-//     (n + 2) 2^-24 |lps[b]| of the reference's FP64 sum (which is itself exact to ~1e-13 relative).
-//     E is the largest such bound.
+//   * Each FP32 sum lps[b] adds n terms of one sign, each rounded once from FP64, in n - 1 rounded
+//     additions: with u = 2^-24 and g = (n + 2) u it is within g / (1 - g) |lps[b]| of the exact sum
+//     (the recursive-summation bound gamma_n = n u / (1 - n u), with two more u for the term
+//     roundings and the reference's own FP64 summation error).  E is the largest such bound; a
+//     column so deep that g reaches 1/2 is never certified.
 //   * The reference's best is certain when the top sum leads the next by more than 2E.
 //   * Its score lps[best] - log10 Sum_{b != best} 10^lps[b] moves by at most 2E when every sum moves
 //     by at most E; here each 10^d (d <= 0) is evaluated from (float)d, adding at most 3e-6 (terms
@@ -228,7 +230,8 @@ __device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, i
       if (lps[b] > lps[best]) best = b;
       E = fmaxf(E, fabsf(lps[b]));
     }
-    const double err_sum = ((double)A.n + 2.0) * 5.9604644775390625e-08 * (double)E * 1.0001;
+    const double g = ((double)A.n + 2.0) * 5.9604644775390625e-08;
+    const double err_sum = (g < 0.5) ? g / (1.0 - g) * (double)E * 1.0001 : INFINITY;
     float second = -3.4028235e38f;
 #pragma unroll
     for (int b = 0; b < 5; b++)
@@ -273,7 +276,8 @@ __device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, i
 // per column gathers its pdata in read order (order-preserving compaction), five lanes add the FP64
 // likelihoods in that order, and the reference's call is made from those sums.
 __global__ void __launch_bounds__(64) k_fix_columns(ImParams P) {
-  extern __shared__ uint32_t fix_keys[];   // P.key_cap
+  extern __shared__ uint32_t fix_lds_keys[];   // P.key_cap, unless the list is in device scratch
+  uint32_t* fix_keys = P.keys_global ? P.key_scratch + (size_t)blockIdx.x * (size_t)P.key_cap : fix_lds_keys;
   __shared__ double fix_lps[5];
   const int lane = threadIdx.x;
   const int n_fix = min(*P.n_fix, P.fix_cap);
@@ -282,7 +286,15 @@ __global__ void __launch_bounds__(64) k_fix_columns(ImParams P) {
   for (int i = blockIdx.x; i < n_fix; i += gridDim.x) {
     const int p = P.fix[i];
     const int64_t w = p / kWaveCols;
+    if (p < 0 || p >= P.n_pos || w >= P.n_windows) {
+      err |= kErrIndex;
+      continue;
+    }
     const int rb = P.win_rb[w], re = P.win_re[w];
+    if (rb < 0 || re < rb || re > P.n_reads) {
+      err |= kErrIndex;
+      continue;
+    }
     int n = 0;
     for (int r0 = rb; r0 < re; r0 += 64) {
       const int r = r0 + lane;
@@ -291,6 +303,10 @@ __global__ void __launch_bounds__(64) k_fix_columns(ImParams P) {
       int obs = 0, strand_idx = 0;
       if (r < re) use = pdata_key(P.reads[r], P.cigar, P.bases, P.quals, p, 0, S, Q, P.base_quality_cutoff, key, obs,
                                   strand_idx, err);
+      if (use && key >= (uint32_t)P.tab_keys) {
+        err |= kErrIndex;
+        key = 0;
+      }
       const unsigned long long mask = __ballot(use);
       if (use) {
         const int idx = n + __popcll(mask & ((1ull << lane) - 1ull));

@@ -45,6 +45,8 @@ struct CovParams {
   uint32_t* err;
 };
 constexpr int kCovTile = 4096;      // columns per block of the coverage scan (256 threads x 16)
+constexpr int kEcChunk = 32;        // consecutive alignments per wavefront work item of k_ec_reads
+constexpr int kEcWaves = 4;         // wavefronts per k_ec_reads block (256 threads)
 
 // One row of the strand-folded likelihood table: a read of quality q from read file rs with
 // MAPQ slot m observing read-strand base o reads row ((m*S + rs)*Q + q)*5 + o.  Hypotheses are in
@@ -90,6 +92,8 @@ struct ImParams {
   int32_t base_quality_cutoff;
   const double* tab_lp;         // per key: log10 P(obs | true base b), b = 0..4
   const double* tab_r;          // per key: 10^(lp - lpmax) for b = 0..4, then lpmax
+  int32_t tab_keys;             // rows of tab_lp / tab_r
+  int32_t n_reads;              // alignments in reads[] (and idrec[])
   const FoldRow* fold;          // strand-folded rows (mslot, read file, quality, read-strand obs);
                                 // row fold_rows is all zeros (the row of a read that adds nothing)
   int32_t fold_rows;
@@ -111,7 +115,9 @@ struct ImParams {
   int32_t* ins;                 // columns with insert slots
   int32_t* n_ins;
   int32_t ins_cap;
-  int32_t key_cap;              // LDS capacity of the scorer's read list
+  int32_t key_cap;              // capacity of a candidate's read list (the most reads of a window)
+  int32_t keys_global;          // 1: read lists in key_scratch (key_cap per block), else in LDS
+  uint32_t* key_scratch;        // device scratch of the read lists of deep samples
   int32_t rows_cap;             // reads whose table rows the scorer caches in LDS
   RaOut* ra_out;
   // columns whose predicted / consensus decision the column pass could not certify
@@ -182,6 +188,8 @@ __global__ void k_coverage_columns(CovTabParams C);
 __global__ void k_coverage_rows(CovTabParams C);
 
 constexpr int kScoreWaves = 4;               // wavefronts per candidate in the exact scorer
+constexpr int kScoreGrid = 2048;             // blocks of the exact scorer (grid-stride over candidates)
+constexpr int kFixGrid = 256;                // blocks of k_fix_columns (at most kScoreGrid)
 constexpr size_t kScoreStaticLds = 28 * 1024;   // bound on the scorer's static LDS
 constexpr size_t kScoreLdsBudget = 52 * 1024;   // static + dynamic LDS of a scorer block: three per CU
 __global__ void k_score_candidates(ImParams P);

@@ -28,7 +28,15 @@ __global__ void __launch_bounds__(64) k_insert_slots(ImParams P) {
   for (int i = blockIdx.x; i < n_ins; i += gridDim.x) {
     const int p = P.ins[i];
     const int64_t w = p / WAVE;
+    if (p < 0 || p >= P.n_pos || w >= P.n_windows) {
+      err |= kErrIndex;
+      continue;
+    }
     const int rb = P.win_rb[w], re = P.win_re[w];
+    if (rb < 0 || re < rb || re > P.n_reads) {
+      err |= kErrIndex;
+      continue;
+    }
     for (int slot = 1; slot < 65536; slot++) {
       if (lane == 0) {
         const int c = atomicAdd(P.n_cand, 1);

@@ -486,6 +486,8 @@ __device__ __noinline__ double ks_less_w(LdsShared* SS, uint32_t nx, uint32_t ny
   const uint32_t m = ny, n = nx;   // cells (i, j), i <= m over y, j <= n over x
   const int64_t thr = (int64_t)by * n - (int64_t)bx * m;
   const uint32_t D = m + n;
+  // m * n < 10000 bounds the lattice: min(m, n) <= 99 cells per diagonal, D <= 10000 boundary bits
+  if (min(m, n) > (uint32_t)kKsDiag || D > 10000u) return __longlong_as_double(0x7ff8000000000000LL);
   // boundary(d): d is 0 or the pooled count at the end of a tie group
   const uint32_t nw = D / 32 + 1;
   for (uint32_t k = lane; k < nw; k += 64) SS->ks_bnd[k] = 0;
@@ -621,8 +623,11 @@ __device__ __forceinline__ int next_index(const double f[5], uint32_t n, int exc
 
 __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3))) k_score_candidates(ImParams P) {
   extern __shared__ __align__(16) unsigned char dyn[];
-  uint32_t* keys = (uint32_t*)dyn;
-  double* rows = (double*)(dyn + (((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15));
+  // the candidate's read list: in LDS, or for columns too deep for it in this block's slice of a
+  // device scratch list (the table-row cache then takes all of the dynamic LDS)
+  uint32_t* keys = P.keys_global ? P.key_scratch + (size_t)blockIdx.x * (size_t)P.key_cap : (uint32_t*)dyn;
+  const size_t key_lds = P.keys_global ? 0 : (((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15);
+  double* rows = (double*)(dyn + key_lds);
   __shared__ ScoreShared SS;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -641,9 +646,23 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
   uint32_t err = 0;
   for (int ci = blockIdx.x; ci < n_cand; ci += gridDim.x) {
     const int2 cd = P.cand[ci];
+    // every index the candidate leads to is checked before it is used: a column inside the
+    // sample, a window inside the index, a read range inside the alignments (block-uniform)
+    if (cd.x < 0 || cd.x >= P.n_pos || cd.y < 0) {
+      err |= kErrIndex;
+      continue;
+    }
     const int p = cd.x, slot = cd.y;
     const int64_t w = p / 64;
+    if (w >= P.n_windows) {
+      err |= kErrIndex;
+      continue;
+    }
     const int rb = P.win_rb[w], re = P.win_re[w];
+    if (rb < 0 || re < rb || re > P.n_reads) {
+      err |= kErrIndex;
+      continue;
+    }
     if (tid < 18) SS.cov[tid / 3][tid % 3] = 0;
     if (tid == 0) SS.n_keys = 0;
     __syncthreads();
@@ -662,6 +681,10 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
       __syncthreads();
       int base = SS.n_keys;
       for (int k = 0; k < wave; k++) base += (int)SS.wcount[k];
+      if (use && key >= (uint32_t)P.tab_keys) {
+        err |= kErrIndex;   // a key outside the likelihood tables: counted, never looked up
+        key = 0;
+      }
       if (use) {
         const int idx = base + __popcll(mask & ((1ull << lane) - 1ull));
         if (idx < P.key_cap) keys[idx] = key;

@@ -105,6 +105,7 @@ struct Session {
   DBuf<unsigned long long> d_counts, d_hist;
   DBuf<uint32_t> d_err;
   DBuf<double> d_tab_lp, d_tab_r;
+  DBuf<uint32_t> d_key_scratch;      // read lists of the scorer and k_fix_columns for deep samples
   DBuf<uint8_t> d_col_flags;
   DBuf<int32_t> d_col_unique, d_col_total;
   DBuf<int2> d_cand;
@@ -393,6 +394,7 @@ static void raise_err_bits(uint32_t err) {
   if (err & 16) m += " quality index outside the read;";
   if (err & 32) m += " CIGAR does not cover its alignment span;";
   if (err & 64) m += " device buffer capacity exceeded;";
+  if (err & 128) throw BsqError(BSQ_ERR_DEVICE, "a device list held an index outside its buffer (skipped)");
   throw BsqError(BSQ_ERR_DATA, m);
 }
 
@@ -493,9 +495,10 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   const size_t lds_table = (size_t)n_bins * sizeof(uint32_t);
   const size_t lds_hist = (size_t)C.n_groups_lds * C.hist_lds * sizeof(uint32_t);
   if (lds_table > 64 * 1024) throw BsqError(BSQ_ERR_LIMIT, "covariate table too large for LDS");
-  // one block per 8 waves' worth of alignment chunks, capped at 8 blocks per CU
-  const int64_t chunks = (S.n_dreads + 15) / 16;
-  int ec_grid = (int)std::max<int64_t>(1, std::min<int64_t>((chunks + 3) / 4, 2048));
+  // one work item per wavefront while that takes fewer than 2,048 blocks (8 per CU), then
+  // grid-stride loops over the items
+  const int64_t chunks = (S.n_dreads + kEcChunk - 1) / kEcChunk;
+  int ec_grid = (int)std::max<int64_t>(1, std::min<int64_t>((chunks + kEcWaves - 1) / kEcWaves, 2048));
   if (ec_grid >= 8) ec_grid &= ~7;   // a multiple of the 8 XCDs
   S.timing.host_pre_ms = now_ms() - th0;
   HIP_OK(hipEventRecord(S.ev0, S.stream));
@@ -696,6 +699,8 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   P.base_quality_cutoff = a->base_quality_cutoff;
   P.tab_lp = S.d_tab_lp.p;
   P.tab_r = S.d_tab_r.p;
+  P.tab_keys = (int32_t)(rr.size() / 6);
+  P.n_reads = (int32_t)S.n_dreads;
   P.fold = S.d_fold.p;
   P.fold_rows = (int32_t)fold.size() - 1;
   const size_t fold_bytes = fold.size() * sizeof(FoldRow);
@@ -716,9 +721,18 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   P.n_ins = S.d_counters.p + 1;
   P.ins_cap = ins_cap;
   P.key_cap = std::max(64, S.max_window_reads);
-  const size_t key_bytes = ((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15;
+  // a candidate's read list (4 B per read of its window) stays in LDS while it takes at most half
+  // of the scorer's dynamic LDS; deeper samples keep it in a device scratch list per block
   const size_t score_dyn_budget = kScoreLdsBudget - kScoreStaticLds;
-  if (key_bytes > score_dyn_budget) throw BsqError(BSQ_ERR_LIMIT, "column depth too large for the exact scorer's LDS list");
+  const size_t key_list_bytes = ((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15;
+  P.keys_global = (key_list_bytes > score_dyn_budget / 2) ? 1 : 0;
+  const size_t key_bytes = P.keys_global ? 0 : key_list_bytes;
+  if (P.keys_global) {
+    S.d_key_scratch.alloc((size_t)kScoreGrid * (size_t)P.key_cap);
+    P.key_scratch = S.d_key_scratch.p;
+  } else {
+    P.key_scratch = nullptr;
+  }
   P.rows_cap = (int32_t)((score_dyn_budget - key_bytes) / (6 * sizeof(double)));
   const size_t score_lds = key_bytes + (size_t)P.rows_cap * 6 * sizeof(double);
   P.ra_out = S.d_ra.p;
@@ -752,7 +766,7 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[0], S.stream));
-  hipLaunchKernelGGL(k_fix_columns, dim3(256), dim3(64), key_bytes, S.stream, P);
+  hipLaunchKernelGGL(k_fix_columns, dim3(kFixGrid), dim3(64), key_bytes, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[3], S.stream));
   // MC/UN runs read only the final column flags and coverage, so they run on the side stream beside
@@ -781,7 +795,7 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   hipLaunchKernelGGL(k_insert_slots, dim3(1024), dim3(64), 0, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[1], S.stream));
-  hipLaunchKernelGGL(k_score_candidates, dim3(2048), dim3(kScoreWaves * 64), score_lds, S.stream, P);
+  hipLaunchKernelGGL(k_score_candidates, dim3(kScoreGrid), dim3(kScoreWaves * 64), score_lds, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.ev1, S.stream));
   int32_t counters[8];

@@ -115,10 +115,11 @@ class BuiltSample:
 COMP = {"A": "T", "C": "G", "G": "C", "T": "A", "N": "N"}
 
 
-def random_alignments(seed, n_targets=2, target_len=400, n_reads=600, max_len=60):
+def random_alignments(seed, n_targets=2, target_len=400, n_reads=600, max_len=60, run_len=(3, 20), clip_len=(1, 6)):
     """Alignments with every CIGAR shape the pileup distinguishes: soft clips, insertions and
     deletions (also adjacent ones), reference skips, pads before insertions, =/X runs, N read bases,
-    both strands, redundant reads and reads ending at a target end.  Sorted by (target, position)."""
+    both strands, redundant reads and reads ending at a target end.  Sorted by (target, position).
+    run_len and clip_len bound the lengths of aligned runs and soft clips."""
     rng = np.random.default_rng(seed)
     refs = []
     for t in range(n_targets):
@@ -131,13 +132,13 @@ def random_alignments(seed, n_targets=2, target_len=400, n_reads=600, max_len=60
         ref = refs[t][1]
         ops = []
         if rng.random() < 0.3:
-            ops.append(("S", int(rng.integers(1, 6))))
+            ops.append(("S", int(rng.integers(*clip_len))))
         if rng.random() < 0.05:
             ops.append(("I", int(rng.integers(1, 3))))
         n_runs = 1 if rng.random() < 0.3 else int(rng.integers(2, 5))
         for k in range(n_runs):
             m = "M" if rng.random() < 0.8 else ("=" if rng.random() < 0.5 else "X")
-            ops.append((m, int(rng.integers(3, 20))))
+            ops.append((m, int(rng.integers(*run_len))))
             if k == n_runs - 1:
                 break
             u = rng.random()
@@ -158,7 +159,7 @@ def random_alignments(seed, n_targets=2, target_len=400, n_reads=600, max_len=60
                 ops.append(("I", 1))
                 ops.append(("D", 1))
         if rng.random() < 0.3:
-            ops.append(("S", int(rng.integers(1, 6))))
+            ops.append(("S", int(rng.integers(*clip_len))))
         span = sum(l for o, l in ops if o in "MDN=X")
         if span >= target_len:
             continue

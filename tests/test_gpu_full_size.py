@@ -1,0 +1,95 @@
+"""GPU parity at the benchmark workloads' own sizes (breseq_amd.workloads), through the pipeline
+breseq runs (error_count -> coverage-distribution fit -> identify_mutations, bsq_session_pipeline):
+
+  * C1 as specified (48,502 bp, 100,000 single-end 100-bp reads, about 206x) and C1 with a planted
+    400-bp deletion: the oracle runs live on the same sample (a few seconds);
+  * C2 (4.6 Mb at 100x, the benchmark's sample) and C3 (4.6 Mb at 500x, polymorphism mode): against
+    the oracle's results committed by tests/golden/make_oracle_fixtures.py.
+
+Histograms, covariate counts and every written table file must be bit-exact, the fitted coverage
+summary identical, the DP regions byte-identical, and the GD text line-identical up to one unit in
+the last printed digit of a floating-point field (device vs glibc libm ulps)."""
+import sys
+
+import pytest
+
+import breseq_amd as B
+import oracle_lib as O
+from breseq_amd import api, workloads
+from full_size import compare_with_fixture, fixture, mode_of, product_pipeline, required_cases
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _files_of_oracle(texts):
+    return {"base_qual": {k: v for k, v in texts.items() if k.startswith("base_qual_")},
+            "coverage_text": {k: v for k, v in texts.items() if k.startswith("coverage_")}}
+
+
+def _names(sample):
+    return [n for g in sample.read_file_groups for n in g]
+
+
+@pytest.mark.parametrize("name", ["C1", "C1_DEL"])
+def test_c1_against_live_oracle(name, tmp_path):
+    say("[gpu] full size", name, "(live oracle)")
+    cfg = getattr(workloads, name)
+    s = B.Sample.simulate(**cfg)
+    assert s.aln.contents.n_reads >= 99_000
+    args = api.make_error_count_args(s)
+    hists, counts, log10p, texts = O.error_count(args)
+    fit = O.fit_coverage_distribution(hists[0], sum(s.target_lengths()))
+    table = O.read_log10_prob_table(texts["error_rates"], len(log10p))
+    gd_o, _ = O.identify_mutations(api.make_identify_args(s, table, args.read_set_max, args.quality_max,
+                                                          [fit["deletion_coverage_propagation_cutoff"]]))
+    fx = dict(histograms=[h.tolist() for h in hists], counts=counts.tolist(), error_rates=texts["error_rates"],
+              fit=fit, gd=gd_o, dp="", **_files_of_oracle(texts))
+    ec, summ, gd, dp_text, files, _ = product_pipeline(s, mode_of(name), str(tmp_path))
+    fx["dp"] = dp_text   # single-end: the accumulator has no paired set; its header is checked in test_gpu_dp
+    n, same, errs = compare_with_fixture(fx, ec, summ, gd, dp_text, files, _names(s))
+    req = required_cases(gd, polymorphism=False)
+    say("[gpu]   entries", n, "identical lines", same, "field errors", errs, req)
+    assert req["RA"] >= 20
+    if name == "C1_DEL":
+        assert req["MC_with_ranges"] >= 1
+
+
+@pytest.mark.parametrize("name", ["C2", "C3"])
+def test_full_size_against_oracle_fixture(name, tmp_path):
+    fx = fixture(name)
+    say("[gpu] full size", name, "simulating", fx["aligned_bases"], "aligned bases")
+    s = B.Sample.simulate(**fx["config"])
+    assert s.aligned_bases() == fx["aligned_bases"] and s.aln.contents.n_reads == fx["n_reads"]
+    say("[gpu]   pipeline")
+    ec, summ, gd, dp_text, files, tm = product_pipeline(s, mode_of(name), str(tmp_path))
+    n, same, errs = compare_with_fixture(fx, ec, summ, gd, dp_text, files, _names(s))
+    req = required_cases(gd, polymorphism=(workloads.MODES[name] == "polymorphism"))
+    say("[gpu]   entries", n, "identical lines", same, "field errors", errs, req,
+        "fit %.2f ms, identify waited %.2f ms" % (tm["fit_ms"], tm["fit_wait_ms"]))
+    assert req["MC_with_ranges"] >= 1 and req["RA_insert_slot"] >= 1
+    if "RA_interior_bounds" in req:
+        assert req["RA_interior_bounds"] >= 1
+
+
+def test_pipeline_equals_the_stages_called_one_by_one():
+    """The overlapped pipeline gives what error_count, the coverage stage and identify give called
+    separately in breseq's order."""
+    say("[gpu] pipeline vs separate stages")
+    s = B.Sample.simulate(**dict(workloads.C1_DEL, ref_length=30000, coverage=80.0))
+    sess = B.Session().load(s)
+    try:
+        ec, summ, gd, dp_text = sess.pipeline(mode=B.CONSENSUS)
+        ec2 = sess.error_count()
+        summ2 = api.analyze_unique_coverage_distributions(s.target_lengths(), [h.astype(float) for h in ec2.histograms])
+        gd2 = sess.identify_mutations(log10_prob=ec2.rates_table, read_set_max=ec2.read_set_max,
+                                      quality_max=ec2.quality_max,
+                                      propagation_cutoffs=[x["deletion_coverage_propagation_cutoff"] for x in summ2])
+    finally:
+        sess.close()
+    assert summ == summ2
+    assert gd == gd2

@@ -216,6 +216,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic_of(dom, cfg),
                      "algorithmic_bytes": dbytes},
     }
+    # per-position error of the FP32 column sums against the FP64 sums breseq forms, over every
+    # column with reads (outside the timed steps)
+    log("[bench] precision audit")
+    out["fp32_vs_fp64"] = sess.precision_audit(ec.rates_table, ec.read_set_max, ec.quality_max,
+                                               [x["deletion_coverage_propagation_cutoff"] for x in summaries],
+                                               mode=B.CONSENSUS, dp=dp)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle, 1 core)")
         out["cpu_baseline"] = cpu_baseline(B, cfg, args.cpu_seconds)

@@ -172,6 +172,19 @@ class Timing(C.Structure):
     ]
 
 
+class PrecisionReport(C.Structure):
+    _fields_ = [
+        ("columns", C.c_int64),
+        ("max_abs_error", C.c_double),
+        ("p99_abs_error", C.c_double),
+        ("mean_abs_error", C.c_double),
+        ("max_rel_error", C.c_double),
+        ("columns_redecided", C.c_int64),
+        ("decision_mismatches", C.c_int64),
+        ("count_mismatches", C.c_int64),
+    ]
+
+
 class CoverageSummary(C.Structure):
     _fields_ = [(k, C.c_double) for k in (
         "deletion_coverage_propagation_cutoff", "nbinom_size_parameter", "nbinom_mean_parameter",
@@ -280,6 +293,7 @@ def lib():
     _decl(L, "bsq_session_identify_unscreened", C.c_int, [vp, P(IdentifyArgs), P(vp), st])
     _decl(L, "bsq_session_coverage_table", C.c_int, [vp, P(CoverageTableArgs), P(vp), c_i64p, st])
     _decl(L, "bsq_session_timing", C.c_int, [vp, P(Timing)])
+    _decl(L, "bsq_session_precision_audit", C.c_int, [vp, P(IdentifyArgs), P(PrecisionReport), st])
     _decl(L, "bsq_session_destroy", None, [vp])
     _decl(L, "bsq_error_count", C.c_int, [P(ErrorCountArgs), P(vp), st])
     _decl(L, "bsq_identify_mutations", C.c_int, [P(IdentifyArgs), P(vp), st])

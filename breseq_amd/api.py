@@ -242,6 +242,15 @@ class Session:
         gd, dp_text = _im_texts(im_res)
         return ec, [_summary_dict(x) for x in summ], gd, dp_text
 
+    def precision_audit(self, log10_prob, read_set_max, quality_max, propagation_cutoffs, **kw):
+        """FP32 column sums against breseq's FP64 ones over one identify pass
+        (bsq_session_precision_audit); arguments as for identify_mutations."""
+        args = make_identify_args(self.sample, log10_prob, read_set_max, quality_max, propagation_cutoffs, **kw)
+        rep = L.PrecisionReport()
+        st = _status()
+        L.check(L.lib().bsq_session_precision_audit(self._h, C.byref(args), C.byref(rep), C.byref(st)), st)
+        return {k: getattr(rep, k) for k, _ in L.PrecisionReport._fields_}
+
     def coverage_table(self, target=0, start=0, end=0, resolution=600, total_only=False, per_read_group=False,
                        csv=False, reference_average=None, n_read_groups=None):
         """BAM2COV table of one region (coverage_output::table, coverage_output.cpp:190-283) from the

@@ -216,6 +216,12 @@ __device__ __forceinline__ double genotype_score_approx(const float lps[5], int 
 __device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, int ref_idx, const float lps[5],
                                               double red_bot, double red_top, int uniq, bool insert_slot,
                                               const Screen& A, uint32_t& err) {
+  if (P.audit) {
+    // diagnostics: the column's FP32 sums and read count, for the precision audit
+    float4* a = (float4*)(P.audit + (size_t)8 * p);
+    a[0] = make_float4(lps[0], lps[1], lps[2], lps[3]);
+    a[1] = make_float4(lps[4], __uint_as_float(A.n), 0.0f, 0.0f);
+  }
   // position_coverage::sum (identify_mutations.h:115-120)
   const double r1 = red_bot + red_top;
   const int total = (int)round((double)uniq) + (int)round(r1);
@@ -272,59 +278,70 @@ __device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, i
   P.col_total[p] = total;
 }
 
+// The exact FP64 genotype sums of column p (slot 0), in read order, by one wavefront: its pdata
+// gathered by order-preserving compaction into keys[], then five lanes add the FP64 likelihoods in
+// that order (identify_mutations.cpp:1561-1657, 1669-1689).  Every lane returns the sums in lps[]
+// and the number of reads.  sums[5] is LDS scratch of the wavefront.
+__device__ int exact_column_sums(const ImParams& P, int p, uint32_t* keys, double* sums, double lps[5], uint32_t& err) {
+  const int lane = threadIdx.x & 63;
+  const int S = P.read_set_max, Q = P.quality_max;
+  const int64_t w = p / kWaveCols;
+  const int rb = P.win_rb[w], re = P.win_re[w];
+  if (rb < 0 || re < rb || re > P.n_reads) {
+    err |= kErrIndex;
+    for (int b = 0; b < 5; b++) lps[b] = 0.0;
+    return 0;
+  }
+  int n = 0;
+  for (int r0 = rb; r0 < re; r0 += 64) {
+    const int r = r0 + lane;
+    bool use = false;
+    uint32_t key = 0;
+    int obs = 0, strand_idx = 0;
+    if (r < re) use = pdata_key(P.reads[r], P.cigar, P.bases, P.quals, p, 0, S, Q, P.base_quality_cutoff, key, obs,
+                                strand_idx, err);
+    if (use && key >= (uint32_t)P.tab_keys) {
+      err |= kErrIndex;
+      key = 0;
+    }
+    const unsigned long long mask = __ballot(use);
+    if (use) {
+      const int idx = n + __popcll(mask & ((1ull << lane) - 1ull));
+      if (idx < P.key_cap) keys[idx] = key;
+      else err |= kErrCapacity;
+    }
+    n += __popcll(mask);
+  }
+  n = min(n, P.key_cap);
+  __syncthreads();
+  if (lane < 5) {
+    double acc = 0.0;
+    for (int j = 0; j < n; j++) acc += P.tab_lp[(size_t)keys[j] * 5 + lane];
+    sums[lane] = acc;
+  }
+  __syncthreads();
+  for (int b = 0; b < 5; b++) lps[b] = sums[b];
+  __syncthreads();
+  return n;
+}
+
 // Exact pure-genotype decisions for the columns the column kernel could not certify: one wavefront
-// per column gathers its pdata in read order (order-preserving compaction), five lanes add the FP64
-// likelihoods in that order, and the reference's call is made from those sums.
+// per column, the reference's call made from its exact sums.
 __global__ void __launch_bounds__(64) k_fix_columns(ImParams P) {
   extern __shared__ uint32_t fix_lds_keys[];   // P.key_cap, unless the list is in device scratch
   uint32_t* fix_keys = P.keys_global ? P.key_scratch + (size_t)blockIdx.x * (size_t)P.key_cap : fix_lds_keys;
   __shared__ double fix_lps[5];
   const int lane = threadIdx.x;
   const int n_fix = min(*P.n_fix, P.fix_cap);
-  const int S = P.read_set_max, Q = P.quality_max;
   uint32_t err = 0;
   for (int i = blockIdx.x; i < n_fix; i += gridDim.x) {
     const int p = P.fix[i];
-    const int64_t w = p / kWaveCols;
-    if (p < 0 || p >= P.n_pos || w >= P.n_windows) {
+    if (p < 0 || p >= P.n_pos || p / kWaveCols >= P.n_windows) {
       err |= kErrIndex;
       continue;
     }
-    const int rb = P.win_rb[w], re = P.win_re[w];
-    if (rb < 0 || re < rb || re > P.n_reads) {
-      err |= kErrIndex;
-      continue;
-    }
-    int n = 0;
-    for (int r0 = rb; r0 < re; r0 += 64) {
-      const int r = r0 + lane;
-      bool use = false;
-      uint32_t key = 0;
-      int obs = 0, strand_idx = 0;
-      if (r < re) use = pdata_key(P.reads[r], P.cigar, P.bases, P.quals, p, 0, S, Q, P.base_quality_cutoff, key, obs,
-                                  strand_idx, err);
-      if (use && key >= (uint32_t)P.tab_keys) {
-        err |= kErrIndex;
-        key = 0;
-      }
-      const unsigned long long mask = __ballot(use);
-      if (use) {
-        const int idx = n + __popcll(mask & ((1ull << lane) - 1ull));
-        if (idx < P.key_cap) fix_keys[idx] = key;
-        else err |= kErrCapacity;
-      }
-      n += __popcll(mask);
-    }
-    n = min(n, P.key_cap);
-    __syncthreads();
-    if (lane < 5) {
-      double acc = 0.0;
-      for (int j = 0; j < n; j++) acc += P.tab_lp[(size_t)fix_keys[j] * 5 + lane];
-      fix_lps[lane] = acc;
-    }
-    __syncthreads();
     double lps[5];
-    for (int b = 0; b < 5; b++) lps[b] = fix_lps[b];
+    const int n = exact_column_sums(P, p, fix_keys, fix_lps, lps, err);
     int best;
     double score;
     pure_genotype(lps, (uint32_t)n, best, score);
@@ -335,6 +352,44 @@ __global__ void __launch_bounds__(64) k_fix_columns(ImParams P) {
       if (cbs >= P.mutation_cutoff) fl |= kPosPredicted;
       if ((best != ref_idx) && !isnan(cbs) && (cbs > 0.0)) fl |= kPosConsensus;
       P.col_flags[p] = fl;
+    }
+    __syncthreads();
+  }
+  if (err) atomicOr(P.err, err);
+}
+
+// Precision audit (diagnostics, after an identify pass that wrote P.audit): for every listed
+// column, the exact FP64 sums beside the column pass's FP32 ones, and whether the final predicted
+// and consensus flags equal the calls the exact sums make.  out[6*i .. 6*i+4] = FP64 sums,
+// out[6*i+5] = reads; *mismatches counts columns whose flags differ.
+__global__ void __launch_bounds__(64) k_audit_columns(ImParams P, const int32_t* cols, int32_t n_cols, double* out,
+                                                      uint32_t* mismatches) {
+  extern __shared__ uint32_t audit_lds_keys[];
+  uint32_t* keys = P.keys_global ? P.key_scratch + (size_t)blockIdx.x * (size_t)P.key_cap : audit_lds_keys;
+  __shared__ double sums[5];
+  const int lane = threadIdx.x;
+  uint32_t err = 0;
+  for (int i = blockIdx.x; i < n_cols; i += gridDim.x) {
+    const int p = cols[i];
+    if (p < 0 || p >= P.n_pos || p / kWaveCols >= P.n_windows) {
+      err |= kErrIndex;
+      continue;
+    }
+    double lps[5];
+    const int n = exact_column_sums(P, p, keys, sums, lps, err);
+    int best;
+    double score;
+    pure_genotype(lps, (uint32_t)n, best, score);
+    const double cbs = score - P.log10_ref_length;
+    const int ref_idx = char_index(P.ref[p]);
+    if (lane == 0) {
+      uint8_t want = 0;
+      if (cbs >= P.mutation_cutoff) want |= kPosPredicted;
+      if ((best != ref_idx) && !isnan(cbs) && (cbs > 0.0)) want |= kPosConsensus;
+      const uint8_t have = P.col_flags[p] & (kPosPredicted | kPosConsensus);
+      if (have != want) atomicAdd(mismatches, 1u);
+      for (int b = 0; b < 5; b++) out[(size_t)6 * i + b] = lps[b];
+      out[(size_t)6 * i + 5] = (double)n;
     }
     __syncthreads();
   }

@@ -125,6 +125,7 @@ struct ImParams {
   int32_t* n_fix;
   int32_t fix_cap;
   uint32_t* err;
+  float* audit;                 // diagnostics: per column 8 floats (FP32 sums, read count), or null
 };
 
 // MC / UN interval found by k_runs (kind 0 MC, 1 UN); columns are global.
@@ -193,6 +194,7 @@ constexpr int kFixGrid = 256;                // blocks of k_fix_columns (at most
 constexpr size_t kScoreStaticLds = 28 * 1024;   // bound on the scorer's static LDS
 constexpr size_t kScoreLdsBudget = 52 * 1024;   // static + dynamic LDS of a scorer block: three per CU
 __global__ void k_score_candidates(ImParams P);
+__global__ void k_audit_columns(ImParams P, const int32_t* cols, int32_t n_cols, double* out, uint32_t* mismatches);
 __global__ void k_runs(RunParams P);
 
 }  // namespace bsq

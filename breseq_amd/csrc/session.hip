@@ -67,6 +67,9 @@ struct Session {
   hipEvent_t evk[8] = {};
   bsq_timing timing{};
   bool loaded = false;
+  bool audit_on = false;              // the next identify pass records its FP32 column sums
+  DBuf<float> d_audit;                // 8 floats per column while audit_on
+  ImParams last_P{};                  // parameters of the last identify pass (the audit reuses them)
   // packed metadata (host)
   int n_targets = 0;
   std::vector<std::string> names;
@@ -751,6 +754,13 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   R.n_runs = S.d_counters.p + 2;
   R.run_cap = run_cap;
   R.err = P.err;
+  P.audit = nullptr;
+  if (S.audit_on) {
+    S.d_audit.alloc((size_t)8 * S.n_pos);
+    HIP_OK(hipMemsetAsync(S.d_audit.p, 0, (size_t)8 * S.n_pos * sizeof(float), S.stream));
+    P.audit = S.d_audit.p;
+  }
+  S.last_P = P;
   HIP_OK(hipMemsetAsync(S.d_counters.p, 0, 8 * sizeof(int32_t), S.stream));
   const double th1 = now_ms();
   // the whole pass is queued without a host round trip: later kernels read the counts on device
@@ -958,6 +968,77 @@ static void run_identify(Session& S, const bsq_identify_args* a, bsq_im_result& 
   for (int attempt = 0; attempt < 3; attempt++)
     if (run_identify_once(S, a, out, screen_off, late)) return;
   throw BsqError(BSQ_ERR_LIMIT, "identify result lists could not be sized for this sample");
+}
+
+// Precision audit of the column pass: one identify pass that keeps its FP32 genotype sums, then the
+// exact FP64 sums of every column with reads (k_audit_columns), compared per column and genotype,
+// and the final predicted / consensus flags compared with the exact calls.
+static void run_precision_audit(Session& S, const bsq_identify_args* a, bsq_precision_report& rep) {
+  bsq_im_result out;
+  S.audit_on = true;
+  try {
+    run_identify(S, a, out, 0);
+  } catch (...) {
+    S.audit_on = false;
+    throw;
+  }
+  S.audit_on = false;
+  const int64_t redecided = (int64_t)S.timing.kernel_detail_ms[7];
+  std::vector<float> f32((size_t)8 * S.n_pos);
+  HIP_OK(hipMemcpy(f32.data(), S.d_audit.p, f32.size() * sizeof(float), hipMemcpyDeviceToHost));
+  std::vector<int32_t> cols;
+  for (int64_t p = 0; p < S.n_pos; p++) {
+    uint32_t n;
+    std::memcpy(&n, &f32[(size_t)8 * p + 5], sizeof n);
+    if (n > 0) cols.push_back((int32_t)p);
+  }
+  rep = bsq_precision_report{};
+  rep.columns = (int64_t)cols.size();
+  rep.columns_redecided = redecided;
+  if (cols.empty()) return;
+  DBuf<int32_t> d_cols;
+  DBuf<double> d_out;
+  DBuf<uint32_t> d_mis;
+  d_cols.alloc(cols.size());
+  d_out.alloc((size_t)6 * cols.size());
+  d_mis.alloc(1);
+  HIP_OK(hipMemcpy(d_cols.p, cols.data(), cols.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_OK(hipMemset(d_mis.p, 0, sizeof(uint32_t)));
+  HIP_OK(hipMemset(S.d_counters.p + 3, 0, sizeof(int32_t)));
+  ImParams P = S.last_P;
+  const size_t key_bytes = P.keys_global ? 0 : (((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15);
+  hipLaunchKernelGGL(k_audit_columns, dim3(kFixGrid), dim3(64), key_bytes, S.stream, P, d_cols.p, (int32_t)cols.size(),
+                     d_out.p, d_mis.p);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(S.stream));
+  int32_t errbits = 0;
+  HIP_OK(hipMemcpy(&errbits, S.d_counters.p + 3, sizeof(int32_t), hipMemcpyDeviceToHost));
+  raise_err_bits((uint32_t)errbits);
+  std::vector<double> x64((size_t)6 * cols.size());
+  uint32_t mis = 0;
+  HIP_OK(hipMemcpy(x64.data(), d_out.p, x64.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&mis, d_mis.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  std::vector<double> abs_err;
+  abs_err.reserve((size_t)5 * cols.size());
+  double sum = 0.0, mx = 0.0, mx_rel = 0.0;
+  for (size_t i = 0; i < cols.size(); i++) {
+    uint32_t n32;
+    std::memcpy(&n32, &f32[(size_t)8 * cols[i] + 5], sizeof n32);
+    if ((double)n32 != x64[6 * i + 5]) rep.count_mismatches++;
+    for (int b = 0; b < 5; b++) {
+      const double e = std::fabs((double)f32[(size_t)8 * cols[i] + b] - x64[6 * i + b]);
+      abs_err.push_back(e);
+      sum += e;
+      mx = std::max(mx, e);
+      if (x64[6 * i + b] != 0.0) mx_rel = std::max(mx_rel, e / std::fabs(x64[6 * i + b]));
+    }
+  }
+  std::sort(abs_err.begin(), abs_err.end());
+  rep.max_abs_error = mx;
+  rep.p99_abs_error = abs_err[(size_t)((abs_err.size() - 1) * 0.99)];
+  rep.mean_abs_error = sum / (double)abs_err.size();
+  rep.max_rel_error = mx_rel;
+  rep.decision_mismatches = mis;
 }
 
 // BAM2COV table of one region (coverage_output::table, coverage_output.cpp:190-283; rows of
@@ -1409,6 +1490,20 @@ int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const b
   } catch (const BsqError& ex) {
     set_status(status, ex.code, ex.what());
     return ex.code;
+  }
+}
+
+int bsq_session_precision_audit(bsq_session* s, const bsq_identify_args* args, bsq_precision_report* out,
+                                bsq_status* status) {
+  try {
+    if (!s || !args || !out) throw BsqError(BSQ_ERR_ARG, "null argument");
+    run_precision_audit(s->s, args, *out);
+    set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& e) {
+    s->s.audit_on = false;
+    set_status(status, e.code, e.what());
+    return e.code;
   }
 }
 

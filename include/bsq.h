@@ -359,6 +359,23 @@ int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const b
                          double deletion_coverage_propagation_cutoff, double deletion_coverage_seed_cutoff,
                          const bsq_identify_args* im, bsq_ec_result** ec_out,
                          bsq_coverage_summary* per_target, bsq_im_result** im_out, bsq_status* status);
+/* Diagnostic without a counterpart in breseq: how far the column pass's FP32 genotype sums are
+ * from the FP64 sums breseq forms (identify_mutations.cpp:1669-1689, in read order), over every
+ * column with reads of one identify pass (args as for bsq_session_identify), in log10 units; and
+ * whether the certified FP32 decisions plus the exact re-decisions gave every column the predicted /
+ * consensus call the exact sums give. */
+typedef struct bsq_precision_report {
+  int64_t columns;              /* columns with at least one likelihood term                    */
+  double max_abs_error;         /* max over columns and genotypes of |FP32 sum - FP64 sum|       */
+  double p99_abs_error;         /* its 99th percentile                                          */
+  double mean_abs_error;
+  double max_rel_error;         /* max |FP32 - FP64| / |FP64|                                    */
+  int64_t columns_redecided;    /* columns the exact re-decision kernel recomputed               */
+  int64_t decision_mismatches;  /* columns whose final calls differ from the exact ones (0)      */
+  int64_t count_mismatches;     /* columns whose FP32 read count differs from the exact (0)      */
+} bsq_precision_report;
+int bsq_session_precision_audit(bsq_session* s, const bsq_identify_args* args, bsq_precision_report* out,
+                                bsq_status* status);
 int bsq_session_timing(const bsq_session* s, bsq_timing* out);
 void bsq_session_destroy(bsq_session* s);
 

@@ -85,3 +85,19 @@ def test_sequence_trims_bit_exact():
     l_o, r_o = O.sequence_trims(seq)
     assert np.array_equal(l_g, l_o)
     assert np.array_equal(r_g, r_o)
+
+
+@pytest.mark.parametrize("name,mode", [("pe150", B.CONSENSUS), ("poly", B.POLYMORPHISM)])
+def test_precision_audit(name, mode):
+    """The column pass's FP32 sums against the FP64 sums in read order: every final call equals the
+    exact one, read counts agree, and the relative error stays within the summation bound
+    (n + 2) 2^-24 for columns of a few hundred reads."""
+    say("[gpu] precision audit", name)
+    s, sess = session_for(name)
+    args, (h_o, c_o, l_o, texts) = oracle_error_count(s)
+    table = O.read_log10_prob_table(texts["error_rates"], len(l_o))
+    rep = sess.precision_audit(table, args.read_set_max, args.quality_max, [5.0] * s.n_targets, mode=mode)
+    say("[gpu]  ", rep)
+    assert rep["columns"] > 0.9 * sum(s.target_lengths())
+    assert rep["decision_mismatches"] == 0 and rep["count_mismatches"] == 0
+    assert 0.0 < rep["max_rel_error"] < 1000 * 2.0 ** -24

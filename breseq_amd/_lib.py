@@ -75,6 +75,9 @@ class ErrorCountArgs(C.Structure):
         ("do_errors", C.c_int32),
         ("output_dir", C.c_char_p),
         ("error_rates_file", C.c_char_p),
+        ("preprocess_stage", C.c_int32),
+        ("unmatched_end_minimum_read_length", C.c_uint32),
+        ("unmatched_end_length_factor", C.c_double),
     ]
 
 
@@ -302,6 +305,7 @@ def lib():
     _decl(L, "bsq_ec_result_log10_prob", C.c_int, [vp, P(c_f64p), c_i64p])
     _decl(L, "bsq_ec_result_rates_table", C.c_int, [vp, P(c_f64p), c_i64p])
     _decl(L, "bsq_ec_result_rates_text", C.c_int, [vp, P(C.c_char_p), c_i64p])
+    _decl(L, "bsq_ec_result_no_pos_hash_pr", C.c_int, [vp, P(c_f64p), c_i64p])
     _decl(L, "bsq_ec_result_free", None, [vp])
     _decl(L, "bsq_im_result_gd_text", C.c_int, [vp, P(C.c_char_p), c_i64p])
     _decl(L, "bsq_im_result_counts", C.c_int, [vp, c_i64p, c_i64p, c_i64p])

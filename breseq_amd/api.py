@@ -178,6 +178,7 @@ class ErrorCountResult:
     rates_table: np.ndarray = None    # the table identify_mutations reads back from error_rates.tab
     rates_text: str = ""
     texts: dict = field(default_factory=dict)
+    no_pos_hash_pr: np.ndarray = None   # preprocess mode, per target
 
 
 class Session:
@@ -317,7 +318,10 @@ def _target_arrays(sample, call_targets):
 
 
 def make_error_count_args(sample, read_set_max=None, quality_max=None, coverage_groups=None, call_targets=None,
-                          output_dir=None, error_rates_file=None, do_coverage=True, do_errors=True):
+                          output_dir=None, error_rates_file=None, do_coverage=True, do_errors=True,
+                          preprocess_stage=False, unmatched_end_minimum_read_length=50, require_match_fraction=0.9):
+    """bsq_error_count_args; the preprocess settings follow Settings (settings.cpp:1272, 1308-1309):
+    unmatched_end_length_factor = 1 - require_match_fraction."""
     n = sample.n_targets
     if quality_max is None:
         quality_max = sample.max_quality() + 1
@@ -340,6 +344,9 @@ def make_error_count_args(sample, read_set_max=None, quality_max=None, coverage_
     args.do_errors = 1 if do_errors else 0
     args.output_dir = output_dir.encode() if output_dir else None
     args.error_rates_file = error_rates_file.encode() if error_rates_file else None
+    args.preprocess_stage = 1 if preprocess_stage else 0
+    args.unmatched_end_minimum_read_length = unmatched_end_minimum_read_length
+    args.unmatched_end_length_factor = 1 - require_match_fraction
     args._keep = (g, call, rf)
     return args
 
@@ -379,9 +386,12 @@ def _ec_result(res, args):
         t = C.c_char_p()
         lib.bsq_ec_result_rates_text(res, C.byref(t), C.byref(n))
         text = C.string_at(t, n.value).decode()
+        lib.bsq_ec_result_no_pos_hash_pr(res, C.byref(pd), C.byref(n))
+        npr = _np(pd, n.value, np.float64).copy()
     finally:
         lib.bsq_ec_result_free(res)
-    return ErrorCountResult(hists, counts, log10p, args.read_set_max, args.quality_max, rates, text)
+    return ErrorCountResult(hists, counts, log10p, args.read_set_max, args.quality_max, rates, text,
+                            no_pos_hash_pr=npr)
 
 
 def mode_settings(mode):

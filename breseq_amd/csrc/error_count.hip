@@ -356,6 +356,59 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* Preprocess mode: where qualifying reads start                                               */
+
+// One thread per alignment: a unique alignment of a scored target whose first query base is
+// aligned (nothing but deletions, skips, hard clips or pads before its first M/=/X run) marks its
+// strand's bit at that column when its stranded aligned end (query_stranded_end_1 over
+// query_bounds_1: soft clips, hard clips and skips at either end excluded; alignment.cpp:120-239)
+// reaches required_junction_read_end_min_coordinate(read length) (settings.h:345-354).  The
+// pileup entry there has query_position_1 == 1 (error_count.cpp:159-166).
+__global__ void __launch_bounds__(256) k_read_starts(EcParams P) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n_reads) return;
+  const DRead R = P.reads[i];
+  const int t = P.pos_target[R.gpos];
+  if (t < 0 || !P.target_scored[t] || R.x1 > 1) return;
+  const uint32_t* c = P.cigar + R.cig_off;
+  const int n = R.n_cigar;
+  int x = R.gpos, col = -1;
+  for (int k = 0; k < n; k++) {
+    const uint32_t op = c[k] & 15u;
+    const int l = (int)(c[k] >> 4);
+    if (op == OP_M || op == OP_EQ || op == OP_X) {
+      if (l > 0) col = x;
+      break;
+    }
+    if (op == OP_I || op == OP_S) break;   // a query base comes first
+    if (op == OP_D || op == OP_N) x += l;
+  }
+  if (col < 0) return;
+  uint32_t start1 = 1, qlen = 0;
+  for (int k = 0; k < n; k++) {
+    const uint32_t op = c[k] & 15u;
+    if (op == OP_M || op == OP_I || op == OP_S || op == OP_EQ || op == OP_X) qlen += c[k] >> 4;
+  }
+  for (int k = 0; k < n; k++) {
+    const uint32_t op = c[k] & 15u;
+    if (op != OP_S && op != OP_H && op != OP_N) break;
+    if (op == OP_S) start1 += c[k] >> 4;
+  }
+  uint32_t end1 = qlen;
+  for (int k = n - 1; k > 0; k--) {
+    const uint32_t op = c[k] & 15u;
+    if (op != OP_S && op != OP_H && op != OP_N) break;
+    if (op == OP_S) end1 -= c[k] >> 4;
+  }
+  const bool rev = (R.flags & kReadReversed) != 0;
+  const uint32_t rl = R.l_qseq;
+  const uint32_t stranded_end = rev ? rl - start1 + 1u : end1;
+  const int32_t m = (int32_t)floor((double)((int32_t)rl - (int32_t)P.end_min_len) * P.end_factor);
+  const uint32_t need = (m <= 0) ? rl : rl - (uint32_t)m;
+  if (stranded_end >= need) atomicOr(&P.start_bits[col >> 4], 1u << ((col & 15) * 2 + (rev ? 1 : 0)));
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* Coverage histogram: prefix scan of the difference arrays                                    */
 
 __device__ __forceinline__ int wave_sum(int v) {
@@ -469,6 +522,9 @@ __global__ void __launch_bounds__(256) k_cov_hist(CovParams C) {
     orr += wtot[1][k];
   }
   uint32_t err = 0;
+  // preprocess mode: (position, strand) pairs without / with a qualifying start, per target
+  int st_t = -1;
+  uint32_t st_none = 0, st_some = 0;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     const int64_t col = c0 + i;
@@ -481,6 +537,24 @@ __global__ void __launch_bounds__(256) k_cov_hist(CovParams C) {
     if (g < C.n_groups_lds && cu < C.hist_lds) atomicAdd(&hist[g * C.hist_lds + cu], 1u);
     else if (cu >= 0 && cu < C.hist_len) atomicAdd(&C.hist[(int64_t)g * C.hist_len + cu], 1ull);
     else err |= kErrCapacity;
+    if (C.start_bits) {
+      if (t != st_t) {
+        if (st_t >= 0) {
+          atomicAdd(&C.start_counts[2 * st_t], (unsigned long long)st_none);
+          atomicAdd(&C.start_counts[2 * st_t + 1], (unsigned long long)st_some);
+        }
+        st_t = t;
+        st_none = st_some = 0;
+      }
+      const uint32_t b = (C.start_bits[col >> 4] >> ((col & 15) * 2)) & 3u;
+      const uint32_t some = (b & 1u) + (b >> 1);
+      st_some += some;
+      st_none += 2u - some;
+    }
+  }
+  if (st_t >= 0) {
+    atomicAdd(&C.start_counts[2 * st_t], (unsigned long long)st_none);
+    atomicAdd(&C.start_counts[2 * st_t + 1], (unsigned long long)st_some);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nh; i += blockDim.x) {

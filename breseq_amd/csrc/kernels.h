@@ -28,6 +28,10 @@ struct EcParams {
   int32_t* cov_diff;            // [0, n_pos+1): unique reads, [n_pos+1, 2n_pos+2): redundant reads
   unsigned long long* counts;   // n_bins
   uint32_t* err;
+  // preprocess mode: 2 bits per column (forward, reverse) set where a qualifying read starts
+  uint32_t* start_bits;         // null unless preprocess
+  uint32_t end_min_len;         // unmatched_end_minimum_read_length
+  double end_factor;            // unmatched_end_length_factor
 };
 
 // Pass 1, coverage histogram from the difference arrays (three-phase prefix scan).
@@ -43,6 +47,8 @@ struct CovParams {
   int32_t hist_len;                 // global histogram length per group
   unsigned long long* hist;         // n_groups * hist_len
   uint32_t* err;
+  const uint32_t* start_bits;       // preprocess mode (EcParams::start_bits), else null
+  unsigned long long* start_counts; // per target: (position, strand) pairs without / with a start
 };
 constexpr int kCovTile = 4096;      // columns per block of the coverage scan (256 threads x 16)
 constexpr int kEcChunk = 32;        // consecutive alignments per wavefront work item of k_ec_reads
@@ -194,6 +200,7 @@ constexpr int kFixGrid = 256;                // blocks of k_fix_columns (at most
 constexpr size_t kScoreStaticLds = 28 * 1024;   // bound on the scorer's static LDS
 constexpr size_t kScoreLdsBudget = 52 * 1024;   // static + dynamic LDS of a scorer block: three per CU
 __global__ void k_score_candidates(ImParams P);
+__global__ void k_read_starts(EcParams P);
 __global__ void k_audit_columns(ImParams P, const int32_t* cols, int32_t n_cols, double* out, uint32_t* mismatches);
 __global__ void k_runs(RunParams P);
 

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <fstream>
 #include <future>
+#include <limits>
 #include <map>
 #include <memory>
 #include <string>
@@ -108,6 +109,8 @@ struct Session {
   DBuf<unsigned long long> d_counts, d_hist;
   DBuf<uint32_t> d_err;
   DBuf<double> d_tab_lp, d_tab_r;
+  DBuf<uint32_t> d_start_bits;        // preprocess mode: read-start bits, 2 per column
+  DBuf<unsigned long long> d_start_counts;   // preprocess mode: per target (no start, start)
   DBuf<uint32_t> d_key_scratch;      // read lists of the scorer and k_fix_columns for deep samples
   DBuf<uint8_t> d_col_flags;
   DBuf<int32_t> d_col_unique, d_col_total;
@@ -418,6 +421,7 @@ struct bsq_session {
 };
 
 struct bsq_ec_result {
+  std::vector<double> no_pos_hash_pr;   // preprocess mode, per target (NaN when not visited)
   std::vector<std::vector<uint32_t>> hist;
   std::vector<uint64_t> counts;
   std::vector<double> log10p;
@@ -495,6 +499,22 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   C.hist_len = hist_len;
   C.hist = S.d_hist.p;
   C.err = S.d_err.p;
+  const bool pre = args->preprocess_stage != 0;
+  P.start_bits = nullptr;
+  P.end_min_len = args->unmatched_end_minimum_read_length;
+  P.end_factor = args->unmatched_end_length_factor;
+  C.start_bits = nullptr;
+  C.start_counts = nullptr;
+  if (pre) {
+    const size_t words = ((size_t)S.n_pos + 15) / 16;
+    S.d_start_bits.alloc(words);
+    S.d_start_counts.alloc((size_t)2 * S.n_targets);
+    HIP_OK(hipMemsetAsync(S.d_start_bits.p, 0, words * sizeof(uint32_t), S.stream));
+    HIP_OK(hipMemsetAsync(S.d_start_counts.p, 0, (size_t)2 * S.n_targets * sizeof(unsigned long long), S.stream));
+    P.start_bits = S.d_start_bits.p;
+    C.start_bits = S.d_start_bits.p;
+    C.start_counts = S.d_start_counts.p;
+  }
   const size_t lds_table = (size_t)n_bins * sizeof(uint32_t);
   const size_t lds_hist = (size_t)C.n_groups_lds * C.hist_lds * sizeof(uint32_t);
   if (lds_table > 64 * 1024) throw BsqError(BSQ_ERR_LIMIT, "covariate table too large for LDS");
@@ -508,6 +528,7 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, S.stream, S.d_cov_diff.p, n_diff, 0);
   hipLaunchKernelGGL(k_ec_reads, dim3(ec_grid), dim3(256), lds_table, S.stream, P);
   HIP_OK(hipGetLastError());
+  if (pre) hipLaunchKernelGGL(k_read_starts, dim3((unsigned)((S.n_dreads + 255) / 256)), dim3(256), 0, S.stream, P);
   HIP_OK(hipEventRecord(S.evk[0], S.stream));
   hipLaunchKernelGGL(k_cov_tile_sums, dim3(n_tiles), dim3(256), 0, S.stream, C);
   hipLaunchKernelGGL(k_cov_tile_scan, dim3(1), dim3(1024), 0, S.stream, C);
@@ -539,6 +560,16 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
     // have entries
     out.hist[g].assign((size_t)(last + 1), 0);
     for (int64_t c = 0; c <= last; c++) out.hist[g][c] = (uint32_t)hist[(size_t)g * hist_len + c];
+  }
+  out.no_pos_hash_pr.assign(S.n_targets, std::numeric_limits<double>::quiet_NaN());
+  if (pre) {
+    std::vector<unsigned long long> sc((size_t)2 * S.n_targets);
+    HIP_OK(hipMemcpy(sc.data(), S.d_start_counts.p, sc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int t = 0; t < S.n_targets; t++) {
+      if (!scored[t]) continue;
+      const double total = (double)(sc[2 * t] + sc[2 * t + 1]);
+      out.no_pos_hash_pr[t] = (total != 0) ? (double)sc[2 * t] / total : 1.0;
+    }
   }
   counts_to_log10_prob(Sm, Q, out.counts, out.log10p);
   out.rates_text = error_rates_text(Sm, Q, out.log10p);
@@ -1571,6 +1602,13 @@ int bsq_ec_result_rates_text(const bsq_ec_result* r, const char** text, int64_t*
   if (!r) return BSQ_ERR_ARG;
   *text = r->rates_text.c_str();
   *len = (int64_t)r->rates_text.size();
+  return BSQ_OK;
+}
+
+int bsq_ec_result_no_pos_hash_pr(const bsq_ec_result* r, const double** v, int64_t* n) {
+  if (!r || !v || !n) return BSQ_ERR_ARG;
+  *v = r->no_pos_hash_pr.data();
+  *n = (int64_t)r->no_pos_hash_pr.size();
   return BSQ_OK;
 }
 

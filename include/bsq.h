@@ -106,6 +106,14 @@ typedef struct bsq_error_count_args {
   int32_t do_errors;               /* write error_rates.tab and base_qual_error_prob.#.tab     */
   const char* output_dir;          /* NULL: compute only, write nothing                        */
   const char* error_rates_file;    /* full path of error_rates.tab (Settings, settings.cpp:1555) */
+  /* preprocess_stage (error_count.h:41-52; caller breseq_cmdline.cpp:1969, coverage only): per
+   * scored target, the fraction of (position, strand) pairs at positions without redundant reads
+   * where no unique read starts (its first query base aligned there) with a stranded aligned end
+   * at or beyond required_junction_read_end_min_coordinate(read length) (settings.h:345-354;
+   * error_count.cpp:159-166, 187-190, 209-228) */
+  int32_t preprocess_stage;
+  uint32_t unmatched_end_minimum_read_length;   /* Settings (50)                              */
+  double unmatched_end_length_factor;           /* Settings: 1 - require_match_fraction       */
 } bsq_error_count_args;
 
 typedef struct bsq_ec_result bsq_ec_result;
@@ -123,6 +131,9 @@ int bsq_ec_result_log10_prob(const bsq_ec_result* r, const double** table, int64
 int bsq_ec_result_rates_table(const bsq_ec_result* r, const double** table, int64_t* n_bins);
 /* Text of error_rates.tab. */
 int bsq_ec_result_rates_text(const bsq_ec_result* r, const char** text, int64_t* len);
+/* Preprocess mode: Summary::preprocess_error_count[seq_id].no_pos_hash_per_position_pr of each
+ * target (NaN for targets the pass did not visit, or without preprocess_stage). */
+int bsq_ec_result_no_pos_hash_pr(const bsq_ec_result* r, const double** per_target, int64_t* n_targets);
 void bsq_ec_result_free(bsq_ec_result* r);
 
 /* ------------------------------------------------------------------------------------------ */

@@ -427,6 +427,7 @@ struct ErrorTable {
 };
 
 struct EcOut {
+  vector<double> no_pos_hash_pr;   // preprocess mode, per target (NaN when not visited)
   vector<vector<uint32_t>> hist;
   vector<uint64_t> counts;
   vector<double> log10p;
@@ -491,25 +492,52 @@ static void error_count(const bsq_error_count_args* args, EcOut& out) {
   if (T.S <= 0 || T.Q <= 0) fail(BSQ_ERR_ARG, "covariate maxima must be positive");
   T.counts.assign(T.nbins(), 0);
   out.hist.assign(args->n_coverage_groups, vector<uint32_t>());
+  out.no_pos_hash_pr.assign(args->ref->n_targets, NAN);
+  // required_junction_read_end_min_coordinate (settings.h:345-354)
+  auto end_min = [&](uint32_t read_length) -> uint32_t {
+    const int32_t m = (int32_t)floor((double)((int32_t)read_length - (int32_t)args->unmatched_end_minimum_read_length) *
+                                     args->unmatched_end_length_factor);
+    return (m <= 0) ? read_length : read_length - (uint32_t)m;
+  };
+  // query_stranded_end_1 (alignment.cpp:228-239 over query_bounds_1)
+  auto stranded_end_1 = [&](int64_t r) -> uint32_t {
+    uint32_t s0, e0;
+    A.query_bounds_0(r, s0, e0);
+    return A.reversed(r) ? A.len(r) - (s0 + 1) + 1 : e0 + 1;
+  };
   for (int32_t tid : target_order(args->ref, args->call_target)) {
     int32_t g = args->coverage_group ? args->coverage_group[tid] : 0;
     if (g < 0 || g >= args->n_coverage_groups) fail(BSQ_ERR_ARG, "coverage group out of range");
     vector<uint32_t>& h = out.hist[g];
+    uint64_t found[2] = {0, 0};   // at_target_start (error_count.cpp:209-214)
     for_each_column(A, tid, args->ref->lengths[tid], [&](int64_t p, const vector<Entry>& col) {
       size_t unique_coverage = 0;
       bool has_redundant = false;
+      int has_query_start[2] = {0, 0};
       for (const Entry& e : col) {
         if (e.is_del) continue;
         if (A.redundancy(e.read) > 1) { has_redundant = true; continue; }
         ++unique_coverage;
+        if (args->preprocess_stage) {
+          if (e.qpos + 1 == 1 && stranded_end_1(e.read) >= end_min(A.len(e.read)))
+            has_query_start[A.reversed(e.read) ? 1 : 0] = 1;
+        }
         if (!args->do_errors) continue;
         count_alignment_position(A, args->ref, tid, args->read_files, T, e, p);
       }
       if (!has_redundant) {
         if (unique_coverage >= h.size()) h.resize(unique_coverage + 1, 0);
         ++h[unique_coverage];
+        if (args->preprocess_stage) {
+          found[has_query_start[0]]++;
+          found[has_query_start[1]]++;
+        }
       }
     });
+    if (args->preprocess_stage) {   // at_target_end (error_count.cpp:217-229)
+      const double total = (double)(found[0] + found[1]);
+      out.no_pos_hash_pr[tid] = (total != 0) ? (double)found[0] / total : 1.0;
+    }
   }
   out.counts = T.counts;
   // counts_to_log10_prob (error_count.cpp:1005-1026)
@@ -1674,6 +1702,12 @@ int orc_ec_text(const orc_ec_out* o, int32_t which, int32_t index, const char** 
   *text = s->c_str();
   *len = (int64_t)s->size();
   return BSQ_OK;
+}
+
+int orc_ec_no_pos_hash_pr(const orc_ec_out* o, const double** v, int64_t* n) {
+  *v = o->o.no_pos_hash_pr.data();
+  *n = (int64_t)o->o.no_pos_hash_pr.size();
+  return 0;
 }
 
 void orc_ec_free(orc_ec_out* o) { delete o; }

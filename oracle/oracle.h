@@ -29,6 +29,8 @@ int orc_ec_log10_prob(const orc_ec_out* o, const double** t, int64_t* n);
 /* which: 0 = error_rates.tab, 1 = coverage distribution of group `index`,
  *        2 = base_qual_error_prob of read file `index` */
 int orc_ec_text(const orc_ec_out* o, int32_t which, int32_t index, const char** text, int64_t* len);
+/* preprocess_stage: no_pos_hash_per_position_pr per target (NaN where not visited) */
+int orc_ec_no_pos_hash_pr(const orc_ec_out* o, const double** per_target, int64_t* n);
 void orc_ec_free(orc_ec_out* o);
 
 /* Parse error_rates.tab text as read_log10_prob_table (error_count.cpp:629-654). */

@@ -35,6 +35,7 @@ def lib():
     O.orc_ec_log10_prob.argtypes = [vp, P(L.c_f64p), L.c_i64p]
     O.orc_ec_text.argtypes = [vp, C.c_int32, C.c_int32, P(C.c_char_p), L.c_i64p]
     O.orc_ec_free.argtypes = [vp]
+    O.orc_ec_no_pos_hash_pr.argtypes = [vp, P(L.c_f64p), L.c_i64p]
     O.orc_read_log10_prob_table.argtypes = [C.c_char_p, L.c_f64p, C.c_int64]
     O.orc_identify_mutations.argtypes = [P(L.IdentifyArgs), P(vp), st]
     O.orc_im_gd_text.argtypes = [vp, P(C.c_char_p), L.c_i64p]
@@ -88,6 +89,8 @@ def error_count(args):
         for r in range(args.read_set_max):
             if O.orc_ec_text(h, 2, r, C.byref(t), C.byref(n)) == 0:
                 texts["base_qual_%d" % r] = C.string_at(t, n.value).decode()
+        O.orc_ec_no_pos_hash_pr(h, C.byref(pd), C.byref(n))
+        texts["no_pos_hash_pr"] = np.ctypeslib.as_array(pd, shape=(n.value,)).copy()
     finally:
         O.orc_ec_free(h)
     return hists, counts, log10p, texts

@@ -55,3 +55,26 @@ def test_identify_cigar_shapes_match_oracle(seed, mode):
     n, same, errs = compare_gd(gd_g, gd_o)
     say("[gpu]   entries", n, "identical", same, errs)
     assert n > 10
+
+
+@pytest.mark.parametrize("seed", [1, 4])
+def test_preprocess_read_starts_match_oracle(seed):
+    """error_count's preprocess mode (coverage only, read starts per target) on every CIGAR shape,
+    and on a simulated paired sample with soft-clipped and redundant reads."""
+    say("[gpu] preprocess read starts seed", seed)
+    if seed == 4:
+        refs, reads = random_alignments(seed, n_targets=3, target_len=3000, n_reads=900, run_len=(30, 120),
+                                        clip_len=(1, 30))
+        s = BuiltSample(Built(refs, reads))
+    else:
+        s = B.Sample.simulate(seed=17, ref_length=40000, read_length=150, coverage=30.0, paired=True,
+                              redundant_fraction=0.05)
+    for frac in (0.9, 0.5):
+        kw = dict(preprocess_stage=True, do_errors=False, require_match_fraction=frac)
+        args = B.api.make_error_count_args(s, **kw)
+        h_o, _, _, texts = O.error_count(args)
+        res = B.api.error_count(s, **kw)
+        for g in range(len(h_o)):
+            assert np.array_equal(res.histograms[g], h_o[g])
+        assert np.array_equal(res.no_pos_hash_pr, texts["no_pos_hash_pr"]), (res.no_pos_hash_pr, texts["no_pos_hash_pr"])
+        assert np.all(res.no_pos_hash_pr < 1.0)

@@ -424,3 +424,52 @@ def test_base_qual_error_prob_normalization_digits():
     cols = [x + y for x in "ACGT." for y in "ACGT."]
     row = dict(zip(cols, lines[1 + 20].split("\t")[1:]))
     assert (row["GA"], row["GC"], row["GG"], row["GT"], row["G."]) == ("0.333333", "0.333333", "0.333333", "NA", "NA")
+
+
+def test_preprocess_read_starts_by_hand():
+    """preprocess_stage (error_count.cpp:159-166, 187-190, 209-228): per scored target, the share of
+    (position, strand) pairs at positions without redundant reads where no unique read has its first
+    query base aligned with a stranded aligned end >= required_junction_read_end_min_coordinate."""
+    import breseq_amd as B
+    from aln_builder import Built, BuiltSample
+
+    rng = np.random.default_rng(3)
+    ref = "".join(rng.choice(list("ACGT"), 400))
+    F, R = 0, 16
+
+    def read(pos, cigar, flag=F, x1=-1):
+        import re as _re
+        qlen = sum(int(n) for n, o in _re.findall(r"(\d+)([MIS=X])", cigar))
+        return dict(pos=pos, cigar=cigar, seq="A" * qlen, flag=flag, x1=x1)
+
+    reads = [
+        read(10, "97M3S"),            # forward, first base aligned at 10, end 97 >= 95: start (10, +)
+        read(20, "3S97M"),            # starts with a clip: no start
+        read(30, "90M10S"),           # forward end 90 < 95: no start
+        read(40, "90M10S", R),        # reverse: stranded end = 100 - 1 + 1 = 100: start (40, -)
+        read(40, "10S90M", R),        # reverse: stranded end = 100 - 11 + 1 = 90 < 95: no start
+        read(50, "2D98M2S"),          # a leading deletion: first base at 52, end 98: start (52, +)
+        read(60, "100M", F, x1=2),    # redundant: no start, and columns 60..159 are not counted
+        read(70, "100M"),             # starts inside the redundant stretch: not counted
+        read(200, "100M", R),         # start (200, -)
+        read(200, "100M"),            # start (200, +)
+        read(210, "1I99M"),           # an insertion first: no start
+    ]
+    reads.sort(key=lambda r: r["pos"])
+    s = BuiltSample(Built([("t0", ref)], reads))
+    args = B.api.make_error_count_args(s, preprocess_stage=True, unmatched_end_minimum_read_length=50,
+                                       require_match_fraction=0.9)
+    _, _, _, texts = O.error_count(args)
+    # required end for 100-base reads: 100 - floor((100 - 50) * (1 - 0.9)) = 95
+    # redundant columns 60..159 drop out: 400 - 100 = 300 positions, 600 (position, strand) pairs;
+    # starts at (10,+), (40,-), (52,+), (200,-), (200,+) -- none inside 60..159
+    assert texts["no_pos_hash_pr"][0] == (600 - 5) / 600
+    # a factor that makes every end qualify adds (30,+) and (40,-) of "10S90M" (same pair) only
+    args2 = B.api.make_error_count_args(s, preprocess_stage=True, unmatched_end_minimum_read_length=50,
+                                        require_match_fraction=0.0)
+    _, _, _, texts2 = O.error_count(args2)
+    # required end 100 - floor(50 * 1.0) = 50: (30,+) qualifies, the reverse 10S90M shares (40,-)
+    assert texts2["no_pos_hash_pr"][0] == (600 - 6) / 600
+    # without preprocess_stage nothing is reported
+    _, _, _, texts3 = O.error_count(B.api.make_error_count_args(s))
+    assert np.isnan(texts3["no_pos_hash_pr"][0])

@@ -8,7 +8,7 @@ from breseq_amd import workloads
 from full_size import GOLDEN, fixture, required_cases
 
 
-@pytest.mark.parametrize("name", ["C2"])
+@pytest.mark.parametrize("name", ["C2", "C3"])
 def test_fixture_matches_its_workload(name):
     if not os.path.exists(os.path.join(GOLDEN, "oracle_%s.json.gz" % name)):
         pytest.fail("fixture of %s missing: run tests/golden/make_oracle_fixtures.py %s" % (name, name))

@@ -3,6 +3,7 @@
 Usage: python -m breseq_amd.build   (or build() from __graft_entry__).
 """
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -42,6 +43,39 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+_LABEL = re.compile(r"^\.LBB\w+:|^; %bb\.\d+:")
+_FUNC = re.compile(r"^(_Z\w+):")
+
+
+def spill_before_exec_restore(asm_text):
+    """Places in gfx950 assembly where a basic block stores a VGPR spill before the block's own
+    exec-mask restore (the s_or_b64 exec, exec, ... that closes a divergent branch).  Such a store
+    runs with the branch's narrowed mask, so the lanes outside it keep whatever the spill slot held
+    before, and the reload after the restore hands those lanes another value.  Returns
+    [(function, line of the block label, spill instruction)]."""
+    hits = []
+    func, block_line, pending, exec_written = None, 0, [], False
+    for i, line in enumerate(asm_text.split("\n")):
+        m = _FUNC.match(line)
+        if m:
+            func = m.group(1)
+        if _LABEL.match(line):
+            block_line, pending, exec_written = i + 1, [], False
+            continue
+        s = line.split(";")[0].strip() if not line.strip().startswith(";") else ""
+        if not s:
+            continue
+        if "Folded Spill" in line and s.startswith(("scratch_store", "buffer_store")) and not exec_written:
+            pending.append(s)
+        if re.match(r"s_\w+\s+exec\b", s):
+            if s.startswith("s_or_b64 exec, exec") and pending:
+                hits.extend((func, block_line, sp) for sp in pending)
+            exec_written, pending = True, []
+        if s.startswith(("s_branch", "s_cbranch", "s_setpc", "s_endpgm")):
+            pending = []
+    return hits
+
+
 def build(verbose=False, jobs=None):
     os.makedirs(OBJ_DIR, exist_ok=True)
     cc = hipcc()
@@ -58,6 +92,11 @@ def build(verbose=False, jobs=None):
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        if src.endswith(".hip"):
+            # the device assembly, checked below for spill placement
+            asm = [cc] + COMMON + HIP_ONLY + ["-x", "hip", "--cuda-device-only", "-S", path, "-o",
+                                              os.path.join(OBJ_DIR, src + ".s")]
+            procs.append((src + " (assembly)", subprocess.Popen(asm, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
     failed = []
     for src, p in procs:
         out, _ = p.communicate()
@@ -70,6 +109,16 @@ def build(verbose=False, jobs=None):
         for src, text in failed:
             sys.stderr.write("---- %s ----\n%s\n" % (src, text))
         raise RuntimeError("compilation failed: " + ", ".join(s for s, _ in failed))
+    bad = []
+    for src in SOURCES:
+        if src.endswith(".hip"):
+            with open(os.path.join(OBJ_DIR, src + ".s")) as f:
+                bad += [(src,) + h for h in spill_before_exec_restore(f.read())]
+    if bad:
+        for src, func, line, spill in bad:
+            sys.stderr.write("%s: %s, block at assembly line %d: %s\n" % (src, func, line, spill))
+        raise RuntimeError("the compiler placed VGPR spills before an exec-mask restore (lanes outside "
+                           "the branch would reload stale values); restructure the named kernels")
     tmp = LIB + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"]
     if verbose:

@@ -10,6 +10,7 @@
 
 #include <string>
 
+#include "engine.h"
 #include "host_util.h"
 
 namespace bsq {
@@ -40,7 +41,8 @@ __global__ void __launch_bounds__(256) k_repeat_extent(const char* __restrict__ 
   m[p] = (uint8_t)mx;
 }
 
-__global__ void __launch_bounds__(256) k_trim_gather(const uint8_t* __restrict__ m, int64_t len, uint8_t* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_trim_gather(const uint8_t* __restrict__ m, int64_t len, uint8_t* __restrict__ left_out,
+                                                     uint8_t* __restrict__ right_out) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= len) return;
   uint32_t right = 0, left = 0;
@@ -53,8 +55,82 @@ __global__ void __launch_bounds__(256) k_trim_gather(const uint8_t* __restrict__
       left = max(left, mp - o);
     }
   }
-  out[x] = (uint8_t)left;
-  out[len + x] = (uint8_t)right;
+  left_out[x] = (uint8_t)left;
+  right_out[x] = (uint8_t)right;
+}
+
+// BAM code -> the character breseq compares (basebam2char, common.h:263-275): A C G T, else N
+__device__ __forceinline__ char trim_char(uint8_t b) {
+  switch (b) {
+    case 1: return 'A';
+    case 2: return 'C';
+    case 4: return 'G';
+    case 8: return 'T';
+    default: return 'N';
+  }
+}
+
+// get_alignment_trims (calculate_trims.cpp:239-285) of every packed alignment, one thread each:
+//   the genomic trims at the alignment's first and last reference positions, the read-edge trims
+//   of its aligned sequence (edge_trims_for_sequence, :183-230: the longest perfect tandem repeat of
+//   unit 1..18 starting at either end, at least 1), the larger of each pair, plus the bases outside
+//   the match (query_start_0 on the left, read length - query_end_1 on the right; soft clips only,
+//   alignment.cpp:248-289).  A repeat compared on the reverse complement matches exactly where the
+//   reversed sequence does (complementing is a bijection on A, C, G, T, N), so the right edge reads
+//   the sequence backwards.  left/right: the sample's per-column trims (global columns).
+__global__ void __launch_bounds__(256) k_alignment_trims(const DRead* __restrict__ reads, int64_t n,
+                                                         const uint32_t* __restrict__ cigar,
+                                                         const uint8_t* __restrict__ bases,
+                                                         const uint8_t* __restrict__ left, const uint8_t* __restrict__ right,
+                                                         int32_t* __restrict__ xl, int32_t* __restrict__ xr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DRead R = reads[i];
+  const uint32_t* c = cigar + R.cig_off;
+  const int nc = R.n_cigar;
+  uint32_t qs1 = 1, qlen = 0;
+  for (int k = 0; k < nc; k++) {
+    if ((c[k] & 15u) != 4u) break;
+    qs1 += c[k] >> 4;
+  }
+  for (int k = 0; k < nc; k++) {
+    const uint32_t op = c[k] & 15u;
+    if (op == 0 || op == 1 || op == 4 || op == 7 || op == 8) qlen += c[k] >> 4;
+  }
+  uint32_t qe1 = qlen;
+  for (int k = nc - 1; k > 0; k--) {
+    if ((c[k] & 15u) != 4u) break;
+    qe1 -= c[k] >> 4;
+  }
+  const uint8_t* s = bases + R.seq_off + (qs1 - 1);
+  const int64_t len = (qe1 >= qs1) ? (int64_t)(qe1 - qs1 + 1) : 0;
+  uint32_t eL = 1, eR = 1;
+  for (int size = 1; size <= 18; size++) {
+    // repeat_match(seq, 0, size, num) (calculate_trims.cpp:44-56), from each end
+    int64_t num = 1;
+    for (;;) {
+      const int64_t off = (int64_t)size * num;
+      if (size - 1 + off >= len) break;
+      bool ok = true;
+      for (int k = 0; k < size && ok; k++) ok = trim_char(s[k]) == trim_char(s[k + off]);
+      if (!ok) break;
+      num++;
+    }
+    if (num > 1) eL = max(eL, (uint32_t)(num * size));
+    num = 1;
+    for (;;) {
+      const int64_t off = (int64_t)size * num;
+      if (size - 1 + off >= len) break;
+      bool ok = true;
+      for (int k = 0; k < size && ok; k++) ok = trim_char(s[len - 1 - k]) == trim_char(s[len - 1 - k - off]);
+      if (!ok) break;
+      num++;
+    }
+    if (num > 1) eR = max(eR, (uint32_t)(num * size));
+  }
+  const uint32_t gL = left[R.gpos], gR = right[R.gend - 1];
+  xl[i] = (int32_t)(max(gL, eL) + (qs1 - 1));
+  xr[i] = (int32_t)(max(gR, eR) + (R.l_qseq - qe1));
 }
 
 }  // namespace bsq
@@ -83,7 +159,7 @@ extern "C" int bsq_sequence_trims(const char* seq, int64_t len, uint8_t* out, bs
     if (hipMemcpy(d_s, seq, (size_t)len, hipMemcpyHostToDevice) != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, "copy failed");
     const int64_t blocks = (len + 255) / 256;
     hipLaunchKernelGGL(k_repeat_extent, dim3((unsigned)blocks), dim3(256), 0, 0, d_s, len, d_m);
-    hipLaunchKernelGGL(k_trim_gather, dim3((unsigned)blocks), dim3(256), 0, 0, d_m, len, d_o);
+    hipLaunchKernelGGL(k_trim_gather, dim3((unsigned)blocks), dim3(256), 0, 0, d_m, len, d_o, d_o + len);
     if (hipGetLastError() != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, "kernel launch failed");
     if (hipMemcpy(out, d_o, 2 * (size_t)len, hipMemcpyDeviceToHost) != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, "copy failed");
     cleanup();

@@ -282,7 +282,8 @@ __device__ __forceinline__ void finish_column(const ImParams& P, int p, int t, i
 // gathered by order-preserving compaction into keys[], then five lanes add the FP64 likelihoods in
 // that order (identify_mutations.cpp:1561-1657, 1669-1689).  Every lane returns the sums in lps[]
 // and the number of reads.  sums[5] is LDS scratch of the wavefront.
-__device__ int exact_column_sums(const ImParams& P, int p, uint32_t* keys, double* sums, double lps[5], uint32_t& err) {
+__device__ __forceinline__ int exact_column_sums(const ImParams& P, int p, uint32_t* keys, double* sums, double lps[5],
+                                                 uint32_t& err) {
   const int lane = threadIdx.x & 63;
   const int S = P.read_set_max, Q = P.quality_max;
   const int64_t w = p / kWaveCols;
@@ -325,11 +326,21 @@ __device__ int exact_column_sums(const ImParams& P, int p, uint32_t* keys, doubl
   return n;
 }
 
+// The read list of a column's wavefront: in LDS, or (kKeysGlobal, deep samples) this block's slice
+// of the device scratch list.  A compile-time choice, so that the list's accesses keep their address
+// space.
+template <bool kKeysGlobal>
+__device__ __forceinline__ uint32_t* column_key_list(const ImParams& P, uint32_t* lds) {
+  if constexpr (kKeysGlobal) return P.key_scratch + (size_t)blockIdx.x * (size_t)P.key_cap;
+  else return lds;
+}
+
 // Exact pure-genotype decisions for the columns the column kernel could not certify: one wavefront
 // per column, the reference's call made from its exact sums.
-__global__ void __launch_bounds__(64) k_fix_columns(ImParams P) {
+template <bool kKeysGlobal>
+__global__ void __launch_bounds__(64) k_fix_columns_t(ImParams P) {
   extern __shared__ uint32_t fix_lds_keys[];   // P.key_cap, unless the list is in device scratch
-  uint32_t* fix_keys = P.keys_global ? P.key_scratch + (size_t)blockIdx.x * (size_t)P.key_cap : fix_lds_keys;
+  uint32_t* fix_keys = column_key_list<kKeysGlobal>(P, fix_lds_keys);
   __shared__ double fix_lps[5];
   const int lane = threadIdx.x;
   const int n_fix = min(*P.n_fix, P.fix_cap);
@@ -362,10 +373,11 @@ __global__ void __launch_bounds__(64) k_fix_columns(ImParams P) {
 // column, the exact FP64 sums beside the column pass's FP32 ones, and whether the final predicted
 // and consensus flags equal the calls the exact sums make.  out[6*i .. 6*i+4] = FP64 sums,
 // out[6*i+5] = reads; *mismatches counts columns whose flags differ.
-__global__ void __launch_bounds__(64) k_audit_columns(ImParams P, const int32_t* cols, int32_t n_cols, double* out,
-                                                      uint32_t* mismatches) {
+template <bool kKeysGlobal>
+__global__ void __launch_bounds__(64) k_audit_columns_t(ImParams P, const int32_t* cols, int32_t n_cols, double* out,
+                                                        uint32_t* mismatches) {
   extern __shared__ uint32_t audit_lds_keys[];
-  uint32_t* keys = P.keys_global ? P.key_scratch + (size_t)blockIdx.x * (size_t)P.key_cap : audit_lds_keys;
+  uint32_t* keys = column_key_list<kKeysGlobal>(P, audit_lds_keys);
   __shared__ double sums[5];
   const int lane = threadIdx.x;
   uint32_t err = 0;
@@ -694,6 +706,10 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
   if (err) atomicOr(P.err, err);
 }
 
+template __global__ void k_fix_columns_t<false>(ImParams P);
+template __global__ void k_fix_columns_t<true>(ImParams P);
+template __global__ void k_audit_columns_t<false>(ImParams P, const int32_t*, int32_t, double*, uint32_t*);
+template __global__ void k_audit_columns_t<true>(ImParams P, const int32_t*, int32_t, double*, uint32_t*);
 template __global__ void k_identify_columns_t<true>(ImParams P);
 template __global__ void k_identify_columns_t<false>(ImParams P);
 

@@ -168,7 +168,8 @@ constexpr size_t kFoldLdsBytes = 48 * 1024;  // largest likelihood table staged 
 template <bool kLds>
 __global__ void k_identify_columns_t(ImParams P);
 __global__ void k_prep_reads(const DRead* reads, int64_t n, int32_t S, int32_t Q, IdRec* out);
-__global__ void k_fix_columns(ImParams P);
+template <bool kKeysGlobal>
+__global__ void k_fix_columns_t(ImParams P);
 __global__ void k_insert_slots(ImParams P);
 // BAM2COV per-column counts over a region (coverage_output.cpp:307-503): lane = column, the window's
 // reads in file order.
@@ -199,9 +200,16 @@ constexpr int kScoreGrid = 2048;             // blocks of the exact scorer (grid
 constexpr int kFixGrid = 256;                // blocks of k_fix_columns (at most kScoreGrid)
 constexpr size_t kScoreStaticLds = 28 * 1024;   // bound on the scorer's static LDS
 constexpr size_t kScoreLdsBudget = 52 * 1024;   // static + dynamic LDS of a scorer block: three per CU
-__global__ void k_score_candidates(ImParams P);
+template <bool kKeysGlobal>
+__global__ void k_score_candidates_t(ImParams P);
+// reference and alignment trims (extra.hip)
+__global__ void k_repeat_extent(const char* s, int64_t len, uint8_t* m);
+__global__ void k_trim_gather(const uint8_t* m, int64_t len, uint8_t* left_out, uint8_t* right_out);
+__global__ void k_alignment_trims(const DRead* reads, int64_t n, const uint32_t* cigar, const uint8_t* bases,
+                                  const uint8_t* left, const uint8_t* right, int32_t* xl, int32_t* xr);
 __global__ void k_read_starts(EcParams P);
-__global__ void k_audit_columns(ImParams P, const int32_t* cols, int32_t n_cols, double* out, uint32_t* mismatches);
+template <bool kKeysGlobal>
+__global__ void k_audit_columns_t(ImParams P, const int32_t* cols, int32_t n_cols, double* out, uint32_t* mismatches);
 __global__ void k_runs(RunParams P);
 
 }  // namespace bsq

@@ -621,12 +621,23 @@ __device__ __forceinline__ int next_index(const double f[5], uint32_t n, int exc
 
 }  // namespace
 
-__global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3))) k_score_candidates(ImParams P) {
+// kKeysGlobal: the candidate's read list is this block's slice of a device scratch list (columns
+// too deep for LDS; the table-row cache then takes all of the dynamic LDS), else it is in LDS.  The
+// location is a compile-time choice, so that each instantiation's accesses to the list keep their
+// address space (LDS instructions in the usual case); one pointer that may be either would make
+// them all generic (flat) accesses.
+template <bool kKeysGlobal>
+__global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3))) k_score_candidates_t(ImParams P) {
   extern __shared__ __align__(16) unsigned char dyn[];
-  // the candidate's read list: in LDS, or for columns too deep for it in this block's slice of a
-  // device scratch list (the table-row cache then takes all of the dynamic LDS)
-  uint32_t* keys = P.keys_global ? P.key_scratch + (size_t)blockIdx.x * (size_t)P.key_cap : (uint32_t*)dyn;
-  const size_t key_lds = P.keys_global ? 0 : (((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15);
+  uint32_t* keys;
+  size_t key_lds;
+  if constexpr (kKeysGlobal) {
+    keys = P.key_scratch + (size_t)blockIdx.x * (size_t)P.key_cap;
+    key_lds = 0;
+  } else {
+    keys = (uint32_t*)dyn;
+    key_lds = ((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15;
+  }
   double* rows = (double*)(dyn + key_lds);
   __shared__ ScoreShared SS;
   const int tid = threadIdx.x;
@@ -645,9 +656,13 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
   const int prof_ph = two_stage ? 2 : 1;   // the phase of the profile likelihoods and first path levels
   uint32_t err = 0;
   for (int ci = blockIdx.x; ci < n_cand; ci += gridDim.x) {
-    const int2 cd = P.cand[ci];
     // every index the candidate leads to is checked before it is used: a column inside the
-    // sample, a window inside the index, a read range inside the alignments (block-uniform)
+    // sample, a window inside the index, a read range inside the alignments.  The values are read
+    // as wave-uniform scalars, so the checks are uniform branches: a skip decided on per-lane
+    // copies of the same value is a divergent branch to the compiler, and around the block's
+    // barriers that has paired barriers across wavefronts wrongly (results read stale LDS).
+    const int2 cd0 = P.cand[ci];
+    const int2 cd = make_int2(__builtin_amdgcn_readfirstlane(cd0.x), __builtin_amdgcn_readfirstlane(cd0.y));
     if (cd.x < 0 || cd.x >= P.n_pos || cd.y < 0) {
       err |= kErrIndex;
       continue;
@@ -658,7 +673,7 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
       err |= kErrIndex;
       continue;
     }
-    const int rb = P.win_rb[w], re = P.win_re[w];
+    const int rb = __builtin_amdgcn_readfirstlane(P.win_rb[w]), re = __builtin_amdgcn_readfirstlane(P.win_re[w]);
     if (rb < 0 || re < rb || re > P.n_reads) {
       err |= kErrIndex;
       continue;
@@ -728,8 +743,12 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
     bool cons = false;
     // ---- phases: each wavefront starts its tasks of the phase, then runs them to completion
     for (int ph = 0;; ph++) {
-      if (lane == 0) {
-        // task slots of this wavefront for this phase
+      {
+        // task slots of this wavefront for this phase, set up by every lane with the same values
+        // (identical LDS stores).  Kept out of a lane-0 branch on purpose: at the join of such a
+        // branch, ahead of the run_tasks call, the compiler has placed a register spill before the
+        // exec-mask restore, so the lanes outside the branch reloaded another value's slot; the
+        // build rejects that spill placement (build.py, spill_before_exec_restore).
         int k = 0;
         auto start = [&](int mode, uint32_t allowed, int v, double x, int res) {
           task_start(W, k++, mode, allowed, v, x, res, (const LdsU32*)SS.obs, (const LdsDouble*)SS.full_f, n,
@@ -1029,5 +1048,8 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
   }
   if (err) atomicOr(P.err, err);
 }
+
+template __global__ void k_score_candidates_t<false>(ImParams P);
+template __global__ void k_score_candidates_t<true>(ImParams P);
 
 }  // namespace bsq

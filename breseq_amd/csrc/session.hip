@@ -93,6 +93,8 @@ struct Session {
   } id_caps;
   std::vector<TargetInfo> h_targets;  // per-target parameters of the last identify (async copy source)
   std::vector<int32_t> h_gpos;        // global first column of each packed read (sorted)
+  std::vector<int64_t> h_src;         // input alignment index of each packed read
+  int64_t n_input_reads = 0;          // alignments the sample was loaded from (packed or not)
   std::vector<int32_t> h_gend_max;    // running maximum of the packed reads' gend, in read order
   // device buffers
   DBuf<DRead> d_reads;
@@ -178,8 +180,9 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   dr.reserve((size_t)a->n_reads);
   std::vector<uint8_t> h_rg;   // @RG index of each packed read (255: none)
   h_rg.reserve((size_t)a->n_reads);
-  std::vector<int64_t> raw_off;
+  std::vector<int64_t> raw_off, src;
   raw_off.reserve((size_t)a->n_reads);
+  src.reserve((size_t)a->n_reads);
   int64_t dev_bytes = 0;
   std::vector<uint32_t> cig;
   std::map<int32_t, uint8_t> mslots;
@@ -205,6 +208,7 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
     d.seq_off = dev_bytes;
     dev_bytes += (lq + 7) & ~(int64_t)7;
     raw_off.push_back(a->seq_off[r]);
+    src.push_back(r);
     const int64_t gp = (int64_t)S.tgt_off[tid] + a->pos[r];
     int64_t span = 0, qlen = 0;
     int n_ref_ops = 0, n_ins = 0;
@@ -289,6 +293,8 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   }
   S.n_read_files = (int32_t)max_rfile + 1;
   S.n_dreads = (int64_t)dr.size();
+  S.h_src = std::move(src);
+  S.n_input_reads = a->n_reads;
   S.h_gpos.resize(dr.size());
   S.h_gend_max.resize(dr.size());
   for (size_t i = 0; i < dr.size(); i++) {
@@ -807,7 +813,10 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[0], S.stream));
-  hipLaunchKernelGGL(k_fix_columns, dim3(kFixGrid), dim3(64), key_bytes, S.stream, P);
+  if (P.keys_global)
+    hipLaunchKernelGGL(k_fix_columns_t<true>, dim3(kFixGrid), dim3(64), 0, S.stream, P);
+  else
+    hipLaunchKernelGGL(k_fix_columns_t<false>, dim3(kFixGrid), dim3(64), key_bytes, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[3], S.stream));
   // MC/UN runs read only the final column flags and coverage, so they run on the side stream beside
@@ -836,7 +845,10 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   hipLaunchKernelGGL(k_insert_slots, dim3(1024), dim3(64), 0, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[1], S.stream));
-  hipLaunchKernelGGL(k_score_candidates, dim3(kScoreGrid), dim3(kScoreWaves * 64), score_lds, S.stream, P);
+  if (P.keys_global)
+    hipLaunchKernelGGL(k_score_candidates_t<true>, dim3(kScoreGrid), dim3(kScoreWaves * 64), score_lds, S.stream, P);
+  else
+    hipLaunchKernelGGL(k_score_candidates_t<false>, dim3(kScoreGrid), dim3(kScoreWaves * 64), score_lds, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.ev1, S.stream));
   int32_t counters[8];
@@ -1038,8 +1050,12 @@ static void run_precision_audit(Session& S, const bsq_identify_args* a, bsq_prec
   HIP_OK(hipMemset(S.d_counters.p + 3, 0, sizeof(int32_t)));
   ImParams P = S.last_P;
   const size_t key_bytes = P.keys_global ? 0 : (((size_t)P.key_cap * sizeof(uint32_t) + 15) & ~(size_t)15);
-  hipLaunchKernelGGL(k_audit_columns, dim3(kFixGrid), dim3(64), key_bytes, S.stream, P, d_cols.p, (int32_t)cols.size(),
-                     d_out.p, d_mis.p);
+  if (P.keys_global)
+    hipLaunchKernelGGL(k_audit_columns_t<true>, dim3(kFixGrid), dim3(64), 0, S.stream, P, d_cols.p,
+                       (int32_t)cols.size(), d_out.p, d_mis.p);
+  else
+    hipLaunchKernelGGL(k_audit_columns_t<false>, dim3(kFixGrid), dim3(64), key_bytes, S.stream, P, d_cols.p,
+                       (int32_t)cols.size(), d_out.p, d_mis.p);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(S.stream));
   int32_t errbits = 0;
@@ -1533,6 +1549,51 @@ int bsq_session_precision_audit(bsq_session* s, const bsq_identify_args* args, b
     return BSQ_OK;
   } catch (const BsqError& e) {
     s->s.audit_on = false;
+    set_status(status, e.code, e.what());
+    return e.code;
+  }
+}
+
+int bsq_session_alignment_trims(bsq_session* s, int32_t* xl, int32_t* xr, int64_t n, bsq_status* status) {
+  try {
+    if (!s || (n > 0 && (!xl || !xr))) throw BsqError(BSQ_ERR_ARG, "null argument");
+    Session& S = s->s;
+    if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
+    if (n != S.n_input_reads) throw BsqError(BSQ_ERR_ARG, "output arrays must hold one entry per loaded alignment");
+    // the SequenceTrims of every target at its global columns: left row, then right row
+    DBuf<uint8_t> d_m, d_trims;
+    DBuf<int32_t> d_xl, d_xr;
+    d_m.alloc((size_t)S.n_pos);
+    d_trims.alloc((size_t)2 * S.n_pos);
+    for (int t = 0; t < S.n_targets; t++) {
+      const int64_t len = S.lengths[t], off = S.tgt_off[t];
+      if (len == 0) continue;
+      const unsigned blocks = (unsigned)((len + 255) / 256);
+      hipLaunchKernelGGL(k_repeat_extent, dim3(blocks), dim3(256), 0, S.stream, S.d_ref.p + off, len, d_m.p + off);
+      hipLaunchKernelGGL(k_trim_gather, dim3(blocks), dim3(256), 0, S.stream, d_m.p + off, len, d_trims.p + off,
+                         d_trims.p + S.n_pos + off);
+    }
+    const int64_t nd = S.n_dreads;
+    d_xl.alloc((size_t)std::max<int64_t>(1, nd));
+    d_xr.alloc((size_t)std::max<int64_t>(1, nd));
+    if (nd > 0)
+      hipLaunchKernelGGL(k_alignment_trims, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, S.stream, S.d_reads.p, nd,
+                         S.d_cigar.p, S.d_bases.p, d_trims.p, d_trims.p + S.n_pos, d_xl.p, d_xr.p);
+    HIP_OK(hipGetLastError());
+    std::vector<int32_t> hl((size_t)nd), hr((size_t)nd);
+    if (nd > 0) {
+      HIP_OK(hipMemcpyAsync(hl.data(), d_xl.p, (size_t)nd * sizeof(int32_t), hipMemcpyDeviceToHost, S.stream));
+      HIP_OK(hipMemcpyAsync(hr.data(), d_xr.p, (size_t)nd * sizeof(int32_t), hipMemcpyDeviceToHost, S.stream));
+    }
+    HIP_OK(hipStreamSynchronize(S.stream));
+    for (int64_t i = 0; i < n; i++) xl[i] = xr[i] = -1;
+    for (int64_t i = 0; i < nd; i++) {
+      xl[S.h_src[(size_t)i]] = hl[(size_t)i];
+      xr[S.h_src[(size_t)i]] = hr[(size_t)i];
+    }
+    set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& e) {
     set_status(status, e.code, e.what());
     return e.code;
   }

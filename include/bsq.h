@@ -370,6 +370,13 @@ int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const b
                          double deletion_coverage_propagation_cutoff, double deletion_coverage_seed_cutoff,
                          const bsq_identify_args* im, bsq_ec_result** ec_out,
                          bsq_coverage_summary* per_target, bsq_im_result** im_out, bsq_status* status);
+/* get_alignment_trims (calculate_trims.cpp:239-285) of every alignment of the loaded sample, the
+ * XL / XR tags breseq writes (alignment.cpp:920-946): the larger of the reference's SequenceTrims
+ * at the alignment's ends and the read-edge trims of its aligned sequence (edge_trims_for_sequence,
+ * :183-230), plus the soft-clipped bases on each side.  xl, xr: n entries, n = the loaded
+ * alignments' n_reads, in their order; -1 for records the session did not pack (unmapped). */
+int bsq_session_alignment_trims(bsq_session* s, int32_t* xl, int32_t* xr, int64_t n, bsq_status* status);
+
 /* Diagnostic without a counterpart in breseq: how far the column pass's FP32 genotype sums are
  * from the FP64 sums breseq forms (identify_mutations.cpp:1669-1689, in read order), over every
  * column with reads of one identify pass (args as for bsq_session_identify), in log10 units; and

@@ -317,6 +317,7 @@ def lib():
     _decl(L, "bsq_test_ra_evidence", C.c_int, [C.c_char_p, P(Reference), P(RaTestSettings), P(C.c_void_p), c_i64p, st])
     _decl(L, "bsq_free_text", None, [C.c_void_p])
     _decl(L, "bsq_sequence_trims", C.c_int, [C.c_char_p, C.c_int64, c_u8p, st])
+    _decl(L, "bsq_sequence_trims_static", C.c_int, [C.c_char_p, C.c_int64, c_u8p, st])
     _decl(L, "bsq_selftest_division", C.c_int,
           [C.c_int32, C.c_uint64, C.c_uint64, P(C.c_uint64), P(C.c_uint64), st])
     _decl(L, "bsq_format_double", C.c_int32, [C.c_double, C.c_uint32, C.c_int32, C.c_char_p, C.c_int32])

@@ -8,12 +8,41 @@
 //   left[x]  = max over the same p of (m[p] - (x - p)).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "engine.h"
 #include "host_util.h"
 
 namespace bsq {
+
+// Device memory of bsq_sequence_trims_static, for callers that cannot allocate: the sequence, its
+// repeat extents and the two trim rows of a sequence of up to kStaticTrimsMax bases (4 bytes per
+// base, 32 MiB).  Every GPU the library runs this entry point on holds its own copy until the
+// process ends.  Host code never uses the array's name as an address: each device's copy is found
+// once with hipGetSymbolAddress (scratch_on_device) and only that pointer reaches kernels and copies.
+constexpr int64_t kStaticTrimsMax = (int64_t)8 << 20;
+static __device__ __align__(256) uint8_t scratch[4 * kStaticTrimsMax];
+
+static std::mutex g_scratch_mu;
+static std::vector<uint8_t*> g_scratch_ptr;   // per device id
+
+// Address of `scratch` on the current device, looked up on the first call on that device.
+static uint8_t* scratch_on_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) throw BsqError(BSQ_ERR_DEVICE, "no current HIP device");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_ptr[(size_t)dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw BsqError(BSQ_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[(size_t)dev] = (uint8_t*)p;
+  }
+  return g_scratch_ptr[(size_t)dev];
+}
 
 __global__ void __launch_bounds__(256) k_repeat_extent(const char* __restrict__ s, int64_t len, uint8_t* __restrict__ m) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -167,6 +196,39 @@ extern "C" int bsq_sequence_trims(const char* seq, int64_t len, uint8_t* out, bs
     return BSQ_OK;
   } catch (const BsqError& e) {
     cleanup();
+    set_status(status, e.code, e.what());
+    return e.code;
+  }
+}
+
+// The trims of bsq_sequence_trims with no device allocation: input, repeat extents and output live
+// in this device's `scratch` (sequences of at most kStaticTrimsMax bases).  Both kernels write
+// every byte they later read, so nothing needs clearing between calls.  Calls that overlap on one
+// GPU share the array and race.
+extern "C" int bsq_sequence_trims_static(const char* seq, int64_t len, uint8_t* out, bsq_status* status) {
+  try {
+    if ((!seq || !out) && len > 0) throw BsqError(BSQ_ERR_ARG, "bad arguments");
+    if (len < 0) throw BsqError(BSQ_ERR_ARG, "negative length");
+    if (len > kStaticTrimsMax)
+      throw BsqError(BSQ_ERR_LIMIT, "sequence of " + std::to_string(len) + " bases exceeds the " +
+                                        std::to_string(kStaticTrimsMax) + "-base capacity of the allocation-free trims");
+    if (len == 0) {
+      set_status(status, BSQ_OK, "");
+      return BSQ_OK;
+    }
+    uint8_t* base = scratch_on_device();
+    char* d_s = (char*)base;
+    uint8_t* d_m = base + kStaticTrimsMax;
+    uint8_t* d_o = base + 2 * kStaticTrimsMax;   // 2 * len bytes: left row, then right row
+    if (hipMemcpy(d_s, seq, (size_t)len, hipMemcpyHostToDevice) != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, "copy failed");
+    const int64_t blocks = (len + 255) / 256;
+    hipLaunchKernelGGL(k_repeat_extent, dim3((unsigned)blocks), dim3(256), 0, 0, d_s, len, d_m);
+    hipLaunchKernelGGL(k_trim_gather, dim3((unsigned)blocks), dim3(256), 0, 0, d_m, len, d_o, d_o + len);
+    if (hipGetLastError() != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, "kernel launch failed");
+    if (hipMemcpy(out, d_o, 2 * (size_t)len, hipMemcpyDeviceToHost) != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, "copy failed");
+    set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& e) {
     set_status(status, e.code, e.what());
     return e.code;
   }

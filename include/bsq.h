@@ -249,6 +249,12 @@ int32_t bsq_format_double(double v, uint32_t precision, int32_t scientific, char
 /* Per-position left/right trims of one reference sequence of length len:
  * out[0..len) = left trims, out[len..2len) = right trims (SequenceTrims layout). */
 int bsq_sequence_trims(const char* seq, int64_t len, uint8_t* out, bsq_status* status);
+/* No counterpart in breseq.  bsq_sequence_trims without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc):
+ * the sequence, its repeat extents and the trims are staged in a static device array of the
+ * library (32 MiB, held on each GPU it runs on until the process ends).  len <= 8388608 bases,
+ * else BSQ_ERR_LIMIT.  Calls that overlap on one GPU share that array and race. */
+int bsq_sequence_trims_static(const char* seq, int64_t len, uint8_t* out, bsq_status* status);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Coverage distribution fit (coverage_distribution.cpp:345-399)                              */

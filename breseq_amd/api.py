@@ -252,6 +252,18 @@ class Session:
         L.check(L.lib().bsq_session_precision_audit(self._h, C.byref(args), C.byref(rep), C.byref(st)), st)
         return {k: getattr(rep, k) for k, _ in L.PrecisionReport._fields_}
 
+    def alignment_trims(self):
+        """XL / XR of every alignment of the loaded sample, in its order (get_alignment_trims,
+        calculate_trims.cpp:239-285): two int32 arrays, -1 where the record was not packed
+        (unmapped, no target, no reference base covered)."""
+        n = int(self.sample.aln.contents.n_reads)
+        xl = np.zeros(n, dtype=np.int32)
+        xr = np.zeros(n, dtype=np.int32)
+        st = _status()
+        L.check(L.lib().bsq_session_alignment_trims(self._h, xl.ctypes.data_as(L.c_i32p), xr.ctypes.data_as(L.c_i32p),
+                                                    n, C.byref(st)), st)
+        return xl, xr
+
     def coverage_table(self, target=0, start=0, end=0, resolution=600, total_only=False, per_read_group=False,
                        csv=False, reference_average=None, n_read_groups=None):
         """BAM2COV table of one region (coverage_output::table, coverage_output.cpp:190-283) from the

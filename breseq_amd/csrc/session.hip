@@ -204,11 +204,6 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
     if (lq > 65535) throw BsqError(BSQ_ERR_LIMIT, "read longer than 65535 bases");
     DRead d;
     memset(&d, 0, sizeof(d));
-    // device layout: every read starts at an 8-byte aligned offset (k_ec_reads loads dwords)
-    d.seq_off = dev_bytes;
-    dev_bytes += (lq + 7) & ~(int64_t)7;
-    raw_off.push_back(a->seq_off[r]);
-    src.push_back(r);
     const int64_t gp = (int64_t)S.tgt_off[tid] + a->pos[r];
     int64_t span = 0, qlen = 0;
     int n_ref_ops = 0, n_ins = 0;
@@ -227,8 +222,13 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
       if (op == 1 || op == 6) n_ins++;
       if (!seen_ref && (op == 1 || op == 4)) y0 += len;
     }
-    if (span == 0) continue;
+    if (span == 0) continue;   // covers no reference base: not in any pileup, not packed
     if (a->pos[r] < 0 || a->pos[r] + span > ref->lengths[tid]) throw BsqError(BSQ_ERR_FORMAT, "alignment extends past its reference");
+    // device layout: every read starts at an 8-byte aligned offset (k_ec_reads loads dwords)
+    d.seq_off = dev_bytes;
+    dev_bytes += (lq + 7) & ~(int64_t)7;
+    raw_off.push_back(a->seq_off[r]);
+    src.push_back(r);
     if (qlen != lq) throw BsqError(BSQ_ERR_FORMAT, "CIGAR query length differs from SEQ length");
     if (gp < last_g) throw BsqError(BSQ_ERR_FORMAT, "alignments are not sorted by coordinate");
     last_g = gp;

@@ -45,6 +45,8 @@
  *                                            common.h:845-866                    test_host.py (printf, expected.gd)
  * SequenceTrims                              calculate_trims.cpp:67-138          test_trims_*: known answers and
  *                                                                                brute force on tiny strings
+ * edge trims, get_alignment_trims (XL/XR)    calculate_trims.cpp:183-285         test_oracle_pins.py
+ *                                                                                test_alignment_trims_by_hand
  * discordant-pair regions (DP enter step,    identify_mutations.cpp:883-908,     test_dp_regions.py: hand-computed
  *   check_discordant_completion, CSV)        1395-1457, 2358-2422, 2550-2573     windows
  * BAM2COV coverage table (table, rows,       coverage_output.cpp:190-283,        test_coverage_table.py: hand-
@@ -1464,6 +1466,45 @@ static void sequence_trims(const char* s, uint32_t len, uint8_t* t) {
   }
 }
 
+// basebam2char (common.h:263-275): the four bases by their BAM codes, anything else 'N'
+static char bam_char(uint8_t b) {
+  switch (b) {
+    case 1: return 'A';
+    case 2: return 'C';
+    case 4: return 'G';
+    case 8: return 'T';
+    default: return 'N';
+  }
+}
+
+// reverse_complement (fastq.h:271-277 with the table at fastq.cpp:1635-1651): A<->T, C<->G,
+// every other character 'N'
+static string reverse_complement(const string& s) {
+  string r(s.size(), 'N');
+  for (size_t i = 0; i < s.size(); i++) {
+    const char c = s[s.size() - 1 - i];
+    r[i] = c == 'A' ? 'T' : c == 'T' ? 'A' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'N';
+  }
+  return r;
+}
+
+// edge_trims_for_sequence (calculate_trims.cpp:183-230): the longest perfect tandem repeat of unit
+// 1..18 starting at the first base (left) and at the first base of the reverse complement (right),
+// at least 1 on each side
+static void edge_trims(const string& seq, uint32_t& L, uint32_t& R) {
+  L = R = 1;
+  const string rc = reverse_complement(seq);
+  for (int side = 0; side < 2; side++) {
+    const string& s = side ? rc : seq;
+    uint32_t& t = side ? R : L;
+    for (uint32_t size = 1; size <= 18; size++) {
+      uint32_t num = 1;
+      while (repeat_match(s.data(), (uint32_t)s.size(), 0, size, num)) num++;
+      if (num > 1) t = std::max(t, num * size);
+    }
+  }
+}
+
 }  // namespace orc
 
 /* ================================================================ C API */
@@ -1795,6 +1836,42 @@ int orc_sequence_trims(const char* seq, int64_t len, uint8_t* out) {
   if (len < 0 || len > 0xFFFFFFFFLL) return BSQ_ERR_ARG;
   orc::sequence_trims(seq, (uint32_t)len, out);
   return BSQ_OK;
+}
+
+// get_alignment_trims (calculate_trims.cpp:239-285), the XL/XR of every alignment: the larger of
+// the reference's SequenceTrims at reference_start_0 / reference_end_0 and the edge trims of
+// query_char_sequence (the bases from query_start_0 to query_end_0, alignment.h:190-196), plus
+// query_start_0 on the left and read_length - query_end_1 on the right (alignment.cpp:248-289).
+// -1 for unmapped records, records without a target or covering no reference base.
+int orc_alignment_trims(const bsq_reference* ref, const bsq_alignments* aln, int32_t* xl, int32_t* xr) {
+  try {
+    const orc::Aln A{aln};
+    vector<vector<uint8_t>> trims((size_t)ref->n_targets);
+    for (int32_t t = 0; t < ref->n_targets; t++) {
+      trims[(size_t)t].resize(2 * (size_t)ref->lengths[t]);
+      orc::sequence_trims(ref->seqs[t], (uint32_t)ref->lengths[t], trims[(size_t)t].data());
+    }
+    for (int64_t r = 0; r < A.n(); r++) {
+      xl[r] = xr[r] = -1;
+      const int32_t tid = aln->tid[r];
+      if ((aln->flag[r] & 0x4) || tid < 0 || A.n_cigar(r) == 0) continue;
+      const int64_t start0 = aln->pos[r], end0 = A.endpos(r) - 1;
+      if (end0 < start0) continue;
+      if (tid >= ref->n_targets || start0 < 0 || end0 >= ref->lengths[tid]) return BSQ_ERR_DATA;
+      const uint32_t qs1 = A.query_start_1(r), qe1 = A.query_end_1(r);
+      string q;
+      for (int64_t p = (int64_t)qs1 - 1; p <= (int64_t)qe1 - 1; p++) q += orc::bam_char(A.base(r, p));
+      uint32_t eL, eR;
+      orc::edge_trims(q, eL, eR);
+      const vector<uint8_t>& T = trims[(size_t)tid];
+      const uint32_t gL = T[(size_t)start0], gR = T[(size_t)ref->lengths[tid] + (size_t)end0];
+      xl[r] = (int32_t)(std::max(gL, eL) + (qs1 - 1));
+      xr[r] = (int32_t)(std::max(gR, eR) + (A.len(r) - qe1));
+    }
+    return BSQ_OK;
+  } catch (const std::exception&) {
+    return BSQ_ERR_DATA;
+  }
 }
 
 double orc_fit_alleles(const double* r, const double* lpmax, const int32_t* obs, int64_t n, const int32_t* allowed,

@@ -52,6 +52,8 @@ void orc_free_text(char* text);
 
 /* SequenceTrims (calculate_trims.cpp:67-138) */
 int orc_sequence_trims(const char* seq, int64_t len, uint8_t* out);
+/* get_alignment_trims (calculate_trims.cpp:183-285): XL / XR of every alignment, -1 when unmapped */
+int orc_alignment_trims(const bsq_reference* ref, const bsq_alignments* aln, int32_t* xl, int32_t* xr);
 
 /* fit_allele_frequencies on explicit per-read data (identify_mutations.cpp:3240-3318):
  * r[n*5] (max-normalized likelihoods), lpmax[n], obs[n] (0..4); allowed[5]; tol.

@@ -43,6 +43,7 @@ def lib():
     O.orc_im_position_scores.argtypes = [vp, P(L.c_f64p), L.c_i64p]
     O.orc_im_free.argtypes = [vp]
     O.orc_sequence_trims.argtypes = [C.c_char_p, C.c_int64, L.c_u8p]
+    O.orc_alignment_trims.argtypes = [P(L.Reference), P(L.Alignments), L.c_i32p, L.c_i32p]
     O.orc_coverage_table.argtypes = [P(L.Reference), P(L.Alignments), P(L.CoverageTableArgs), P(vp), L.c_i64p, st]
     O.orc_free_text.argtypes = [vp]
     O.orc_fisher_exact_2x2.restype = C.c_double
@@ -155,6 +156,17 @@ def ks_less(x, y):
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
     return lib().orc_ks_test_less(x.ctypes.data_as(L.c_f64p), x.size, y.ctypes.data_as(L.c_f64p), y.size)
+
+
+def alignment_trims(ref_ptr, aln_ptr):
+    """XL / XR of every alignment (get_alignment_trims, calculate_trims.cpp:183-285); -1 where the
+    record is unmapped, has no target or covers no reference base."""
+    n = int(aln_ptr.contents.n_reads)
+    xl = np.zeros(n, dtype=np.int32)
+    xr = np.zeros(n, dtype=np.int32)
+    rc = lib().orc_alignment_trims(ref_ptr, aln_ptr, xl.ctypes.data_as(L.c_i32p), xr.ctypes.data_as(L.c_i32p))
+    assert rc == 0, rc
+    return xl, xr
 
 
 def pileup_column(aln_ptr, tid, pos0, cap=4096):

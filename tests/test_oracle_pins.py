@@ -473,3 +473,61 @@ def test_preprocess_read_starts_by_hand():
     # without preprocess_stage nothing is reported
     _, _, _, texts3 = O.error_count(B.api.make_error_count_args(s))
     assert np.isnan(texts3["no_pos_hash_pr"][0])
+
+
+# ---------------------------------------------------------------- alignment trims (XL / XR)
+
+# A reference whose only tandem repeat is the AAAA run at 3..6.  Its SequenceTrims, worked out from
+# calculate_trims.cpp:67-138: the longest repeat starting at each position is 1 everywhere except
+# 4, 3, 2 at 3, 4, 5; right trims scatter o + 1 over o < m from each position and left trims
+# m - o over o <= m.
+TRIM_REF = "ACGAAAATCG"
+TRIM_LEFT = [1, 1, 1, 4, 3, 2, 1, 1, 1, 1]
+TRIM_RIGHT = [1, 1, 1, 1, 2, 3, 4, 1, 1, 1]
+
+# (pos, cigar, seq, flag) -> XL, XR by hand (calculate_trims.cpp:183-285): max(genomic trim at the
+# first / last reference base, edge repeat of the aligned bases; the right edge on the reverse
+# complement) + the soft-clipped bases on that side.
+TRIM_READS = [
+    # genomic 1 / 1; aligned "AAC": left repeat AA = 2, right (GTT) 1; plus the 2 clipped bases
+    (0, "2S3M", "AAAAC", 0, 4, 1),
+    # ends at 6 (right trim 4); reverse complement of GATT is AATC: AA = 2
+    (2, "2M1D2M", "GATT", 0, 1, 4),
+    # starts at 3 (left 4), ends at 6 (right 4); CAAT has no repeat at either end
+    (3, "4M", "CAAT", 16, 4, 4),
+    # hard clips are not part of the read: AAA gives 3 / 3; genomic 4 (at 3) / 3 (at 5)
+    (3, "2H3M1H", "AAA", 0, 4, 3),
+    (4, "3M", "CGT", 0, 3, 4),
+    # GCACAC: none on the left; its reverse complement GTGTGC starts with GT GT = 4
+    (4, "6M", "GCACAC", 16, 3, 4),
+    # covers no reference base: not trimmed (-1)
+    (5, "5S", "AAAAA", 0, -1, -1),
+    # unmapped: -1
+    (5, "3M", "AAA", 4, -1, -1),
+    (7, "2M", "TC", 0, 1, 1),
+    # aligned "TCG" ends at 9; two clipped bases on the right
+    (7, "3M2S", "TCGTT", 0, 1, 3),
+]
+
+
+def trim_sample():
+    reads = [dict(pos=p, cigar=c, seq=s, flag=f) for p, c, s, f, _, _ in TRIM_READS]
+    return BuiltSample(Built([("t0", TRIM_REF)], reads))
+
+
+def test_alignment_trims_by_hand():
+    left, right = O.sequence_trims(TRIM_REF)
+    assert list(left) == TRIM_LEFT and list(right) == TRIM_RIGHT
+    s = trim_sample()
+    xl, xr = O.alignment_trims(s.ref, s.aln)
+    assert list(xl) == [r[4] for r in TRIM_READS]
+    assert list(xr) == [r[5] for r in TRIM_READS]
+
+
+def test_simulator_trims_equal_oracle():
+    # the sample generator writes XL / XR from its own restatement of get_alignment_trims
+    s = api.Sample.simulate(seed=5, ref_length=20000, read_length=100, coverage=30.0, paired=True, n_snps=10,
+                            n_indels=2)
+    xl, xr = O.alignment_trims(s.ref, s.aln)
+    a = s.arrays()
+    assert np.array_equal(a["xl"], xl) and np.array_equal(a["xr"], xr)

@@ -252,6 +252,12 @@ class Session:
         L.check(L.lib().bsq_session_precision_audit(self._h, C.byref(args), C.byref(rep), C.byref(st)), st)
         return {k: getattr(rep, k) for k, _ in L.PrecisionReport._fields_}
 
+    def set_gd_writer(self, on_device=True):
+        """Print later genome diffs on the GPU (the default) or assemble them on the host."""
+        st = _status()
+        L.check(L.lib().bsq_session_set_gd_writer(self._h, 1 if on_device else 0, C.byref(st)), st)
+        return self
+
     def alignment_trims(self):
         """XL / XR of every alignment of the loaded sample, in its order (get_alignment_trims,
         calculate_trims.cpp:239-285): two int32 arrays, -1 where the record was not packed

@@ -22,6 +22,7 @@ SOURCES = [
     "extra.hip",
     "selftest.hip",
     "coverage.hip",
+    "gd_emit.hip",
     "sample.cpp",
     "host_util.cpp",
     "evidence.cpp",

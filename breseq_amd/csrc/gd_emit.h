@@ -1,0 +1,63 @@
+// gd_emit.h -- device writer of ra_mc_evidence.gd (gd_emit.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "engine.h"
+#include "kernels.h"
+
+namespace bsq {
+
+constexpr int kGdSlot = 1024;            // bytes of one entry's line slot
+constexpr int kGdMaxEntries = 1 << 20;   // entries one pass may write on the device
+constexpr int kGdFmtStride = 512;        // bytes per value of k_format_doubles
+
+// One genome-diff entry: numbering key (k0..k2: target in traversal order, emission column, kind)
+// and output key (o0..o2: evidence type, seq_id order, position, the type's fields, ID).
+struct GdEnt {
+  uint64_t k0, k1, k2;
+  uint64_t o0, o1, o2;
+  int32_t type;   // 0 RA, 1 MC, 2 UN, 3 MC of a whole target (failed coverage fit)
+  int32_t src;    // candidate index; n_cand + run index; n_cand + n_runs + failed-target index
+  int32_t t;      // target
+  uint32_t id;
+};
+
+struct GdParams {
+  const RaOut* ra;
+  int32_t n_cand;
+  const RunRec* runs;
+  int32_t n_runs;
+  const int32_t* whole;   // targets whose coverage fit failed (whole-target MC), in traversal order
+  int32_t n_whole;
+  const int32_t* pos_target;
+  int32_t n_pos;
+  const TargetInfo* targets;
+  int32_t n_targets;
+  const int32_t* rank;      // target -> its place in the traversal order
+  const int32_t* lexrank;   // target -> its place in seq_id byte order
+  const char* names;        // seq_ids, concatenated
+  const int32_t* name_off;  // n_targets + 1 offsets into names
+  int32_t places;           // polymorphism_precision_places
+  GdEnt* ents;
+  int32_t* n_ents;
+  int32_t ent_cap;
+  int32_t* n_type;          // entries written per type: RA, MC, UN
+  uint32_t* err;            // 1 an index out of range, 2 entry list full, 4 a line beyond its slot
+  char* slots;              // ent_cap * kGdSlot
+  int32_t* lens;            // ent_cap
+};
+
+__global__ void k_gd_collect(GdParams P);
+__global__ void k_gd_rank(const GdEnt* ents, const int32_t* n_ents, int cap, int which, int32_t* rank);
+__global__ void k_gd_number(GdEnt* ents, const int32_t* n_ents, int cap, const int32_t* rank);
+__global__ void k_gd_lines(GdParams P, const int32_t* out_rank);
+__global__ void k_gd_offsets(const int32_t* lens, const int32_t* n_ents, int cap, int64_t* offsets, int64_t* total);
+__global__ void k_gd_gather(const char* slots, const int32_t* lens, const int64_t* offsets, const int32_t* n_ents,
+                            int cap, char* text);
+__global__ void k_format_doubles(const double* v, int64_t n, int32_t precision, int32_t scientific, char* out,
+                                 int32_t* lens);
+
+}  // namespace bsq

@@ -1,0 +1,411 @@
+// gd_emit.hip -- ra_mc_evidence.gd written on the device from the identify pass's results: the RA
+// entries of the exact scorer, the MC / UN runs and the whole-target MC entries of failed coverage
+// fits (identify_mutations.cpp:1836-1910, 2262-2344, 2972-3007; cGenomeDiff::write and sort,
+// genome_diff.cpp:685-830, genome_diff_entry.cpp:566-700).
+//
+//   k_gd_collect   one thread per candidate / run / failed target: the entry, its numbering key
+//                  (target in traversal order, emission column, kind: the order add() numbers
+//                  entries in) and its output key (evidence type, seq_id, position, then the
+//                  type's own fields);
+//   k_gd_rank      the rank of every key among all keys (keys are unique), in LDS tiles;
+//   k_gd_number    the ID (numbering rank + 1) joins the output key;
+//   k_gd_lines     the text of each entry into a fixed slot at its output rank, with its length;
+//   k_gd_offsets   exclusive prefix sum of the line lengths (one block);
+//   k_gd_gather    the lines packed into one text, one wavefront per line.
+// Numbers are printed by the exact formatter of dec_format.h (printf's %.*e and %.*f), so the text
+// is byte-identical to the host writer's.
+#include <hip/hip_runtime.h>
+
+#include "dec_format.h"
+#include "gd_emit.h"
+
+namespace bsq {
+
+namespace {
+
+__device__ const char kIdxChar[6] = {'A', 'C', 'G', 'T', '.', 'N'};
+
+struct Sink {
+  char* p;
+  int n;
+  int cap;
+  __device__ void c(char ch) {
+    if (n < cap) p[n] = ch;
+    n++;
+  }
+  __device__ void s(const char* x) {
+    while (*x) c(*x++);
+  }
+  __device__ void mem(const char* x, int k) {
+    for (int i = 0; i < k; i++) c(x[i]);
+  }
+  __device__ void i64(int64_t v) {
+    char b[24];
+    int k = 0;
+    uint64_t u = v < 0 ? (uint64_t)0 - (uint64_t)v : (uint64_t)v;
+    do {
+      b[k++] = (char)('0' + u % 10u);
+      u /= 10u;
+    } while (u);
+    if (v < 0) c('-');
+    while (k) c(b[--k]);
+  }
+  // "\t<key>=<printf %.*e or %.*f of v>"
+  __device__ void field_d(const char* key, double v, int prec, bool sci) {
+    char b[dec::kMaxText];
+    const int k = sci ? dec::format_sci(v, prec, b) : dec::format_fixed(v, prec, b);
+    if (k <= 0) return;
+    c('\t');
+    s(key);
+    c('=');
+    mem(b, k);
+  }
+  __device__ void key(const char* k) {
+    c('\t');
+    s(k);
+    c('=');
+  }
+};
+
+__device__ __forceinline__ bool key_less(const GdEnt& a, const GdEnt& b, int which) {
+  const uint64_t a0 = which ? a.o0 : a.k0, a1 = which ? a.o1 : a.k1, a2 = which ? a.o2 : a.k2;
+  const uint64_t b0 = which ? b.o0 : b.k0, b1 = which ? b.o1 : b.k1, b2 = which ? b.o2 : b.k2;
+  if (a0 != b0) return a0 < b0;
+  if (a1 != b1) return a1 < b1;
+  return a2 < b2;
+}
+
+// allele_model::reported_frequency (identify_mutations.cpp:3042-3086): below half a read, 0
+__device__ __forceinline__ double reported(const RaOut& r, int b) {
+  if (r.n == 0 || b >= 5) return 0.0;
+  return (r.f[b] < 0.5 / (double)r.n) ? 0.0 : r.f[b];
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_gd_collect(GdParams P) {
+  const int64_t total = (int64_t)P.n_cand + P.n_runs + P.n_whole;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    GdEnt e;
+    e.o2 = 0;
+    e.id = 0;
+    e.src = (int32_t)i;
+    if (i < P.n_cand) {
+      const RaOut& r = P.ra[i];
+      if (!r.emitted) continue;
+      if (r.gpos < 0 || r.gpos >= P.n_pos) {
+        atomicOr(P.err, 1u);
+        continue;
+      }
+      const int t = P.pos_target[r.gpos];
+      if (t < 0 || t >= P.n_targets) {
+        atomicOr(P.err, 1u);
+        continue;
+      }
+      const uint64_t position = (uint64_t)(r.gpos - P.targets[t].off + 1);
+      e.type = 0;
+      e.t = t;
+      e.k0 = (uint64_t)P.rank[t];
+      e.k1 = position;
+      e.k2 = 2ull + (uint64_t)(uint32_t)r.slot;
+      e.o0 = (0ull << 56) | ((uint64_t)P.lexrank[t] << 32) | position;
+      e.o1 = ((uint64_t)(uint32_t)r.slot << 32) | ((uint64_t)(uint8_t)kIdxChar[min((int)r.ref_idx, 5)] << 8) |
+             (uint64_t)(uint8_t)kIdxChar[min((int)r.variant, 5)];
+      atomicAdd(P.n_type + 0, 1);
+    } else if (i < (int64_t)P.n_cand + P.n_runs) {
+      const RunRec& rr = P.runs[i - P.n_cand];
+      if (rr.kind == 0 && !rr.seed) continue;   // an MC run whose seed was never reached
+      if (rr.s < 0 || rr.s >= P.n_pos || rr.e < rr.s || rr.e >= P.n_pos) {
+        atomicOr(P.err, 1u);
+        continue;
+      }
+      const int t = P.pos_target[rr.s];
+      if (t < 0 || t >= P.n_targets) {
+        atomicOr(P.err, 1u);
+        continue;
+      }
+      const int64_t base = P.targets[t].off;
+      const uint64_t pos = (uint64_t)(rr.s - base + 1), end = (uint64_t)(rr.e - base + 1);
+      e.t = t;
+      e.k0 = (uint64_t)P.rank[t];
+      e.k1 = end + 1;
+      if (rr.kind == 0) {
+        const uint64_t red_start = rr.red_start >= 0 ? (uint64_t)(rr.red_start - base + 1) : pos;
+        const uint64_t red_end = rr.red_end >= 0 ? (uint64_t)(rr.red_end - base + 1) : end;
+        e.type = 1;
+        e.k2 = 0;
+        e.o0 = (1ull << 56) | ((uint64_t)P.lexrank[t] << 32) | pos;
+        e.o1 = (end << 32) | (uint32_t)(red_start - pos);
+        e.o2 = (uint64_t)(uint32_t)(end - red_end) << 32;
+        atomicAdd(P.n_type + 1, 1);
+      } else {
+        e.type = 2;
+        e.k2 = 1;
+        e.o0 = (2ull << 56) | ((uint64_t)P.lexrank[t] << 32) | pos;
+        e.o1 = end << 32;
+        atomicAdd(P.n_type + 2, 1);
+      }
+    } else {
+      const int t = P.whole[i - P.n_cand - P.n_runs];
+      if (t < 0 || t >= P.n_targets) {
+        atomicOr(P.err, 1u);
+        continue;
+      }
+      const uint64_t len = (uint64_t)P.targets[t].len;
+      e.type = 3;   // MC of a whole target whose coverage fit failed
+      e.t = t;
+      e.k0 = (uint64_t)P.rank[t];
+      e.k1 = len + 2;
+      e.k2 = 0;
+      e.o0 = (1ull << 56) | ((uint64_t)P.lexrank[t] << 32) | 1ull;
+      e.o1 = len << 32;
+      atomicAdd(P.n_type + 1, 1);
+    }
+    const int k = atomicAdd(P.n_ents, 1);
+    if (k < P.ent_cap) P.ents[k] = e;
+    else atomicOr(P.err, 2u);
+  }
+}
+
+// rank of every entry's key (numbering key, or output key) among all n keys, 256 keys per LDS tile
+__global__ void __launch_bounds__(256) k_gd_rank(const GdEnt* ents, const int32_t* n_ents, int cap, int which,
+                                                 int32_t* rank) {
+  __shared__ GdEnt tile[256];
+  // the count read as a wave-uniform scalar: the tile loop below holds barriers
+  const int n = __builtin_amdgcn_readfirstlane(min(*n_ents, cap));
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  GdEnt me;
+  if (i < n) me = ents[i];
+  int r = 0;
+  for (int j0 = 0; j0 < n; j0 += 256) {
+    __syncthreads();
+    if (j0 + (int)threadIdx.x < n) tile[threadIdx.x] = ents[j0 + threadIdx.x];
+    __syncthreads();
+    const int m = min(256, n - j0);
+    if (i < n)
+      for (int j = 0; j < m; j++) r += key_less(tile[j], me, which) ? 1 : 0;
+  }
+  if (i < n) rank[i] = r;
+}
+
+// the ID (numbering rank + 1) becomes the last output key, as cGenomeDiff::sort breaks ties by it
+__global__ void __launch_bounds__(256) k_gd_number(GdEnt* ents, const int32_t* n_ents, int cap, const int32_t* rank) {
+  const int n = min(*n_ents, cap);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t id = (uint32_t)rank[i] + 1u;
+  ents[i].id = id;
+  ents[i].o2 |= id;
+}
+
+// The line of each entry into slot out_rank (cGenomeDiff::write: type, ID, evidence '.', the
+// positional fields, then key=value fields in name order).
+__global__ void __launch_bounds__(64) k_gd_lines(GdParams P, const int32_t* out_rank) {
+  const int n = min(*P.n_ents, P.ent_cap);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const GdEnt e = P.ents[i];
+    const int slot = out_rank[i];
+    if (slot < 0 || slot >= n) {
+      atomicOr(P.err, 1u);
+      continue;
+    }
+    Sink o{P.slots + (size_t)slot * kGdSlot, 0, kGdSlot};
+    const int t = e.t;
+    const char* name = P.names + P.name_off[t];
+    const int name_len = P.name_off[t + 1] - P.name_off[t];
+    const int64_t base = P.targets[t].off;
+    o.s(e.type == 0 ? "RA" : (e.type == 2 ? "UN" : "MC"));
+    o.c('\t');
+    o.i64(e.id);
+    o.s("\t.\t");
+    o.mem(name, name_len);
+    o.c('\t');
+    if (e.type == 0) {
+      const RaOut r = P.ra[e.src];
+      const int places = P.places;
+      o.i64(r.gpos - base + 1);
+      o.c('\t');
+      o.i64(r.slot);
+      o.c('\t');
+      o.c(kIdxChar[min((int)r.ref_idx, 5)]);
+      o.c('\t');
+      o.c(kIdxChar[min((int)r.variant, 5)]);
+      // fields in name order (identify_mutations.cpp:1836-1910 fills them; the entry's map orders them)
+      bool any = false;
+      for (int b = 0; b < 5; b++) {
+        const double fr = reported(r, b);
+        if (fr <= 0.0) continue;
+        char buf[dec::kMaxText];
+        const int k = dec::format_sci(fr, places, buf);
+        if (k <= 0) continue;
+        if (!any) o.key("allele_frequencies");
+        else o.c(',');
+        any = true;
+        o.c(kIdxChar[b]);
+        o.c(':');
+        o.mem(buf, k);
+      }
+      o.field_d("fisher_strand_p_value", r.fisher_p, 5, true);
+      o.field_d("frequency", reported(r, r.variant), places, true);
+      o.field_d("frequency_lower", r.lower, places, true);
+      o.field_d("frequency_upper", r.upper, places, true);
+      o.field_d("ks_quality_p_value", r.ks_p, 5, true);
+      o.key("major_base");
+      o.c(kIdxChar[min((int)r.major, 5)]);
+      auto cov = [&](const char* k, int b) {
+        o.key(k);
+        if (b >= 5) {
+          o.s("0/0");
+          return;
+        }
+        o.i64((int32_t)r.cov[b][0]);
+        o.c('/');
+        o.i64((int32_t)r.cov[b][1]);
+      };
+      cov("major_cov", r.major);
+      o.field_d("major_frequency", reported(r, r.major), places, true);
+      o.key("minor_base");
+      o.c(kIdxChar[min((int)r.minor, 5)]);
+      cov("minor_cov", r.minor);
+      cov("new_cov", r.variant);
+      cov("ref_cov", r.ref_idx);
+      o.field_d("score", r.variant_score, 1, false);
+      o.key("total_cov");
+      o.i64(r.total_top);
+      o.c('/');
+      o.i64(r.total_bot);
+    } else if (e.type == 1 || e.type == 3) {
+      const uint64_t pos = e.o0 & 0xffffffffull, end = e.o1 >> 32;
+      o.i64((int64_t)pos);
+      o.c('\t');
+      o.i64((int64_t)end);
+      o.c('\t');
+      o.i64((int64_t)(e.o1 & 0xffffffffull));
+      o.c('\t');
+      o.i64((int64_t)(e.o2 >> 32));
+      // left_inside_cov, left_outside_cov, right_inside_cov, right_outside_cov: %.0f, NA when unset
+      int32_t li = 0, lo = -1, ri = 0, ro = -1;
+      if (e.type == 1) {
+        const RunRec& rr = P.runs[e.src - P.n_cand];
+        li = rr.li;
+        lo = rr.lo;
+        ri = rr.ri;
+        ro = rr.ro;
+      }
+      const double nan = __longlong_as_double(0x7ff8000000000000LL);
+      o.field_d("left_inside_cov", (double)li, 0, false);
+      o.field_d("left_outside_cov", lo < 0 ? nan : (double)lo, 0, false);
+      o.field_d("right_inside_cov", (double)ri, 0, false);
+      o.field_d("right_outside_cov", ro < 0 ? nan : (double)ro, 0, false);
+    } else {
+      o.i64((int64_t)(e.o0 & 0xffffffffull));
+      o.c('\t');
+      o.i64((int64_t)(e.o1 >> 32));
+    }
+    o.c('\n');
+    if (o.n > kGdSlot) atomicOr(P.err, 4u);
+    P.lens[slot] = min(o.n, kGdSlot);
+  }
+}
+
+// exclusive prefix sum of the n line lengths, offsets[n] = the text length (one block)
+__global__ void __launch_bounds__(1024) k_gd_offsets(const int32_t* lens, const int32_t* n_ents, int cap, int64_t* offsets,
+                                                     int64_t* total) {
+  __shared__ int64_t part[1024];
+  const int n = __builtin_amdgcn_readfirstlane(min(*n_ents, cap));
+  const int per = (n + 1023) / 1024;
+  const int b = threadIdx.x * per, e = min(n, b + per);
+  int64_t s = 0;
+  for (int i = b; i < e; i++) s += lens[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int64_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int64_t run = part[threadIdx.x] - s;
+  for (int i = b; i < e; i++) {
+    offsets[i] = run;
+    run += lens[i];
+  }
+  if (threadIdx.x == 1023) {
+    offsets[n] = part[1023];
+    *total = part[1023];
+  }
+}
+
+// line k into the text at offsets[k], one wavefront per line
+__global__ void __launch_bounds__(256) k_gd_gather(const char* slots, const int32_t* lens, const int64_t* offsets,
+                                                   const int32_t* n_ents, int cap, char* text) {
+  const int n = min(*n_ents, cap);
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, lane = threadIdx.x % 64;
+  const int waves = gridDim.x * blockDim.x / 64;
+  for (int k = wave; k < n; k += waves) {
+    const int len = lens[k];
+    const char* src = slots + (size_t)k * kGdSlot;
+    char* dst = text + offsets[k];
+    for (int i = lane; i < len; i += 64) dst[i] = src[i];
+  }
+}
+
+// Exact decimal text of v (the formatter the writer uses), one value per thread; out holds
+// kGdFmtStride bytes per value, lens the text lengths.
+__global__ void __launch_bounds__(64) k_format_doubles(const double* v, int64_t n, int32_t precision, int32_t scientific,
+                                                       char* out, int32_t* lens) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    char b[dec::kMaxText];
+    const int k = scientific ? dec::format_sci(v[i], precision, b) : dec::format_fixed(v[i], precision, b);
+    lens[i] = k;
+    for (int j = 0; j < k && j < kGdFmtStride; j++) out[i * kGdFmtStride + j] = b[j];
+  }
+}
+
+}  // namespace bsq
+
+#include <string>
+#include <vector>
+
+#include "host_util.h"
+
+// The device formatter over n values (test entry: the device's text is compared with printf's).
+extern "C" int bsq_format_doubles_device(const double* v, int64_t n, int32_t precision, int32_t scientific, char* out,
+                                         int32_t* lens, bsq_status* status) {
+  using namespace bsq;
+  double* d_v = nullptr;
+  char* d_out = nullptr;
+  int32_t* d_len = nullptr;
+  auto cleanup = [&]() {
+    if (d_v) (void)hipFree(d_v);
+    if (d_out) (void)hipFree(d_out);
+    if (d_len) (void)hipFree(d_len);
+  };
+  try {
+    if (n < 0 || (n > 0 && (!v || !out || !lens))) throw BsqError(BSQ_ERR_ARG, "bad arguments");
+    if (n == 0) {
+      set_status(status, BSQ_OK, "");
+      return BSQ_OK;
+    }
+    if (hipMalloc((void**)&d_v, (size_t)n * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&d_out, (size_t)n * kGdFmtStride) != hipSuccess ||
+        hipMalloc((void**)&d_len, (size_t)n * sizeof(int32_t)) != hipSuccess)
+      throw BsqError(BSQ_ERR_DEVICE, "device allocation failed");
+    if (hipMemcpy(d_v, v, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      throw BsqError(BSQ_ERR_DEVICE, "copy failed");
+    const int64_t blocks = std::min<int64_t>(4096, (n + 63) / 64);
+    hipLaunchKernelGGL(k_format_doubles, dim3((unsigned)blocks), dim3(64), 0, 0, d_v, n, precision, scientific, d_out, d_len);
+    if (hipGetLastError() != hipSuccess) throw BsqError(BSQ_ERR_DEVICE, "kernel launch failed");
+    if (hipMemcpy(out, d_out, (size_t)n * kGdFmtStride, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(lens, d_len, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+      throw BsqError(BSQ_ERR_DEVICE, "copy failed");
+    cleanup();
+    set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& e) {
+    cleanup();
+    set_status(status, e.code, e.what());
+    return e.code;
+  }
+}
+

@@ -24,6 +24,8 @@
 #include "evidence.h"
 #include "covfit.h"
 #include "host_util.h"
+#include "dec_format.h"
+#include "gd_emit.h"
 #include "kernels.h"
 #include "sample.h"
 
@@ -126,6 +128,12 @@ struct Session {
   DBuf<uint8_t> d_rg;                         // @RG index of each read (255: none)
   DBuf<int32_t> d_ins;
   DBuf<RaOut> d_ra;
+  // device writer of the genome diff (gd_emit.hip)
+  bool gd_on_device = true;
+  DBuf<GdEnt> d_gd_ents;
+  DBuf<int32_t> d_gd_rank_id, d_gd_rank_out, d_gd_lens, d_gd_meta, d_gd_cnt;
+  DBuf<int64_t> d_gd_off;
+  DBuf<char> d_gd_slots, d_gd_names, d_gd_text;
   DBuf<int2> d_events, d_runs;
   DBuf<int4> d_run_out;
 
@@ -673,6 +681,125 @@ static void build_likelihood_tables(const Session& S, const bsq_identify_args* a
 // overlaps the column pass.
 using LateCutoffs = std::shared_future<std::vector<double>>;
 
+// The genome diff of one identify pass written on the device (gd_emit.hip): the scorer's RA results,
+// the MC / UN runs and one MC per target in `whole` (failed coverage fit) become numbered, sorted
+// lines; only the text comes back.  rank: each target's place in the traversal order.
+static std::string emit_gd_on_device(Session& S, int32_t n_cand, int32_t n_runs, const std::vector<int32_t>& whole,
+                                     const std::vector<int64_t>& rank, uint32_t places, int64_t& n_ra, int64_t& n_mc,
+                                     int64_t& n_un) {
+  const int64_t total = (int64_t)n_cand + n_runs + (int64_t)whole.size();
+  if (total > kGdMaxEntries) throw BsqError(BSQ_ERR_LIMIT, "more genome diff entries than the device writer holds");
+  if (places > (uint32_t)dec::kMaxPrecision) throw BsqError(BSQ_ERR_ARG, "polymorphism_precision_places above 60");
+  const int cap = (int)std::max<int64_t>(1, total);
+  const int T = S.n_targets;
+  S.d_gd_ents.alloc((size_t)cap);
+  S.d_gd_rank_id.alloc((size_t)cap);
+  S.d_gd_rank_out.alloc((size_t)cap);
+  S.d_gd_lens.alloc((size_t)cap);
+  S.d_gd_off.alloc((size_t)cap + 1);
+  S.d_gd_slots.alloc((size_t)cap * kGdSlot);
+  // per-pass tables: traversal rank, seq_id byte-order rank, name offsets, failed targets
+  std::vector<int32_t> order_lex(T);
+  for (int t = 0; t < T; t++) order_lex[t] = t;
+  std::sort(order_lex.begin(), order_lex.end(), [&](int x, int y) { return S.names[x] < S.names[y]; });
+  std::vector<int32_t> meta((size_t)3 * T + 1 + whole.size() + 1);
+  int32_t* m_rank = meta.data();
+  int32_t* m_lex = m_rank + T;
+  int32_t* m_noff = m_lex + T;
+  int32_t* m_whole = m_noff + T + 1;
+  std::string names;
+  for (int t = 0; t < T; t++) {
+    m_rank[t] = (int32_t)std::max<int64_t>(0, rank[t]);
+    m_noff[t] = (int32_t)names.size();
+    names += S.names[t];
+  }
+  m_noff[T] = (int32_t)names.size();
+  for (int k = 0; k < T; k++) m_lex[order_lex[k]] = k;
+  for (size_t k = 0; k < whole.size(); k++) m_whole[k] = whole[k];
+  S.d_gd_meta.alloc(meta.size());
+  S.d_gd_names.alloc(std::max<size_t>(1, names.size()));
+  S.d_gd_cnt.alloc(8);
+  HIP_OK(hipMemcpyAsync(S.d_gd_meta.p, meta.data(), meta.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
+  if (!names.empty())
+    HIP_OK(hipMemcpyAsync(S.d_gd_names.p, names.data(), names.size(), hipMemcpyHostToDevice, S.stream));
+  HIP_OK(hipMemsetAsync(S.d_gd_cnt.p, 0, 8 * sizeof(int32_t), S.stream));
+  GdParams G;
+  G.ra = S.d_ra.p;
+  G.n_cand = n_cand;
+  G.runs = S.d_runrec.p;
+  G.n_runs = n_runs;
+  G.whole = S.d_gd_meta.p + (m_whole - meta.data());
+  G.n_whole = (int32_t)whole.size();
+  G.pos_target = S.d_pos_target.p;
+  G.n_pos = (int32_t)S.n_pos;
+  G.targets = S.d_targets.p;
+  G.n_targets = T;
+  G.rank = S.d_gd_meta.p;
+  G.lexrank = S.d_gd_meta.p + T;
+  G.names = S.d_gd_names.p;
+  G.name_off = S.d_gd_meta.p + 2 * T;
+  G.places = (int32_t)places;
+  G.ents = S.d_gd_ents.p;
+  G.n_ents = S.d_gd_cnt.p + 0;
+  G.ent_cap = cap;
+  G.n_type = S.d_gd_cnt.p + 1;
+  G.err = (uint32_t*)(S.d_gd_cnt.p + 4);
+  G.slots = S.d_gd_slots.p;
+  G.lens = S.d_gd_lens.p;
+  int64_t* d_total = (int64_t*)(S.d_gd_cnt.p + 6);
+  const unsigned rank_grid = (unsigned)((cap + 255) / 256);
+  hipLaunchKernelGGL(k_gd_collect, dim3((unsigned)std::min<int64_t>(1024, (total + 255) / 256 + 1)), dim3(256), 0, S.stream, G);
+  hipLaunchKernelGGL(k_gd_rank, dim3(rank_grid), dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, 0, S.d_gd_rank_id.p);
+  hipLaunchKernelGGL(k_gd_number, dim3(rank_grid), dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, S.d_gd_rank_id.p);
+  hipLaunchKernelGGL(k_gd_rank, dim3(rank_grid), dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, 1, S.d_gd_rank_out.p);
+  hipLaunchKernelGGL(k_gd_lines, dim3((unsigned)std::min(1024, (cap + 63) / 64)), dim3(64), 0, S.stream, G, S.d_gd_rank_out.p);
+  hipLaunchKernelGGL(k_gd_offsets, dim3(1), dim3(1024), 0, S.stream, S.d_gd_lens.p, G.n_ents, cap, S.d_gd_off.p, d_total);
+  HIP_OK(hipGetLastError());
+  int32_t cnt[8];
+  HIP_OK(hipMemcpyAsync(cnt, S.d_gd_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, S.stream));
+  HIP_OK(hipStreamSynchronize(S.stream));
+  const uint32_t gerr = (uint32_t)cnt[4];
+  if (gerr & 1u) throw BsqError(BSQ_ERR_DEVICE, "genome diff writer: an index outside its list");
+  if (gerr & 2u) throw BsqError(BSQ_ERR_DEVICE, "genome diff writer: entry list overflow");
+  if (gerr & 4u) throw BsqError(BSQ_ERR_LIMIT, "genome diff writer: an entry longer than its line slot");
+  int64_t text_len;
+  std::memcpy(&text_len, &cnt[6], sizeof(text_len));
+  const int n = cnt[0];
+  n_ra = cnt[1];
+  n_mc = cnt[2];
+  n_un = cnt[3];
+  std::string text = "#=GENOME_DIFF\t1.0\n";
+  const size_t head = text.size();
+  text.resize(head + (size_t)text_len);
+  if (n > 0 && text_len > 0) {
+    S.d_gd_text.alloc((size_t)text_len);
+    hipLaunchKernelGGL(k_gd_gather, dim3((unsigned)std::min(1024, (n + 3) / 4)), dim3(256), 0, S.stream, S.d_gd_slots.p,
+                       S.d_gd_lens.p, S.d_gd_off.p, G.n_ents, cap, S.d_gd_text.p);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(&text[head], S.d_gd_text.p, (size_t)text_len, hipMemcpyDeviceToHost, S.stream));
+    HIP_OK(hipStreamSynchronize(S.stream));
+  }
+  return text;
+}
+
+// The files of an identify pass: the genome diff text and the DP candidate regions.
+static void finish_identify_outputs(Session& S, const bsq_identify_args* a, bsq_im_result& out,
+                                    std::future<std::string>& dp_text, const double& dp_ms, double run_host_ms) {
+  if (a->gd_file) {
+    std::ofstream f(a->gd_file);
+    if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + a->gd_file);
+    f << out.text;
+  }
+  out.dp_text = dp_text.get();
+  S.timing.kernel_detail_ms[0] = dp_ms;
+  if (a->dp_file) {
+    std::ofstream f(a->dp_file);
+    if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + a->dp_file);
+    f << out.dp_text;
+  }
+  S.timing.kernel_detail_ms[5] = run_host_ms;
+}
+
 static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_result& out, int screen_off,
                               const LateCutoffs* late) {
   if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
@@ -871,7 +998,7 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   std::vector<GdEntry> run_entries;
   HIP_OK(hipStreamSynchronize(S.stream2));
   const double tr0 = now_ms();
-  if (n_runs_seen <= run_cap) {   // past the cap the call fails below, with the other limit checks
+  if (!S.gd_on_device && n_runs_seen <= run_cap) {   // past the cap the call fails below, with the other limit checks
     std::vector<RunRec> runs(n_runs_seen);
     if (n_runs_seen)
       HIP_OK(hipMemcpyAsync(runs.data(), S.d_runrec.p, n_runs_seen * sizeof(RunRec), hipMemcpyDeviceToHost, S.stream2));
@@ -946,6 +1073,24 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   raise_err_bits(errbits);
   const int n_cand = counters[0];
   S.timing.candidates = n_cand;
+  if (S.gd_on_device) {
+    const double t1 = now_ms();
+    std::vector<int32_t> whole;
+    if (!a->skip_missing_coverage_prediction)
+      for (int t : order)
+        if (prop_final[t] < 0.0) whole.push_back(t);
+    int64_t n_ra = 0, n_mc = 0, n_un = 0;
+    out.text = emit_gd_on_device(S, n_cand, n_runs_seen, whole, rank, a->polymorphism_precision_places, n_ra, n_mc,
+                                 n_un);
+    out.n_ra = n_ra;
+    out.n_mc = n_mc;
+    out.n_un = n_un;
+    S.timing.d2h_ms = 0.0;
+    finish_identify_outputs(S, a, out, dp_text, dp_ms, run_host_ms);
+    S.timing.host_post_ms = now_ms() - t1;
+    S.timing.host_pre_ms = th1 - th0;
+    return true;
+  }
   const double t0 = now_ms();
   std::vector<RaOut> ra(n_cand);
   if (n_cand) HIP_OK(hipMemcpyAsync(ra.data(), S.d_ra.p, n_cand * sizeof(RaOut), hipMemcpyDeviceToHost, S.stream));
@@ -988,19 +1133,7 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
     }
   }
   out.text = write_gd(entries);
-  if (a->gd_file) {
-    std::ofstream f(a->gd_file);
-    if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + a->gd_file);
-    f << out.text;
-  }
-  out.dp_text = dp_text.get();
-  S.timing.kernel_detail_ms[0] = dp_ms;
-  if (a->dp_file) {
-    std::ofstream f(a->dp_file);
-    if (!f) throw BsqError(BSQ_ERR_IO, std::string("could not write ") + a->dp_file);
-    f << out.dp_text;
-  }
-  S.timing.kernel_detail_ms[5] = run_host_ms;
+  finish_identify_outputs(S, a, out, dp_text, dp_ms, run_host_ms);
   S.timing.host_post_ms = now_ms() - t1;
   S.timing.host_pre_ms = th1 - th0;
   return true;
@@ -1597,6 +1730,16 @@ int bsq_session_alignment_trims(bsq_session* s, int32_t* xl, int32_t* xr, int64_
     set_status(status, e.code, e.what());
     return e.code;
   }
+}
+
+int bsq_session_set_gd_writer(bsq_session* s, int32_t on_device, bsq_status* status) {
+  if (!s) {
+    set_status(status, BSQ_ERR_ARG, "null session");
+    return BSQ_ERR_ARG;
+  }
+  s->s.gd_on_device = on_device != 0;
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
 }
 
 int bsq_session_timing(const bsq_session* s, bsq_timing* out) {

@@ -353,6 +353,16 @@ int bsq_session_error_count(bsq_session* s, const bsq_error_count_args* args,
 /* identify_mutations on the loaded sample. */
 int bsq_session_identify(bsq_session* s, const bsq_identify_args* args, bsq_im_result** out,
                          bsq_status* status);
+/* Which writer prints the genome diff of later identify passes (no counterpart in breseq: both
+ * write the same text, cGenomeDiff::write, genome_diff.cpp:685-830): on_device 1 (the default)
+ * numbers, sorts and prints the entries on the GPU and returns only the text; 0 assembles them on
+ * the host from the scorer's results. */
+int bsq_session_set_gd_writer(bsq_session* s, int32_t on_device, bsq_status* status);
+/* Diagnostic without a counterpart in breseq: the device writer's number formatter (printf's %.*e
+ * when scientific, else %.*f; NaN prints NA) over n values; out holds 512 bytes per value, lens the
+ * text lengths (-1 for a precision above 60). */
+int bsq_format_doubles_device(const double* v, int64_t n, int32_t precision, int32_t scientific, char* out,
+                              int32_t* lens, bsq_status* status);
 /* identify_mutations with the candidate screen disabled: every covered column goes through the
  * exact FP64 scorer.  Same output as bsq_session_identify; used to show the screen drops nothing. */
 int bsq_session_identify_unscreened(bsq_session* s, const bsq_identify_args* args,

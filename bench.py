@@ -115,6 +115,7 @@ def main():
     ap.add_argument("--ref-length", type=int, default=C2["ref_length"])
     ap.add_argument("--coverage", type=float, default=C2["coverage"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-audit", action="store_true", help="skip the FP32-vs-FP64 precision report")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -218,10 +219,13 @@ def main():
     }
     # per-position error of the FP32 column sums against the FP64 sums breseq forms, over every
     # column with reads (outside the timed steps)
-    log("[bench] precision audit")
-    out["fp32_vs_fp64"] = sess.precision_audit(ec.rates_table, ec.read_set_max, ec.quality_max,
-                                               [x["deletion_coverage_propagation_cutoff"] for x in summaries],
-                                               mode=B.CONSENSUS, dp=dp)
+    if args.no_audit:
+        out["fp32_vs_fp64"] = None
+    else:
+        log("[bench] precision audit")
+        out["fp32_vs_fp64"] = sess.precision_audit(ec.rates_table, ec.read_set_max, ec.quality_max,
+                                                   [x["deletion_coverage_propagation_cutoff"] for x in summaries],
+                                                   mode=B.CONSENSUS, dp=dp)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle, 1 core)")
         out["cpu_baseline"] = cpu_baseline(B, cfg, args.cpu_seconds)

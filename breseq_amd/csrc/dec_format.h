@@ -22,6 +22,7 @@ namespace dec {
 constexpr int kMaxPrecision = 60;    // digits after the point
 constexpr int kDigitCap = 420;       // integer digits (<= 309) or leading fraction zeros (<= 323) + precision + 2
 constexpr int kMaxText = 400;        // longest text written (fixed format of 1e308 with 60 decimals)
+constexpr int kWords = 36;           // 32-bit words of the fraction numerator: s <= 1074 bits plus a spare word
 
 // Exact decimal digits of |v| (finite): int_digits[0..n_int) (no leading zeros; none when the
 // integer part is 0), then the fraction as F / 2^s in words (little-endian, 32-bit).
@@ -29,31 +30,44 @@ struct Expansion {
   char digits[kDigitCap];
   int n_int;           // integer digits in digits[0..n_int)
   int n;               // digits produced so far (integer digits, then fraction digits)
-  uint32_t F[36];      // fraction numerator below 2^s (s <= 1074), one spare word above it
-  int nw;              // words of F in use
+  uint32_t F[kWords];  // fraction numerator below 2^s (s <= 1074), zero above
+  int nw;              // words of F that can be non-zero (0: an integer, no fraction)
   int s;               // fraction bits
 };
 
 BSQ_HD inline bool frac_nonzero(const Expansion& x) {
-  for (int i = 0; i < x.nw; i++)
-    if (x.F[i]) return true;
-  return false;
+  uint32_t any = 0;
+#pragma unroll
+  for (int i = 0; i < kWords; i++) any |= x.F[i];
+  return any != 0;
 }
 
 // Nine more fraction digits per step: F *= 10^9, and the part at or above bit s (below 10^9, so
-// within the words s/32 and s/32 + 1) is the next nine digits.
+// within the words s/32 and s/32 + 1) is the next nine digits.  Every loop over F has a constant
+// trip count, so F can live in registers.
 BSQ_HD inline void more_digits(Expansion& x, int want) {
+  const int wi = x.s >> 5, sh = x.s & 31;
+  const uint32_t keep_mask = sh ? ((1u << sh) - 1u) : 0u;
   while (x.n < want && x.n + 9 <= kDigitCap) {
     uint64_t carry = 0;
-    for (int i = 0; i < x.nw; i++) {
+#pragma unroll
+    for (int i = 0; i < kWords; i++) {
       const uint64_t t = (uint64_t)x.F[i] * 1000000000ull + carry;
       x.F[i] = (uint32_t)t;
       carry = t >> 32;
     }
-    const int wi = x.s >> 5, sh = x.s & 31;
-    uint32_t chunk = (uint32_t)((((uint64_t)x.F[wi + 1] << 32) | x.F[wi]) >> sh);
-    x.F[wi] &= sh ? ((1u << sh) - 1u) : 0u;
-    x.F[wi + 1] = 0;
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int i = 0; i < kWords; i++) {
+      if (i == wi) lo = x.F[i];
+      if (i == wi + 1) hi = x.F[i];
+    }
+    uint32_t chunk = (uint32_t)((((uint64_t)hi << 32) | lo) >> sh);
+#pragma unroll
+    for (int i = 0; i < kWords; i++) {
+      if (i == wi) x.F[i] &= keep_mask;
+      else if (i > wi) x.F[i] = 0;
+    }
     for (int k = 8; k >= 0; k--) {
       x.digits[x.n + k] = (char)('0' + chunk % 10u);
       chunk /= 10u;
@@ -77,6 +91,8 @@ BSQ_HD inline void expand(Expansion& x, uint64_t bits) {
   x.n_int = 0;
   x.nw = 0;
   x.s = 0;
+#pragma unroll
+  for (int i = 0; i < kWords; i++) x.F[i] = 0;
   // integer part
   uint32_t I[34];
   int ni = 0;
@@ -125,10 +141,11 @@ BSQ_HD inline void expand(Expansion& x, uint64_t bits) {
   if (x.s > 0) {
     const int s = x.s;
     x.nw = (s >> 5) + 2;   // room for the digits above bit s after a multiplication
-    for (int i = 0; i < x.nw; i++) x.F[i] = 0;
     const uint64_t f = (s >= 64) ? m : (m & ((1ull << s) - 1ull));
     x.F[0] = (uint32_t)f;
-    if (x.nw > 1) x.F[1] = (uint32_t)(f >> 32);
+    x.F[1] = (uint32_t)(f >> 32);
+  } else {
+    x.F[0] = x.F[1] = 0;
   }
 }
 
@@ -265,6 +282,12 @@ BSQ_HD inline int format_fixed(double v, int precision, char* out) {
     for (int i = int_len; i < nk; i++) out[n++] = keep[i];
   }
   return n;
+}
+
+// %.*e when scientific, else %.*f: one entry for callers that pick the form per value, so that
+// threads formatting different fields run the same code.
+BSQ_HD inline int format_number(double v, int precision, bool scientific, char* out) {
+  return scientific ? format_sci(v, precision, out) : format_fixed(v, precision, out);
 }
 
 }  // namespace dec

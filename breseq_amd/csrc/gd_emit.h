@@ -13,6 +13,9 @@ namespace bsq {
 constexpr int kGdSlot = 1024;            // bytes of one entry's line slot
 constexpr int kGdMaxEntries = 1 << 20;   // entries one pass may write on the device
 constexpr int kGdFmtStride = 512;        // bytes per value of k_format_doubles
+constexpr int kGdPieces = 16;            // pieces of one line, one per lane of a 16-lane group
+constexpr int kGdPiece = 480;            // bytes of one piece (a printed number with its key fits)
+constexpr int kGdRankTile = 64;          // keys per LDS tile of k_gd_rank
 
 // One genome-diff entry: numbering key (k0..k2: target in traversal order, emission column, kind)
 // and output key (o0..o2: evidence type, seq_id order, position, the type's fields, ID).
@@ -45,8 +48,11 @@ struct GdParams {
   int32_t* n_ents;
   int32_t ent_cap;
   int32_t* n_type;          // entries written per type: RA, MC, UN
-  uint32_t* err;            // 1 an index out of range, 2 entry list full, 4 a line beyond its slot
+  uint32_t* err;            // 1 an index out of range, 2 entry list full, 4 a line beyond its slot,
+                            // 8 a number the formatter refused
   char* slots;              // ent_cap * kGdSlot
+  int32_t* rank_id;         // ent_cap numbering ranks, zeroed by k_gd_collect
+  int32_t* rank_out;        // ent_cap output ranks, zeroed by k_gd_collect
   int32_t* lens;            // ent_cap
 };
 

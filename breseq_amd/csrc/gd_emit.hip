@@ -7,9 +7,10 @@
 //                  (target in traversal order, emission column, kind: the order add() numbers
 //                  entries in) and its output key (evidence type, seq_id, position, then the
 //                  type's own fields);
-//   k_gd_rank      the rank of every key among all keys (keys are unique), in LDS tiles;
+//   k_gd_rank      the rank of every key among all keys (keys are unique), tile by tile;
 //   k_gd_number    the ID (numbering rank + 1) joins the output key;
-//   k_gd_lines     the text of each entry into a fixed slot at its output rank, with its length;
+//   k_gd_lines     the text of each entry into a fixed slot at its output rank, with its length
+//                  (the numbers of a line printed by sixteen lanes side by side);
 //   k_gd_offsets   exclusive prefix sum of the line lengths (one block);
 //   k_gd_gather    the lines packed into one text, one wavefront per line.
 // Numbers are printed by the exact formatter of dec_format.h (printf's %.*e and %.*f), so the text
@@ -50,16 +51,6 @@ struct Sink {
     if (v < 0) c('-');
     while (k) c(b[--k]);
   }
-  // "\t<key>=<printf %.*e or %.*f of v>"
-  __device__ void field_d(const char* key, double v, int prec, bool sci) {
-    char b[dec::kMaxText];
-    const int k = sci ? dec::format_sci(v, prec, b) : dec::format_fixed(v, prec, b);
-    if (k <= 0) return;
-    c('\t');
-    s(key);
-    c('=');
-    mem(b, k);
-  }
   __device__ void key(const char* k) {
     c('\t');
     s(k);
@@ -85,6 +76,10 @@ __device__ __forceinline__ double reported(const RaOut& r, int b) {
 
 __global__ void __launch_bounds__(256) k_gd_collect(GdParams P) {
   const int64_t total = (int64_t)P.n_cand + P.n_runs + P.n_whole;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.ent_cap; i += (int64_t)gridDim.x * blockDim.x) {
+    P.rank_id[i] = 0;
+    P.rank_out[i] = 0;
+  }
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     GdEnt e;
     e.o2 = 0;
@@ -167,25 +162,25 @@ __global__ void __launch_bounds__(256) k_gd_collect(GdParams P) {
   }
 }
 
-// rank of every entry's key (numbering key, or output key) among all n keys, 256 keys per LDS tile
+// rank of every entry's key (numbering key, or output key) among all n keys: block (x, y) counts
+// the keys of tile y (kGdRankTile keys in LDS) below each of its 256 entries and adds the count to
+// the entry's rank (zeroed by k_gd_collect)
 __global__ void __launch_bounds__(256) k_gd_rank(const GdEnt* ents, const int32_t* n_ents, int cap, int which,
                                                  int32_t* rank) {
-  __shared__ GdEnt tile[256];
-  // the count read as a wave-uniform scalar: the tile loop below holds barriers
+  __shared__ GdEnt tile[kGdRankTile];
+  // the count read as a wave-uniform scalar: every thread reaches the barrier below
   const int n = __builtin_amdgcn_readfirstlane(min(*n_ents, cap));
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  GdEnt me;
-  if (i < n) me = ents[i];
+  const int j0 = blockIdx.y * kGdRankTile;
+  if (j0 >= n) return;
+  if (threadIdx.x < kGdRankTile && j0 + (int)threadIdx.x < n) tile[threadIdx.x] = ents[j0 + threadIdx.x];
+  __syncthreads();
+  if (i >= n) return;
+  const GdEnt me = ents[i];
+  const int m = min(kGdRankTile, n - j0);
   int r = 0;
-  for (int j0 = 0; j0 < n; j0 += 256) {
-    __syncthreads();
-    if (j0 + (int)threadIdx.x < n) tile[threadIdx.x] = ents[j0 + threadIdx.x];
-    __syncthreads();
-    const int m = min(256, n - j0);
-    if (i < n)
-      for (int j = 0; j < m; j++) r += key_less(tile[j], me, which) ? 1 : 0;
-  }
-  if (i < n) rank[i] = r;
+  for (int j = 0; j < m; j++) r += key_less(tile[j], me, which) ? 1 : 0;
+  if (r) atomicAdd(rank + i, r);
 }
 
 // the ID (numbering rank + 1) becomes the last output key, as cGenomeDiff::sort breaks ties by it
@@ -199,112 +194,203 @@ __global__ void __launch_bounds__(256) k_gd_number(GdEnt* ents, const int32_t* n
 }
 
 // The line of each entry into slot out_rank (cGenomeDiff::write: type, ID, evidence '.', the
-// positional fields, then key=value fields in name order).
+// positional fields, then key=value fields in name order).  A line is cut into kGdPieces pieces,
+// one per lane of a 16-lane group (four entries per wavefront): the positional header, each
+// allele frequency, each printed number with its key, the coverage pairs.  Every lane prints its
+// piece into LDS, the group's lengths give each piece its offset, and the pieces are copied into
+// the slot side by side, so the numbers of one line are formatted in parallel.
 __global__ void __launch_bounds__(64) k_gd_lines(GdParams P, const int32_t* out_rank) {
-  const int n = min(*P.n_ents, P.ent_cap);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const GdEnt e = P.ents[i];
-    const int slot = out_rank[i];
-    if (slot < 0 || slot >= n) {
-      atomicOr(P.err, 1u);
-      continue;
-    }
-    Sink o{P.slots + (size_t)slot * kGdSlot, 0, kGdSlot};
-    const int t = e.t;
-    const char* name = P.names + P.name_off[t];
-    const int name_len = P.name_off[t + 1] - P.name_off[t];
-    const int64_t base = P.targets[t].off;
-    o.s(e.type == 0 ? "RA" : (e.type == 2 ? "UN" : "MC"));
-    o.c('\t');
-    o.i64(e.id);
-    o.s("\t.\t");
-    o.mem(name, name_len);
-    o.c('\t');
-    if (e.type == 0) {
-      const RaOut r = P.ra[e.src];
-      const int places = P.places;
-      o.i64(r.gpos - base + 1);
-      o.c('\t');
-      o.i64(r.slot);
-      o.c('\t');
-      o.c(kIdxChar[min((int)r.ref_idx, 5)]);
-      o.c('\t');
-      o.c(kIdxChar[min((int)r.variant, 5)]);
-      // fields in name order (identify_mutations.cpp:1836-1910 fills them; the entry's map orders them)
-      bool any = false;
-      for (int b = 0; b < 5; b++) {
-        const double fr = reported(r, b);
-        if (fr <= 0.0) continue;
-        char buf[dec::kMaxText];
-        const int k = dec::format_sci(fr, places, buf);
-        if (k <= 0) continue;
-        if (!any) o.key("allele_frequencies");
-        else o.c(',');
-        any = true;
-        o.c(kIdxChar[b]);
-        o.c(':');
-        o.mem(buf, k);
-      }
-      o.field_d("fisher_strand_p_value", r.fisher_p, 5, true);
-      o.field_d("frequency", reported(r, r.variant), places, true);
-      o.field_d("frequency_lower", r.lower, places, true);
-      o.field_d("frequency_upper", r.upper, places, true);
-      o.field_d("ks_quality_p_value", r.ks_p, 5, true);
-      o.key("major_base");
-      o.c(kIdxChar[min((int)r.major, 5)]);
-      auto cov = [&](const char* k, int b) {
-        o.key(k);
-        if (b >= 5) {
-          o.s("0/0");
-          return;
+  __shared__ char piece[64][kGdPiece];
+  __shared__ int32_t plen[64];
+  const int lane = threadIdx.x, grp = lane / kGdPieces, k = lane % kGdPieces;
+  const int n = __builtin_amdgcn_readfirstlane(min(*P.n_ents, P.ent_cap));
+  const int step = gridDim.x * (64 / kGdPieces);
+  for (int i0 = blockIdx.x * (64 / kGdPieces); i0 < n; i0 += step) {
+    const int i = i0 + grp;
+    int slot = -1;
+    Sink o{piece[lane], 0, kGdPiece};
+    bool num = false, nsci = true;   // the piece ends with a number: nv printed with nprec digits
+    double nv = 0.0;
+    int nprec = 0;
+    if (i < n) {
+      const GdEnt e = P.ents[i];
+      slot = out_rank[i];
+      if (slot < 0 || slot >= n) {
+        atomicOr(P.err, 1u);
+        slot = -1;
+      } else if (k == 0) {
+        // type, ID, evidence, seq_id and the positional fields
+        const int t = e.t;
+        o.s(e.type == 0 ? "RA" : (e.type == 2 ? "UN" : "MC"));
+        o.c('\t');
+        o.i64(e.id);
+        o.s("\t.\t");
+        o.mem(P.names + P.name_off[t], P.name_off[t + 1] - P.name_off[t]);
+        o.c('\t');
+        if (e.type == 0) {
+          const RaOut& r = P.ra[e.src];
+          o.i64(r.gpos - P.targets[t].off + 1);
+          o.c('\t');
+          o.i64(r.slot);
+          o.c('\t');
+          o.c(kIdxChar[min((int)r.ref_idx, 5)]);
+          o.c('\t');
+          o.c(kIdxChar[min((int)r.variant, 5)]);
+        } else {
+          o.i64((int64_t)(e.o0 & 0xffffffffull));
+          o.c('\t');
+          o.i64((int64_t)(e.o1 >> 32));
+          if (e.type != 2) {
+            o.c('\t');
+            o.i64((int64_t)(e.o1 & 0xffffffffull));
+            o.c('\t');
+            o.i64((int64_t)(e.o2 >> 32));
+          }
         }
-        o.i64((int32_t)r.cov[b][0]);
-        o.c('/');
-        o.i64((int32_t)r.cov[b][1]);
-      };
-      cov("major_cov", r.major);
-      o.field_d("major_frequency", reported(r, r.major), places, true);
-      o.key("minor_base");
-      o.c(kIdxChar[min((int)r.minor, 5)]);
-      cov("minor_cov", r.minor);
-      cov("new_cov", r.variant);
-      cov("ref_cov", r.ref_idx);
-      o.field_d("score", r.variant_score, 1, false);
-      o.key("total_cov");
-      o.i64(r.total_top);
-      o.c('/');
-      o.i64(r.total_bot);
-    } else if (e.type == 1 || e.type == 3) {
-      const uint64_t pos = e.o0 & 0xffffffffull, end = e.o1 >> 32;
-      o.i64((int64_t)pos);
-      o.c('\t');
-      o.i64((int64_t)end);
-      o.c('\t');
-      o.i64((int64_t)(e.o1 & 0xffffffffull));
-      o.c('\t');
-      o.i64((int64_t)(e.o2 >> 32));
-      // left_inside_cov, left_outside_cov, right_inside_cov, right_outside_cov: %.0f, NA when unset
-      int32_t li = 0, lo = -1, ri = 0, ro = -1;
-      if (e.type == 1) {
-        const RunRec& rr = P.runs[e.src - P.n_cand];
-        li = rr.li;
-        lo = rr.lo;
-        ri = rr.ri;
-        ro = rr.ro;
+      } else if (e.type == 0 || e.type == 1 || e.type == 3) {
+        // the piece's text before its number (if any) is written here; the numbers of all lanes
+        // are then printed by one call below, so that the lanes run the formatter together
+        if (e.type == 0) {
+          // RA fields (identify_mutations.cpp:1836-1910), in the entry's name order
+          const RaOut& r = P.ra[e.src];
+          const int places = P.places;
+          auto cov = [&](const char* key, int b) {
+            o.key(key);
+            if (b >= 5) {
+              o.s("0/0");
+              return;
+            }
+            o.i64((int32_t)r.cov[b][0]);
+            o.c('/');
+            o.i64((int32_t)r.cov[b][1]);
+          };
+          if (k <= 5) {
+            const int b = k - 1;
+            const double fr = reported(r, b);
+            if (fr > 0.0) {
+              bool first = true;
+              for (int q = 0; q < b; q++) first = first && !(reported(r, q) > 0.0);
+              if (first) o.key("allele_frequencies");
+              else o.c(',');
+              o.c(kIdxChar[b]);
+              o.c(':');
+              num = true;
+              nv = fr;
+              nprec = places;
+            }
+          } else if (k == 11) {
+            o.key("major_base");
+            o.c(kIdxChar[min((int)r.major, 5)]);
+            cov("major_cov", r.major);
+          } else if (k == 13) {
+            o.key("minor_base");
+            o.c(kIdxChar[min((int)r.minor, 5)]);
+            cov("minor_cov", r.minor);
+            cov("new_cov", r.variant);
+            cov("ref_cov", r.ref_idx);
+          } else if (k == 15) {
+            o.key("total_cov");
+            o.i64(r.total_top);
+            o.c('/');
+            o.i64(r.total_bot);
+            o.c('\n');
+          } else {
+            num = true;
+            const char* key;
+            if (k == 6) {
+              key = "fisher_strand_p_value";
+              nv = r.fisher_p;
+              nprec = 5;
+            } else if (k == 7) {
+              key = "frequency";
+              nv = reported(r, r.variant);
+              nprec = places;
+            } else if (k == 8) {
+              key = "frequency_lower";
+              nv = r.lower;
+              nprec = places;
+            } else if (k == 9) {
+              key = "frequency_upper";
+              nv = r.upper;
+              nprec = places;
+            } else if (k == 10) {
+              key = "ks_quality_p_value";
+              nv = r.ks_p;
+              nprec = 5;
+            } else if (k == 12) {
+              key = "major_frequency";
+              nv = reported(r, r.major);
+              nprec = places;
+            } else {
+              key = "score";
+              nv = r.variant_score;
+              nprec = 1;
+              nsci = false;
+            }
+            o.key(key);
+          }
+        } else {
+          // MC: left_inside_cov, left_outside_cov, right_inside_cov, right_outside_cov (%.0f, NA
+          // when unset); a whole-target MC has 0 inside and nothing outside
+          int32_t li = 0, lo = -1, ri = 0, ro = -1;
+          if (e.type == 1) {
+            const RunRec& rr = P.runs[e.src - P.n_cand];
+            li = rr.li;
+            lo = rr.lo;
+            ri = rr.ri;
+            ro = rr.ro;
+          }
+          const double nan = __longlong_as_double(0x7ff8000000000000LL);
+          if (k >= 1 && k <= 4) {
+            num = true;
+            nprec = 0;
+            nsci = false;
+            if (k == 1) {
+              o.key("left_inside_cov");
+              nv = (double)li;
+            } else if (k == 2) {
+              o.key("left_outside_cov");
+              nv = lo < 0 ? nan : (double)lo;
+            } else if (k == 3) {
+              o.key("right_inside_cov");
+              nv = (double)ri;
+            } else {
+              o.key("right_outside_cov");
+              nv = ro < 0 ? nan : (double)ro;
+            }
+          } else if (k == 5) {
+            o.c('\n');
+          }
+        }
+      } else if (k == 1) {
+        o.c('\n');
       }
-      const double nan = __longlong_as_double(0x7ff8000000000000LL);
-      o.field_d("left_inside_cov", (double)li, 0, false);
-      o.field_d("left_outside_cov", lo < 0 ? nan : (double)lo, 0, false);
-      o.field_d("right_inside_cov", (double)ri, 0, false);
-      o.field_d("right_outside_cov", ro < 0 ? nan : (double)ro, 0, false);
-    } else {
-      o.i64((int64_t)(e.o0 & 0xffffffffull));
-      o.c('\t');
-      o.i64((int64_t)(e.o1 >> 32));
     }
-    o.c('\n');
-    if (o.n > kGdSlot) atomicOr(P.err, 4u);
-    P.lens[slot] = min(o.n, kGdSlot);
+    if (num) {
+      char buf[dec::kMaxText];
+      const int m = dec::format_number(nv, nprec, nsci, buf);
+      if (m > 0) o.mem(buf, m);
+      else atomicOr(P.err, 8u);
+    }
+    if (o.n > kGdPiece) atomicOr(P.err, 4u);
+    plen[lane] = min(o.n, kGdPiece);
+    __syncthreads();
+    int off = 0, total = 0;
+    for (int q = 0; q < kGdPieces; q++) {
+      const int l = plen[grp * kGdPieces + q];
+      if (q < k) off += l;
+      total += l;
+    }
+    if (slot >= 0) {
+      if (total > kGdSlot) {
+        atomicOr(P.err, 4u);
+      } else {
+        char* dst = P.slots + (size_t)slot * kGdSlot + off;
+        for (int q = 0; q < plen[lane]; q++) dst[q] = piece[lane][q];
+        if (k == 0) P.lens[slot] = total;
+      }
+    }
+    __syncthreads();
   }
 }
 

@@ -745,14 +745,16 @@ static std::string emit_gd_on_device(Session& S, int32_t n_cand, int32_t n_runs,
   G.n_type = S.d_gd_cnt.p + 1;
   G.err = (uint32_t*)(S.d_gd_cnt.p + 4);
   G.slots = S.d_gd_slots.p;
+  G.rank_id = S.d_gd_rank_id.p;
+  G.rank_out = S.d_gd_rank_out.p;
   G.lens = S.d_gd_lens.p;
   int64_t* d_total = (int64_t*)(S.d_gd_cnt.p + 6);
-  const unsigned rank_grid = (unsigned)((cap + 255) / 256);
+  const dim3 rank_grid((unsigned)((cap + 255) / 256), (unsigned)((cap + kGdRankTile - 1) / kGdRankTile));
   hipLaunchKernelGGL(k_gd_collect, dim3((unsigned)std::min<int64_t>(1024, (total + 255) / 256 + 1)), dim3(256), 0, S.stream, G);
-  hipLaunchKernelGGL(k_gd_rank, dim3(rank_grid), dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, 0, S.d_gd_rank_id.p);
-  hipLaunchKernelGGL(k_gd_number, dim3(rank_grid), dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, S.d_gd_rank_id.p);
-  hipLaunchKernelGGL(k_gd_rank, dim3(rank_grid), dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, 1, S.d_gd_rank_out.p);
-  hipLaunchKernelGGL(k_gd_lines, dim3((unsigned)std::min(1024, (cap + 63) / 64)), dim3(64), 0, S.stream, G, S.d_gd_rank_out.p);
+  hipLaunchKernelGGL(k_gd_rank, rank_grid, dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, 0, S.d_gd_rank_id.p);
+  hipLaunchKernelGGL(k_gd_number, dim3(rank_grid.x), dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, S.d_gd_rank_id.p);
+  hipLaunchKernelGGL(k_gd_rank, rank_grid, dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, 1, S.d_gd_rank_out.p);
+  hipLaunchKernelGGL(k_gd_lines, dim3((unsigned)std::min(2048, (cap + 3) / 4)), dim3(64), 0, S.stream, G, S.d_gd_rank_out.p);
   hipLaunchKernelGGL(k_gd_offsets, dim3(1), dim3(1024), 0, S.stream, S.d_gd_lens.p, G.n_ents, cap, S.d_gd_off.p, d_total);
   HIP_OK(hipGetLastError());
   int32_t cnt[8];
@@ -762,6 +764,7 @@ static std::string emit_gd_on_device(Session& S, int32_t n_cand, int32_t n_runs,
   if (gerr & 1u) throw BsqError(BSQ_ERR_DEVICE, "genome diff writer: an index outside its list");
   if (gerr & 2u) throw BsqError(BSQ_ERR_DEVICE, "genome diff writer: entry list overflow");
   if (gerr & 4u) throw BsqError(BSQ_ERR_LIMIT, "genome diff writer: an entry longer than its line slot");
+  if (gerr & 8u) throw BsqError(BSQ_ERR_ARG, "genome diff writer: a number precision it cannot print");
   int64_t text_len;
   std::memcpy(&text_len, &cnt[6], sizeof(text_len));
   const int n = cnt[0];

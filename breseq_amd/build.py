@@ -28,6 +28,7 @@ SOURCES = [
     "evidence.cpp",
     "covfit.cpp",
     "dp_regions.cpp",
+    "simreads.cpp",
 ]
 
 # -ffp-contract=off: the FP64 scorer must round every multiply and add separately, as the

@@ -498,6 +498,27 @@ typedef struct bsq_sim_config {
 
 int bsq_simulate(const bsq_sim_config* cfg, bsq_sample** out, bsq_status* status);
 
+/* SIMULATE-READS (do_simulate_reads, breseq_cmdline.cpp:1150-1330; cSimFastqSequence::simulate_read,
+ * simulate_single_ends, simulate_paired_ends, fastq.cpp:1254-1583; cPortableRandom,
+ * portable_random.cpp:22-88): byte-identical FASTQ for a given seed.  The option defaults are those
+ * of the command (bsq_simreads_defaults).  The 'tiled' mode is not provided. */
+typedef struct bsq_simreads_args {
+  int32_t mode;                 /* 0 single-end, 1 paired-end                                   */
+  int32_t circular;             /* the sequence is circular: fragments may wrap                  */
+  uint32_t n_reads;             /* reads (single-end) or pairs; 0: from coverage                 */
+  uint32_t coverage;            /* --coverage, used when n_reads is 0                            */
+  uint32_t read_length;         /* --length                                                      */
+  uint32_t fragment_mean;       /* --mean                                                        */
+  uint32_t fragment_stdev;      /* --stdev                                                       */
+  uint32_t seed;                /* --seed                                                        */
+  uint32_t quality_start, quality_end, quality_stdev, quality_min, quality_max;
+  uint32_t indel_error_rate_per_million;
+} bsq_simreads_args;
+int bsq_simreads_defaults(bsq_simreads_args* args);
+/* FASTQ text of the reads (fastq_2: the second mates, paired-end only); release with bsq_free_text. */
+int bsq_simulate_reads(const char* seq, int64_t len, const bsq_simreads_args* args, char** fastq_1,
+                       int64_t* len_1, char** fastq_2, int64_t* len_2, bsq_status* status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -222,6 +222,19 @@ class CoverageTableArgs(C.Structure):
     ]
 
 
+class SimReadsArgs(C.Structure):
+    """bsq_simreads_args: the options of SIMULATE-READS."""
+    _fields_ = [("mode", C.c_int32), ("circular", C.c_int32), ("n_reads", C.c_uint32), ("coverage", C.c_uint32),
+                ("read_length", C.c_uint32), ("fragment_mean", C.c_uint32), ("fragment_stdev", C.c_uint32),
+                ("seed", C.c_uint32), ("quality_start", C.c_uint32), ("quality_end", C.c_uint32),
+                ("quality_stdev", C.c_uint32), ("quality_min", C.c_uint32), ("quality_max", C.c_uint32),
+                ("indel_error_rate_per_million", C.c_uint32)]
+
+
+class SimMutation(C.Structure):
+    _fields_ = [("pos", C.c_int64), ("ref_len", C.c_int32), ("alt", C.c_char_p)]
+
+
 class SimConfig(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64),
@@ -281,6 +294,10 @@ def lib():
     vp = C.c_void_p
     st = P(Status)
     _decl(L, "bsq_simulate", C.c_int, [P(SimConfig), P(vp), st])
+    _decl(L, "bsq_simreads_defaults", C.c_int, [P(SimReadsArgs)])
+    _decl(L, "bsq_simulate_reads", C.c_int, [C.c_char_p, C.c_int64, P(SimReadsArgs), P(vp), c_i64p, P(vp), c_i64p, st])
+    _decl(L, "bsq_simulate_reads_sample", C.c_int, [C.c_char_p, C.c_char_p, C.c_int64, P(SimMutation), C.c_int32,
+                                                     P(SimReadsArgs), C.c_int32, P(vp), st])
     _decl(L, "bsq_sample_read", C.c_int, [C.c_char_p, C.c_char_p, P(vp), st])
     _decl(L, "bsq_sample_write_bam", C.c_int, [vp, C.c_char_p, st])
     _decl(L, "bsq_sample_write_fasta", C.c_int, [vp, C.c_char_p, st])

@@ -56,6 +56,26 @@ class Sample:
         return cls(h, groups)
 
     @classmethod
+    def simulate_breseq(cls, sequence, name="ref", mutations=(), mapq=42, **options):
+        """Reads as breseq's SIMULATE-READS makes them (options as bsq_simreads_args fields, the
+        command's defaults otherwise) from `sequence` with the mutations (pos, ref_len, alt)
+        applied, aligned to `sequence` by construction."""
+        a = simreads_args(**options)
+        muts = (L.SimMutation * max(1, len(mutations)))()
+        keep = []
+        for k, (pos, ref_len, alt) in enumerate(mutations):
+            b = alt.encode()
+            keep.append(b)
+            muts[k] = L.SimMutation(int(pos), int(ref_len), b)
+        seq = sequence.encode()
+        h = C.c_void_p()
+        st = _status()
+        L.check(L.lib().bsq_simulate_reads_sample(name.encode(), seq, len(seq), muts, len(mutations), C.byref(a),
+                                                  int(mapq), C.byref(h), C.byref(st)), st)
+        groups = [["reads_1", "reads_2"]] if a.mode == 1 else [["reads_1"]]
+        return cls(h, groups)
+
+    @classmethod
     def read(cls, bam_path, fasta_path, read_file_groups=None):
         h = C.c_void_p()
         st = _status()
@@ -285,6 +305,37 @@ class Session:
             return C.string_at(out, n.value).decode()
         finally:
             L.lib().bsq_free_text(out)
+
+
+def simreads_args(**options):
+    """bsq_simreads_args with the defaults of SIMULATE-READS; mode may be given as 'single' or
+    'paired-end'."""
+    a = L.SimReadsArgs()
+    L.lib().bsq_simreads_defaults(C.byref(a))
+    for k, v in options.items():
+        if k == "mode" and isinstance(v, str):
+            v = {"single": 0, "paired-end": 1, "paired": 1}[v]
+        setattr(a, k, int(v))
+    return a
+
+
+def simulate_reads_fastq(sequence, **options):
+    """FASTQ text of SIMULATE-READS (a pair of texts in paired-end mode)."""
+    a = simreads_args(**options)
+    p1, p2 = C.c_void_p(), C.c_void_p()
+    n1, n2 = C.c_int64(), C.c_int64()
+    st = _status()
+    seq = sequence.encode()
+    L.check(L.lib().bsq_simulate_reads(seq, len(seq), C.byref(a), C.byref(p1), C.byref(n1), C.byref(p2), C.byref(n2),
+                                       C.byref(st)), st)
+    try:
+        t1 = C.string_at(p1, n1.value).decode()
+        t2 = C.string_at(p2, n2.value).decode() if a.mode == 1 else None
+    finally:
+        L.lib().bsq_free_text(p1)
+        if a.mode == 1:
+            L.lib().bsq_free_text(p2)
+    return (t1, t2) if a.mode == 1 else t1
 
 
 def coverage_table_args(sample, target=0, start=0, end=0, resolution=600, total_only=False, per_read_group=False,

@@ -336,3 +336,178 @@ extern "C" int bsq_simulate_reads(const char* seq, int64_t len, const bsq_simrea
     return e.code;
   }
 }
+
+#include "sample.h"
+
+namespace {
+
+// The genome the reads come from: `ref` with the mutations applied (sorted, non-overlapping), and
+// for each of its bases the reference base it stands for (-1 for an inserted base).  The first
+// min(ref_len, alt length) bases of a mutation replace reference bases one for one.
+void apply_mutations(const std::string& ref, const bsq_sim_mutation* m, int32_t n, std::string& genome,
+                     std::vector<int64_t>& orig) {
+  genome.clear();
+  orig.clear();
+  int64_t at = 0;
+  for (int32_t k = 0; k < n; k++) {
+    const int64_t p = m[k].pos;
+    const int64_t rl = m[k].ref_len;
+    const std::string alt = m[k].alt ? m[k].alt : "";
+    if (p < at || rl < 0 || p + rl > (int64_t)ref.size())
+      throw BsqError(BSQ_ERR_ARG, "mutations must be sorted, non-overlapping and inside the sequence");
+    for (; at < p; at++) {
+      genome += ref[(size_t)at];
+      orig.push_back(at);
+    }
+    for (size_t j = 0; j < alt.size(); j++) {
+      genome += (char)std::toupper((unsigned char)alt[j]);
+      orig.push_back((int64_t)j < rl ? p + (int64_t)j : -1);
+    }
+    at = p + rl;
+  }
+  for (; at < (int64_t)ref.size(); at++) {
+    genome += ref[(size_t)at];
+    orig.push_back(at);
+  }
+}
+
+void push_cigar(std::vector<uint32_t>& cig, uint32_t op, uint32_t n) {
+  if (n == 0) return;
+  if (!cig.empty() && (cig.back() & 15u) == op) cig.back() += n << 4;
+  else cig.push_back((n << 4) | op);
+}
+
+}  // namespace
+
+extern "C" int bsq_simulate_reads_sample(const char* name, const char* seq, int64_t len, const bsq_sim_mutation* muts,
+                                         int32_t n_muts, const bsq_simreads_args* args, int32_t mapq,
+                                         bsq_sample** out, bsq_status* status) {
+  try {
+    if (!name || !seq || len <= 0 || !args || !out || n_muts < 0 || (n_muts > 0 && !muts))
+      throw BsqError(BSQ_ERR_ARG, "bad arguments");
+    if (args->circular) throw BsqError(BSQ_ERR_ARG, "aligned samples are built from linear sequences only");
+    if (mapq < 0 || mapq > 255) throw BsqError(BSQ_ERR_ARG, "mapq outside 0-255");
+    std::string ref(seq, (size_t)len);
+    for (char& c : ref) c = (char)std::toupper((unsigned char)c);
+    std::string genome;
+    std::vector<int64_t> orig;
+    apply_mutations(ref, muts, n_muts, genome, orig);
+    std::vector<bsq::SimReadOut> reads;
+    bsq::simulate_reads(genome, *args, &reads, nullptr, nullptr);
+    const bool paired = args->mode == 1;
+    // each read's alignment to the original sequence, walked through the genome's base map:
+    // inserted genome bases become I, reference bases the genome skips become D
+    struct Rec {
+      int64_t pos = -1, end = -1;   // 0-based first and last aligned reference base
+      std::vector<uint32_t> cig;
+    };
+    std::vector<Rec> rec(reads.size());
+    bsq::parallel_ranges((int64_t)reads.size(), bsq::host_threads(), [&](int, int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; i++) {
+        const bsq::SimReadOut& r = reads[(size_t)i];
+        Rec& R = rec[(size_t)i];
+        int64_t g = r.left0, prev = -1;
+        for (const auto& op : r.ops) {
+          if (op.first == 'I') {
+            push_cigar(R.cig, 1, op.second);
+            continue;
+          }
+          for (uint32_t k = 0; k < op.second; k++, g++) {
+            const int64_t o = orig[(size_t)g];
+            if (op.first == 'D') continue;   // a simulated deletion: the reference gap shows at the next base
+            if (o < 0) {
+              push_cigar(R.cig, 1, 1);
+              continue;
+            }
+            if (prev >= 0 && o > prev + 1) push_cigar(R.cig, 2, (uint32_t)(o - prev - 1));
+            if (R.pos < 0) R.pos = o;
+            push_cigar(R.cig, 0, 1);
+            prev = o;
+          }
+        }
+        R.end = prev;
+      }
+    });
+    bsq_sample* smp = new bsq_sample();
+    bsq::Sample& S = smp->s;
+    S.names.push_back(name);
+    S.seqs.push_back(ref);
+    // file order: by first aligned base, then generation order; reads without an aligned base dropped
+    std::vector<size_t> order;
+    for (size_t i = 0; i < reads.size(); i++)
+      if (rec[i].pos >= 0) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return rec[a].pos < rec[b].pos; });
+    std::vector<uint8_t> gtrims(2 * (size_t)len);
+    bsq::host_sequence_trims(ref.data(), len, gtrims.data());
+    // mate of each read of a pair: the other read generated from the same fragment
+    std::vector<int64_t> mate(reads.size(), -1);
+    if (paired)
+      for (size_t i = 0; i + 1 < reads.size(); i += 2) {
+        mate[i] = (int64_t)i + 1;
+        mate[i + 1] = (int64_t)i;
+      }
+    const size_t nr = order.size();
+    S.tid.assign(nr, 0);
+    S.rg.assign(nr, 0);
+    S.x1.assign(nr, 1);
+    S.cigar_off.assign(nr + 1, 0);
+    S.seq_off.assign(nr + 1, 0);
+    static const uint8_t kCode[5] = {1, 2, 4, 8, 15};
+    for (size_t k = 0; k < nr; k++) {
+      const size_t i = order[k];
+      const bsq::SimReadOut& r = reads[i];
+      const Rec& R = rec[i];
+      S.pos.push_back((int32_t)R.pos);
+      uint16_t flag = r.strand == -1 ? 0x10 : 0;
+      int32_t mpos = -1, tl = 0;
+      int8_t xp = -1;
+      if (paired) {
+        const int64_t j = mate[i];
+        flag |= 0x1 | (r.mate == 1 ? 0x40 : 0x80);
+        if (j >= 0 && rec[(size_t)j].pos >= 0) {
+          flag |= 0x2;
+          if (reads[(size_t)j].strand == -1) flag |= 0x20;
+          mpos = (int32_t)rec[(size_t)j].pos;
+          const int64_t lo = std::min(R.pos, rec[(size_t)j].pos), hi = std::max(R.end, rec[(size_t)j].end);
+          tl = (int32_t)(hi - lo + 1) * (R.pos <= rec[(size_t)j].pos ? 1 : -1);
+          xp = 0;   // FR
+        } else {
+          flag |= 0x8;
+        }
+      }
+      S.flag.push_back(flag);
+      S.mapq.push_back((uint8_t)mapq);
+      S.mate_pos.push_back(mpos);
+      S.tlen.push_back(tl);
+      S.xp.push_back(xp);
+      S.mate_tid.push_back(paired ? 0 : -1);
+      S.read_names.push_back(r.name);
+      for (uint32_t c : R.cig) S.cigar.push_back(c);
+      S.cigar_off[k + 1] = (int64_t)S.cigar.size();
+      for (size_t q = 0; q < r.seq.size(); q++) {
+        const char c = r.seq[q];
+        S.seq.push_back(kCode[c == 'A' ? 0 : c == 'C' ? 1 : c == 'G' ? 2 : c == 'T' ? 3 : 4]);
+        S.qual.push_back((uint8_t)(r.qual[q] - 33));
+      }
+      S.seq_off[k + 1] = (int64_t)S.seq.size();
+      // XL / XR as breseq writes them (get_alignment_trims): no soft clips here
+      uint32_t el = 1, er = 1;
+      bsq::host_edge_trims(r.seq, el, er);
+      S.xl.push_back((int32_t)std::max<uint32_t>(gtrims[(size_t)R.pos], el));
+      S.xr.push_back((int32_t)std::max<uint32_t>(gtrims[(size_t)len + (size_t)R.end], er));
+    }
+    S.rg_ids.push_back("0");
+    S.rg_libs.push_back(paired ? "reads" : "reads_1");
+    S.rf_base.push_back(0);
+    S.rf_count.push_back(paired ? 2 : 1);
+    S.rf_names.push_back("reads_1");
+    if (paired) S.rf_names.push_back("reads_2");
+    S.finalize();
+    *out = smp;
+    bsq::set_status(status, BSQ_OK, "");
+    return BSQ_OK;
+  } catch (const BsqError& e) {
+    bsq::set_status(status, e.code, e.what());
+    return e.code;
+  }
+}

@@ -518,6 +518,20 @@ int bsq_simreads_defaults(bsq_simreads_args* args);
 /* FASTQ text of the reads (fastq_2: the second mates, paired-end only); release with bsq_free_text. */
 int bsq_simulate_reads(const char* seq, int64_t len, const bsq_simreads_args* args, char** fastq_1,
                        int64_t* len_1, char** fastq_2, int64_t* len_2, bsq_status* status);
+/* A planted mutation: ref_len reference bases from pos (0-based) replaced by alt (a SNP when both
+ * are one base long, an insertion before pos when ref_len is 0, a deletion when alt is empty). */
+typedef struct bsq_sim_mutation {
+  int64_t pos;
+  int32_t ref_len;
+  const char* alt;
+} bsq_sim_mutation;
+/* No counterpart in breseq, which aligns the simulated reads with bowtie2: the reads SIMULATE-READS
+ * makes from seq with the mutations applied, aligned to seq by construction (the bases each read was
+ * copied from; simulated and planted insertions as I, deletions as D), sorted, with XL/XR as
+ * get_alignment_trims writes them, MAPQ mapq and X1 1; a linear sequence of one target `name`. */
+int bsq_simulate_reads_sample(const char* name, const char* seq, int64_t len, const bsq_sim_mutation* muts,
+                              int32_t n_muts, const bsq_simreads_args* args, int32_t mapq, bsq_sample** out,
+                              bsq_status* status);
 
 #ifdef __cplusplus
 }

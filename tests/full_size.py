@@ -78,3 +78,32 @@ def required_cases(gd_text, polymorphism):
     if not polymorphism:
         out.pop("RA_interior_bounds")
     return out
+
+
+def c1_simreads_sample():
+    """C1 made the way its configuration describes it: breseq's SIMULATE-READS (-m single -l 100
+    -n 100000 --quality-start 32 --quality-end 32 --quality-stdev 6 --quality-min 2 --quality-max 41)
+    on the lambda genome (48,502 bp, the reference's test data) with 20 SNPs and 2 small indels
+    planted, aligned by construction."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "simreads_lambda.json.gz")
+    with gzip.open(path, "rt") as f:
+        ref = json.load(f)["reference"]
+    rng = np.random.default_rng(2026)
+    sites = []
+    while len(sites) < 22:
+        p = int(rng.integers(200, len(ref) - 200))
+        if all(abs(p - q) >= 30 for q in sites):
+            sites.append(p)
+    muts = []
+    for k, p in enumerate(sites):
+        if k < 20:
+            muts.append((p, 1, str(rng.choice([c for c in "ACGT" if c != ref[p]]))))
+        elif k == 20:
+            muts.append((p, 0, "".join(rng.choice(list("ACGT"), int(rng.integers(1, 6))))))
+        else:
+            muts.append((p, int(rng.integers(1, 6)), ""))
+    muts.sort()
+    return B.Sample.simulate_breseq(ref, name="NC_001416", mutations=muts, mode="single", n_reads=100000,
+                                    read_length=100, seed=1, quality_start=32, quality_end=32, quality_stdev=6,
+                                    quality_min=2, quality_max=41)
+

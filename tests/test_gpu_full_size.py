@@ -1,8 +1,9 @@
 """GPU parity at the benchmark workloads' own sizes (breseq_amd.workloads), through the pipeline
 breseq runs (error_count -> coverage-distribution fit -> identify_mutations, bsq_session_pipeline):
 
-  * C1 as specified (48,502 bp, 100,000 single-end 100-bp reads, about 206x) and C1 with a planted
-    400-bp deletion: the oracle runs live on the same sample (a few seconds);
+  * C1 as specified (48,502 bp, 100,000 single-end 100-bp reads, about 206x), C1 with a planted
+    400-bp deletion, and C1 as breseq's own SIMULATE-READS makes it on the lambda genome (20 SNPs
+    and 2 indels planted, reads aligned by construction): the oracle runs live on the same sample;
   * C2 (4.6 Mb at 100x, the benchmark's sample) and C3 (4.6 Mb at 500x, polymorphism mode): against
     the oracle's results committed by tests/golden/make_oracle_fixtures.py.
 
@@ -16,7 +17,7 @@ import pytest
 import breseq_amd as B
 import oracle_lib as O
 from breseq_amd import api, workloads
-from full_size import compare_with_fixture, fixture, mode_of, product_pipeline, required_cases
+from full_size import c1_simreads_sample, compare_with_fixture, fixture, mode_of, product_pipeline, required_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -35,11 +36,14 @@ def _names(sample):
     return [n for g in sample.read_file_groups for n in g]
 
 
-@pytest.mark.parametrize("name", ["C1", "C1_DEL"])
+@pytest.mark.parametrize("name", ["C1", "C1_DEL", "C1_SIMREADS"])
 def test_c1_against_live_oracle(name, tmp_path):
     say("[gpu] full size", name, "(live oracle)")
-    cfg = getattr(workloads, name)
-    s = B.Sample.simulate(**cfg)
+    if name == "C1_SIMREADS":
+        s = c1_simreads_sample()
+        name = "C1"
+    else:
+        s = B.Sample.simulate(**getattr(workloads, name))
     assert s.aln.contents.n_reads >= 99_000
     args = api.make_error_count_args(s)
     hists, counts, log10p, texts = O.error_count(args)

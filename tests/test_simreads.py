@@ -4,13 +4,13 @@ N(350, 40), seed 1, the command's default quality and indel-error model.  The FA
 expected files byte for byte (tests/golden/simreads_lambda.json.gz, written from the reference's
 test data by tests/golden/make_simreads_fixture.py).  Further checks: the single-end mode's
 placement and name fields, and the quality model's bounds."""
-import ctypes as C
 import gzip
 import json
 import os
 
 import pytest
 
+import breseq_amd as B
 from breseq_amd import _lib as L
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "simreads_lambda.json.gz")
@@ -21,38 +21,10 @@ def fixture():
         return json.load(f)
 
 
-class SimArgs(C.Structure):
-    _fields_ = [("mode", C.c_int32), ("circular", C.c_int32), ("n_reads", C.c_uint32), ("coverage", C.c_uint32),
-                ("read_length", C.c_uint32), ("fragment_mean", C.c_uint32), ("fragment_stdev", C.c_uint32),
-                ("seed", C.c_uint32), ("quality_start", C.c_uint32), ("quality_end", C.c_uint32),
-                ("quality_stdev", C.c_uint32), ("quality_min", C.c_uint32), ("quality_max", C.c_uint32),
-                ("indel_error_rate_per_million", C.c_uint32)]
-
-
 def simulate(seq, **kw):
-    lib = L.lib()
-    lib.bsq_simreads_defaults.argtypes = [C.POINTER(SimArgs)]
-    lib.bsq_simulate_reads.argtypes = [C.c_char_p, C.c_int64, C.POINTER(SimArgs), C.POINTER(C.c_void_p),
-                                       C.POINTER(C.c_int64), C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
-                                       C.POINTER(L.Status)]
-    a = SimArgs()
-    assert lib.bsq_simreads_defaults(C.byref(a)) == 0
-    for k, v in kw.items():
-        setattr(a, k, v)
-    p1, p2 = C.c_void_p(), C.c_void_p()
-    n1, n2 = C.c_int64(), C.c_int64()
-    st = L.Status()
-    b = seq.encode()
-    rc = lib.bsq_simulate_reads(b, len(b), C.byref(a), C.byref(p1), C.byref(n1), C.byref(p2), C.byref(n2), C.byref(st))
-    if rc != 0:
-        raise L.BsqError(rc, st.message.decode())
-    t1 = C.string_at(p1, n1.value).decode()
-    lib.bsq_free_text(p1)
-    t2 = None
-    if p2.value:
-        t2 = C.string_at(p2, n2.value).decode()
-        lib.bsq_free_text(p2)
-    return t1, t2
+    """(FASTQ text of read 1, of read 2 or None) through bsq_simulate_reads."""
+    out = B.api.simulate_reads_fastq(seq, **kw)
+    return out if isinstance(out, tuple) else (out, None)
 
 
 def test_paired_end_matches_the_reference_fixture():
@@ -102,3 +74,74 @@ def test_bad_arguments_are_refused():
         simulate("ACGT" * 10, mode=0, n_reads=1, read_length=100)   # read longer than a linear sequence
     with pytest.raises(L.BsqError):
         simulate("ACGT" * 100, mode=0, n_reads=1, read_length=50, quality_min=30, quality_max=20)
+
+
+# ---------------------------------------------------------------- aligned samples from the simulator
+
+import numpy as np  # noqa: E402
+
+OPS = "MIDNSHP=X"
+CODE = {1: "A", 2: "C", 4: "G", 8: "T", 15: "N"}
+COMP = {"A": "T", "C": "G", "G": "C", "T": "A", "N": "N"}
+
+
+def lambda_mutations(ref):
+    return [(1000, 1, "G" if ref[1000] != "G" else "A"), (5000, 0, "TTA"), (9000, 3, ""), (20000, 1, "C" if ref[20000] != "C" else "T")]
+
+
+def test_aligned_sample_walks_back_to_the_reference():
+    ref = fixture()["reference"]
+    muts = lambda_mutations(ref)
+    opts = dict(mode="single", n_reads=12000, read_length=100, seed=5, quality_start=32, quality_end=32,
+                quality_stdev=6)
+    s = B.Sample.simulate_breseq(ref, name="lambda", mutations=muts, **opts)
+    a = s.arrays()
+    n = a["pos"].size
+    assert n == 12000 and np.all(np.diff(a["pos"]) >= 0)
+    # the same reads as FASTQ, from the mutated sequence: each record's bases, in read direction
+    genome = ref[:1000] + muts[0][2] + ref[1001:5000] + "TTA" + ref[5000:9000] + ref[9003:20000] + muts[3][2] + ref[20001:]
+    fq = records(B.api.simulate_reads_fastq(genome, **opts))
+    by_name = {r[0][1:]: r for r in fq}
+    mism = total = 0
+    alt_at = {1000: 0, 20000: 0}
+    cover = {1000: 0, 20000: 0}
+    ins_seen = del_seen = 0
+    names = s.read_names()
+    ref_codes = np.frombuffer(ref.encode(), dtype=np.uint8)
+    lut = np.zeros(16, dtype=np.uint8)
+    for k, c in CODE.items():
+        lut[k] = ord(c)
+    for i in range(n):
+        cig = [(int(c) >> 4, OPS[int(c) & 15]) for c in a["cigar"][a["cigar_off"][i]:a["cigar_off"][i + 1]]]
+        codes = a["seq"][a["seq_off"][i]:a["seq_off"][i + 1]]
+        seq = lut[codes].tobytes().decode()
+        f = by_name[names[i]]
+        rev = bool(a["flag"][i] & 16)
+        assert f[1] == ("".join(COMP[c] for c in reversed(seq)) if rev else seq)
+        assert sum(l for l, o in cig if o in "MI") == len(seq)
+        p, q = int(a["pos"][i]), 0
+        if len(cig) == 1 and not any(p <= x < p + len(seq) for x in alt_at):
+            total += len(seq)
+            mism += int(np.count_nonzero(lut[codes] != ref_codes[p:p + len(seq)]))
+            continue
+        for l, o in cig:
+            if o == "M":
+                for k in range(l):
+                    total += 1
+                    mism += seq[q + k] != ref[p + k]
+                    if p + k in alt_at:
+                        cover[p + k] += 1
+                        alt_at[p + k] += seq[q + k] != ref[p + k]
+                p += l
+                q += l
+            elif o == "I":
+                ins_seen += p == 5000 and l == 3
+                q += l
+            elif o == "D":
+                del_seen += p == 9000 and l == 3
+                p += l
+    # substitution errors at Q ~32 plus the planted SNPs
+    assert mism / total < 0.004
+    for k in alt_at:
+        assert cover[k] > 10 and alt_at[k] >= 0.9 * cover[k], (k, alt_at[k], cover[k])
+    assert ins_seen > 10 and del_seen > 10

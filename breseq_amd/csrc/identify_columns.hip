@@ -64,6 +64,7 @@ struct Entry {
   uint32_t b, q;       // base code and quality at qpos (b = '.' for deletions)
   uint32_t qd;         // deletions: quality of the next read base in read direction
   bool del_ok;         // deletions: that base exists and is not N
+  bool del_oob;        // deletions: that base lies outside the read (an error if the entry is used)
   bool trimmed;        // is_trimmed at slot 0 (alignment.h:389-410)
 };
 
@@ -149,6 +150,7 @@ __device__ __forceinline__ void prepare_general(const ImParams& P, const DRead& 
   e.q = 0;
   e.qd = 0;
   e.del_ok = false;
+  e.del_oob = false;
   e.trimmed = is_trimmed(R, e.qpos, e.ondel < 0);
   if (e.cover && !is_del) {
     e.b = sq[e.qpos];
@@ -158,7 +160,9 @@ __device__ __forceinline__ void prepare_general(const ImParams& P, const DRead& 
     const int rev = (R.flags & kReadReversed) ? 1 : 0;
     const int qq = qpos + 1 - rev;
     if (qq < 0 || qq >= (int)R.l_qseq) {
-      err |= kErrQueryIndex;
+      // reported only if the entry is used: breseq skips redundant and trimmed reads before it
+      // looks at this base (identify_mutations.cpp:1591-1626)
+      e.del_oob = true;
     } else if (sq[qq] != 15) {
       e.qd = ql[qq];
       e.del_ok = true;
@@ -541,13 +545,17 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
           const uint32_t fr = rdlane(f_row, j);
           const uint32_t fl = rdlane(f_len, j);
           const uint32_t clo = rdlane(f_clo, j);
-          const bool valid = lane_ok && dj < (fl & 0xFFFFu) && bq != 15u;
+          // the conditions combine with & (not &&): evaluated in full on every lane, they are
+          // selects and mask operations, where short-circuit forms become branches around the
+          // loaded bytes
+          const bool valid = lane_ok & (dj < (fl & 0xFFFFu)) & (bq != 15u);
           uniq += valid ? 1 : 0;
-          const bool cand = valid && ((uint32_t)p - clo) < (fl >> 16);
-          const bool acgt = ((0x116u >> min(bq, 16u)) & 1u) != 0u;
-          const uint32_t o_rs = __builtin_amdgcn_ubfe(kLut, 2u * min(bq, 16u), 2u);
-          const bool ok = cand && acgt && (qq - (uint32_t)qcut) < qspan;
-          if (cand && !ok) {
+          const bool cand = valid & (((uint32_t)p - clo) < (fl >> 16));
+          const uint32_t bqc = min(bq, 16u);
+          const bool acgt = ((0x116u >> bqc) & 1u) != 0u;
+          const uint32_t o_rs = __builtin_amdgcn_ubfe(kLut, 2u * bqc, 2u);
+          const bool ok = cand & acgt & ((qq - (uint32_t)qcut) < qspan);
+          if (cand & !ok) {
             // not counted: quality below the cutoff, or an error (the reference asserts)
             // (the base-code test restated, so that the common path keeps one mask of it)
             if (__popc(bq) != 1 || bq > 8u) err |= kErrBaseCode;
@@ -572,17 +580,18 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
           A.SL += T.lpmax;
           A.Dall += dg;
           A.n += ok ? 1u : 0u;
-          if (ok && o_rs != ref_rs) {
+          if (ok & (o_rs != ref_rs)) {
+            // one branch for the rare non-reference observation; inside it every class is
+            // updated by a select (adding 0 leaves a sum unchanged), not by a branch per class
             const uint32_t obs = rev ? (o_rs ^ 3u) : o_rs;
             const float e = T.t[e_ref];
             const float g = e - dg;
             A.Gabs += fabsf(e) + fabsf(dg);
 #pragma unroll
             for (int o = 0; o < 4; o++) {
-              if (obs == (uint32_t)o) {
-                A.G[o] += g;
-                A.cnt[o] += 1u;
-              }
+              const bool hit = obs == (uint32_t)o;
+              A.G[o] += hit ? g : 0.0f;
+              A.cnt[o] += hit ? 1u : 0u;
             }
           }
         };
@@ -611,7 +620,8 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
         const uint32_t xf = rdlane(rc.z, j);
         const uint32_t x1 = xf & 0xFFFFu, rflags = xf >> 16;
         const uint32_t row_base = rdlane(rc.y, j);
-        // entry bits: 1 cover, 2 deletion, 4 deletion quality usable, 8 trimmed, 16 insertion follows
+        // entry bits: 1 cover, 2 deletion, 4 deletion quality usable, 8 trimmed, 16 insertion follows,
+        // 32 deletion quality base outside the read
         uint32_t eb;
         uint32_t qd = 0u;
         if (rflags & kReadSimple) {
@@ -625,7 +635,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
           Entry E;
           prepare_general(P, load_read(P.reads, r0 + j), p, E, err);
           eb = ((E.cover && lane_ok) ? 1u : 0u) | (E.is_del ? 2u : 0u) | (E.del_ok ? 4u : 0u) |
-               (E.trimmed ? 8u : 0u) | (E.ondel > 0 ? 16u : 0u);
+               (E.trimmed ? 8u : 0u) | (E.ondel > 0 ? 16u : 0u) | (E.del_oob ? 32u : 0u);
           bq = E.b;
           qq = E.q;
           qd = E.qd;
@@ -651,6 +661,8 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
         const uint32_t o4 = base_idx4(bq);
         const uint32_t obs = is_del ? 4u : o4;
         const uint32_t q = is_del ? qd : qq;
+        // a deletion whose quality base lies outside the read, on an untrimmed unique read
+        if (valid && !(eb & 8u) && (eb & 32u)) err |= kErrQueryIndex;
         const bool cand = valid && !(eb & 8u) && (!is_del || (eb & 4u));
         const bool bad_base = cand && !is_del && o4 > 3u;
         bool ok = cand && !bad_base && q >= (uint32_t)qcut;
@@ -684,10 +696,9 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
           A.Gabs += fabsf(e) + fabsf(dg);
 #pragma unroll
           for (int o = 0; o < 5; o++) {
-            if ((int)obs == o) {
-              A.G[o] += g;
-              A.cnt[o] += 1u;
-            }
+            const bool hit = (int)obs == o;
+            A.G[o] += hit ? g : 0.0f;
+            A.cnt[o] += hit ? 1u : 0u;
           }
         }
       }

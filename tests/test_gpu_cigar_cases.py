@@ -78,3 +78,38 @@ def test_preprocess_read_starts_match_oracle(seed):
             assert np.array_equal(res.histograms[g], h_o[g])
         assert np.array_equal(res.no_pos_hash_pr, texts["no_pos_hash_pr"]), (res.no_pos_hash_pr, texts["no_pos_hash_pr"])
         assert np.all(res.no_pos_hash_pr < 1.0)
+
+
+def test_deletion_next_to_the_last_base():
+    """A deletion one base before the read's end: its quality comes from the base after it in read
+    direction, which for a forward read lies past the end.  breseq reaches that base only for reads
+    that are neither redundant nor trimmed at the deletion (identify_mutations.cpp:1591-1626); XL/XR
+    as breseq writes them always trim the read's last base, so such reads are skipped, and both the
+    product and the oracle must accept them.  Without XL/XR both must refuse them."""
+    say("[gpu] deletion next to the last base")
+    ref = "ACGTTGCAAGATAGCTTACGGATCCATGCAAGGCTTAACGTGACCTGA"
+    reads = []
+    for k in range(12):
+        reads.append(dict(pos=2 + k, cigar="8M1D1M", seq=ref[2 + k:10 + k] + ref[11 + k], xl=1, xr=1))
+        reads.append(dict(pos=3 + k, cigar="10M", seq=ref[3 + k:13 + k], xl=1, xr=1))
+    reads.sort(key=lambda r: r["pos"])
+    for trimmed in (True, False):
+        rr = [dict(r) for r in reads]
+        if not trimmed:
+            for r in rr:
+                r["xl"] = r["xr"] = -1
+        s = BuiltSample(Built([("t0", ref)], rr))
+        args = B.api.make_error_count_args(s)
+        _, _, l_o, texts = O.error_count(args)
+        table = O.read_log10_prob_table(texts["error_rates"], len(l_o))
+        iargs = B.api.make_identify_args(s, table, args.read_set_max, args.quality_max, [1.0])
+        if trimmed:
+            gd_o, _ = O.identify_mutations(iargs)
+            gd_g = B.api.identify_mutations(s, table, args.read_set_max, args.quality_max, [1.0])
+            n, same, errs = compare_gd(gd_g, gd_o)
+            assert n == same
+        else:
+            with pytest.raises(B.BsqError):
+                O.identify_mutations(iargs)
+            with pytest.raises(B.BsqError):
+                B.api.identify_mutations(s, table, args.read_set_max, args.quality_max, [1.0])

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <charconv>
 #include <chrono>
 #include <cmath>
@@ -16,6 +17,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -160,6 +162,74 @@ static uint32_t read_file_of(const bsq_read_files* rf, const bsq_alignments* a, 
 
 static bool ref_consuming(uint32_t op) { return op == 0 || op == 2 || op == 3 || op == 7 || op == 8; }
 
+// Pinned staging buffers for host-to-device copies of large pageable arrays, shared by the sessions
+// of a device: two 32 MiB buffers per set, host threads fill one while the copy engine drains the
+// other.  A set is taken for one upload and put back afterwards.
+struct StagingSet {
+  int device = -1;
+  uint8_t* buf[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+};
+constexpr size_t kStageBytes = (size_t)32 << 20;
+static std::mutex g_stage_mu;
+static std::vector<StagingSet*> g_stage_free;
+
+static StagingSet* take_staging(int device) {
+  {
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    for (size_t i = 0; i < g_stage_free.size(); i++)
+      if (g_stage_free[i]->device == device) {
+        StagingSet* s = g_stage_free[i];
+        g_stage_free.erase(g_stage_free.begin() + (std::ptrdiff_t)i);
+        return s;
+      }
+  }
+  StagingSet* s = new StagingSet();
+  s->device = device;
+  for (int b = 0; b < 2; b++) {
+    HIP_OK(hipHostMalloc((void**)&s->buf[b], kStageBytes, hipHostMallocDefault));
+    HIP_OK(hipEventCreateWithFlags(&s->done[b], hipEventDisableTiming));
+  }
+  return s;
+}
+
+static void put_staging(StagingSet* s) {
+  std::lock_guard<std::mutex> g(g_stage_mu);
+  g_stage_free.push_back(s);
+}
+
+// dst (device) <- src (pageable host), in 32 MiB chunks through a staging set; the copies are
+// queued on S.stream, and the set is returned once the last one has completed.  Small arrays go
+// straight to hipMemcpyAsync.
+static void upload_staged(Session& S, void* dst, const void* src, size_t bytes) {
+  if (bytes < (kStageBytes >> 2)) {
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S.stream));
+    return;
+  }
+  StagingSet* st = take_staging(S.device);
+  try {
+    size_t i = 0;
+    for (size_t off = 0; off < bytes; off += kStageBytes, i++) {
+      const int b = (int)(i & 1);
+      const size_t n = std::min(kStageBytes, bytes - off);
+      if (i >= 2) HIP_OK(hipEventSynchronize(st->done[b]));   // the copy that last read this buffer
+      const int parts = 8;
+      pool_run(parts, [&](int k) {
+        const size_t p0 = n * (size_t)k / parts, p1 = n * (size_t)(k + 1) / parts;
+        memcpy(st->buf[b] + p0, (const uint8_t*)src + off + p0, p1 - p0);
+      });
+      HIP_OK(hipMemcpyAsync((uint8_t*)dst + off, st->buf[b], n, hipMemcpyHostToDevice, S.stream));
+      HIP_OK(hipEventRecord(st->done[b], S.stream));
+    }
+    for (int b = 0; b < 2; b++) HIP_OK(hipEventSynchronize(st->done[b]));
+  } catch (...) {
+    (void)hipStreamSynchronize(S.stream);
+    put_staging(st);
+    throw;
+  }
+  put_staging(st);
+}
+
 // Pack the alignments into the device CSR (DRead records + window index) and upload.
 static void session_load(Session& S, const bsq_reference* ref, const bsq_alignments* a, const bsq_read_files* rf) {
   const double t0 = now_ms();
@@ -183,129 +253,249 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
     memcpy(&S.h_ref[S.tgt_off[t]], ref->seqs[t], (size_t)ref->lengths[t]);
     std::fill(S.h_pos_target.begin() + S.tgt_off[t], S.h_pos_target.begin() + S.tgt_off[t] + ref->lengths[t], t);
   }
-  // reads
-  std::vector<DRead> dr;
-  dr.reserve((size_t)a->n_reads);
-  std::vector<uint8_t> h_rg;   // @RG index of each packed read (255: none)
-  h_rg.reserve((size_t)a->n_reads);
-  std::vector<int64_t> raw_off, src;
-  raw_off.reserve((size_t)a->n_reads);
-  src.reserve((size_t)a->n_reads);
-  int64_t dev_bytes = 0;
-  std::vector<uint32_t> cig;
-  std::map<int32_t, uint8_t> mslots;
-  std::vector<uint8_t> mq_of;
+  // reads, packed in two parallel passes over the input: (1) each alignment's record and whether it
+  // is packed (or the first invariant it breaks); (2) after prefix sums over the packed ones, the
+  // byte and CIGAR offsets and the copies.  Errors are those of the first offending alignment in
+  // input order, as a sequential walk would report them.
+  const int64_t n_in = a->n_reads;
+  std::vector<uint8_t> keep((size_t)n_in, 0);
+  struct Fail {
+    int64_t r = -1;
+    int code = 0;
+    std::string msg;
+  };
+  const int n_chunks = (int)std::min<int64_t>(256, std::max<int64_t>(1, n_in / 4096));
+  auto chunk_begin = [&](int c) { return n_in * c / n_chunks; };
+  std::vector<Fail> fails(n_chunks);
+  std::vector<int64_t> c_kept(n_chunks, 0), c_bytes(n_chunks, 0), c_cig(n_chunks, 0), c_nab(n_chunks, 0);
+  std::vector<int32_t> c_maxspan(n_chunks, 1);
+  std::vector<uint32_t> c_maxrf(n_chunks, 0);
+  std::vector<std::array<uint8_t, 256>> c_mq(n_chunks);
+  pool_run(n_chunks, [&](int c) {
+    Fail& f = fails[(size_t)c];
+    std::array<uint8_t, 256>& mq = c_mq[(size_t)c];
+    mq.fill(0);
+    auto fail = [&](int64_t r, int code, const char* msg) {
+      f.r = r;
+      f.code = code;
+      f.msg = msg;
+    };
+    // the chunk's sums are kept in locals and stored once: neighbouring chunks' slots share cache
+    // lines, and updating them per alignment from many threads would bounce those lines
+    int64_t kept = 0, bytes = 0, ncig = 0, nab_c = 0;
+    int32_t mspan = 1;
+    uint32_t mrf = 0;
+    for (int64_t r = chunk_begin(c); r < chunk_begin(c + 1) && f.r < 0; r++) {
+      if (a->flag[r] & 0x4) continue;
+      const int32_t tid = a->tid[r];
+      if (tid < 0) continue;
+      if (tid >= ref->n_targets) {
+        fail(r, BSQ_ERR_FORMAT, "alignment target id out of range");
+        break;
+      }
+      const int64_t c0 = a->cigar_off[r], c1 = a->cigar_off[r + 1];
+      const int64_t nc = c1 - c0;
+      if (nc == 0) continue;
+      if (nc > 255) {
+        fail(r, BSQ_ERR_LIMIT, "alignment with more than 255 CIGAR operations");
+        break;
+      }
+      const int64_t lq = a->seq_off[r + 1] - a->seq_off[r];
+      if (lq > 65535) {
+        fail(r, BSQ_ERR_LIMIT, "read longer than 65535 bases");
+        break;
+      }
+      int64_t span = 0, qlen = 0;
+      for (int64_t k = c0; k < c1; k++) {
+        const uint32_t op = a->cigar[k] & 15, len = a->cigar[k] >> 4;
+        if (ref_consuming(op)) span += len;
+        if (op == 0 || op == 1 || op == 4 || op == 7 || op == 8) qlen += len;
+      }
+      if (span == 0) continue;   // covers no reference base: not in any pileup, not packed
+      if (a->pos[r] < 0 || a->pos[r] + span > ref->lengths[tid]) {
+        fail(r, BSQ_ERR_FORMAT, "alignment extends past its reference");
+        break;
+      }
+      if (qlen != lq) {
+        fail(r, BSQ_ERR_FORMAT, "CIGAR query length differs from SEQ length");
+        break;
+      }
+      const int32_t x1 = (a->x1 && a->x1[r] >= 0) ? a->x1[r] : 1;
+      if (x1 > 65535) {
+        fail(r, BSQ_ERR_LIMIT, "X1 redundancy above 65535");
+        break;
+      }
+      const uint32_t rfi = read_file_of(rf, a, r);
+      if (rfi > 255) {
+        fail(r, BSQ_ERR_LIMIT, "more than 256 read files");
+        break;
+      }
+      if (a->rg && a->rg[r] >= 255) {
+        fail(r, BSQ_ERR_LIMIT, "more than 255 read groups");
+        break;
+      }
+      mq[a->mapq[r]] = 1;
+      keep[(size_t)r] = 1;
+      kept++;
+      bytes += (lq + 7) & ~(int64_t)7;
+      ncig += nc;
+      nab_c += span;
+      mspan = std::max<int32_t>(mspan, (int32_t)span);
+      mrf = std::max(mrf, rfi);
+    }
+    c_kept[(size_t)c] = kept;
+    c_bytes[(size_t)c] = bytes;
+    c_cig[(size_t)c] = ncig;
+    c_nab[(size_t)c] = nab_c;
+    c_maxspan[(size_t)c] = mspan;
+    c_maxrf[(size_t)c] = mrf;
+  });
+  std::vector<int64_t> o_kept(n_chunks + 1, 0), o_bytes(n_chunks + 1, 0), o_cig(n_chunks + 1, 0);
+  for (int c = 0; c < n_chunks; c++) {
+    o_kept[c + 1] = o_kept[c] + c_kept[c];
+    o_bytes[c + 1] = o_bytes[c] + c_bytes[c];
+    o_cig[c + 1] = o_cig[c] + c_cig[c];
+  }
+  int64_t fail_r = -1;
+  Fail first;
+  for (int c = 0; c < n_chunks; c++)
+    if (fails[c].r >= 0) {
+      first = fails[c];
+      fail_r = fails[c].r;
+      break;
+    }
+  const int64_t n_pack = o_kept[n_chunks];
+  // the packed arrays are written (and so first touched) by the threads that fill them
+  std::unique_ptr<DRead[]> dr(new DRead[(size_t)std::max<int64_t>(1, n_pack)]);
+  std::vector<uint8_t> h_rg((size_t)n_pack);   // @RG index of each packed read (255: none)
+  std::vector<int64_t> raw_off((size_t)n_pack), src((size_t)n_pack);
+  std::vector<uint32_t> cig((size_t)o_cig[n_chunks]);
+  pool_run(n_chunks, [&](int c) {
+    int64_t k = o_kept[c], bytes = o_bytes[c], co = o_cig[c];
+    for (int64_t r = chunk_begin(c); r < chunk_begin(c + 1); r++) {
+      if (!keep[(size_t)r]) continue;
+      const int64_t c0 = a->cigar_off[r], c1 = a->cigar_off[r + 1];
+      const int64_t lq = a->seq_off[r + 1] - a->seq_off[r];
+      int64_t span = 0, qlen = 0;
+      int n_ref_ops = 0, n_ins = 0;
+      uint32_t first_ref_op = 0;
+      int64_t y0 = 0;
+      bool seen_ref = false;
+      for (int64_t q = c0; q < c1; q++) {
+        const uint32_t op = a->cigar[q] & 15, len = a->cigar[q] >> 4;
+        cig[(size_t)(co + (q - c0))] = a->cigar[q];
+        if (ref_consuming(op)) {
+          span += len;
+          if (!seen_ref) first_ref_op = op;
+          seen_ref = true;
+          n_ref_ops++;
+        }
+        if (op == 0 || op == 1 || op == 4 || op == 7 || op == 8) qlen += len;
+        if (op == 1 || op == 6) n_ins++;
+        if (!seen_ref && (op == 1 || op == 4)) y0 += len;
+      }
+      DRead d;
+      memset(&d, 0, sizeof(d));
+      const int64_t gp = (int64_t)S.tgt_off[a->tid[r]] + a->pos[r];
+      d.seq_off = bytes;
+      d.cig_off = (uint32_t)co;
+      d.gpos = (int32_t)gp;
+      d.gend = (int32_t)(gp + span);
+      d.l_qseq = (uint16_t)lq;
+      d.n_cigar = (uint8_t)(c1 - c0);
+      const bool simple = (n_ref_ops == 1) && (n_ins == 0) && (first_ref_op == 0 || first_ref_op == 7 || first_ref_op == 8);
+      d.flags = (uint8_t)(((a->flag[r] & 0x10) ? kReadReversed : 0) | (simple ? kReadSimple : 0));
+      d.y0 = (uint16_t)y0;
+      // query_start_1 / query_end_1 / query_bounds_1 (alignment.cpp:125-289)
+      {
+        uint32_t qs = 1;
+        for (int64_t q = c0; q < c1; q++) {
+          if ((a->cigar[q] & 15) != 4) break;
+          qs += a->cigar[q] >> 4;
+        }
+        int32_t qe = (int32_t)qlen;
+        for (int64_t q = c1 - 1; q > c0; q--) {
+          if ((a->cigar[q] & 15) != 4) break;
+          qe -= (int32_t)(a->cigar[q] >> 4);
+        }
+        int32_t be = (int32_t)qlen;
+        for (int64_t q = c1 - 1; q > c0; q--) {
+          const uint32_t op = a->cigar[q] & 15;
+          if (op != 4 && op != 5 && op != 3) break;
+          if (op == 4) be -= (int32_t)(a->cigar[q] >> 4);
+        }
+        d.qs0 = (uint16_t)(qs - 1);
+        d.qe0 = (uint16_t)(qe - 1);
+        d.qbe0 = (uint16_t)(be - 1);
+      }
+      d.x1 = (uint16_t)((a->x1 && a->x1[r] >= 0) ? a->x1[r] : 1);
+      d.xl = (int16_t)((a->xl && a->xl[r] >= 0) ? std::min(a->xl[r], 32767) : -1);
+      d.xr = (int16_t)((a->xr && a->xr[r] >= 0) ? std::min(a->xr[r], 32767) : -1);
+      d.rfile = (uint8_t)read_file_of(rf, a, r);
+      d.mslot = a->mapq[r];   // remapped below
+      bytes += (lq + 7) & ~(int64_t)7;
+      co += c1 - c0;
+      raw_off[(size_t)k] = a->seq_off[r];
+      src[(size_t)k] = r;
+      h_rg[(size_t)k] = (a->rg && a->rg[r] >= 0) ? (uint8_t)a->rg[r] : (uint8_t)255;
+      dr[(size_t)k] = d;
+      k++;
+    }
+  });
+  // coordinate order over consecutive packed alignments; the sequential walk reports whichever
+  // comes first in input order: an alignment that starts before its packed predecessor, or the
+  // first alignment that breaks another invariant
+  {
+    std::vector<int64_t> unsorted_at(n_chunks, -1);
+    pool_run(n_chunks, [&](int c) {
+      for (int64_t k = std::max<int64_t>(1, o_kept[c]); k < o_kept[c + 1]; k++)
+        if (dr[(size_t)k].gpos < dr[(size_t)k - 1].gpos) {
+          unsorted_at[(size_t)c] = src[(size_t)k];
+          break;
+        }
+    });
+    int64_t sort_r = -1;
+    for (int c = 0; c < n_chunks; c++)
+      if (unsorted_at[c] >= 0) {
+        sort_r = unsorted_at[c];
+        break;
+      }
+    if (sort_r >= 0 && (fail_r < 0 || sort_r < fail_r))
+      throw BsqError(BSQ_ERR_FORMAT, "alignments are not sorted by coordinate");
+    if (fail_r >= 0) throw BsqError(first.code, first.msg);
+  }
+  const int64_t dev_bytes = o_bytes[n_chunks];
   int32_t maxspan = 1;
-  int64_t last_g = -1;
   int64_t nab = 0;
   uint32_t max_rfile = 0;
-  for (int64_t r = 0; r < a->n_reads; r++) {
-    if (a->flag[r] & 0x4) continue;
-    const int32_t tid = a->tid[r];
-    if (tid < 0) continue;
-    if (tid >= ref->n_targets) throw BsqError(BSQ_ERR_FORMAT, "alignment target id out of range");
-    const int64_t c0 = a->cigar_off[r], c1 = a->cigar_off[r + 1];
-    const int64_t nc = c1 - c0;
-    if (nc == 0) continue;
-    if (nc > 255) throw BsqError(BSQ_ERR_LIMIT, "alignment with more than 255 CIGAR operations");
-    const int64_t lq = a->seq_off[r + 1] - a->seq_off[r];
-    if (lq > 65535) throw BsqError(BSQ_ERR_LIMIT, "read longer than 65535 bases");
-    DRead d;
-    memset(&d, 0, sizeof(d));
-    const int64_t gp = (int64_t)S.tgt_off[tid] + a->pos[r];
-    int64_t span = 0, qlen = 0;
-    int n_ref_ops = 0, n_ins = 0;
-    uint32_t first_ref_op = 0;
-    int64_t y0 = 0;
-    bool seen_ref = false;
-    for (int64_t k = c0; k < c1; k++) {
-      const uint32_t op = a->cigar[k] & 15, len = a->cigar[k] >> 4;
-      if (ref_consuming(op)) {
-        span += len;
-        if (!seen_ref) first_ref_op = op;
-        seen_ref = true;
-        n_ref_ops++;
-      }
-      if (op == 0 || op == 1 || op == 4 || op == 7 || op == 8) qlen += len;
-      if (op == 1 || op == 6) n_ins++;
-      if (!seen_ref && (op == 1 || op == 4)) y0 += len;
-    }
-    if (span == 0) continue;   // covers no reference base: not in any pileup, not packed
-    if (a->pos[r] < 0 || a->pos[r] + span > ref->lengths[tid]) throw BsqError(BSQ_ERR_FORMAT, "alignment extends past its reference");
-    // device layout: every read starts at an 8-byte aligned offset (k_ec_reads loads dwords)
-    d.seq_off = dev_bytes;
-    dev_bytes += (lq + 7) & ~(int64_t)7;
-    raw_off.push_back(a->seq_off[r]);
-    src.push_back(r);
-    if (qlen != lq) throw BsqError(BSQ_ERR_FORMAT, "CIGAR query length differs from SEQ length");
-    if (gp < last_g) throw BsqError(BSQ_ERR_FORMAT, "alignments are not sorted by coordinate");
-    last_g = gp;
-    d.gpos = (int32_t)gp;
-    d.gend = (int32_t)(gp + span);
-    d.l_qseq = (uint16_t)lq;
-    d.n_cigar = (uint8_t)nc;
-    d.cig_off = (uint32_t)cig.size();
-    for (int64_t k = c0; k < c1; k++) cig.push_back(a->cigar[k]);
-    const bool simple = (n_ref_ops == 1) && (n_ins == 0) && (first_ref_op == 0 || first_ref_op == 7 || first_ref_op == 8);
-    d.flags = (uint8_t)(((a->flag[r] & 0x10) ? kReadReversed : 0) | (simple ? kReadSimple : 0));
-    d.y0 = (uint16_t)y0;
-    // query_start_1 / query_end_1 / query_bounds_1 (alignment.cpp:125-289)
-    {
-      uint32_t qs = 1;
-      for (int64_t k = c0; k < c1; k++) {
-        if ((a->cigar[k] & 15) != 4) break;
-        qs += a->cigar[k] >> 4;
-      }
-      int32_t qe = (int32_t)qlen;
-      for (int64_t k = c1 - 1; k > c0; k--) {
-        if ((a->cigar[k] & 15) != 4) break;
-        qe -= (int32_t)(a->cigar[k] >> 4);
-      }
-      int32_t be = (int32_t)qlen;
-      for (int64_t k = c1 - 1; k > c0; k--) {
-        const uint32_t op = a->cigar[k] & 15;
-        if (op != 4 && op != 5 && op != 3) break;
-        if (op == 4) be -= (int32_t)(a->cigar[k] >> 4);
-      }
-      d.qs0 = (uint16_t)(qs - 1);
-      d.qe0 = (uint16_t)(qe - 1);
-      d.qbe0 = (uint16_t)(be - 1);
-    }
-    const int32_t x1 = (a->x1 && a->x1[r] >= 0) ? a->x1[r] : 1;
-    if (x1 > 65535) throw BsqError(BSQ_ERR_LIMIT, "X1 redundancy above 65535");
-    d.x1 = (uint16_t)x1;
-    d.xl = (int16_t)((a->xl && a->xl[r] >= 0) ? std::min(a->xl[r], 32767) : -1);
-    d.xr = (int16_t)((a->xr && a->xr[r] >= 0) ? std::min(a->xr[r], 32767) : -1);
-    const uint32_t rfi = read_file_of(rf, a, r);
-    if (rfi > 255) throw BsqError(BSQ_ERR_LIMIT, "more than 256 read files");
-    max_rfile = std::max(max_rfile, rfi);
-    d.rfile = (uint8_t)rfi;
-    if (a->rg && a->rg[r] >= 255) throw BsqError(BSQ_ERR_LIMIT, "more than 255 read groups");
-    h_rg.push_back((a->rg && a->rg[r] >= 0) ? (uint8_t)a->rg[r] : (uint8_t)255);
-    auto it = mslots.find(a->mapq[r]);
-    if (it == mslots.end()) it = mslots.emplace(a->mapq[r], (uint8_t)0).first;
-    d.mslot = a->mapq[r];   // remapped below
-    maxspan = std::max<int32_t>(maxspan, (int32_t)span);
-    nab += span;
-    dr.push_back(d);
+  std::array<uint8_t, 256> mq_seen{};
+  for (int c = 0; c < n_chunks; c++) {
+    maxspan = std::max(maxspan, c_maxspan[c]);
+    nab += c_nab[c];
+    max_rfile = std::max(max_rfile, c_maxrf[c]);
+    for (int v = 0; v < 256; v++) mq_seen[v] |= c_mq[c][v];
   }
   // MAPQ slots in ascending MAPQ order
   S.mapq_values.clear();
   {
+    std::array<uint8_t, 256> slot_of{};
     uint8_t k = 0;
-    for (auto& kv : mslots) {
-      kv.second = k++;
-      S.mapq_values.push_back(kv.first);
-    }
-    for (DRead& d : dr) d.mslot = mslots[d.mslot];
+    for (int v = 0; v < 256; v++)
+      if (mq_seen[v]) {
+        slot_of[v] = k++;
+        S.mapq_values.push_back((uint8_t)v);
+      }
+    pool_run(n_chunks, [&](int c) {
+      for (int64_t i = o_kept[c]; i < o_kept[c + 1]; i++) dr[(size_t)i].mslot = slot_of[dr[(size_t)i].mslot];
+    });
   }
   S.n_read_files = (int32_t)max_rfile + 1;
-  S.n_dreads = (int64_t)dr.size();
+  S.n_dreads = (int64_t)(size_t)n_pack;
   S.h_src = std::move(src);
   S.n_input_reads = a->n_reads;
-  S.h_gpos.resize(dr.size());
-  S.h_gend_max.resize(dr.size());
-  for (size_t i = 0; i < dr.size(); i++) {
+  S.h_gpos.resize((size_t)n_pack);
+  S.h_gend_max.resize((size_t)n_pack);
+  for (size_t i = 0; i < (size_t)n_pack; i++) {
     S.h_gpos[i] = dr[i].gpos;
     S.h_gend_max[i] = std::max(dr[i].gend, i ? S.h_gend_max[i - 1] : (int32_t)-1);
   }
@@ -320,8 +510,8 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
     int32_t mx = 0;
     for (int64_t w = 0; w < S.n_windows; w++) {
       const int64_t w0 = w * kWave, w1 = w0 + kWave;
-      while (lo < dr.size() && (int64_t)dr[lo].gpos < w0 - maxspan) lo++;
-      while (hi < dr.size() && (int64_t)dr[hi].gpos < w1) hi++;
+      while (lo < (size_t)n_pack && (int64_t)dr[lo].gpos < w0 - maxspan) lo++;
+      while (hi < (size_t)n_pack && (int64_t)dr[hi].gpos < w1) hi++;
       rb[w] = (int32_t)lo;
       re[w] = (int32_t)hi;
       mx = std::max<int32_t>(mx, (int32_t)(hi - lo));
@@ -331,8 +521,8 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   const double t1 = now_ms();
   // upload
   const double t2 = now_ms();
-  S.d_reads.alloc(dr.size());
-  HIP_OK(hipMemcpyAsync(S.d_reads.p, dr.data(), dr.size() * sizeof(DRead), hipMemcpyHostToDevice, S.stream));
+  S.d_reads.alloc((size_t)n_pack);
+  HIP_OK(hipMemcpyAsync(S.d_reads.p, dr.get(), (size_t)n_pack * sizeof(DRead), hipMemcpyHostToDevice, S.stream));
   S.d_rg.alloc(h_rg.size());
   HIP_OK(hipMemcpyAsync(S.d_rg.p, h_rg.data(), h_rg.size(), hipMemcpyHostToDevice, S.stream));
   S.d_cigar.alloc(cig.size());
@@ -345,16 +535,16 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   S.d_quals.alloc((size_t)dev_bytes + 16);
   HIP_OK(hipMemsetAsync(S.d_bases.p, 0, (size_t)dev_bytes + 16, S.stream));
   HIP_OK(hipMemsetAsync(S.d_quals.p, 0, (size_t)dev_bytes + 16, S.stream));
-  if (nbases && !dr.empty()) {
+  if (nbases && n_pack > 0) {
     DBuf<uint8_t> raw_b, raw_q;
     DBuf<int64_t> d_raw_off;
     raw_b.alloc(nbases);
     raw_q.alloc(nbases);
     d_raw_off.alloc(raw_off.size());
-    HIP_OK(hipMemcpyAsync(raw_b.p, a->seq, nbases, hipMemcpyHostToDevice, S.stream));
-    HIP_OK(hipMemcpyAsync(raw_q.p, a->qual, nbases, hipMemcpyHostToDevice, S.stream));
+    upload_staged(S, raw_b.p, a->seq, nbases);
+    upload_staged(S, raw_q.p, a->qual, nbases);
     HIP_OK(hipMemcpyAsync(d_raw_off.p, raw_off.data(), raw_off.size() * sizeof(int64_t), hipMemcpyHostToDevice, S.stream));
-    hipLaunchKernelGGL(k_repack_reads, dim3(2048), dim3(256), 0, S.stream, S.d_reads.p, d_raw_off.p, (int64_t)dr.size(),
+    hipLaunchKernelGGL(k_repack_reads, dim3(2048), dim3(256), 0, S.stream, S.d_reads.p, d_raw_off.p, (int64_t)n_pack,
                        raw_b.p, raw_q.p, S.d_bases.p, S.d_quals.p);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(S.stream));
@@ -363,17 +553,21 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   HIP_OK(hipMemcpyAsync(S.d_ref.p, S.h_ref.data(), S.h_ref.size(), hipMemcpyHostToDevice, S.stream));
   {
     std::vector<uint8_t> ri(S.h_ref.size());
-    for (size_t i = 0; i < ri.size(); i++) {
-      switch (S.h_ref[i]) {
-        case 'A': ri[i] = 0; break;
-        case 'C': ri[i] = 1; break;
-        case 'G': ri[i] = 2; break;
-        case 'T': ri[i] = 3; break;
-        case '.': ri[i] = kRefDot; break;
-        case 'N': ri[i] = kRefN; break;
-        default: ri[i] = kRefOther; break;
+    const int nch = (int)std::min<size_t>(64, std::max<size_t>(1, ri.size() >> 16));
+    pool_run(nch, [&](int c) {
+      const size_t b = ri.size() * (size_t)c / nch, e = ri.size() * (size_t)(c + 1) / nch;
+      for (size_t i = b; i < e; i++) {
+        switch (S.h_ref[i]) {
+          case 'A': ri[i] = 0; break;
+          case 'C': ri[i] = 1; break;
+          case 'G': ri[i] = 2; break;
+          case 'T': ri[i] = 3; break;
+          case '.': ri[i] = kRefDot; break;
+          case 'N': ri[i] = kRefN; break;
+          default: ri[i] = kRefOther; break;
+        }
       }
-    }
+    });
     S.d_ref_idx.alloc(ri.size() + kRefIdxFrontPad);
     HIP_OK(hipMemsetAsync(S.d_ref_idx.p, kRefOther, ri.size() + kRefIdxFrontPad, S.stream));
     HIP_OK(hipMemcpyAsync(S.d_ref_idx.p + kRefIdxFrontPad, ri.data(), ri.size(), hipMemcpyHostToDevice, S.stream));

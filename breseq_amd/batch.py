@@ -3,14 +3,16 @@
 One sample's path (error_count, the coverage-distribution fit, identify_mutations; the order of
 breseq_cmdline.cpp:2293-2471) leaves the GPU partly idle: the exact scorer is a latency chain over
 few candidates, and the host formats results between the passes.  Several samples in flight at once
-fill those gaps.  Each sample gets its own device-resident session, with its own HIP streams, and a
-host thread that drives it; the device runs the kernels of different samples concurrently.
+fill those gaps.  Each worker thread drives its own device-resident session, with its own HIP
+streams, and loads the samples it runs into it one after another; the device runs the kernels of
+different samples concurrently.
 
 * ``run_batch`` runs a list of samples with at most ``parallel`` of them on the device at a time.
 * ``Server`` admits requests from a queue while their estimated device memory fits a budget, runs
   up to ``parallel`` at once, and records each request's latency from submission to finished GD.
 
-Every sample's results are those of the same sample run alone (the sessions share nothing).
+Every sample's results are those of the same sample run alone: a session reused for the next sample
+keeps only its streams and device buffers, and the load replaces everything in them.
 """
 import queue
 import threading
@@ -59,14 +61,61 @@ def run_pipeline(sample, mode=CONSENSUS, dp=None, session=None, **identify_kw):
                 total_s=time.perf_counter() - t0, aligned_bases=sample.aligned_bases())
 
 
-def run_batch(jobs, parallel=4):
-    """Runs jobs = [(sample, mode, dp), ...] with up to ``parallel`` in flight.  Returns (results in
-    job order, wall seconds)."""
+class SessionPool:
+    """One device session per worker thread, kept across the samples that thread runs: loading the
+    next sample reuses the session's streams and every device buffer that is already large enough,
+    so a served sample pays for its own data, not for device allocations."""
+
+    def __init__(self):
+        self._local = threading.local()
+        self._all = []
+        self._lock = threading.Lock()
+
+    def get(self):
+        sess = getattr(self._local, "session", None)
+        if sess is None:
+            sess = Session()
+            self._local.session = sess
+            with self._lock:
+                self._all.append(sess)
+        return sess
+
+    def close(self):
+        with self._lock:
+            for sess in self._all:
+                sess.close()
+            self._all = []
+
+
+def run_pooled(pool, sample, mode=CONSENSUS, dp=None, **identify_kw):
+    """run_pipeline on the calling thread's session of `pool`, the sample loaded into it first."""
     t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=max(1, parallel)) as ex:
-        futs = [ex.submit(run_pipeline, s, m, d) for s, m, d in jobs]
-        results = [f.result() for f in futs]
-    return results, time.perf_counter() - t0
+    sess = pool.get().load(sample)
+    t1 = time.perf_counter()
+    r = run_pipeline(sample, mode, dp, session=sess, **identify_kw)
+    r["load_s"] = t1 - t0
+    r["total_s"] = time.perf_counter() - t0
+    return r
+
+
+def run_batch(jobs, parallel=4, pool=None):
+    """Runs jobs = [(sample, mode, dp), ...] with up to ``parallel`` in flight, each worker thread on
+    its own reused session (a new one per sample when ``pool`` is False).  Returns (results in job
+    order, wall seconds)."""
+    own = pool is None
+    pool = SessionPool() if own else pool
+    t0 = time.perf_counter()
+    try:
+        with ThreadPoolExecutor(max_workers=max(1, parallel)) as ex:
+            if pool is False:
+                futs = [ex.submit(run_pipeline, s, m, d) for s, m, d in jobs]
+            else:
+                futs = [ex.submit(run_pooled, pool, s, m, d) for s, m, d in jobs]
+            results = [f.result() for f in futs]
+        return results, time.perf_counter() - t0
+    finally:
+        if own:
+            pool.close()
 
 
 class Server:
@@ -85,6 +134,7 @@ class Server:
         self.results = {}
         self.errors = {}
         self.pool = ThreadPoolExecutor(max_workers=self.parallel)
+        self.sessions = SessionPool()
         self.closed = False
         self.dispatcher = threading.Thread(target=self._dispatch, daemon=True)
         self.dispatcher.start()
@@ -109,7 +159,7 @@ class Server:
 
     def _run(self, key, sample, mode, dp, need, t_submit):
         try:
-            r = run_pipeline(sample, mode, dp)
+            r = run_pooled(self.sessions, sample, mode, dp)
             r["latency_s"] = time.perf_counter() - t_submit
             with self.cv:
                 self.results[key] = r
@@ -129,6 +179,7 @@ class Server:
             self.pending.put(None)
             self.dispatcher.join()
             self.pool.shutdown(wait=True)
+            self.sessions.close()
         return self.results, self.errors
 
 

@@ -15,7 +15,7 @@ def test_server_admission(monkeypatch):
     def fake_bytes(sample):
         return sizes[sample]
 
-    def fake_pipeline(sample, mode, dp):
+    def fake_pipeline(pool, sample, mode, dp):
         with lock:
             state["running"] += 1
             state["bytes"] += sizes[sample]
@@ -30,7 +30,7 @@ def test_server_admission(monkeypatch):
         return {"gd": "gd%d" % sample, "dp": None}
 
     monkeypatch.setattr(batch, "device_bytes", fake_bytes)
-    monkeypatch.setattr(batch, "run_pipeline", fake_pipeline)
+    monkeypatch.setattr(batch, "run_pooled", fake_pipeline)
     srv = batch.Server(parallel=3, hbm_budget=12 << 20)
     for i in range(12):
         srv.submit(i, i)

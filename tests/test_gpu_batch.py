@@ -56,3 +56,14 @@ def test_server_equals_alone():
         assert results[i]["gd"] == a["gd"]
         assert results[i]["dp"] == a["dp"]
         assert results[i]["latency_s"] > 0.0
+
+
+def test_reused_session_equals_alone():
+    say("[gpu] one session reused across samples of every shape vs each alone")
+    js = jobs()
+    alone = [batch.run_pipeline(s, m, d) for s, m, d in js]
+    order = [0, 1, 2, 3, 2, 0, 3, 1]   # larger and smaller samples, both modes, one after another
+    reused, _ = batch.run_batch([js[i] for i in order], parallel=1)
+    for i, r in zip(order, reused):
+        assert r["gd"] == alone[i]["gd"]
+        assert r["dp"] == alone[i]["dp"]

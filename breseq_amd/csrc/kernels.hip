@@ -63,75 +63,128 @@ __global__ void __launch_bounds__(64) k_insert_slots(ImParams P) {
 /* ------------------------------------------------------------------------------------------ */
 /* MC / UN runs                                                                                */
 
-// One thread per column; the thread at the first column of a run walks it to its end, so every
-// run is found and described by exactly one thread:
+// Runs are found by the columns that open them and walked by a whole wavefront, 256 columns a step
+// (four coalesced loads per lane), so a run of L columns costs L / 256 dependent steps:
 //   MC: maximal runs with unique coverage <= the propagation cutoff; the seed is reached if any
 //       column of the run, or the column closing it, has total coverage <= the seed cutoff
 //       (identify_mutations.cpp:2270-2281); the redundant sub-range follows :2326-2343.
 //   UN: maximal runs of columns without a predicted base (:2972-3007).
-__global__ void __launch_bounds__(256) k_runs(RunParams P) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P.n_pos) return;
-  const int t = P.pos_target[p];
-  if (t < 0) return;
-  const TargetInfo T = P.targets[t];
-  if (!T.scored || T.prop_cutoff < 0.0) return;
+// The reference's walk over an MC run keeps red_start at the column before the first column without
+// redundant coverage and red_end at the column after the last one, each -1 when the run starts
+// (ends) there; a run with redundant coverage throughout has red_start at its last column.  The
+// walk below gets both from the first and last such column of the run.
+constexpr int kRunSpan = 4 * WAVE;
+
+__device__ __forceinline__ int first_in(uint64_t m) { return __builtin_ctzll(m); }
+__device__ __forceinline__ int last_in(uint64_t m) { return 63 - __builtin_clzll(m); }
+
+__device__ void walk_mc_run(const RunParams& P, int s, int lane) {
+  const TargetInfo T = P.targets[P.pos_target[s]];
   const int first = T.off, last = T.off + T.len - 1;
-  auto in_mc = [&](int q) -> bool { return (double)P.col_unique[q] <= T.prop_cutoff; };
-  auto in_un = [&](int q) -> bool { return !(P.col_flags[q] & kPosPredicted); };
-  if (P.do_mc && in_mc(p) && (p == first || !in_mc(p - 1))) {
-    bool seed = false, reached_zero = false;
-    int red_start = -1, red_end = -1;
-    int q = p;
-    for (; q <= last && in_mc(q); q++) {
-      const uint8_t fl = P.col_flags[q];
-      if (fl & kPosSeed) seed = true;
-      if (!(fl & kPosRedundant)) {
-        reached_zero = true;
-        red_end = -1;
-      } else if (!reached_zero) {
-        red_start = q;
-      } else if (red_end < 0) {
-        red_end = q;
+  bool seed = false;
+  int fz = -1, lz = -1, e = last;
+  for (int q0 = s;; q0 += kRunSpan) {
+    bool in[4];
+    uint8_t fl[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int q = q0 + j * WAVE + lane;
+      const bool valid = q <= last;
+      const int32_t u = valid ? P.col_unique[q] : 0;
+      fl[j] = valid ? P.col_flags[q] : 0;
+      in[j] = valid && (double)u <= T.prop_cutoff;
+    }
+    int end = kRunSpan;   // first index of this step outside the run
+#pragma unroll
+    for (int j = 3; j >= 0; j--) {
+      const uint64_t out = __ballot(!in[j]);
+      if (out) end = j * WAVE + first_in(out);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const bool member = j * WAVE + lane < end;
+      seed |= member && (fl[j] & kPosSeed);
+      const uint64_t nr = __ballot(member && !(fl[j] & kPosRedundant));
+      if (nr) {
+        if (fz < 0) fz = q0 + j * WAVE + first_in(nr);
+        lz = q0 + j * WAVE + last_in(nr);
       }
     }
-    const int e = q - 1;
-    if (e < last && (P.col_flags[e + 1] & kPosSeed)) seed = true;
-    const int k = atomicAdd(P.n_runs, 1);
-    if (k < P.run_cap) {
-      RunRec R;
-      R.kind = 0;
-      R.s = p;
-      R.e = e;
-      R.seed = seed ? 1 : 0;
-      R.red_start = red_start;
-      R.red_end = red_end;
-      R.lo = (p > first) ? P.col_unique[p - 1] : -1;
-      R.li = P.col_unique[p];
-      R.ri = P.col_unique[e];
-      R.ro = (e < last) ? P.col_unique[e + 1] : -1;
-      P.runs[k] = R;
-    } else {
-      atomicOr(P.err, kErrCapacity);
+    if (end < kRunSpan) {
+      e = q0 + end - 1;
+      break;
     }
   }
-  if (in_un(p) && (p == first || !in_un(p - 1))) {
-    int q = p;
-    while (q <= last && in_un(q)) q++;
-    const int k = atomicAdd(P.n_runs, 1);
-    if (k < P.run_cap) {
-      RunRec R;
-      R.kind = 1;
-      R.s = p;
-      R.e = q - 1;
-      R.seed = 0;
-      R.red_start = R.red_end = -1;
-      R.lo = R.li = R.ri = R.ro = 0;
-      P.runs[k] = R;
-    } else {
-      atomicOr(P.err, kErrCapacity);
+  const bool any_seed = __any(seed);
+  if (lane != 0) return;
+  RunRec R;
+  R.kind = 0;
+  R.s = s;
+  R.e = e;
+  R.seed = (any_seed || (e < last && (P.col_flags[e + 1] & kPosSeed))) ? 1 : 0;
+  R.red_start = fz < 0 ? e : (fz > s ? fz - 1 : -1);
+  R.red_end = (fz >= 0 && lz < e) ? lz + 1 : -1;
+  R.lo = (s > first) ? P.col_unique[s - 1] : -1;
+  R.li = P.col_unique[s];
+  R.ri = P.col_unique[e];
+  R.ro = (e < last) ? P.col_unique[e + 1] : -1;
+  const int k = atomicAdd(P.n_runs, 1);
+  if (k < P.run_cap) P.runs[k] = R;
+  else atomicOr(P.err, kErrCapacity);
+}
+
+__device__ void walk_un_run(const RunParams& P, int s, int lane) {
+  const TargetInfo T = P.targets[P.pos_target[s]];
+  const int last = T.off + T.len - 1;
+  int e = last;
+  for (int q0 = s;; q0 += kRunSpan) {
+    int end = kRunSpan;
+#pragma unroll
+    for (int j = 3; j >= 0; j--) {
+      const int q = q0 + j * WAVE + lane;
+      const bool in = q <= last && !(P.col_flags[q] & kPosPredicted);
+      const uint64_t out = __ballot(!in);
+      if (out) end = j * WAVE + first_in(out);
+    }
+    if (end < kRunSpan) {
+      e = q0 + end - 1;
+      break;
     }
   }
+  if (lane != 0) return;
+  RunRec R;
+  R.kind = 1;
+  R.s = s;
+  R.e = e;
+  R.seed = 0;
+  R.red_start = R.red_end = -1;
+  R.lo = R.li = R.ri = R.ro = 0;
+  const int k = atomicAdd(P.n_runs, 1);
+  if (k < P.run_cap) P.runs[k] = R;
+  else atomicOr(P.err, kErrCapacity);
+}
+
+// One lane per column marks the runs that open there; the wavefront then walks each of its marked
+// runs in column order.  Every lane stays to the end: the walks need all 64.
+__global__ void __launch_bounds__(256) k_runs(RunParams P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (WAVE - 1);
+  bool mc_open = false, un_open = false;
+  if (p < P.n_pos) {
+    const int t = P.pos_target[p];
+    if (t >= 0) {
+      const TargetInfo T = P.targets[t];
+      if (T.scored && T.prop_cutoff >= 0.0) {
+        const bool at_first = p == T.off;
+        if (P.do_mc && (double)P.col_unique[p] <= T.prop_cutoff)
+          mc_open = at_first || !((double)P.col_unique[p - 1] <= T.prop_cutoff);
+        if (!(P.col_flags[p] & kPosPredicted)) un_open = at_first || (P.col_flags[p - 1] & kPosPredicted);
+      }
+    }
+  }
+  const int base = uniform_i(p - lane);
+  for (uint64_t m = __ballot(mc_open); m; m &= m - 1) walk_mc_run(P, base + first_in(m), lane);
+  for (uint64_t m = __ballot(un_open); m; m &= m - 1) walk_un_run(P, base + first_in(m), lane);
 }
 
 }  // namespace bsq

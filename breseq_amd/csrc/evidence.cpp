@@ -17,19 +17,25 @@ namespace bsq {
 
 static const char kIdx[6] = {'A', 'C', 'G', 'T', '.', 'N'};
 
+void counts_to_log10_prob_range(int S, int Q, const std::vector<uint64_t>& counts, double* out, int q0, int q1) {
+  // the table is quality-major: the bins of one quality are the 25 * S consecutive entries from q * 25 * S
+  (void)Q;
+  const size_t blk = (size_t)25 * S;
+  for (int q = q0; q < q1; q++) {
+    const uint64_t* c = counts.data() + (size_t)q * blk;
+    double* o = out + (size_t)q * blk;
+    // summed over obs_base for each (read_set, ref_base); the denominator's logarithm once each
+    double den[5 * 256];
+    for (size_t j = 0; j < (size_t)5 * S; j++) den[j] = 0.0;
+    for (size_t j = 0; j < blk; j++) den[j % ((size_t)5 * S)] += (double)c[j];
+    for (size_t j = 0; j < (size_t)5 * S; j++) den[j] = std::log10(den[j] + 5.0);
+    for (size_t j = 0; j < blk; j++) o[j] = std::log10((double)c[j] + 1.0) - den[j % ((size_t)5 * S)];
+  }
+}
+
 void counts_to_log10_prob(int S, int Q, const std::vector<uint64_t>& counts, std::vector<double>& out) {
-  const size_t nb = (size_t)S * 25 * Q;
-  // summed sub-table over obs_base (covariates read_set, ref_base, quality)
-  std::vector<double> sums((size_t)S * 5 * Q, 0.0);
-  for (size_t i = 0; i < nb; i++) {
-    const size_t rs = i % S, rb = (i / S) % 5, q = i / (25 * (size_t)S);
-    sums[rs + (size_t)S * rb + (size_t)5 * S * q] += (double)counts[i];
-  }
-  out.resize(nb);
-  for (size_t i = 0; i < nb; i++) {
-    const size_t rs = i % S, rb = (i / S) % 5, q = i / (25 * (size_t)S);
-    out[i] = std::log10((double)counts[i] + 1.0) - std::log10(sums[rs + (size_t)S * rb + (size_t)5 * S * q] + 5.0);
-  }
+  out.resize((size_t)S * 25 * Q);
+  counts_to_log10_prob_range(S, Q, counts, out.data(), 0, Q);
 }
 
 std::string error_rates_text(int S, int Q, const std::vector<double>& log10p) {

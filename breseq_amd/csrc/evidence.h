@@ -15,6 +15,8 @@ namespace bsq {
 
 // counts_to_log10_prob (error_count.cpp:1005-1026)
 void counts_to_log10_prob(int S, int Q, const std::vector<uint64_t>& counts, std::vector<double>& out);
+// the entries of qualities [q0, q1) only, into out (the whole table's storage); S <= 256
+void counts_to_log10_prob_range(int S, int Q, const std::vector<uint64_t>& counts, double* out, int q0, int q1);
 // write_log10_prob_table (error_count.cpp:660-690)
 std::string error_rates_text(int S, int Q, const std::vector<double>& log10p);
 // print_coverage (error_count.cpp:239-253)

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -160,6 +161,57 @@ std::string fmt_g(double v, int precision) {
     return buf;
   }
   return std::string(buf, (size_t)(r.ptr - buf));
+}
+
+double g6_round_trip(double v) {
+  const double a = std::fabs(v);
+  if (!std::isfinite(v) || !(a >= 1e-9 && a < 1e15)) return strtod(fmt_g(v).c_str(), nullptr);
+  typedef unsigned __int128 u128;
+  static const double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8,  1e9,  1e10, 1e11,
+                                    1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+  static const uint64_t kPow5[17] = {1ull,           5ull,           25ull,          125ull,         625ull,
+                                     3125ull,        15625ull,       78125ull,       390625ull,      1953125ull,
+                                     9765625ull,     48828125ull,    244140625ull,   1220703125ull,  6103515625ull,
+                                     30517578125ull, 152587890625ull};
+  // a = m * 2^be exactly
+  int e2 = 0;
+  const double fr = std::frexp(a, &e2);
+  const uint64_t m = (uint64_t)std::ldexp(fr, 53);
+  const int be = e2 - 53;
+  // floor(log10(a)) is floor((e2 - 1) log10 2) or one more; a wrong guess is corrected below
+  int d = (int)std::floor((double)(e2 - 1) * 0.30102999566398120);
+  const int d1 = d + 1;
+  if (d1 >= 0 ? a >= kPow10[d1] : a * kPow10[-d1] >= 1.0) d = d1;
+  for (int tries = 0; tries < 4; tries++) {
+    const int k = 5 - d;   // w = a * 10^k rounded, six digits when d is right
+    // a * 10^k = N / D with N = m * 5^max(k,0) and D = 5^max(-k,0) * 2^sh, sh = -(be + k) > 0 here
+    const int sh = -(be + k);
+    if (k < -16 || k > 16 || sh <= 0 || sh >= 120) break;
+    u128 w;
+    if (k >= 0) {   // D is a power of two: shift and mask
+      const u128 N = (u128)m * kPow5[k];
+      w = N >> sh;
+      const u128 r = N & ((((u128)1) << sh) - 1), h = ((u128)1) << (sh - 1);
+      if (r > h || (r == h && (w & 1u))) w++;
+    } else {
+      const u128 D = ((u128)kPow5[-k]) << sh;
+      w = (u128)m / D;
+      const u128 r = (u128)m - w * D;
+      if (2 * r > D || (2 * r == D && (w & 1u))) w++;
+    }
+    if (w >= 1000000u) {
+      d++;
+      continue;
+    }
+    if (w < 100000u) {
+      d--;
+      continue;
+    }
+    const double x = (double)(uint64_t)w;
+    const double y = (k >= 0) ? x / kPow10[k] : x * kPow10[-k];
+    return v < 0 ? -y : y;
+  }
+  return strtod(fmt_g(v).c_str(), nullptr);
 }
 
 static bool repeat_at(const char* s, int64_t len, int64_t pos, int64_t size, int64_t num) {

@@ -28,6 +28,13 @@ std::string fmt_double(double v, unsigned precision, bool scientific);
 int fmt_double_buf(double v, unsigned precision, bool scientific, char* buf, size_t len);
 // iostream default formatting of a double (%g, 6 significant digits).
 std::string fmt_g(double v, int precision = 6);
+// The double that reading fmt_g(v) back with strtod gives, without the text: exact 128-bit
+// integer arithmetic finds the six significant digits w and the scale k that %g prints (rounded half
+// to even on v's exact binary value), and w / 10^k or w * 10^-k is then one correctly rounded IEEE
+// operation, since w < 2^53 and 10^|k| <= 10^22 are exact doubles (Clinger's fast path of
+// decimal-to-binary conversion).  Values it does not cover (0, non-finite, |v| outside
+// [1e-9, 1e15)) go through the text.
+double g6_round_trip(double v);
 
 // Host threads for parallel host work: the hardware threads, at most 16.
 inline int host_threads() {

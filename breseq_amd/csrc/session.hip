@@ -62,6 +62,27 @@ struct DBuf {
   ~DBuf() { release(); }
 };
 
+// Pinned host memory, so that a hipMemcpyAsync from it returns at once; it is rewritten only after
+// the stream that read it has been synchronized.
+template <class T>
+struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) count = 1;
+    HIP_OK(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~PinnedBuf() { release(); }
+};
+
 static const int kWave = 64;
 
 struct Session {
@@ -115,6 +136,9 @@ struct Session {
   DBuf<unsigned long long> d_counts, d_hist;
   DBuf<uint32_t> d_err;
   DBuf<double> d_tab_lp, d_tab_r;
+  PinnedBuf<double> h_tab_r;       // the scorer's ratio table, uploaded while the column pass runs
+  PinnedBuf<unsigned long long> h_ec_counts, h_ec_hist;   // error_count's tables, copied back
+  PinnedBuf<uint32_t> h_ec_err;
   DBuf<uint32_t> d_start_bits;        // preprocess mode: read-start bits, 2 per column
   DBuf<unsigned long long> d_start_counts;   // preprocess mode: per target (no start, start)
   DBuf<uint32_t> d_key_scratch;      // read lists of the scorer and k_fix_columns for deep samples
@@ -645,7 +669,10 @@ struct bsq_im_result {
 
 namespace bsq {
 
-static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec_result& out) {
+// defer_rates_text: leave out.rates_text to the caller, which prints it off the critical path
+// (only when nothing is written to files here).
+static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec_result& out,
+                            bool defer_rates_text = false) {
   if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
   S.timing.fit_ms = S.timing.fit_wait_ms = S.timing.pipeline_ec_ms = 0.0;
   S.timing.pipeline_ec_kernel_ms[0] = S.timing.pipeline_ec_kernel_ms[1] = 0.0;
@@ -743,7 +770,18 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   hipLaunchKernelGGL(k_cov_hist, dim3(n_tiles), dim3(256), lds_hist, S.stream, C);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.ev1, S.stream));
+  // the tables come back with the kernels' own completion: pinned copies queued behind them
+  S.h_ec_counts.alloc((size_t)n_bins);
+  S.h_ec_hist.alloc((size_t)G * hist_len);
+  HIP_OK(hipMemcpyAsync(S.h_ec_counts.p, S.d_counts.p, n_bins * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        S.stream));
+  HIP_OK(hipMemcpyAsync(S.h_ec_hist.p, S.d_hist.p, (size_t)G * hist_len * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, S.stream));
+  S.h_ec_err.alloc(1);
+  HIP_OK(hipMemcpyAsync(S.h_ec_err.p, S.d_err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, S.stream));
+  const double t0 = now_ms();
   HIP_OK(hipStreamSynchronize(S.stream));
+  S.timing.d2h_ms = now_ms() - t0;
   float ms = 0, ms_reads = 0;
   HIP_OK(hipEventElapsedTime(&ms, S.ev0, S.ev1));
   HIP_OK(hipEventElapsedTime(&ms_reads, S.ev0, S.evk[0]));
@@ -751,14 +789,11 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
   for (double& d : S.timing.kernel_detail_ms) d = 0.0;
   S.timing.kernel_detail_ms[0] = ms_reads;
   S.timing.kernel_detail_ms[5] = ms - ms_reads;
-  check_err(S);
-  const double t0 = now_ms();
-  std::vector<unsigned long long> counts(n_bins), hist((size_t)G * hist_len);
-  HIP_OK(hipMemcpy(counts.data(), S.d_counts.p, n_bins * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-  HIP_OK(hipMemcpy(hist.data(), S.d_hist.p, hist.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-  S.timing.d2h_ms = now_ms() - t0;
+  raise_err_bits(*S.h_ec_err.p);
+  const unsigned long long* counts = S.h_ec_counts.p;
+  const unsigned long long* hist = S.h_ec_hist.p;
   const double t1 = now_ms();
-  out.counts.assign(counts.begin(), counts.end());
+  out.counts.assign(counts, counts + n_bins);
   out.hist.assign(G, std::vector<uint32_t>());
   for (int g = 0; g < G; g++) {
     int64_t last = -1;
@@ -779,12 +814,19 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
       out.no_pos_hash_pr[t] = (total != 0) ? (double)sc[2 * t] / total : 1.0;
     }
   }
-  counts_to_log10_prob(Sm, Q, out.counts, out.log10p);
-  out.rates_text = error_rates_text(Sm, Q, out.log10p);
+  // the Laplace table and the table identify_mutations reads back from error_rates.tab (each value
+  // as strtod returns it from its %g text, computed directly), split over host threads by quality
+  out.log10p.resize((size_t)n_bins);
+  out.rates_table.resize((size_t)n_bins);
   {
-    int s2 = 0, q2 = 0;
-    if (!parse_error_rates(out.rates_text, out.rates_table, s2, q2)) throw BsqError(BSQ_ERR_FORMAT, "error rate table round trip failed");
+    const int tasks = std::min(8, Q);
+    pool_run(tasks, [&](int k) {
+      const int q0 = Q * k / tasks, q1 = Q * (k + 1) / tasks;
+      counts_to_log10_prob_range(Sm, Q, out.counts, out.log10p.data(), q0, q1);
+      for (size_t i = (size_t)25 * Sm * q0; i < (size_t)25 * Sm * q1; i++) out.rates_table[i] = g6_round_trip(out.log10p[i]);
+    });
   }
+  if (!defer_rates_text || args->output_dir) out.rates_text = error_rates_text(Sm, Q, out.log10p);
   if (args->output_dir) {
     const std::string dir = args->output_dir;
     if (args->do_coverage) {
@@ -818,51 +860,78 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
 //   fold: keyed by (mslot, read file, quality, read-strand obs), hypotheses in read-strand
 //                order; a bottom-strand read reads the row of the complemented observation with
 //                its hypotheses complemented (the reference complements both before the lookup).
-static void build_likelihood_tables(const Session& S, const bsq_identify_args* a, std::vector<double>& lp,
-                                    std::vector<double>& rr, std::vector<FoldRow>& fold) {
+// A bottom-strand key looks up the same probabilities as the top-strand key of the complemented
+// observation, with the hypotheses complemented, so its values are that key's values permuted:
+// only the top-strand keys are computed.  The column pass needs lp and fold; rr (the scorer's
+// ratios) is built by the second call while the column pass runs.
+struct LikelihoodTables {
+  std::vector<double> lp, rr;
+  std::vector<FoldRow> fold;
+  size_t keys = 0;
+};
+
+static inline int comp_class(int i) { return i < 4 ? 3 - i : 4; }
+
+static void build_column_tables(const Session& S, const bsq_identify_args* a, LikelihoodTables& L) {
   const int Sm = a->read_set_max, Q = a->quality_max;
   const size_t nm = S.mapq_values.size();
-  const size_t nkeys = nm * Sm * Q * 10;
-  lp.assign(nkeys * 5, 0.0);
-  rr.assign(nkeys * 6, 0.0);
+  L.keys = nm * Sm * Q * 10;
+  L.lp.assign(L.keys * 5, 0.0);
   // one extra all-zero row: the row a filtered read reads
-  fold.assign(nm * Sm * Q * 5 + 1, FoldRow{{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, 0.0f, 0.0f, 0.0f});
+  L.fold.assign(nm * Sm * Q * 5 + 1, FoldRow{{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, 0.0f, 0.0f, 0.0f});
+  // the bins and rows of one quality depend on nothing else (a thread pool's wake-up costs more
+  // than the few thousand logarithms this takes, so the qualities run in order)
   std::vector<double> prob(a->n_bins);
-  for (int64_t i = 0; i < a->n_bins; i++) prob[i] = pow(10, a->log10_prob_table[i]);
-  auto comp = [](int i) { return i < 4 ? 3 - i : 4; };
-  for (size_t m = 0; m < nm; m++) {
-    const double incorrect = pow(10, -static_cast<double>(S.mapq_values[m]) / 10);
-    const double correct = 1 - incorrect;
-    const double uniform = 1.0 / 5.0;
-    for (int rs = 0; rs < Sm; rs++)
-      for (int q = 0; q < Q; q++)
-        for (int obs = 0; obs < 5; obs++) {
-          for (int sb = 0; sb < 2; sb++) {
-            const size_t key = (((m * Sm + rs) * Q + q) * 10) + obs * 2 + sb;
-            const bool top = (sb == 0);
-            const int o = top ? obs : comp(obs);
+  auto qualities = [&](int q0, int q1) {
+    for (int64_t i = (int64_t)25 * Sm * q0; i < (int64_t)25 * Sm * q1; i++) prob[i] = pow(10, a->log10_prob_table[i]);
+    for (size_t m = 0; m < nm; m++) {
+      const double incorrect = pow(10, -static_cast<double>(S.mapq_values[m]) / 10);
+      const double correct = 1 - incorrect;
+      const double uniform = 1.0 / 5.0;
+      for (int rs = 0; rs < Sm; rs++)
+        for (int q = q0; q < q1; q++) {
+          const size_t kq = ((m * Sm + rs) * Q + q) * 10;
+          for (int obs = 0; obs < 5; obs++) {
             double l[5];
             double mx = -std::numeric_limits<double>::max();
             for (int b = 0; b < 5; b++) {
-              const int refb = top ? b : comp(b);
-              double pr = correct * prob[rs + Sm * refb + 5 * Sm * o + 25 * Sm * q] + incorrect * uniform;
+              double pr = correct * prob[rs + Sm * b + 5 * Sm * obs + 25 * Sm * q] + incorrect * uniform;
               if (pr < 0.0) pr = 0.0;
               l[b] = log10(pr);
               mx = std::max(mx, l[b]);
             }
+            const size_t top = kq + obs * 2, bot = kq + comp_class(obs) * 2 + 1;
             for (int b = 0; b < 5; b++) {
-              lp[key * 5 + b] = l[b];
-              rr[key * 6 + b] = pow(10, l[b] - mx);
+              L.lp[top * 5 + b] = l[b];
+              L.lp[bot * 5 + comp_class(b)] = l[b];
             }
-            rr[key * 6 + 5] = mx;
-            if (top) {   // read-strand row == top-strand row for a top-strand read
-              const size_t row = ((m * Sm + rs) * Q + q) * 5 + obs;
-              for (int b = 0; b < 5; b++) fold[row].t[b] = (float)l[b];
-              fold[row].lpmax = (float)mx;
-              fold[row].diag = (float)l[obs];
-            }
+            // read-strand row == top-strand row for a top-strand read
+            const size_t row = ((m * Sm + rs) * Q + q) * 5 + obs;
+            for (int b = 0; b < 5; b++) L.fold[row].t[b] = (float)l[b];
+            L.fold[row].lpmax = (float)mx;
+            L.fold[row].diag = (float)l[obs];
           }
         }
+    }
+  };
+  qualities(0, Q);
+}
+
+static void build_scorer_ratios(LikelihoodTables& L) {
+  L.rr.assign(L.keys * 6, 0.0);
+  for (size_t k = 0; k < L.keys; k += 2) {   // top-strand keys; the bottom key of comp(obs) follows
+    const double* l = &L.lp[k * 5];
+    const size_t obs = (k % 10) / 2;
+    const size_t bot = k - obs * 2 + comp_class((int)obs) * 2 + 1;
+    double mx = -std::numeric_limits<double>::max();
+    for (int b = 0; b < 5; b++) mx = std::max(mx, l[b]);
+    for (int b = 0; b < 5; b++) {
+      const double r = pow(10, l[b] - mx);
+      L.rr[k * 6 + b] = r;
+      L.rr[bot * 6 + comp_class(b)] = r;
+    }
+    L.rr[k * 6 + 5] = mx;
+    L.rr[bot * 6 + 5] = mx;
   }
 }
 
@@ -1007,6 +1076,12 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   if (a->n_bins != (int64_t)Sm * 25 * Q) throw BsqError(BSQ_ERR_ARG, "error table size does not match its covariates");
   if (!a->deletion_propagation_cutoff || !a->deletion_seed_cutoff) throw BsqError(BSQ_ERR_ARG, "missing per-target cutoffs");
   const double th0 = now_ms();
+  // the per-call alignment view depends only on the sample and the covariate maxima: it is built on
+  // the device while the host prepares the tables
+  S.d_idrec.alloc((size_t)std::max<int64_t>(1, S.n_dreads));
+  HIP_OK(hipEventRecord(S.ev0, S.stream));
+  hipLaunchKernelGGL(k_prep_reads, dim3(1024), dim3(256), 0, S.stream, S.d_reads.p, S.n_dreads, Sm, Q, S.d_idrec.p);
+  HIP_OK(hipGetLastError());
   // targets
   std::vector<TargetInfo> T(S.n_targets);
   for (int t = 0; t < S.n_targets; t++) {
@@ -1021,14 +1096,13 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   }
   S.d_targets.alloc(T.size());
   HIP_OK(hipMemcpyAsync(S.d_targets.p, T.data(), T.size() * sizeof(TargetInfo), hipMemcpyHostToDevice, S.stream));
-  std::vector<double> lp, rr;
-  std::vector<FoldRow> fold;
-  build_likelihood_tables(S, a, lp, rr, fold);
-  S.d_tab_lp.alloc(lp.size());
-  S.d_tab_r.alloc(rr.size());
+  LikelihoodTables L;
+  build_column_tables(S, a, L);
+  const std::vector<FoldRow>& fold = L.fold;
+  S.d_tab_lp.alloc(L.lp.size());
+  S.d_tab_r.alloc(L.keys * 6);
   S.d_fold.alloc(fold.size());
-  HIP_OK(hipMemcpyAsync(S.d_tab_lp.p, lp.data(), lp.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
-  HIP_OK(hipMemcpyAsync(S.d_tab_r.p, rr.data(), rr.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
+  HIP_OK(hipMemcpyAsync(S.d_tab_lp.p, L.lp.data(), L.lp.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
   HIP_OK(hipMemcpyAsync(S.d_fold.p, fold.data(), fold.size() * sizeof(FoldRow), hipMemcpyHostToDevice, S.stream));
   double total_len = 0;
   for (int t = 0; t < S.n_targets; t++) total_len += (double)S.lengths[t];
@@ -1044,7 +1118,6 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   S.d_runrec.alloc(run_cap);
   const int fix_cap = std::max<int>(std::max<int>(1 << 16, S.n_pos / 256), S.id_caps.fix);
   S.d_fix.alloc(fix_cap);
-  S.d_idrec.alloc((size_t)std::max<int64_t>(1, S.n_dreads));
   ImParams P;
   P.reads = S.d_reads.p;
   P.idrec = S.d_idrec.p;
@@ -1063,7 +1136,7 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   P.base_quality_cutoff = a->base_quality_cutoff;
   P.tab_lp = S.d_tab_lp.p;
   P.tab_r = S.d_tab_r.p;
-  P.tab_keys = (int32_t)(rr.size() / 6);
+  P.tab_keys = (int32_t)L.keys;
   P.n_reads = (int32_t)S.n_dreads;
   P.fold = S.d_fold.p;
   P.fold_rows = (int32_t)fold.size() - 1;
@@ -1125,8 +1198,6 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   HIP_OK(hipMemsetAsync(S.d_counters.p, 0, 8 * sizeof(int32_t), S.stream));
   const double th1 = now_ms();
   // the whole pass is queued without a host round trip: later kernels read the counts on device
-  HIP_OK(hipEventRecord(S.ev0, S.stream));
-  hipLaunchKernelGGL(k_prep_reads, dim3(1024), dim3(256), 0, S.stream, S.d_reads.p, S.n_dreads, Sm, Q, S.d_idrec.p);
   {
     int64_t g = std::min<int64_t>(std::max<int64_t>(1, (S.n_windows + kColWaves - 1) / kColWaves), 2048);
     if (g >= 8) g &= ~(int64_t)7;   // a multiple of the 8 XCDs
@@ -1143,6 +1214,11 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
     hipLaunchKernelGGL(k_fix_columns_t<false>, dim3(kFixGrid), dim3(64), key_bytes, S.stream, P);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[3], S.stream));
+  // the scorer's ratio table, built while the column pass runs
+  build_scorer_ratios(L);
+  S.h_tab_r.alloc(L.rr.size());
+  std::copy(L.rr.begin(), L.rr.end(), S.h_tab_r.p);
+  HIP_OK(hipMemcpyAsync(S.d_tab_r.p, S.h_tab_r.p, L.rr.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
   // MC/UN runs read only the final column flags and coverage, so they run on the side stream beside
   // the insert slots and the scorer, and the host turns them into entries while the scorer runs
   HIP_OK(hipStreamWaitEvent(S.stream2, S.evk[3], 0));
@@ -1781,7 +1857,8 @@ int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const b
     if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
     if (cov->n_targets != S.n_targets || !cov->target_lengths) throw BsqError(BSQ_ERR_ARG, "coverage stage targets do not match the sample");
     const double e0 = now_ms();
-    run_error_count(S, ec, *e);
+    const bool text_later = !ec->output_dir;
+    run_error_count(S, ec, *e, text_later);
     const double ec_wall = now_ms() - e0;
     const double ec_k0 = S.timing.kernel_detail_ms[0], ec_k1 = S.timing.kernel_detail_ms[5];
     const int NT = S.n_targets;
@@ -1827,6 +1904,9 @@ int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const b
     args->n_coverage_groups = G;
     double* fit_ms = &S.timing.fit_ms;
     S.timing.fit_ms = 0.0;
+    // the error-rate text is printed on the fit's thread after the fit, while the column pass runs
+    bsq_ec_result* ep = e.get();
+    const int ec_sm = ec->read_set_max, ec_q = ec->quality_max;
     LateCutoffs late = std::async(std::launch::async, [=]() {
       const double f0 = now_ms();
       if (fit) coverage_stage(args.get(), summaries->data(), host_threads());
@@ -1835,6 +1915,7 @@ int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const b
       std::vector<double> c(NT);
       for (int t = 0; t < NT; t++) c[t] = (*summaries)[t].deletion_coverage_propagation_cutoff;
       *fit_ms = now_ms() - f0;
+      if (text_later) ep->rates_text = error_rates_text(ec_sm, ec_q, ep->log10p);
       return c;
     }).share();
     // targets whose sign the fit decides wait for it before the column pass

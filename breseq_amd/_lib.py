@@ -316,6 +316,7 @@ def lib():
     _decl(L, "bsq_session_precision_audit", C.c_int, [vp, P(IdentifyArgs), P(PrecisionReport), st])
     _decl(L, "bsq_session_alignment_trims", C.c_int, [vp, c_i32p, c_i32p, C.c_int64, st])
     _decl(L, "bsq_session_set_gd_writer", C.c_int, [vp, C.c_int32, st])
+    _decl(L, "bsq_session_set_gd_capacity", C.c_int, [vp, C.c_int32, st])
     _decl(L, "bsq_format_doubles_device", C.c_int, [c_f64p, C.c_int64, C.c_int32, C.c_int32, C.c_char_p, c_i32p, st])
     _decl(L, "bsq_session_destroy", None, [vp])
     _decl(L, "bsq_error_count", C.c_int, [P(ErrorCountArgs), P(vp), st])

@@ -278,6 +278,12 @@ class Session:
         L.check(L.lib().bsq_session_set_gd_writer(self._h, 1 if on_device else 0, C.byref(st)), st)
         return self
 
+    def set_gd_capacity(self, entries):
+        """Entries the device writer's lists hold before a pass needs more (default 4096)."""
+        st = _status()
+        L.check(L.lib().bsq_session_set_gd_capacity(self._h, int(entries), C.byref(st)), st)
+        return self
+
     def alignment_trims(self):
         """XL / XR of every alignment of the loaded sample, in its order (get_alignment_trims,
         calculate_trims.cpp:239-285): two int32 arrays, -1 where the record was not packed

@@ -23,16 +23,20 @@ struct GdEnt {
   uint64_t k0, k1, k2;
   uint64_t o0, o1, o2;
   int32_t type;   // 0 RA, 1 MC, 2 UN, 3 MC of a whole target (failed coverage fit)
-  int32_t src;    // candidate index; n_cand + run index; n_cand + n_runs + failed-target index
+  int32_t src;    // index into its own list: candidate, MC/UN run, or failed target
   int32_t t;      // target
   uint32_t id;
 };
 
 struct GdParams {
+  // the lists as the identify pass left them: counts read on the device (clamped to the list
+  // capacities), so the writer is queued behind the scorer without a host round trip
   const RaOut* ra;
-  int32_t n_cand;
+  const int32_t* n_cand;
+  int32_t cand_cap;
   const RunRec* runs;
-  int32_t n_runs;
+  const int32_t* n_runs;
+  int32_t run_cap;
   const int32_t* whole;   // targets whose coverage fit failed (whole-target MC), in traversal order
   int32_t n_whole;
   const int32_t* pos_target;
@@ -57,6 +61,7 @@ struct GdParams {
 };
 
 __global__ void k_gd_collect(GdParams P);
+constexpr int kGdRankGridX = 16, kGdRankGridY = 64;   // fixed grid of k_gd_rank (grid-stride over entries x tiles)
 __global__ void k_gd_rank(const GdEnt* ents, const int32_t* n_ents, int cap, int which, int32_t* rank);
 __global__ void k_gd_number(GdEnt* ents, const int32_t* n_ents, int cap, const int32_t* rank);
 __global__ void k_gd_lines(GdParams P, const int32_t* out_rank);

@@ -75,7 +75,8 @@ __device__ __forceinline__ double reported(const RaOut& r, int b) {
 }  // namespace
 
 __global__ void __launch_bounds__(256) k_gd_collect(GdParams P) {
-  const int64_t total = (int64_t)P.n_cand + P.n_runs + P.n_whole;
+  const int64_t n_cand = min(*P.n_cand, P.cand_cap), n_runs = min(*P.n_runs, P.run_cap);
+  const int64_t total = n_cand + n_runs + P.n_whole;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.ent_cap; i += (int64_t)gridDim.x * blockDim.x) {
     P.rank_id[i] = 0;
     P.rank_out[i] = 0;
@@ -84,8 +85,8 @@ __global__ void __launch_bounds__(256) k_gd_collect(GdParams P) {
     GdEnt e;
     e.o2 = 0;
     e.id = 0;
-    e.src = (int32_t)i;
-    if (i < P.n_cand) {
+    if (i < n_cand) {
+      e.src = (int32_t)i;
       const RaOut& r = P.ra[i];
       if (!r.emitted) continue;
       if (r.gpos < 0 || r.gpos >= P.n_pos) {
@@ -107,8 +108,9 @@ __global__ void __launch_bounds__(256) k_gd_collect(GdParams P) {
       e.o1 = ((uint64_t)(uint32_t)r.slot << 32) | ((uint64_t)(uint8_t)kIdxChar[min((int)r.ref_idx, 5)] << 8) |
              (uint64_t)(uint8_t)kIdxChar[min((int)r.variant, 5)];
       atomicAdd(P.n_type + 0, 1);
-    } else if (i < (int64_t)P.n_cand + P.n_runs) {
-      const RunRec& rr = P.runs[i - P.n_cand];
+    } else if (i < n_cand + n_runs) {
+      e.src = (int32_t)(i - n_cand);
+      const RunRec& rr = P.runs[e.src];
       if (rr.kind == 0 && !rr.seed) continue;   // an MC run whose seed was never reached
       if (rr.s < 0 || rr.s >= P.n_pos || rr.e < rr.s || rr.e >= P.n_pos) {
         atomicOr(P.err, 1u);
@@ -141,7 +143,8 @@ __global__ void __launch_bounds__(256) k_gd_collect(GdParams P) {
         atomicAdd(P.n_type + 2, 1);
       }
     } else {
-      const int t = P.whole[i - P.n_cand - P.n_runs];
+      e.src = (int32_t)(i - n_cand - n_runs);
+      const int t = P.whole[e.src];
       if (t < 0 || t >= P.n_targets) {
         atomicOr(P.err, 1u);
         continue;
@@ -162,35 +165,38 @@ __global__ void __launch_bounds__(256) k_gd_collect(GdParams P) {
   }
 }
 
-// rank of every entry's key (numbering key, or output key) among all n keys: block (x, y) counts
-// the keys of tile y (kGdRankTile keys in LDS) below each of its 256 entries and adds the count to
-// the entry's rank (zeroed by k_gd_collect)
+// rank of every entry's key (numbering key, or output key) among all n keys: block (x, y) takes
+// the tiles y, y + gridDim.y, ... (kGdRankTile keys in LDS each) and, for each, counts the keys of
+// the tile below each of the entries x * 256 + tid, stepping by gridDim.x * 256, adding the count
+// to the entry's rank (zeroed by k_gd_collect).  n is read on the device, so the grid is fixed.
 __global__ void __launch_bounds__(256) k_gd_rank(const GdEnt* ents, const int32_t* n_ents, int cap, int which,
                                                  int32_t* rank) {
   __shared__ GdEnt tile[kGdRankTile];
-  // the count read as a wave-uniform scalar: every thread reaches the barrier below
+  // the count read as a wave-uniform scalar: every thread of the block runs the same tile loop and
+  // reaches its barriers
   const int n = __builtin_amdgcn_readfirstlane(min(*n_ents, cap));
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j0 = blockIdx.y * kGdRankTile;
-  if (j0 >= n) return;
-  if (threadIdx.x < kGdRankTile && j0 + (int)threadIdx.x < n) tile[threadIdx.x] = ents[j0 + threadIdx.x];
-  __syncthreads();
-  if (i >= n) return;
-  const GdEnt me = ents[i];
-  const int m = min(kGdRankTile, n - j0);
-  int r = 0;
-  for (int j = 0; j < m; j++) r += key_less(tile[j], me, which) ? 1 : 0;
-  if (r) atomicAdd(rank + i, r);
+  for (int j0 = blockIdx.y * kGdRankTile; j0 < n; j0 += gridDim.y * kGdRankTile) {
+    __syncthreads();   // the previous tile is no longer read
+    if (threadIdx.x < kGdRankTile && j0 + (int)threadIdx.x < n) tile[threadIdx.x] = ents[j0 + threadIdx.x];
+    __syncthreads();
+    const int m = min(kGdRankTile, n - j0);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      const GdEnt me = ents[i];
+      int r = 0;
+      for (int j = 0; j < m; j++) r += key_less(tile[j], me, which) ? 1 : 0;
+      if (r) atomicAdd(rank + i, r);
+    }
+  }
 }
 
 // the ID (numbering rank + 1) becomes the last output key, as cGenomeDiff::sort breaks ties by it
 __global__ void __launch_bounds__(256) k_gd_number(GdEnt* ents, const int32_t* n_ents, int cap, const int32_t* rank) {
   const int n = min(*n_ents, cap);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t id = (uint32_t)rank[i] + 1u;
-  ents[i].id = id;
-  ents[i].o2 |= id;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t id = (uint32_t)rank[i] + 1u;
+    ents[i].id = id;
+    ents[i].o2 |= id;
+  }
 }
 
 // The line of each entry into slot out_rank (cGenomeDiff::write: type, ID, evidence '.', the
@@ -334,7 +340,7 @@ __global__ void __launch_bounds__(64) k_gd_lines(GdParams P, const int32_t* out_
           // when unset); a whole-target MC has 0 inside and nothing outside
           int32_t li = 0, lo = -1, ri = 0, ro = -1;
           if (e.type == 1) {
-            const RunRec& rr = P.runs[e.src - P.n_cand];
+            const RunRec& rr = P.runs[e.src];
             li = rr.li;
             lo = rr.lo;
             ri = rr.ri;

@@ -116,7 +116,13 @@ struct Session {
   struct {
     int32_t cand = 0, ins = 0, runs = 0, fix = 0;
   } id_caps;
-  std::vector<TargetInfo> h_targets;  // per-target parameters of the last identify (async copy source)
+  // pinned sources and destinations of the identify pass's small copies, so that queuing them never
+  // waits for the device (a pageable copy does)
+  PinnedBuf<TargetInfo> h_targets0;    // per-target parameters as the pass starts
+  PinnedBuf<TargetInfo> h_targets;     // per-target parameters with the fitted cutoffs
+  PinnedBuf<double> h_tab_lp;          // the likelihood tables of the column pass
+  PinnedBuf<FoldRow> h_fold;
+  PinnedBuf<int32_t> h_counters;       // [0..7] the pass's counters, [8] MC/UN runs found
   std::vector<int32_t> h_gpos;        // global first column of each packed read (sorted)
   std::vector<int64_t> h_src;         // input alignment index of each packed read
   int64_t n_input_reads = 0;          // alignments the sample was loaded from (packed or not)
@@ -156,6 +162,10 @@ struct Session {
   DBuf<RaOut> d_ra;
   // device writer of the genome diff (gd_emit.hip)
   bool gd_on_device = true;
+  int gd_cap = 4096;                       // entries the writer's lists hold (grows when a pass needs more)
+  size_t gd_text_hint = (size_t)64 << 10;  // text bytes copied back with the writer's counters
+  PinnedBuf<int32_t> h_gd_cnt, h_gd_meta;
+  PinnedBuf<char> h_gd_text, h_gd_names;
   DBuf<GdEnt> d_gd_ents;
   DBuf<int32_t> d_gd_rank_id, d_gd_rank_out, d_gd_lens, d_gd_meta, d_gd_cnt;
   DBuf<int64_t> d_gd_off;
@@ -946,53 +956,75 @@ using LateCutoffs = std::shared_future<std::vector<double>>;
 
 // The genome diff of one identify pass written on the device (gd_emit.hip): the scorer's RA results,
 // the MC / UN runs and one MC per target in `whole` (failed coverage fit) become numbered, sorted
-// lines; only the text comes back.  rank: each target's place in the traversal order.
-static std::string emit_gd_on_device(Session& S, int32_t n_cand, int32_t n_runs, const std::vector<int32_t>& whole,
-                                     const std::vector<int64_t>& rank, uint32_t places, int64_t& n_ra, int64_t& n_mc,
-                                     int64_t& n_un) {
-  const int64_t total = (int64_t)n_cand + n_runs + (int64_t)whole.size();
-  if (total > kGdMaxEntries) throw BsqError(BSQ_ERR_LIMIT, "more genome diff entries than the device writer holds");
-  if (places > (uint32_t)dec::kMaxPrecision) throw BsqError(BSQ_ERR_ARG, "polymorphism_precision_places above 60");
-  const int cap = (int)std::max<int64_t>(1, total);
+// lines; only the text comes back.  The kernels read the list counts on the device, so the whole
+// writer is queued behind the scorer and the MC/UN runs, with no host round trip; its counters and
+// a first stretch of the text come back by pinned copies queued behind it.  rank: each target's
+// place in the traversal order.
+struct GdInputs {
+  const int32_t* n_cand;   // device counters of the identify pass
+  int32_t cand_cap;
+  const int32_t* n_runs;
+  int32_t run_cap;
+  std::vector<int32_t> whole;
+  std::vector<int64_t> rank;
+  uint32_t places;
+};
+
+struct GdQueued {
+  int cap = 0;          // entries the writer's lists hold
+  size_t prefix = 0;    // text bytes copied back with the counters
+};
+
+static GdQueued queue_gd_on_device(Session& S, const GdInputs& in) {
+  if (in.places > (uint32_t)dec::kMaxPrecision) throw BsqError(BSQ_ERR_ARG, "polymorphism_precision_places above 60");
   const int T = S.n_targets;
-  S.d_gd_ents.alloc((size_t)cap);
-  S.d_gd_rank_id.alloc((size_t)cap);
-  S.d_gd_rank_out.alloc((size_t)cap);
-  S.d_gd_lens.alloc((size_t)cap);
-  S.d_gd_off.alloc((size_t)cap + 1);
-  S.d_gd_slots.alloc((size_t)cap * kGdSlot);
+  GdQueued q;
+  q.cap = std::max(1, std::min(S.gd_cap, kGdMaxEntries));
+  S.d_gd_ents.alloc((size_t)q.cap);
+  S.d_gd_rank_id.alloc((size_t)q.cap);
+  S.d_gd_rank_out.alloc((size_t)q.cap);
+  S.d_gd_lens.alloc((size_t)q.cap);
+  S.d_gd_off.alloc((size_t)q.cap + 1);
+  S.d_gd_slots.alloc((size_t)q.cap * kGdSlot);
+  S.d_gd_text.alloc((size_t)q.cap * kGdSlot);
   // per-pass tables: traversal rank, seq_id byte-order rank, name offsets, failed targets
   std::vector<int32_t> order_lex(T);
   for (int t = 0; t < T; t++) order_lex[t] = t;
   std::sort(order_lex.begin(), order_lex.end(), [&](int x, int y) { return S.names[x] < S.names[y]; });
-  std::vector<int32_t> meta((size_t)3 * T + 1 + whole.size() + 1);
-  int32_t* m_rank = meta.data();
+  const size_t n_meta = (size_t)3 * T + 1 + in.whole.size() + 1;
+  S.h_gd_meta.alloc(n_meta);
+  int32_t* m_rank = S.h_gd_meta.p;
   int32_t* m_lex = m_rank + T;
   int32_t* m_noff = m_lex + T;
   int32_t* m_whole = m_noff + T + 1;
   std::string names;
   for (int t = 0; t < T; t++) {
-    m_rank[t] = (int32_t)std::max<int64_t>(0, rank[t]);
+    m_rank[t] = (int32_t)std::max<int64_t>(0, in.rank[t]);
     m_noff[t] = (int32_t)names.size();
     names += S.names[t];
   }
   m_noff[T] = (int32_t)names.size();
   for (int k = 0; k < T; k++) m_lex[order_lex[k]] = k;
-  for (size_t k = 0; k < whole.size(); k++) m_whole[k] = whole[k];
-  S.d_gd_meta.alloc(meta.size());
+  for (size_t k = 0; k < in.whole.size(); k++) m_whole[k] = in.whole[k];
+  m_whole[in.whole.size()] = 0;
+  S.h_gd_names.alloc(std::max<size_t>(1, names.size()));
+  std::copy(names.begin(), names.end(), S.h_gd_names.p);
+  S.d_gd_meta.alloc(n_meta);
   S.d_gd_names.alloc(std::max<size_t>(1, names.size()));
   S.d_gd_cnt.alloc(8);
-  HIP_OK(hipMemcpyAsync(S.d_gd_meta.p, meta.data(), meta.size() * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
+  HIP_OK(hipMemcpyAsync(S.d_gd_meta.p, S.h_gd_meta.p, n_meta * sizeof(int32_t), hipMemcpyHostToDevice, S.stream));
   if (!names.empty())
-    HIP_OK(hipMemcpyAsync(S.d_gd_names.p, names.data(), names.size(), hipMemcpyHostToDevice, S.stream));
+    HIP_OK(hipMemcpyAsync(S.d_gd_names.p, S.h_gd_names.p, names.size(), hipMemcpyHostToDevice, S.stream));
   HIP_OK(hipMemsetAsync(S.d_gd_cnt.p, 0, 8 * sizeof(int32_t), S.stream));
   GdParams G;
   G.ra = S.d_ra.p;
-  G.n_cand = n_cand;
+  G.n_cand = in.n_cand;
+  G.cand_cap = in.cand_cap;
   G.runs = S.d_runrec.p;
-  G.n_runs = n_runs;
-  G.whole = S.d_gd_meta.p + (m_whole - meta.data());
-  G.n_whole = (int32_t)whole.size();
+  G.n_runs = in.n_runs;
+  G.run_cap = in.run_cap;
+  G.whole = S.d_gd_meta.p + (m_whole - S.h_gd_meta.p);
+  G.n_whole = (int32_t)in.whole.size();
   G.pos_target = S.d_pos_target.p;
   G.n_pos = (int32_t)S.n_pos;
   G.targets = S.d_targets.p;
@@ -1001,10 +1033,10 @@ static std::string emit_gd_on_device(Session& S, int32_t n_cand, int32_t n_runs,
   G.lexrank = S.d_gd_meta.p + T;
   G.names = S.d_gd_names.p;
   G.name_off = S.d_gd_meta.p + 2 * T;
-  G.places = (int32_t)places;
+  G.places = (int32_t)in.places;
   G.ents = S.d_gd_ents.p;
   G.n_ents = S.d_gd_cnt.p + 0;
-  G.ent_cap = cap;
+  G.ent_cap = q.cap;
   G.n_type = S.d_gd_cnt.p + 1;
   G.err = (uint32_t*)(S.d_gd_cnt.p + 4);
   G.slots = S.d_gd_slots.p;
@@ -1012,40 +1044,63 @@ static std::string emit_gd_on_device(Session& S, int32_t n_cand, int32_t n_runs,
   G.rank_out = S.d_gd_rank_out.p;
   G.lens = S.d_gd_lens.p;
   int64_t* d_total = (int64_t*)(S.d_gd_cnt.p + 6);
-  const dim3 rank_grid((unsigned)((cap + 255) / 256), (unsigned)((cap + kGdRankTile - 1) / kGdRankTile));
-  hipLaunchKernelGGL(k_gd_collect, dim3((unsigned)std::min<int64_t>(1024, (total + 255) / 256 + 1)), dim3(256), 0, S.stream, G);
+  const int cap = q.cap;
+  const dim3 rank_grid(kGdRankGridX, kGdRankGridY);
+  hipLaunchKernelGGL(k_gd_collect, dim3(256), dim3(256), 0, S.stream, G);
   hipLaunchKernelGGL(k_gd_rank, rank_grid, dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, 0, S.d_gd_rank_id.p);
-  hipLaunchKernelGGL(k_gd_number, dim3(rank_grid.x), dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, S.d_gd_rank_id.p);
+  hipLaunchKernelGGL(k_gd_number, dim3(64), dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, S.d_gd_rank_id.p);
   hipLaunchKernelGGL(k_gd_rank, rank_grid, dim3(256), 0, S.stream, S.d_gd_ents.p, G.n_ents, cap, 1, S.d_gd_rank_out.p);
   hipLaunchKernelGGL(k_gd_lines, dim3((unsigned)std::min(2048, (cap + 3) / 4)), dim3(64), 0, S.stream, G, S.d_gd_rank_out.p);
   hipLaunchKernelGGL(k_gd_offsets, dim3(1), dim3(1024), 0, S.stream, S.d_gd_lens.p, G.n_ents, cap, S.d_gd_off.p, d_total);
+  hipLaunchKernelGGL(k_gd_gather, dim3(256), dim3(256), 0, S.stream, S.d_gd_slots.p, S.d_gd_lens.p, S.d_gd_off.p, G.n_ents,
+                     cap, S.d_gd_text.p);
   HIP_OK(hipGetLastError());
-  int32_t cnt[8];
-  HIP_OK(hipMemcpyAsync(cnt, S.d_gd_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, S.stream));
-  HIP_OK(hipStreamSynchronize(S.stream));
-  const uint32_t gerr = (uint32_t)cnt[4];
-  if (gerr & 1u) throw BsqError(BSQ_ERR_DEVICE, "genome diff writer: an index outside its list");
-  if (gerr & 2u) throw BsqError(BSQ_ERR_DEVICE, "genome diff writer: entry list overflow");
-  if (gerr & 4u) throw BsqError(BSQ_ERR_LIMIT, "genome diff writer: an entry longer than its line slot");
-  if (gerr & 8u) throw BsqError(BSQ_ERR_ARG, "genome diff writer: a number precision it cannot print");
-  int64_t text_len;
-  std::memcpy(&text_len, &cnt[6], sizeof(text_len));
-  const int n = cnt[0];
-  n_ra = cnt[1];
-  n_mc = cnt[2];
-  n_un = cnt[3];
-  std::string text = "#=GENOME_DIFF\t1.0\n";
-  const size_t head = text.size();
-  text.resize(head + (size_t)text_len);
-  if (n > 0 && text_len > 0) {
-    S.d_gd_text.alloc((size_t)text_len);
-    hipLaunchKernelGGL(k_gd_gather, dim3((unsigned)std::min(1024, (n + 3) / 4)), dim3(256), 0, S.stream, S.d_gd_slots.p,
-                       S.d_gd_lens.p, S.d_gd_off.p, G.n_ents, cap, S.d_gd_text.p);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(&text[head], S.d_gd_text.p, (size_t)text_len, hipMemcpyDeviceToHost, S.stream));
-    HIP_OK(hipStreamSynchronize(S.stream));
+  S.h_gd_cnt.alloc(8);
+  HIP_OK(hipMemcpyAsync(S.h_gd_cnt.p, S.d_gd_cnt.p, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, S.stream));
+  q.prefix = std::min((size_t)q.cap * kGdSlot, S.gd_text_hint);
+  S.h_gd_text.alloc(q.prefix);
+  HIP_OK(hipMemcpyAsync(S.h_gd_text.p, S.d_gd_text.p, q.prefix, hipMemcpyDeviceToHost, S.stream));
+  return q;
+}
+
+// After the stream has been synchronized: the writer's checks and the text.  A writer whose entry
+// list overflowed is queued again with room for every entry, after the stream is idle.
+static std::string finish_gd_on_device(Session& S, const GdInputs& in, GdQueued q, int64_t& n_ra, int64_t& n_mc,
+                                       int64_t& n_un) {
+  for (int attempt = 0;; attempt++) {
+    const int32_t* cnt = S.h_gd_cnt.p;
+    const uint32_t gerr = (uint32_t)cnt[4];
+    const int n = cnt[0];
+    if ((gerr & 2u) || n > q.cap) {
+      if (n > kGdMaxEntries || attempt > 0) throw BsqError(BSQ_ERR_LIMIT, "more genome diff entries than the device writer holds");
+      S.gd_cap = std::min<int64_t>(kGdMaxEntries, (int64_t)n + n / 4 + 1024);
+      q = queue_gd_on_device(S, in);
+      HIP_OK(hipStreamSynchronize(S.stream));
+      continue;
+    }
+    if (gerr & 1u) throw BsqError(BSQ_ERR_DEVICE, "genome diff writer: an index outside its list");
+    if (gerr & 4u) throw BsqError(BSQ_ERR_LIMIT, "genome diff writer: an entry longer than its line slot");
+    if (gerr & 8u) throw BsqError(BSQ_ERR_ARG, "genome diff writer: a number precision it cannot print");
+    int64_t text_len;
+    std::memcpy(&text_len, &cnt[6], sizeof(text_len));
+    n_ra = cnt[1];
+    n_mc = cnt[2];
+    n_un = cnt[3];
+    std::string text = "#=GENOME_DIFF\t1.0\n";
+    const size_t head = text.size();
+    text.resize(head + (size_t)text_len);
+    if (n > 0 && text_len > 0) {
+      if ((size_t)text_len <= q.prefix) {
+        std::memcpy(&text[head], S.h_gd_text.p, (size_t)text_len);
+      } else {
+        HIP_OK(hipMemcpyAsync(&text[head], S.d_gd_text.p, (size_t)text_len, hipMemcpyDeviceToHost, S.stream));
+        HIP_OK(hipStreamSynchronize(S.stream));
+      }
+    }
+    // the next pass copies back twice this text's length with its counters
+    S.gd_text_hint = std::max<size_t>((size_t)64 << 10, 2 * (size_t)text_len);
+    return text;
   }
-  return text;
 }
 
 // The files of an identify pass: the genome diff text and the DP candidate regions.
@@ -1076,6 +1131,17 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   if (a->n_bins != (int64_t)Sm * 25 * Q) throw BsqError(BSQ_ERR_ARG, "error table size does not match its covariates");
   if (!a->deletion_propagation_cutoff || !a->deletion_seed_cutoff) throw BsqError(BSQ_ERR_ARG, "missing per-target cutoffs");
   const double th0 = now_ms();
+  // the DP candidate regions need nothing from the device: their own thread, from the start
+  std::vector<const char*> tn;
+  for (const std::string& n : S.names) tn.push_back(n.c_str());
+  double dp_ms = 0.0;
+  std::future<std::string> dp_text = std::async(std::launch::async, [&] {
+    const double d0 = now_ms();
+    std::string t = dp_regions_csv(S.n_targets, tn.data(), S.lengths.data(), a->call_target,
+                                   a->deletion_propagation_cutoff, a->dp, S.dp_reads);
+    dp_ms = now_ms() - d0;
+    return t;
+  });
   // the per-call alignment view depends only on the sample and the covariate maxima: it is built on
   // the device while the host prepares the tables
   S.d_idrec.alloc((size_t)std::max<int64_t>(1, S.n_dreads));
@@ -1095,15 +1161,21 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
       throw BsqError(BSQ_ERR_ARG, "deletion seed cutoff above the propagation cutoff is not supported");
   }
   S.d_targets.alloc(T.size());
-  HIP_OK(hipMemcpyAsync(S.d_targets.p, T.data(), T.size() * sizeof(TargetInfo), hipMemcpyHostToDevice, S.stream));
+  S.h_targets0.alloc(T.size());
+  std::copy(T.begin(), T.end(), S.h_targets0.p);
+  HIP_OK(hipMemcpyAsync(S.d_targets.p, S.h_targets0.p, T.size() * sizeof(TargetInfo), hipMemcpyHostToDevice, S.stream));
   LikelihoodTables L;
   build_column_tables(S, a, L);
   const std::vector<FoldRow>& fold = L.fold;
   S.d_tab_lp.alloc(L.lp.size());
   S.d_tab_r.alloc(L.keys * 6);
   S.d_fold.alloc(fold.size());
-  HIP_OK(hipMemcpyAsync(S.d_tab_lp.p, L.lp.data(), L.lp.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
-  HIP_OK(hipMemcpyAsync(S.d_fold.p, fold.data(), fold.size() * sizeof(FoldRow), hipMemcpyHostToDevice, S.stream));
+  S.h_tab_lp.alloc(L.lp.size());
+  S.h_fold.alloc(fold.size());
+  std::copy(L.lp.begin(), L.lp.end(), S.h_tab_lp.p);
+  std::copy(fold.begin(), fold.end(), S.h_fold.p);
+  HIP_OK(hipMemcpyAsync(S.d_tab_lp.p, S.h_tab_lp.p, L.lp.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
+  HIP_OK(hipMemcpyAsync(S.d_fold.p, S.h_fold.p, fold.size() * sizeof(FoldRow), hipMemcpyHostToDevice, S.stream));
   double total_len = 0;
   for (int t = 0; t < S.n_targets; t++) total_len += (double)S.lengths[t];
   S.d_col_flags.alloc(S.n_pos);
@@ -1219,8 +1291,18 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   S.h_tab_r.alloc(L.rr.size());
   std::copy(L.rr.begin(), L.rr.end(), S.h_tab_r.p);
   HIP_OK(hipMemcpyAsync(S.d_tab_r.p, S.h_tab_r.p, L.rr.size() * sizeof(double), hipMemcpyHostToDevice, S.stream));
-  // MC/UN runs read only the final column flags and coverage, so they run on the side stream beside
-  // the insert slots and the scorer, and the host turns them into entries while the scorer runs
+  // the insert slots and the scorer need no coverage cutoff: queued before the host waits for the fit
+  hipLaunchKernelGGL(k_insert_slots, dim3(1024), dim3(64), 0, S.stream, P);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(S.evk[1], S.stream));
+  if (P.keys_global)
+    hipLaunchKernelGGL(k_score_candidates_t<true>, dim3(kScoreGrid), dim3(kScoreWaves * 64), score_lds, S.stream, P);
+  else
+    hipLaunchKernelGGL(k_score_candidates_t<false>, dim3(kScoreGrid), dim3(kScoreWaves * 64), score_lds, S.stream, P);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(S.ev1, S.stream));
+  // MC/UN runs read only the final column flags and coverage and the fitted cutoffs, so they run on
+  // the side stream beside the insert slots and the scorer
   HIP_OK(hipStreamWaitEvent(S.stream2, S.evk[3], 0));
   std::vector<double> prop_final(a->deletion_propagation_cutoff, a->deletion_propagation_cutoff + S.n_targets);
   if (late) {
@@ -1232,44 +1314,41 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
       prop_final[t] = c[t];
       T[t].prop_cutoff = c[t];
     }
-    S.h_targets = T;
-    HIP_OK(hipMemcpyAsync(S.d_targets.p, S.h_targets.data(), T.size() * sizeof(TargetInfo), hipMemcpyHostToDevice,
+    S.h_targets.alloc(T.size());
+    std::copy(T.begin(), T.end(), S.h_targets.p);
+    HIP_OK(hipMemcpyAsync(S.d_targets.p, S.h_targets.p, T.size() * sizeof(TargetInfo), hipMemcpyHostToDevice,
                           S.stream2));
   }
   HIP_OK(hipEventRecord(S.evk[4], S.stream2));
   hipLaunchKernelGGL(k_runs, dim3((S.n_pos + 255) / 256), dim3(256), 0, S.stream2, R);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[5], S.stream2));
-  int32_t n_runs_seen = 0;
-  HIP_OK(hipMemcpyAsync(&n_runs_seen, S.d_counters.p + 2, sizeof(int32_t), hipMemcpyDeviceToHost, S.stream2));
-  hipLaunchKernelGGL(k_insert_slots, dim3(1024), dim3(64), 0, S.stream, P);
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(S.evk[1], S.stream));
-  if (P.keys_global)
-    hipLaunchKernelGGL(k_score_candidates_t<true>, dim3(kScoreGrid), dim3(kScoreWaves * 64), score_lds, S.stream, P);
-  else
-    hipLaunchKernelGGL(k_score_candidates_t<false>, dim3(kScoreGrid), dim3(kScoreWaves * 64), score_lds, S.stream, P);
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(S.ev1, S.stream));
-  int32_t counters[8];
-  HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, sizeof(counters), hipMemcpyDeviceToHost, S.stream));
-  // host work that needs no scorer output, done while the scorer runs: the DP candidate regions
-  // (their own thread) and the MC/UN entries
-  std::vector<const char*> tn;
-  for (const std::string& n : S.names) tn.push_back(n.c_str());
-  double dp_ms = 0.0;
-  std::future<std::string> dp_text = std::async(std::launch::async, [&] {
-    const double d0 = now_ms();
-    std::string t = dp_regions_csv(S.n_targets, tn.data(), S.lengths.data(), a->call_target,
-                                   a->deletion_propagation_cutoff, a->dp, S.dp_reads);
-    dp_ms = now_ms() - d0;
-    return t;
-  });
+  S.h_counters.alloc(9);
+  int32_t* counters = S.h_counters.p;
+  HIP_OK(hipMemcpyAsync(counters + 8, S.d_counters.p + 2, sizeof(int32_t), hipMemcpyDeviceToHost, S.stream2));
   std::vector<int32_t> order = target_order(S, a->call_target);
   std::vector<int64_t> rank(S.n_targets, -1);
   for (size_t i = 0; i < order.size(); i++) rank[order[i]] = (int64_t)i;
+  GdInputs gd_in;
+  GdQueued gd_q;
+  if (S.gd_on_device) {
+    // the writer reads the MC/UN runs too: the stream waits for them
+    HIP_OK(hipStreamWaitEvent(S.stream, S.evk[5], 0));
+    gd_in.n_cand = S.d_counters.p + 0;
+    gd_in.cand_cap = cand_cap;
+    gd_in.n_runs = S.d_counters.p + 2;
+    gd_in.run_cap = run_cap;
+    if (!a->skip_missing_coverage_prediction)
+      for (int t : order)
+        if (prop_final[t] < 0.0) gd_in.whole.push_back(t);
+    gd_in.rank = rank;
+    gd_in.places = a->polymorphism_precision_places;
+    gd_q = queue_gd_on_device(S, gd_in);
+  }
+  HIP_OK(hipMemcpyAsync(counters, S.d_counters.p, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, S.stream));
   std::vector<GdEntry> run_entries;
   HIP_OK(hipStreamSynchronize(S.stream2));
+  const int32_t n_runs_seen = counters[8];
   const double tr0 = now_ms();
   if (!S.gd_on_device && n_runs_seen <= run_cap) {   // past the cap the call fails below, with the other limit checks
     std::vector<RunRec> runs(n_runs_seen);
@@ -1348,13 +1427,8 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   S.timing.candidates = n_cand;
   if (S.gd_on_device) {
     const double t1 = now_ms();
-    std::vector<int32_t> whole;
-    if (!a->skip_missing_coverage_prediction)
-      for (int t : order)
-        if (prop_final[t] < 0.0) whole.push_back(t);
     int64_t n_ra = 0, n_mc = 0, n_un = 0;
-    out.text = emit_gd_on_device(S, n_cand, n_runs_seen, whole, rank, a->polymorphism_precision_places, n_ra, n_mc,
-                                 n_un);
+    out.text = finish_gd_on_device(S, gd_in, gd_q, n_ra, n_mc, n_un);
     out.n_ra = n_ra;
     out.n_mc = n_mc;
     out.n_un = n_un;
@@ -2016,6 +2090,16 @@ int bsq_session_set_gd_writer(bsq_session* s, int32_t on_device, bsq_status* sta
     return BSQ_ERR_ARG;
   }
   s->s.gd_on_device = on_device != 0;
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
+}
+
+int bsq_session_set_gd_capacity(bsq_session* s, int32_t entries, bsq_status* status) {
+  if (!s || entries < 1 || entries > kGdMaxEntries) {
+    set_status(status, BSQ_ERR_ARG, "null session, or a writer capacity outside 1..2^20");
+    return BSQ_ERR_ARG;
+  }
+  s->s.gd_cap = entries;
   set_status(status, BSQ_OK, "");
   return BSQ_OK;
 }

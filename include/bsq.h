@@ -358,6 +358,10 @@ int bsq_session_identify(bsq_session* s, const bsq_identify_args* args, bsq_im_r
  * numbers, sorts and prints the entries on the GPU and returns only the text; 0 assembles them on
  * the host from the scorer's results. */
 int bsq_session_set_gd_writer(bsq_session* s, int32_t on_device, bsq_status* status);
+/* Entries the device writer's lists hold (no counterpart in breseq; default 4096, at most 2^20).
+ * A pass whose genome diff has more entries is written again with room for all of them, and the
+ * session keeps the larger size; a caller expecting large genome diffs can size it once. */
+int bsq_session_set_gd_capacity(bsq_session* s, int32_t entries, bsq_status* status);
 /* Diagnostic without a counterpart in breseq: the device writer's number formatter (printf's %.*e
  * when scientific, else %.*f; NaN prints NA) over n values; out holds 512 bytes per value, lens the
  * text lengths (-1 for a precision above 60). */

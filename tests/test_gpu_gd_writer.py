@@ -63,13 +63,19 @@ def test_device_formatter_matches_printf():
     assert device_format(np.array([np.nan]), 3, True) == ["NA"]
 
 
-def both_writers(s, table, S, Q, prop, **kw):
+def both_writers(s, table, S, Q, prop, capacity=None, **kw):
     sess = B.Session().load(s)
     try:
         sess.set_gd_writer(False)
         host = B.api.identify_mutations(s, table, S, Q, prop, session=sess, **kw)
         sess.set_gd_writer(True)
+        if capacity is not None:
+            sess.set_gd_capacity(capacity)
         dev = B.api.identify_mutations(s, table, S, Q, prop, session=sess, **kw)
+        if capacity is not None:
+            # the session kept the grown lists: a second pass fits at once
+            again = B.api.identify_mutations(s, table, S, Q, prop, session=sess, **kw)
+            assert again == dev
     finally:
         sess.close()
     return host, dev
@@ -131,3 +137,14 @@ def test_writer_mc_with_redundant_ranges():
     host, dev = both_writers(s, table, args.read_set_max, args.quality_max, [15.0] * s.n_targets)
     assert dev == host
     assert counts(dev)["MC"] >= 1
+
+
+def test_writer_lists_grow_when_a_pass_needs_more():
+    say("[gpu] gd writers: entry lists smaller than the pass needs")
+    s = make_sample("poly")
+    args, (_, _, l_o, texts) = oracle_error_count(s)
+    table = O.read_log10_prob_table(texts["error_rates"], len(l_o))
+    host, dev = both_writers(s, table, args.read_set_max, args.quality_max, [8.0] * s.n_targets, capacity=5,
+                             mode=B.POLYMORPHISM)
+    assert dev == host
+    assert sum(counts(dev).values()) > 5

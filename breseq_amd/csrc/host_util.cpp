@@ -115,6 +115,54 @@ class Pool {
 
 void pool_run(int n_tasks, const std::function<void(int)>& fn) { Pool::get().run(n_tasks, fn); }
 
+namespace {
+
+// Threads for async_submit: a task goes to an idle thread, or to a new one when none is idle.
+class AsyncWorkers {
+ public:
+  static AsyncWorkers& get() {
+    static AsyncWorkers* w = new AsyncWorkers();   // never destroyed, like the pool
+    return *w;
+  }
+  void submit(std::function<void()> task) {
+    bool start = false;
+    {
+      std::lock_guard<std::mutex> g(m_);
+      tasks_.push_back(std::move(task));
+      if (idle_ < (int)tasks_.size()) {
+        start = true;
+        threads_++;
+      }
+    }
+    if (start) std::thread([this] { loop(); }).detach();
+    else cv_.notify_one();
+  }
+
+ private:
+  void loop() {
+    for (;;) {
+      std::function<void()> task;
+      {
+        std::unique_lock<std::mutex> g(m_);
+        idle_++;
+        cv_.wait(g, [&] { return !tasks_.empty(); });
+        idle_--;
+        task = std::move(tasks_.front());
+        tasks_.pop_front();
+      }
+      task();
+    }
+  }
+  std::mutex m_;
+  std::condition_variable cv_;
+  std::deque<std::function<void()>> tasks_;
+  int idle_ = 0, threads_ = 0;
+};
+
+}  // namespace
+
+void async_submit(std::function<void()> task) { AsyncWorkers::get().submit(std::move(task)); }
+
 void set_status(bsq_status* st, int code, const std::string& msg) {
   if (!st) return;
   st->code = code;

@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <cstdint>
 #include <functional>
+#include <future>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -74,6 +76,32 @@ void parallel_ranges(int64_t n, int threads, F fn) {
 // ended.  Several callers may submit at once: each takes part in its own job, so a busy pool
 // never blocks one.  An exception in a task is rethrown here as a BsqError.
 void pool_run(int n_tasks, const std::function<void(int)>& fn);
+
+// Runs task on a host thread that lives for the process and returns at once: a thread is started
+// only when every existing one is busy, so host work beside the device passes pays no thread start
+// per call.
+void async_submit(std::function<void()> task);
+
+// fn() on such a thread; the future holds its result or its exception.  Unlike std::async's, the
+// future does not wait in its destructor: callers whose fn refers to their locals wait for it
+// (WaitOnExit) before those go.
+template <class F>
+auto run_async(F fn) -> std::future<decltype(fn())> {
+  using R = decltype(fn());
+  auto task = std::make_shared<std::packaged_task<R()>>(std::move(fn));
+  std::future<R> f = task->get_future();
+  async_submit([task] { (*task)(); });
+  return f;
+}
+
+// Waits for a future (or shared future) when the scope ends, also when it ends by an exception.
+template <class Fut>
+struct WaitOnExit {
+  Fut& f;
+  ~WaitOnExit() {
+    if (f.valid()) f.wait();
+  }
+};
 
 // SequenceTrims (calculate_trims.cpp:67-138): out[0..len) left, out[len..2len) right.
 void host_sequence_trims(const char* seq, int64_t len, uint8_t* out);

@@ -145,6 +145,7 @@ struct Session {
   PinnedBuf<double> h_tab_r;       // the scorer's ratio table, uploaded while the column pass runs
   PinnedBuf<unsigned long long> h_ec_counts, h_ec_hist;   // error_count's tables, copied back
   PinnedBuf<uint32_t> h_ec_err;
+  int32_t idrec_sm = -1, idrec_q = -1;   // covariate maxima of an alignment view already queued
   DBuf<uint32_t> d_start_bits;        // preprocess mode: read-start bits, 2 per column
   DBuf<unsigned long long> d_start_counts;   // preprocess mode: per target (no start, start)
   DBuf<uint32_t> d_key_scratch;      // read lists of the scorer and k_fix_columns for deep samples
@@ -267,6 +268,7 @@ static void upload_staged(Session& S, void* dst, const void* src, size_t bytes) 
 // Pack the alignments into the device CSR (DRead records + window index) and upload.
 static void session_load(Session& S, const bsq_reference* ref, const bsq_alignments* a, const bsq_read_files* rf) {
   const double t0 = now_ms();
+  S.idrec_sm = S.idrec_q = -1;   // no alignment view of the new sample yet
   if (!ref || !a) throw BsqError(BSQ_ERR_ARG, "null reference or alignments");
   S.n_targets = ref->n_targets;
   S.names.clear();
@@ -681,8 +683,10 @@ namespace bsq {
 
 // defer_rates_text: leave out.rates_text to the caller, which prints it off the critical path
 // (only when nothing is written to files here).
+// queue_idrec: also queue identify's alignment view for these covariate maxima behind the tables'
+// copies (the pipeline's identify pass follows), so that it is built while the host works.
 static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec_result& out,
-                            bool defer_rates_text = false) {
+                            bool defer_rates_text = false, bool queue_idrec = false) {
   if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "session has no sample loaded");
   S.timing.fit_ms = S.timing.fit_wait_ms = S.timing.pipeline_ec_ms = 0.0;
   S.timing.pipeline_ec_kernel_ms[0] = S.timing.pipeline_ec_kernel_ms[1] = 0.0;
@@ -789,8 +793,17 @@ static void run_error_count(Session& S, const bsq_error_count_args* args, bsq_ec
                         hipMemcpyDeviceToHost, S.stream));
   S.h_ec_err.alloc(1);
   HIP_OK(hipMemcpyAsync(S.h_ec_err.p, S.d_err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, S.stream));
+  HIP_OK(hipEventRecord(S.evk[7], S.stream));
+  S.idrec_sm = S.idrec_q = -1;
+  if (queue_idrec) {
+    S.d_idrec.alloc((size_t)std::max<int64_t>(1, S.n_dreads));
+    hipLaunchKernelGGL(k_prep_reads, dim3(1024), dim3(256), 0, S.stream, S.d_reads.p, S.n_dreads, Sm, Q, S.d_idrec.p);
+    HIP_OK(hipGetLastError());
+    S.idrec_sm = Sm;
+    S.idrec_q = Q;
+  }
   const double t0 = now_ms();
-  HIP_OK(hipStreamSynchronize(S.stream));
+  HIP_OK(hipEventSynchronize(S.evk[7]));
   S.timing.d2h_ms = now_ms() - t0;
   float ms = 0, ms_reads = 0;
   HIP_OK(hipEventElapsedTime(&ms, S.ev0, S.ev1));
@@ -1135,19 +1148,23 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   std::vector<const char*> tn;
   for (const std::string& n : S.names) tn.push_back(n.c_str());
   double dp_ms = 0.0;
-  std::future<std::string> dp_text = std::async(std::launch::async, [&] {
+  std::future<std::string> dp_text = run_async([&] {
     const double d0 = now_ms();
     std::string t = dp_regions_csv(S.n_targets, tn.data(), S.lengths.data(), a->call_target,
                                    a->deletion_propagation_cutoff, a->dp, S.dp_reads);
     dp_ms = now_ms() - d0;
     return t;
   });
+  WaitOnExit<std::future<std::string>> dp_wait{dp_text};   // it reads this frame's locals
   // the per-call alignment view depends only on the sample and the covariate maxima: it is built on
   // the device while the host prepares the tables
-  S.d_idrec.alloc((size_t)std::max<int64_t>(1, S.n_dreads));
   HIP_OK(hipEventRecord(S.ev0, S.stream));
-  hipLaunchKernelGGL(k_prep_reads, dim3(1024), dim3(256), 0, S.stream, S.d_reads.p, S.n_dreads, Sm, Q, S.d_idrec.p);
-  HIP_OK(hipGetLastError());
+  if (S.idrec_sm != Sm || S.idrec_q != Q) {   // not already queued by the pipeline's error_count
+    S.d_idrec.alloc((size_t)std::max<int64_t>(1, S.n_dreads));
+    hipLaunchKernelGGL(k_prep_reads, dim3(1024), dim3(256), 0, S.stream, S.d_reads.p, S.n_dreads, Sm, Q, S.d_idrec.p);
+    HIP_OK(hipGetLastError());
+  }
+  S.idrec_sm = S.idrec_q = -1;   // a later pass builds its own
   // targets
   std::vector<TargetInfo> T(S.n_targets);
   for (int t = 0; t < S.n_targets; t++) {
@@ -1932,7 +1949,7 @@ int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const b
     if (cov->n_targets != S.n_targets || !cov->target_lengths) throw BsqError(BSQ_ERR_ARG, "coverage stage targets do not match the sample");
     const double e0 = now_ms();
     const bool text_later = !ec->output_dir;
-    run_error_count(S, ec, *e, text_later);
+    run_error_count(S, ec, *e, text_later, true);
     const double ec_wall = now_ms() - e0;
     const double ec_k0 = S.timing.kernel_detail_ms[0], ec_k1 = S.timing.kernel_detail_ms[5];
     const int NT = S.n_targets;
@@ -1981,7 +1998,7 @@ int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const b
     // the error-rate text is printed on the fit's thread after the fit, while the column pass runs
     bsq_ec_result* ep = e.get();
     const int ec_sm = ec->read_set_max, ec_q = ec->quality_max;
-    LateCutoffs late = std::async(std::launch::async, [=]() {
+    LateCutoffs late = run_async([=]() {
       const double f0 = now_ms();
       if (fit) coverage_stage(args.get(), summaries->data(), host_threads());
       (void)H;
@@ -1992,6 +2009,7 @@ int bsq_session_pipeline(bsq_session* s, const bsq_error_count_args* ec, const b
       if (text_later) ep->rates_text = error_rates_text(ec_sm, ec_q, ep->log10p);
       return c;
     }).share();
+    WaitOnExit<LateCutoffs> late_wait{late};   // it writes the error_count result
     // targets whose sign the fit decides wait for it before the column pass
     bool wait_first = false;
     for (int t = 0; t < NT; t++) wait_first = wait_first || sign[t] < -1.0;

@@ -136,6 +136,8 @@ struct Session {
   DBuf<int32_t> d_cov_diff;       // pass-1 coverage difference arrays (unique, redundant)
   DBuf<int2> d_tile_sum;          // pass-1 coverage scan tile sums
   DBuf<int32_t> d_pos_target, d_win_rb, d_win_re, d_target_group, d_target_scored;
+  DBuf<uint8_t> d_raw_b, d_raw_q;     // the caller's bases and qualities, as uploaded
+  DBuf<int64_t> d_raw_off;
   DBuf<FoldRow> d_fold;
   DBuf<RunRec> d_runrec;
   DBuf<TargetInfo> d_targets;
@@ -572,16 +574,16 @@ static void session_load(Session& S, const bsq_reference* ref, const bsq_alignme
   HIP_OK(hipMemsetAsync(S.d_bases.p, 0, (size_t)dev_bytes + 16, S.stream));
   HIP_OK(hipMemsetAsync(S.d_quals.p, 0, (size_t)dev_bytes + 16, S.stream));
   if (nbases && n_pack > 0) {
-    DBuf<uint8_t> raw_b, raw_q;
-    DBuf<int64_t> d_raw_off;
-    raw_b.alloc(nbases);
-    raw_q.alloc(nbases);
-    d_raw_off.alloc(raw_off.size());
-    upload_staged(S, raw_b.p, a->seq, nbases);
-    upload_staged(S, raw_q.p, a->qual, nbases);
-    HIP_OK(hipMemcpyAsync(d_raw_off.p, raw_off.data(), raw_off.size() * sizeof(int64_t), hipMemcpyHostToDevice, S.stream));
-    hipLaunchKernelGGL(k_repack_reads, dim3(2048), dim3(256), 0, S.stream, S.d_reads.p, d_raw_off.p, (int64_t)n_pack,
-                       raw_b.p, raw_q.p, S.d_bases.p, S.d_quals.p);
+    // the uploaded arrays stay with the session for its next sample (no allocation per load)
+    S.d_raw_b.alloc(nbases);
+    S.d_raw_q.alloc(nbases);
+    S.d_raw_off.alloc(raw_off.size());
+    upload_staged(S, S.d_raw_b.p, a->seq, nbases);
+    upload_staged(S, S.d_raw_q.p, a->qual, nbases);
+    HIP_OK(hipMemcpyAsync(S.d_raw_off.p, raw_off.data(), raw_off.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                          S.stream));
+    hipLaunchKernelGGL(k_repack_reads, dim3(2048), dim3(256), 0, S.stream, S.d_reads.p, S.d_raw_off.p, (int64_t)n_pack,
+                       S.d_raw_b.p, S.d_raw_q.p, S.d_bases.p, S.d_quals.p);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(S.stream));
   }

@@ -116,6 +116,47 @@ def batched_block(n_samples, parallel=8, rate=4.0, max_length=2_000_000, seed=7)
     }
 
 
+def end_to_end_block(B, cfg, dp, ref_length=1_000_000, runs=2):
+    """The path from files: a sample of the bench's shape at ref_length bp written as BAM + FASTA
+    (not timed), then read back (BGZF inflate and record decode on the host cores), loaded into a
+    new session and run through the pipeline to the GD text.  Best of `runs` (the first also warms
+    the file cache)."""
+    import shutil
+    import tempfile
+    c = dict(cfg)
+    c["ref_length"] = ref_length
+    sample = B.Sample.simulate(**c)
+    d = tempfile.mkdtemp(prefix="bsq_e2e_")
+    try:
+        bam, fa = os.path.join(d, "s.bam"), os.path.join(d, "s.fasta")
+        t = time.time()
+        sample.write_bam(bam)
+        sample.write_fasta(fa)
+        log("[bench]   BAM written in %.1f s (%.0f MB)" % (time.time() - t, os.path.getsize(bam) / 1e6))
+        best = None
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            s = B.Sample.read(bam, fa)
+            t1 = time.perf_counter()
+            sess = B.Session().load(s)
+            t2 = time.perf_counter()
+            try:
+                sess.pipeline(mode=B.CONSENSUS, dp=dp)
+            finally:
+                t3 = time.perf_counter()
+                sess.close()
+            r = dict(read_ms=(t1 - t0) * 1e3, load_ms=(t2 - t1) * 1e3, pipeline_ms=(t3 - t2) * 1e3, total_ms=(t3 - t0) * 1e3)
+            if best is None or r["total_ms"] < best["total_ms"]:
+                best = r
+        nab = sample.aligned_bases()
+        best.update(workload="%d bp reference at %.0fx, 2x150 bp paired, as BAM (%.0f MB) + FASTA files; a new "
+                             "session per run" % (ref_length, c["coverage"], os.path.getsize(bam) / 1e6),
+                    aligned_bases=nab, aligned_bases_per_s=nab / (best["total_ms"] / 1e3), runs=runs)
+        return best
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def cpu_baseline(B, cfg, target_seconds):
     """The oracle (single-threaded C++ restatement of the reference path) on a bounded sample of
     the same workload: same generator settings on a shorter reference."""
@@ -159,6 +200,7 @@ def main():
     ap.add_argument("--latency-runs", type=int, default=5,
                     help="submit-to-GD runs of the sample (new session, load, error_count, fit, identify)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched / served block")
+    ap.add_argument("--no-end-to-end", action="store_true", help="skip the BAM-to-GD block")
     ap.add_argument("--batched-samples", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -297,6 +339,11 @@ def main():
     else:
         log("[bench] batched and served samples")
         out["batched"] = batched_block(args.batched_samples)
+    if args.no_end_to_end:
+        out["end_to_end"] = None
+    else:
+        log("[bench] BAM to GD")
+        out["end_to_end"] = end_to_end_block(B, cfg, dp)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle, 1 core)")
         out["cpu_baseline"] = cpu_baseline(B, cfg, args.cpu_seconds)

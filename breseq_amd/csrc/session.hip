@@ -10,6 +10,7 @@
 #include <array>
 #include <charconv>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -208,31 +209,60 @@ struct StagingSet {
   hipEvent_t done[2] = {nullptr, nullptr};
 };
 constexpr size_t kStageBytes = (size_t)32 << 20;
+// Sets per device: more uploads at once than this share the PCIe link without gaining bandwidth,
+// so a further upload waits for a set to come back instead of pinning more memory.
+constexpr int kStageSetsPerDevice = 3;
 static std::mutex g_stage_mu;
+static std::condition_variable g_stage_cv;
 static std::vector<StagingSet*> g_stage_free;
+static std::map<int, int> g_stage_count;   // sets created per device
 
 static StagingSet* take_staging(int device) {
   {
-    std::lock_guard<std::mutex> g(g_stage_mu);
-    for (size_t i = 0; i < g_stage_free.size(); i++)
-      if (g_stage_free[i]->device == device) {
-        StagingSet* s = g_stage_free[i];
-        g_stage_free.erase(g_stage_free.begin() + (std::ptrdiff_t)i);
-        return s;
+    std::unique_lock<std::mutex> g(g_stage_mu);
+    for (;;) {
+      for (size_t i = 0; i < g_stage_free.size(); i++)
+        if (g_stage_free[i]->device == device) {
+          StagingSet* s = g_stage_free[i];
+          g_stage_free.erase(g_stage_free.begin() + (std::ptrdiff_t)i);
+          return s;
+        }
+      if (g_stage_count[device] < kStageSetsPerDevice) {
+        g_stage_count[device]++;
+        break;
       }
+      g_stage_cv.wait(g);
+    }
   }
   StagingSet* s = new StagingSet();
   s->device = device;
-  for (int b = 0; b < 2; b++) {
-    HIP_OK(hipHostMalloc((void**)&s->buf[b], kStageBytes, hipHostMallocDefault));
-    HIP_OK(hipEventCreateWithFlags(&s->done[b], hipEventDisableTiming));
+  try {
+    for (int b = 0; b < 2; b++) {
+      HIP_OK(hipHostMalloc((void**)&s->buf[b], kStageBytes, hipHostMallocDefault));
+      HIP_OK(hipEventCreateWithFlags(&s->done[b], hipEventDisableTiming));
+    }
+  } catch (...) {
+    for (int b = 0; b < 2; b++) {
+      if (s->buf[b]) (void)hipHostFree(s->buf[b]);
+      if (s->done[b]) (void)hipEventDestroy(s->done[b]);
+    }
+    delete s;
+    {
+      std::lock_guard<std::mutex> g(g_stage_mu);
+      g_stage_count[device]--;
+    }
+    g_stage_cv.notify_all();
+    throw;
   }
   return s;
 }
 
 static void put_staging(StagingSet* s) {
-  std::lock_guard<std::mutex> g(g_stage_mu);
-  g_stage_free.push_back(s);
+  {
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    g_stage_free.push_back(s);
+  }
+  g_stage_cv.notify_all();
 }
 
 // dst (device) <- src (pageable host), in 32 MiB chunks through a staging set; the copies are

@@ -4,6 +4,7 @@
 #include <charconv>
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -314,19 +315,87 @@ std::string write_gd(std::vector<GdEntry>& entries) {
 
 namespace {
 
-struct Line {
-  bool is_entry = false;
-  std::string raw;                 // header / comment lines
-  std::string type, id, evidence;
-  std::vector<std::string> spec;   // positional fields
-  std::map<std::string, std::string> kv;
+// The fields of one entry, sorted by name as the std::map of a cGenomeDiff entry keeps them (its
+// writer prints them in that order); a repeated name keeps its last value.  A flat sorted vector:
+// an entry has about twenty fields.
+struct Fields {
+  std::vector<std::pair<std::string, std::string>> kv;
+
+  std::vector<std::pair<std::string, std::string>>::iterator lower(const std::string& k) {
+    return std::lower_bound(kv.begin(), kv.end(), k,
+                            [](const std::pair<std::string, std::string>& a, const std::string& b) { return a.first < b; });
+  }
+  int find(const std::string& k) const {
+    auto it = std::lower_bound(kv.begin(), kv.end(), k,
+                               [](const std::pair<std::string, std::string>& a, const std::string& b) { return a.first < b; });
+    return (it != kv.end() && it->first == k) ? (int)(it - kv.begin()) : -1;
+  }
+  bool has(const std::string& k) const { return find(k) >= 0; }
+  const std::string& get(const std::string& k) const {
+    static const std::string empty;
+    const int i = find(k);
+    return i < 0 ? empty : kv[(size_t)i].second;
+  }
+  std::string& at(const std::string& k) {   // inserted empty when absent, as std::map::operator[]
+    auto it = lower(k);
+    if (it == kv.end() || it->first != k) it = kv.insert(it, std::make_pair(k, std::string()));
+    return it->second;
+  }
+  void erase(const std::string& k) {
+    auto it = lower(k);
+    if (it != kv.end() && it->first == k) kv.erase(it);
+  }
+  // from name=value fields in line order: sorted by name, the last of a repeated name kept
+  void assign(std::vector<std::pair<std::string, std::string>>&& fields) {
+    kv = std::move(fields);
+    std::stable_sort(kv.begin(), kv.end(),
+                     [](const std::pair<std::string, std::string>& a, const std::pair<std::string, std::string>& b) {
+                       return a.first < b.first;
+                     });
+    size_t w = 0;
+    for (size_t r = 0; r < kv.size(); r++) {
+      if (w > 0 && kv[w - 1].first == kv[r].first) kv[w - 1].second = std::move(kv[r].second);
+      else if (w != r) kv[w++] = std::move(kv[r]);
+      else w++;
+    }
+    kv.resize(w);
+  }
 };
 
-double from_string_d(const std::string& s) {   // from_string<double>: 0 when not a number
+bool is_digit(char c) { return c >= '0' && c <= '9'; }
+
+// from_string<double> (common.h:892-899): the value a stream extraction leaves, whether or not it
+// fails; that is 0 when the text holds no number and +-DBL_MAX when it is out of range.  (The
+// reference asserts on an empty string; here it reads as 0.)  A plain decimal number ([+-] digits
+// [. digits] [e [+-] digits]) is read with strtod, which rounds it as the stream does; anything
+// else, and a value out of range, takes the stream itself.
+double from_string_d(const std::string& s) {
+  const char* p = s.c_str();
+  const size_t n = s.size();
+  size_t i = 0, nd = 0;
+  if (i < n && (p[i] == '+' || p[i] == '-')) i++;
+  while (i < n && is_digit(p[i])) i++, nd++;
+  if (i < n && p[i] == '.') {
+    i++;
+    while (i < n && is_digit(p[i])) i++, nd++;
+  }
+  bool plain = nd > 0;
+  if (plain && i < n && (p[i] == 'e' || p[i] == 'E')) {
+    size_t e = i + 1;
+    if (e < n && (p[e] == '+' || p[e] == '-')) e++;
+    const size_t e0 = e;
+    while (e < n && is_digit(p[e])) e++;
+    plain = e > e0;
+    i = e;
+  }
+  if (plain && i == n) {
+    errno = 0;
+    const double v = strtod(p, nullptr);
+    if (errno != ERANGE) return v;
+  }
   std::istringstream is(s);
   double t = 0.0;
   is >> t;
-  if (is.fail()) return 0.0;
   return t;
 }
 
@@ -339,43 +408,53 @@ double double_from_string(const std::string& in) {   // common.h:939-950
   return from_string_d(in);
 }
 
-std::vector<std::string> split(const std::string& s, char d) {
-  std::vector<std::string> v;
-  std::string cur;
-  for (char c : s) {
-    if (c == d) { v.push_back(cur); cur.clear(); }
-    else cur.push_back(c);
+// the two numbers of an "a/b" field (b is 0 when absent)
+void pair_of(const std::string& v, double& a, double& b) {
+  const size_t k = v.find('/');
+  if (k == std::string::npos) {
+    a = from_string_d(v);
+    b = 0.0;
+  } else {
+    a = from_string_d(v.substr(0, k));
+    b = from_string_d(v.substr(k + 1, v.find('/', k + 1) == std::string::npos ? std::string::npos : v.find('/', k + 1) - k - 1));
   }
-  v.push_back(cur);
-  return v;
 }
 
 struct Ra {
-  std::map<std::string, std::string>& m;
-  bool has(const std::string& k) const { return m.count(k) != 0; }
-  std::string get(const std::string& k) const {
-    auto it = m.find(k);
-    return it == m.end() ? std::string() : it->second;
-  }
+  Fields& m;
+  bool has(const std::string& k) const { return m.has(k); }
+  const std::string& get(const std::string& k) const { return m.get(k); }
   void add_reject(const std::string& reason, const std::string& field = "reject") {
-    if (!has(field)) { m[field] = reason; return; }
-    std::vector<std::string> cur = split(m[field], ',');
-    for (auto& c : cur) if (c == reason) return;
-    m[field] += "," + reason;
+    if (!has(field)) {
+      m.at(field) = reason;
+      return;
+    }
+    std::string& cur = m.at(field);
+    size_t b = 0;
+    for (;;) {   // a reason already in the comma-separated list is not added again
+      const size_t e = cur.find(',', b);
+      if (cur.compare(b, (e == std::string::npos ? cur.size() : e) - b, reason) == 0) return;
+      if (e == std::string::npos) break;
+      b = e + 1;
+    }
+    cur += ",";
+    cur += reason;
   }
 };
 
 struct RefAccess {
   const bsq_reference* ref;
   int target(const std::string& seq_id) const {
-    for (int i = 0; ref && i < ref->n_targets; i++) if (seq_id == ref->names[i]) return i;
+    for (int i = 0; ref && i < ref->n_targets; i++)
+      if (seq_id == ref->names[i]) return i;
     return -1;
   }
-  std::string base1(int t, int64_t pos1) const {   // get_sequence_1(pos, pos); "" when outside
-    if (t < 0 || pos1 < 1 || pos1 > ref->lengths[t]) return "";
-    return std::string(1, ref->seqs[t][pos1 - 1]);
-  }
   int64_t length(int t) const { return t < 0 ? 0 : ref->lengths[t]; }
+  // get_sequence_1(pos, pos) == b, where a position outside the target reads as ""
+  bool base_is(int t, int64_t pos1, const std::string& b) const {
+    if (t < 0 || pos1 < 1 || pos1 > ref->lengths[t]) return b.empty();
+    return b.size() == 1 && ref->seqs[t][pos1 - 1] == b[0];
+  }
 };
 
 double score_of(Ra& ra) {
@@ -399,9 +478,11 @@ bool freq_bounds(Ra& ra, double& lo, double& hi) {
     return true;
   }
   if (!ra.has("total_cov") || !ra.has("frequency")) return false;
-  std::vector<std::string> tb = split(ra.get("total_cov"), '/');
-  if (tb.size() < 2) return false;
-  double n = from_string_d(tb[0]) + from_string_d(tb[1]);
+  const std::string& tc = ra.get("total_cov");
+  if (tc.find('/') == std::string::npos) return false;
+  double a, b;
+  pair_of(tc, a, b);
+  double n = a + b;
   if (!(n > 0.0)) return false;
   double k = from_string_d(ra.get("frequency")) * n;
   lo = bd_lower(k, n);
@@ -417,66 +498,59 @@ void bias(Ra& ra, const bsq_ra_test_settings& s) {   // rejected_RA_polymorphism
 }
 
 void poly_coverage(Ra& ra, const bsq_ra_test_settings& s) {   // rejected_RA_polymorphism_coverage
+  double a, b;
   if (s.polymorphism_minimum_variant_coverage_each_strand > 0) {
     bool passed = true;
-    std::string ref_base = ra.has("ref_base") ? ra.get("ref_base") : "";
+    const std::string& ref_base = ra.get("ref_base");
     bool major_var = ra.has("major_base") && ra.get("major_base") != ref_base;
     bool minor_var = ra.has("minor_base") && ra.get("minor_base") != ref_base;
     if (!major_var && !minor_var) major_var = minor_var = true;
     const double c = s.polymorphism_minimum_variant_coverage_each_strand;
     if (major_var && ra.has("major_cov")) {
-      auto tb = split(ra.get("major_cov"), '/');
-      passed = passed && from_string_d(tb[0]) >= c;
-      passed = passed && from_string_d(tb.size() > 1 ? tb[1] : "") >= c;
+      pair_of(ra.get("major_cov"), a, b);
+      passed = passed && a >= c && b >= c;
     }
     if (minor_var && ra.has("minor_cov")) {
-      auto tb = split(ra.get("minor_cov"), '/');
-      passed = passed && from_string_d(tb[0]) >= c;
-      passed = passed && from_string_d(tb.size() > 1 ? tb[1] : "") >= c;
+      pair_of(ra.get("minor_cov"), a, b);
+      passed = passed && a >= c && b >= c;
     }
     if (!passed) ra.add_reject("VARIANT_STRAND_COVERAGE");
   }
   if (s.polymorphism_minimum_total_coverage_each_strand > 0) {
-    auto tb = split(ra.get("total_cov"), '/');
-    double top = from_string_d(tb[0]), bot = from_string_d(tb.size() > 1 ? tb[1] : "");
-    if (!(top >= s.polymorphism_minimum_total_coverage_each_strand && bot >= s.polymorphism_minimum_total_coverage_each_strand))
+    pair_of(ra.get("total_cov"), a, b);
+    if (!(a >= s.polymorphism_minimum_total_coverage_each_strand && b >= s.polymorphism_minimum_total_coverage_each_strand))
       ra.add_reject("TOTAL_STRAND_COVERAGE");
   }
   if (s.polymorphism_minimum_variant_coverage > 0) {
     bool passed = true;
-    auto tb = split(ra.get("major_cov"), '/');
-    passed = passed && (from_string_d(tb[0]) + from_string_d(tb.size() > 1 ? tb[1] : "") >= s.polymorphism_minimum_variant_coverage);
-    tb = split(ra.get("minor_cov"), '/');
-    passed = passed && (from_string_d(tb[0]) + from_string_d(tb.size() > 1 ? tb[1] : "") >= s.polymorphism_minimum_variant_coverage);
+    pair_of(ra.get("major_cov"), a, b);
+    passed = passed && (a + b >= s.polymorphism_minimum_variant_coverage);
+    pair_of(ra.get("minor_cov"), a, b);
+    passed = passed && (a + b >= s.polymorphism_minimum_variant_coverage);
     if (!passed) ra.add_reject("VARIANT_COVERAGE");
   }
   if (s.polymorphism_minimum_total_coverage > 0) {
-    auto tb = split(ra.get("total_cov"), '/');
-    if (from_string_d(tb[0]) + from_string_d(tb.size() > 1 ? tb[1] : "") < s.polymorphism_minimum_total_coverage) ra.add_reject("TOTAL_COVERAGE");
+    pair_of(ra.get("total_cov"), a, b);
+    if (a + b < s.polymorphism_minimum_total_coverage) ra.add_reject("TOTAL_COVERAGE");
   }
 }
 
 void cons_coverage(Ra& ra, const bsq_ra_test_settings& s) {   // rejected_RA_consensus_coverage
-  auto two = [&](const std::string& k, double& a, double& b) {
-    auto tb = split(ra.get(k), '/');
-    a = from_string_d(tb[0]);
-    b = from_string_d(tb.size() > 1 ? tb[1] : "");
-  };
   double a, b;
   if (s.consensus_minimum_variant_coverage_each_strand > 0) {
-    two("major_cov", a, b);
+    pair_of(ra.get("major_cov"), a, b);
     if (a < s.consensus_minimum_variant_coverage_each_strand || b < s.consensus_minimum_variant_coverage_each_strand) ra.add_reject("VARIANT_STRAND_COVERAGE");
   }
   if (s.consensus_minimum_total_coverage_each_strand > 0) {
-    two("total_cov", a, b);
+    pair_of(ra.get("total_cov"), a, b);
     if (a < s.consensus_minimum_total_coverage_each_strand || b < s.consensus_minimum_total_coverage_each_strand) ra.add_reject("TOTAL_STRAND_COVERAGE");
   }
   if (s.consensus_minimum_variant_coverage > 0) {
-    two("major_cov", a, b);
+    pair_of(ra.get("major_cov"), a, b);
     if (a + b < s.consensus_minimum_variant_coverage) ra.add_reject("VARIANT_COVERAGE");
   }
   if (s.consensus_minimum_total_coverage > 0) {
-    two("total_cov", a, b);
+    pair_of(ra.get("total_cov"), a, b);
     if (a + b < s.consensus_minimum_total_coverage) ra.add_reject("TOTAL_COVERAGE");
   }
 }
@@ -484,26 +558,25 @@ void cons_coverage(Ra& ra, const bsq_ra_test_settings& s) {   // rejected_RA_con
 void homopolymer(Ra& ra, const RefAccess& R, int32_t indel_len, int32_t surround_len, bool no_indels) {   // :400-487
   const std::string rb = ra.get("ref_base"), nb = ra.get("new_base");
   if (indel_len && (rb == "." || nb == ".")) {
-    const std::string seq_id = ra.get("seq_id");
-    const int t = R.target(seq_id);
+    const int t = R.target(ra.get("seq_id"));
     int32_t mut_pos = (int32_t)from_string_d(ra.get("position"));
     bool is_ins = from_string_d(ra.get("insert_position")) > 0;
-    std::string mb = (rb == ".") ? nb : rb;
+    const std::string& mb = (rb == ".") ? nb : rb;
     int32_t hl = 0;
     bool no_match = false;
     if (is_ins) {
-      if (R.base1(t, mut_pos) != mb) {
+      if (!R.base_is(t, mut_pos, mb)) {
         mut_pos++;
         if (mut_pos > (int32_t)R.length(t)) no_match = true;
-        else if (R.base1(t, mut_pos) != mb) no_match = true;
+        else if (!R.base_is(t, mut_pos, mb)) no_match = true;
       }
     }
     if (!no_match) {
       int32_t sp = mut_pos - 1;
-      while (R.base1(t, sp) == mb && sp > 0) sp--;
+      while (R.base_is(t, sp, mb) && sp > 0) sp--;
       sp++;
       int32_t ep = mut_pos + 1;
-      while (R.base1(t, ep) == mb && ep <= (int32_t)R.length(t)) ep++;
+      while (R.base_is(t, ep, mb) && ep <= (int32_t)R.length(t)) ep++;
       ep--;
       hl = ep - sp + 1;
     }
@@ -514,13 +587,13 @@ void homopolymer(Ra& ra, const RefAccess& R, int32_t indel_len, int32_t surround
     int32_t mut_pos = (int32_t)from_string_d(ra.get("position"));
     int32_t sp = mut_pos - 1;
     while (sp >= 1) {
-      if (R.base1(t, sp) != nb) break;
+      if (!R.base_is(t, sp, nb)) break;
       sp--;
     }
     sp++;
     int32_t ep = mut_pos + 1;
     while (ep <= (int32_t)R.length(t)) {
-      if (R.base1(t, ep) != nb) break;
+      if (!R.base_is(t, ep, nb)) break;
       ep++;
     }
     ep--;
@@ -539,10 +612,11 @@ bool test_mode(Ra& ra, const RefAccess& R, const bsq_ra_test_settings& s, bool p
   cons_coverage(ra, s);
   homopolymer(ra, R, s.consensus_reject_indel_homopolymer_length, s.consensus_reject_surrounding_homopolymer_length, false);
   if (!ra.has("reject")) {
-    ra.m["prediction"] = "consensus";
+    ra.m.at("prediction") = "consensus";
     return ra.get("ref_base") == ra.get("major_base");
   }
-  ra.m["consensus_reject"] = ra.m["reject"];
+  std::string moved = ra.get("reject");   // a copy: at() may insert and move the fields
+  ra.m.at("consensus_reject") = moved;
   ra.m.erase("reject");
   if (score < s.polymorphism_log10_e_value_cutoff) ra.add_reject("SCORE_CUTOFF");
   if (s.polymorphism_frequency_cutoff > 0.0 && lower < s.polymorphism_frequency_cutoff) ra.add_reject("FREQUENCY_CUTOFF");
@@ -551,14 +625,16 @@ bool test_mode(Ra& ra, const RefAccess& R, const bsq_ra_test_settings& s, bool p
   homopolymer(ra, R, s.polymorphism_reject_indel_homopolymer_length, s.polymorphism_reject_surrounding_homopolymer_length,
               s.polymorphism_no_indels != 0);
   if (!ra.has("reject")) {
-    ra.m["prediction"] = "polymorphism";
+    ra.m.at("prediction") = "polymorphism";
     return false;
   }
-  ra.m["polymorphism_reject"] = ra.m["reject"];
+  moved = ra.get("reject");
+  ra.m.at("polymorphism_reject") = moved;
   ra.m.erase("reject");
   if (!poly_mode) return true;
-  ra.m["prediction"] = "polymorphism";
-  ra.m["reject"] = ra.m["polymorphism_reject"];
+  ra.m.at("prediction") = "polymorphism";
+  moved = ra.get("polymorphism_reject");
+  ra.m.at("reject") = moved;
   ra.m.erase("polymorphism_reject");
   return false;
 }
@@ -611,45 +687,72 @@ double bd_upper(double k, double n) { return (k >= n) ? 1.0 : ibeta_inv(k + 1, n
 
 }  // namespace
 
+// Each line is read once: entries are split at tabs, their name=value fields sorted (the order the
+// reference's writer prints), RA entries classified, and every entry printed back.
 std::string test_ra_evidence_text(const std::string& text, const bsq_reference* ref, const bsq_ra_test_settings& s) {
-  std::vector<Line> lines;
-  std::istringstream in(text);
-  std::string l;
-  while (std::getline(in, l)) {
-    Line L;
-    if (l.empty() || l[0] == '#') {
-      L.raw = l;
-      lines.push_back(L);
-      continue;
-    }
-    std::vector<std::string> f = split(l, '\t');
-    if (f.size() < 3) throw BsqError(BSQ_ERR_FORMAT, "malformed genome diff line: " + l);
-    L.is_entry = true;
-    L.type = f[0];
-    L.id = f[1];
-    L.evidence = f[2];
-    for (size_t i = 3; i < f.size(); i++) {
-      size_t eq = f[i].find('=');
-      if (eq == std::string::npos) L.spec.push_back(f[i]);
-      else L.kv[f[i].substr(0, eq)] = f[i].substr(eq + 1);
-    }
-    lines.push_back(L);
-  }
   RefAccess R{ref};
   std::string out;
-  for (Line& L : lines) {
-    if (!L.is_entry) {
-      out += L.raw + "\n";
+  out.reserve(text.size() + text.size() / 8);
+  std::vector<std::string> spec;
+  std::vector<std::pair<std::string, std::string>> fields;
+  Fields m;
+  // a malformed line anywhere fails the whole text before any entry is tested
+  for (size_t b = 0; b < text.size();) {
+    size_t e = text.find('\n', b);
+    if (e == std::string::npos) e = text.size();
+    const char* l = text.data() + b;
+    const size_t ln = e - b;
+    b = e + 1;
+    if (ln == 0 || l[0] == '#') continue;
+    const void* t1 = memchr(l, '\t', ln);
+    const void* t2 = t1 ? memchr((const char*)t1 + 1, '\t', ln - (size_t)((const char*)t1 + 1 - l)) : nullptr;
+    if (!t2) throw BsqError(BSQ_ERR_FORMAT, "malformed genome diff line: " + std::string(l, ln));
+  }
+  size_t b = 0;
+  while (b < text.size()) {
+    size_t e = text.find('\n', b);
+    if (e == std::string::npos) e = text.size();
+    const char* l = text.data() + b;
+    const size_t ln = e - b;
+    b = e + 1;
+    if (ln == 0 || l[0] == '#') {
+      out.append(l, ln);
+      out += '\n';
       continue;
     }
-    if (L.type == "RA") {
-      if (L.spec.size() < 5) throw BsqError(BSQ_ERR_FORMAT, "RA entry without its 5 positional fields");
-      std::map<std::string, std::string> m = L.kv;
-      m["seq_id"] = L.spec[0];
-      m["position"] = L.spec[1];
-      m["insert_position"] = L.spec[2];
-      m["ref_base"] = L.spec[3];
-      m["new_base"] = L.spec[4];
+    // split at tabs, keeping empty fields
+    spec.clear();
+    fields.clear();
+    std::string type, id, evidence;
+    int nf = 0;
+    size_t f0 = 0;
+    for (size_t k = 0; k <= ln; k++) {
+      if (k < ln && l[k] != '\t') continue;
+      const char* f = l + f0;
+      const size_t fn = k - f0;
+      if (nf == 0) type.assign(f, fn);
+      else if (nf == 1) id.assign(f, fn);
+      else if (nf == 2) evidence.assign(f, fn);
+      else {
+        const void* eq = memchr(f, '=', fn);
+        if (!eq) spec.emplace_back(f, fn);
+        else {
+          const size_t kl = (size_t)((const char*)eq - f);
+          fields.emplace_back(std::string(f, kl), std::string(f + kl + 1, fn - kl - 1));
+        }
+      }
+      nf++;
+      f0 = k + 1;
+    }
+    m.assign(std::move(fields));
+    fields = std::vector<std::pair<std::string, std::string>>();
+    if (type == "RA") {
+      if (spec.size() < 5) throw BsqError(BSQ_ERR_FORMAT, "RA entry without its 5 positional fields");
+      m.at("seq_id") = spec[0];
+      m.at("position") = spec[1];
+      m.at("insert_position") = spec[2];
+      m.at("ref_base") = spec[3];
+      m.at("new_base") = spec[4];
       Ra ra{m};
       if (!ra.has("score") && !ra.has("consensus_score") && !ra.has("polymorphism_score"))
         throw BsqError(BSQ_ERR_FORMAT, "Expected field 'score' in evidence item");
@@ -658,15 +761,24 @@ std::string test_ra_evidence_text(const std::string& text, const bsq_reference* 
       if (ra.has("user_defined")) del = false;
       if (del) continue;
       for (const char* k : {"seq_id", "position", "insert_position", "ref_base", "new_base"}) m.erase(k);
-      L.kv = m;
     }
-    out += L.type + "\t" + L.id + "\t" + L.evidence;
-    for (const auto& sp : L.spec) out += "\t" + sp;
-    for (const auto& kv : L.kv) {
+    out += type;
+    out += '\t';
+    out += id;
+    out += '\t';
+    out += evidence;
+    for (const auto& sp : spec) {
+      out += '\t';
+      out += sp;
+    }
+    for (const auto& kv : m.kv) {
       if (kv.second.empty() || kv.first[0] == '_') continue;
-      out += "\t" + kv.first + "=" + kv.second;
+      out += '\t';
+      out += kv.first;
+      out += '=';
+      out += kv.second;
     }
-    out += "\n";
+    out += '\n';
   }
   return out;
 }

@@ -202,6 +202,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched / served block")
     ap.add_argument("--no-end-to-end", action="store_true", help="skip the BAM-to-GD block")
     ap.add_argument("--batched-samples", type=int, default=16)
+    ap.add_argument("--no-tune", action="store_true", help="keep the column pass's default launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (the per-sample work "
@@ -222,9 +223,24 @@ def main():
     nab, n_reads, n_cols = tm["aligned_bases"], tm["n_reads"], tm["n_positions"]
     log("[bench] loaded: %d aligned bases, %d reads, %d columns; pack %.0f ms, H2D %.0f ms"
         % (nab, n_reads, n_cols, tm["pack_ms"], tm["h2d_ms"]))
+    warm = None
     for i in range(args.warmup):
-        run_step(sess, B.CONSENSUS, dp)
+        warm = run_step(sess, B.CONSENSUS, dp)
         log("[bench] warmup step %d done" % i)
+    # the column pass's launch, chosen by the library's autotuner on this sample (untimed)
+    column_launch = None
+    if not args.no_tune:
+        if warm is None:
+            warm = run_step(sess, B.CONSENSUS, dp)
+        t = time.perf_counter()
+        column_launch = sess.tune_columns(warm[0].rates_table, warm[0].read_set_max, warm[0].quality_max,
+                                          [x["deletion_coverage_propagation_cutoff"] for x in warm[1]],
+                                          repeats=3, mode=B.CONSENSUS, dp=dp)
+        column_launch["tune_ms"] = (time.perf_counter() - t) * 1000.0
+        log("[bench] column launch: %d blocks, table in %s, %.3f ms (default %.3f ms), %d launches timed in %.0f ms"
+            % (column_launch["grid"], "LDS" if column_launch["table_in_lds"] else "global memory", column_launch["ms"],
+               column_launch["default_ms"], column_launch["n_tried"], column_launch["tune_ms"]))
+        run_step(sess, B.CONSENSUS, dp)
     step_ms = []
     ec_k, im_k = [], []
     host = {"ec_call": [], "im_call": [], "im_host_pre": [], "im_d2h": [], "im_host_post": [], "fit": [],
@@ -327,6 +343,7 @@ def main():
     }
     # per-position error of the FP32 column sums against the FP64 sums breseq forms, over every
     # column with reads (outside the timed steps)
+    out["column_launch"] = column_launch
     if args.no_audit:
         out["fp32_vs_fp64"] = None
     else:

@@ -164,6 +164,9 @@ struct Session {
   DBuf<uint8_t> d_rg;                         // @RG index of each read (255: none)
   DBuf<int32_t> d_ins;
   DBuf<RaOut> d_ra;
+  // launch of the column pass (bsq_session_set_column_launch, bsq_session_tune_columns)
+  int32_t col_blocks = 0;   // 0: one window per wavefront, at most 2048 blocks
+  int32_t col_lds = 1;      // 1: the likelihood table staged in LDS when it fits; 0: read from global memory
   // device writer of the genome diff (gd_emit.hip)
   bool gd_on_device = true;
   int gd_cap = 4096;                       // entries the writer's lists hold (grows when a pass needs more)
@@ -1020,6 +1023,16 @@ struct GdQueued {
   size_t prefix = 0;    // text bytes copied back with the counters
 };
 
+// Blocks of the column pass: the session's setting, or the default of one window per wavefront
+// capped at 2048 blocks; never more than one window per wavefront, and a multiple of the 8 XCDs
+// from 8 blocks up (the windows of one block then sit on one XCD's L2).
+static int64_t column_blocks(const Session& S) {
+  const int64_t full = std::max<int64_t>(1, (S.n_windows + kColWaves - 1) / kColWaves);
+  int64_t g = std::min<int64_t>(full, S.col_blocks > 0 ? (int64_t)S.col_blocks : 2048);
+  if (g >= 8) g &= ~(int64_t)7;
+  return g;
+}
+
 static GdQueued queue_gd_on_device(Session& S, const GdInputs& in) {
   if (in.places > (uint32_t)dec::kMaxPrecision) throw BsqError(BSQ_ERR_ARG, "polymorphism_precision_places above 60");
   const int T = S.n_targets;
@@ -1262,7 +1275,7 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   P.fold = S.d_fold.p;
   P.fold_rows = (int32_t)fold.size() - 1;
   const size_t fold_bytes = fold.size() * sizeof(FoldRow);
-  P.fold_lds = (fold_bytes <= kFoldLdsBytes) ? 1 : 0;
+  P.fold_lds = (fold_bytes <= kFoldLdsBytes && S.col_lds) ? 1 : 0;
   P.log10_ref_length = log10(total_len);
   P.mutation_cutoff = a->mutation_cutoff;
   P.polymorphism_cutoff = a->polymorphism_cutoff;
@@ -1320,8 +1333,7 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   const double th1 = now_ms();
   // the whole pass is queued without a host round trip: later kernels read the counts on device
   {
-    int64_t g = std::min<int64_t>(std::max<int64_t>(1, (S.n_windows + kColWaves - 1) / kColWaves), 2048);
-    if (g >= 8) g &= ~(int64_t)7;   // a multiple of the 8 XCDs
+    const int64_t g = column_blocks(S);
     if (P.fold_lds)
       hipLaunchKernelGGL(k_identify_columns_t<true>, dim3((int)g), dim3(kColWaves * 64), fold_bytes, S.stream, P);
     else
@@ -2152,6 +2164,139 @@ int bsq_session_set_gd_capacity(bsq_session* s, int32_t entries, bsq_status* sta
   s->s.gd_cap = entries;
   set_status(status, BSQ_OK, "");
   return BSQ_OK;
+}
+
+int bsq_session_set_column_launch(bsq_session* s, int32_t blocks, int32_t table_in_lds, bsq_status* status) {
+  if (!s || blocks < 0 || (table_in_lds != 0 && table_in_lds != 1)) {
+    set_status(status, BSQ_ERR_ARG, "null session, negative blocks, or table_in_lds not 0 or 1");
+    return BSQ_ERR_ARG;
+  }
+  s->s.col_blocks = blocks;
+  s->s.col_lds = table_in_lds;
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Launches chosen by bsq_session_tune_columns, shared by the sessions of the process: a sample
+// of the same shape on the same device reuses the choice.  The shape is the mean aligned length
+// per alignment (25-base steps), the mean depth (half octaves) and the likelihood-table size
+// (read files x qualities), which between them set the work per window and the LDS footprint.
+struct ColTuneKey {
+  int device, read_len, depth, rows;
+  bool operator<(const ColTuneKey& o) const {
+    if (device != o.device) return device < o.device;
+    if (read_len != o.read_len) return read_len < o.read_len;
+    if (depth != o.depth) return depth < o.depth;
+    return rows < o.rows;
+  }
+};
+std::mutex g_col_tune_mu;
+std::map<ColTuneKey, bsq_column_launch> g_col_tune;
+}  // namespace
+
+extern "C" {
+
+int bsq_session_tune_columns(bsq_session* s, const bsq_identify_args* args, int32_t repeats, bsq_column_launch* out,
+                             bsq_status* status) {
+  BSQ_TRY
+  if (!s || !args || !out) throw BsqError(BSQ_ERR_ARG, "null argument");
+  if (repeats < 1 || repeats > 100) throw BsqError(BSQ_ERR_ARG, "repeats outside 1..100");
+  Session& S = s->s;
+  if (!S.loaded) throw BsqError(BSQ_ERR_ARG, "no sample loaded");
+  HIP_OK(hipSetDevice(S.device));
+  const double per_read = (double)S.aligned_bases / (double)std::max<int64_t>(1, S.n_dreads);
+  const double depth = (double)S.aligned_bases / (double)std::max<int32_t>(1, S.n_pos);
+  const ColTuneKey key{S.device, (int)std::lround(per_read / 25.0), (int)std::floor(2.0 * std::log2(std::max(1.0, depth))),
+                       args->read_set_max * args->quality_max};
+  {
+    std::lock_guard<std::mutex> lk(g_col_tune_mu);
+    auto it = g_col_tune.find(key);
+    if (it != g_col_tune.end()) {
+      *out = it->second;
+      out->from_cache = 1;
+      S.col_blocks = out->blocks;
+      S.col_lds = out->table_in_lds;
+      set_status(status, BSQ_OK, "");
+      return BSQ_OK;
+    }
+  }
+  const int32_t keep_blocks = S.col_blocks, keep_lds = S.col_lds;
+  bsq_column_launch best{};
+  try {
+    // the best of `repeats` identify passes, by the column pass's own HIP-event time
+    auto time_launch = [&](int32_t blocks, int32_t lds) {
+      S.col_blocks = blocks;
+      S.col_lds = lds;
+      double t = INFINITY;
+      for (int r = 0; r < repeats; r++) {
+        bsq_im_result res;
+        run_identify(S, args, res, 0);
+        t = std::min(t, S.timing.kernel_detail_ms[1]);
+      }
+      return t;
+    };
+    {
+      bsq_im_result warm;   // first pass of the sample: uploads and table builds
+      S.col_blocks = 0;
+      S.col_lds = 1;
+      run_identify(S, args, warm, 0);
+    }
+    const bool fits = S.last_P.fold_lds != 0;
+    const double default_ms = time_launch(0, 1);
+    best.blocks = 0;
+    best.table_in_lds = 1;
+    const int64_t g_default = column_blocks(S);
+    best.grid = (int32_t)g_default;
+    best.ms = default_ms;
+    best.default_ms = default_ms;
+    best.n_tried = 1;
+    // Candidate grids: fixed sizes, and the grids that give every wavefront of a block the same
+    // number k of windows (the last block aside), which leaves no wavefront idle at a block's end.
+    std::vector<int64_t> grids;
+    auto add_grid = [&](int64_t b) {
+      S.col_blocks = (int32_t)std::min<int64_t>(std::max<int64_t>(b, 1), INT32_MAX);
+      const int64_t g = column_blocks(S);
+      if (g != g_default && std::find(grids.begin(), grids.end(), g) == grids.end()) grids.push_back(g);
+    };
+    for (int64_t b : {(int64_t)256, (int64_t)512, (int64_t)1024, (int64_t)2048}) add_grid(b);
+    for (int64_t k : {1, 2, 3, 4, 5, 6, 8, 12, 16}) {
+      const int64_t g = (S.n_windows + kColWaves * k - 1) / (kColWaves * k);
+      add_grid((g + 7) / 8 * 8);
+    }
+    // another launch replaces the default only when it is at least 2% faster
+    auto consider = [&](int64_t g, int32_t lds) {
+      const double t = time_launch((int32_t)g, lds);
+      best.n_tried++;
+      const bool is_default = best.blocks == 0 && best.table_in_lds == 1;
+      if (t < (is_default ? 0.98 * best.ms : best.ms)) {
+        best.blocks = (int32_t)g;
+        best.table_in_lds = lds;
+        best.grid = (int32_t)g;
+        best.ms = t;
+      }
+    };
+    for (int64_t g : grids) consider(g, fits ? 1 : 0);
+    // the table in global memory at the best grid (the same launch when the table does not fit)
+    if (fits) consider(best.grid, 0);
+  } catch (...) {
+    S.col_blocks = keep_blocks;
+    S.col_lds = keep_lds;
+    throw;
+  }
+  S.col_blocks = best.blocks;
+  S.col_lds = best.table_in_lds;
+  best.from_cache = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_col_tune_mu);
+    g_col_tune[key] = best;
+  }
+  *out = best;
+  set_status(status, BSQ_OK, "");
+  return BSQ_OK;
+  BSQ_CATCH(status)
 }
 
 int bsq_session_timing(const bsq_session* s, bsq_timing* out) {

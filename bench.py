@@ -237,8 +237,10 @@ def main():
                                           [x["deletion_coverage_propagation_cutoff"] for x in warm[1]],
                                           repeats=3, mode=B.CONSENSUS, dp=dp)
         column_launch["tune_ms"] = (time.perf_counter() - t) * 1000.0
-        log("[bench] column launch: %d blocks, table in %s, %.3f ms (default %.3f ms), %d launches timed in %.0f ms"
-            % (column_launch["grid"], "LDS" if column_launch["table_in_lds"] else "global memory", column_launch["ms"],
+        log("[bench] column launch: %d blocks of %d wavefronts, table in %s, %.3f ms (default %.3f ms), %d launches "
+            "timed in %.0f ms"
+            % (column_launch["grid"], column_launch["waves"], "LDS" if column_launch["table_in_lds"] else "global memory",
+               column_launch["ms"],
                column_launch["default_ms"], column_launch["n_tried"], column_launch["tune_ms"]))
         run_step(sess, B.CONSENSUS, dp)
     step_ms = []
@@ -295,7 +297,7 @@ def main():
     s2 = B.Session()
     if column_launch is not None:
         # a server reuses the launch tuned for a sample of this shape
-        s2.set_column_launch(column_launch["blocks"], column_launch["table_in_lds"])
+        s2.set_column_launch(column_launch["blocks"], column_launch["table_in_lds"], column_launch["waves"])
     s2.load(sample)
     run_step(s2, B.CONSENSUS, dp)
     cold_ms = (time.perf_counter() - t0) * 1000.0

@@ -162,7 +162,7 @@ class ColumnLaunch(C.Structure):
         ("grid", C.c_int32),
         ("n_tried", C.c_int32),
         ("from_cache", C.c_int32),
-        ("pad", C.c_int32),
+        ("waves", C.c_int32),
         ("ms", C.c_double),
         ("default_ms", C.c_double),
     ]
@@ -330,7 +330,7 @@ def lib():
     _decl(L, "bsq_session_alignment_trims", C.c_int, [vp, c_i32p, c_i32p, C.c_int64, st])
     _decl(L, "bsq_session_set_gd_writer", C.c_int, [vp, C.c_int32, st])
     _decl(L, "bsq_session_set_gd_capacity", C.c_int, [vp, C.c_int32, st])
-    _decl(L, "bsq_session_set_column_launch", C.c_int, [vp, C.c_int32, C.c_int32, st])
+    _decl(L, "bsq_session_set_column_launch", C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, st])
     _decl(L, "bsq_session_tune_columns", C.c_int, [vp, P(IdentifyArgs), C.c_int32, P(ColumnLaunch), st])
     _decl(L, "bsq_format_doubles_device", C.c_int, [c_f64p, C.c_int64, C.c_int32, C.c_int32, C.c_char_p, c_i32p, st])
     _decl(L, "bsq_session_destroy", None, [vp])

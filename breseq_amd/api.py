@@ -284,11 +284,12 @@ class Session:
         L.check(L.lib().bsq_session_set_gd_capacity(self._h, int(entries), C.byref(st)), st)
         return self
 
-    def set_column_launch(self, blocks=0, table_in_lds=True):
-        """Launch shape of later column passes: blocks of 8 wavefronts (0: the default rule) and
-        the likelihood table in LDS when it fits, or in global memory."""
+    def set_column_launch(self, blocks=0, table_in_lds=True, waves=0):
+        """Launch shape of later column passes: blocks (0: the default rule) of `waves` wavefronts
+        (4 or 8; 0: 8), and the likelihood table in LDS when it fits, or in global memory."""
         st = _status()
-        L.check(L.lib().bsq_session_set_column_launch(self._h, int(blocks), 1 if table_in_lds else 0, C.byref(st)), st)
+        L.check(L.lib().bsq_session_set_column_launch(self._h, int(blocks), 1 if table_in_lds else 0, int(waves),
+                                                      C.byref(st)), st)
         return self
 
     def tune_columns(self, log10_prob, read_set_max, quality_max, propagation_cutoffs, repeats=3, **kw):
@@ -299,7 +300,7 @@ class Session:
         out = L.ColumnLaunch()
         st = _status()
         L.check(L.lib().bsq_session_tune_columns(self._h, C.byref(args), int(repeats), C.byref(out), C.byref(st)), st)
-        return {k: getattr(out, k) for k, _ in L.ColumnLaunch._fields_ if k != "pad"}
+        return {k: getattr(out, k) for k, _ in L.ColumnLaunch._fields_}
 
     def alignment_trims(self):
         """XL / XR of every alignment of the loaded sample, in its order (get_alignment_trims,

@@ -412,8 +412,8 @@ __global__ void __launch_bounds__(64) k_audit_columns_t(ImParams P, const int32_
   if (err) atomicOr(P.err, err);
 }
 
-template <bool kLds>
-__global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams P) {
+template <bool kLds, int kWaves>
+__global__ void __launch_bounds__(kWaves * 64) k_identify_columns_t(ImParams P) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Table<kLds> tab;
   if constexpr (kLds) {
@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(kColWaves * 64) k_identify_columns_t(ImParams 
   // counted qualities: qcut <= q < Q, i.e. (q - qcut) < qspan as unsigned
   const uint32_t qspan = (Q > qcut) ? (uint32_t)(Q - qcut) : 0u;
 
-  for (int64_t w = w_begin + wave; w < w_end; w += kColWaves) {
+  for (int64_t w = w_begin + wave; w < w_end; w += kWaves) {
     const int p = (int)(w * kWaveCols + lane);
     const int t = (p < P.n_pos) ? P.pos_target[p] : -1;
     const bool lane_ok = (t >= 0) && P.targets[t].scored && (P.targets[t].prop_cutoff >= 0.0);
@@ -721,7 +721,9 @@ template __global__ void k_fix_columns_t<false>(ImParams P);
 template __global__ void k_fix_columns_t<true>(ImParams P);
 template __global__ void k_audit_columns_t<false>(ImParams P, const int32_t*, int32_t, double*, uint32_t*);
 template __global__ void k_audit_columns_t<true>(ImParams P, const int32_t*, int32_t, double*, uint32_t*);
-template __global__ void k_identify_columns_t<true>(ImParams P);
-template __global__ void k_identify_columns_t<false>(ImParams P);
+template __global__ void k_identify_columns_t<true, 8>(ImParams P);
+template __global__ void k_identify_columns_t<false, 8>(ImParams P);
+template __global__ void k_identify_columns_t<true, 4>(ImParams P);
+template __global__ void k_identify_columns_t<false, 4>(ImParams P);
 
 }  // namespace bsq

@@ -163,9 +163,9 @@ __global__ void k_ec_reads(EcParams P);
 __global__ void k_cov_tile_sums(CovParams C);
 __global__ void k_cov_tile_scan(CovParams C);
 __global__ void k_cov_hist(CovParams C);
-constexpr int kColWaves = 8;                 // wavefronts per block of the column kernel
+constexpr int kColWaves = 8;                 // wavefronts per block of the column kernel (default; 4 also built)
 constexpr size_t kFoldLdsBytes = 48 * 1024;  // largest likelihood table staged in LDS (1535 rows)
-template <bool kLds>
+template <bool kLds, int kWaves>
 __global__ void k_identify_columns_t(ImParams P);
 __global__ void k_prep_reads(const DRead* reads, int64_t n, int32_t S, int32_t Q, IdRec* out);
 template <bool kKeysGlobal>

@@ -168,6 +168,7 @@ struct Session {
   // launch of the column pass (bsq_session_set_column_launch, bsq_session_tune_columns)
   int32_t col_blocks = 0;   // 0: one window per wavefront, at most 2048 blocks
   int32_t col_lds = 1;      // 1: the likelihood table staged in LDS when it fits; 0: read from global memory
+  int32_t col_waves = kColWaves;   // wavefronts per block: 8 or 4
   // device writer of the genome diff (gd_emit.hip)
   bool gd_on_device = true;
   int gd_cap = 4096;                       // entries the writer's lists hold (grows when a pass needs more)
@@ -1029,7 +1030,7 @@ struct GdQueued {
 // capped at 2048 blocks; never more than one window per wavefront, and a multiple of the 8 XCDs
 // from 8 blocks up (the windows of one block then sit on one XCD's L2).
 static int64_t column_blocks(const Session& S) {
-  const int64_t full = std::max<int64_t>(1, (S.n_windows + kColWaves - 1) / kColWaves);
+  const int64_t full = std::max<int64_t>(1, (S.n_windows + S.col_waves - 1) / S.col_waves);
   int64_t g = std::min<int64_t>(full, S.col_blocks > 0 ? (int64_t)S.col_blocks : 2048);
   if (g >= 8) g &= ~(int64_t)7;
   return g;
@@ -1337,10 +1338,14 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   {
     const int64_t g = column_blocks(S);
     HIP_OK(hipEventRecord(S.evc, S.stream));
-    if (P.fold_lds)
-      hipLaunchKernelGGL(k_identify_columns_t<true>, dim3((int)g), dim3(kColWaves * 64), fold_bytes, S.stream, P);
-    else
-      hipLaunchKernelGGL(k_identify_columns_t<false>, dim3((int)g), dim3(kColWaves * 64), 0, S.stream, P);
+    const dim3 blk(S.col_waves * 64);
+    if (S.col_waves == 4) {
+      if (P.fold_lds) hipLaunchKernelGGL((k_identify_columns_t<true, 4>), dim3((int)g), blk, fold_bytes, S.stream, P);
+      else hipLaunchKernelGGL((k_identify_columns_t<false, 4>), dim3((int)g), blk, 0, S.stream, P);
+    } else {
+      if (P.fold_lds) hipLaunchKernelGGL((k_identify_columns_t<true, 8>), dim3((int)g), blk, fold_bytes, S.stream, P);
+      else hipLaunchKernelGGL((k_identify_columns_t<false, 8>), dim3((int)g), blk, 0, S.stream, P);
+    }
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(S.evk[0], S.stream));
@@ -2170,13 +2175,15 @@ int bsq_session_set_gd_capacity(bsq_session* s, int32_t entries, bsq_status* sta
   return BSQ_OK;
 }
 
-int bsq_session_set_column_launch(bsq_session* s, int32_t blocks, int32_t table_in_lds, bsq_status* status) {
-  if (!s || blocks < 0 || (table_in_lds != 0 && table_in_lds != 1)) {
-    set_status(status, BSQ_ERR_ARG, "null session, negative blocks, or table_in_lds not 0 or 1");
+int bsq_session_set_column_launch(bsq_session* s, int32_t blocks, int32_t table_in_lds, int32_t waves,
+                                  bsq_status* status) {
+  if (!s || blocks < 0 || (table_in_lds != 0 && table_in_lds != 1) || (waves != 0 && waves != 4 && waves != 8)) {
+    set_status(status, BSQ_ERR_ARG, "null session, negative blocks, table_in_lds not 0 or 1, or waves not 0, 4 or 8");
     return BSQ_ERR_ARG;
   }
   s->s.col_blocks = blocks;
   s->s.col_lds = table_in_lds;
+  s->s.col_waves = waves ? waves : kColWaves;
   set_status(status, BSQ_OK, "");
   return BSQ_OK;
 }
@@ -2223,17 +2230,19 @@ int bsq_session_tune_columns(bsq_session* s, const bsq_identify_args* args, int3
       out->from_cache = 1;
       S.col_blocks = out->blocks;
       S.col_lds = out->table_in_lds;
+      S.col_waves = out->waves;
       set_status(status, BSQ_OK, "");
       return BSQ_OK;
     }
   }
-  const int32_t keep_blocks = S.col_blocks, keep_lds = S.col_lds;
+  const int32_t keep_blocks = S.col_blocks, keep_lds = S.col_lds, keep_waves = S.col_waves;
   bsq_column_launch best{};
   try {
     // the best of `repeats` identify passes, by the column pass's own HIP-event time
-    auto time_launch = [&](int32_t blocks, int32_t lds) {
+    auto time_launch = [&](int32_t blocks, int32_t lds, int32_t waves) {
       S.col_blocks = blocks;
       S.col_lds = lds;
+      S.col_waves = waves;
       double t = INFINITY;
       for (int r = 0; r < repeats; r++) {
         bsq_im_result res;
@@ -2246,52 +2255,62 @@ int bsq_session_tune_columns(bsq_session* s, const bsq_identify_args* args, int3
       bsq_im_result warm;   // first pass of the sample: uploads and table builds
       S.col_blocks = 0;
       S.col_lds = 1;
+      S.col_waves = kColWaves;
       run_identify(S, args, warm, 0);
     }
     const bool fits = S.last_P.fold_lds != 0;
-    const double default_ms = time_launch(0, 1);
+    const double default_ms = time_launch(0, 1, kColWaves);
     best.blocks = 0;
     best.table_in_lds = 1;
+    best.waves = kColWaves;
     const int64_t g_default = column_blocks(S);
     best.grid = (int32_t)g_default;
     best.ms = default_ms;
     best.default_ms = default_ms;
     best.n_tried = 1;
-    // Candidate grids: fixed sizes, and the grids that give every wavefront of a block the same
-    // number k of windows (the last block aside), which leaves no wavefront idle at a block's end.
-    std::vector<int64_t> grids;
-    auto add_grid = [&](int64_t b) {
-      S.col_blocks = (int32_t)std::min<int64_t>(std::max<int64_t>(b, 1), INT32_MAX);
-      const int64_t g = column_blocks(S);
-      if (g != g_default && std::find(grids.begin(), grids.end(), g) == grids.end()) grids.push_back(g);
-    };
-    for (int64_t b : {(int64_t)256, (int64_t)512, (int64_t)1024, (int64_t)2048}) add_grid(b);
-    for (int64_t k : {1, 2, 3, 4, 5, 6, 8, 12, 16}) {
-      const int64_t g = (S.n_windows + kColWaves * k - 1) / (kColWaves * k);
-      add_grid((g + 7) / 8 * 8);
-    }
     // another launch replaces the default only when it is at least 2% faster
-    auto consider = [&](int64_t g, int32_t lds) {
-      const double t = time_launch((int32_t)g, lds);
+    auto consider = [&](int64_t g, int32_t lds, int32_t waves) {
+      const double t = time_launch((int32_t)g, lds, waves);
       best.n_tried++;
-      const bool is_default = best.blocks == 0 && best.table_in_lds == 1;
+      const bool is_default = best.blocks == 0 && best.table_in_lds == 1 && best.waves == kColWaves;
       if (t < (is_default ? 0.98 * best.ms : best.ms)) {
         best.blocks = (int32_t)g;
         best.table_in_lds = lds;
+        best.waves = waves;
         best.grid = (int32_t)g;
         best.ms = t;
       }
     };
-    for (int64_t g : grids) consider(g, fits ? 1 : 0);
-    // the table in global memory at the best grid (the same launch when the table does not fit)
-    if (fits) consider(best.grid, 0);
+    // Candidate grids for blocks of 8 and of 4 wavefronts: fixed sizes, and the grids that give
+    // every wavefront of a block the same number k of windows (the last block aside), which leaves
+    // no wavefront idle at a block's end.
+    for (int32_t waves : {8, 4}) {
+      S.col_waves = waves;
+      std::vector<int64_t> grids;
+      auto add_grid = [&](int64_t b) {
+        S.col_blocks = (int32_t)std::min<int64_t>(std::max<int64_t>(b, 1), INT32_MAX);
+        const int64_t g = column_blocks(S);
+        if (!(waves == kColWaves && g == g_default) && std::find(grids.begin(), grids.end(), g) == grids.end())
+          grids.push_back(g);
+      };
+      for (int64_t b : {(int64_t)256, (int64_t)512, (int64_t)1024, (int64_t)2048}) add_grid(b);
+      for (int64_t k : {1, 2, 3, 4, 5, 6, 8, 12, 16}) {
+        const int64_t g = (S.n_windows + waves * k - 1) / (waves * k);
+        add_grid((g + 7) / 8 * 8);
+      }
+      for (int64_t g : grids) consider(g, fits ? 1 : 0, waves);
+    }
+    // the table in global memory at the best launch (the same launch when the table does not fit)
+    if (fits) consider(best.grid, 0, best.waves);
   } catch (...) {
     S.col_blocks = keep_blocks;
     S.col_lds = keep_lds;
+    S.col_waves = keep_waves;
     throw;
   }
   S.col_blocks = best.blocks;
   S.col_lds = best.table_in_lds;
+  S.col_waves = best.waves;
   best.from_cache = 0;
   {
     std::lock_guard<std::mutex> lk(g_col_tune_mu);

@@ -417,11 +417,12 @@ int bsq_session_precision_audit(bsq_session* s, const bsq_identify_args* args, b
 int bsq_session_timing(const bsq_session* s, bsq_timing* out);
 
 /* Launch shape of the column pass of later identify passes (no counterpart in breseq, whose pileup
- * has no launch): `blocks` blocks of 8 wavefronts (0, the default: one 64-column window per
- * wavefront, at most 2048 blocks; never more than one window per wavefront, rounded down to a
- * multiple of 8 from 8 up), and the likelihood table staged in LDS when it fits (table_in_lds 1,
- * the default) or read from global memory (0).  Results do not depend on it. */
-int bsq_session_set_column_launch(bsq_session* s, int32_t blocks, int32_t table_in_lds, bsq_status* status);
+ * has no launch): `blocks` blocks (0, the default: one 64-column window per wavefront, at most 2048
+ * blocks; never more than one window per wavefront, rounded down to a multiple of 8 from 8 up) of
+ * `waves` wavefronts (4 or 8; 0 is the default, 8), and the likelihood table staged in LDS when it
+ * fits (table_in_lds 1, the default) or read from global memory (0).  Results do not depend on it. */
+int bsq_session_set_column_launch(bsq_session* s, int32_t blocks, int32_t table_in_lds, int32_t waves,
+                                  bsq_status* status);
 
 typedef struct bsq_column_launch {
   int32_t blocks;         /* the setting chosen (0: the default rule)                        */
@@ -429,7 +430,7 @@ typedef struct bsq_column_launch {
   int32_t grid;           /* blocks the chosen launch runs                                   */
   int32_t n_tried;        /* launches timed                                                  */
   int32_t from_cache;     /* 1 when an earlier tuning of the same sample shape was reused    */
-  int32_t pad;
+  int32_t waves;          /* wavefronts per block of the chosen launch                       */
   double ms;              /* column pass time of the chosen launch (best of the repeats)     */
   double default_ms;      /* column pass time of the default launch                          */
 } bsq_column_launch;
@@ -437,8 +438,9 @@ typedef struct bsq_column_launch {
 /* Autotuner of the column pass (no counterpart in breseq): runs identify on the loaded sample with
  * `args` under candidate launches, `repeats` times each, keeps the fastest by the pass's own
  * HIP-event time (the default unless another is at least 2% faster) and sets it on the session.
- * The candidates are grids of 256-2048 blocks and the grids that give every wavefront of a block k
- * windows (k = 1-16), with the table in LDS, then the table in global memory at the best grid.
+ * The candidates are blocks of 8 and of 4 wavefronts, in grids of 256-2048 blocks and the grids
+ * that give every wavefront of a block k windows (k = 1-16), with the table in LDS, then the table
+ * in global memory at the best of those.
  * The choice is cached for the process by device, mean aligned length per alignment, mean depth
  * and table size; a later sample of the same shape reuses it without timing anything. */
 int bsq_session_tune_columns(bsq_session* s, const bsq_identify_args* args, int32_t repeats, bsq_column_launch* out,

@@ -2,8 +2,8 @@
 (bsq_session_tune_columns).
 
 * The genome diff does not depend on the launch: one block for the whole sample, grids that are
-  and are not multiples of the 8 XCDs, more blocks than windows, and the likelihood table in LDS
-  and in global memory all print the same text as the default launch.
+  and are not multiples of the 8 XCDs, more blocks than windows, blocks of 4 and of 8 wavefronts,
+  and the likelihood table in LDS and in global memory all print the same text as the default.
 * The tuner times at least the default and one other launch, keeps one no slower than the
   default, and a second session holding a sample of the same shape reuses the choice untimed.
 """
@@ -37,12 +37,13 @@ def test_genome_diff_does_not_depend_on_launch(name, mode):
     try:
         ref = B.api.identify_mutations(s, table, S, Q, prop, session=sess, mode=mode)
         assert ref.count("\nRA\t") > 5
-        for blocks, lds in [(1, True), (1, False), (8, True), (8, False), (13, True), (24, False), (100000, True),
-                            (100000, False), (0, False), (0, True)]:
-            say("[gpu] column launch", name, blocks, lds)
-            sess.set_column_launch(blocks, lds)
+        for blocks, lds, waves in [(1, True, 8), (1, False, 4), (8, True, 4), (8, False, 8), (13, True, 8),
+                                   (24, False, 4), (100000, True, 4), (100000, False, 8), (0, True, 4), (0, False, 0),
+                                   (0, True, 0)]:
+            say("[gpu] column launch", name, blocks, lds, waves)
+            sess.set_column_launch(blocks, lds, waves)
             got = B.api.identify_mutations(s, table, S, Q, prop, session=sess, mode=mode)
-            assert got == ref, (blocks, lds)
+            assert got == ref, (blocks, lds, waves)
     finally:
         sess.close()
 
@@ -53,6 +54,8 @@ def test_launch_arguments_are_checked():
     try:
         with pytest.raises(B.BsqError):
             sess.set_column_launch(-1, True)
+        with pytest.raises(B.BsqError):
+            sess.set_column_launch(0, True, waves=2)
         with pytest.raises(B.BsqError):
             sess.tune_columns(table, S, Q, prop, repeats=0)
     finally:
@@ -82,7 +85,8 @@ def test_tuner_keeps_a_timed_launch_and_reuses_it():
     try:
         second = second_sess.tune_columns(table, S, Q, prop, repeats=1)
         assert second["from_cache"] == 1
-        assert (second["blocks"], second["table_in_lds"]) == (first["blocks"], first["table_in_lds"])
+        assert (second["blocks"], second["table_in_lds"], second["waves"]) == \
+            (first["blocks"], first["table_in_lds"], first["waves"])
         assert B.api.identify_mutations(s, table, S, Q, prop, session=second_sess) == want
     finally:
         second_sess.close()

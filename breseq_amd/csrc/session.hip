@@ -1031,13 +1031,22 @@ struct GdQueued {
   size_t prefix = 0;    // text bytes copied back with the counters
 };
 
-// Blocks of the column pass: the session's setting, or the default of one window per wavefront
-// capped at 2048 blocks; never more than one window per wavefront, and a multiple of the 8 XCDs
-// from 8 blocks up (the windows of one block then sit on one XCD's L2).
+// Blocks of the column pass.  A set count is kept to at most one window per wavefront and rounded
+// down to a multiple of the 8 XCDs from 8 blocks up (the windows of one block then sit on one XCD's
+// L2).  The default takes the fewest windows per wavefront k that keeps the grid within 2048
+// blocks, in the grid that gives every wavefront exactly k windows (the last block aside), rounded
+// up to a multiple of 8: a wavefront with one window more than the others holds its block.
 static int64_t column_blocks(const Session& S) {
-  const int64_t full = std::max<int64_t>(1, (S.n_windows + S.col_waves - 1) / S.col_waves);
-  int64_t g = std::min<int64_t>(full, S.col_blocks > 0 ? (int64_t)S.col_blocks : 2048);
-  if (g >= 8) g &= ~(int64_t)7;
+  const int64_t w = S.col_waves;
+  const int64_t full = std::max<int64_t>(1, (S.n_windows + w - 1) / w);
+  if (S.col_blocks > 0) {
+    int64_t g = std::min<int64_t>(full, (int64_t)S.col_blocks);
+    if (g >= 8) g &= ~(int64_t)7;
+    return g;
+  }
+  const int64_t k = std::max<int64_t>(1, (S.n_windows + w * 2048 - 1) / (w * 2048));
+  int64_t g = std::max<int64_t>(1, (S.n_windows + w * k - 1) / (w * k));
+  if (g >= 8) g = (g + 7) / 8 * 8;
   return g;
 }
 

@@ -417,8 +417,9 @@ int bsq_session_precision_audit(bsq_session* s, const bsq_identify_args* args, b
 int bsq_session_timing(const bsq_session* s, bsq_timing* out);
 
 /* Launch shape of the column pass of later identify passes (no counterpart in breseq, whose pileup
- * has no launch): `blocks` blocks (0, the default: one 64-column window per wavefront, at most 2048
- * blocks; never more than one window per wavefront, rounded down to a multiple of 8 from 8 up) of
+ * has no launch): `blocks` blocks (0, the default: the grid within 2048 blocks that gives every
+ * wavefront the same number of 64-column windows, the fewest that fit; a set count is kept to at
+ * most one window per wavefront and rounded down to a multiple of 8 from 8 up) of
  * `waves` wavefronts (4 or 8; 0 is the default, 8), and the likelihood table staged in LDS when it
  * fits (table_in_lds 1, the default) or read from global memory (0).  Results do not depend on it. */
 int bsq_session_set_column_launch(bsq_session* s, int32_t blocks, int32_t table_in_lds, int32_t waves,

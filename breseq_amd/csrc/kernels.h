@@ -117,6 +117,7 @@ struct ImParams {
   // candidates
   int2* cand;                   // (global column, insert slot)
   int32_t* n_cand;
+  int32_t* score_next;   // work counter of the exact scorer: the next candidate a block takes (zeroed per pass)
   int32_t cand_cap;
   int32_t* ins;                 // columns with insert slots
   int32_t* n_ins;
@@ -196,7 +197,7 @@ __global__ void k_coverage_columns(CovTabParams C);
 __global__ void k_coverage_rows(CovTabParams C);
 
 constexpr int kScoreWaves = 4;               // wavefronts per candidate in the exact scorer
-constexpr int kScoreGrid = 2048;             // blocks of the exact scorer (grid-stride over candidates)
+constexpr int kScoreGrid = 2048;             // blocks of the exact scorer (each draws candidates from a work counter)
 constexpr int kFixGrid = 256;                // blocks of k_fix_columns (at most kScoreGrid)
 constexpr size_t kScoreStaticLds = 28 * 1024;   // bound on the scorer's static LDS
 constexpr size_t kScoreLdsBudget = 52 * 1024;   // static + dynamic LDS of a scorer block: three per CU

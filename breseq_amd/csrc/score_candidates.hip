@@ -655,7 +655,16 @@ __global__ void __launch_bounds__(kSW * 64) __attribute__((amdgpu_waves_per_eu(3
   const bool two_stage = n_cand > 2 * (int)gridDim.x;
   const int prof_ph = two_stage ? 2 : 1;   // the phase of the profile likelihoods and first path levels
   uint32_t err = 0;
-  for (int ci = blockIdx.x; ci < n_cand; ci += gridDim.x) {
+  // Blocks take candidates from the pass's work counter, one at a time: a candidate's cost varies
+  // several-fold (emitted ones go on to the profiles and bisections), so a fixed stride over the
+  // grid would leave blocks idle while others work through a run of costly candidates.
+  __shared__ int s_next;
+  for (;;) {
+    if (tid == 0) s_next = atomicAdd(P.score_next, 1);
+    __syncthreads();
+    const int ci = __builtin_amdgcn_readfirstlane(s_next);
+    __syncthreads();   // every wavefront has read s_next before the next draw overwrites it
+    if (ci >= n_cand) break;
     // every index the candidate leads to is checked before it is used: a column inside the
     // sample, a window inside the index, a read range inside the alignments.  The values are read
     // as wave-uniform scalars, so the checks are uniform branches: a skip decided on per-lane

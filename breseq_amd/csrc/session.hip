@@ -1304,6 +1304,7 @@ static bool run_identify_once(Session& S, const bsq_identify_args* a, bsq_im_res
   P.col_total = S.d_col_total.p;
   P.cand = S.d_cand.p;
   P.n_cand = S.d_counters.p + 0;
+  P.score_next = S.d_counters.p + 5;
   P.cand_cap = cand_cap;
   P.ins = S.d_ins.p;
   P.n_ins = S.d_counters.p + 1;
